@@ -1,0 +1,189 @@
+/*
+ * mdforce.h - C ABI of the MI355X force engine for md_sim's force evaluation.
+ *
+ * Replaces the reference's force module entry points, which take no arguments and
+ * communicate through module variables:
+ *   force_init  src/forcem.f90:54-81   (constants, k-set, static wall S(k), wall tail
+ *                                       tables, first neighbour list)   -> mdf_create
+ *   force_get   src/forcem.f90:93-129  (list skin test + rebuild, real-space loops,
+ *                                       reciprocal sum, slab correction, angle terms,
+ *                                       COM force/torque/energy reduction) -> mdf_compute
+ *   force_zero  src/forcem.f90:84-90   (running-average reset)         -> mdf_zero
+ *   stop '...' at the reference's check sites                          -> status codes
+ * and the neighbour-list module nnlist (src/nnlist.f90:14-164), which the library owns.
+ *
+ * Conventions
+ *  - Site order is the reference's (src/sysdef.f90:381-419): rigid 3-site molecules
+ *    first (sites 3m, 3m+1, 3m+2, lead = 3m), then other moving sites, then frozen
+ *    crystal ("wall") sites.  Indices in this ABI are 0-based; site types are the
+ *    reference's 1-based type numbers.
+ *  - Positions are unwrapped, SoA, FP64, Angstrom; energies kJ/mol; forces kJ/mol/A.
+ *  - Frozen-site positions are taken once, at mdf_create (the reference computes the
+ *    static wall structure factor once, src/forcem.f90:766-767).
+ *  - One handle = one system on one GPU.  Calls on a handle are synchronous (outputs
+ *    valid on return) except mdf_compute_device, and are not re-entrant per handle.
+ */
+#ifndef MDFORCE_H
+#define MDFORCE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes; each stop site of the reference maps to one */
+enum {
+    MDF_OK = 0,
+    MDF_ERR_ARG = 1,            /* invalid argument or unsupported geometry            */
+    MDF_ERR_HIP = 2,            /* HIP runtime error                                   */
+    MDF_ERR_LIST_OVERFLOW = 3,  /* 'nnlist_make(): nnlist too big!'  nnlist.f90:136-139,157-160 */
+    MDF_ERR_VDW_TYPE = 4,       /* 'unknown interaction type'  forcem.f90:208-211,326-328,455-456 */
+    MDF_ERR_KSET = 5,           /* 'too large kcount'  forcem.f90:751-754,788-791      */
+    MDF_ERR_NONFINITE = 6,      /* non-finite energy (the reference only prints, forcem.f90:544-547) */
+    MDF_ERR_NO_DEVICE = 7       /* no HIP device visible                               */
+};
+
+enum { MDF_FP64 = 0, MDF_FP32_ACC64 = 1 };               /* arithmetic of pair and k-sum kernels */
+enum { MDF_CONST_REFERENCE = 0, MDF_CONST_EXACT = 1 };   /* widened single literals / exact     */
+enum { MDF_ERFC_AS7126 = 0, MDF_ERFC_ACCURATE = 1 };     /* A&S 7.1.26 as forcem.f90:11-13,335-337 */
+enum { MDF_ALPHA_REFERENCE = 0, MDF_ALPHA_GIVEN = 1 };   /* alfa = 3.09/rcut (forcem.f90:59) or given */
+enum { MDF_Z_OPEN = 0,            /* forcem.f90: min image in x,y only                       */
+       MDF_Z_PERIODIC = 1,        /* z min image in pair loops and in the list               */
+       MDF_Z_PERIODIC_OPEN_LIST = 2 /* forcem_back.f90 pair loops over an open-z list (nnlist.f90:129,150) */ };
+
+typedef struct mdf_topology {
+    int32_t n_total;         /* ntota                                                 */
+    int32_t n_move;          /* nmove: moving sites                                   */
+    int32_t n_rigid_sites;   /* nriga = 3 * n_rig                                     */
+    int32_t n_liquid_sites;  /* nliqa (rigid + mobile atoms; surface sites follow)   */
+    int32_t n_rig;           /* rigid 3-site molecules                                */
+    int32_t n_types;         /* nit                                                   */
+    int32_t n_ltype;         /* nltype: compared with a site type in the wall-force
+                                bookkeeping, as forcem.f90:481,540 do                 */
+    const int32_t *site_type;    /* [n_total] 1-based                                 */
+    const double  *charge;       /* [n_total] e                                       */
+    const int32_t *nsi;          /* [n_types] sites carried by a lead of the type, 0 = not a lead */
+    const int32_t *ijtype;       /* [n_types*n_types] row-major, 1-based pair number  */
+    const int32_t *vdw_kind;     /* [n_pairs] 0 none, 1 Lennard-Jones, 2 Buckingham   */
+    const double  *vdw_eps4;     /* [n_pairs] 4 eps          (sysdef.f90:372)         */
+    const double  *vdw_sig6;     /* [n_pairs] sigma^6        (sysdef.f90:373)         */
+    const double  *vdw_a;        /* [n_pairs] Buckingham A                            */
+    const double  *vdw_rhoi;     /* [n_pairs] 1/rho          (sysdef.f90:374)         */
+    const double  *vdw_c;        /* [n_pairs] Buckingham C                            */
+    /* distance constraints of surface groups: excluded from the pair list
+       (nnlist.f90:41-43,69-71) and their self term (forcem.f90:808-819)              */
+    int32_t        n_constraints;
+    const int32_t *con_a, *con_b;   /* [n_constraints] 0-based site indices           */
+    const double  *con_len;         /* [n_constraints] constrained length             */
+    /* harmonic angles of surface groups (forcem.f90:565-652)                         */
+    int32_t        n_angles;
+    const int32_t *ang_i, *ang_j, *ang_k;  /* [n_angles] 0-based                      */
+    const double  *ang_kt, *ang_theta;     /* [n_angles] force constant, radians      */
+    /* wall tail-correction inputs (force_real_init, forcem.f90:167-228)             */
+    double  wall_lw;             /* lw: walls at z = +-lw/2                            */
+    double  wall_cell[3];        /* cx, cy, cz                                         */
+    int32_t wall_nss[2];         /* nss of the two crystal site types                 */
+} mdf_topology;
+
+typedef struct mdf_params {
+    double  box[3];          /* lx, ly, lz; the reference needs lx = ly = min (forcem.f90:674-685) */
+    double  rcut;            /* group cutoff on the lead-lead distance (forcem.f90:299,429) */
+    double  skin;            /* Verlet skin (nnlist.f90:23-26, forcem.f90:245-250)  */
+    double  alpha;           /* Ewald alpha, used with MDF_ALPHA_GIVEN               */
+    int32_t alpha_mode;
+    int32_t kmax;            /* k index bound; klmax/kmmax/knmax from the box (forcem.f90:673-678) */
+    int32_t z_mode;          /* MDF_Z_*                                              */
+    int32_t ew3dc;           /* 1: Yeh-Berkowitz slab correction (forcem.f90:923-928) */
+    int32_t wall_lr;         /* 1: wall long-range tail correction (forcem.f90:509-543) */
+    int32_t precision;       /* MDF_FP64 / MDF_FP32_ACC64                            */
+    int32_t const_mode;      /* MDF_CONST_*                                          */
+    int32_t erfc_mode;       /* MDF_ERFC_*                                           */
+    int32_t max_neighbors;   /* per-lead list capacity, 0 = default                  */
+    int32_t device;          /* HIP device ordinal                                   */
+} mdf_params;
+
+typedef struct mdf_outputs {
+    /* arrays: caller-owned, may be NULL to skip                                       */
+    double *f_site;          /* [3*n_move]  x block, y block, z block: site forces
+                                before the COM reduction (fx, fy, fz of force_sum)    */
+    double *f_com;           /* [3*n_rig]   COM forces fcx, fcy, fcz                  */
+    double *torque;          /* [3*n_rig]   tx, ty, tz                                */
+    double *e_vdw;           /* [n_pairs]   elj                                       */
+    double *e_coul_real;     /* [n_pairs]   eqq                                       */
+    /* scalars                                                                         */
+    double e_recip;          /* uewa   */
+    double e_recip_ll, e_recip_wl, e_recip_ww;   /* ewqLL, ewqWL, ewqWW */
+    double e_self;           /* uself  */
+    double e_slab;           /* urf    */
+    double e_wall_lr;        /* ulr    */
+    double e_conf;           /* uconf  */
+    double e_avex;           /* uavex after force_sum */
+    double press_wall;       /* press  */
+    double f_wall[9];        /* fwsx(-1:1), fwsy(-1:1), fwsz(-1:1) */
+    double virial[9];        /* site-pair virial tensor W_ab (row-major): real-space pairs,
+                                reciprocal and slab terms; the reference has none      */
+    int32_t list_rebuilt;    /* 1 if this call rebuilt the neighbour list              */
+    int32_t n_pairs_listed;  /* ipack  */
+    int32_t n_wall_pairs_listed; /* wpack */
+    int32_t n_kvec;          /* k-vectors in the set */
+} mdf_outputs;
+
+typedef struct mdf_system mdf_system;
+
+/* force_init: validate, derive constants and tables, upload, build the first list.  */
+int mdf_create(const mdf_topology *top, const mdf_params *par,
+               const double *x, const double *y, const double *z,
+               const double *cx, const double *cy, const double *cz,
+               mdf_system **out);
+
+/* force_get on host arrays: x,y,z have n_total entries (only moving sites are read),
+   cx,cy,cz have n_rig entries.  Outputs are written into *out.                       */
+int mdf_compute(mdf_system *sys,
+                const double *x, const double *y, const double *z,
+                const double *cx, const double *cy, const double *cz,
+                mdf_outputs *out);
+
+/* force_get on device-resident inputs (n_move and n_rig entries).  Enqueued on the
+   handle's stream and not waited for; results stay on the device until
+   mdf_read_outputs.  stream may be NULL for the handle's own stream.                  */
+int mdf_compute_device(mdf_system *sys,
+                       const double *d_x, const double *d_y, const double *d_z,
+                       const double *d_cx, const double *d_cy, const double *d_cz,
+                       void *stream);
+
+/* Wait for the last mdf_compute_device and copy its results into *out.              */
+int mdf_read_outputs(mdf_system *sys, mdf_outputs *out);
+
+/* Device pointers of the last results: site forces [3*n_move], COM forces and torques
+   [3*n_rig] each (valid until the next compute).                                      */
+int mdf_device_results(mdf_system *sys, const double **d_f_site, const double **d_f_com,
+                       const double **d_torque);
+
+/* Pair list as it stands (the reference's list1/list2 and listw1/listw2 sets, 0-based,
+   i < j, ordered by i then j).  Capacities in, counts out.                            */
+int mdf_get_pairs(mdf_system *sys, int32_t *i_mm, int32_t *j_mm, int32_t cap_mm, int32_t *n_mm,
+                  int32_t *i_mw, int32_t *j_mw, int32_t cap_mw, int32_t *n_mw);
+
+/* force_zero */
+int mdf_zero(mdf_system *sys);
+
+/* k-vector set of the handle: (kl,km,kn) triples and Ak, in the reference's order.   */
+int mdf_get_kvectors(mdf_system *sys, int32_t *kvec, double *ak, int32_t cap, int32_t *n);
+
+/* Last structure factor S(k) = S_moving + S_wall (re, im interleaved) per k-vector.  */
+int mdf_get_structure_factor(mdf_system *sys, double *s, int32_t cap);
+
+void mdf_destroy(mdf_system *sys);
+
+/* Message for the last failing call (per thread).                                    */
+const char *mdf_last_error(void);
+
+/* Kernel timing of the last compute: names and milliseconds per internal stage,
+   measured with HIP events on the handle's stream.  n in: capacity; out: count.       */
+int mdf_stage_times(mdf_system *sys, const char **names, float *ms, int32_t cap, int32_t *n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

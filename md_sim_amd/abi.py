@@ -1,0 +1,164 @@
+"""ctypes mirror of include/mdforce.h and the conversion System -> ABI arguments.
+
+The host side of the engine mirrors the reference's force module (src/forcem.f90:54-129):
+``create`` ~ force_init, ``compute`` ~ force_get, ``zero`` ~ force_zero, and a non-zero
+status ~ the reference's ``stop`` (raised as MDForceError with the library's message).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Dict, Optional
+
+import numpy as np
+
+MDF_OK = 0
+STATUS_NAMES = {0: "OK", 1: "ERR_ARG", 2: "ERR_HIP", 3: "ERR_LIST_OVERFLOW", 4: "ERR_VDW_TYPE",
+                5: "ERR_KSET", 6: "ERR_NONFINITE", 7: "ERR_NO_DEVICE"}
+FP64, FP32_ACC64 = 0, 1
+CONST_REFERENCE, CONST_EXACT = 0, 1
+ERFC_AS7126, ERFC_ACCURATE = 0, 1
+ALPHA_REFERENCE, ALPHA_GIVEN = 0, 1
+Z_OPEN, Z_PERIODIC, Z_PERIODIC_OPEN_LIST = 0, 1, 2
+
+_pi32 = C.POINTER(C.c_int32)
+_pd = C.POINTER(C.c_double)
+
+
+class Topology(C.Structure):
+    _fields_ = [("n_total", C.c_int32), ("n_move", C.c_int32), ("n_rigid_sites", C.c_int32),
+                ("n_liquid_sites", C.c_int32), ("n_rig", C.c_int32), ("n_types", C.c_int32),
+                ("n_ltype", C.c_int32),
+                ("site_type", _pi32), ("charge", _pd), ("nsi", _pi32), ("ijtype", _pi32),
+                ("vdw_kind", _pi32), ("vdw_eps4", _pd), ("vdw_sig6", _pd), ("vdw_a", _pd),
+                ("vdw_rhoi", _pd), ("vdw_c", _pd),
+                ("n_constraints", C.c_int32), ("con_a", _pi32), ("con_b", _pi32), ("con_len", _pd),
+                ("n_angles", C.c_int32), ("ang_i", _pi32), ("ang_j", _pi32), ("ang_k", _pi32),
+                ("ang_kt", _pd), ("ang_theta", _pd),
+                ("wall_lw", C.c_double), ("wall_cell", C.c_double * 3), ("wall_nss", C.c_int32 * 2)]
+
+
+class Params(C.Structure):
+    _fields_ = [("box", C.c_double * 3), ("rcut", C.c_double), ("skin", C.c_double),
+                ("alpha", C.c_double), ("alpha_mode", C.c_int32), ("kmax", C.c_int32),
+                ("z_mode", C.c_int32), ("ew3dc", C.c_int32), ("wall_lr", C.c_int32),
+                ("precision", C.c_int32), ("const_mode", C.c_int32), ("erfc_mode", C.c_int32),
+                ("max_neighbors", C.c_int32), ("device", C.c_int32)]
+
+
+class Outputs(C.Structure):
+    _fields_ = [("f_site", _pd), ("f_com", _pd), ("torque", _pd), ("e_vdw", _pd), ("e_coul_real", _pd),
+                ("e_recip", C.c_double), ("e_recip_ll", C.c_double), ("e_recip_wl", C.c_double),
+                ("e_recip_ww", C.c_double), ("e_self", C.c_double), ("e_slab", C.c_double),
+                ("e_wall_lr", C.c_double), ("e_conf", C.c_double), ("e_avex", C.c_double),
+                ("press_wall", C.c_double), ("f_wall", C.c_double * 9), ("virial", C.c_double * 9),
+                ("list_rebuilt", C.c_int32), ("n_pairs_listed", C.c_int32),
+                ("n_wall_pairs_listed", C.c_int32), ("n_kvec", C.c_int32)]
+
+
+SCALAR_OUTPUTS = ("e_recip", "e_recip_ll", "e_recip_wl", "e_recip_ww", "e_self", "e_slab",
+                  "e_wall_lr", "e_conf", "e_avex", "press_wall")
+
+
+def _p(a: np.ndarray, t):
+    return a.ctypes.data_as(C.POINTER(t)) if a is not None and a.size else None
+
+
+@dataclass
+class SystemArrays:
+    """Contiguous arrays a Topology/Params pair points into (kept alive with them)."""
+    top: Topology
+    par: Params
+    keep: Dict[str, np.ndarray]
+    n_pairs: int
+
+
+def build_abi(system, precision: int = FP64, const_mode: int = CONST_REFERENCE,
+              erfc_mode: int = ERFC_AS7126, alpha: Optional[float] = None,
+              z_mode: Optional[int] = None, ew3dc: Optional[bool] = None,
+              wall_lr: Optional[bool] = None, max_neighbors: int = 0, device: int = 0) -> SystemArrays:
+    """Topology and parameters of a ``systems.System``.  Defaults follow the reference
+    variant the system corresponds to: slab systems use src/forcem.f90 (open z, slab
+    correction and wall tail on); bulk systems use src/forcem_back.f90 (periodic z in the
+    pair loops over the reference's open-z list, no slab or wall terms)."""
+    s = system
+    ijtype, kind, eps4, sig6, A, rhoi, Cc = s.vdw_tables()
+    keep = dict(site_type=np.ascontiguousarray(s.site_types(), np.int32),
+                charge=np.ascontiguousarray(s.charges(), np.float64),
+                nsi=np.ascontiguousarray(s.nsi(), np.int32),
+                ijtype=np.ascontiguousarray(ijtype.ravel(), np.int32),
+                vdw_kind=kind, vdw_eps4=eps4, vdw_sig6=sig6, vdw_a=A, vdw_rhoi=rhoi, vdw_c=Cc)
+    top = Topology()
+    top.n_total, top.n_move, top.n_rigid_sites = s.ntota, s.nmove, s.nriga
+    top.n_liquid_sites, top.n_rig, top.n_types, top.n_ltype = s.nliqa, s.nrig, s.nit, s.nltype
+    top.site_type = _p(keep["site_type"], C.c_int32)
+    top.charge = _p(keep["charge"], C.c_double)
+    top.nsi = _p(keep["nsi"], C.c_int32)
+    top.ijtype = _p(keep["ijtype"], C.c_int32)
+    top.vdw_kind = _p(keep["vdw_kind"], C.c_int32)
+    for k in ("vdw_eps4", "vdw_sig6", "vdw_a", "vdw_rhoi", "vdw_c"):
+        setattr(top, k, _p(keep[k], C.c_double))
+    top.n_constraints = 0
+    top.n_angles = 0
+    cr = s.crystal
+    top.wall_lw = cr.lw
+    top.wall_cell[:] = list(cr.cell)
+    top.wall_nss[:] = [cr.sites[0][1], cr.sites[1][1]]
+    par = Params()
+    par.box[:] = list(s.box)
+    par.rcut, par.skin = s.rcut, s.skin
+    par.alpha = alpha if alpha is not None else 0.0
+    par.alpha_mode = ALPHA_GIVEN if alpha is not None else ALPHA_REFERENCE
+    par.kmax = s.kmax
+    bulk = bool(s.z_periodic)
+    par.z_mode = z_mode if z_mode is not None else (Z_PERIODIC_OPEN_LIST if bulk else Z_OPEN)
+    par.ew3dc = int(ew3dc if ew3dc is not None else not bulk)
+    par.wall_lr = int(wall_lr if wall_lr is not None else not bulk)
+    par.precision, par.const_mode, par.erfc_mode = precision, const_mode, erfc_mode
+    par.max_neighbors = max_neighbors
+    par.device = device
+    return SystemArrays(top, par, keep, s.nijt)
+
+
+def positions(system):
+    pos = system.site_positions()
+    x = np.ascontiguousarray(pos[:, 0])
+    y = np.ascontiguousarray(pos[:, 1])
+    z = np.ascontiguousarray(pos[:, 2])
+    com = system.com
+    return x, y, z, np.ascontiguousarray(com[:, 0]), np.ascontiguousarray(com[:, 1]), np.ascontiguousarray(com[:, 2])
+
+
+class ForceResult(dict):
+    """Outputs of one force evaluation as numpy arrays / floats."""
+
+    @property
+    def f_site(self):
+        return self["f_site"]
+
+
+def make_outputs(n_move: int, n_rig: int, n_pairs: int):
+    arrs = dict(f_site=np.zeros(3 * n_move), f_com=np.zeros(3 * max(n_rig, 0)),
+                torque=np.zeros(3 * max(n_rig, 0)), e_vdw=np.zeros(n_pairs), e_coul_real=np.zeros(n_pairs))
+    o = Outputs()
+    for k, a in arrs.items():
+        setattr(o, k, _p(a, C.c_double))
+    return o, arrs
+
+
+def collect(o: Outputs, arrs, n_move: int, n_rig: int) -> ForceResult:
+    r = ForceResult()
+    r["f_site"] = arrs["f_site"].reshape(3, n_move).copy()
+    r["f_com"] = arrs["f_com"].reshape(3, n_rig).copy()
+    r["torque"] = arrs["torque"].reshape(3, n_rig).copy()
+    r["e_vdw"] = arrs["e_vdw"].copy()
+    r["e_coul_real"] = arrs["e_coul_real"].copy()
+    for k in SCALAR_OUTPUTS:
+        r[k] = float(getattr(o, k))
+    r["f_wall"] = np.array(o.f_wall[:]).reshape(3, 3)
+    r["virial"] = np.array(o.virial[:]).reshape(3, 3)
+    r["list_rebuilt"] = int(o.list_rebuilt)
+    r["n_pairs_listed"] = int(o.n_pairs_listed)
+    r["n_wall_pairs_listed"] = int(o.n_wall_pairs_listed)
+    r["n_kvec"] = int(o.n_kvec)
+    return r

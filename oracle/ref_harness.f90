@@ -1,0 +1,221 @@
+!----------------------------------------------------------------------------
+! Golden-vector harness for the reference force path (test infrastructure).
+!
+! Replaces the reference's program unit (src/main.f90:19-108): it performs the
+! same set-up as `input`/`init` (src/main.f90:113-315) for one MPI rank, sets the
+! site offsets the way move_predict does (src/move.f90:157-161), runs the pieces
+! of one force_get (src/forcem.f90:93-129) and dumps inputs and outputs to a
+! binary stream file.  Site forces are dumped before force_sum, which overwrites
+! fx(1:nrig) with COM sums (src/forcem.f90:1019-1024).
+!
+! Dump record: name (16 chars), kind (int64: 1 real*8, 2 int64), count (int64),
+! data.  Built only in the development container, from the reference sources.
+!----------------------------------------------------------------------------
+program ref_harness
+    use move
+    implicit none
+    include 'mpif.h'
+    integer :: ierr, u
+    integer*8 :: i, ix, n, kl, km, kn, kcnt, kmn, knn, nsel
+    character*40 :: base
+    character*256 :: outname
+    real*8, allocatable :: f1x(:), f1y(:), f1z(:), f2x(:), f2y(:), f2z(:)
+    real*8, allocatable :: kak(:), kre(:), kim(:), kvec(:)
+    integer*8, allocatable :: ibuf(:)
+
+    nproc = 1
+    me = 0
+    call mpi_init(ierr)
+    call mpi_comm_size(mpi_comm_world, nproc, ierr)
+    call mpi_comm_rank(mpi_comm_world, me, ierr)
+    if (nproc /= 1) stop 'ref_harness: run on one rank'
+
+    call get_command_argument(1, outname)
+    open(7, file='harness.inp', status='old')
+    read(7, *) base
+    read(7, *) dt, rcut, skin, rdel, kmax
+    close(7)
+
+    ! the tot.inp switches the force path reads; all optional features off
+    temper = -1.0
+    timeq = 0.0
+    timrun = 0.0
+    timprn = dt
+    l2d = .false.
+    lhb = .false.
+    ldiff = .false.
+    lcouett = .false.
+    lfext = .false.
+    lchempot = .false.
+    dshx = 0.0 ; dshy = 0.0 ; dshz = 0.0
+    n = len_trim(base)
+    filfld = base(1:n)//'.fld'
+    filtop = base(1:n)//'.top'
+    filcfi = base(1:n)//'.cfg'
+    filcfo = base(1:n)//'_n.cfg'
+    filmeas = base(1:n)//'.crl'
+
+    call sysdef_zero
+    call sysdef_input
+    call sysdef_alloc
+    call config_zero
+    call config_input
+    call sysdef_init
+    call config_init
+    call force_init
+    call move_init
+    call measure_init
+
+    ! site offsets from the lead site, as the predictor leaves them
+    do i = 1, nrig
+        ix = 3*(i-1)
+        rsx(ix+2:ix+3) = x0(ix+2:ix+3) - x0(ix+1)
+        rsy(ix+2:ix+3) = y0(ix+2:ix+3) - y0(ix+1)
+        rsz(ix+2:ix+3) = z0(ix+2:ix+3) - z0(ix+1)
+    end do
+
+    ! accumulators as force_get zeroes them
+    fx = 0.0 ; fy = 0.0 ; fz = 0.0
+    fwsx = 0.0 ; fwsy = 0.0 ; fwsz = 0.0
+    vir = 0.0 ; elj = 0.0 ; eqq = 0.0 ; uewa = 0.0
+
+    allocate(f1x(nmove), f1y(nmove), f1z(nmove), f2x(nmove), f2y(nmove), f2z(nmove))
+    call force_real
+    f1x = fx ; f1y = fy ; f1z = fz
+    call force_fouri
+    call force_intra
+    f2x = fx ; f2y = fy ; f2z = fz
+
+    ! selected k-vectors in the reference's enumeration order (src/forcem.f90:870-915)
+    nsel = 0
+    do ix = 1, 2
+    if (ix == 2) allocate(kak(nsel), kre(nsel), kim(nsel), kvec(3*nsel))
+    kcnt = 0
+    nsel = 0
+    kmn = 0
+    knn = 1
+    do kl = 0, klmax
+        do km = kmn, kmmax
+            do kn = knn, knmax
+                kcnt = kcnt + 1
+                if (myk(kcnt) >= 0) then
+                    nsel = nsel + 1
+                    if (ix == 1) cycle
+                    kak(nsel) = Ak(kcnt)
+                    kre(nsel) = dble(Qsum(kcnt))
+                    kim(nsel) = dimag(Qsum(kcnt))
+                    kvec(3*nsel-2) = dble(kl)
+                    kvec(3*nsel-1) = dble(km)
+                    kvec(3*nsel) = dble(kn)
+                end if
+            end do
+            knn = -knmax
+        end do
+        kmn = -kmmax
+    end do
+    end do
+
+    call force_sum
+
+    u = 11
+    open(u, file=trim(outname), access='stream', form='unformatted', status='replace')
+    allocate(ibuf(max(ntota, int(ipack,8), int(wpack,8), 10_8)))
+    ibuf(1:10) = (/ ntota, nmove, nriga, nliqa, nrig, nwall, nijt, int(ipack,8), int(wpack,8), nsel /)
+    call put_i(u, 'counts', ibuf, 10_8)
+    call put_r(u, 'x0', x0, ntota)
+    call put_r(u, 'y0', y0, ntota)
+    call put_r(u, 'z0', z0, ntota)
+    call put_r(u, 'qs', qs, ntota)
+    ibuf(1:ntota) = itype(1:ntota)
+    call put_i(u, 'itype', ibuf, ntota)
+    call put_r(u, 'cx0', cx0, nrig)
+    call put_r(u, 'cy0', cy0, nrig)
+    call put_r(u, 'cz0', cz0, nrig)
+    ibuf(1:ipack) = list1(1:ipack)
+    call put_i(u, 'list1', ibuf, int(ipack,8))
+    ibuf(1:ipack) = list2(1:ipack)
+    call put_i(u, 'list2', ibuf, int(ipack,8))
+    ibuf(1:wpack) = listw1(1:wpack)
+    call put_i(u, 'listw1', ibuf, int(wpack,8))
+    ibuf(1:wpack) = listw2(1:wpack)
+    call put_i(u, 'listw2', ibuf, int(wpack,8))
+    call put_r(u, 'f_real_x', f1x, nmove)
+    call put_r(u, 'f_real_y', f1y, nmove)
+    call put_r(u, 'f_real_z', f1z, nmove)
+    call put_r(u, 'f_site_x', f2x, nmove)
+    call put_r(u, 'f_site_y', f2y, nmove)
+    call put_r(u, 'f_site_z', f2z, nmove)
+    call put_r(u, 'fcx', fcx, nrig)
+    call put_r(u, 'fcy', fcy, nrig)
+    call put_r(u, 'fcz', fcz, nrig)
+    call put_r(u, 'tx', tx, nrig)
+    call put_r(u, 'ty', ty, nrig)
+    call put_r(u, 'tz', tz, nrig)
+    call put_r(u, 'elj', elj, nijt)
+    call put_r(u, 'eqq', eqq, nijt)
+    call put_r(u, 'k_ak', kak, nsel)
+    call put_r(u, 'k_sre', kre, nsel)
+    call put_r(u, 'k_sim', kim, nsel)
+    call put_r(u, 'k_vec', kvec, 3*nsel)
+    call put_s(u, 'uewa', uewa)
+    call put_s(u, 'uself', uself)
+    call put_s(u, 'urf', urf)
+    call put_s(u, 'ulr', ulr)
+    call put_s(u, 'uconf', uconf)
+    call put_s(u, 'press', press)
+    call put_s(u, 'ewqWW', ewqWW)
+    call put_s(u, 'ewqLL', ewqLL)
+    call put_s(u, 'ewqWL', ewqWL)
+    call put_s(u, 'uavex', uavex)
+    call put_s(u, 'alfa', alfa)
+    call put_s(u, 'facq', facq)
+    call put_s(u, 'spii', spii)
+    call put_s(u, 'rskinsq', rskinsq)
+    call put_r(u, 'fwsx', fwsx, 3_8)
+    call put_r(u, 'fwsy', fwsy, 3_8)
+    call put_r(u, 'fwsz', fwsz, 3_8)
+    ibuf(1:3) = (/ int(klmax,8), int(kmmax,8), int(knmax,8) /)
+    call put_i(u, 'kmaxes', ibuf, 3_8)
+    close(u)
+
+    call mpi_finalize(ierr)
+
+contains
+
+    subroutine put_hdr(u, name, kind, cnt)
+        integer :: u
+        character(len=*) :: name
+        integer*8 :: kind, cnt
+        character(len=16) :: nm
+        nm = name
+        write(u) nm, kind, cnt
+    end subroutine put_hdr
+
+    subroutine put_r(u, name, a, cnt)
+        integer :: u
+        character(len=*) :: name
+        integer*8 :: cnt
+        real*8 :: a(*)
+        call put_hdr(u, name, 1_8, cnt)
+        if (cnt > 0) write(u) a(1:cnt)
+    end subroutine put_r
+
+    subroutine put_s(u, name, a)
+        integer :: u
+        character(len=*) :: name
+        real*8 :: a
+        real*8 :: b(1)
+        b(1) = a
+        call put_r(u, name, b, 1_8)
+    end subroutine put_s
+
+    subroutine put_i(u, name, a, cnt)
+        integer :: u
+        character(len=*) :: name
+        integer*8 :: cnt
+        integer*8 :: a(*)
+        call put_hdr(u, name, 2_8, cnt)
+        if (cnt > 0) write(u) a(1:cnt)
+    end subroutine put_i
+
+end program ref_harness

@@ -1,0 +1,53 @@
+"""Run the reference golden-vector harness (oracle/_ref, built by oracle/build_ref.sh) on a
+System and parse its dump.  Test infrastructure: used only to make and check golden
+vectors in the development container, never by the product path."""
+from __future__ import annotations
+
+import os
+import subprocess
+import tempfile
+from typing import Dict
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF_DIR = os.path.join(HERE, "_ref")
+
+
+def ref_available() -> bool:
+    return all(os.access(os.path.join(REF_DIR, "ref_" + v), os.X_OK) for v in ("slab", "bulk"))
+
+
+def parse_dump(path: str) -> Dict[str, np.ndarray]:
+    raw = open(path, "rb").read()
+    out: Dict[str, np.ndarray] = {}
+    pos = 0
+    while pos < len(raw):
+        name = raw[pos:pos + 16].decode().strip()
+        kind, cnt = np.frombuffer(raw, dtype=np.int64, count=2, offset=pos + 16)
+        pos += 32
+        dt = np.float64 if kind == 1 else np.int64
+        out[name] = np.frombuffer(raw, dtype=dt, count=int(cnt), offset=pos).copy()
+        pos += 8 * int(cnt)
+    return out
+
+
+def run_reference(system, workdir: str = None, timeout: float = 600.0) -> Dict[str, np.ndarray]:
+    """Write the reference's input files for `system`, run the matching harness variant
+    (bulk = 3-D periodic, slab otherwise) and return the parsed dump."""
+    from md_sim_amd.refio import write_reference_inputs
+
+    variant = "bulk" if system.z_periodic else "slab"
+    exe = os.path.join(REF_DIR, "ref_" + variant)
+    if not os.access(exe, os.X_OK):
+        raise FileNotFoundError(exe)
+    own = workdir is None
+    d = tempfile.mkdtemp(prefix="mdref_") if own else workdir
+    write_reference_inputs(system, d)
+    out = os.path.join(d, "dump.bin")
+    r = subprocess.run([exe, out], cwd=d, capture_output=True, text=True, timeout=timeout)
+    if r.returncode != 0 or not os.path.exists(out):
+        raise RuntimeError("reference harness failed (%d):\n%s\n%s" % (r.returncode, r.stdout[-2000:], r.stderr[-2000:]))
+    dump = parse_dump(out)
+    dump["_stdout"] = np.array(r.stdout)
+    return dump

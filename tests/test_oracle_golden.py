@@ -1,0 +1,72 @@
+"""The CPU oracle against vectors produced by the reference itself (tests/golden)."""
+import hashlib
+
+import numpy as np
+import pytest
+
+from md_sim_amd import systems
+from helpers import golden, golden_positions, rel_err
+
+TOL = 1e-12  # oracle vs reference: same algorithm, same op order, FP64
+
+
+@pytest.mark.parametrize("name", ["bulk216", "pore_small"])
+def test_oracle_matches_reference_golden(oracle_lib, name):
+    g = golden(name)
+    s = systems.config(name)
+    pos = s.site_positions()
+    # the fixture's positions are what the generator makes
+    assert np.array_equal(g["x0"], pos[:, 0]) and np.array_equal(g["z0"], pos[:, 2])
+    assert np.array_equal(g["itype"], s.site_types())
+    assert np.array_equal(g["qs"], s.charges())
+    o = oracle_lib.Oracle(s)
+    r = o.compute(*golden_positions(g))
+    # pair sets: bit-exact, in the reference's order (1-based in the fixture)
+    l1, l2, w1, w2 = o.pairs()
+    assert np.array_equal(l1 + 1, g["list1"]) and np.array_equal(l2 + 1, g["list2"])
+    assert np.array_equal(w1 + 1, g["listw1"]) and np.array_equal(w2 + 1, g["listw2"])
+    f = r["f_site"]
+    for a, k in enumerate("xyz"):
+        assert rel_err(f[a], g["f_site_" + k]) < TOL
+    assert rel_err(r["torque"][0], g["tx"]) < TOL
+    assert rel_err(r["torque"][2], g["tz"]) < TOL
+    if name != "bulk216":  # the 3-D variant's one-rank force_sum copies site forces into fcx
+        assert rel_err(r["f_com"][1], g["fcy"]) < TOL
+    assert rel_err(r["e_vdw"], g["elj"]) < TOL
+    assert rel_err(r["e_coul_real"], g["eqq"]) < TOL
+    for k, gk in [("e_recip", "uewa"), ("e_self", "uself"), ("e_slab", "urf"), ("e_wall_lr", "ulr"),
+                  ("e_conf", "uconf"), ("press_wall", "press"), ("e_recip_ll", "ewqLL"),
+                  ("e_recip_ww", "ewqWW"), ("e_recip_wl", "ewqWL"), ("e_avex", "uavex")]:
+        assert abs(r[k] - g[gk].item()) <= TOL * max(abs(g[gk].item()), 1.0), k
+    kv, ak, sf = o.kvectors()
+    assert np.array_equal(kv.ravel(), g["k_vec"].astype(int))
+    assert rel_err(ak, g["k_ak"]) < TOL
+    assert rel_err(sf.real, g["k_sre"]) < TOL and rel_err(sf.imag, g["k_sim"]) < TOL
+    c = o.constants()
+    assert c["alfa"] == g["alfa"].item() and c["facq"] == g["facq"].item() and c["spii"] == g["spii"].item()
+    assert c["rskinsq"] == g["rskinsq"].item()
+
+
+def test_oracle_matches_reference_slab2048(oracle_lib):
+    g = golden("slab2048")
+    s = systems.config("slab2048")
+    pos = s.site_positions()
+    assert hashlib.sha256(np.concatenate([pos[:, 0], pos[:, 1], pos[:, 2]]).tobytes()).hexdigest() == str(g["pos_digest"])
+    o = oracle_lib.Oracle(s)
+    r = o.compute_system()
+    l1, l2, w1, w2 = o.pairs()
+    from golden.make_golden import pair_digest
+    assert pair_digest(l1 + 1, l2 + 1) == str(g["list_digest"])
+    assert pair_digest(w1 + 1, w2 + 1) == str(g["wlist_digest"])
+    sel = g["sel"]
+    for a, k in enumerate("xyz"):
+        assert rel_err(r["f_site"][a][sel], g["f_site_" + k]) < TOL
+    ms = g["msel"]
+    assert rel_err(r["f_com"][2][ms], g["fcz"]) < TOL
+    assert rel_err(r["torque"][1][ms], g["ty"]) < TOL
+    for k, gk in [("e_recip", "uewa"), ("e_conf", "uconf"), ("press_wall", "press"), ("e_wall_lr", "ulr"),
+                  ("e_slab", "urf"), ("e_self", "uself")]:
+        assert abs(r[k] - g[gk].item()) <= TOL * abs(g[gk].item()), k
+    kv, ak, sf = o.kvectors()
+    assert np.array_equal(kv[g["ksel"]], g["k_vec"].astype(int))
+    assert rel_err(sf.real[g["ksel"]], g["k_sre"]) < TOL
