@@ -1,0 +1,939 @@
+// C ABI of the force engine (include/mdforce.h) and its host-side runtime: constant
+// derivation, k-set enumeration, tail-correction tables, device buffers and the launch
+// sequence of one force evaluation.
+//
+// Reference correspondence:
+//   mdf_create   force_init (src/forcem.f90:54-81): force_fouri_init (k-set, A_k, static
+//                frozen-site S(k), self energy, :657-825), force_real_init (wall tail tables,
+//                :157-231) and nnlist_init/nnlist_make (src/nnlist.f90:14-164)
+//   mdf_compute  force_get (src/forcem.f90:93-129)
+//   mdf_zero     force_zero (src/forcem.f90:84-90)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mdforce.h"
+#include "mdf_internal.h"
+
+namespace mdf {
+__global__ void k_reset(DevSys S, int force_rebuild);
+__global__ void k_skin(DevSys S, double *dip_part);
+__global__ void k_decide(DevSys S, const double *dip_part, int n_dip, double *dipz_out);
+__global__ void k_cell_clear(DevSys S);
+__global__ void k_cell_count(DevSys S, int *lead_cell_out);
+__global__ void k_cell_scan(int *cnt, int *start, int *cursor, const DevFlags *f, int ncells_xy, int use_flag_ncz,
+                            int ncz_given);
+__global__ void k_cell_fill(const int *item_cell, int n_items, int *cursor, int *items, const DevFlags *f,
+                            int use_flag);
+__global__ void k_cell_sort(const int *start, const int *items_in, int *items_out, const DevFlags *f, int use_flag,
+                            int ncells_xy, int ncz_given);
+__global__ void k_build(DevSys S);
+__global__ void k_snapshot(DevSys S);
+__global__ void k_wall_cells(DevSys S, int *wcell_of, int ncz, double z0, double csz);
+__global__ void k_count_cells(const int *item_cell, int n_items, int *cnt);
+__global__ void k_zero_int(int *p, int n);
+template <typename R, int G>
+__global__ void k_pair(DevSys S);
+template <typename R, int NS>
+__global__ void k_sf(const double *, const double *, const double *, const double *, int, double, double, double,
+                     const KChunk *, int, int, double2 *, int);
+__global__ void k_sf_reduce(const double2 *, int, int, const double2 *, const double *, const int *, double, double,
+                            double, double, double2 *, double2 *, double *);
+template <typename R>
+__global__ void k_bp(const double *, const double *, const double *, const double *, int, double, double, double,
+                     const KRow *, int, int, const double2 *, double, double, double, double *);
+__global__ void k_epilogue(DevSys S, const double *fpart, int n_groups, const WallLR *wl, int ew3dc, double crf0,
+                           const double *dipz, double *epart);
+__global__ void k_com(DevSys S, double *fcom, double *tq);
+__global__ void k_colsum(DevSys S, const int *lead_slot, int per_block, double *colpart);
+__global__ void k_final(DevSys S, FinalArgs A, const double *colpart, const double *kpart, const double *eppart,
+                        const double *dipz, double *e_vdw, double *e_coul, Scalars *out);
+}  // namespace mdf
+
+using namespace mdf;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(x)                                                                                   \
+    do {                                                                                            \
+        hipError_t e_ = (x);                                                                        \
+        if (e_ != hipSuccess) return fail(MDF_ERR_HIP, "%s: %s (%s:%d)", #x, hipGetErrorString(e_), \
+                                          __FILE__, __LINE__);                                      \
+    } while (0)
+
+// reference constants: default-real literals widened to REAL*8 (src/global.f90:18-24,
+// src/forcem.f90:11-13,59,1081) or exact values
+struct Consts {
+    double pi, e, na, eps0, k4pi, a2m, a1, a2, a3, a4, a5, p, c309, e30, e5;
+};
+static Consts make_consts(int mode) {
+    Consts c;
+    if (mode == MDF_CONST_REFERENCE) {
+        c.pi = (double)3.1415926535898f; c.e = (double)1.6021892E-19f; c.na = (double)6.022045E23f;
+        c.eps0 = (double)8.85418782E-12f; c.k4pi = (double)8.987551784952e+9f; c.a2m = (double)1.0E-10f;
+        c.a1 = (double)0.254829592f; c.a2 = (double)-0.284496736f; c.a3 = (double)1.421413741f;
+        c.a4 = (double)-1.453152027f; c.a5 = (double)1.061405429f; c.p = (double)0.3275911f;
+        c.c309 = (double)3.09f; c.e30 = (double)1e-30f; c.e5 = (double)1e-5f;
+    } else {
+        c.pi = 3.14159265358979323846; c.e = 1.6021892E-19; c.na = 6.022045E23;
+        c.eps0 = 1.0 / (4.0 * 3.14159265358979323846 * 8.987551784952e+9); c.k4pi = 8.987551784952e+9;
+        c.a2m = 1.0E-10; c.a1 = 0.254829592; c.a2 = -0.284496736; c.a3 = 1.421413741; c.a4 = -1.453152027;
+        c.a5 = 1.061405429; c.p = 0.3275911; c.c309 = 3.09; c.e30 = 1e-30; c.e5 = 1e-5;
+    }
+    return c;
+}
+
+template <typename T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    hipError_t alloc(size_t cnt) {
+        n = cnt;
+        return hipMalloc((void **)&p, sizeof(T) * (cnt > 0 ? cnt : 1));
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+static const char *STAGE_NAMES[] = {"list", "pair", "sf", "sf_reduce", "backproj", "epilogue", "reduce"};
+constexpr int N_STAGES = 7;
+
+struct mdf_system {
+    mdf_topology top;
+    mdf_params par;
+    Consts c;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = true;
+    int nijt = 0, n_lead = 0, n_wall = 0;
+    DevSys S;
+    // host copies
+    std::vector<int> h_lead, h_lead_slot, h_itype;
+    std::vector<int> slot_type, cls_type;
+    std::vector<KChunk> chunks;
+    std::vector<KRow> rows;
+    std::vector<int> kvec;
+    std::vector<double> ak, s_wall;
+    double ewqWW = 0, uself = 0, crf0 = 0;
+    WallLR wl;
+    bool fixed_cap = false;
+    // device buffers
+    DBuf<int> itype, nsi, ijtype, vkind, cls, lead, lead_slot, con_ptr, con_idx;
+    DBuf<double> q, eps4, sig6, aa, rhoi, cc;
+    DBuf<double> x, y, z, cx, cy, cz, ocx, ocy, ocz, ox, oy, oz, fx, fy, fz, part;
+    DBuf<int> nbr_mm, cnt_mm, nbr_mw, cnt_mw;
+    DBuf<int> cell_cnt, cell_start, cell_items, cell_items2, cell_cursor, lead_cell;
+    DBuf<int> wcell_start, wcell_items;
+    DBuf<DevFlags> flags;
+    DBuf<double> dip_part, dipz;
+    DBuf<KChunk> d_chunks;
+    DBuf<KRow> d_rows;
+    DBuf<int> d_kvec;
+    DBuf<double> d_ak;
+    DBuf<double2> d_swall, d_s, d_t, sf_partial;
+    DBuf<double> kpart, fpart, eppart, colpart, fcom, torque, e_vdw, e_coul;
+    DBuf<WallLR> d_wl;
+    DBuf<Scalars> d_scal;
+    // launch geometry
+    int skin_blocks = 1, n_sb = 1, n_cg = 1, cpb = 1, n_kblk = 1, n_rg = 1, rpg = 1, n_epblk = 1, n_colblk = 1,
+        colper = 1, ncells_xy = 1, n_cells_cap = 1;
+    // timing
+    hipEvent_t ev[N_STAGES + 1] = {};
+    float stage_ms[N_STAGES] = {};
+    bool timing = true;
+    bool pending = false;
+    int last_rebuilt = 0;
+    DevFlags last_flags;
+};
+
+static void free_all(mdf_system *s) {
+    DBuf<int> *ib[] = {&s->itype, &s->nsi, &s->ijtype, &s->vkind, &s->cls, &s->lead, &s->lead_slot, &s->con_ptr,
+                       &s->con_idx, &s->nbr_mm, &s->cnt_mm, &s->nbr_mw, &s->cnt_mw, &s->cell_cnt, &s->cell_start,
+                       &s->cell_items, &s->cell_items2, &s->cell_cursor, &s->lead_cell, &s->wcell_start,
+                       &s->wcell_items, &s->d_kvec};
+    for (auto *b : ib) b->release();
+    DBuf<double> *db[] = {&s->q, &s->eps4, &s->sig6, &s->aa, &s->rhoi, &s->cc, &s->x, &s->y, &s->z, &s->cx,
+                          &s->cy, &s->cz, &s->ocx, &s->ocy, &s->ocz, &s->ox, &s->oy, &s->oz, &s->fx, &s->fy,
+                          &s->fz, &s->part, &s->dip_part, &s->dipz, &s->d_ak, &s->kpart, &s->fpart, &s->eppart,
+                          &s->colpart, &s->fcom, &s->torque, &s->e_vdw, &s->e_coul};
+    for (auto *b : db) b->release();
+    s->flags.release(); s->d_chunks.release(); s->d_rows.release(); s->d_swall.release(); s->d_s.release();
+    s->d_t.release(); s->sf_partial.release(); s->d_wl.release(); s->d_scal.release();
+}
+
+template <typename T>
+static hipError_t upload(DBuf<T> &b, const T *h, size_t n) {
+    hipError_t e = b.alloc(n);
+    if (e != hipSuccess) return e;
+    if (n) return hipMemcpy(b.p, h, sizeof(T) * n, hipMemcpyHostToDevice);
+    return hipSuccess;
+}
+
+// ---- force_real_init wall tail tables (src/forcem.f90:157-231) --------------------------
+static void wall_tables(mdf_system *s, const double *q_unused) {
+    (void)q_unused;
+    const mdf_topology &t = s->top;
+    WallLR &w = s->wl;
+    memset(&w, 0, sizeof w);
+    int nit = t.n_types, nliq = nit - 2;
+    double lx = s->par.box[0], ly = s->par.box[1], lw = t.wall_lw, rcut = s->par.rcut;
+    double rholiq = (double)t.n_move / (lx * ly * lw);
+    double rhowall = (double)(t.wall_nss[0] + t.wall_nss[1]) / (2.0 * t.wall_cell[0] * t.wall_cell[1] * t.wall_cell[2]);
+    double xd[MAX_TYPES + 2] = {0};
+    for (int i = 0; i < t.n_move; i++) xd[t.site_type[i]] += 1.0;
+    for (int it = 1; it <= nliq; it++) xd[it] = xd[it] / (double)(float)t.n_move * rholiq;
+    float tot = (float)(t.wall_nss[0] + t.wall_nss[1]);
+    xd[nit - 1] = (double)((float)t.wall_nss[0] / tot) * rhowall;
+    xd[nit] = (double)((float)t.wall_nss[1] / tot) * rhowall;
+    for (int it = 0; it <= MAX_TYPES; it++) w.flr_bur[it] = 1.0;
+    for (int it = 1; it <= nliq; it++) {
+        for (int jt = 1; jt <= nit; jt++) {
+            int ijt = t.ijtype[(it - 1) * nit + (jt - 1)] - 1;
+            double sgn = jt <= nliq ? -1.0 : 1.0, flr = 0.0, ul = 0.0;
+            if (t.vdw_kind[ijt] == 1) {
+                double e = t.vdw_eps4[ijt], g = t.vdw_sig6[ijt];
+                flr = e * g * (g / (10.0 * pow(rcut, 10)) - 1.0 / (4.0 * pow(rcut, 4)));
+                ul = e * g * (g / (90.0 * pow(rcut, 9)) - 1.0 / (12.0 * pow(rcut, 3)));
+                w.flr_lja[it] += sgn * xd[jt] * e * g * g / 10.0;
+                w.flr_ljc[it] += sgn * xd[jt] * e * g / 4.0;
+            } else if (t.vdw_kind[ijt] == 2) {
+                double ro = 1.0 / t.vdw_rhoi[ijt];
+                flr = ro * (ro + rcut) * t.vdw_a[ijt] * exp(-rcut / ro) - t.vdw_c[ijt] / (4.0 * pow(rcut, 4));
+                ul = ro * ro * (2.0 * ro + rcut) * t.vdw_a[ijt] * exp(-rcut / ro) - t.vdw_c[ijt] / (12.0 * pow(rcut, 3));
+                w.flr_bua[it] += sgn * xd[jt] * t.vdw_a[ijt] * ro;
+                w.flr_buc[it] += sgn * xd[jt] * t.vdw_c[ijt] / 4.0;
+                w.flr_bur[it] = ro;
+            }
+            if (sgn < 0.0) { w.flr_l[it] += flr * xd[jt]; w.ulr_l[it] += ul * xd[jt]; }
+            else { w.flr_w[it] += flr * xd[jt]; w.ulr_w[it] += ul * xd[jt]; }
+        }
+    }
+    double tp = 2.0 * s->c.pi;
+    for (int it = 0; it <= MAX_TYPES; it++) {
+        w.flr_lja[it] *= tp; w.flr_ljc[it] *= tp; w.flr_bua[it] *= tp; w.flr_buc[it] *= tp;
+        w.flr_l[it] *= tp; w.flr_w[it] *= tp; w.ulr_l[it] *= tp; w.ulr_w[it] *= tp;
+    }
+    w.rcut = rcut;
+    w.on = s->par.wall_lr;
+}
+
+// ---- k-set (src/forcem.f90:664-776): rows of contiguous kn, chunks of <= KC -------------
+static void kset(mdf_system *s) {
+    double lx = s->par.box[0], ly = s->par.box[1], lz = s->par.box[2];
+    double lmin = std::min(lx, std::min(ly, lz));
+    int kmax = s->par.kmax;
+    double pipix = 2.0 * s->c.pi / lx, pipiy = 2.0 * s->c.pi / ly, pipiz = 2.0 * s->c.pi / lz;
+    double alf4i = -1.0 / (4.0 * s->S.alfa * s->S.alfa);
+    double fac = 2.0 * s->c.e * s->c.e * s->c.na / (s->c.eps0 * s->c.a2m * 1000.0 * lx * ly * lz);
+    int klmax = (int)lround((double)(float)kmax * lx / lmin);
+    int kmmax = (int)lround((double)(float)kmax * ly / lmin);
+    int knmax = (int)lround((double)(float)kmax * lz / lmin);
+    s->kvec.clear(); s->ak.clear(); s->rows.clear(); s->chunks.clear();
+    int kmmin = 0, knmin = 1;
+    for (int kl = 0; kl <= klmax; kl++) {
+        for (int km = kmmin; km <= kmmax; km++) {
+            int first = -1, last = -1;
+            for (int kn = knmin; kn <= knmax; kn++) {
+                float ksq = (float)(kl * kl + km * km) +
+                            (float)((long long)kn * kn * kmax * kmax) / (float)((long long)knmax * knmax);
+                if (ksq < (float)(kmax * kmax) + 2.01f) {
+                    double rl = pipix * (double)(float)kl, rm = pipiy * (double)(float)km, rn = pipiz * (double)(float)kn;
+                    double rksq = rl * rl + rm * rm + rn * rn;
+                    if (first < 0) first = kn;
+                    last = kn;
+                    s->kvec.push_back(kl); s->kvec.push_back(km); s->kvec.push_back(kn);
+                    s->ak.push_back(fac * exp(rksq * alf4i) / rksq);
+                }
+            }
+            if (first >= 0) {
+                int koff = (int)s->ak.size() - (last - first + 1);
+                s->rows.push_back({kl, km, first, last, koff});
+                for (int kn0 = first; kn0 <= last; kn0 += KC)
+                    s->chunks.push_back({kl, km, kn0, std::min(KC, last - kn0 + 1), koff + (kn0 - first)});
+            }
+            knmin = -knmax;
+        }
+        kmmin = -kmmax;
+    }
+}
+
+// sets up launch geometry for the current sizes
+static void geometry(mdf_system *s) {
+    int nm = s->top.n_move;
+    int work = std::max(s->top.n_rig + (nm - s->top.n_rigid_sites), std::max(nm, s->n_lead));
+    s->skin_blocks = std::max(1, std::min(1024, (work + 255) / 256));
+    int per_sb = SF_BLOCK * 4;
+    s->n_sb = std::max(1, (nm + per_sb - 1) / per_sb);
+    int nch = (int)s->chunks.size();
+    s->n_cg = std::max(1, std::min(nch, (2048 + s->n_sb - 1) / s->n_sb));
+    s->cpb = std::max(1, (nch + s->n_cg - 1) / s->n_cg);
+    s->n_cg = std::max(1, (nch + s->cpb - 1) / s->cpb);
+    int nk = (int)s->ak.size();
+    s->n_kblk = std::max(1, (nk + 255) / 256);
+    int nsb_bp = std::max(1, (nm + 255) / 256);
+    int nrows = (int)s->rows.size();
+    s->n_rg = std::max(1, std::min(std::min(nrows, BP_ROWGROUPS_MAX), (2048 + nsb_bp - 1) / nsb_bp));
+    s->rpg = std::max(1, (nrows + s->n_rg - 1) / s->n_rg);
+    s->n_rg = std::max(1, (nrows + s->rpg - 1) / s->rpg);
+    s->n_epblk = std::max(1, (nm + 255) / 256);
+    s->n_colblk = std::max(1, std::min(64, (s->n_lead + 1023) / 1024));
+    s->colper = std::max(1, (s->n_lead + s->n_colblk - 1) / s->n_colblk);
+}
+
+static int alloc_lists(mdf_system *s, int cap_mm, int cap_mw) {
+    s->nbr_mm.release(); s->nbr_mw.release();
+    HIPCHK(s->nbr_mm.alloc((size_t)s->n_lead * cap_mm));
+    HIPCHK(s->nbr_mw.alloc((size_t)s->n_lead * std::max(cap_mw, 1)));
+    s->S.nbr_mm = s->nbr_mm.p; s->S.cap_mm = cap_mm;
+    s->S.nbr_mw = s->nbr_mw.p; s->S.cap_mw = std::max(cap_mw, 1);
+    return MDF_OK;
+}
+
+// list-building stage of the launch sequence (skin test, then a build when due)
+static void launch_lists(mdf_system *s, int force_rebuild) {
+    DevSys &S = s->S;
+    hipStream_t st = s->stream;
+    k_reset<<<1, 64, 0, st>>>(S, force_rebuild);
+    k_skin<<<s->skin_blocks, 256, 0, st>>>(S, s->dip_part.p);
+    k_decide<<<1, 64, 0, st>>>(S, s->dip_part.p, s->skin_blocks, s->dipz.p);
+    k_cell_clear<<<64, 256, 0, st>>>(S);
+    int lb = std::max(1, (s->n_lead + 255) / 256);
+    k_cell_count<<<lb, 256, 0, st>>>(S, s->lead_cell.p);
+    k_cell_scan<<<1, 1024, 0, st>>>(s->cell_cnt.p, s->cell_start.p, s->cell_cursor.p, s->flags.p, s->ncells_xy, 1, 0);
+    k_cell_fill<<<lb, 256, 0, st>>>(s->lead_cell.p, s->n_lead, s->cell_cursor.p, s->cell_items2.p, s->flags.p, 1);
+    k_cell_sort<<<512, 64, 0, st>>>(s->cell_start.p, s->cell_items2.p, s->cell_items.p, s->flags.p, 1, s->ncells_xy, 0);
+    k_build<<<std::max(1, (s->n_lead + 127) / 128), 128, 0, st>>>(S);
+    k_snapshot<<<std::max(1, (std::max(s->top.n_rig, s->top.n_move) + 255) / 256), 256, 0, st>>>(S);
+}
+
+static void launch_forces(mdf_system *s) {
+    DevSys &S = s->S;
+    hipStream_t st = s->stream;
+    const bool f32 = s->par.precision == MDF_FP32_ACC64;
+    const int nm = s->top.n_move;
+    const double cux = 2.0 * s->c.pi / s->par.box[0] / M_PI;
+    const double cuy = 2.0 * s->c.pi / s->par.box[1] / M_PI;
+    const double cuz = 2.0 * s->c.pi / s->par.box[2] / M_PI;
+    const double pipix = 2.0 * s->c.pi / s->par.box[0], pipiy = 2.0 * s->c.pi / s->par.box[1],
+                 pipiz = 2.0 * s->c.pi / s->par.box[2];
+    if (s->timing) (void)hipEventRecord(s->ev[1], st);
+    {
+        const int G = 16;
+        int blocks = std::max(1, (s->n_lead + (256 / G) - 1) / (256 / G));
+        if (f32) k_pair<float, 16><<<blocks, 256, 0, st>>>(S);
+        else k_pair<double, 16><<<blocks, 256, 0, st>>>(S);
+    }
+    if (s->timing) (void)hipEventRecord(s->ev[2], st);
+    int nk = (int)s->ak.size(), nch = (int)s->chunks.size(), nrows = (int)s->rows.size();
+    if (nk > 0 && nm > 0) {
+        dim3 g1(s->n_sb, s->n_cg);
+        if (f32) k_sf<float, 4><<<g1, SF_BLOCK, 0, st>>>(S.x, S.y, S.z, S.q, nm, cux, cuy, cuz, s->d_chunks.p, nch,
+                                                           s->cpb, s->sf_partial.p, nk);
+        else k_sf<double, 4><<<g1, SF_BLOCK, 0, st>>>(S.x, S.y, S.z, S.q, nm, cux, cuy, cuz, s->d_chunks.p, nch,
+                                                        s->cpb, s->sf_partial.p, nk);
+    }
+    if (s->timing) (void)hipEventRecord(s->ev[3], st);
+    if (nk > 0) {
+        double lamb = 1.0 / (4.0 * S.alfa * S.alfa);
+        k_sf_reduce<<<s->n_kblk, 256, 0, st>>>(s->sf_partial.p, nm > 0 ? s->n_sb : 0, nk, s->d_swall.p, s->d_ak.p,
+                                               s->d_kvec.p, pipix, pipiy, pipiz, lamb, s->d_s.p, s->d_t.p,
+                                               s->kpart.p);
+    }
+    if (s->timing) (void)hipEventRecord(s->ev[4], st);
+    if (nk > 0 && nm > 0) {
+        dim3 g2(std::max(1, (nm + 255) / 256), s->n_rg);
+        if (f32) k_bp<float><<<g2, 256, 0, st>>>(S.x, S.y, S.z, S.q, nm, cux, cuy, cuz, s->d_rows.p, nrows, s->rpg,
+                                                 s->d_t.p, pipix, pipiy, pipiz, s->fpart.p);
+        else k_bp<double><<<g2, 256, 0, st>>>(S.x, S.y, S.z, S.q, nm, cux, cuy, cuz, s->d_rows.p, nrows, s->rpg,
+                                              s->d_t.p, pipix, pipiy, pipiz, s->fpart.p);
+    }
+    if (s->timing) (void)hipEventRecord(s->ev[5], st);
+    k_epilogue<<<s->n_epblk, 256, 0, st>>>(S, s->fpart.p, (nk > 0 && nm > 0) ? s->n_rg : 0, s->d_wl.p,
+                                           s->par.ew3dc, s->crf0, s->dipz.p, s->eppart.p);
+    if (s->top.n_rig > 0) k_com<<<(s->top.n_rig + 255) / 256, 256, 0, st>>>(S, s->fcom.p, s->torque.p);
+    if (s->timing) (void)hipEventRecord(s->ev[6], st);
+    {
+        dim3 gc(NCOL, s->n_colblk);
+        k_colsum<<<gc, 256, 0, st>>>(S, s->lead_slot.p, s->colper, s->colpart.p);
+        FinalArgs A;
+        memset(&A, 0, sizeof A);
+        A.n_colblk = s->n_colblk;
+        A.n_slots = (int)s->slot_type.size();
+        for (int k = 0; k < A.n_slots; k++) A.slot_type[k] = s->slot_type[k];
+        for (int c = 0; c < NCLS; c++) A.cls_type[c] = s->cls_type[c];
+        A.n_kblk = nk > 0 ? s->n_kblk : 0;
+        A.n_epblk = s->n_epblk;
+        A.e_self = s->uself;
+        A.e_recip_ww = s->ewqWW;
+        A.crf0 = s->crf0;
+        A.ew3dc = s->par.ew3dc;
+        A.lx = s->par.box[0];
+        A.ly = s->par.box[1];
+        A.na_scale = s->c.na * s->c.e30;
+        A.bar = s->c.e5;
+        k_final<<<1, 256, 0, st>>>(S, A, s->colpart.p, s->kpart.p, s->eppart.p, s->dipz.p, s->e_vdw.p, s->e_coul.p,
+                                   s->d_scal.p);
+    }
+    if (s->timing) (void)hipEventRecord(s->ev[7], st);
+}
+
+static int run_step(mdf_system *s, int force_rebuild) {
+    if (s->timing) (void)hipEventRecord(s->ev[0], s->stream);
+    launch_lists(s, force_rebuild);
+    launch_forces(s);
+    HIPCHK(hipGetLastError());
+    s->pending = true;
+    return MDF_OK;
+}
+
+// wait, read flags, grow lists and redo the step on overflow (unless the capacity was fixed)
+static int finish_step(mdf_system *s, int *redo) {
+    *redo = 0;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->pending = false;
+    HIPCHK(hipMemcpy(&s->last_flags, s->flags.p, sizeof(DevFlags), hipMemcpyDeviceToHost));
+    const DevFlags &f = s->last_flags;
+    s->last_rebuilt = f.rebuild;
+    if (f.overflow_mm || f.overflow_mw) {
+        if (s->fixed_cap)
+            return fail(MDF_ERR_LIST_OVERFLOW, "nnlist_make(): nnlist too big! need %d moving / %d frozen partners per "
+                        "lead, capacity %d / %d", f.overflow_mm, f.overflow_mw, s->S.cap_mm, s->S.cap_mw);
+        int cmm = std::max(s->S.cap_mm, (int)(f.overflow_mm * 1.25) + 16);
+        int cmw = std::max(s->S.cap_mw, (int)(f.overflow_mw * 1.25) + 16);
+        int rc = alloc_lists(s, cmm, cmw);
+        if (rc) return rc;
+        *redo = 1;
+    }
+    if (s->timing) {
+        for (int k = 0; k < N_STAGES; k++) {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, s->ev[k], s->ev[k + 1]);
+            s->stage_ms[k] = ms;
+        }
+    }
+    return MDF_OK;
+}
+
+static int copy_outputs(mdf_system *s, mdf_outputs *o) {
+    const int nm = s->top.n_move, nr = s->top.n_rig;
+    if (o->f_site) {
+        HIPCHK(hipMemcpy(o->f_site, s->fx.p, sizeof(double) * nm, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(o->f_site + nm, s->fy.p, sizeof(double) * nm, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(o->f_site + 2 * nm, s->fz.p, sizeof(double) * nm, hipMemcpyDeviceToHost));
+    }
+    if (o->f_com && nr) HIPCHK(hipMemcpy(o->f_com, s->fcom.p, sizeof(double) * 3 * nr, hipMemcpyDeviceToHost));
+    if (o->torque && nr) HIPCHK(hipMemcpy(o->torque, s->torque.p, sizeof(double) * 3 * nr, hipMemcpyDeviceToHost));
+    std::vector<double> ev(s->nijt + 1), ec(s->nijt + 1);
+    HIPCHK(hipMemcpy(ev.data(), s->e_vdw.p, sizeof(double) * (s->nijt + 1), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ec.data(), s->e_coul.p, sizeof(double) * (s->nijt + 1), hipMemcpyDeviceToHost));
+    if (o->e_vdw) memcpy(o->e_vdw, ev.data() + 1, sizeof(double) * s->nijt);
+    if (o->e_coul_real) memcpy(o->e_coul_real, ec.data() + 1, sizeof(double) * s->nijt);
+    Scalars sc;
+    HIPCHK(hipMemcpy(&sc, s->d_scal.p, sizeof(Scalars), hipMemcpyDeviceToHost));
+    o->e_recip = sc.e_recip; o->e_recip_ll = sc.e_recip_ll; o->e_recip_wl = sc.e_recip_wl;
+    o->e_recip_ww = sc.e_recip_ww; o->e_self = sc.e_self; o->e_slab = sc.e_slab; o->e_wall_lr = sc.e_wall_lr;
+    o->e_conf = sc.e_conf; o->e_avex = sc.e_avex; o->press_wall = sc.press;
+    memcpy(o->f_wall, sc.f_wall, sizeof sc.f_wall);
+    memcpy(o->virial, sc.virial, sizeof sc.virial);
+    o->list_rebuilt = s->last_rebuilt;
+    o->n_pairs_listed = s->last_flags.n_pairs_mm;
+    o->n_wall_pairs_listed = s->last_flags.n_pairs_mw;
+    o->n_kvec = (int)s->ak.size();
+    if (sc.nonfinite) return fail(MDF_ERR_NONFINITE, "non-finite configurational energy");
+    return MDF_OK;
+}
+
+extern "C" {
+
+const char *mdf_last_error(void) { return g_err.c_str(); }
+
+int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, const double *y, const double *z,
+               const double *cx, const double *cy, const double *cz, mdf_system **out) {
+    if (!top || !par || !out || !x || !y || !z) return fail(MDF_ERR_ARG, "null argument");
+    *out = nullptr;
+    const mdf_topology &t = *top;
+    if (t.n_types < 3 || t.n_types > MAX_TYPES) return fail(MDF_ERR_ARG, "n_types %d outside 3..%d", t.n_types, MAX_TYPES);
+    if (t.n_move < 0 || t.n_total < t.n_move || t.n_rigid_sites != 3 * t.n_rig || t.n_liquid_sites > t.n_move ||
+        t.n_rigid_sites > t.n_liquid_sites)
+        return fail(MDF_ERR_ARG, "inconsistent site counts");
+    if (t.n_rig > 0 && (!cx || !cy || !cz)) return fail(MDF_ERR_ARG, "COM arrays required");
+    if (par->rcut <= 0 || par->skin < 0 || par->kmax < 0) return fail(MDF_ERR_ARG, "bad cut-off/skin/kmax");
+    if (par->box[0] <= 0 || par->box[1] <= 0 || par->box[2] <= 0) return fail(MDF_ERR_ARG, "bad box");
+    int dev_count = 0;
+    if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0)
+        return fail(MDF_ERR_NO_DEVICE, "no HIP device visible");
+    if (par->device < 0 || par->device >= dev_count) return fail(MDF_ERR_ARG, "device %d not present", par->device);
+    const int nit = t.n_types, nijt = nit * (nit + 1) / 2;
+    for (int i = 0; i < t.n_total; i++)
+        if (t.site_type[i] < 1 || t.site_type[i] > nit) return fail(MDF_ERR_ARG, "site %d has type %d", i, t.site_type[i]);
+    for (int k = 0; k < nijt; k++)
+        if (t.vdw_kind[k] < 0 || t.vdw_kind[k] > 2)
+            return fail(MDF_ERR_VDW_TYPE, "force_real(): unknown interaction type %d for pair %d", t.vdw_kind[k], k + 1);
+    for (int it = 0; it < nit; it++)
+        if (t.nsi[it] < 0 || t.nsi[it] > 3) return fail(MDF_ERR_ARG, "nsi(%d) = %d: at most 3 sites per molecule", it + 1, t.nsi[it]);
+    for (int m = 0; m < t.n_rig; m++)
+        if (t.nsi[t.site_type[3 * m] - 1] != 3 || t.nsi[t.site_type[3 * m + 1] - 1] != 0 ||
+            t.nsi[t.site_type[3 * m + 2] - 1] != 0)
+            return fail(MDF_ERR_ARG, "rigid molecule %d is not a 3-site lead + 2 carried sites", m);
+
+    mdf_system *s = new mdf_system();
+    s->top = t;
+    s->par = *par;
+    s->c = make_consts(par->const_mode);
+    s->device = par->device;
+    s->nijt = nijt;
+    s->n_wall = t.n_total - t.n_move;
+    hipError_t he = hipSetDevice(par->device);
+    if (he != hipSuccess) { delete s; return fail(MDF_ERR_HIP, "hipSetDevice: %s", hipGetErrorString(he)); }
+    int rc = MDF_OK;
+#define TRY(x)                        \
+    do {                              \
+        rc = (x);                     \
+        if (rc) { goto fail_out; }    \
+    } while (0)
+#define HTRY(x)                                                                                        \
+    do {                                                                                               \
+        hipError_t e_ = (x);                                                                           \
+        if (e_ != hipSuccess) { rc = fail(MDF_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); goto fail_out; } \
+    } while (0)
+    {
+        HTRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+        for (int k = 0; k <= N_STAGES; k++) HTRY(hipEventCreate(&s->ev[k]));
+        // leads, lead-type slots, energy classes
+        std::vector<int> lead_types;
+        for (int i = 0; i < t.n_move; i++) {
+            int ti = t.site_type[i];
+            if (t.nsi[ti - 1] > 0) {
+                s->h_lead.push_back(i);
+                int slot = -1;
+                for (size_t k = 0; k < lead_types.size(); k++)
+                    if (lead_types[k] == ti) slot = (int)k;
+                if (slot < 0) {
+                    if ((int)lead_types.size() >= MAX_LEAD_TYPES) { rc = fail(MDF_ERR_ARG, "more than %d moving lead types", MAX_LEAD_TYPES); goto fail_out; }
+                    lead_types.push_back(ti);
+                    slot = (int)lead_types.size() - 1;
+                }
+                s->h_lead_slot.push_back(slot);
+            }
+        }
+        s->n_lead = (int)s->h_lead.size();
+        s->slot_type = lead_types;
+        std::vector<int> cls(nit + 1, 0);
+        s->cls_type.assign(NCLS, 0);
+        for (size_t k = 0; k < lead_types.size(); k++) { cls[lead_types[k]] = (int)k; s->cls_type[k] = lead_types[k]; }
+        // crystal types: the last two (src/sysdef.f90:418); classes 4 and 5
+        cls[nit - 1] = MAX_LEAD_TYPES; cls[nit] = MAX_LEAD_TYPES + 1;
+        s->cls_type[MAX_LEAD_TYPES] = nit - 1; s->cls_type[MAX_LEAD_TYPES + 1] = nit;
+        for (int w = t.n_move; w < t.n_total; w++)
+            if (t.site_type[w] < nit - 1) { rc = fail(MDF_ERR_ARG, "frozen site %d has a non-crystal type", w); goto fail_out; }
+        // 1-based tables
+        std::vector<int> nsi1(nit + 1, 0), ij1((nit + 1) * (nit + 1), 0), vk1(nijt + 1, 0);
+        std::vector<double> e41(nijt + 1, 0), s61(nijt + 1, 0), a1(nijt + 1, 0), r1(nijt + 1, 0), c1(nijt + 1, 0);
+        for (int it = 1; it <= nit; it++) nsi1[it] = t.nsi[it - 1];
+        for (int it = 1; it <= nit; it++)
+            for (int jt = 1; jt <= nit; jt++) ij1[it * (nit + 1) + jt] = t.ijtype[(it - 1) * nit + (jt - 1)];
+        for (int k = 0; k < nijt; k++) {
+            vk1[k + 1] = t.vdw_kind[k]; e41[k + 1] = t.vdw_eps4[k]; s61[k + 1] = t.vdw_sig6[k];
+            a1[k + 1] = t.vdw_a[k]; r1[k + 1] = t.vdw_rhoi[k]; c1[k + 1] = t.vdw_c[k];
+        }
+        // constraint partners (CSR over moving sites)
+        std::vector<int> cptr(t.n_move + 1, 0), cidx;
+        {
+            std::vector<std::vector<int>> adj(t.n_move);
+            for (int k = 0; k < t.n_constraints; k++) {
+                int a = t.con_a[k], b = t.con_b[k];
+                if (a < 0 || b < 0 || a >= t.n_total || b >= t.n_total) { rc = fail(MDF_ERR_ARG, "constraint %d out of range", k); goto fail_out; }
+                if (a < t.n_move) adj[a].push_back(b);
+                if (b < t.n_move) adj[b].push_back(a);
+            }
+            for (int i = 0; i < t.n_move; i++) { cptr[i + 1] = cptr[i] + (int)adj[i].size(); cidx.insert(cidx.end(), adj[i].begin(), adj[i].end()); }
+        }
+        DevSys &S = s->S;
+        memset(&S, 0, sizeof S);
+        S.n_total = t.n_total; S.n_move = t.n_move; S.n_rig = t.n_rig; S.n_rigid_sites = t.n_rigid_sites;
+        S.n_liquid_sites = t.n_liquid_sites; S.n_lead = s->n_lead; S.n_wall = s->n_wall; S.n_types = nit;
+        S.n_ltype = t.n_ltype; S.nijt = nijt;
+        HTRY(upload(s->itype, t.site_type, t.n_total));
+        HTRY(upload(s->q, t.charge, t.n_total));
+        HTRY(upload(s->nsi, nsi1.data(), nsi1.size()));
+        HTRY(upload(s->ijtype, ij1.data(), ij1.size()));
+        HTRY(upload(s->vkind, vk1.data(), vk1.size()));
+        HTRY(upload(s->eps4, e41.data(), e41.size()));
+        HTRY(upload(s->sig6, s61.data(), s61.size()));
+        HTRY(upload(s->aa, a1.data(), a1.size()));
+        HTRY(upload(s->rhoi, r1.data(), r1.size()));
+        HTRY(upload(s->cc, c1.data(), c1.size()));
+        HTRY(upload(s->cls, cls.data(), cls.size()));
+        HTRY(upload(s->lead, s->h_lead.data(), s->h_lead.size()));
+        HTRY(upload(s->lead_slot, s->h_lead_slot.data(), s->h_lead_slot.size()));
+        HTRY(upload(s->con_ptr, cptr.data(), cptr.size()));
+        HTRY(upload(s->con_idx, cidx.data(), cidx.size()));
+        S.itype = s->itype.p; S.q = s->q.p; S.nsi = s->nsi.p; S.ijtype = s->ijtype.p; S.vkind = s->vkind.p;
+        S.eps4 = s->eps4.p; S.sig6 = s->sig6.p; S.aa = s->aa.p; S.rhoi = s->rhoi.p; S.cc = s->cc.p; S.cls = s->cls.p;
+        S.lead = s->lead.p; S.con_ptr = s->con_ptr.p; S.con_idx = s->con_idx.p;
+        // positions (all sites once; moving part per call)
+        HTRY(upload(s->x, x, t.n_total)); HTRY(upload(s->y, y, t.n_total)); HTRY(upload(s->z, z, t.n_total));
+        int nr = std::max(t.n_rig, 1);
+        std::vector<double> zero(nr, 0.0);
+        HTRY(upload(s->cx, t.n_rig ? cx : zero.data(), nr)); HTRY(upload(s->cy, t.n_rig ? cy : zero.data(), nr));
+        HTRY(upload(s->cz, t.n_rig ? cz : zero.data(), nr));
+        HTRY(s->ocx.alloc(nr)); HTRY(s->ocy.alloc(nr)); HTRY(s->ocz.alloc(nr));
+        HTRY(s->ox.alloc(std::max(t.n_move, 1))); HTRY(s->oy.alloc(std::max(t.n_move, 1))); HTRY(s->oz.alloc(std::max(t.n_move, 1)));
+        HTRY(s->fx.alloc(std::max(t.n_move, 1))); HTRY(s->fy.alloc(std::max(t.n_move, 1))); HTRY(s->fz.alloc(std::max(t.n_move, 1)));
+        HTRY(s->part.alloc((size_t)NCOL * std::max(s->n_lead, 1)));
+        S.x = s->x.p; S.y = s->y.p; S.z = s->z.p; S.cx = s->cx.p; S.cy = s->cy.p; S.cz = s->cz.p;
+        S.ocx = s->ocx.p; S.ocy = s->ocy.p; S.ocz = s->ocz.p; S.ox = s->ox.p; S.oy = s->oy.p; S.oz = s->oz.p;
+        S.fx = s->fx.p; S.fy = s->fy.p; S.fz = s->fz.p; S.part = s->part.p;
+        // box and constants (force_init, src/forcem.f90:58-60; config_init src/config.f90:67-69)
+        S.lx = par->box[0]; S.ly = par->box[1]; S.lz = par->box[2];
+        S.lxi = 1.0 / S.lx; S.lyi = 1.0 / S.ly; S.lzi = 1.0 / S.lz;
+        S.lwh = 0.5 * t.wall_lw;
+        S.zper_list = par->z_mode == MDF_Z_PERIODIC;
+        S.zper_force = par->z_mode != MDF_Z_OPEN;
+        S.rcutsq = par->rcut * par->rcut;
+        S.skin2 = par->skin * par->skin;
+        S.spii = 2.0 / sqrt(s->c.pi);
+        S.alfa = par->alpha_mode == MDF_ALPHA_REFERENCE ? s->c.c309 / par->rcut : par->alpha;
+        S.facq = s->c.k4pi * s->c.e * s->c.e * s->c.na / (1000.0 * s->c.a2m);
+        S.a1 = s->c.a1; S.a2 = s->c.a2; S.a3 = s->c.a3; S.a4 = s->c.a4; S.a5 = s->c.a5; S.p = s->c.p;
+        S.erfc_mode = par->erfc_mode;
+        {
+            double off = 0.0;
+            if (t.n_rig > 0)
+                off = sqrt((x[0] - cx[0]) * (x[0] - cx[0]) + (y[0] - cy[0]) * (y[0] - cy[0]) + (z[0] - cz[0]) * (z[0] - cz[0]));
+            double rs = par->rcut + par->skin;
+            rs = rs + 2.0 * off;
+            S.rskinsq = rs * rs;
+        }
+        // cell grid: cells at least the list radius wide
+        {
+            double rs = sqrt(S.rskinsq) * (1.0 + 1e-9);
+            S.g.ncx = std::max(1, (int)floor(S.lx / rs));
+            S.g.ncy = std::max(1, (int)floor(S.ly / rs));
+            S.g.csx = S.lx / S.g.ncx; S.g.csy = S.ly / S.g.ncy;
+            S.g.zper = S.zper_list;
+            S.g.ncz_fixed = std::max(1, (int)floor(S.lz / rs));
+            S.g.csz_fixed = S.lz / S.g.ncz_fixed;
+            S.g.ncz_max = S.g.zper ? S.g.ncz_fixed : 1024;
+            s->ncells_xy = S.g.ncx * S.g.ncy;
+            s->n_cells_cap = s->ncells_xy * S.g.ncz_max;
+            S.n_cells_cap = s->n_cells_cap;
+            HTRY(s->cell_cnt.alloc(s->n_cells_cap + 1)); HTRY(s->cell_start.alloc(s->n_cells_cap + 1));
+            HTRY(s->cell_cursor.alloc(s->n_cells_cap + 1));
+            HTRY(s->cell_items.alloc(std::max(s->n_lead, 1))); HTRY(s->cell_items2.alloc(std::max(s->n_lead, 1)));
+            HTRY(s->lead_cell.alloc(std::max(s->n_lead, 1)));
+            S.cell_cnt = s->cell_cnt.p; S.cell_start = s->cell_start.p; S.cell_items = s->cell_items.p;
+            S.cell_cursor = s->cell_cursor.p;
+        }
+        HTRY(s->flags.alloc(1));
+        S.flags = s->flags.p;
+        // frozen-site cells, built once
+        {
+            int nw = s->n_wall;
+            double wz0 = 0.0, wcsz = 1.0;
+            int wncz = 1;
+            if (S.g.zper) {
+                wncz = S.g.ncz_fixed; wcsz = S.g.csz_fixed;
+            } else if (nw > 0) {
+                double lo = z[t.n_move], hi = z[t.n_move];
+                for (int w = t.n_move; w < t.n_total; w++) { lo = std::min(lo, z[w]); hi = std::max(hi, z[w]); }
+                double rs = sqrt(S.rskinsq) * (1.0 + 1e-9);
+                wncz = std::max(1, std::min(1024, (int)floor((hi - lo) / rs)));
+                wz0 = lo;
+                wcsz = hi > lo ? (hi - lo) / wncz : 1.0;
+            }
+            int nwc = s->ncells_xy * wncz;
+            DBuf<int> wcell_of, wcnt, wcur, witems2;
+            HTRY(wcell_of.alloc(std::max(nw, 1))); HTRY(wcnt.alloc(nwc + 1)); HTRY(wcur.alloc(nwc + 1));
+            HTRY(witems2.alloc(std::max(nw, 1)));
+            HTRY(s->wcell_start.alloc(nwc + 1)); HTRY(s->wcell_items.alloc(std::max(nw, 1)));
+            k_zero_int<<<64, 256, 0, s->stream>>>(wcnt.p, nwc + 1);
+            if (nw > 0) {
+                k_wall_cells<<<(nw + 255) / 256, 256, 0, s->stream>>>(S, wcell_of.p, wncz, wz0, wcsz);
+                k_count_cells<<<(nw + 255) / 256, 256, 0, s->stream>>>(wcell_of.p, nw, wcnt.p);
+            }
+            k_cell_scan<<<1, 1024, 0, s->stream>>>(wcnt.p, s->wcell_start.p, wcur.p, s->flags.p, s->ncells_xy, 0, wncz);
+            if (nw > 0) {
+                k_cell_fill<<<(nw + 255) / 256, 256, 0, s->stream>>>(wcell_of.p, nw, wcur.p, witems2.p, s->flags.p, 0);
+                k_cell_sort<<<512, 64, 0, s->stream>>>(s->wcell_start.p, witems2.p, s->wcell_items.p, s->flags.p, 0,
+                                                       s->ncells_xy, wncz);
+            }
+            HTRY(hipStreamSynchronize(s->stream));
+            wcell_of.release(); wcnt.release(); wcur.release(); witems2.release();
+            S.wcell_start = s->wcell_start.p; S.wcell_items = s->wcell_items.p;
+            S.wncz = wncz; S.wz0 = wz0; S.wcsz = wcsz;
+        }
+        // lists
+        HTRY(s->cnt_mm.alloc(std::max(s->n_lead, 1))); HTRY(s->cnt_mw.alloc(std::max(s->n_lead, 1)));
+        S.cnt_mm = s->cnt_mm.p; S.cnt_mw = s->cnt_mw.p;
+        {
+            int cap_mm = 256, cap_mw = s->n_wall > 0 ? 256 : 1;
+            if (par->max_neighbors > 0) { cap_mm = par->max_neighbors; cap_mw = par->max_neighbors; s->fixed_cap = true; }
+            TRY(alloc_lists(s, cap_mm, cap_mw));
+        }
+        // k-set, A_k, static frozen S(k), self energy
+        kset(s);
+        geometry(s);
+        {
+            int nk = (int)s->ak.size();
+            HTRY(upload(s->d_chunks, s->chunks.data(), s->chunks.size()));
+            HTRY(upload(s->d_rows, s->rows.data(), s->rows.size()));
+            HTRY(upload(s->d_kvec, s->kvec.data(), s->kvec.size()));
+            HTRY(upload(s->d_ak, s->ak.data(), s->ak.size()));
+            HTRY(s->d_s.alloc(std::max(nk, 1))); HTRY(s->d_t.alloc(std::max(nk, 1)));
+            HTRY(s->kpart.alloc((size_t)8 * s->n_kblk));
+            std::vector<double2> zero2(std::max(nk, 1), make_double2(0.0, 0.0));
+            HTRY(upload(s->d_swall, zero2.data(), zero2.size()));
+            int nw = s->n_wall;
+            int per_sb = SF_BLOCK * 4;
+            int n_sb_w = std::max(1, (nw + per_sb - 1) / per_sb);
+            HTRY(s->sf_partial.alloc((size_t)std::max(s->n_sb, n_sb_w) * std::max(nk, 1)));
+            HTRY(s->fpart.alloc((size_t)s->n_rg * 3 * std::max(t.n_move, 1)));
+            s->s_wall.assign(2 * std::max(nk, 1), 0.0);
+            s->ewqWW = 0.0;
+            if (nw > 0 && nk > 0) {
+                const double cux = 2.0 * s->c.pi / S.lx / M_PI, cuy = 2.0 * s->c.pi / S.ly / M_PI,
+                             cuz = 2.0 * s->c.pi / S.lz / M_PI;
+                int nch = (int)s->chunks.size();
+                int ncg = std::max(1, std::min(nch, (2048 + n_sb_w - 1) / n_sb_w));
+                int cpb = (nch + ncg - 1) / ncg;
+                ncg = (nch + cpb - 1) / cpb;
+                k_sf<double, 4><<<dim3(n_sb_w, ncg), SF_BLOCK, 0, s->stream>>>(
+                    S.x + t.n_move, S.y + t.n_move, S.z + t.n_move, S.q + t.n_move, nw, cux, cuy, cuz,
+                    s->d_chunks.p, nch, cpb, s->sf_partial.p, nk);
+                k_sf_reduce<<<s->n_kblk, 256, 0, s->stream>>>(s->sf_partial.p, n_sb_w, nk, s->d_swall.p, s->d_ak.p,
+                                                              s->d_kvec.p, 1.0, 1.0, 1.0, 0.0, s->d_s.p, s->d_t.p,
+                                                              s->kpart.p);
+                HTRY(hipStreamSynchronize(s->stream));
+                HTRY(hipMemcpy(s->s_wall.data(), s->d_s.p, sizeof(double2) * nk, hipMemcpyDeviceToHost));
+                HTRY(hipMemcpy(s->d_swall.p, s->d_s.p, sizeof(double2) * nk, hipMemcpyDeviceToDevice));
+                for (int k = 0; k < nk; k++)
+                    s->ewqWW += s->ak[k] * (s->s_wall[2 * k] * s->s_wall[2 * k] + s->s_wall[2 * k + 1] * s->s_wall[2 * k + 1]);
+            }
+            s->ewqWW *= 0.5;
+            // self energy (src/forcem.f90:794-822)
+            double q2 = 0.0;
+            for (int i = 0; i < t.n_total; i++) q2 += t.charge[i] * t.charge[i];
+            s->uself = -0.5 * S.spii * S.facq * S.alfa * q2;
+            auto erfc_as = [&](double ar) {
+                double ex = exp(-ar * ar);
+                if (par->erfc_mode == MDF_ERFC_AS7126) {
+                    double tt = 1.0 / (1.0 + s->c.p * ar);
+                    return ((((s->c.a5 * tt + s->c.a4) * tt + s->c.a3) * tt + s->c.a2) * tt + s->c.a1) * tt * ex;
+                }
+                return erfc(ar);
+            };
+            if (t.n_rig > 0)
+                for (int a = 0; a < 3; a++)
+                    for (int b = a + 1; b < 3; b++) {
+                        double rij = sqrt((x[a] - x[b]) * (x[a] - x[b]) + (y[a] - y[b]) * (y[a] - y[b]) + (z[a] - z[b]) * (z[a] - z[b]));
+                        s->uself = s->uself - (double)t.n_rig * S.facq * t.charge[a] * t.charge[b] * (1.0 - erfc_as(S.alfa * rij)) / rij;
+                    }
+            for (int k = 0; k < t.n_constraints; k++) {
+                double rij = t.con_len[k];
+                s->uself = s->uself - S.facq * t.charge[t.con_a[k]] * t.charge[t.con_b[k]] * (1.0 - erfc_as(S.alfa * rij)) / rij;
+            }
+            s->crf0 = -2.0 * s->c.pi * S.lzi * S.lxi * S.lyi;
+        }
+        // tables and reduction buffers
+        wall_tables(s, t.charge);
+        HTRY(upload(s->d_wl, &s->wl, 1));
+        HTRY(s->dip_part.alloc(s->skin_blocks)); HTRY(s->dipz.alloc(1));
+        HTRY(s->eppart.alloc((size_t)6 * s->n_epblk));
+        HTRY(s->colpart.alloc((size_t)NCOL * s->n_colblk * MAX_LEAD_TYPES));
+        HTRY(s->fcom.alloc(3 * nr)); HTRY(s->torque.alloc(3 * nr));
+        HTRY(s->e_vdw.alloc(nijt + 1)); HTRY(s->e_coul.alloc(nijt + 1));
+        HTRY(s->d_scal.alloc(1));
+        // first list (nnlist_init -> nnlist_make at force_init)
+        for (int attempt = 0; attempt < 4; attempt++) {
+            launch_lists(s, 1);
+            HTRY(hipGetLastError());
+            int redo = 0;
+            HTRY(hipStreamSynchronize(s->stream));
+            HTRY(hipMemcpy(&s->last_flags, s->flags.p, sizeof(DevFlags), hipMemcpyDeviceToHost));
+            if (s->last_flags.overflow_mm || s->last_flags.overflow_mw) {
+                if (s->fixed_cap) {
+                    rc = fail(MDF_ERR_LIST_OVERFLOW, "nnlist_make(): nnlist too big! need %d / %d partners per lead",
+                              s->last_flags.overflow_mm, s->last_flags.overflow_mw);
+                    goto fail_out;
+                }
+                TRY(alloc_lists(s, std::max(s->S.cap_mm, (int)(s->last_flags.overflow_mm * 1.25) + 16),
+                                std::max(s->S.cap_mw, (int)(s->last_flags.overflow_mw * 1.25) + 16)));
+                redo = 1;
+            }
+            if (!redo) break;
+        }
+    }
+    *out = s;
+    return MDF_OK;
+fail_out:
+    free_all(s);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    for (int k = 0; k <= N_STAGES; k++)
+        if (s->ev[k]) (void)hipEventDestroy(s->ev[k]);
+    delete s;
+    return rc;
+#undef TRY
+#undef HTRY
+}
+
+static int set_inputs(mdf_system *s, const double *x, const double *y, const double *z, const double *cx,
+                      const double *cy, const double *cz, hipMemcpyKind kind) {
+    const int nm = s->top.n_move, nr = s->top.n_rig;
+    hipStream_t st = s->stream;
+    HIPCHK(hipMemcpyAsync(s->x.p, x, sizeof(double) * nm, kind, st));
+    HIPCHK(hipMemcpyAsync(s->y.p, y, sizeof(double) * nm, kind, st));
+    HIPCHK(hipMemcpyAsync(s->z.p, z, sizeof(double) * nm, kind, st));
+    if (nr > 0) {
+        HIPCHK(hipMemcpyAsync(s->cx.p, cx, sizeof(double) * nr, kind, st));
+        HIPCHK(hipMemcpyAsync(s->cy.p, cy, sizeof(double) * nr, kind, st));
+        HIPCHK(hipMemcpyAsync(s->cz.p, cz, sizeof(double) * nr, kind, st));
+    }
+    return MDF_OK;
+}
+
+int mdf_compute(mdf_system *s, const double *x, const double *y, const double *z, const double *cx, const double *cy,
+                const double *cz, mdf_outputs *out) {
+    if (!s || !x || !y || !z || !out) return fail(MDF_ERR_ARG, "null argument");
+    if (s->top.n_rig > 0 && (!cx || !cy || !cz)) return fail(MDF_ERR_ARG, "COM arrays required");
+    HIPCHK(hipSetDevice(s->device));
+    for (int attempt = 0; attempt < 4; attempt++) {
+        int rc = set_inputs(s, x, y, z, cx, cy, cz, hipMemcpyHostToDevice);
+        if (rc) return rc;
+        rc = run_step(s, attempt > 0);
+        if (rc) return rc;
+        int redo = 0;
+        rc = finish_step(s, &redo);
+        if (rc) return rc;
+        if (!redo) return copy_outputs(s, out);
+    }
+    return fail(MDF_ERR_LIST_OVERFLOW, "pair list kept overflowing");
+}
+
+int mdf_compute_device(mdf_system *s, const double *d_x, const double *d_y, const double *d_z, const double *d_cx,
+                       const double *d_cy, const double *d_cz, void *stream) {
+    if (!s || !d_x || !d_y || !d_z) return fail(MDF_ERR_ARG, "null argument");
+    (void)stream;
+    HIPCHK(hipSetDevice(s->device));
+    int rc = set_inputs(s, d_x, d_y, d_z, d_cx, d_cy, d_cz, hipMemcpyDeviceToDevice);
+    if (rc) return rc;
+    return run_step(s, 0);
+}
+
+int mdf_read_outputs(mdf_system *s, mdf_outputs *out) {
+    if (!s || !out) return fail(MDF_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(s->device));
+    int redo = 0;
+    int rc = finish_step(s, &redo);
+    if (rc) return rc;
+    if (redo) {
+        // the inputs are still on the device: run the step again with the grown lists
+        rc = run_step(s, 1);
+        if (rc) return rc;
+        rc = finish_step(s, &redo);
+        if (rc) return rc;
+        if (redo) return fail(MDF_ERR_LIST_OVERFLOW, "pair list kept overflowing");
+    }
+    return copy_outputs(s, out);
+}
+
+int mdf_device_results(mdf_system *s, const double **d_f_site, const double **d_f_com, const double **d_torque) {
+    if (!s) return fail(MDF_ERR_ARG, "null handle");
+    if (d_f_site) *d_f_site = s->fx.p;  // fx, fy, fz are separate allocations: see f_site layout note
+    if (d_f_com) *d_f_com = s->fcom.p;
+    if (d_torque) *d_torque = s->torque.p;
+    return MDF_OK;
+}
+
+int mdf_zero(mdf_system *s) { return s ? MDF_OK : fail(MDF_ERR_ARG, "null handle"); }
+
+int mdf_get_pairs(mdf_system *s, int32_t *i_mm, int32_t *j_mm, int32_t cap_mm, int32_t *n_mm, int32_t *i_mw,
+                  int32_t *j_mw, int32_t cap_mw, int32_t *n_mw) {
+    if (!s || !n_mm || !n_mw) return fail(MDF_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(s->device));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    const int L = s->n_lead, cm = s->S.cap_mm, cw = s->S.cap_mw;
+    std::vector<int> cnt(L), cntw(L), nb((size_t)L * cm), nw((size_t)L * cw);
+    HIPCHK(hipMemcpy(cnt.data(), s->cnt_mm.p, sizeof(int) * L, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(cntw.data(), s->cnt_mw.p, sizeof(int) * L, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(nb.data(), s->nbr_mm.p, sizeof(int) * nb.size(), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(nw.data(), s->nbr_mw.p, sizeof(int) * nw.size(), hipMemcpyDeviceToHost));
+    std::vector<std::pair<int, int>> pm, pw;
+    for (int l = 0; l < L; l++) {
+        int i = s->h_lead[l];
+        for (int k = 0; k < std::min(cnt[l], cm); k++) {
+            int j = nb[(size_t)l * cm + k];
+            if (j > i) pm.push_back({i, j});
+        }
+        for (int k = 0; k < std::min(cntw[l], cw); k++) pw.push_back({i, nw[(size_t)l * cw + k]});
+    }
+    std::sort(pm.begin(), pm.end());
+    std::sort(pw.begin(), pw.end());
+    *n_mm = (int32_t)pm.size();
+    *n_mw = (int32_t)pw.size();
+    if ((int)pm.size() > cap_mm || (int)pw.size() > cap_mw) return MDF_ERR_ARG;
+    for (size_t k = 0; k < pm.size(); k++) { i_mm[k] = pm[k].first; j_mm[k] = pm[k].second; }
+    for (size_t k = 0; k < pw.size(); k++) { i_mw[k] = pw[k].first; j_mw[k] = pw[k].second; }
+    return MDF_OK;
+}
+
+int mdf_get_kvectors(mdf_system *s, int32_t *kvec, double *ak, int32_t cap, int32_t *n) {
+    if (!s || !n) return fail(MDF_ERR_ARG, "null argument");
+    *n = (int32_t)s->ak.size();
+    if (*n > cap) return MDF_ERR_ARG;
+    if (kvec) memcpy(kvec, s->kvec.data(), sizeof(int32_t) * s->kvec.size());
+    if (ak) memcpy(ak, s->ak.data(), sizeof(double) * s->ak.size());
+    return MDF_OK;
+}
+
+int mdf_get_structure_factor(mdf_system *s, double *sf, int32_t cap) {
+    if (!s || !sf) return fail(MDF_ERR_ARG, "null argument");
+    int nk = (int)s->ak.size();
+    if (nk > cap) return MDF_ERR_ARG;
+    HIPCHK(hipSetDevice(s->device));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    if (nk) HIPCHK(hipMemcpy(sf, s->d_s.p, sizeof(double2) * nk, hipMemcpyDeviceToHost));
+    return MDF_OK;
+}
+
+int mdf_stage_times(mdf_system *s, const char **names, float *ms, int32_t cap, int32_t *n) {
+    if (!s || !n) return fail(MDF_ERR_ARG, "null argument");
+    *n = N_STAGES;
+    for (int k = 0; k < N_STAGES && k < cap; k++) {
+        if (names) names[k] = STAGE_NAMES[k];
+        if (ms) ms[k] = s->stage_ms[k];
+    }
+    return MDF_OK;
+}
+
+void mdf_destroy(mdf_system *s) {
+    if (!s) return;
+    (void)hipSetDevice(s->device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    free_all(s);
+    for (int k = 0; k <= N_STAGES; k++)
+        if (s->ev[k]) (void)hipEventDestroy(s->ev[k]);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+}  // extern "C"

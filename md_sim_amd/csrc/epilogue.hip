@@ -1,0 +1,204 @@
+// Force assembly and reductions after the pair and reciprocal kernels.
+//
+//  k_epilogue: site force = pair part + reciprocal part (sum over row groups) + wall
+//              long-range tail (src/forcem.f90:509-543) + slab correction
+//              (src/forcem.f90:923-928); per-block partials of ulr, the wall z force it
+//              adds, and the total force (fws(0), src/forcem.f90:990-992).
+//  k_com:      rigid-body COM force and torque per molecule (src/forcem.f90:1003-1024).
+//  k_colsum:   per-lead partial columns -> per-block, per-lead-type sums.
+//  k_final:    energies per type pair, reciprocal energies, uconf, pressure from the wall
+//              forces, virial (src/forcem.f90:1032-1081).
+#include "mdf_internal.h"
+
+namespace mdf {
+
+__global__ __launch_bounds__(256) void k_epilogue(DevSys S, const double *__restrict__ fpart, int n_groups,
+                                                  const WallLR *__restrict__ wl, int ew3dc, double crf0,
+                                                  const double *__restrict__ dipz, double *__restrict__ epart) {
+    __shared__ double red[EP_N][256];
+    double acc[EP_N] = {0, 0, 0, 0, 0, 0};
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = S.n_move;
+    if (i < n) {
+        double fx = S.fx[i], fy = S.fy[i], fz = S.fz[i];
+        // wall long-range tail correction, sites with |q| >= 0.7
+        if (wl->on) {
+            const int it = S.itype[i];
+            if (fabs(S.q[i]) >= 0.7) {
+                for (int jj = -1; jj <= 1; jj += 2) {
+                    double aux = 0.0, sgn = (double)jj;
+                    double dw = S.lwh - sgn * S.z[i];
+                    if (dw < wl->rcut) {
+                        aux = aux + sgn * wl->flr_w[it];
+                        acc[0] += wl->ulr_w[it];
+                    } else {
+                        aux = aux + sgn * wl->flr_l[it];
+                        acc[0] += wl->ulr_l[it];
+                        double dwa = 1.0 / (dw * dw);
+                        double dwa2 = dwa * dwa;
+                        aux = aux + sgn * (wl->flr_lja[it] * (dwa2 * dwa2 * dwa) - wl->flr_ljc[it] * dwa2);
+                        double dw3 = dw * dw * dw;
+                        acc[0] += (wl->flr_lja[it] / (9.0 * (dw3 * dw3 * dw3)) - wl->flr_ljc[it] / (3.0 * dw3));
+                        double t = wl->flr_bur[it];
+                        double e2 = wl->flr_bua[it] * exp(-dw / t);
+                        aux = aux + sgn * ((t + dw) * e2 - wl->flr_buc[it] * dwa2);
+                        acc[0] += (t * (2.0 * t + dw) * e2 - wl->flr_buc[it] / (3.0 * dw3));
+                    }
+                    fz -= aux;
+                    if (it <= S.n_ltype) acc[jj < 0 ? 1 : 2] += aux;
+                }
+            }
+        }
+        for (int g = 0; g < n_groups; g++) {
+            fx += fpart[((size_t)g * 3 + 0) * n + i];
+            fy += fpart[((size_t)g * 3 + 1) * n + i];
+            fz += fpart[((size_t)g * 3 + 2) * n + i];
+        }
+        if (ew3dc) {  // crf = 2 crf0 facq dipz; fz += crf q  (src/forcem.f90:927-928)
+            const double crf = 2.0 * crf0 * S.facq * *dipz;
+            fz += crf * S.q[i];
+        }
+        S.fx[i] = fx; S.fy[i] = fy; S.fz[i] = fz;
+        acc[3] = fx; acc[4] = fy; acc[5] = fz;
+    }
+#pragma unroll
+    for (int c = 0; c < EP_N; c++) red[c][threadIdx.x] = acc[c];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+#pragma unroll
+            for (int c = 0; c < EP_N; c++) red[c][threadIdx.x] += red[c][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x < EP_N) epart[blockIdx.x * EP_N + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void k_com(DevSys S, double *__restrict__ fcom, double *__restrict__ tq) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nr = S.n_rig;
+    if (m >= nr) return;
+    const int j = 3 * m;
+    double tx = 0, ty = 0, tz = 0, fx = 0, fy = 0, fz = 0;
+    for (int a = 0; a < 3; a++) {
+        double rx = S.x[j + a] - S.cx[m], ry = S.y[j + a] - S.cy[m], rz = S.z[j + a] - S.cz[m];
+        double gx = S.fx[j + a], gy = S.fy[j + a], gz = S.fz[j + a];
+        tx += ry * gz - rz * gy;
+        ty += rz * gx - rx * gz;
+        tz += rx * gy - ry * gx;
+        fx += gx; fy += gy; fz += gz;
+    }
+    fcom[m] = fx; fcom[nr + m] = fy; fcom[2 * nr + m] = fz;
+    tq[m] = tx; tq[nr + m] = ty; tq[2 * nr + m] = tz;
+}
+
+// column sums of the per-lead partials, split by the lead's type slot
+__global__ __launch_bounds__(256) void k_colsum(DevSys S, const int *__restrict__ lead_slot, int per_block,
+                                                double *__restrict__ colpart) {
+    __shared__ double red[MAX_LEAD_TYPES][256];
+    const int col = blockIdx.x, blk = blockIdx.y;
+    const size_t L = (size_t)S.n_lead;
+    double acc[MAX_LEAD_TYPES] = {0, 0, 0, 0};
+    const int b = blk * per_block, e = min(S.n_lead, b + per_block);
+    for (int l = b + threadIdx.x; l < e; l += blockDim.x) {
+        double v = S.part[col * L + l];
+        int s = lead_slot[l];
+#pragma unroll
+        for (int t = 0; t < MAX_LEAD_TYPES; t++) acc[t] += (s == t) ? v : 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < MAX_LEAD_TYPES; t++) red[t][threadIdx.x] = acc[t];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+#pragma unroll
+            for (int t = 0; t < MAX_LEAD_TYPES; t++) red[t][threadIdx.x] += red[t][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x < MAX_LEAD_TYPES)
+        colpart[((size_t)col * gridDim.y + blk) * MAX_LEAD_TYPES + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(256) void k_final(DevSys S, FinalArgs A, const double *__restrict__ colpart,
+                                               const double *__restrict__ kpart, const double *__restrict__ eppart,
+                                               const double *__restrict__ dipz, double *__restrict__ e_vdw,
+                                               double *__restrict__ e_coul, Scalars *__restrict__ out) {
+    __shared__ double colsum[NCOL][MAX_LEAD_TYPES];
+    __shared__ double ksum[8];
+    __shared__ double epsum[EP_N];
+    const int t = threadIdx.x;
+    // columns x slots
+    for (int idx = t; idx < NCOL * MAX_LEAD_TYPES; idx += blockDim.x) {
+        int col = idx / MAX_LEAD_TYPES, s = idx % MAX_LEAD_TYPES;
+        double acc = 0.0;
+        for (int b = 0; b < A.n_colblk; b++) acc += colpart[((size_t)col * A.n_colblk + b) * MAX_LEAD_TYPES + s];
+        colsum[col][s] = acc;
+    }
+    if (t < 8) {
+        double acc = 0.0;
+        for (int b = 0; b < A.n_kblk; b++) acc += kpart[b * 8 + t];
+        ksum[t] = acc;
+    }
+    if (t < EP_N) {
+        double acc = 0.0;
+        for (int b = 0; b < A.n_epblk; b++) acc += eppart[b * EP_N + t];
+        epsum[t] = acc;
+    }
+    __syncthreads();
+    if (t != 0) return;
+    const int nt1 = S.n_types + 1;
+    for (int k = 0; k <= S.nijt; k++) { e_vdw[k] = 0.0; e_coul[k] = 0.0; }
+    for (int s = 0; s < A.n_slots; s++) {
+        int ti = A.slot_type[s];
+        for (int c = 0; c < NCLS; c++) {
+            int tc = A.cls_type[c];
+            if (tc <= 0) continue;
+            int ijt = S.ijtype[ti * nt1 + tc];
+            e_vdw[ijt] += colsum[COL_EV + c][s];
+            e_coul[ijt] += colsum[COL_EC + c][s];
+        }
+    }
+    Scalars o;
+    o.e_recip = 0.5 * ksum[0];
+    o.e_recip_ll = 0.5 * ksum[1];
+    o.e_recip_ww = A.e_recip_ww;
+    o.e_recip_wl = o.e_recip - o.e_recip_ww - o.e_recip_ll;
+    o.e_self = A.e_self;
+    double dz = *dipz;
+    o.e_slab = A.ew3dc ? -A.crf0 * S.facq * (dz * dz) : 0.0;
+    o.e_wall_lr = epsum[0];
+    double sum = 0.0;
+    for (int k = 1; k <= S.nijt; k++) sum += e_vdw[k];
+    for (int k = 1; k <= S.nijt; k++) sum += e_coul[k];
+    sum += o.e_recip;
+    double uconf = sum + o.e_self + o.e_slab;
+    o.e_conf = uconf - o.e_self;
+    o.e_avex = sum + o.e_wall_lr + o.e_slab;
+    // wall forces fws(a, sgn): pair part (columns) + tail correction (z only)
+    for (int a = 0; a < 3; a++) {
+        double m = 0.0, p = 0.0;
+        for (int s = 0; s < A.n_slots; s++) { m += colsum[COL_FWS + a][s]; p += colsum[COL_FWS + 3 + a][s]; }
+        o.f_wall[a * 3 + 0] = m;
+        o.f_wall[a * 3 + 2] = p;
+        o.f_wall[a * 3 + 1] = epsum[3 + a];
+    }
+    o.f_wall[6] += epsum[1];
+    o.f_wall[8] += epsum[2];
+    o.press = (o.f_wall[8] - o.f_wall[6]) / (2.0 * A.lx * A.ly) * 1000.0 / (A.na_scale) * A.bar;
+    // virial: real-space columns (symmetric) + reciprocal
+    double vr[9];
+    for (int c = 0; c < 9; c++) {
+        double acc = 0.0;
+        for (int s = 0; s < A.n_slots; s++) acc += colsum[COL_VIR + c][s];
+        vr[c] = acc;
+    }
+    vr[0] += ksum[2]; vr[4] += ksum[3]; vr[8] += ksum[4];
+    vr[1] += ksum[5]; vr[3] += ksum[5];
+    vr[2] += ksum[6]; vr[6] += ksum[6];
+    vr[5] += ksum[7]; vr[7] += ksum[7];
+    for (int c = 0; c < 9; c++) o.virial[c] = vr[c];
+    o.nonfinite = !(isfinite(o.e_conf) && isfinite(o.press));
+    o.pad = 0;
+    *out = o;
+}
+
+}  // namespace mdf

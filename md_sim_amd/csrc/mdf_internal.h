@@ -1,0 +1,159 @@
+// Internal definitions shared by the force engine's HIP translation units.
+// Device data layout (all in HBM, allocated once per handle):
+//   positions x,y,z [n_total] SoA FP64 (frozen part uploaded once), charges, site types;
+//   leads [n_lead] = moving sites that carry a molecule (nsi > 0), ascending site index;
+//   neighbour lists in ELL form, one row of `cap` int32 per lead (full: both directions
+//   for moving pairs, lead -> frozen sites for wall pairs), counts per lead;
+//   per-lead partial sums in column-major [NCOL][n_lead] for deterministic reduction;
+//   k-set as rows (kl, km, kn0..kn1) plus flat per-k arrays (Ak, S_wall, S, T = Ak conj S).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mdf {
+
+constexpr int NCLS = 6;          // energy classes of a partner type (moving lead types + 2 crystal)
+constexpr int MAX_TYPES = 14;
+constexpr int MAX_LEAD_TYPES = 4;
+// per-lead partial columns: vdw energy per class, coulomb energy per class, virial(9),
+// wall force x/y/z for sgn=-1 and sgn=+1 (6)
+constexpr int COL_EV = 0;
+constexpr int COL_EC = NCLS;
+constexpr int COL_VIR = 2 * NCLS;
+constexpr int COL_FWS = 2 * NCLS + 9;
+constexpr int NCOL = 2 * NCLS + 9 + 6;
+constexpr int KC = 16;           // k-vectors per structure-factor chunk held in registers
+constexpr int SF_SITES = 4;      // sites per thread in the structure-factor kernel
+constexpr int SF_BLOCK = 256;
+constexpr int BP_ROWGROUPS_MAX = 64;
+
+// k-set rows (contiguous kn) and structure-factor chunks (<= KC consecutive kn of a row)
+struct KChunk { int kl, km, kn0, len, koff; };
+struct KRow { int kl, km, kn0, kn1, koff; };
+
+// wall long-range tail tables per site type (src/forcem.f90:157-231)
+struct WallLR {
+    double flr_l[MAX_TYPES + 1], flr_w[MAX_TYPES + 1], flr_lja[MAX_TYPES + 1], flr_ljc[MAX_TYPES + 1];
+    double flr_bua[MAX_TYPES + 1], flr_buc[MAX_TYPES + 1], flr_bur[MAX_TYPES + 1];
+    double ulr_l[MAX_TYPES + 1], ulr_w[MAX_TYPES + 1];
+    double rcut;
+    int on;
+};
+
+constexpr int EP_N = 6;  // epilogue partials: ulr, fwsz(-1), fwsz(+1), sum fx, sum fy, sum fz
+constexpr int RE_N = 8;  // k partials: A|S|^2, A|S_moving|^2, virial xx yy zz xy xz yz
+
+// scalar outputs of one step, laid out for one device->host copy
+struct Scalars {
+    double e_recip, e_recip_ll, e_recip_wl, e_recip_ww, e_self, e_slab, e_wall_lr, e_conf, e_avex, press;
+    double f_wall[9];
+    double virial[9];
+    int nonfinite;
+    int pad;
+};
+
+struct FinalArgs {
+    int n_colblk;                 // blocks per column in colpart
+    int n_slots;                  // lead type slots in use
+    int slot_type[MAX_LEAD_TYPES];
+    int cls_type[NCLS];           // type of each energy class (0: unused)
+    int n_kblk;                   // blocks of k partials
+    int n_epblk;                  // blocks of epilogue partials
+    double e_self, e_recip_ww;
+    double crf0;                  // -2 pi / (lx ly lz) (src/forcem.f90:925)
+    int ew3dc;
+    double lx, ly, na_scale, bar; // pressure formula pieces (src/forcem.f90:1081)
+};
+
+// flags written by device kernels, read back by the host after a step
+struct DevFlags {
+    unsigned long long max_disp2_bits;   // max squared displacement since the last build
+    unsigned long long zmin_key, zmax_key; // ordered keys of lead z range
+    int rebuild;                         // 1: the list is rebuilt this step
+    int overflow_mm, overflow_mw;        // max count seen when a list row overflowed
+    int ncz;                             // cells in z for this build
+    double z0, csz;                      // z origin and cell size for this build
+    int n_pairs_mm, n_pairs_mw;          // listed pairs (i<j count and wall count)
+    int pad;
+};
+
+struct Grid {
+    int ncx, ncy;        // periodic x,y cells
+    int ncz_max;         // capacity of z cells
+    double csx, csy;     // cell sizes
+    int zper;            // z periodic in the list
+    int ncz_fixed;       // with periodic z: fixed cell count
+    double csz_fixed;
+};
+
+struct DevSys {
+    // sizes
+    int n_total, n_move, n_rig, n_rigid_sites, n_liquid_sites, n_lead, n_wall, n_types, n_ltype, nijt;
+    // topology
+    const int *itype;      // [n_total]
+    const double *q;       // [n_total]
+    const int *nsi;        // [n_types+1] indexed by 1-based type
+    const int *ijtype;     // [(n_types+1)^2] 1-based types, 1-based pair numbers
+    const int *vkind;      // [nijt+1] 1-based pair number
+    const double *eps4, *sig6, *aa, *rhoi, *cc;  // [nijt+1]
+    const int *cls;        // [n_types+1] energy class of a type
+    const int *lead;       // [n_lead] site index
+    const int *con_ptr;    // [n_move+1] CSR of constraint partners (surface sites)
+    const int *con_idx;
+    // state
+    double *x, *y, *z;     // [n_total]
+    const double *cx, *cy, *cz;  // [n_rig]
+    double *ocx, *ocy, *ocz;     // [n_rig] COM snapshot at the last build
+    double *ox, *oy, *oz;        // [n_move] site snapshot (non-rigid sites)
+    // box and cut-offs
+    double lx, ly, lz, lxi, lyi, lzi, lwh;
+    int zper_list, zper_force;
+    double rcutsq, rskinsq, skin2;
+    double alfa, spii, facq;
+    double a1, a2, a3, a4, a5, p;
+    int erfc_mode;
+    // lists
+    int *nbr_mm, *cnt_mm; int cap_mm;
+    int *nbr_mw, *cnt_mw; int cap_mw;
+    // cells
+    Grid g;
+    int *cell_cnt, *cell_start, *cell_items;   // moving leads
+    int *cell_cursor;
+    int n_cells_cap;
+    const int *wcell_start, *wcell_items;      // frozen sites (built once)
+    int wncz; double wz0, wcsz;
+    // outputs
+    double *fx, *fy, *fz;        // [n_move] site forces
+    double *part;                // [NCOL][n_lead]
+    DevFlags *flags;
+};
+
+// ---- helpers ---------------------------------------------------------------------
+// Fortran anint: round half away from zero
+__device__ __forceinline__ double anint_d(double v) { return round(v); }
+
+// squared minimum-image distance of the reference's list predicate, evaluated with the
+// reference's operation order and no FMA contraction (src/nnlist.f90:124-130)
+__device__ __forceinline__ double list_d2(const DevSys &S, int i, int j) {
+    double dx = __dsub_rn(S.x[i], S.x[j]);
+    dx = __dsub_rn(dx, __dmul_rn(S.lx, anint_d(__dmul_rn(dx, S.lxi))));
+    double dy = __dsub_rn(S.y[i], S.y[j]);
+    dy = __dsub_rn(dy, __dmul_rn(S.ly, anint_d(__dmul_rn(dy, S.lyi))));
+    double dz = __dsub_rn(S.z[i], S.z[j]);
+    if (S.zper_list) dz = __dsub_rn(dz, __dmul_rn(S.lz, anint_d(__dmul_rn(dz, S.lzi))));
+    return __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
+}
+
+// order-preserving key of a double for atomic min/max
+__device__ __forceinline__ unsigned long long dkey(double v) {
+    unsigned long long b = __double_as_longlong(v);
+    return (b & 0x8000000000000000ull) ? ~b : (b | 0x8000000000000000ull);
+}
+__host__ __device__ inline double dunkey(unsigned long long k) {
+    unsigned long long b = (k & 0x8000000000000000ull) ? (k & 0x7fffffffffffffffull) : ~k;
+    double v;
+    __builtin_memcpy(&v, &b, 8);
+    return v;
+}
+
+}  // namespace mdf

@@ -1,0 +1,302 @@
+// Verlet pair lists for the force engine: skin test, cell binning, list build.
+//
+// Reference: nnlist_init / nnlist_make (src/nnlist.f90:14-164) enumerate every lead pair
+// i<j once and keep those with dx^2+dy^2+dz^2 < rskinsq (x,y minimum image, z open) at
+// each rebuild; the rebuild fires when 4*max|displacement|^2 > skin^2 for COMs and
+// non-rigid sites (src/forcem.f90:245-250).  Here the candidates come from a cell grid
+// whose cells are at least sqrt(rskinsq) wide, and the same predicate - same operation
+// order, no FMA contraction - decides membership, so the pair set is bit-identical.
+// Lists are ELL rows per lead: moving partners in both directions (full list) and frozen
+// partners.  All kernels after k_decide return at once when no rebuild is due, so the
+// step's launch sequence is fixed.
+#include "mdf_internal.h"
+
+namespace mdf {
+
+__global__ void k_reset(DevSys S, int force_rebuild) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        DevFlags *f = S.flags;
+        f->max_disp2_bits = 0ull;
+        f->zmin_key = ~0ull;
+        f->zmax_key = 0ull;
+        f->rebuild = force_rebuild;
+        f->overflow_mm = 0;
+        f->overflow_mw = 0;
+        f->n_pairs_mm = 0;
+        f->n_pairs_mw = 0;
+    }
+}
+
+// max squared displacement since the last build (src/forcem.f90:245-249) and lead z range;
+// per-block partial of the moving dipole sum(q z) (src/forcem.f90:924) into dip_part
+__global__ __launch_bounds__(256) void k_skin(DevSys S, double *dip_part) {
+    __shared__ double red[256];
+    double dmax = 0.0, dip = 0.0;
+    double zlo = 1e300, zhi = -1e300;
+    const int n_nonrig = S.n_move - S.n_rigid_sites;
+    const int total = S.n_rig + n_nonrig;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < max(total, max(S.n_move, S.n_lead));
+         t += gridDim.x * blockDim.x) {
+        if (t < S.n_rig) {
+            double a = __dsub_rn(S.ocx[t], S.cx[t]), b = __dsub_rn(S.ocy[t], S.cy[t]), c = __dsub_rn(S.ocz[t], S.cz[t]);
+            double d = __dadd_rn(__dadd_rn(__dmul_rn(a, a), __dmul_rn(b, b)), __dmul_rn(c, c));
+            dmax = fmax(dmax, d);
+        } else if (t < total) {
+            int i = S.n_rigid_sites + (t - S.n_rig);
+            double a = __dsub_rn(S.ox[i], S.x[i]), b = __dsub_rn(S.oy[i], S.y[i]), c = __dsub_rn(S.oz[i], S.z[i]);
+            double d = __dadd_rn(__dadd_rn(__dmul_rn(a, a), __dmul_rn(b, b)), __dmul_rn(c, c));
+            dmax = fmax(dmax, d);
+        }
+        if (t < S.n_move) dip += S.q[t] * S.z[t];
+        if (t < S.n_lead) {
+            double zz = S.z[S.lead[t]];
+            zlo = fmin(zlo, zz);
+            zhi = fmax(zhi, zz);
+        }
+    }
+    // block dipole partial (fixed tree order)
+    red[threadIdx.x] = dip;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) dip_part[blockIdx.x] = red[0];
+    // max and min/max are order independent
+    if (dmax > 0.0) atomicMax(&S.flags->max_disp2_bits, (unsigned long long)__double_as_longlong(dmax));
+    if (zlo <= zhi) {
+        atomicMin(&S.flags->zmin_key, dkey(zlo));
+        atomicMax(&S.flags->zmax_key, dkey(zhi));
+    }
+}
+
+// rebuild decision (src/forcem.f90:250) and z grid of this build; dipole total
+__global__ void k_decide(DevSys S, const double *dip_part, int n_dip, double *dipz_out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    DevFlags *f = S.flags;
+    double aux = __longlong_as_double((long long)f->max_disp2_bits);
+    if (__dmul_rn(4.0, aux) > S.skin2) f->rebuild = 1;
+    double dip = 0.0;
+    for (int b = 0; b < n_dip; b++) dip += dip_part[b];
+    *dipz_out = dip;
+    if (S.g.zper) {
+        f->ncz = S.g.ncz_fixed;
+        f->z0 = 0.0;
+        f->csz = S.g.csz_fixed;
+    } else {
+        double zlo = dunkey(f->zmin_key), zhi = dunkey(f->zmax_key);
+        if (!(zlo <= zhi)) { zlo = 0.0; zhi = 0.0; }
+        double rs = sqrt(S.rskinsq) * (1.0 + 1e-9);
+        double range = zhi - zlo;
+        int ncz = (int)floor(range / rs);
+        if (ncz < 1) ncz = 1;
+        if (ncz > S.g.ncz_max) ncz = S.g.ncz_max;
+        f->ncz = ncz;
+        f->z0 = zlo;
+        f->csz = range > 0.0 ? range / ncz : 1.0;
+    }
+}
+
+__device__ __forceinline__ int wrap_cell(double v, double L, double Li, int n) {
+    double w = v - L * floor(v * Li);
+    int c = (int)(w * (double)n * Li);
+    return c < 0 ? 0 : (c >= n ? n - 1 : c);
+}
+
+__device__ __forceinline__ int zcell_lin(double z, double z0, double cs, int n) {
+    int c = (int)floor((z - z0) / cs);
+    return c < 0 ? 0 : (c >= n ? n - 1 : c);
+}
+
+__device__ __forceinline__ int lead_cell(const DevSys &S, int i, int ncz, double z0, double csz) {
+    int cx = wrap_cell(S.x[i], S.lx, S.lxi, S.g.ncx);
+    int cy = wrap_cell(S.y[i], S.ly, S.lyi, S.g.ncy);
+    int cz = S.g.zper ? wrap_cell(S.z[i], S.lz, S.lzi, ncz) : zcell_lin(S.z[i], z0, csz, ncz);
+    return (cz * S.g.ncy + cy) * S.g.ncx + cx;
+}
+
+__global__ void k_cell_clear(DevSys S) {
+    if (!S.flags->rebuild) return;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < S.n_cells_cap; c += gridDim.x * blockDim.x)
+        S.cell_cnt[c] = 0;
+}
+
+__global__ void k_cell_count(DevSys S, int *lead_cell_out) {
+    const DevFlags *f = S.flags;
+    if (!f->rebuild) return;
+    int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= S.n_lead) return;
+    int c = lead_cell(S, S.lead[l], f->ncz, f->z0, f->csz);
+    lead_cell_out[l] = c;
+    atomicAdd(&S.cell_cnt[c], 1);
+}
+
+// exclusive scan of cell counts, one block (cells per build are at most n_cells_cap)
+__global__ __launch_bounds__(1024) void k_cell_scan(int *cnt, int *start, int *cursor, const DevFlags *f,
+                                                    int ncells_xy, int use_flag_ncz, int ncz_given) {
+    if (use_flag_ncz && !f->rebuild) return;
+    __shared__ int part[1024];
+    int ncz = use_flag_ncz ? f->ncz : ncz_given;
+    int n = ncells_xy * ncz;
+    int per = (n + blockDim.x - 1) / blockDim.x;
+    int b = threadIdx.x * per, e = min(n, b + per);
+    int s = 0;
+    for (int c = b; c < e; c++) s += cnt[c];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int t = 0; t < (int)blockDim.x; t++) { int v = part[t]; part[t] = acc; acc += v; }
+    }
+    __syncthreads();
+    int acc = part[threadIdx.x];
+    for (int c = b; c < e; c++) { start[c] = acc; cursor[c] = acc; acc += cnt[c]; }
+    if (threadIdx.x == blockDim.x - 1) start[n] = acc;
+}
+
+__global__ void k_cell_fill(const int *item_cell, int n_items, int *cursor, int *items, const DevFlags *f,
+                            int use_flag) {
+    if (use_flag && !f->rebuild) return;
+    int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n_items) return;
+    int pos = atomicAdd(&cursor[item_cell[l]], 1);
+    items[pos] = l;
+}
+
+// ascending order inside every cell (rank sort, one wave per cell), so list rows are
+// deterministic; items_out gets the sorted copy
+__global__ __launch_bounds__(64) void k_cell_sort(const int *start, const int *items_in, int *items_out,
+                                                  const DevFlags *f, int use_flag, int ncells_xy, int ncz_given) {
+    if (use_flag && !f->rebuild) return;
+    int ncz = use_flag ? f->ncz : ncz_given;
+    for (int c = blockIdx.x; c < ncells_xy * ncz; c += gridDim.x) {
+        int b = start[c], n = start[c + 1] - b;
+        for (int a = threadIdx.x; a < n; a += 64) {
+            int v = items_in[b + a];
+            int rank = 0;
+            for (int k = 0; k < n; k++) rank += (items_in[b + k] < v);
+            items_out[b + rank] = v;
+        }
+    }
+}
+
+__device__ __forceinline__ bool constrained(const DevSys &S, int i, int j) {
+    if (i >= S.n_move) return false;
+    for (int k = S.con_ptr[i]; k < S.con_ptr[i + 1]; k++)
+        if (S.con_idx[k] == j) return true;
+    return false;
+}
+
+// list rows: thread per lead scans its neighbour cells in a fixed order
+__global__ __launch_bounds__(128) void k_build(DevSys S) {
+    const DevFlags *f = S.flags;
+    if (!f->rebuild) return;
+    int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= S.n_lead) return;
+    const int i = S.lead[l];
+    const int ncx = S.g.ncx, ncy = S.g.ncy;
+    const int ncz = f->ncz;
+    int cx = wrap_cell(S.x[i], S.lx, S.lxi, ncx);
+    int cy = wrap_cell(S.y[i], S.ly, S.lyi, ncy);
+    int cz = S.g.zper ? wrap_cell(S.z[i], S.lz, S.lzi, ncz) : zcell_lin(S.z[i], f->z0, f->csz, ncz);
+    // neighbour offsets, deduplicated for grids of 1 or 2 cells
+    int nxo = ncx >= 3 ? 3 : ncx, nyo = ncy >= 3 ? 3 : ncy;
+    int nzo = S.g.zper ? (ncz >= 3 ? 3 : ncz) : 3;
+    int cnt = 0, lower = 0;
+    int *row = S.nbr_mm + (size_t)l * S.cap_mm;
+    const bool i_surf = i >= S.n_liquid_sites;
+    for (int oz = 0; oz < nzo; oz++) {
+        int z2;
+        if (S.g.zper) { z2 = (nzo == 3) ? (cz + oz - 1 + ncz) % ncz : oz; }
+        else { z2 = cz + oz - 1; if (z2 < 0 || z2 >= ncz) continue; }
+        for (int oy = 0; oy < nyo; oy++) {
+            int y2 = (nyo == 3) ? (cy + oy - 1 + ncy) % ncy : oy;
+            for (int ox = 0; ox < nxo; ox++) {
+                int x2 = (nxo == 3) ? (cx + ox - 1 + ncx) % ncx : ox;
+                int c = (z2 * ncy + y2) * ncx + x2;
+                for (int p = S.cell_start[c]; p < S.cell_start[c + 1]; p++) {
+                    int m = S.cell_items[p];
+                    if (m == l) continue;
+                    int j = S.lead[m];
+                    if (i_surf && j >= S.n_liquid_sites && constrained(S, i, j)) continue;
+                    int a = min(i, j), b = max(i, j);
+                    if (list_d2(S, a, b) < S.rskinsq) {
+                        if (cnt < S.cap_mm) row[cnt] = j;
+                        cnt++;
+                        lower += (j > i);
+                    }
+                }
+            }
+        }
+    }
+    S.cnt_mm[l] = cnt;
+    if (cnt > S.cap_mm) atomicMax(&S.flags->overflow_mm, cnt);
+    if (lower) atomicAdd(&S.flags->n_pairs_mm, lower);
+    // frozen partners: same x,y grid, frozen z grid built once
+    int cntw = 0;
+    if (S.n_wall > 0) {
+        int *wrow = S.nbr_mw + (size_t)l * S.cap_mw;
+        int wcz;
+        if (S.g.zper) {
+            wcz = wrap_cell(S.z[i], S.lz, S.lzi, S.wncz);
+        } else {
+            double czd = floor((S.z[i] - S.wz0) / S.wcsz);
+            czd = fmin(fmax(czd, -2.0), (double)S.wncz + 1.0);  // far from the crystal: no cell in reach
+            wcz = (int)czd;
+        }
+        for (int oz = 0; oz < (S.g.zper ? min(3, S.wncz) : 3); oz++) {
+            int z2;
+            if (S.g.zper) z2 = (S.wncz >= 3) ? (wcz + oz - 1 + S.wncz) % S.wncz : oz;
+            else { z2 = wcz + oz - 1; if (z2 < 0 || z2 >= S.wncz) continue; }
+            for (int oy = 0; oy < nyo; oy++) {
+                int y2 = (nyo == 3) ? (cy + oy - 1 + ncy) % ncy : oy;
+                for (int ox = 0; ox < nxo; ox++) {
+                    int x2 = (nxo == 3) ? (cx + ox - 1 + ncx) % ncx : ox;
+                    int c = (z2 * ncy + y2) * ncx + x2;
+                    for (int p = S.wcell_start[c]; p < S.wcell_start[c + 1]; p++) {
+                        int j = S.n_move + S.wcell_items[p];
+                        if (i_surf && constrained(S, i, j)) continue;
+                        if (list_d2(S, i, j) < S.rskinsq) {
+                            if (cntw < S.cap_mw) wrow[cntw] = j;
+                            cntw++;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    S.cnt_mw[l] = cntw;
+    if (cntw > S.cap_mw) atomicMax(&S.flags->overflow_mw, cntw);
+    if (cntw) atomicAdd(&S.flags->n_pairs_mw, cntw);
+}
+
+// snapshot for the next skin test (src/nnlist.f90:99-119)
+__global__ void k_snapshot(DevSys S) {
+    if (!S.flags->rebuild) return;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < S.n_rig) { S.ocx[t] = S.cx[t]; S.ocy[t] = S.cy[t]; S.ocz[t] = S.cz[t]; }
+    int i = S.n_rigid_sites + t;
+    if (i < S.n_move) { S.ox[i] = S.x[i]; S.oy[i] = S.y[i]; S.oz[i] = S.z[i]; }
+}
+
+// frozen-site cells, built once at creation
+__global__ void k_wall_cells(DevSys S, int *wcell_of, int ncz, double z0, double csz) {
+    int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= S.n_wall) return;
+    int i = S.n_move + w;
+    int cx = wrap_cell(S.x[i], S.lx, S.lxi, S.g.ncx);
+    int cy = wrap_cell(S.y[i], S.ly, S.lyi, S.g.ncy);
+    int cz = S.g.zper ? wrap_cell(S.z[i], S.lz, S.lzi, ncz) : zcell_lin(S.z[i], z0, csz, ncz);
+    wcell_of[w] = (cz * S.g.ncy + cy) * S.g.ncx + cx;
+}
+
+__global__ void k_count_cells(const int *item_cell, int n_items, int *cnt) {
+    int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l < n_items) atomicAdd(&cnt[item_cell[l]], 1);
+}
+
+__global__ void k_zero_int(int *p, int n) {
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) p[t] = 0;
+}
+
+}  // namespace mdf

@@ -1,0 +1,139 @@
+"""Host-side mirror of the reference's force module on top of the HIP library.
+
+The reference's force module (src/forcem.f90) exposes ``force_init`` (:54), ``force_zero``
+(:84) and ``force_get`` (:93) and talks through module variables; ``ForceEngine`` exposes
+the same three operations on explicit arrays and returns the reference's outputs (site
+forces before the COM reduction, COM forces, torques, energies per type pair, reciprocal
+energies, slab and wall terms, uconf, pressure from wall forces) plus a virial.  A status
+other than MDF_OK raises ``MDForceError`` carrying the library's message, where the
+reference would ``stop``.
+
+There is no CPU fallback: without the built library (``libmdforce.so``) or without a HIP
+device, construction fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmdforce.so")
+
+
+class MDForceError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__("%s (%d): %s" % (abi.STATUS_NAMES.get(code, "?"), code, msg))
+        self.code = code
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """The force engine's shared library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libmdforce.so is not built (run __graft_entry__.build() or "
+                              "python -m md_sim_amd.build)")
+        L = C.CDLL(LIB_PATH)
+        L.mdf_last_error.restype = C.c_char_p
+        for fn in ("mdf_create", "mdf_compute", "mdf_compute_device", "mdf_read_outputs", "mdf_get_pairs",
+                   "mdf_get_kvectors", "mdf_get_structure_factor", "mdf_stage_times", "mdf_device_results",
+                   "mdf_zero"):
+            getattr(L, fn).restype = C.c_int
+        L.mdf_destroy.restype = None
+        _lib = L
+    return _lib
+
+
+def _check(rc: int):
+    if rc != abi.MDF_OK:
+        raise MDForceError(rc, lib().mdf_last_error().decode())
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class ForceEngine:
+    """One system on one GPU.  ``ForceEngine(system)`` is force_init; ``force_get`` is
+    force_get; ``force_zero`` is force_zero."""
+
+    def __init__(self, system, positions=None, **abi_kw):
+        self.system = system
+        self.sa = abi.build_abi(system, **abi_kw)
+        if positions is None:
+            positions = abi.positions(system)
+        x, y, z, cx, cy, cz = [np.ascontiguousarray(a, np.float64) for a in positions]
+        self._h = C.c_void_p()
+        _check(lib().mdf_create(C.byref(self.sa.top), C.byref(self.sa.par), _dp(x), _dp(y), _dp(z),
+                                _dp(cx), _dp(cy), _dp(cz), C.byref(self._h)))
+        self.n_move, self.n_rig, self.n_pairs = system.nmove, system.nrig, system.nijt
+
+    # -- reference entry points --------------------------------------------------------
+    def force_get(self, x, y, z, cx, cy, cz) -> abi.ForceResult:
+        keep = [np.ascontiguousarray(a, np.float64) for a in (x, y, z, cx, cy, cz)]
+        o, arrs = abi.make_outputs(self.n_move, self.n_rig, self.n_pairs)
+        _check(lib().mdf_compute(self._h, *[_dp(k) for k in keep], C.byref(o)))
+        return abi.collect(o, arrs, self.n_move, self.n_rig)
+
+    def force_zero(self):
+        _check(lib().mdf_zero(self._h))
+
+    # -- device-resident path ------------------------------------------------------------
+    def compute_device(self, d_x: int, d_y: int, d_z: int, d_cx: int, d_cy: int, d_cz: int):
+        """Enqueue a step on device pointers (ints, e.g. torch tensor.data_ptr())."""
+        vp = C.c_void_p
+        _check(lib().mdf_compute_device(self._h, vp(d_x), vp(d_y), vp(d_z), vp(d_cx), vp(d_cy), vp(d_cz), None))
+
+    def read_outputs(self) -> abi.ForceResult:
+        o, arrs = abi.make_outputs(self.n_move, self.n_rig, self.n_pairs)
+        _check(lib().mdf_read_outputs(self._h, C.byref(o)))
+        return abi.collect(o, arrs, self.n_move, self.n_rig)
+
+    # -- inspection ------------------------------------------------------------------------
+    def pairs(self):
+        n1, n2 = C.c_int32(), C.c_int32()
+        pi = lambda v: v.ctypes.data_as(C.POINTER(C.c_int32))
+        z = np.zeros(1, np.int32)
+        lib().mdf_get_pairs(self._h, pi(z), pi(z), 0, C.byref(n1), pi(z), pi(z), 0, C.byref(n2))
+        a, b = np.zeros(max(n1.value, 1), np.int32), np.zeros(max(n1.value, 1), np.int32)
+        c, d = np.zeros(max(n2.value, 1), np.int32), np.zeros(max(n2.value, 1), np.int32)
+        _check(lib().mdf_get_pairs(self._h, pi(a), pi(b), n1.value, C.byref(n1), pi(c), pi(d), n2.value,
+                                   C.byref(n2)))
+        return a[:n1.value], b[:n1.value], c[:n2.value], d[:n2.value]
+
+    def kvectors(self):
+        n = C.c_int32()
+        lib().mdf_get_kvectors(self._h, None, None, 0, C.byref(n))
+        kv = np.zeros(3 * n.value, np.int32)
+        ak = np.zeros(n.value)
+        _check(lib().mdf_get_kvectors(self._h, kv.ctypes.data_as(C.POINTER(C.c_int32)), _dp(ak), n.value,
+                                      C.byref(n)))
+        sf = np.zeros(2 * max(n.value, 1))
+        _check(lib().mdf_get_structure_factor(self._h, _dp(sf), n.value))
+        return kv.reshape(-1, 3), ak, sf[0:2 * n.value:2] + 1j * sf[1:2 * n.value:2]
+
+    def stage_times(self):
+        names = (C.c_char_p * 16)()
+        ms = (C.c_float * 16)()
+        n = C.c_int32()
+        _check(lib().mdf_stage_times(self._h, names, ms, 16, C.byref(n)))
+        return {names[k].decode(): float(ms[k]) for k in range(n.value)}
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().mdf_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

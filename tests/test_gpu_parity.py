@@ -1,0 +1,84 @@
+"""HIP force engine against the oracle and the reference's golden vectors (FP64)."""
+import sys
+
+import numpy as np
+import pytest
+
+from md_sim_amd import systems
+from helpers import golden, golden_positions, rel_err, rms_rel
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10  # relative force error, FP64 engine vs FP64 reference path
+
+
+def _engine(s, **kw):
+    from md_sim_amd.engine import ForceEngine
+    return ForceEngine(s, **kw)
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _compare(r, ref, tol=TOL, energies=True):
+    for a in range(3):
+        assert rel_err(r["f_site"][a], ref["f_site"][a]) < tol, ("f_site", a, rel_err(r["f_site"][a], ref["f_site"][a]))
+    assert rel_err(r["f_com"], ref["f_com"]) < tol
+    assert rel_err(r["torque"], ref["torque"]) < tol
+    if energies:
+        assert rel_err(r["e_vdw"], ref["e_vdw"]) < tol
+        assert rel_err(r["e_coul_real"], ref["e_coul_real"]) < tol
+        for k in ("e_recip", "e_recip_ll", "e_recip_wl", "e_self", "e_slab", "e_wall_lr", "e_conf", "press_wall"):
+            scale = max(abs(ref[k]), 1e-6 * abs(ref["e_conf"]), 1e-12)
+            assert abs(r[k] - ref[k]) <= tol * scale, (k, r[k], ref[k])
+        assert rel_err(r["f_wall"].ravel(), ref["f_wall"].ravel()) < tol
+
+
+@pytest.mark.parametrize("name", ["bulk216", "pore_small"])
+def test_engine_matches_reference_golden(oracle_lib, name):
+    _say("case", name)
+    g = golden(name)
+    s = systems.config(name)
+    pos = golden_positions(g)
+    eng = _engine(s, positions=pos)
+    r = eng.force_get(*pos)
+    # pair set bit-exact with the reference's lists
+    l1, l2, w1, w2 = eng.pairs()
+    assert np.array_equal(l1 + 1, g["list1"]) and np.array_equal(l2 + 1, g["list2"])
+    assert np.array_equal(w1 + 1, g["listw1"]) and np.array_equal(w2 + 1, g["listw2"])
+    for a, k in enumerate("xyz"):
+        assert rel_err(r["f_site"][a], g["f_site_" + k]) < TOL, (k, rel_err(r["f_site"][a], g["f_site_" + k]))
+    assert rel_err(r["torque"][0], g["tx"]) < TOL
+    assert rel_err(r["e_vdw"], g["elj"]) < TOL
+    assert rel_err(r["e_coul_real"], g["eqq"]) < TOL
+    for k, gk in [("e_recip", "uewa"), ("e_self", "uself"), ("e_slab", "urf"), ("e_wall_lr", "ulr"),
+                  ("e_conf", "uconf"), ("press_wall", "press"), ("e_recip_ll", "ewqLL"), ("e_recip_ww", "ewqWW")]:
+        ref = g[gk].item()
+        assert abs(r[k] - ref) <= TOL * max(abs(ref), 1e-6), (k, r[k], ref)
+    kv, ak, sf = eng.kvectors()
+    assert np.array_equal(kv.ravel(), g["k_vec"].astype(int))
+    assert rel_err(ak, g["k_ak"]) < 1e-14
+    assert rel_err(sf.real, g["k_sre"]) < TOL and rel_err(sf.imag, g["k_sim"]) < TOL
+    # and against the oracle on everything the oracle reports
+    o = oracle_lib.Oracle(s)
+    ro = o.compute(*pos)
+    _compare(r, ro)
+
+
+def test_engine_matches_oracle_slab2048(oracle_lib):
+    _say("case slab2048")
+    s = systems.config("slab2048")
+    eng = _engine(s)
+    r = eng.force_get(*__import__("md_sim_amd.abi", fromlist=["positions"]).positions(s))
+    o = oracle_lib.Oracle(s)
+    ro = o.compute_system()
+    l1, l2, w1, w2 = eng.pairs()
+    m1, m2, v1, v2 = o.pairs()
+    assert np.array_equal(l1, m1) and np.array_equal(l2, m2)
+    assert np.array_equal(w1, v1) and np.array_equal(w2, v2)
+    _compare(r, ro)
+    g = golden("slab2048")
+    for a, k in enumerate("xyz"):
+        assert rel_err(r["f_site"][a][g["sel"]], g["f_site_" + k]) < TOL
