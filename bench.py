@@ -1,0 +1,200 @@
+"""Benchmark: MD steps/s (forces + Ewald) at 16k H2O on one MI355X.
+
+A step is one force evaluation of the reference's force path (force_get,
+src/forcem.f90:93-129) on device-resident positions: pair-list skin test (and rebuild when
+due), real-space erfc + LJ/Buckingham pair kernel, reciprocal k-sum (structure factor and
+back-projection), slab and wall corrections, COM forces/torques and energies.  The
+positions follow a synthetic ballistic trajectory (every molecule translated rigidly by a
+fixed thermal-size displacement per step, seeded), so the list is rebuilt as often as the
+skin criterion demands.
+
+Workload (default ``slab16k``): 16384 rigid SPC/E waters on a frozen 6144-ion rock-salt
+slab, box 90.24 x 90.24 x 270.72 A, rc 10 A, skin 1 A, k-index 11/11/33 (8589 k-vectors),
+FP64 (``--precision fp32`` selects FP32 arithmetic with FP64 accumulation).
+
+Prints one JSON line.  ``roofline`` is for the dominant stage, the reciprocal sum: its
+algorithmic work 18 flop per (k-vector, moving site) + 8 flop per ((kl,km) row, site)
+over the stage's mean kernel time, measured with HIP events on the engine's stream over
+the timed steps.  ``cpu_baseline`` times the reference's own force path (built from its
+sources into oracle/_ref) on a few force evaluations of the same system on one core, or
+the CPU oracle when that build is absent.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "MD steps/s (forces+Ewald) at 16k H2O; surrogate-MTS speedup at fixed force err"
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
+PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "reciprocal_traffic_slab16k.json")
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="slab16k")
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--cpu-reps", type=int, default=3, help="force evaluations in the CPU baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--step-displacement", type=float, default=0.0075,
+                    help="rms per-axis displacement of a molecule per step (A)")
+    return ap.parse_args()
+
+
+def cpu_baseline(system, reps: int):
+    """Reference force path on the host (one core): mean seconds per force evaluation."""
+    from oracle import refrun
+    if refrun.ref_available():
+        d = refrun.run_reference(system, nrep=reps, timeout=1200)
+        n, sec = d["_timing"]
+        return dict(value=n / sec, unit="steps/s", cores=1, kind="reference",
+                    sample="%d force evaluations (no list rebuild) of %s by the reference's own "
+                           "force_real/force_fouri/force_intra/force_sum, built from its Fortran "
+                           "sources with amdflang -O2, one MPI rank" % (int(n), system.name))
+    from md_sim_amd import abi
+    from oracle import pyoracle
+    o = pyoracle.Oracle(system)
+    pos = abi.positions(system)
+    o.compute(*pos)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        o.compute(*pos)
+    sec = time.perf_counter() - t0
+    return dict(value=reps / sec, unit="steps/s", cores=1, kind="port",
+                sample="%d force evaluations (no list rebuild) of %s by the C restatement of the "
+                       "reference (oracle/oracle.c), one thread" % (reps, system.name))
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from md_sim_amd import abi, systems
+    from md_sim_amd.engine import ForceEngine
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda:0")
+    s = systems.config(args.workload)
+    prec = abi.FP32_ACC64 if args.precision == "fp32" else abi.FP64
+    x, y, z, cx, cy, cz = abi.positions(s)
+    nm, nr = s.nmove, s.nrig
+    eng = ForceEngine(s, precision=prec)
+
+    # synthetic ballistic trajectory: one displacement per molecule (or mobile ion) per step
+    rng = np.random.default_rng(12345)
+    mol_v = rng.normal(0.0, args.step_displacement, size=(nr, 3))
+    site_v = np.zeros((nm, 3))
+    site_v[:3 * nr] = np.repeat(mol_v, 3, axis=0)
+    site_v[3 * nr:] = rng.normal(0.0, args.step_displacement, size=(nm - 3 * nr, 3))
+    base = torch.tensor(np.stack([x[:nm], y[:nm], z[:nm]]), dtype=torch.float64, device=dev)
+    vel = torch.tensor(site_v.T.copy(), dtype=torch.float64, device=dev)
+    cbase = torch.tensor(np.stack([cx, cy, cz]), dtype=torch.float64, device=dev)
+    cvel = torch.tensor(mol_v.T.copy(), dtype=torch.float64, device=dev)
+    n_frames = args.warmup + args.steps
+    frames = torch.empty((n_frames, 3, nm), dtype=torch.float64, device=dev)
+    cframes = torch.empty((n_frames, 3, max(nr, 1)), dtype=torch.float64, device=dev)
+    for t in range(n_frames):
+        frames[t] = base + t * vel
+        cframes[t, :, :nr] = cbase + t * cvel
+    torch.cuda.synchronize()
+
+    def step(t):
+        f, c = frames[t], cframes[t]
+        eng.compute_device(f[0].data_ptr(), f[1].data_ptr(), f[2].data_ptr(),
+                           c[0].data_ptr(), c[1].data_ptr(), c[2].data_ptr())
+
+    for t in range(args.warmup):
+        step(t)
+    eng.read_outputs()
+    torch.cuda.synchronize()
+    eng.reset_stage_times()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for t in range(args.warmup, n_frames):
+        step(t)
+    last = eng.read_outputs()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    sec = t1 - t0
+    stages = eng.stage_times()
+
+    kv, ak, _ = eng.kvectors()
+    n_k = len(ak)
+    n_rows = len(set(map(tuple, kv[:, :2].tolist())))
+    flops = 18.0 * n_k * nm + 8.0 * n_rows * nm
+    t_recip_ms = stages["sf"] + stages["sf_reduce"] + stages["backproj"]
+    achieved = flops / (t_recip_ms * 1e-3) / 1e12
+    peak = PEAK_TFLOPS[args.precision]
+    traffic = None
+    if os.path.exists(PROFILE_TRAFFIC):
+        try:
+            traffic = json.load(open(PROFILE_TRAFFIC)).get(args.precision)
+        except Exception:
+            traffic = None
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline(s, args.cpu_reps)
+        except Exception as e:  # the GPU figure stands on its own
+            cpu = {"error": str(e)[:200]}
+
+    out = {
+        "metric": METRIC,
+        "value": args.steps / sec,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sec / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32 arithmetic, f64 accumulation",
+        "data": "synthetic (seeded lattice water on a rock-salt slab, ballistic trajectory)",
+        "config": {
+            "workload": "%s: %d SPC/E waters + %d frozen ions, box %.2f x %.2f x %.2f A, rc %.1f A, "
+                        "skin %.1f A, k-index %d (%d k-vectors), %s; one force evaluation per step"
+                        % (s.name, nr, s.nwall, s.box[0], s.box[1], s.box[2], s.rcut, s.skin, s.kmax, n_k,
+                           args.precision),
+            "global_batch": 1,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "kernel": "reciprocal sum (k_sf + k_sf_reduce + k_bp)",
+            "bound": "mfma",
+            "pipe": "FP64 vector ALU (MI355X FP64 vector peak = FP64 matrix peak)" if args.precision == "fp64"
+                    else "FP32 vector ALU",
+            "achieved": achieved,
+            "peak": peak,
+            "unit": "TFLOP/s",
+            "frac": achieved / peak,
+            "traffic": traffic,
+            "algorithmic_flop_per_launch": flops,
+            "mean_ms": t_recip_ms,
+        },
+        "cpu_baseline": cpu,
+        "stages_ms": stages,
+        "list_rebuilt_last": last["list_rebuilt"],
+        "e_conf_last": last["e_conf"],
+    }
+    if cpu and "value" in cpu:
+        out["speedup_vs_cpu"] = out["value"] / cpu["value"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -179,9 +179,11 @@ void mdf_destroy(mdf_system *sys);
 /* Message for the last failing call (per thread).                                    */
 const char *mdf_last_error(void);
 
-/* Kernel timing of the last compute: names and milliseconds per internal stage,
-   measured with HIP events on the handle's stream.  n in: capacity; out: count.       */
+/* Stage timing: names and mean milliseconds per step of each internal stage over the
+   steps since mdf_reset_stage_times (HIP events recorded on the handle's stream around
+   each stage).  cap: capacity of names/ms; n out: number of stages.                  */
 int mdf_stage_times(mdf_system *sys, const char **names, float *ms, int32_t cap, int32_t *n);
+int mdf_reset_stage_times(mdf_system *sys);
 
 #ifdef __cplusplus
 }

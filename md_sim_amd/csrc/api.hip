@@ -156,9 +156,16 @@ struct mdf_system {
     // launch geometry
     int skin_blocks = 1, n_sb = 1, n_cg = 1, cpb = 1, n_kblk = 1, n_rg = 1, rpg = 1, n_epblk = 1, n_colblk = 1,
         colper = 1, ncells_xy = 1, n_cells_cap = 1;
-    // timing
-    hipEvent_t ev[N_STAGES + 1] = {};
-    float stage_ms[N_STAGES] = {};
+    // timing: a ring of event sets, one per in-flight step; stage durations are accumulated
+    // when a set is reused or at the next synchronisation
+    static constexpr int NRING = 32;
+    hipEvent_t ring[NRING][N_STAGES + 1] = {};
+    bool ring_used[NRING] = {};
+    long long step_no = 0;
+    hipEvent_t *ev = nullptr;       // events of the step being launched
+    double stage_sum[N_STAGES] = {};
+    long long stage_steps = 0;
+    float stage_ms[N_STAGES] = {};  // last completed step
     bool timing = true;
     bool pending = false;
     int last_rebuilt = 0;
@@ -251,20 +258,21 @@ static void kset(mdf_system *s) {
     int kmmin = 0, knmin = 1;
     for (int kl = 0; kl <= klmax; kl++) {
         for (int km = kmmin; km <= kmmax; km++) {
-            int first = -1, last = -1;
+            bool found = false;
+            int first = 0, last = 0;
             for (int kn = knmin; kn <= knmax; kn++) {
                 float ksq = (float)(kl * kl + km * km) +
                             (float)((long long)kn * kn * kmax * kmax) / (float)((long long)knmax * knmax);
                 if (ksq < (float)(kmax * kmax) + 2.01f) {
                     double rl = pipix * (double)(float)kl, rm = pipiy * (double)(float)km, rn = pipiz * (double)(float)kn;
                     double rksq = rl * rl + rm * rm + rn * rn;
-                    if (first < 0) first = kn;
+                    if (!found) { first = kn; found = true; }
                     last = kn;
                     s->kvec.push_back(kl); s->kvec.push_back(km); s->kvec.push_back(kn);
                     s->ak.push_back(fac * exp(rksq * alf4i) / rksq);
                 }
             }
-            if (first >= 0) {
+            if (found) {
                 int koff = (int)s->ak.size() - (last - first + 1);
                 s->rows.push_back({kl, km, first, last, koff});
                 for (int kn0 = first; kn0 <= last; kn0 += KC)
@@ -396,8 +404,37 @@ static void launch_forces(mdf_system *s) {
     if (s->timing) (void)hipEventRecord(s->ev[7], st);
 }
 
+static void drain_slot(mdf_system *s, int slot) {
+    if (!s->ring_used[slot]) return;
+    (void)hipEventSynchronize(s->ring[slot][N_STAGES]);
+    for (int k = 0; k < N_STAGES; k++) {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, s->ring[slot][k], s->ring[slot][k + 1]);
+        s->stage_sum[k] += ms;
+        s->stage_ms[k] = ms;
+    }
+    s->stage_steps++;
+    s->ring_used[slot] = false;
+}
+
+static void drain_all(mdf_system *s) {
+    // oldest first, so stage_ms ends as the last step's
+    for (int d = 0; d < mdf_system::NRING; d++) {
+        long long n = s->step_no - mdf_system::NRING + d;
+        if (n < 0) continue;
+        drain_slot(s, (int)(n % mdf_system::NRING));
+    }
+}
+
 static int run_step(mdf_system *s, int force_rebuild) {
-    if (s->timing) (void)hipEventRecord(s->ev[0], s->stream);
+    if (s->timing) {
+        int slot = (int)(s->step_no % mdf_system::NRING);
+        drain_slot(s, slot);
+        s->ev = s->ring[slot];
+        s->ring_used[slot] = true;
+        s->step_no++;
+        (void)hipEventRecord(s->ev[0], s->stream);
+    }
     launch_lists(s, force_rebuild);
     launch_forces(s);
     HIPCHK(hipGetLastError());
@@ -423,13 +460,7 @@ static int finish_step(mdf_system *s, int *redo) {
         if (rc) return rc;
         *redo = 1;
     }
-    if (s->timing) {
-        for (int k = 0; k < N_STAGES; k++) {
-            float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, s->ev[k], s->ev[k + 1]);
-            s->stage_ms[k] = ms;
-        }
-    }
+    if (s->timing) drain_all(s);
     return MDF_OK;
 }
 
@@ -517,7 +548,8 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
     } while (0)
     {
         HTRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
-        for (int k = 0; k <= N_STAGES; k++) HTRY(hipEventCreate(&s->ev[k]));
+        for (int r = 0; r < mdf_system::NRING; r++)
+            for (int k = 0; k <= N_STAGES; k++) HTRY(hipEventCreate(&s->ring[r][k]));
         // leads, lead-type slots, energy classes
         std::vector<int> lead_types;
         for (int i = 0; i < t.n_move; i++) {
@@ -692,6 +724,25 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
         }
         // k-set, A_k, static frozen S(k), self energy
         kset(s);
+        {
+            // rows and chunks must tile the flat k list exactly
+            size_t nk = s->ak.size(), covered = 0, ccov = 0;
+            for (const KRow &r : s->rows) {
+                for (int kn = r.kn0; kn <= r.kn1; kn++) {
+                    size_t k = (size_t)r.koff + (kn - r.kn0);
+                    if (k >= nk || s->kvec[3 * k] != r.kl || s->kvec[3 * k + 1] != r.km || s->kvec[3 * k + 2] != kn) {
+                        rc = fail(MDF_ERR_KSET, "k-set row table inconsistent at row (%d,%d)", r.kl, r.km);
+                        goto fail_out;
+                    }
+                }
+                covered += (size_t)(r.kn1 - r.kn0 + 1);
+            }
+            for (const KChunk &c : s->chunks) ccov += (size_t)c.len;
+            if (covered != nk || ccov != nk) {
+                rc = fail(MDF_ERR_KSET, "k-set rows cover %zu, chunks %zu of %zu k-vectors", covered, ccov, nk);
+                goto fail_out;
+            }
+        }
         geometry(s);
         {
             int nk = (int)s->ak.size();
@@ -788,8 +839,9 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
 fail_out:
     free_all(s);
     if (s->stream) (void)hipStreamDestroy(s->stream);
-    for (int k = 0; k <= N_STAGES; k++)
-        if (s->ev[k]) (void)hipEventDestroy(s->ev[k]);
+    for (int r = 0; r < mdf_system::NRING; r++)
+        for (int k = 0; k <= N_STAGES; k++)
+            if (s->ring[r][k]) (void)hipEventDestroy(s->ring[r][k]);
     delete s;
     return rc;
 #undef TRY
@@ -917,11 +969,24 @@ int mdf_get_structure_factor(mdf_system *s, double *sf, int32_t cap) {
 
 int mdf_stage_times(mdf_system *s, const char **names, float *ms, int32_t cap, int32_t *n) {
     if (!s || !n) return fail(MDF_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(s->device));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    drain_all(s);
     *n = N_STAGES;
     for (int k = 0; k < N_STAGES && k < cap; k++) {
         if (names) names[k] = STAGE_NAMES[k];
-        if (ms) ms[k] = s->stage_ms[k];
+        if (ms) ms[k] = s->stage_steps > 0 ? (float)(s->stage_sum[k] / (double)s->stage_steps) : 0.f;
     }
+    return MDF_OK;
+}
+
+int mdf_reset_stage_times(mdf_system *s) {
+    if (!s) return fail(MDF_ERR_ARG, "null handle");
+    HIPCHK(hipSetDevice(s->device));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    drain_all(s);
+    for (int k = 0; k < N_STAGES; k++) s->stage_sum[k] = 0.0;
+    s->stage_steps = 0;
     return MDF_OK;
 }
 
@@ -930,8 +995,9 @@ void mdf_destroy(mdf_system *s) {
     (void)hipSetDevice(s->device);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     free_all(s);
-    for (int k = 0; k <= N_STAGES; k++)
-        if (s->ev[k]) (void)hipEventDestroy(s->ev[k]);
+    for (int r = 0; r < mdf_system::NRING; r++)
+        for (int k = 0; k <= N_STAGES; k++)
+            if (s->ring[r][k]) (void)hipEventDestroy(s->ring[r][k]);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
 }

@@ -45,7 +45,7 @@ def lib() -> C.CDLL:
         L.mdf_last_error.restype = C.c_char_p
         for fn in ("mdf_create", "mdf_compute", "mdf_compute_device", "mdf_read_outputs", "mdf_get_pairs",
                    "mdf_get_kvectors", "mdf_get_structure_factor", "mdf_stage_times", "mdf_device_results",
-                   "mdf_zero"):
+                   "mdf_zero", "mdf_reset_stage_times"):
             getattr(L, fn).restype = C.c_int
         L.mdf_destroy.restype = None
         _lib = L
@@ -126,6 +126,9 @@ class ForceEngine:
         n = C.c_int32()
         _check(lib().mdf_stage_times(self._h, names, ms, 16, C.byref(n)))
         return {names[k].decode(): float(ms[k]) for k in range(n.value)}
+
+    def reset_stage_times(self):
+        _check(lib().mdf_reset_stage_times(self._h))
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:

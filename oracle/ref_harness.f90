@@ -16,7 +16,8 @@ program ref_harness
     implicit none
     include 'mpif.h'
     integer :: ierr, u
-    integer*8 :: i, ix, n, kl, km, kn, kcnt, kmn, knn, nsel
+    integer*8 :: i, ix, n, kl, km, kn, kcnt, kmn, knn, nsel, nrep, irep, c0, c1, crate
+    character*32 :: arg2
     character*40 :: base
     character*256 :: outname
     real*8, allocatable :: f1x(:), f1y(:), f1z(:), f2x(:), f2y(:), f2z(:)
@@ -31,6 +32,10 @@ program ref_harness
     if (nproc /= 1) stop 'ref_harness: run on one rank'
 
     call get_command_argument(1, outname)
+    ! optional second argument: number of extra timed force evaluations
+    nrep = 0
+    call get_command_argument(2, arg2)
+    if (len_trim(arg2) > 0) read(arg2, *) nrep
     open(7, file='harness.inp', status='old')
     read(7, *) base
     read(7, *) dt, rcut, skin, rdel, kmax
@@ -177,6 +182,22 @@ program ref_harness
     ibuf(1:3) = (/ int(klmax,8), int(kmmax,8), int(knmax,8) /)
     call put_i(u, 'kmaxes', ibuf, 3_8)
     close(u)
+
+    ! timing: repeated force evaluations on the same configuration (no list rebuild)
+    if (nrep > 0) then
+        call system_clock(c0, crate)
+        do irep = 1, nrep
+            fx = 0.0 ; fy = 0.0 ; fz = 0.0
+            fwsx = 0.0 ; fwsy = 0.0 ; fwsz = 0.0
+            vir = 0.0 ; elj = 0.0 ; eqq = 0.0 ; uewa = 0.0
+            call force_real
+            call force_fouri
+            call force_intra
+            call force_sum
+        end do
+        call system_clock(c1)
+        write(*, '(a,i8,a,es16.8)') 'TIMING nrep=', nrep, ' seconds=', dble(c1 - c0)/dble(crate)
+    end if
 
     call mpi_finalize(ierr)
 

@@ -4,6 +4,7 @@ vectors in the development container, never by the product path."""
 from __future__ import annotations
 
 import os
+import re
 import subprocess
 import tempfile
 from typing import Dict
@@ -32,7 +33,7 @@ def parse_dump(path: str) -> Dict[str, np.ndarray]:
     return out
 
 
-def run_reference(system, workdir: str = None, timeout: float = 600.0) -> Dict[str, np.ndarray]:
+def run_reference(system, workdir: str = None, timeout: float = 600.0, nrep: int = 0) -> Dict[str, np.ndarray]:
     """Write the reference's input files for `system`, run the matching harness variant
     (bulk = 3-D periodic, slab otherwise) and return the parsed dump."""
     from md_sim_amd.refio import write_reference_inputs
@@ -45,9 +46,14 @@ def run_reference(system, workdir: str = None, timeout: float = 600.0) -> Dict[s
     d = tempfile.mkdtemp(prefix="mdref_") if own else workdir
     write_reference_inputs(system, d)
     out = os.path.join(d, "dump.bin")
-    r = subprocess.run([exe, out], cwd=d, capture_output=True, text=True, timeout=timeout)
+    args = [exe, out] + ([str(nrep)] if nrep > 0 else [])
+    r = subprocess.run(args, cwd=d, capture_output=True, text=True, timeout=timeout)
     if r.returncode != 0 or not os.path.exists(out):
         raise RuntimeError("reference harness failed (%d):\n%s\n%s" % (r.returncode, r.stdout[-2000:], r.stderr[-2000:]))
     dump = parse_dump(out)
     dump["_stdout"] = np.array(r.stdout)
+    for line in r.stdout.splitlines():
+        m = re.match(r"\s*TIMING nrep=\s*(\d+)\s+seconds=\s*(\S+)", line)
+        if m:
+            dump["_timing"] = np.array([float(m.group(1)), float(m.group(2))])
     return dump
