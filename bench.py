@@ -4,9 +4,9 @@ A step is one force evaluation of the reference's force path (force_get,
 src/forcem.f90:93-129) on device-resident positions: pair-list skin test (and rebuild when
 due), real-space erfc + LJ/Buckingham pair kernel, reciprocal k-sum (structure factor and
 back-projection), slab and wall corrections, COM forces/torques and energies.  The
-positions follow a synthetic ballistic trajectory (every molecule translated rigidly by a
-fixed thermal-size displacement per step, seeded), so the list is rebuilt as often as the
-skin criterion demands.
+positions follow a synthetic trajectory (every molecule oscillates rigidly about its start
+with a seeded amplitude of ~0.25 A and an 80-step period), so the list is rebuilt as often
+as the skin criterion demands.
 
 Workload (default ``slab16k``): 16384 rigid SPC/E waters on a frozen 6144-ion rock-salt
 slab, box 90.24 x 90.24 x 270.72 A, rc 10 A, skin 1 A, k-index 11/11/33 (8589 k-vectors),
@@ -14,8 +14,9 @@ FP64 (``--precision fp32`` selects FP32 arithmetic with FP64 accumulation).
 
 Prints one JSON line.  ``roofline`` is for the dominant stage, the reciprocal sum: its
 algorithmic work 18 flop per (k-vector, moving site) + 8 flop per ((kl,km) row, site)
-over the stage's mean kernel time, measured with HIP events on the engine's stream over
-the timed steps.  ``cpu_baseline`` times the reference's own force path (built from its
+over the stage's mean kernel time, measured with HIP events on the engine's stream over a
+second pass of the same steps run with per-stage events (the timed pass replays a captured
+HIP graph, inside which events cannot be timed).  ``cpu_baseline`` times the reference's own force path (built from its
 sources into oracle/_ref) on a few force evaluations of the same system on one core, or
 the CPU oracle when that build is absent.
 """
@@ -44,8 +45,9 @@ def parse():
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--cpu-reps", type=int, default=3, help="force evaluations in the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--step-displacement", type=float, default=0.0075,
-                    help="rms per-axis displacement of a molecule per step (A)")
+    ap.add_argument("--amplitude", type=float, default=0.25,
+                    help="rms per-axis oscillation amplitude of a molecule (A)")
+    ap.add_argument("--period", type=float, default=80.0, help="oscillation period (steps)")
     return ap.parse_args()
 
 
@@ -93,22 +95,26 @@ def main():
     nm, nr = s.nmove, s.nrig
     eng = ForceEngine(s, precision=prec)
 
-    # synthetic ballistic trajectory: one displacement per molecule (or mobile ion) per step
+    # synthetic trajectory: every molecule (and mobile ion) oscillates rigidly about its start,
+    # x(t) = x0 + a sin(2 pi t / P), a ~ N(0, amp) per axis, so the skin test fires and the
+    # list is rebuilt repeatedly while molecules never run into each other
     rng = np.random.default_rng(12345)
-    mol_v = rng.normal(0.0, args.step_displacement, size=(nr, 3))
-    site_v = np.zeros((nm, 3))
-    site_v[:3 * nr] = np.repeat(mol_v, 3, axis=0)
-    site_v[3 * nr:] = rng.normal(0.0, args.step_displacement, size=(nm - 3 * nr, 3))
+    mol_a = np.clip(rng.normal(0.0, args.amplitude, size=(nr, 3)), -3 * args.amplitude, 3 * args.amplitude)
+    site_a = np.zeros((nm, 3))
+    site_a[:3 * nr] = np.repeat(mol_a, 3, axis=0)
+    site_a[3 * nr:] = np.clip(rng.normal(0.0, args.amplitude, size=(nm - 3 * nr, 3)), -3 * args.amplitude,
+                              3 * args.amplitude)
     base = torch.tensor(np.stack([x[:nm], y[:nm], z[:nm]]), dtype=torch.float64, device=dev)
-    vel = torch.tensor(site_v.T.copy(), dtype=torch.float64, device=dev)
+    amp = torch.tensor(site_a.T.copy(), dtype=torch.float64, device=dev)
     cbase = torch.tensor(np.stack([cx, cy, cz]), dtype=torch.float64, device=dev)
-    cvel = torch.tensor(mol_v.T.copy(), dtype=torch.float64, device=dev)
+    camp = torch.tensor(mol_a.T.copy(), dtype=torch.float64, device=dev)
     n_frames = args.warmup + args.steps
     frames = torch.empty((n_frames, 3, nm), dtype=torch.float64, device=dev)
     cframes = torch.empty((n_frames, 3, max(nr, 1)), dtype=torch.float64, device=dev)
     for t in range(n_frames):
-        frames[t] = base + t * vel
-        cframes[t, :, :nr] = cbase + t * cvel
+        ph = float(np.sin(2.0 * np.pi * t / args.period))
+        frames[t] = base + ph * amp
+        cframes[t, :, :nr] = cbase + ph * camp
     torch.cuda.synchronize()
 
     def step(t):
@@ -118,7 +124,7 @@ def main():
 
     for t in range(args.warmup):
         step(t)
-    eng.read_outputs()
+    first = eng.read_outputs()
     torch.cuda.synchronize()
     eng.reset_stage_times()
     torch.cuda.synchronize()
@@ -129,7 +135,16 @@ def main():
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     sec = t1 - t0
+    step_gpu_ms = eng.stage_times()["step"]
+    # kernel durations: the same trajectory once more with HIP events between the stages
+    # (direct launches; events inside a replayed graph cannot be timed)
+    eng.set_stage_timing(True)
+    eng.reset_stage_times()
+    for t in range(args.warmup, n_frames):
+        step(t)
+    eng.read_outputs()
     stages = eng.stage_times()
+    eng.set_stage_timing(False)
 
     kv, ak, _ = eng.kvectors()
     n_k = len(ak)
@@ -164,7 +179,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32 arithmetic, f64 accumulation",
-        "data": "synthetic (seeded lattice water on a rock-salt slab, ballistic trajectory)",
+        "data": "synthetic (seeded lattice water on a rock-salt slab, oscillating trajectory)",
         "config": {
             "workload": "%s: %d SPC/E waters + %d frozen ions, box %.2f x %.2f x %.2f A, rc %.1f A, "
                         "skin %.1f A, k-index %d (%d k-vectors), %s; one force evaluation per step"
@@ -188,7 +203,8 @@ def main():
         },
         "cpu_baseline": cpu,
         "stages_ms": stages,
-        "list_rebuilt_last": last["list_rebuilt"],
+        "step_gpu_ms_timed": step_gpu_ms,
+        "list_builds_timed": last["n_list_builds"] - first["n_list_builds"],
         "e_conf_last": last["e_conf"],
     }
     if cpu and "value" in cpu:

@@ -127,6 +127,7 @@ typedef struct mdf_outputs {
     int32_t n_pairs_listed;  /* ipack  */
     int32_t n_wall_pairs_listed; /* wpack */
     int32_t n_kvec;          /* k-vectors in the set */
+    int32_t n_list_builds;   /* list builds since mdf_create, the first one included */
 } mdf_outputs;
 
 typedef struct mdf_system mdf_system;
@@ -179,9 +180,13 @@ void mdf_destroy(mdf_system *sys);
 /* Message for the last failing call (per thread).                                    */
 const char *mdf_last_error(void);
 
-/* Stage timing: names and mean milliseconds per step of each internal stage over the
-   steps since mdf_reset_stage_times (HIP events recorded on the handle's stream around
-   each stage).  cap: capacity of names/ms; n out: number of stages.                  */
+/* Timing.  Steps normally replay one captured HIP graph; with stage timing on they use
+   direct launches with HIP events between the stages instead (events recorded during
+   graph capture cannot be timed).  mdf_stage_times gives names and mean milliseconds per
+   step over the steps since mdf_reset_stage_times: the stages (from steps run with stage
+   timing on) followed by "step", the whole step measured in both modes.  cap: capacity
+   of names/ms; n out: number of entries.                                              */
+int mdf_set_stage_timing(mdf_system *sys, int32_t on);
 int mdf_stage_times(mdf_system *sys, const char **names, float *ms, int32_t cap, int32_t *n);
 int mdf_reset_stage_times(mdf_system *sys);
 

@@ -53,7 +53,7 @@ class Outputs(C.Structure):
                 ("e_wall_lr", C.c_double), ("e_conf", C.c_double), ("e_avex", C.c_double),
                 ("press_wall", C.c_double), ("f_wall", C.c_double * 9), ("virial", C.c_double * 9),
                 ("list_rebuilt", C.c_int32), ("n_pairs_listed", C.c_int32),
-                ("n_wall_pairs_listed", C.c_int32), ("n_kvec", C.c_int32)]
+                ("n_wall_pairs_listed", C.c_int32), ("n_kvec", C.c_int32), ("n_list_builds", C.c_int32)]
 
 
 SCALAR_OUTPUTS = ("e_recip", "e_recip_ll", "e_recip_wl", "e_recip_ww", "e_self", "e_slab",
@@ -161,4 +161,5 @@ def collect(o: Outputs, arrs, n_move: int, n_rig: int) -> ForceResult:
     r["n_pairs_listed"] = int(o.n_pairs_listed)
     r["n_wall_pairs_listed"] = int(o.n_wall_pairs_listed)
     r["n_kvec"] = int(o.n_kvec)
+    r["n_list_builds"] = int(o.n_list_builds)
     return r

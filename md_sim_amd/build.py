@@ -41,8 +41,22 @@ def build(verbose: bool = False, jobs: int = 5) -> str:
             print(out)
     if failed:
         raise RuntimeError("\n".join("%s:\n%s" % f for f in failed))
+    # Link against a symbol-less stub whose soname is the generic "libamdhip64.so": the
+    # library then needs the HIP runtime under that name, found through the rpath in
+    # /opt/rocm/lib when the process has none, and shared with the runtime a host process
+    # (PyTorch-ROCm bundles one under the same name) has already loaded - one HIP runtime
+    # per process instead of two.
+    stubdir = os.path.join(objdir, "stub")
+    os.makedirs(stubdir, exist_ok=True)
+    stub_src = os.path.join(stubdir, "stub.c")
+    with open(stub_src, "w") as f:
+        f.write("/* link-time stand-in carrying the soname of the HIP runtime */\n")
+    subprocess.check_call(["gcc", "-shared", "-fPIC", "-Wl,-soname,libamdhip64.so", "-o",
+                           os.path.join(stubdir, "libamdhip64.so"), stub_src])
+    rocm_lib = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")
     tmp = LIB + ".tmp"
-    subprocess.check_call([hipcc, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp, *objs])
+    subprocess.check_call(["g++", "-shared", "-fPIC", "-o", tmp, *objs, "-L" + stubdir, "-Wl,--no-as-needed", "-lamdhip64", "-Wl,--as-needed",
+                           "-Wl,--disable-new-dtags", "-Wl,-rpath," + rocm_lib])
     os.replace(tmp, LIB)
     return LIB
 

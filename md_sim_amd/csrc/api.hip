@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -54,6 +55,19 @@ __global__ void k_com(DevSys S, double *fcom, double *tq);
 __global__ void k_colsum(DevSys S, const int *lead_slot, int per_block, double *colpart);
 __global__ void k_final(DevSys S, FinalArgs A, const double *colpart, const double *kpart, const double *eppart,
                         const double *dipz, double *e_vdw, double *e_coul, Scalars *out);
+
+// device-resident inputs -> the handle's buffers, one launch
+__global__ void k_gather(DevSys S, const double *__restrict__ x, const double *__restrict__ y,
+                         const double *__restrict__ z, const double *__restrict__ cx, const double *__restrict__ cy,
+                         const double *__restrict__ cz) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < S.n_move) { S.x[t] = x[t]; S.y[t] = y[t]; S.z[t] = z[t]; }
+    if (t < S.n_rig) {
+        const_cast<double *>(S.cx)[t] = cx[t];
+        const_cast<double *>(S.cy)[t] = cy[t];
+        const_cast<double *>(S.cz)[t] = cz[t];
+    }
+}
 }  // namespace mdf
 
 using namespace mdf;
@@ -114,8 +128,8 @@ struct DBuf {
     }
 };
 
-static const char *STAGE_NAMES[] = {"list", "pair", "sf", "sf_reduce", "backproj", "epilogue", "reduce"};
-constexpr int N_STAGES = 7;
+static const char *STAGE_NAMES[] = {"list", "pair", "sf", "sf_reduce", "backproj", "epilogue", "reduce", "step"};
+constexpr int N_STAGES = 7;  // timed stages; entry N_STAGES of the names is the whole step
 
 struct mdf_system {
     mdf_topology top;
@@ -163,10 +177,17 @@ struct mdf_system {
     bool ring_used[NRING] = {};
     long long step_no = 0;
     hipEvent_t *ev = nullptr;       // events of the step being launched
+    // one captured step graph per ring slot (each records its slot's events)
+    hipGraphExec_t graph[NRING] = {};
+    bool use_graphs = true;
     double stage_sum[N_STAGES] = {};
-    long long stage_steps = 0;
-    float stage_ms[N_STAGES] = {};  // last completed step
-    bool timing = true;
+    long long stage_steps = 0;       // steps with per-stage events (direct launches)
+    double step_sum = 0.0;
+    long long step_steps = 0;        // steps with start/end events (any mode)
+    bool slot_staged[NRING] = {};
+    float stage_ms[N_STAGES] = {};   // last completed step
+    bool stage_timing = false;       // per-stage events: direct launches instead of graphs
+    long long timing_failures = 0;
     bool pending = false;
     int last_rebuilt = 0;
     DevFlags last_flags;
@@ -307,7 +328,14 @@ static void geometry(mdf_system *s) {
     s->colper = std::max(1, (s->n_lead + s->n_colblk - 1) / s->n_colblk);
 }
 
+static void drop_graphs(mdf_system *s) {
+    for (int r = 0; r < mdf_system::NRING; r++)
+        if (s->graph[r]) { (void)hipGraphExecDestroy(s->graph[r]); s->graph[r] = nullptr; }
+}
+
 static int alloc_lists(mdf_system *s, int cap_mm, int cap_mw) {
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    drop_graphs(s);
     s->nbr_mm.release(); s->nbr_mw.release();
     HIPCHK(s->nbr_mm.alloc((size_t)s->n_lead * cap_mm));
     HIPCHK(s->nbr_mw.alloc((size_t)s->n_lead * std::max(cap_mw, 1)));
@@ -343,14 +371,14 @@ static void launch_forces(mdf_system *s) {
     const double cuz = 2.0 * s->c.pi / s->par.box[2] / M_PI;
     const double pipix = 2.0 * s->c.pi / s->par.box[0], pipiy = 2.0 * s->c.pi / s->par.box[1],
                  pipiz = 2.0 * s->c.pi / s->par.box[2];
-    if (s->timing) (void)hipEventRecord(s->ev[1], st);
+    if (s->ev) (void)hipEventRecord(s->ev[1], st);
     {
         const int G = 16;
         int blocks = std::max(1, (s->n_lead + (256 / G) - 1) / (256 / G));
         if (f32) k_pair<float, 16><<<blocks, 256, 0, st>>>(S);
         else k_pair<double, 16><<<blocks, 256, 0, st>>>(S);
     }
-    if (s->timing) (void)hipEventRecord(s->ev[2], st);
+    if (s->ev) (void)hipEventRecord(s->ev[2], st);
     int nk = (int)s->ak.size(), nch = (int)s->chunks.size(), nrows = (int)s->rows.size();
     if (nk > 0 && nm > 0) {
         dim3 g1(s->n_sb, s->n_cg);
@@ -359,14 +387,14 @@ static void launch_forces(mdf_system *s) {
         else k_sf<double, 4><<<g1, SF_BLOCK, 0, st>>>(S.x, S.y, S.z, S.q, nm, cux, cuy, cuz, s->d_chunks.p, nch,
                                                         s->cpb, s->sf_partial.p, nk);
     }
-    if (s->timing) (void)hipEventRecord(s->ev[3], st);
+    if (s->ev) (void)hipEventRecord(s->ev[3], st);
     if (nk > 0) {
         double lamb = 1.0 / (4.0 * S.alfa * S.alfa);
         k_sf_reduce<<<s->n_kblk, 256, 0, st>>>(s->sf_partial.p, nm > 0 ? s->n_sb : 0, nk, s->d_swall.p, s->d_ak.p,
                                                s->d_kvec.p, pipix, pipiy, pipiz, lamb, s->d_s.p, s->d_t.p,
                                                s->kpart.p);
     }
-    if (s->timing) (void)hipEventRecord(s->ev[4], st);
+    if (s->ev) (void)hipEventRecord(s->ev[4], st);
     if (nk > 0 && nm > 0) {
         dim3 g2(std::max(1, (nm + 255) / 256), s->n_rg);
         if (f32) k_bp<float><<<g2, 256, 0, st>>>(S.x, S.y, S.z, S.q, nm, cux, cuy, cuz, s->d_rows.p, nrows, s->rpg,
@@ -374,11 +402,11 @@ static void launch_forces(mdf_system *s) {
         else k_bp<double><<<g2, 256, 0, st>>>(S.x, S.y, S.z, S.q, nm, cux, cuy, cuz, s->d_rows.p, nrows, s->rpg,
                                               s->d_t.p, pipix, pipiy, pipiz, s->fpart.p);
     }
-    if (s->timing) (void)hipEventRecord(s->ev[5], st);
+    if (s->ev) (void)hipEventRecord(s->ev[5], st);
     k_epilogue<<<s->n_epblk, 256, 0, st>>>(S, s->fpart.p, (nk > 0 && nm > 0) ? s->n_rg : 0, s->d_wl.p,
                                            s->par.ew3dc, s->crf0, s->dipz.p, s->eppart.p);
     if (s->top.n_rig > 0) k_com<<<(s->top.n_rig + 255) / 256, 256, 0, st>>>(S, s->fcom.p, s->torque.p);
-    if (s->timing) (void)hipEventRecord(s->ev[6], st);
+    if (s->ev) (void)hipEventRecord(s->ev[6], st);
     {
         dim3 gc(NCOL, s->n_colblk);
         k_colsum<<<gc, 256, 0, st>>>(S, s->lead_slot.p, s->colper, s->colpart.p);
@@ -401,19 +429,30 @@ static void launch_forces(mdf_system *s) {
         k_final<<<1, 256, 0, st>>>(S, A, s->colpart.p, s->kpart.p, s->eppart.p, s->dipz.p, s->e_vdw.p, s->e_coul.p,
                                    s->d_scal.p);
     }
-    if (s->timing) (void)hipEventRecord(s->ev[7], st);
+    if (s->ev) (void)hipEventRecord(s->ev[7], st);
 }
 
 static void drain_slot(mdf_system *s, int slot) {
     if (!s->ring_used[slot]) return;
-    (void)hipEventSynchronize(s->ring[slot][N_STAGES]);
-    for (int k = 0; k < N_STAGES; k++) {
-        float ms = 0.f;
-        (void)hipEventElapsedTime(&ms, s->ring[slot][k], s->ring[slot][k + 1]);
-        s->stage_sum[k] += ms;
-        s->stage_ms[k] = ms;
+    hipEvent_t *e = s->ring[slot];
+    bool ok = hipEventSynchronize(e[N_STAGES]) == hipSuccess;
+    float tot = 0.f;
+    ok = ok && hipEventElapsedTime(&tot, e[0], e[N_STAGES]) == hipSuccess;
+    float ms[N_STAGES];
+    bool staged = ok && s->slot_staged[slot];
+    for (int k = 0; k < N_STAGES && staged; k++) {
+        ms[k] = 0.f;
+        staged = hipEventElapsedTime(&ms[k], e[k], e[k + 1]) == hipSuccess;
     }
-    s->stage_steps++;
+    if (ok) { s->step_sum += tot; s->step_steps++; }
+    if (staged) {
+        for (int k = 0; k < N_STAGES; k++) { s->stage_sum[k] += ms[k]; s->stage_ms[k] = ms[k]; }
+        s->stage_steps++;
+    }
+    if (!ok || (s->slot_staged[slot] && !staged)) {
+        s->timing_failures++;
+        (void)hipGetLastError();  // timing is best effort: do not leave a sticky error behind
+    }
     s->ring_used[slot] = false;
 }
 
@@ -426,17 +465,41 @@ static void drain_all(mdf_system *s) {
     }
 }
 
+// one step: the fixed launch sequence, replayed from a captured graph when possible (a
+// forced rebuild goes through direct launches)
 static int run_step(mdf_system *s, int force_rebuild) {
-    if (s->timing) {
-        int slot = (int)(s->step_no % mdf_system::NRING);
-        drain_slot(s, slot);
-        s->ev = s->ring[slot];
-        s->ring_used[slot] = true;
-        s->step_no++;
-        (void)hipEventRecord(s->ev[0], s->stream);
+    int slot = (int)(s->step_no % mdf_system::NRING);
+    drain_slot(s, slot);
+    s->ev = s->ring[slot];
+    s->ring_used[slot] = true;
+    s->step_no++;
+    hipEvent_t *ev = s->ev;
+    if (s->use_graphs && !force_rebuild && !s->stage_timing) {
+        // the graph holds no events (HIP cannot time events recorded during capture); the
+        // step is bracketed by events recorded outside it
+        s->slot_staged[slot] = false;
+        if (!s->graph[0]) {
+            hipGraph_t g = nullptr;
+            s->ev = nullptr;
+            HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeRelaxed));
+            launch_lists(s, 0);
+            launch_forces(s);
+            hipError_t ec = hipStreamEndCapture(s->stream, &g);
+            s->ev = ev;
+            if (ec != hipSuccess) return fail(MDF_ERR_HIP, "graph capture: %s", hipGetErrorString(ec));
+            ec = hipGraphInstantiate(&s->graph[0], g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ec != hipSuccess) return fail(MDF_ERR_HIP, "graph instantiate: %s", hipGetErrorString(ec));
+        }
+        (void)hipEventRecord(ev[0], s->stream);
+        HIPCHK(hipGraphLaunch(s->graph[0], s->stream));
+        (void)hipEventRecord(ev[N_STAGES], s->stream);
+    } else {
+        s->slot_staged[slot] = true;
+        (void)hipEventRecord(ev[0], s->stream);
+        launch_lists(s, force_rebuild);
+        launch_forces(s);
     }
-    launch_lists(s, force_rebuild);
-    launch_forces(s);
     HIPCHK(hipGetLastError());
     s->pending = true;
     return MDF_OK;
@@ -460,7 +523,7 @@ static int finish_step(mdf_system *s, int *redo) {
         if (rc) return rc;
         *redo = 1;
     }
-    if (s->timing) drain_all(s);
+    drain_all(s);
     return MDF_OK;
 }
 
@@ -489,6 +552,7 @@ static int copy_outputs(mdf_system *s, mdf_outputs *o) {
     o->n_pairs_listed = s->last_flags.n_pairs_mm;
     o->n_wall_pairs_listed = s->last_flags.n_pairs_mw;
     o->n_kvec = (int)s->ak.size();
+    o->n_list_builds = s->last_flags.n_builds;
     if (sc.nonfinite) return fail(MDF_ERR_NONFINITE, "non-finite configurational energy");
     return MDF_OK;
 }
@@ -526,6 +590,7 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
             t.nsi[t.site_type[3 * m + 2] - 1] != 0)
             return fail(MDF_ERR_ARG, "rigid molecule %d is not a 3-site lead + 2 carried sites", m);
 
+    (void)hipGetLastError();  // clear an error left by an earlier, unrelated call
     mdf_system *s = new mdf_system();
     s->top = t;
     s->par = *par;
@@ -533,6 +598,10 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
     s->device = par->device;
     s->nijt = nijt;
     s->n_wall = t.n_total - t.n_move;
+    {
+        const char *g = getenv("MDF_GRAPHS");
+        if (g && g[0] == '0') s->use_graphs = false;
+    }
     hipError_t he = hipSetDevice(par->device);
     if (he != hipSuccess) { delete s; return fail(MDF_ERR_HIP, "hipSetDevice: %s", hipGetErrorString(he)); }
     int rc = MDF_OK;
@@ -677,6 +746,11 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
             S.cell_cursor = s->cell_cursor.p;
         }
         HTRY(s->flags.alloc(1));
+        {
+            DevFlags f0;
+            memset(&f0, 0, sizeof f0);
+            HTRY(hipMemcpy(s->flags.p, &f0, sizeof f0, hipMemcpyHostToDevice));
+        }
         S.flags = s->flags.p;
         // frozen-site cells, built once
         {
@@ -885,9 +959,10 @@ int mdf_compute_device(mdf_system *s, const double *d_x, const double *d_y, cons
                        const double *d_cy, const double *d_cz, void *stream) {
     if (!s || !d_x || !d_y || !d_z) return fail(MDF_ERR_ARG, "null argument");
     (void)stream;
+    if (s->top.n_rig > 0 && (!d_cx || !d_cy || !d_cz)) return fail(MDF_ERR_ARG, "COM arrays required");
     HIPCHK(hipSetDevice(s->device));
-    int rc = set_inputs(s, d_x, d_y, d_z, d_cx, d_cy, d_cz, hipMemcpyDeviceToDevice);
-    if (rc) return rc;
+    int n = std::max(s->top.n_move, s->top.n_rig);
+    k_gather<<<std::max(1, (n + 255) / 256), 256, 0, s->stream>>>(s->S, d_x, d_y, d_z, d_cx, d_cy, d_cz);
     return run_step(s, 0);
 }
 
@@ -972,11 +1047,24 @@ int mdf_stage_times(mdf_system *s, const char **names, float *ms, int32_t cap, i
     HIPCHK(hipSetDevice(s->device));
     HIPCHK(hipStreamSynchronize(s->stream));
     drain_all(s);
-    *n = N_STAGES;
+    *n = N_STAGES + 1;
     for (int k = 0; k < N_STAGES && k < cap; k++) {
         if (names) names[k] = STAGE_NAMES[k];
         if (ms) ms[k] = s->stage_steps > 0 ? (float)(s->stage_sum[k] / (double)s->stage_steps) : 0.f;
     }
+    if (N_STAGES < cap) {
+        if (names) names[N_STAGES] = STAGE_NAMES[N_STAGES];
+        if (ms) ms[N_STAGES] = s->step_steps > 0 ? (float)(s->step_sum / (double)s->step_steps) : 0.f;
+    }
+    return MDF_OK;
+}
+
+int mdf_set_stage_timing(mdf_system *s, int32_t on) {
+    if (!s) return fail(MDF_ERR_ARG, "null handle");
+    HIPCHK(hipSetDevice(s->device));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    drain_all(s);
+    s->stage_timing = on != 0;
     return MDF_OK;
 }
 
@@ -987,6 +1075,8 @@ int mdf_reset_stage_times(mdf_system *s) {
     drain_all(s);
     for (int k = 0; k < N_STAGES; k++) s->stage_sum[k] = 0.0;
     s->stage_steps = 0;
+    s->step_sum = 0.0;
+    s->step_steps = 0;
     return MDF_OK;
 }
 
@@ -994,6 +1084,7 @@ void mdf_destroy(mdf_system *s) {
     if (!s) return;
     (void)hipSetDevice(s->device);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
+    drop_graphs(s);
     free_all(s);
     for (int r = 0; r < mdf_system::NRING; r++)
         for (int k = 0; k <= N_STAGES; k++)

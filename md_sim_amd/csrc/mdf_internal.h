@@ -74,7 +74,7 @@ struct DevFlags {
     int ncz;                             // cells in z for this build
     double z0, csz;                      // z origin and cell size for this build
     int n_pairs_mm, n_pairs_mw;          // listed pairs (i<j count and wall count)
-    int pad;
+    int n_builds;                        // builds since creation (not reset per step)
 };
 
 struct Grid {

@@ -22,8 +22,6 @@ __global__ void k_reset(DevSys S, int force_rebuild) {
         f->rebuild = force_rebuild;
         f->overflow_mm = 0;
         f->overflow_mw = 0;
-        f->n_pairs_mm = 0;
-        f->n_pairs_mw = 0;
     }
 }
 
@@ -76,6 +74,12 @@ __global__ void k_decide(DevSys S, const double *dip_part, int n_dip, double *di
     DevFlags *f = S.flags;
     double aux = __longlong_as_double((long long)f->max_disp2_bits);
     if (__dmul_rn(4.0, aux) > S.skin2) f->rebuild = 1;
+    if (f->rebuild) {
+        // pair counts describe the standing list (ipack/wpack): restart them for this build
+        f->n_builds += 1;
+        f->n_pairs_mm = 0;
+        f->n_pairs_mw = 0;
+    }
     double dip = 0.0;
     for (int b = 0; b < n_dip; b++) dip += dip_part[b];
     *dipz_out = dip;

@@ -41,11 +41,18 @@ def lib() -> C.CDLL:
         if not os.path.exists(LIB_PATH):
             raise ImportError("libmdforce.so is not built (run __graft_entry__.build() or "
                               "python -m md_sim_amd.build)")
+        # The library needs the HIP runtime as "libamdhip64.so".  PyTorch-ROCm ships its own
+        # runtime under that name; loading torch first makes the library bind to it, so a
+        # process that also uses torch tensors has one HIP runtime, not two.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         L.mdf_last_error.restype = C.c_char_p
         for fn in ("mdf_create", "mdf_compute", "mdf_compute_device", "mdf_read_outputs", "mdf_get_pairs",
                    "mdf_get_kvectors", "mdf_get_structure_factor", "mdf_stage_times", "mdf_device_results",
-                   "mdf_zero", "mdf_reset_stage_times"):
+                   "mdf_zero", "mdf_reset_stage_times", "mdf_set_stage_timing"):
             getattr(L, fn).restype = C.c_int
         L.mdf_destroy.restype = None
         _lib = L
@@ -126,6 +133,9 @@ class ForceEngine:
         n = C.c_int32()
         _check(lib().mdf_stage_times(self._h, names, ms, 16, C.byref(n)))
         return {names[k].decode(): float(ms[k]) for k in range(n.value)}
+
+    def set_stage_timing(self, on: bool):
+        _check(lib().mdf_set_stage_timing(self._h, int(bool(on))))
 
     def reset_stage_times(self):
         _check(lib().mdf_reset_stage_times(self._h))

@@ -43,7 +43,7 @@ module mdforce_c
         real(c_double) :: e_recip, e_recip_ll, e_recip_wl, e_recip_ww, e_self, e_slab, &
                           e_wall_lr, e_conf, e_avex, press_wall
         real(c_double) :: f_wall(9), virial(9)
-        integer(c_int) :: list_rebuilt, n_pairs_listed, n_wall_pairs_listed, n_kvec
+        integer(c_int) :: list_rebuilt, n_pairs_listed, n_wall_pairs_listed, n_kvec, n_list_builds
     end type mdf_outputs
 
     interface
