@@ -45,6 +45,7 @@ def parse():
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--cpu-reps", type=int, default=3, help="force evaluations in the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--virial", action="store_true", help="also accumulate the pair virial")
     ap.add_argument("--amplitude", type=float, default=0.25,
                     help="rms per-axis oscillation amplitude of a molecule (A)")
     ap.add_argument("--period", type=float, default=80.0, help="oscillation period (steps)")
@@ -93,7 +94,8 @@ def main():
     prec = abi.FP32_ACC64 if args.precision == "fp32" else abi.FP64
     x, y, z, cx, cy, cz = abi.positions(s)
     nm, nr = s.nmove, s.nrig
-    eng = ForceEngine(s, precision=prec)
+    # the reference computes no virial; it is an opt-in extra of this engine
+    eng = ForceEngine(s, precision=prec, virial=args.virial)
 
     # synthetic trajectory: every molecule (and mobile ion) oscillates rigidly about its start,
     # x(t) = x0 + a sin(2 pi t / P), a ~ N(0, amp) per axis, so the skin test fires and the

@@ -101,6 +101,8 @@ typedef struct mdf_params {
     int32_t erfc_mode;       /* MDF_ERFC_*                                           */
     int32_t max_neighbors;   /* per-lead list capacity, 0 = default                  */
     int32_t device;          /* HIP device ordinal                                   */
+    int32_t virial;          /* 1: accumulate the real-space pair virial (the reference
+                                has none; 0 leaves that part of mdf_outputs.virial zero) */
 } mdf_params;
 
 typedef struct mdf_outputs {

@@ -43,7 +43,7 @@ class Params(C.Structure):
                 ("alpha", C.c_double), ("alpha_mode", C.c_int32), ("kmax", C.c_int32),
                 ("z_mode", C.c_int32), ("ew3dc", C.c_int32), ("wall_lr", C.c_int32),
                 ("precision", C.c_int32), ("const_mode", C.c_int32), ("erfc_mode", C.c_int32),
-                ("max_neighbors", C.c_int32), ("device", C.c_int32)]
+                ("max_neighbors", C.c_int32), ("device", C.c_int32), ("virial", C.c_int32)]
 
 
 class Outputs(C.Structure):
@@ -76,7 +76,8 @@ class SystemArrays:
 def build_abi(system, precision: int = FP64, const_mode: int = CONST_REFERENCE,
               erfc_mode: int = ERFC_AS7126, alpha: Optional[float] = None,
               z_mode: Optional[int] = None, ew3dc: Optional[bool] = None,
-              wall_lr: Optional[bool] = None, max_neighbors: int = 0, device: int = 0) -> SystemArrays:
+              wall_lr: Optional[bool] = None, max_neighbors: int = 0, device: int = 0,
+              virial: bool = True) -> SystemArrays:
     """Topology and parameters of a ``systems.System``.  Defaults follow the reference
     variant the system corresponds to: slab systems use src/forcem.f90 (open z, slab
     correction and wall tail on); bulk systems use src/forcem_back.f90 (periodic z in the
@@ -117,6 +118,7 @@ def build_abi(system, precision: int = FP64, const_mode: int = CONST_REFERENCE,
     par.precision, par.const_mode, par.erfc_mode = precision, const_mode, erfc_mode
     par.max_neighbors = max_neighbors
     par.device = device
+    par.virial = int(bool(virial))
     return SystemArrays(top, par, keep, s.nijt)
 
 

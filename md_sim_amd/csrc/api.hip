@@ -39,16 +39,17 @@ __global__ void k_snapshot(DevSys S);
 __global__ void k_wall_cells(DevSys S, int *wcell_of, int ncz, double z0, double csz);
 __global__ void k_count_cells(const int *item_cell, int n_items, int *cnt);
 __global__ void k_zero_int(int *p, int n);
-template <typename R, int G>
+template <typename R, int G, bool VIR, bool ROLLED>
 __global__ void k_pair(DevSys S);
 template <typename R, int NS>
 __global__ void k_sf(const double *, const double *, const double *, const double *, int, double, double, double,
-                     const KChunk *, int, int, double2 *, int);
+                     const KRow *, int, int, double2 *, int);
 __global__ void k_sf_reduce(const double2 *, int, int, const double2 *, const double *, const int *, double, double,
-                            double, double, double2 *, double2 *, double *);
-template <typename R>
+                            double, double, double2 *, double4 *, double *);
+template <typename R, int NS>
 __global__ void k_bp(const double *, const double *, const double *, const double *, int, double, double, double,
-                     const KRow *, int, int, const double2 *, double, double, double, double *);
+                     const KRow *, const int *, const double4 *, double, double, double, double *);
+constexpr int BP_NS = 4;   // sites per back-projection thread
 __global__ void k_epilogue(DevSys S, const double *fpart, int n_groups, const WallLR *wl, int ew3dc, double crf0,
                            const double *dipz, double *epart);
 __global__ void k_com(DevSys S, double *fcom, double *tq);
@@ -145,6 +146,7 @@ struct mdf_system {
     std::vector<int> slot_type, cls_type;
     std::vector<KChunk> chunks;
     std::vector<KRow> rows;
+    std::vector<int> bp_row_start;
     std::vector<int> kvec;
     std::vector<double> ak, s_wall;
     double ewqWW = 0, uself = 0, crf0 = 0;
@@ -161,9 +163,10 @@ struct mdf_system {
     DBuf<double> dip_part, dipz;
     DBuf<KChunk> d_chunks;
     DBuf<KRow> d_rows;
-    DBuf<int> d_kvec;
+    DBuf<int> d_kvec, d_bp_row_start;
     DBuf<double> d_ak;
-    DBuf<double2> d_swall, d_s, d_t, sf_partial;
+    DBuf<double2> d_swall, d_s, sf_partial;
+    DBuf<double4> d_t;
     DBuf<double> kpart, fpart, eppart, colpart, fcom, torque, e_vdw, e_coul;
     DBuf<WallLR> d_wl;
     DBuf<Scalars> d_scal;
@@ -180,6 +183,7 @@ struct mdf_system {
     // one captured step graph per ring slot (each records its slot's events)
     hipGraphExec_t graph[NRING] = {};
     bool use_graphs = true;
+    bool pair_rolled = false;   // site pairs of a molecule pair as a rolled loop (fewer registers)
     double stage_sum[N_STAGES] = {};
     long long stage_steps = 0;       // steps with per-stage events (direct launches)
     double step_sum = 0.0;
@@ -204,7 +208,7 @@ static void free_all(mdf_system *s) {
                           &s->fz, &s->part, &s->dip_part, &s->dipz, &s->d_ak, &s->kpart, &s->fpart, &s->eppart,
                           &s->colpart, &s->fcom, &s->torque, &s->e_vdw, &s->e_coul};
     for (auto *b : db) b->release();
-    s->flags.release(); s->d_chunks.release(); s->d_rows.release(); s->d_swall.release(); s->d_s.release();
+    s->flags.release(); s->d_bp_row_start.release(); s->d_chunks.release(); s->d_rows.release(); s->d_swall.release(); s->d_s.release();
     s->d_t.release(); s->sf_partial.release(); s->d_wl.release(); s->d_scal.release();
 }
 
@@ -312,17 +316,36 @@ static void geometry(mdf_system *s) {
     s->skin_blocks = std::max(1, std::min(1024, (work + 255) / 256));
     int per_sb = SF_BLOCK * 4;
     s->n_sb = std::max(1, (nm + per_sb - 1) / per_sb);
-    int nch = (int)s->chunks.size();
-    s->n_cg = std::max(1, std::min(nch, (2048 + s->n_sb - 1) / s->n_sb));
-    s->cpb = std::max(1, (nch + s->n_cg - 1) / s->n_cg);
-    s->n_cg = std::max(1, (nch + s->cpb - 1) / s->cpb);
+    int nrow = (int)s->rows.size();
+    s->n_cg = std::max(1, std::min(nrow, (2048 + s->n_sb - 1) / s->n_sb));   // row groups
+    s->cpb = std::max(1, (nrow + s->n_cg - 1) / s->n_cg);                     // rows per group
+    s->n_cg = std::max(1, (nrow + s->cpb - 1) / s->cpb);
     int nk = (int)s->ak.size();
     s->n_kblk = std::max(1, (nk + 255) / 256);
-    int nsb_bp = std::max(1, (nm + 255) / 256);
+    // back-projection row groups: about equal k counts, each at most BP_GROUP_K k-vectors (the
+    // LDS stage), and enough groups that sites x groups fill the GPU
+    int nsb_bp = std::max(1, (nm + 256 * BP_NS - 1) / (256 * BP_NS));
     int nrows = (int)s->rows.size();
-    s->n_rg = std::max(1, std::min(std::min(nrows, BP_ROWGROUPS_MAX), (2048 + nsb_bp - 1) / nsb_bp));
-    s->rpg = std::max(1, (nrows + s->n_rg - 1) / s->n_rg);
-    s->n_rg = std::max(1, (nrows + s->rpg - 1) / s->rpg);
+    int nk_all = (int)s->ak.size();
+    int want = std::max(1, std::min(std::min(nrows, BP_ROWGROUPS_MAX), (1024 + nsb_bp - 1) / nsb_bp));
+    want = std::max(want, (nk_all + BP_GROUP_K - 1) / BP_GROUP_K);
+    s->bp_row_start.clear();
+    s->bp_row_start.push_back(0);
+    {
+        const double per = (double)nk_all / want;
+        int group_k = 0;
+        for (int r = 0; r < nrows; r++) {
+            const int len = s->rows[r].kn1 - s->rows[r].kn0 + 1;
+            if (r > s->bp_row_start.back() && (group_k + len > BP_GROUP_K || group_k >= per)) {
+                s->bp_row_start.push_back(r);
+                group_k = 0;
+            }
+            group_k += len;
+        }
+        s->bp_row_start.push_back(nrows);
+    }
+    s->n_rg = (int)s->bp_row_start.size() - 1;
+    s->rpg = 0;
     s->n_epblk = std::max(1, (nm + 255) / 256);
     s->n_colblk = std::max(1, std::min(64, (s->n_lead + 1023) / 1024));
     s->colper = std::max(1, (s->n_lead + s->n_colblk - 1) / s->n_colblk);
@@ -350,14 +373,14 @@ static void launch_lists(mdf_system *s, int force_rebuild) {
     hipStream_t st = s->stream;
     k_reset<<<1, 64, 0, st>>>(S, force_rebuild);
     k_skin<<<s->skin_blocks, 256, 0, st>>>(S, s->dip_part.p);
-    k_decide<<<1, 64, 0, st>>>(S, s->dip_part.p, s->skin_blocks, s->dipz.p);
+    k_decide<<<1, 256, 0, st>>>(S, s->dip_part.p, s->skin_blocks, s->dipz.p);
     k_cell_clear<<<64, 256, 0, st>>>(S);
     int lb = std::max(1, (s->n_lead + 255) / 256);
     k_cell_count<<<lb, 256, 0, st>>>(S, s->lead_cell.p);
     k_cell_scan<<<1, 1024, 0, st>>>(s->cell_cnt.p, s->cell_start.p, s->cell_cursor.p, s->flags.p, s->ncells_xy, 1, 0);
     k_cell_fill<<<lb, 256, 0, st>>>(s->lead_cell.p, s->n_lead, s->cell_cursor.p, s->cell_items2.p, s->flags.p, 1);
     k_cell_sort<<<512, 64, 0, st>>>(s->cell_start.p, s->cell_items2.p, s->cell_items.p, s->flags.p, 1, s->ncells_xy, 0);
-    k_build<<<std::max(1, (s->n_lead + 127) / 128), 128, 0, st>>>(S);
+    k_build<<<std::max(1, (s->n_lead + 3) / 4), 256, 0, st>>>(S);  // one wave per lead
     k_snapshot<<<std::max(1, (std::max(s->top.n_rig, s->top.n_move) + 255) / 256), 256, 0, st>>>(S);
 }
 
@@ -375,16 +398,24 @@ static void launch_forces(mdf_system *s) {
     {
         const int G = 16;
         int blocks = std::max(1, (s->n_lead + (256 / G) - 1) / (256 / G));
-        if (f32) k_pair<float, 16><<<blocks, 256, 0, st>>>(S);
-        else k_pair<double, 16><<<blocks, 256, 0, st>>>(S);
+        const bool vir = s->par.virial != 0, rolled = s->pair_rolled;
+#define MDF_PAIR(R, V, RO) k_pair<R, 16, V, RO><<<blocks, 256, 0, st>>>(S)
+        if (f32) {
+            if (vir) { if (rolled) MDF_PAIR(float, true, true); else MDF_PAIR(float, true, false); }
+            else { if (rolled) MDF_PAIR(float, false, true); else MDF_PAIR(float, false, false); }
+        } else {
+            if (vir) { if (rolled) MDF_PAIR(double, true, true); else MDF_PAIR(double, true, false); }
+            else { if (rolled) MDF_PAIR(double, false, true); else MDF_PAIR(double, false, false); }
+        }
+#undef MDF_PAIR
     }
     if (s->ev) (void)hipEventRecord(s->ev[2], st);
-    int nk = (int)s->ak.size(), nch = (int)s->chunks.size(), nrows = (int)s->rows.size();
+    int nk = (int)s->ak.size(), nrows = (int)s->rows.size();
     if (nk > 0 && nm > 0) {
         dim3 g1(s->n_sb, s->n_cg);
-        if (f32) k_sf<float, 4><<<g1, SF_BLOCK, 0, st>>>(S.x, S.y, S.z, S.q, nm, cux, cuy, cuz, s->d_chunks.p, nch,
+        if (f32) k_sf<float, 4><<<g1, SF_BLOCK, 0, st>>>(S.x, S.y, S.z, S.q, nm, cux, cuy, cuz, s->d_rows.p, nrows,
                                                            s->cpb, s->sf_partial.p, nk);
-        else k_sf<double, 4><<<g1, SF_BLOCK, 0, st>>>(S.x, S.y, S.z, S.q, nm, cux, cuy, cuz, s->d_chunks.p, nch,
+        else k_sf<double, 4><<<g1, SF_BLOCK, 0, st>>>(S.x, S.y, S.z, S.q, nm, cux, cuy, cuz, s->d_rows.p, nrows,
                                                         s->cpb, s->sf_partial.p, nk);
     }
     if (s->ev) (void)hipEventRecord(s->ev[3], st);
@@ -396,11 +427,11 @@ static void launch_forces(mdf_system *s) {
     }
     if (s->ev) (void)hipEventRecord(s->ev[4], st);
     if (nk > 0 && nm > 0) {
-        dim3 g2(std::max(1, (nm + 255) / 256), s->n_rg);
-        if (f32) k_bp<float><<<g2, 256, 0, st>>>(S.x, S.y, S.z, S.q, nm, cux, cuy, cuz, s->d_rows.p, nrows, s->rpg,
-                                                 s->d_t.p, pipix, pipiy, pipiz, s->fpart.p);
-        else k_bp<double><<<g2, 256, 0, st>>>(S.x, S.y, S.z, S.q, nm, cux, cuy, cuz, s->d_rows.p, nrows, s->rpg,
-                                              s->d_t.p, pipix, pipiy, pipiz, s->fpart.p);
+        dim3 g2(std::max(1, (nm + 256 * BP_NS - 1) / (256 * BP_NS)), s->n_rg);
+        if (f32) k_bp<float, BP_NS><<<g2, 256, 0, st>>>(S.x, S.y, S.z, S.q, nm, cux, cuy, cuz, s->d_rows.p,
+                                                 s->d_bp_row_start.p, s->d_t.p, pipix, pipiy, pipiz, s->fpart.p);
+        else k_bp<double, BP_NS><<<g2, 256, 0, st>>>(S.x, S.y, S.z, S.q, nm, cux, cuy, cuz, s->d_rows.p,
+                                              s->d_bp_row_start.p, s->d_t.p, pipix, pipiy, pipiz, s->fpart.p);
     }
     if (s->ev) (void)hipEventRecord(s->ev[5], st);
     k_epilogue<<<s->n_epblk, 256, 0, st>>>(S, s->fpart.p, (nk > 0 && nm > 0) ? s->n_rg : 0, s->d_wl.p,
@@ -601,6 +632,8 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
     {
         const char *g = getenv("MDF_GRAPHS");
         if (g && g[0] == '0') s->use_graphs = false;
+        const char *pr = getenv("MDF_PAIR_ROLLED");
+        if (pr && pr[0] == '1') s->pair_rolled = true;
     }
     hipError_t he = hipSetDevice(par->device);
     if (he != hipSuccess) { delete s; return fail(MDF_ERR_HIP, "hipSetDevice: %s", hipGetErrorString(he)); }
@@ -810,6 +843,12 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
                     }
                 }
                 covered += (size_t)(r.kn1 - r.kn0 + 1);
+                if (r.kn1 - r.kn0 + 1 > BP_GROUP_K || (r.kn0 > 1) || (r.kn0 < 0 && r.kn1 < 0)) {
+                    // the back-projection stages a whole row in LDS and walks kn from 0/1 up and -1 down
+                    rc = fail(MDF_ERR_KSET, "k-set row (%d,%d) spans kn %d..%d, unsupported", r.kl, r.km, r.kn0,
+                              r.kn1);
+                    goto fail_out;
+                }
             }
             for (const KChunk &c : s->chunks) ccov += (size_t)c.len;
             if (covered != nk || ccov != nk) {
@@ -822,9 +861,11 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
             int nk = (int)s->ak.size();
             HTRY(upload(s->d_chunks, s->chunks.data(), s->chunks.size()));
             HTRY(upload(s->d_rows, s->rows.data(), s->rows.size()));
+            HTRY(upload(s->d_bp_row_start, s->bp_row_start.data(), s->bp_row_start.size()));
             HTRY(upload(s->d_kvec, s->kvec.data(), s->kvec.size()));
             HTRY(upload(s->d_ak, s->ak.data(), s->ak.size()));
-            HTRY(s->d_s.alloc(std::max(nk, 1))); HTRY(s->d_t.alloc(std::max(nk, 1)));
+            HTRY(s->d_s.alloc(std::max(nk, 1)));
+            HTRY(s->d_t.alloc(nk + 4));   // padded: the back-projection prefetches one entry past a row
             HTRY(s->kpart.alloc((size_t)8 * s->n_kblk));
             std::vector<double2> zero2(std::max(nk, 1), make_double2(0.0, 0.0));
             HTRY(upload(s->d_swall, zero2.data(), zero2.size()));
@@ -838,13 +879,13 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
             if (nw > 0 && nk > 0) {
                 const double cux = 2.0 * s->c.pi / S.lx / M_PI, cuy = 2.0 * s->c.pi / S.ly / M_PI,
                              cuz = 2.0 * s->c.pi / S.lz / M_PI;
-                int nch = (int)s->chunks.size();
-                int ncg = std::max(1, std::min(nch, (2048 + n_sb_w - 1) / n_sb_w));
-                int cpb = (nch + ncg - 1) / ncg;
-                ncg = (nch + cpb - 1) / cpb;
+                int nrow = (int)s->rows.size();
+                int ncg = std::max(1, std::min(nrow, (2048 + n_sb_w - 1) / n_sb_w));
+                int rpb = (nrow + ncg - 1) / ncg;
+                ncg = (nrow + rpb - 1) / rpb;
                 k_sf<double, 4><<<dim3(n_sb_w, ncg), SF_BLOCK, 0, s->stream>>>(
                     S.x + t.n_move, S.y + t.n_move, S.z + t.n_move, S.q + t.n_move, nw, cux, cuy, cuz,
-                    s->d_chunks.p, nch, cpb, s->sf_partial.p, nk);
+                    s->d_rows.p, nrow, rpb, s->sf_partial.p, nk);
                 k_sf_reduce<<<s->n_kblk, 256, 0, s->stream>>>(s->sf_partial.p, n_sb_w, nk, s->d_swall.p, s->d_ak.p,
                                                               s->d_kvec.p, 1.0, 1.0, 1.0, 0.0, s->d_s.p, s->d_t.p,
                                                               s->kpart.p);

@@ -133,15 +133,27 @@ __global__ __launch_bounds__(256) void k_final(DevSys S, FinalArgs A, const doub
         for (int b = 0; b < A.n_colblk; b++) acc += colpart[((size_t)col * A.n_colblk + b) * MAX_LEAD_TYPES + s];
         colsum[col][s] = acc;
     }
-    if (t < 8) {
-        double acc = 0.0;
-        for (int b = 0; b < A.n_kblk; b++) acc += kpart[b * 8 + t];
-        ksum[t] = acc;
-    }
-    if (t < EP_N) {
-        double acc = 0.0;
-        for (int b = 0; b < A.n_epblk; b++) acc += eppart[b * EP_N + t];
-        epsum[t] = acc;
+    // k partials and epilogue partials: 32 lanes per quantity, then a fixed-order shuffle tree
+    {
+        __shared__ double tmp[RE_N + EP_N][32];
+        const int q = t >> 5, l = t & 31;  // 8 quantities per pass of 256 threads
+        for (int pass = 0; pass < 2; pass++) {
+            const int qq = q + 8 * pass;
+            double acc = 0.0;
+            if (qq < RE_N) {
+                for (int b = l; b < A.n_kblk; b += 32) acc += kpart[b * RE_N + qq];
+            } else if (qq < RE_N + EP_N) {
+                for (int b = l; b < A.n_epblk; b += 32) acc += eppart[b * EP_N + (qq - RE_N)];
+            }
+            if (qq < RE_N + EP_N) tmp[qq][l] = acc;
+        }
+        __syncthreads();
+        if (t < RE_N + EP_N) {
+            double acc = 0.0;
+            for (int l2 = 0; l2 < 32; l2++) acc += tmp[t][l2];
+            if (t < RE_N) ksum[t] = acc;
+            else epsum[t - RE_N] = acc;
+        }
     }
     __syncthreads();
     if (t != 0) return;

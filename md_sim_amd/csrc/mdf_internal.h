@@ -26,6 +26,7 @@ constexpr int KC = 16;           // k-vectors per structure-factor chunk held in
 constexpr int SF_SITES = 4;      // sites per thread in the structure-factor kernel
 constexpr int SF_BLOCK = 256;
 constexpr int BP_ROWGROUPS_MAX = 64;
+constexpr int BP_GROUP_K = 1024;     // k-vectors per back-projection row group (its LDS stage)
 
 // k-set rows (contiguous kn) and structure-factor chunks (<= KC consecutive kn of a row)
 struct KChunk { int kl, km, kn0, len, koff; };

@@ -69,8 +69,18 @@ __global__ __launch_bounds__(256) void k_skin(DevSys S, double *dip_part) {
 }
 
 // rebuild decision (src/forcem.f90:250) and z grid of this build; dipole total
-__global__ void k_decide(DevSys S, const double *dip_part, int n_dip, double *dipz_out) {
+__global__ __launch_bounds__(256) void k_decide(DevSys S, const double *dip_part, int n_dip, double *dipz_out) {
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < n_dip; b += blockDim.x) acc += dip_part[b];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    *dipz_out = red[0];
     DevFlags *f = S.flags;
     double aux = __longlong_as_double((long long)f->max_disp2_bits);
     if (__dmul_rn(4.0, aux) > S.skin2) f->rebuild = 1;
@@ -80,9 +90,6 @@ __global__ void k_decide(DevSys S, const double *dip_part, int n_dip, double *di
         f->n_pairs_mm = 0;
         f->n_pairs_mw = 0;
     }
-    double dip = 0.0;
-    for (int b = 0; b < n_dip; b++) dip += dip_part[b];
-    *dipz_out = dip;
     if (S.g.zper) {
         f->ncz = S.g.ncz_fixed;
         f->z0 = 0.0;
@@ -191,87 +198,114 @@ __device__ __forceinline__ bool constrained(const DevSys &S, int i, int j) {
     return false;
 }
 
-// list rows: thread per lead scans its neighbour cells in a fixed order
-__global__ __launch_bounds__(128) void k_build(DevSys S) {
+// list rows: one wave per lead.  The wave walks the lead's neighbour cells in a fixed
+// order, tests 64 candidates at a time and appends the hits with a ballot prefix count,
+// so every row holds its partners in exactly the order a serial scan would give.
+__global__ __launch_bounds__(256) void k_build(DevSys S) {
     const DevFlags *f = S.flags;
     if (!f->rebuild) return;
-    int l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= S.n_lead) return;
+    const int lane = threadIdx.x & 63;
+    const int l = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (l >= S.n_lead) return;   // whole wave leaves together
     const int i = S.lead[l];
     const int ncx = S.g.ncx, ncy = S.g.ncy;
     const int ncz = f->ncz;
-    int cx = wrap_cell(S.x[i], S.lx, S.lxi, ncx);
-    int cy = wrap_cell(S.y[i], S.ly, S.lyi, ncy);
-    int cz = S.g.zper ? wrap_cell(S.z[i], S.lz, S.lzi, ncz) : zcell_lin(S.z[i], f->z0, f->csz, ncz);
-    // neighbour offsets, deduplicated for grids of 1 or 2 cells
-    int nxo = ncx >= 3 ? 3 : ncx, nyo = ncy >= 3 ? 3 : ncy;
-    int nzo = S.g.zper ? (ncz >= 3 ? 3 : ncz) : 3;
-    int cnt = 0, lower = 0;
+    const double xi = S.x[i], yi = S.y[i], zi = S.z[i];
+    const int cx = wrap_cell(xi, S.lx, S.lxi, ncx);
+    const int cy = wrap_cell(yi, S.ly, S.lyi, ncy);
+    const int cz = S.g.zper ? wrap_cell(zi, S.lz, S.lzi, ncz) : zcell_lin(zi, f->z0, f->csz, ncz);
+    const int nxo = ncx >= 3 ? 3 : ncx, nyo = ncy >= 3 ? 3 : ncy;
+    const int nzo = S.g.zper ? (ncz >= 3 ? 3 : ncz) : 3;
     int *row = S.nbr_mm + (size_t)l * S.cap_mm;
     const bool i_surf = i >= S.n_liquid_sites;
+    int cnt = 0, lower = 0;
     for (int oz = 0; oz < nzo; oz++) {
         int z2;
         if (S.g.zper) { z2 = (nzo == 3) ? (cz + oz - 1 + ncz) % ncz : oz; }
         else { z2 = cz + oz - 1; if (z2 < 0 || z2 >= ncz) continue; }
         for (int oy = 0; oy < nyo; oy++) {
-            int y2 = (nyo == 3) ? (cy + oy - 1 + ncy) % ncy : oy;
+            const int y2 = (nyo == 3) ? (cy + oy - 1 + ncy) % ncy : oy;
             for (int ox = 0; ox < nxo; ox++) {
-                int x2 = (nxo == 3) ? (cx + ox - 1 + ncx) % ncx : ox;
-                int c = (z2 * ncy + y2) * ncx + x2;
-                for (int p = S.cell_start[c]; p < S.cell_start[c + 1]; p++) {
-                    int m = S.cell_items[p];
-                    if (m == l) continue;
-                    int j = S.lead[m];
-                    if (i_surf && j >= S.n_liquid_sites && constrained(S, i, j)) continue;
-                    int a = min(i, j), b = max(i, j);
-                    if (list_d2(S, a, b) < S.rskinsq) {
-                        if (cnt < S.cap_mm) row[cnt] = j;
-                        cnt++;
+                const int x2 = (nxo == 3) ? (cx + ox - 1 + ncx) % ncx : ox;
+                const int c = (z2 * ncy + y2) * ncx + x2;
+                const int b = S.cell_start[c], e = S.cell_start[c + 1];
+                for (int p0 = b; p0 < e; p0 += 64) {
+                    const int p = p0 + lane;
+                    bool hit = false;
+                    int j = 0;
+                    if (p < e) {
+                        const int m = S.cell_items[p];
+                        j = S.lead[m];
+                        if (m != l && !(i_surf && j >= S.n_liquid_sites && constrained(S, i, j))) {
+                            const int a = min(i, j), bb = max(i, j);
+                            hit = list_d2(S, a, bb) < S.rskinsq;
+                        }
+                    }
+                    const unsigned long long msk = __ballot(hit);
+                    if (hit) {
+                        const int pos = cnt + __popcll(msk & ((1ull << lane) - 1ull));
+                        if (pos < S.cap_mm) row[pos] = j;
                         lower += (j > i);
                     }
+                    cnt += __popcll(msk);
                 }
             }
         }
     }
-    S.cnt_mm[l] = cnt;
-    if (cnt > S.cap_mm) atomicMax(&S.flags->overflow_mm, cnt);
-    if (lower) atomicAdd(&S.flags->n_pairs_mm, lower);
+    // wave totals
+    for (int o = 32; o > 0; o >>= 1) lower += __shfl_xor(lower, o, 64);
+    if (lane == 0) {
+        S.cnt_mm[l] = cnt;
+        if (cnt > S.cap_mm) atomicMax(&S.flags->overflow_mm, cnt);
+        if (lower) atomicAdd(&S.flags->n_pairs_mm, lower);
+    }
     // frozen partners: same x,y grid, frozen z grid built once
     int cntw = 0;
     if (S.n_wall > 0) {
         int *wrow = S.nbr_mw + (size_t)l * S.cap_mw;
         int wcz;
         if (S.g.zper) {
-            wcz = wrap_cell(S.z[i], S.lz, S.lzi, S.wncz);
+            wcz = wrap_cell(zi, S.lz, S.lzi, S.wncz);
         } else {
-            double czd = floor((S.z[i] - S.wz0) / S.wcsz);
+            double czd = floor((zi - S.wz0) / S.wcsz);
             czd = fmin(fmax(czd, -2.0), (double)S.wncz + 1.0);  // far from the crystal: no cell in reach
             wcz = (int)czd;
         }
-        for (int oz = 0; oz < (S.g.zper ? min(3, S.wncz) : 3); oz++) {
+        const int nwz = S.g.zper ? min(3, S.wncz) : 3;
+        for (int oz = 0; oz < nwz; oz++) {
             int z2;
             if (S.g.zper) z2 = (S.wncz >= 3) ? (wcz + oz - 1 + S.wncz) % S.wncz : oz;
             else { z2 = wcz + oz - 1; if (z2 < 0 || z2 >= S.wncz) continue; }
             for (int oy = 0; oy < nyo; oy++) {
-                int y2 = (nyo == 3) ? (cy + oy - 1 + ncy) % ncy : oy;
+                const int y2 = (nyo == 3) ? (cy + oy - 1 + ncy) % ncy : oy;
                 for (int ox = 0; ox < nxo; ox++) {
-                    int x2 = (nxo == 3) ? (cx + ox - 1 + ncx) % ncx : ox;
-                    int c = (z2 * ncy + y2) * ncx + x2;
-                    for (int p = S.wcell_start[c]; p < S.wcell_start[c + 1]; p++) {
-                        int j = S.n_move + S.wcell_items[p];
-                        if (i_surf && constrained(S, i, j)) continue;
-                        if (list_d2(S, i, j) < S.rskinsq) {
-                            if (cntw < S.cap_mw) wrow[cntw] = j;
-                            cntw++;
+                    const int x2 = (nxo == 3) ? (cx + ox - 1 + ncx) % ncx : ox;
+                    const int c = (z2 * ncy + y2) * ncx + x2;
+                    const int b = S.wcell_start[c], e = S.wcell_start[c + 1];
+                    for (int p0 = b; p0 < e; p0 += 64) {
+                        const int p = p0 + lane;
+                        bool hit = false;
+                        int j = 0;
+                        if (p < e) {
+                            j = S.n_move + S.wcell_items[p];
+                            if (!(i_surf && constrained(S, i, j))) hit = list_d2(S, i, j) < S.rskinsq;
                         }
+                        const unsigned long long msk = __ballot(hit);
+                        if (hit) {
+                            const int pos = cntw + __popcll(msk & ((1ull << lane) - 1ull));
+                            if (pos < S.cap_mw) wrow[pos] = j;
+                        }
+                        cntw += __popcll(msk);
                     }
                 }
             }
         }
     }
-    S.cnt_mw[l] = cntw;
-    if (cntw > S.cap_mw) atomicMax(&S.flags->overflow_mw, cntw);
-    if (cntw) atomicAdd(&S.flags->n_pairs_mw, cntw);
+    if (lane == 0) {
+        S.cnt_mw[l] = cntw;
+        if (cntw > S.cap_mw) atomicMax(&S.flags->overflow_mw, cntw);
+        if (cntw) atomicAdd(&S.flags->n_pairs_mw, cntw);
+    }
 }
 
 // snapshot for the next skin test (src/nnlist.f90:99-119)

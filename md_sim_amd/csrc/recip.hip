@@ -30,98 +30,167 @@ __device__ __forceinline__ void sincospi_r<float>(double t, float &s, float &c) 
     sincospif((float)r, &s, &c);
 }
 
+// Recursion along kn: FP64 uses the Chebyshev form (state e_n, e_{n-1}); FP32 the
+// complex product with exp(i pi w) (state e_n only).  cs = cos(pi w), sn = sin(pi w); dir is
+// +1 going up in kn, -1 going down.
+template <typename R>
+struct KnWalk {
+    R er, ei, pr, pi_;
+    __device__ __forceinline__ void start(R Pr, R Pi, R cs, R sn, int dir) {
+        // dir +1: e_0 = P (kn = 0), e_-1 = P conj(step); dir -1: e_0 = P conj(step) (kn = -1), prev = P
+        R qr = Pr * cs + Pi * sn, qi = Pi * cs - Pr * sn;
+        if (dir > 0) { er = Pr; ei = Pi; pr = qr; pi_ = qi; }
+        else { er = qr; ei = qi; pr = Pr; pi_ = Pi; }
+    }
+    __device__ __forceinline__ void step(R cs, R sn, int dir) {
+        if constexpr (sizeof(R) == 8) {
+            R c2 = R(2) * cs;
+            R nr = c2 * er - pr, ni = c2 * ei - pi_;
+            pr = er; pi_ = ei; er = nr; ei = ni;
+        } else {
+            R s = dir > 0 ? sn : -sn;
+            R nr = er * cs - ei * s, ni = er * s + ei * cs;
+            er = nr; ei = ni;
+        }
+    }
+};
+
+// Row phase P = q exp(i pi (kl u + km v)): one sincospi at a block's first row and when kl
+// changes, otherwise one complex product with exp(i pi v) per km step.
+template <typename R>
+__device__ __forceinline__ void row_phase(bool fresh, int kl, int km, double u, double v, R q, R vc, R vs, R &Pr,
+                                          R &Pi) {
+    if (fresh) {
+        R s0, c0;
+        sincospi_r<R>((double)kl * u + (double)km * v, s0, c0);
+        Pr = q * c0; Pi = q * s0;
+    } else {
+        R nr = Pr * vc - Pi * vs, ni = Pr * vs + Pi * vc;
+        Pr = nr; Pi = ni;
+    }
+}
+
+// the same, reloading the site's coordinates and charge for a fresh start (keeps them out of
+// registers in kernels that hold several sites per thread)
+// out of line on purpose: several inlined copies of sincospi would be interleaved by the
+// scheduler and hold their temporaries at once; a fresh phase is rare (once per kl)
+template <typename R>
+__device__ __attribute__((noinline)) void fresh_phase(int kl, int km, const double *__restrict__ x,
+                                                      const double *__restrict__ y, const double *__restrict__ q,
+                                                      int i, bool ok, double cux, double cuy, R *P) {
+    R s0, c0;
+    const double u = ok ? x[i] * cux : 0.0, v = ok ? y[i] * cuy : 0.0;
+    const R qq = ok ? (R)q[i] : R(0);
+    sincospi_r<R>((double)kl * u + (double)km * v, s0, c0);
+    P[0] = qq * c0;
+    P[1] = qq * s0;
+}
+
+template <typename R>
+__device__ __forceinline__ void row_phase_reload(bool fresh, int kl, int km, const double *__restrict__ x,
+                                                 const double *__restrict__ y, const double *__restrict__ q, int i,
+                                                 bool ok, double cux, double cuy, R vc, R vs, R &Pr, R &Pi) {
+    if (fresh) {
+        R P[2];
+        fresh_phase<R>(kl, km, x, y, q, i, ok, cux, cuy, P);
+        Pr = P[0]; Pi = P[1];
+    } else {
+        R nr = Pr * vc - Pi * vs, ni = Pr * vs + Pi * vc;
+        Pr = nr; Pi = ni;
+    }
+}
+
+constexpr int SF_KC = 8;   // kn per accumulation chunk
+
 template <typename R, int NS>
 __global__ __launch_bounds__(SF_BLOCK) void k_sf(const double *__restrict__ x, const double *__restrict__ y,
                                                  const double *__restrict__ z, const double *__restrict__ q, int n,
                                                  double cux, double cuy, double cuz,
-                                                 const KChunk *__restrict__ chunks, int n_chunks, int cpb,
+                                                 const KRow *__restrict__ rows, int n_rows, int rpb,
                                                  double2 *__restrict__ partial, int nk) {
     constexpr int NW = SF_BLOCK / 64;
-    __shared__ double red[NW][KC][64];   // one half (KC doubles) of the accumulators at a time
+    __shared__ double red[NW][2 * SF_KC][64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int sb = blockIdx.x;
-    // this thread's sites (coalesced: site = sb*SF_BLOCK*NS + m*SF_BLOCK + tid)
-    double su[NS], sv[NS], sw[NS];
-    R sq[NS], stc[NS], sts[NS];
+    double su[NS], sv[NS];
+    R sq[NS], wc[NS], ws[NS], vc[NS], vs[NS], Pr[NS], Pi[NS];
 #pragma unroll
     for (int m = 0; m < NS; m++) {
-        int i = sb * SF_BLOCK * NS + m * SF_BLOCK + threadIdx.x;
-        if (i < n) {
-            su[m] = x[i] * cux; sv[m] = y[i] * cuy; sw[m] = z[i] * cuz; sq[m] = (R)q[i];
-        } else {
-            su[m] = sv[m] = sw[m] = 0.0; sq[m] = R(0);
-        }
-        sincospi_r<R>(sw[m], sts[m], stc[m]);
+        const int i = sb * SF_BLOCK * NS + m * SF_BLOCK + threadIdx.x;
+        double w = 0.0;
+        if (i < n) { su[m] = x[i] * cux; sv[m] = y[i] * cuy; w = z[i] * cuz; sq[m] = (R)q[i]; }
+        else { su[m] = 0.0; sv[m] = 0.0; sq[m] = R(0); }
+        sincospi_r<R>(w, ws[m], wc[m]);
+        sincospi_r<R>(sv[m], vs[m], vc[m]);
     }
-    const int c_begin = blockIdx.y * cpb, c_end = min(n_chunks, c_begin + cpb);
-    for (int cidx = c_begin; cidx < c_end; cidx++) {
-        const KChunk ch = chunks[cidx];
-        R ar[KC], ai[KC];
+    const int r_begin = blockIdx.y * rpb, r_end = min(n_rows, r_begin + rpb);
+    int pkl = -1, pkm = 0;
+    for (int r = r_begin; r < r_end; r++) {
+        const KRow row = rows[r];
+        const bool fresh = (r == r_begin) || row.kl != pkl || row.km != pkm + 1;
+        pkl = row.kl; pkm = row.km;
 #pragma unroll
-        for (int k = 0; k < KC; k++) { ar[k] = R(0); ai[k] = R(0); }
+        for (int m = 0; m < NS; m++) row_phase<R>(fresh, row.kl, row.km, su[m], sv[m], sq[m], vc[m], vs[m], Pr[m], Pi[m]);
+        // two walks from kn = 0: upward over [max(kn0,0), kn1], downward over [kn0, -1]
+        for (int dir = 1; dir >= -1; dir -= 2) {
+            int s0, s1;   // first and last kn of the walk, inclusive, in walk order
+            if (dir > 0) { s0 = max(row.kn0, 0); s1 = row.kn1; }
+            else { s0 = -1; s1 = row.kn0; }
+            if ((dir > 0 && s0 > s1) || (dir < 0 && s0 < s1)) continue;
+            KnWalk<R> st[NS];
 #pragma unroll
-        for (int m = 0; m < NS; m++) {
-            double t0 = (double)ch.kl * su[m] + (double)ch.km * sv[m] + (double)ch.kn0 * sw[m];
-            R s0, c0;
-            sincospi_r<R>(t0, s0, c0);
-            R er = sq[m] * c0, ei = sq[m] * s0;
-            if constexpr (sizeof(R) == 8) {
-                // Chebyshev: e_{n+1} = 2c e_n - e_{n-1}
-                const R c2 = R(2) * stc[m];
-                R pr = er * stc[m] + ei * sts[m];   // e_{-1} = e_0 * conj(step)
-                R pi_ = ei * stc[m] - er * sts[m];
+            for (int m = 0; m < NS; m++) {
+                st[m].start(Pr[m], Pi[m], wc[m], ws[m], dir);
+                if (dir > 0 && s0 == 1) st[m].step(wc[m], ws[m], dir);   // the (0,0) row starts at kn = 1
+            }
+            const int total = dir > 0 ? s1 - s0 + 1 : s0 - s1 + 1;
+            for (int c = 0; c < total; c += SF_KC) {
+                const int len = min(SF_KC, total - c);
+                R ar[SF_KC], ai[SF_KC];
 #pragma unroll
-                for (int k = 0; k < KC; k++) {
-                    ar[k] += er; ai[k] += ei;
-                    R nr = c2 * er - pr, ni = c2 * ei - pi_;
-                    pr = er; pi_ = ei; er = nr; ei = ni;
+                for (int k = 0; k < SF_KC; k++) { ar[k] = R(0); ai[k] = R(0); }
+#pragma unroll
+                for (int m = 0; m < NS; m++) {
+#pragma unroll
+                    for (int k = 0; k < SF_KC; k++) {
+                        ar[k] += st[m].er; ai[k] += st[m].ei;
+                        st[m].step(wc[m], ws[m], dir);
+                    }
                 }
-            } else {
+                // block reduction through LDS: thread t sums value t/16 over lanes (t%16)*4..+3
+                // of every wave, then a 16-lane shuffle tree
 #pragma unroll
-                for (int k = 0; k < KC; k++) {
-                    ar[k] += er; ai[k] += ei;
-                    R nr = er * stc[m] - ei * sts[m];
-                    R ni = er * sts[m] + ei * stc[m];
-                    er = nr; ei = ni;
+                for (int k = 0; k < SF_KC; k++) {
+                    red[wave][2 * k][lane] = (double)ar[k];
+                    red[wave][2 * k + 1][lane] = (double)ai[k];
                 }
+                __syncthreads();
+                const int vidx = threadIdx.x >> 4, p = threadIdx.x & 15;
+                double sum = 0.0;
+#pragma unroll
+                for (int wv = 0; wv < NW; wv++)
+#pragma unroll
+                    for (int l = 0; l < 4; l++) sum += red[wv][vidx][p * 4 + l];
+                sum += __shfl_xor(sum, 1, 64);
+                sum += __shfl_xor(sum, 2, 64);
+                sum += __shfl_xor(sum, 4, 64);
+                sum += __shfl_xor(sum, 8, 64);
+                const int k = vidx >> 1;
+                if (p == 0 && k < len) {
+                    const int kn = dir > 0 ? s0 + c + k : s0 - c - k;
+                    double *dst = reinterpret_cast<double *>(&partial[(size_t)sb * nk + row.koff + (kn - row.kn0)]);
+                    dst[vidx & 1] = sum;
+                }
+                __syncthreads();
             }
-        }
-        // block reduction: real parts then imaginary parts through LDS
-#pragma unroll
-        for (int half = 0; half < 2; half++) {
-#pragma unroll
-            for (int k = 0; k < KC; k++) red[wave][k][lane] = (double)(half == 0 ? ar[k] : ai[k]);
-            __syncthreads();
-            // thread t: value k = t / 16, slice p = t % 16 of the 64*NW partial sums
-            const int k = threadIdx.x >> 4, p = threadIdx.x & 15;
-            double s = 0.0;
-            if (k < KC) {
-#pragma unroll
-                for (int w = 0; w < NW; w++)
-#pragma unroll
-                    for (int l = 0; l < 4; l++) s += red[w][k][p * 4 + l];
-            }
-            s += __shfl_xor(s, 1, 64);
-            s += __shfl_xor(s, 2, 64);
-            s += __shfl_xor(s, 4, 64);
-            s += __shfl_xor(s, 8, 64);
-            if (p == 0 && k < ch.len) {
-                double *dst = reinterpret_cast<double *>(&partial[(size_t)sb * nk + ch.koff + k]);
-                dst[half] = s;
-            }
-            __syncthreads();
         }
     }
 }
 
 template __global__ void k_sf<double, 4>(const double *, const double *, const double *, const double *, int, double,
-                                        double, double, const KChunk *, int, int, double2 *, int);
+                                        double, double, const KRow *, int, int, double2 *, int);
 template __global__ void k_sf<float, 4>(const double *, const double *, const double *, const double *, int, double,
-                                       double, double, const KChunk *, int, int, double2 *, int);
-template __global__ void k_sf<double, 8>(const double *, const double *, const double *, const double *, int, double,
-                                        double, double, const KChunk *, int, int, double2 *, int);
-template __global__ void k_sf<float, 8>(const double *, const double *, const double *, const double *, int, double,
-                                       double, double, const KChunk *, int, int, double2 *, int);
+                                       double, double, const KRow *, int, int, double2 *, int);
 
 // S(k) = sum over site blocks + static frozen part; T(k) = A_k conj(S(k)); per-block
 // partial energies and virial:  eK = A|S|^2, eLL = A|S_moving|^2, vir_ab = 1/2 eK (d_ab - g k_a k_b)
@@ -129,7 +198,7 @@ __global__ __launch_bounds__(256) void k_sf_reduce(const double2 *__restrict__ p
                                                    const double2 *__restrict__ s_wall, const double *__restrict__ ak,
                                                    const int *__restrict__ kvec, double pipix, double pipiy,
                                                    double pipiz, double lamb, double2 *__restrict__ s_out,
-                                                   double2 *__restrict__ t_out, double *__restrict__ epart) {
+                                                   double4 *__restrict__ t_out, double *__restrict__ epart) {
     __shared__ double red[RE_N][256];
     double acc[RE_N];
 #pragma unroll
@@ -145,7 +214,8 @@ __global__ __launch_bounds__(256) void k_sf_reduce(const double2 *__restrict__ p
         double qr = sr + w.x, qi = si + w.y;
         s_out[k] = make_double2(qr, qi);
         double a = ak[k];
-        t_out[k] = make_double2(a * qr, -a * qi);
+        const double kn = (double)kvec[3 * k + 2];
+        t_out[k] = make_double4(a * qr, -a * qi, kn * (a * qr), kn * (-a * qi));
         double ek = a * (qr * qr + qi * qi);
         acc[0] = ek;
         acc[1] = a * (sr * sr + si * si);
@@ -171,73 +241,108 @@ __global__ __launch_bounds__(256) void k_sf_reduce(const double2 *__restrict__ p
     if (threadIdx.x < RE_N) epart[blockIdx.x * RE_N + threadIdx.x] = red[threadIdx.x][0];
 }
 
+// T4[k] = (A conj S, kn A conj S): per k-vector the force pieces are Im(T e) and kn Im(T e),
+// each two FMAs on the separated real and imaginary products
 template <typename R>
+__device__ __forceinline__ void bp_acc(const double4 &t, R er, R ei, R &a1, R &a2, R &z1, R &z2) {
+    a1 += (R)t.x * ei;
+    a2 += (R)t.y * er;
+    z1 += (R)t.z * ei;
+    z2 += (R)t.w * er;
+}
+
+// Rows are split into groups of similar k count (host-balanced, row_start[g]..row_start[g+1]);
+// a block stages its group's k data (at most BP_LDS_K entries) in LDS once and every wave
+// reads it with uniform-address LDS loads, instead of scalar loads that miss the scalar cache.
+constexpr int BP_LDS_K = 1024;
+
+template <typename R, int NS>
 __global__ __launch_bounds__(256) void k_bp(const double *__restrict__ x, const double *__restrict__ y,
                                             const double *__restrict__ z, const double *__restrict__ q, int n,
                                             double cux, double cuy, double cuz, const KRow *__restrict__ rows,
-                                            int n_rows, int rpg, const double2 *__restrict__ T, double pipix,
-                                            double pipiy, double pipiz, double *__restrict__ fpart) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+                                            const int *__restrict__ row_start, const double4 *__restrict__ T,
+                                            double pipix, double pipiy, double pipiz, double *__restrict__ fpart) {
+    // NS sites per thread: every k datum read from LDS serves NS sites
+    __shared__ double4 sT[BP_LDS_K + 2];
     const int g = blockIdx.y;
-    const bool ok = i < n;
-    const int ii = ok ? i : 0;
-    const double u = x[ii] * cux, v = y[ii] * cuy, w = z[ii] * cuz;
-    const R qq = ok ? (R)q[ii] : R(0);
-    R sts, stc;
-    sincospi_r<R>(w, sts, stc);
-    double fx = 0.0, fy = 0.0, fz = 0.0;
-    const int r_begin = g * rpg, r_end = min(n_rows, r_begin + rpg);
+    const int r_begin = row_start[g], r_end = row_start[g + 1];
+    const int k_begin = rows[r_begin].koff;
+    const int k_count = rows[r_end - 1].koff + (rows[r_end - 1].kn1 - rows[r_end - 1].kn0 + 1) - k_begin;
+    for (int t = threadIdx.x; t < k_count + 2; t += blockDim.x)
+        sT[t] = T[k_begin + t];   // T is padded past its last entry
+    __syncthreads();
+    int idx[NS];
+    R wc[NS], ws[NS], vc[NS], vs[NS], Pr[NS], Pi[NS];
+    double fx[NS], fy[NS], fz[NS];
+#pragma unroll
+    for (int m = 0; m < NS; m++) {
+        idx[m] = blockIdx.x * blockDim.x * NS + m * blockDim.x + threadIdx.x;
+        const bool ok = idx[m] < n;
+        const double w = ok ? z[idx[m]] * cuz : 0.0, v = ok ? y[idx[m]] * cuy : 0.0;
+        sincospi_r<R>(w, ws[m], wc[m]);
+        sincospi_r<R>(v, vs[m], vc[m]);
+        Pr[m] = R(0); Pi[m] = R(0);
+        fx[m] = 0.0; fy[m] = 0.0; fz[m] = 0.0;
+    }
+    int pkl = -1, pkm = 0;
     for (int r = r_begin; r < r_end; r++) {
         const KRow row = rows[r];
-        double t0 = (double)row.kl * u + (double)row.km * v + (double)row.kn0 * w;
-        R s0, c0;
-        sincospi_r<R>(t0, s0, c0);
-        R er = qq * c0, ei = qq * s0;
-        R acc = R(0), accz = R(0);
-        const int len = row.kn1 - row.kn0 + 1;
-        const double2 *Tr = T + row.koff;
-        if constexpr (sizeof(R) == 8) {
-            const R c2 = R(2) * stc;
-            R pr = er * stc + ei * sts, pi_ = ei * stc - er * sts;
-            R kn = (R)row.kn0;
-            for (int k = 0; k < len; k++) {
-                const double2 t = Tr[k];
-                R tt = t.x * ei + t.y * er;
-                acc += tt;
-                accz += kn * tt;
-                kn += R(1);
-                R nr = c2 * er - pr, ni = c2 * ei - pi_;
-                pr = er; pi_ = ei; er = nr; ei = ni;
+        const bool fresh = (r == r_begin) || row.kl != pkl || row.km != pkm + 1;
+        pkl = row.kl; pkm = row.km;
+        R acc[NS], accz[NS];
+#pragma unroll
+        for (int m = 0; m < NS; m++) {
+            row_phase_reload<R>(fresh, row.kl, row.km, x, y, q, idx[m], idx[m] < n, cux, cuy, vc[m], vs[m], Pr[m],
+                                Pi[m]);
+            acc[m] = R(0); accz[m] = R(0);
+        }
+        for (int dir = 1; dir >= -1; dir -= 2) {
+            int s0, s1;
+            if (dir > 0) { s0 = max(row.kn0, 0); s1 = row.kn1; }
+            else { s0 = -1; s1 = row.kn0; }
+            if ((dir > 0 && s0 > s1) || (dir < 0 && s0 < s1)) continue;
+            KnWalk<R> st[NS];
+#pragma unroll
+            for (int m = 0; m < NS; m++) {
+                st[m].start(Pr[m], Pi[m], wc[m], ws[m], dir);
+                if (dir > 0 && s0 == 1) st[m].step(wc[m], ws[m], dir);
             }
-        } else {
-            R kn = (R)row.kn0;
-            for (int k = 0; k < len; k++) {
-                const double2 t = Tr[k];
-                R tt = (R)t.x * ei + (R)t.y * er;
-                acc += tt;
-                accz += kn * tt;
-                kn += R(1);
-                R nr = er * stc - ei * sts;
-                R ni = er * sts + ei * stc;
-                er = nr; ei = ni;
+            const int total = dir > 0 ? s1 - s0 + 1 : s0 - s1 + 1;
+            const double4 *Tr = sT + (row.koff - k_begin) + (s0 - row.kn0);
+            for (int j = 0; j < total; j++) {
+                const double4 tv = Tr[dir * j];
+#pragma unroll
+                for (int m = 0; m < NS; m++) {
+                    acc[m] += (R)tv.x * st[m].ei;
+                    acc[m] += (R)tv.y * st[m].er;
+                    accz[m] += (R)tv.z * st[m].ei;
+                    accz[m] += (R)tv.w * st[m].er;
+                    st[m].step(wc[m], ws[m], dir);
+                }
             }
         }
-        fx += (double)acc * (pipix * (double)row.kl);
-        fy += (double)acc * (pipiy * (double)row.km);
-        fz += (double)accz * pipiz;
+#pragma unroll
+        for (int m = 0; m < NS; m++) {
+            fx[m] += (double)acc[m] * (pipix * (double)row.kl);
+            fy[m] += (double)acc[m] * (pipiy * (double)row.km);
+            fz[m] += (double)accz[m] * pipiz;
+        }
     }
-    if (ok) {
-        fpart[((size_t)g * 3 + 0) * n + i] = fx;
-        fpart[((size_t)g * 3 + 1) * n + i] = fy;
-        fpart[((size_t)g * 3 + 2) * n + i] = fz;
+#pragma unroll
+    for (int m = 0; m < NS; m++) {
+        if (idx[m] < n) {
+            fpart[((size_t)g * 3 + 0) * n + idx[m]] = fx[m];
+            fpart[((size_t)g * 3 + 1) * n + idx[m]] = fy[m];
+            fpart[((size_t)g * 3 + 2) * n + idx[m]] = fz[m];
+        }
     }
 }
 
-template __global__ void k_bp<double>(const double *, const double *, const double *, const double *, int, double,
-                                      double, double, const KRow *, int, int, const double2 *, double, double, double,
-                                      double *);
-template __global__ void k_bp<float>(const double *, const double *, const double *, const double *, int, double,
-                                     double, double, const KRow *, int, int, const double2 *, double, double, double,
-                                     double *);
+template __global__ void k_bp<double, 4>(const double *, const double *, const double *, const double *, int, double,
+                                         double, double, const KRow *, const int *, const double4 *, double, double,
+                                         double, double *);
+template __global__ void k_bp<float, 4>(const double *, const double *, const double *, const double *, int, double,
+                                        double, double, const KRow *, const int *, const double4 *, double, double,
+                                        double, double *);
 
 }  // namespace mdf

@@ -12,7 +12,7 @@ program mdf_force_once
     type(mdf_params) :: par
     type(mdf_outputs) :: out
     type(c_ptr) :: sys
-    integer(c_int) :: hdr(8), ipar(10), nss(2), nsteps, rc
+    integer(c_int) :: hdr(8), ipar(11), nss(2), nsteps, rc
     integer(c_int), allocatable, target :: itype(:), nsi(:), ijtype(:), vkind(:)
     real(c_double), allocatable, target :: q(:), eps4(:), sig6(:), aa(:), rhoi(:), cc(:)
     real(c_double), allocatable, target :: x(:), y(:), z(:), cx(:), cy(:), cz(:)
@@ -49,7 +49,7 @@ program mdf_force_once
     par%box = dpar(1:3); par%rcut = dpar(4); par%skin = dpar(5); par%alpha = dpar(6)
     par%alpha_mode = ipar(1); par%kmax = ipar(2); par%z_mode = ipar(3); par%ew3dc = ipar(4)
     par%wall_lr = ipar(5); par%precision = ipar(6); par%const_mode = ipar(7)
-    par%erfc_mode = ipar(8); par%max_neighbors = ipar(9); par%device = ipar(10)
+    par%erfc_mode = ipar(8); par%max_neighbors = ipar(9); par%device = ipar(10); par%virial = ipar(11)
 
     rc = mdf_create(top, par, c_loc(x), c_loc(y), c_loc(z), c_loc(cx), c_loc(cy), c_loc(cz), sys)
     if (rc /= MDF_OK) then

@@ -35,7 +35,7 @@ module mdforce_c
         real(c_double) :: box(3)
         real(c_double) :: rcut, skin, alpha
         integer(c_int) :: alpha_mode, kmax, z_mode, ew3dc, wall_lr, precision, const_mode, &
-                          erfc_mode, max_neighbors, device
+                          erfc_mode, max_neighbors, device, virial
     end type mdf_params
 
     type, bind(C) :: mdf_outputs

@@ -23,7 +23,7 @@ def write_sysfile(system, path, positions=None, nsteps=0, dt=None, **abi_kw):
         w(list(t.wall_nss), np.int32)
         w([*p.box, p.rcut, p.skin, p.alpha], np.float64)
         w([p.alpha_mode, p.kmax, p.z_mode, p.ew3dc, p.wall_lr, p.precision, p.const_mode, p.erfc_mode,
-           p.max_neighbors, p.device], np.int32)
+           p.max_neighbors, p.device, p.virial], np.int32)
         for a in (x, y, z, cx, cy, cz):
             w(a, np.float64)
         w([nsteps], np.int32)
