@@ -77,11 +77,19 @@ def build_abi(system, precision: int = FP64, const_mode: int = CONST_REFERENCE,
               erfc_mode: int = ERFC_AS7126, alpha: Optional[float] = None,
               z_mode: Optional[int] = None, ew3dc: Optional[bool] = None,
               wall_lr: Optional[bool] = None, max_neighbors: int = 0, device: int = 0,
-              virial: bool = True) -> SystemArrays:
+              virial: bool = True, constraints=None, angles=None,
+              n_liquid_sites: Optional[int] = None) -> SystemArrays:
     """Topology and parameters of a ``systems.System``.  Defaults follow the reference
     variant the system corresponds to: slab systems use src/forcem.f90 (open z, slab
     correction and wall tail on); bulk systems use src/forcem_back.f90 (periodic z in the
-    pair loops over the reference's open-z list, no slab or wall terms)."""
+    pair loops over the reference's open-z list, no slab or wall terms).
+
+    ``constraints`` = (site_a, site_b, length) and ``angles`` = (i, j, k, kt, theta) add the
+    reference's surface-group terms (0-based sites): constrained surface pairs are left out
+    of the pair list and enter the self energy (src/nnlist.f90:41-43,68-70;
+    src/forcem.f90:808-819); angles add k(theta - theta0) bending forces on j and k
+    (src/forcem.f90:565-614).  ``n_liquid_sites`` moves the liquid/surface boundary
+    (``nliqa``): moving sites from that index on are surface sites."""
     s = system
     ijtype, kind, eps4, sig6, A, rhoi, Cc = s.vdw_tables()
     keep = dict(site_type=np.ascontiguousarray(s.site_types(), np.int32),
@@ -99,8 +107,25 @@ def build_abi(system, precision: int = FP64, const_mode: int = CONST_REFERENCE,
     top.vdw_kind = _p(keep["vdw_kind"], C.c_int32)
     for k in ("vdw_eps4", "vdw_sig6", "vdw_a", "vdw_rhoi", "vdw_c"):
         setattr(top, k, _p(keep[k], C.c_double))
+    if n_liquid_sites is not None:
+        top.n_liquid_sites = int(n_liquid_sites)
     top.n_constraints = 0
+    if constraints is not None:
+        ca, cb, cl = constraints
+        keep["con_a"] = np.ascontiguousarray(ca, np.int32)
+        keep["con_b"] = np.ascontiguousarray(cb, np.int32)
+        keep["con_len"] = np.ascontiguousarray(cl, np.float64)
+        top.n_constraints = len(keep["con_a"])
+        top.con_a, top.con_b = _p(keep["con_a"], C.c_int32), _p(keep["con_b"], C.c_int32)
+        top.con_len = _p(keep["con_len"], C.c_double)
     top.n_angles = 0
+    if angles is not None:
+        ai, aj, ak, kt, th = angles
+        for k, v, t in (("ang_i", ai, np.int32), ("ang_j", aj, np.int32), ("ang_k", ak, np.int32),
+                        ("ang_kt", kt, np.float64), ("ang_theta", th, np.float64)):
+            keep[k] = np.ascontiguousarray(v, t)
+            setattr(top, k, _p(keep[k], C.c_int32 if t is np.int32 else C.c_double))
+        top.n_angles = len(keep["ang_i"])
     cr = s.crystal
     top.wall_lw = cr.lw
     top.wall_cell[:] = list(cr.cell)

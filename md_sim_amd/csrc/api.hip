@@ -153,8 +153,8 @@ struct mdf_system {
     WallLR wl;
     bool fixed_cap = false;
     // device buffers
-    DBuf<int> itype, nsi, ijtype, vkind, cls, lead, lead_slot, con_ptr, con_idx;
-    DBuf<double> q, eps4, sig6, aa, rhoi, cc;
+    DBuf<int> itype, nsi, ijtype, vkind, cls, lead, lead_slot, con_ptr, con_idx, ang_ptr, ang_inc, ang_ijk;
+    DBuf<double> q, eps4, sig6, aa, rhoi, cc, ang_kt, ang_th;
     DBuf<double> x, y, z, cx, cy, cz, ocx, ocy, ocz, ox, oy, oz, fx, fy, fz, part;
     DBuf<int> nbr_mm, cnt_mm, nbr_mw, cnt_mw;
     DBuf<int> cell_cnt, cell_start, cell_items, cell_items2, cell_cursor, lead_cell;
@@ -199,14 +199,14 @@ struct mdf_system {
 
 static void free_all(mdf_system *s) {
     DBuf<int> *ib[] = {&s->itype, &s->nsi, &s->ijtype, &s->vkind, &s->cls, &s->lead, &s->lead_slot, &s->con_ptr,
-                       &s->con_idx, &s->nbr_mm, &s->cnt_mm, &s->nbr_mw, &s->cnt_mw, &s->cell_cnt, &s->cell_start,
+                       &s->con_idx, &s->ang_ptr, &s->ang_inc, &s->ang_ijk, &s->nbr_mm, &s->cnt_mm, &s->nbr_mw, &s->cnt_mw, &s->cell_cnt, &s->cell_start,
                        &s->cell_items, &s->cell_items2, &s->cell_cursor, &s->lead_cell, &s->wcell_start,
                        &s->wcell_items, &s->d_kvec};
     for (auto *b : ib) b->release();
     DBuf<double> *db[] = {&s->q, &s->eps4, &s->sig6, &s->aa, &s->rhoi, &s->cc, &s->x, &s->y, &s->z, &s->cx,
                           &s->cy, &s->cz, &s->ocx, &s->ocy, &s->ocz, &s->ox, &s->oy, &s->oz, &s->fx, &s->fy,
                           &s->fz, &s->part, &s->dip_part, &s->dipz, &s->d_ak, &s->kpart, &s->fpart, &s->eppart,
-                          &s->colpart, &s->fcom, &s->torque, &s->e_vdw, &s->e_coul};
+                          &s->colpart, &s->fcom, &s->torque, &s->e_vdw, &s->e_coul, &s->ang_kt, &s->ang_th};
     for (auto *b : db) b->release();
     s->flags.release(); s->d_bp_row_start.release(); s->d_chunks.release(); s->d_rows.release(); s->d_swall.release(); s->d_s.release();
     s->d_t.release(); s->sf_partial.release(); s->d_wl.release(); s->d_scal.release();
@@ -602,6 +602,10 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
         t.n_rigid_sites > t.n_liquid_sites)
         return fail(MDF_ERR_ARG, "inconsistent site counts");
     if (t.n_rig > 0 && (!cx || !cy || !cz)) return fail(MDF_ERR_ARG, "COM arrays required");
+    if (t.n_constraints < 0 || (t.n_constraints > 0 && (!t.con_a || !t.con_b || !t.con_len)))
+        return fail(MDF_ERR_ARG, "constraint arrays required");
+    if (t.n_angles < 0 || (t.n_angles > 0 && (!t.ang_i || !t.ang_j || !t.ang_k || !t.ang_kt || !t.ang_theta)))
+        return fail(MDF_ERR_ARG, "angle arrays required");
     if (par->rcut <= 0 || par->skin < 0 || par->kmax < 0) return fail(MDF_ERR_ARG, "bad cut-off/skin/kmax");
     if (par->box[0] <= 0 || par->box[1] <= 0 || par->box[2] <= 0) return fail(MDF_ERR_ARG, "bad box");
     int dev_count = 0;
@@ -721,6 +725,36 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
         HTRY(upload(s->lead_slot, s->h_lead_slot.data(), s->h_lead_slot.size()));
         HTRY(upload(s->con_ptr, cptr.data(), cptr.size()));
         HTRY(upload(s->con_idx, cidx.data(), cidx.size()));
+        // angle terms: the reference adds each angle's force to its sites j and k only
+        // (src/forcem.f90:603-614); a CSR per moving site lets one thread per site sum its
+        // own terms without atomics
+        {
+            std::vector<int> aptr(t.n_move + 1, 0), ainc, aijk(3 * (size_t)std::max(t.n_angles, 0));
+            std::vector<std::vector<int>> inc(t.n_move);
+            for (int a = 0; a < t.n_angles; a++) {
+                const int ii = t.ang_i[a], jj = t.ang_j[a], kk = t.ang_k[a];
+                if (ii < 0 || jj < 0 || kk < 0 || ii >= t.n_total || jj >= t.n_total || kk >= t.n_total ||
+                    !std::isfinite(t.ang_kt[a]) || !std::isfinite(t.ang_theta[a])) {
+                    rc = fail(MDF_ERR_ARG, "angle %d invalid", a);
+                    goto fail_out;
+                }
+                aijk[3 * a] = ii; aijk[3 * a + 1] = jj; aijk[3 * a + 2] = kk;
+                if (jj < t.n_move) inc[jj].push_back(2 * a);
+                if (kk < t.n_move) inc[kk].push_back(2 * a + 1);
+            }
+            for (int i = 0; i < t.n_move; i++) {
+                aptr[i + 1] = aptr[i] + (int)inc[i].size();
+                ainc.insert(ainc.end(), inc[i].begin(), inc[i].end());
+            }
+            HTRY(upload(s->ang_ptr, aptr.data(), aptr.size()));
+            HTRY(upload(s->ang_inc, ainc.data(), ainc.size()));
+            HTRY(upload(s->ang_ijk, aijk.data(), aijk.size()));
+            HTRY(upload(s->ang_kt, t.ang_kt, (size_t)std::max(t.n_angles, 0)));
+            HTRY(upload(s->ang_th, t.ang_theta, (size_t)std::max(t.n_angles, 0)));
+            S.n_angles = std::max(t.n_angles, 0);
+            S.ang_ptr = s->ang_ptr.p; S.ang_inc = s->ang_inc.p; S.ang_ijk = s->ang_ijk.p;
+            S.ang_kt = s->ang_kt.p; S.ang_th = s->ang_th.p;
+        }
         S.itype = s->itype.p; S.q = s->q.p; S.nsi = s->nsi.p; S.ijtype = s->ijtype.p; S.vkind = s->vkind.p;
         S.eps4 = s->eps4.p; S.sig6 = s->sig6.p; S.aa = s->aa.p; S.rhoi = s->rhoi.p; S.cc = s->cc.p; S.cls = s->cls.p;
         S.lead = s->lead.p; S.con_ptr = s->con_ptr.p; S.con_idx = s->con_idx.p;

@@ -2,7 +2,8 @@
 //
 //  k_epilogue: site force = pair part + reciprocal part (sum over row groups) + wall
 //              long-range tail (src/forcem.f90:509-543) + slab correction
-//              (src/forcem.f90:923-928); per-block partials of ulr, the wall z force it
+//              (src/forcem.f90:923-928) + angle bending (src/forcem.f90:565-614);
+//              per-block partials of ulr, the wall z force it
 //              adds, and the total force (fws(0), src/forcem.f90:990-992).
 //  k_com:      rigid-body COM force and torque per molecule (src/forcem.f90:1003-1024).
 //  k_colsum:   per-lead partial columns -> per-block, per-lead-type sums.
@@ -11,6 +12,36 @@
 #include "mdf_internal.h"
 
 namespace mdf {
+
+// Harmonic angle bending of the force_intra term (src/forcem.f90:565-614): a = r_k - r_j,
+// b = r_j - r_i (x,y minimum image; z too in the 3-D periodic form), cos = -a.b/|a||b|;
+// the force of k(theta - theta0) acts on the end site k and the vertex site j, never on i.
+// role 0 adds the vertex part, role 1 the end-site part.
+__device__ void angle_force(const DevSys &S, int a, int role, double &fx, double &fy, double &fz) {
+    const int i = S.ang_ijk[3 * a], j = S.ang_ijk[3 * a + 1], k = S.ang_ijk[3 * a + 2];
+    double ax = S.x[k] - S.x[j]; ax = ax - S.lx * anint_d(ax * S.lxi);
+    double ay = S.y[k] - S.y[j]; ay = ay - S.ly * anint_d(ay * S.lyi);
+    double az = S.z[k] - S.z[j];
+    double bx = S.x[j] - S.x[i]; bx = bx - S.lx * anint_d(bx * S.lxi);
+    double by = S.y[j] - S.y[i]; by = by - S.ly * anint_d(by * S.lyi);
+    double bz = S.z[j] - S.z[i];
+    if (S.zper_force) {
+        az = az - S.lz * anint_d(az * S.lzi);
+        bz = bz - S.lz * anint_d(bz * S.lzi);
+    }
+    const double ai = 1.0 / sqrt(ax * ax + ay * ay + az * az);
+    const double bi = 1.0 / sqrt(bx * bx + by * by + bz * bz);
+    const double ab = ax * bx + ay * by + az * bz;
+    const double cang = -ab * ai * bi;
+    const double fij = S.ang_kt[a] * (acos(cang) - S.ang_th[a]) / sqrt(1.0 - cang * cang) * ai * bi;
+    const double bb = ab * bi * bi, aa = ab * ai * ai;
+    const double tx = aa * ax - bx, ty = aa * ay - by, tz = aa * az - bz;
+    if (role) {
+        fx += fij * tx; fy += fij * ty; fz += fij * tz;
+    } else {
+        fx += fij * (bb * bx - ax - tx); fy += fij * (bb * by - ay - ty); fz += fij * (bb * bz - az - tz);
+    }
+}
 
 __global__ __launch_bounds__(256) void k_epilogue(DevSys S, const double *__restrict__ fpart, int n_groups,
                                                   const WallLR *__restrict__ wl, int ew3dc, double crf0,
@@ -58,6 +89,11 @@ __global__ __launch_bounds__(256) void k_epilogue(DevSys S, const double *__rest
             const double crf = 2.0 * crf0 * S.facq * *dipz;
             fz += crf * S.q[i];
         }
+        if (S.n_angles > 0)
+            for (int p = S.ang_ptr[i]; p < S.ang_ptr[i + 1]; p++) {
+                const int e = S.ang_inc[p];
+                angle_force(S, e >> 1, e & 1, fx, fy, fz);
+            }
         S.fx[i] = fx; S.fy[i] = fy; S.fz[i] = fz;
         acc[3] = fx; acc[4] = fy; acc[5] = fz;
     }

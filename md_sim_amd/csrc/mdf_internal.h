@@ -101,6 +101,12 @@ struct DevSys {
     const int *lead;       // [n_lead] site index
     const int *con_ptr;    // [n_move+1] CSR of constraint partners (surface sites)
     const int *con_idx;
+    // angle bending (force_intra): per moving site the angles whose force moves it
+    int n_angles;
+    const int *ang_ptr;    // [n_move+1] CSR over moving sites
+    const int *ang_inc;    // angle * 2 + role (0: vertex site j, 1: end site k)
+    const int *ang_ijk;    // [3 * n_angles] sites i, j, k
+    const double *ang_kt, *ang_th;  // [n_angles] force constant, equilibrium angle (rad)
     // state
     double *x, *y, *z;     // [n_total]
     const double *cx, *cy, *cz;  // [n_rig]

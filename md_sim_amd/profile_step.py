@@ -1,0 +1,52 @@
+"""A short, fixed run of the force step for hardware-counter profiling.
+
+``python -m md_sim_amd.profile_step [--workload slab16k] [--precision fp64] [--steps 5]``
+runs ``--steps`` force evaluations on device-resident positions with direct kernel launches
+(no graph replay, so a profiler attributes counters to each dispatch), then one plain
+device-to-device copy of ``--copy-mib`` MiB.  The copy is the calibration for byte
+counters: it fetches and writes exactly that many bytes, so the ratio of its measured
+FETCH_SIZE / WRITE_SIZE to the true size scales the counters of the engine's kernels.
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import sys
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="slab16k")
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--copy-mib", type=int, default=256)
+    a = ap.parse_args(argv)
+    os.environ["MDF_GRAPHS"] = "0"
+    import torch
+
+    from . import abi, systems
+    from .engine import ForceEngine
+
+    s = systems.config(a.workload)
+    prec = abi.FP32_ACC64 if a.precision == "fp32" else abi.FP64
+    pos = abi.positions(s)
+    eng = ForceEngine(s, precision=prec, virial=False)
+    dev = [torch.tensor(v[:s.nmove] if k < 3 else v, dtype=torch.float64, device="cuda:0")
+           for k, v in enumerate(pos)]
+    for _ in range(a.steps):
+        eng.compute_device(*[t.data_ptr() for t in dev])
+    r = eng.read_outputs()
+    print("steps done: e_conf %.6f" % r["e_conf"], flush=True)
+    n = a.copy_mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print("copy done: %d MiB" % a.copy_mib, flush=True)
+    eng.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

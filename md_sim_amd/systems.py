@@ -411,6 +411,34 @@ def water_on_slab(n_water: int, ncell: int, nlayers: int, seed: int, rcut: float
     return sysm
 
 
+def rocksalt_bulk(ncell: int = 4, a: float = 5.64, rcut: float = 11.0, kmax: int = 10,
+                  shake: float = 0.0, seed: int = 1200) -> System:
+    """A periodic rock-salt crystal of mobile +-1 e ions (ncell^3 conventional cells, no
+    water, no van der Waals), 3-D periodic: its Coulomb energy per ion pair is the
+    Madelung constant times -facq / (a/2).  ``shake`` displaces every ion by a seeded
+    uniform amount in [-shake, shake] per axis."""
+    rng = np.random.default_rng(seed)
+    h = 0.5 * a
+    n = 2 * ncell
+    ii, jj, kk = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    pos = np.stack([(ii + 0.25) * h, (jj + 0.25) * h, (kk + 0.25) * h], -1).reshape(-1, 3)
+    cation = ((ii + jj + kk) % 2 == 0).ravel()
+    pos = np.concatenate([pos[cation], pos[~cation]])
+    if shake:
+        pos = pos + rng.uniform(-shake, shake, size=pos.shape)
+    box = n * h
+    ncat = int(cation.sum())
+    moltypes = [_water_type(0),
+                MolType("Na", "atmol", ncat, [("Na", 1, MASS_CAT, 1.0)]),
+                MolType("Cl", "atmol", len(pos) - ncat, [("Cl", 1, MASS_AN, -1.0)]),
+                _crystal_type(0, a, box)]
+    sysm = System("nacl%d" % ncell, (box, box, box), moltypes, {}, np.zeros((0, 3)), np.zeros((0, 4)),
+                  pos, np.zeros((0, 3)), np.zeros(0, np.int32), rcut=rcut, kmax=kmax, skin=1.0,
+                  z_periodic=True)
+    sysm.meta.update(seed=seed, config="rock-salt crystal of mobile ions")
+    return sysm
+
+
 # ---------------------------------------------------------------------------
 # the named workloads
 # ---------------------------------------------------------------------------
@@ -423,7 +451,9 @@ def config(name: str) -> System:
       rc 10, k 8/8/24;
     * ``slab16k``  - 16384 waters on a 16x16x(6 layers) slab (6144 ions), rc 10,
       k 11/11/33 (|k|max of slab2048 kept);
-    * ``slab16k_k8`` - the same with k 8/8/24.
+    * ``slab16k_k8`` - the same with k 8/8/24;
+    * ``nacl4``    - 512 mobile ions in a periodic 4x4x4-cell rock-salt crystal (closed-form
+      Madelung energy).
     """
     if name == "bulk216":
         return bulk_water()
@@ -436,4 +466,6 @@ def config(name: str) -> System:
         return water_on_slab(16384, 16, 6, seed=1002, rcut=10.0, kmax=11, lw=80.0, name=name)
     if name == "slab16k_k8":
         return water_on_slab(16384, 16, 6, seed=1002, rcut=10.0, kmax=8, lw=80.0, name=name)
+    if name == "nacl4":
+        return rocksalt_bulk(4)
     raise KeyError(name)
