@@ -5,6 +5,7 @@ from __future__ import annotations
 
 import os
 import re
+import shutil
 import subprocess
 import tempfile
 from typing import Dict
@@ -57,3 +58,32 @@ def run_reference(system, workdir: str = None, timeout: float = 600.0, nrep: int
         if m:
             dump["_timing"] = np.array([float(m.group(1)), float(m.group(2))])
     return dump
+
+
+def run_nve(system, nsteps: int, variant: str = None, env=None, workdir: str = None,
+            timeout: float = 600.0) -> np.ndarray:
+    """Run an NVE trajectory driver (oracle/md_nve.f90) on `system` for `nsteps` steps of
+    system.dt: variant 'slab'/'bulk' is the reference's own force module, 'gpu' the engine's
+    drop-in module.  Returns (nsteps, 6): step, htot, uconf, ke, tempnow, ulr."""
+    from md_sim_amd.refio import write_reference_inputs
+
+    if variant is None:
+        variant = "bulk" if system.z_periodic else "slab"
+    exe = os.path.join(REF_DIR, "md_nve_" + variant)
+    if not os.access(exe, os.X_OK):
+        raise FileNotFoundError(exe)
+    own = workdir is None
+    d = tempfile.mkdtemp(prefix="mdnve_") if own else workdir
+    write_reference_inputs(system, d)
+    out = os.path.join(d, "traj.txt")
+    e = dict(os.environ)
+    if env:
+        e.update(env)
+    r = subprocess.run([exe, out, str(int(nsteps))], cwd=d, capture_output=True, text=True, timeout=timeout,
+                       env=e)
+    if r.returncode != 0 or not os.path.exists(out):
+        raise RuntimeError("NVE driver failed (%d):\n%s\n%s" % (r.returncode, r.stdout[-2000:], r.stderr[-2000:]))
+    t = np.loadtxt(out, ndmin=2)
+    if own:
+        shutil.rmtree(d, ignore_errors=True)
+    return t
