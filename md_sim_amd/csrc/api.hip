@@ -23,13 +23,11 @@
 #include "mdf_internal.h"
 
 namespace mdf {
-__global__ void k_reset(DevSys S, int force_rebuild);
 __global__ void k_skin(DevSys S, double *dip_part);
-__global__ void k_decide(DevSys S, const double *dip_part, int n_dip, double *dipz_out);
-__global__ void k_cell_clear(DevSys S);
+__global__ void k_decide(DevSys S, const double *dip_part, int n_dip, double *dipz_out, int force_rebuild);
+__global__ void k_cell_scan_build(DevSys S);
 __global__ void k_cell_count(DevSys S, int *lead_cell_out);
-__global__ void k_cell_scan(int *cnt, int *start, int *cursor, const DevFlags *f, int ncells_xy, int use_flag_ncz,
-                            int ncz_given);
+__global__ void k_cell_scan(int *cnt, int *start, int *cursor, int n);
 __global__ void k_cell_fill(const int *item_cell, int n_items, int *cursor, int *items, const DevFlags *f,
                             int use_flag);
 __global__ void k_cell_sort(const int *start, const int *items_in, int *items_out, const DevFlags *f, int use_flag,
@@ -51,9 +49,9 @@ __global__ void k_bp(const double *, const double *, const double *, const doubl
                      const KRow *, const int *, const double4 *, double, double, double, double *);
 constexpr int BP_NS = 4;   // sites per back-projection thread
 __global__ void k_epilogue(DevSys S, const double *fpart, int n_groups, const WallLR *wl, int ew3dc, double crf0,
-                           const double *dipz, double *epart);
-__global__ void k_com(DevSys S, double *fcom, double *tq);
+                           const double *dipz, double *epart, double *fcom, double *tq);
 __global__ void k_colsum(DevSys S, const int *lead_slot, int per_block, double *colpart);
+__global__ void k_pair_totals(DevSys S);
 __global__ void k_final(DevSys S, FinalArgs A, const double *colpart, const double *kpart, const double *eppart,
                         const double *dipz, double *e_vdw, double *e_coul, Scalars *out);
 
@@ -156,7 +154,7 @@ struct mdf_system {
     DBuf<int> itype, nsi, ijtype, vkind, cls, lead, lead_slot, con_ptr, con_idx, ang_ptr, ang_inc, ang_ijk;
     DBuf<double> q, eps4, sig6, aa, rhoi, cc, ang_kt, ang_th;
     DBuf<double> x, y, z, cx, cy, cz, ocx, ocy, ocz, ox, oy, oz, fx, fy, fz, part;
-    DBuf<int> nbr_mm, cnt_mm, nbr_mw, cnt_mw;
+    DBuf<int> nbr_mm, cnt_mm, nbr_mw, cnt_mw, cnt_lo;
     DBuf<int> cell_cnt, cell_start, cell_items, cell_items2, cell_cursor, lead_cell;
     DBuf<int> wcell_start, wcell_items;
     DBuf<DevFlags> flags;
@@ -184,6 +182,8 @@ struct mdf_system {
     hipGraphExec_t graph[NRING] = {};
     bool use_graphs = true;
     bool pair_rolled = false;   // site pairs of a molecule pair as a rolled loop (fewer registers)
+    // launch-shape targets of the reciprocal kernels (blocks per launch; the row groups follow)
+    int sf_target_blocks = 4096, bp_target_blocks = 1024;
     double stage_sum[N_STAGES] = {};
     long long stage_steps = 0;       // steps with per-stage events (direct launches)
     double step_sum = 0.0;
@@ -199,7 +199,7 @@ struct mdf_system {
 
 static void free_all(mdf_system *s) {
     DBuf<int> *ib[] = {&s->itype, &s->nsi, &s->ijtype, &s->vkind, &s->cls, &s->lead, &s->lead_slot, &s->con_ptr,
-                       &s->con_idx, &s->ang_ptr, &s->ang_inc, &s->ang_ijk, &s->nbr_mm, &s->cnt_mm, &s->nbr_mw, &s->cnt_mw, &s->cell_cnt, &s->cell_start,
+                       &s->con_idx, &s->ang_ptr, &s->ang_inc, &s->ang_ijk, &s->nbr_mm, &s->cnt_mm, &s->nbr_mw, &s->cnt_mw, &s->cnt_lo, &s->cell_cnt, &s->cell_start,
                        &s->cell_items, &s->cell_items2, &s->cell_cursor, &s->lead_cell, &s->wcell_start,
                        &s->wcell_items, &s->d_kvec};
     for (auto *b : ib) b->release();
@@ -317,17 +317,17 @@ static void geometry(mdf_system *s) {
     int per_sb = SF_BLOCK * 4;
     s->n_sb = std::max(1, (nm + per_sb - 1) / per_sb);
     int nrow = (int)s->rows.size();
-    s->n_cg = std::max(1, std::min(nrow, (2048 + s->n_sb - 1) / s->n_sb));   // row groups
+    s->n_cg = std::max(1, std::min(nrow, (s->sf_target_blocks + s->n_sb - 1) / s->n_sb));   // row groups
     s->cpb = std::max(1, (nrow + s->n_cg - 1) / s->n_cg);                     // rows per group
     s->n_cg = std::max(1, (nrow + s->cpb - 1) / s->cpb);
     int nk = (int)s->ak.size();
-    s->n_kblk = std::max(1, (nk + 255) / 256);
+    s->n_kblk = std::max(1, (nk + SFR_K - 1) / SFR_K);
     // back-projection row groups: about equal k counts, each at most BP_GROUP_K k-vectors (the
     // LDS stage), and enough groups that sites x groups fill the GPU
     int nsb_bp = std::max(1, (nm + 256 * BP_NS - 1) / (256 * BP_NS));
     int nrows = (int)s->rows.size();
     int nk_all = (int)s->ak.size();
-    int want = std::max(1, std::min(std::min(nrows, BP_ROWGROUPS_MAX), (1024 + nsb_bp - 1) / nsb_bp));
+    int want = std::max(1, std::min(std::min(nrows, BP_ROWGROUPS_MAX), (s->bp_target_blocks + nsb_bp - 1) / nsb_bp));
     want = std::max(want, (nk_all + BP_GROUP_K - 1) / BP_GROUP_K);
     s->bp_row_start.clear();
     s->bp_row_start.push_back(0);
@@ -346,7 +346,7 @@ static void geometry(mdf_system *s) {
     }
     s->n_rg = (int)s->bp_row_start.size() - 1;
     s->rpg = 0;
-    s->n_epblk = std::max(1, (nm + 255) / 256);
+    s->n_epblk = std::max(1, (nm + EPI_BLOCK - 1) / EPI_BLOCK);
     s->n_colblk = std::max(1, std::min(64, (s->n_lead + 1023) / 1024));
     s->colper = std::max(1, (s->n_lead + s->n_colblk - 1) / s->n_colblk);
 }
@@ -371,13 +371,11 @@ static int alloc_lists(mdf_system *s, int cap_mm, int cap_mw) {
 static void launch_lists(mdf_system *s, int force_rebuild) {
     DevSys &S = s->S;
     hipStream_t st = s->stream;
-    k_reset<<<1, 64, 0, st>>>(S, force_rebuild);
     k_skin<<<s->skin_blocks, 256, 0, st>>>(S, s->dip_part.p);
-    k_decide<<<1, 256, 0, st>>>(S, s->dip_part.p, s->skin_blocks, s->dipz.p);
-    k_cell_clear<<<64, 256, 0, st>>>(S);
+    k_decide<<<1, 256, 0, st>>>(S, s->dip_part.p, s->skin_blocks, s->dipz.p, force_rebuild);
     int lb = std::max(1, (s->n_lead + 255) / 256);
     k_cell_count<<<lb, 256, 0, st>>>(S, s->lead_cell.p);
-    k_cell_scan<<<1, 1024, 0, st>>>(s->cell_cnt.p, s->cell_start.p, s->cell_cursor.p, s->flags.p, s->ncells_xy, 1, 0);
+    k_cell_scan_build<<<1, 1024, 0, st>>>(S);
     k_cell_fill<<<lb, 256, 0, st>>>(s->lead_cell.p, s->n_lead, s->cell_cursor.p, s->cell_items2.p, s->flags.p, 1);
     k_cell_sort<<<512, 64, 0, st>>>(s->cell_start.p, s->cell_items2.p, s->cell_items.p, s->flags.p, 1, s->ncells_xy, 0);
     k_build<<<std::max(1, (s->n_lead + 3) / 4), 256, 0, st>>>(S);  // one wave per lead
@@ -434,13 +432,12 @@ static void launch_forces(mdf_system *s) {
                                               s->d_bp_row_start.p, s->d_t.p, pipix, pipiy, pipiz, s->fpart.p);
     }
     if (s->ev) (void)hipEventRecord(s->ev[5], st);
-    k_epilogue<<<s->n_epblk, 256, 0, st>>>(S, s->fpart.p, (nk > 0 && nm > 0) ? s->n_rg : 0, s->d_wl.p,
-                                           s->par.ew3dc, s->crf0, s->dipz.p, s->eppart.p);
-    if (s->top.n_rig > 0) k_com<<<(s->top.n_rig + 255) / 256, 256, 0, st>>>(S, s->fcom.p, s->torque.p);
+    k_epilogue<<<s->n_epblk, EPI_BLOCK, 0, st>>>(S, s->fpart.p, (nk > 0 && nm > 0) ? s->n_rg : 0, s->d_wl.p,
+                                                 s->par.ew3dc, s->crf0, s->dipz.p, s->eppart.p, s->fcom.p,
+                                                 s->torque.p);
     if (s->ev) (void)hipEventRecord(s->ev[6], st);
     {
         dim3 gc(NCOL, s->n_colblk);
-        k_colsum<<<gc, 256, 0, st>>>(S, s->lead_slot.p, s->colper, s->colpart.p);
         FinalArgs A;
         memset(&A, 0, sizeof A);
         A.n_colblk = s->n_colblk;
@@ -457,6 +454,7 @@ static void launch_forces(mdf_system *s) {
         A.ly = s->par.box[1];
         A.na_scale = s->c.na * s->c.e30;
         A.bar = s->c.e5;
+        k_colsum<<<gc, 256, 0, st>>>(S, s->lead_slot.p, s->colper, s->colpart.p);
         k_final<<<1, 256, 0, st>>>(S, A, s->colpart.p, s->kpart.p, s->eppart.p, s->dipz.p, s->e_vdw.p, s->e_coul.p,
                                    s->d_scal.p);
     }
@@ -638,6 +636,10 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
         if (g && g[0] == '0') s->use_graphs = false;
         const char *pr = getenv("MDF_PAIR_ROLLED");
         if (pr && pr[0] == '1') s->pair_rolled = true;
+        const char *sb = getenv("MDF_SF_BLOCKS");
+        if (sb && atoi(sb) > 0) s->sf_target_blocks = atoi(sb);
+        const char *bb = getenv("MDF_BP_BLOCKS");
+        if (bb && atoi(bb) > 0) s->bp_target_blocks = atoi(bb);
     }
     hipError_t he = hipSetDevice(par->device);
     if (he != hipSuccess) { delete s; return fail(MDF_ERR_HIP, "hipSetDevice: %s", hipGetErrorString(he)); }
@@ -819,6 +821,7 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
             HTRY(hipMemcpy(s->flags.p, &f0, sizeof f0, hipMemcpyHostToDevice));
         }
         S.flags = s->flags.p;
+        HTRY(hipMemset(s->cell_cnt.p, 0, sizeof(int) * (s->n_cells_cap + 1)));
         // frozen-site cells, built once
         {
             int nw = s->n_wall;
@@ -844,7 +847,7 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
                 k_wall_cells<<<(nw + 255) / 256, 256, 0, s->stream>>>(S, wcell_of.p, wncz, wz0, wcsz);
                 k_count_cells<<<(nw + 255) / 256, 256, 0, s->stream>>>(wcell_of.p, nw, wcnt.p);
             }
-            k_cell_scan<<<1, 1024, 0, s->stream>>>(wcnt.p, s->wcell_start.p, wcur.p, s->flags.p, s->ncells_xy, 0, wncz);
+            k_cell_scan<<<1, 1024, 0, s->stream>>>(wcnt.p, s->wcell_start.p, wcur.p, s->ncells_xy * wncz);
             if (nw > 0) {
                 k_cell_fill<<<(nw + 255) / 256, 256, 0, s->stream>>>(wcell_of.p, nw, wcur.p, witems2.p, s->flags.p, 0);
                 k_cell_sort<<<512, 64, 0, s->stream>>>(s->wcell_start.p, witems2.p, s->wcell_items.p, s->flags.p, 0,
@@ -857,7 +860,8 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
         }
         // lists
         HTRY(s->cnt_mm.alloc(std::max(s->n_lead, 1))); HTRY(s->cnt_mw.alloc(std::max(s->n_lead, 1)));
-        S.cnt_mm = s->cnt_mm.p; S.cnt_mw = s->cnt_mw.p;
+        HTRY(s->cnt_lo.alloc(std::max(s->n_lead, 1)));
+        S.cnt_mm = s->cnt_mm.p; S.cnt_mw = s->cnt_mw.p; S.cnt_lo = s->cnt_lo.p;
         {
             int cap_mm = 256, cap_mw = s->n_wall > 0 ? 256 : 1;
             if (par->max_neighbors > 0) { cap_mm = par->max_neighbors; cap_mw = par->max_neighbors; s->fixed_cap = true; }
@@ -957,7 +961,7 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
         // tables and reduction buffers
         wall_tables(s, t.charge);
         HTRY(upload(s->d_wl, &s->wl, 1));
-        HTRY(s->dip_part.alloc(s->skin_blocks)); HTRY(s->dipz.alloc(1));
+        HTRY(s->dip_part.alloc(4 * (size_t)s->skin_blocks)); HTRY(s->dipz.alloc(1));
         HTRY(s->eppart.alloc((size_t)6 * s->n_epblk));
         HTRY(s->colpart.alloc((size_t)NCOL * s->n_colblk * MAX_LEAD_TYPES));
         HTRY(s->fcom.alloc(3 * nr)); HTRY(s->torque.alloc(3 * nr));
@@ -966,6 +970,7 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
         // first list (nnlist_init -> nnlist_make at force_init)
         for (int attempt = 0; attempt < 4; attempt++) {
             launch_lists(s, 1);
+            k_pair_totals<<<1, 256, 0, s->stream>>>(S);
             HTRY(hipGetLastError());
             int redo = 0;
             HTRY(hipStreamSynchronize(s->stream));

@@ -5,7 +5,8 @@
 //              (src/forcem.f90:923-928) + angle bending (src/forcem.f90:565-614);
 //              per-block partials of ulr, the wall z force it
 //              adds, and the total force (fws(0), src/forcem.f90:990-992).
-//  k_com:      rigid-body COM force and torque per molecule (src/forcem.f90:1003-1024).
+//              Each block also forms its molecules' COM forces and torques
+//              (src/forcem.f90:1003-1024).
 //  k_colsum:   per-lead partial columns -> per-block, per-lead-type sums.
 //  k_final:    energies per type pair, reciprocal energies, uconf, pressure from the wall
 //              forces, virial (src/forcem.f90:1032-1081).
@@ -43,13 +44,19 @@ __device__ void angle_force(const DevSys &S, int a, int role, double &fx, double
     }
 }
 
-__global__ __launch_bounds__(256) void k_epilogue(DevSys S, const double *__restrict__ fpart, int n_groups,
-                                                  const WallLR *__restrict__ wl, int ew3dc, double crf0,
-                                                  const double *__restrict__ dipz, double *__restrict__ epart) {
-    __shared__ double red[EP_N][256];
+// One block covers EPI_BLOCK = 3 x 64 consecutive sites, i.e. 64 whole rigid molecules
+// (rigid sites come first, three per molecule), so the block also forms those molecules'
+// COM forces and torques (src/forcem.f90:1003-1024) from the forces it has just finished.
+__global__ __launch_bounds__(EPI_BLOCK) void k_epilogue(DevSys S, const double *__restrict__ fpart, int n_groups,
+                                                        const WallLR *__restrict__ wl, int ew3dc, double crf0,
+                                                        const double *__restrict__ dipz, double *__restrict__ epart,
+                                                        double *__restrict__ fcom, double *__restrict__ tq) {
+    __shared__ double sf[3][EPI_BLOCK];
+    __shared__ double wsum[EPI_BLOCK / 64][EP_N];
     double acc[EP_N] = {0, 0, 0, 0, 0, 0};
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = S.n_move;
+    sf[0][threadIdx.x] = 0.0; sf[1][threadIdx.x] = 0.0; sf[2][threadIdx.x] = 0.0;
     if (i < n) {
         double fx = S.fx[i], fy = S.fy[i], fz = S.fz[i];
         // wall long-range tail correction, sites with |q| >= 0.7
@@ -95,36 +102,39 @@ __global__ __launch_bounds__(256) void k_epilogue(DevSys S, const double *__rest
                 angle_force(S, e >> 1, e & 1, fx, fy, fz);
             }
         S.fx[i] = fx; S.fy[i] = fy; S.fz[i] = fz;
+        sf[0][threadIdx.x] = fx; sf[1][threadIdx.x] = fy; sf[2][threadIdx.x] = fz;
         acc[3] = fx; acc[4] = fy; acc[5] = fz;
     }
+    // block partials: a fixed butterfly per wave, then the waves in order
 #pragma unroll
-    for (int c = 0; c < EP_N; c++) red[c][threadIdx.x] = acc[c];
+    for (int c = 0; c < EP_N; c++)
+        for (int o = 32; o > 0; o >>= 1) acc[c] += __shfl_xor(acc[c], o, 64);
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int c = 0; c < EP_N; c++) wsum[threadIdx.x >> 6][c] = acc[c];
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s)
-#pragma unroll
-            for (int c = 0; c < EP_N; c++) red[c][threadIdx.x] += red[c][threadIdx.x + s];
-        __syncthreads();
+    if (threadIdx.x < EP_N) {
+        double v = 0.0;
+        for (int w = 0; w < EPI_BLOCK / 64; w++) v += wsum[w][threadIdx.x];
+        epart[blockIdx.x * EP_N + threadIdx.x] = v;
     }
-    if (threadIdx.x < EP_N) epart[blockIdx.x * EP_N + threadIdx.x] = red[threadIdx.x][0];
-}
-
-__global__ void k_com(DevSys S, double *__restrict__ fcom, double *__restrict__ tq) {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    // COM force and torque of the block's molecules
     const int nr = S.n_rig;
-    if (m >= nr) return;
-    const int j = 3 * m;
-    double tx = 0, ty = 0, tz = 0, fx = 0, fy = 0, fz = 0;
-    for (int a = 0; a < 3; a++) {
-        double rx = S.x[j + a] - S.cx[m], ry = S.y[j + a] - S.cy[m], rz = S.z[j + a] - S.cz[m];
-        double gx = S.fx[j + a], gy = S.fy[j + a], gz = S.fz[j + a];
-        tx += ry * gz - rz * gy;
-        ty += rz * gx - rx * gz;
-        tz += rx * gy - ry * gx;
-        fx += gx; fy += gy; fz += gz;
+    const int m = blockIdx.x * (EPI_BLOCK / 3) + threadIdx.x;
+    if (threadIdx.x < EPI_BLOCK / 3 && m < nr) {
+        const int j = 3 * m, lj = 3 * threadIdx.x;
+        double tx = 0, ty = 0, tz = 0, gx = 0, gy = 0, gz = 0;
+        for (int a = 0; a < 3; a++) {
+            const double rx = S.x[j + a] - S.cx[m], ry = S.y[j + a] - S.cy[m], rz = S.z[j + a] - S.cz[m];
+            const double hx = sf[0][lj + a], hy = sf[1][lj + a], hz = sf[2][lj + a];
+            tx += ry * hz - rz * hy;
+            ty += rz * hx - rx * hz;
+            tz += rx * hy - ry * hx;
+            gx += hx; gy += hy; gz += hz;
+        }
+        fcom[m] = gx; fcom[nr + m] = gy; fcom[2 * nr + m] = gz;
+        tq[m] = tx; tq[nr + m] = ty; tq[2 * nr + m] = tz;
     }
-    fcom[m] = fx; fcom[nr + m] = fy; fcom[2 * nr + m] = fz;
-    tq[m] = tx; tq[nr + m] = ty; tq[2 * nr + m] = tz;
 }
 
 // column sums of the per-lead partials, split by the lead's type slot
@@ -153,6 +163,29 @@ __global__ __launch_bounds__(256) void k_colsum(DevSys S, const int *__restrict_
     if (threadIdx.x < MAX_LEAD_TYPES)
         colpart[((size_t)col * gridDim.y + blk) * MAX_LEAD_TYPES + threadIdx.x] = red[threadIdx.x][0];
 }
+
+// listed pair counts (ipack, wpack) after a rebuild: the per-row counts summed in fixed
+// order by one block of 256 threads
+__device__ void pair_totals(const DevSys &S) {
+    __shared__ long long npair[2][256];
+    const int t = threadIdx.x;
+    const bool rb = S.flags->rebuild != 0;
+    if (rb) {
+        long long a = 0, b = 0;
+        for (int l = t; l < S.n_lead; l += 256) { a += S.cnt_lo[l]; b += S.cnt_mw[l]; }
+        npair[0][t] = a; npair[1][t] = b;
+    }
+    __syncthreads();
+    if (rb && t == 0) {
+        long long a = 0, b = 0;
+        for (int k = 0; k < 256; k++) { a += npair[0][k]; b += npair[1][k]; }
+        S.flags->n_pairs_mm = (int)a;
+        S.flags->n_pairs_mw = (int)b;
+    }
+}
+
+// the same after the list build at creation (no force step follows it)
+__global__ __launch_bounds__(256) void k_pair_totals(DevSys S) { pair_totals(S); }
 
 __global__ __launch_bounds__(256) void k_final(DevSys S, FinalArgs A, const double *__restrict__ colpart,
                                                const double *__restrict__ kpart, const double *__restrict__ eppart,
@@ -191,7 +224,7 @@ __global__ __launch_bounds__(256) void k_final(DevSys S, FinalArgs A, const doub
             else epsum[t - RE_N] = acc;
         }
     }
-    __syncthreads();
+    pair_totals(S);
     if (t != 0) return;
     const int nt1 = S.n_types + 1;
     for (int k = 0; k <= S.nijt; k++) { e_vdw[k] = 0.0; e_coul[k] = 0.0; }

@@ -25,6 +25,7 @@ constexpr int NCOL = 2 * NCLS + 9 + 6;
 constexpr int KC = 16;           // k-vectors per structure-factor chunk held in registers
 constexpr int SF_SITES = 4;      // sites per thread in the structure-factor kernel
 constexpr int SF_BLOCK = 256;
+constexpr int SFR_K = 64;        // k-vectors per structure-factor reduction block (one wave wide)
 constexpr int BP_ROWGROUPS_MAX = 64;
 constexpr int BP_GROUP_K = 1024;     // k-vectors per back-projection row group (its LDS stage)
 
@@ -41,6 +42,7 @@ struct WallLR {
     int on;
 };
 
+constexpr int EPI_BLOCK = 192;  // epilogue block: 64 rigid molecules x 3 sites
 constexpr int EP_N = 6;  // epilogue partials: ulr, fwsz(-1), fwsz(+1), sum fx, sum fy, sum fz
 constexpr int RE_N = 8;  // k partials: A|S|^2, A|S_moving|^2, virial xx yy zz xy xz yz
 
@@ -68,8 +70,6 @@ struct FinalArgs {
 
 // flags written by device kernels, read back by the host after a step
 struct DevFlags {
-    unsigned long long max_disp2_bits;   // max squared displacement since the last build
-    unsigned long long zmin_key, zmax_key; // ordered keys of lead z range
     int rebuild;                         // 1: the list is rebuilt this step
     int overflow_mm, overflow_mw;        // max count seen when a list row overflowed
     int ncz;                             // cells in z for this build
@@ -121,6 +121,7 @@ struct DevSys {
     int erfc_mode;
     // lists
     int *nbr_mm, *cnt_mm; int cap_mm;
+    int *cnt_lo;                 // [n_lead] partners j > i per row (the reference's i<j pairs)
     int *nbr_mw, *cnt_mw; int cap_mw;
     // cells
     Grid g;
@@ -134,6 +135,7 @@ struct DevSys {
     double *part;                // [NCOL][n_lead]
     DevFlags *flags;
 };
+
 
 // ---- helpers ---------------------------------------------------------------------
 // Fortran anint: round half away from zero
@@ -149,18 +151,6 @@ __device__ __forceinline__ double list_d2(const DevSys &S, int i, int j) {
     double dz = __dsub_rn(S.z[i], S.z[j]);
     if (S.zper_list) dz = __dsub_rn(dz, __dmul_rn(S.lz, anint_d(__dmul_rn(dz, S.lzi))));
     return __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
-}
-
-// order-preserving key of a double for atomic min/max
-__device__ __forceinline__ unsigned long long dkey(double v) {
-    unsigned long long b = __double_as_longlong(v);
-    return (b & 0x8000000000000000ull) ? ~b : (b | 0x8000000000000000ull);
-}
-__host__ __device__ inline double dunkey(unsigned long long k) {
-    unsigned long long b = (k & 0x8000000000000000ull) ? (k & 0x7fffffffffffffffull) : ~k;
-    double v;
-    __builtin_memcpy(&v, &b, 8);
-    return v;
 }
 
 }  // namespace mdf

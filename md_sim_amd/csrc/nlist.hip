@@ -7,23 +7,11 @@
 // whose cells are at least sqrt(rskinsq) wide, and the same predicate - same operation
 // order, no FMA contraction - decides membership, so the pair set is bit-identical.
 // Lists are ELL rows per lead: moving partners in both directions (full list) and frozen
-// partners.  All kernels after k_decide return at once when no rebuild is due, so the
+// partners.  All kernels after the skin test return at once when no rebuild is due, so the
 // step's launch sequence is fixed.
 #include "mdf_internal.h"
 
 namespace mdf {
-
-__global__ void k_reset(DevSys S, int force_rebuild) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        DevFlags *f = S.flags;
-        f->max_disp2_bits = 0ull;
-        f->zmin_key = ~0ull;
-        f->zmax_key = 0ull;
-        f->rebuild = force_rebuild;
-        f->overflow_mm = 0;
-        f->overflow_mw = 0;
-    }
-}
 
 // max squared displacement since the last build (src/forcem.f90:245-249) and lead z range;
 // per-block partial of the moving dipole sum(q z) (src/forcem.f90:924) into dip_part
@@ -52,38 +40,69 @@ __global__ __launch_bounds__(256) void k_skin(DevSys S, double *dip_part) {
             zhi = fmax(zhi, zz);
         }
     }
-    // block dipole partial (fixed tree order)
+    // block partials, no atomics: part[0][b] dipole (fixed tree order), part[1][b] max
+    // displacement, part[2][b] / part[3][b] lead z range (order independent)
+    __shared__ double ext[3][4];
+    for (int o = 32; o > 0; o >>= 1) {
+        dmax = fmax(dmax, __shfl_xor(dmax, o, 64));
+        zlo = fmin(zlo, __shfl_xor(zlo, o, 64));
+        zhi = fmax(zhi, __shfl_xor(zhi, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        ext[0][threadIdx.x >> 6] = dmax; ext[1][threadIdx.x >> 6] = zlo; ext[2][threadIdx.x >> 6] = zhi;
+    }
     red[threadIdx.x] = dip;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
         __syncthreads();
     }
-    if (threadIdx.x == 0) dip_part[blockIdx.x] = red[0];
-    // max and min/max are order independent
-    if (dmax > 0.0) atomicMax(&S.flags->max_disp2_bits, (unsigned long long)__double_as_longlong(dmax));
-    if (zlo <= zhi) {
-        atomicMin(&S.flags->zmin_key, dkey(zlo));
-        atomicMax(&S.flags->zmax_key, dkey(zhi));
+    if (threadIdx.x == 0) {
+        const int nb = gridDim.x;
+        dip_part[blockIdx.x] = red[0];
+        dip_part[nb + blockIdx.x] = fmax(fmax(ext[0][0], ext[0][1]), fmax(ext[0][2], ext[0][3]));
+        dip_part[2 * nb + blockIdx.x] = fmin(fmin(ext[1][0], ext[1][1]), fmin(ext[1][2], ext[1][3]));
+        dip_part[3 * nb + blockIdx.x] = fmax(fmax(ext[2][0], ext[2][1]), fmax(ext[2][2], ext[2][3]));
     }
 }
 
-// rebuild decision (src/forcem.f90:250) and z grid of this build; dipole total
-__global__ __launch_bounds__(256) void k_decide(DevSys S, const double *dip_part, int n_dip, double *dipz_out) {
-    __shared__ double red[256];
-    double acc = 0.0;
-    for (int b = threadIdx.x; b < n_dip; b += blockDim.x) acc += dip_part[b];
-    red[threadIdx.x] = acc;
+// rebuild decision (src/forcem.f90:250) and z grid of this build; dipole total.  One block,
+// over k_skin's per-block partials.  It also clears the overflow flags for this step's build
+// (so no reset launch is needed).
+__global__ __launch_bounds__(256) void k_decide(DevSys S, const double *dip_part, int n_dip, double *dipz_out,
+                                                int force_rebuild) {
+    __shared__ double red2[256];
+    __shared__ double ext[3][4];
+    double acc = 0.0, dmax = 0.0, zlo = 1e300, zhi = -1e300;
+    for (int b = threadIdx.x; b < n_dip; b += blockDim.x) {
+        acc += dip_part[b];
+        dmax = fmax(dmax, dip_part[n_dip + b]);
+        zlo = fmin(zlo, dip_part[2 * n_dip + b]);
+        zhi = fmax(zhi, dip_part[3 * n_dip + b]);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        dmax = fmax(dmax, __shfl_xor(dmax, o, 64));
+        zlo = fmin(zlo, __shfl_xor(zlo, o, 64));
+        zhi = fmax(zhi, __shfl_xor(zhi, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        ext[0][threadIdx.x >> 6] = dmax; ext[1][threadIdx.x >> 6] = zlo; ext[2][threadIdx.x >> 6] = zhi;
+    }
+    red2[threadIdx.x] = acc;
     __syncthreads();
     for (int st = 128; st > 0; st >>= 1) {
-        if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        if (threadIdx.x < st) red2[threadIdx.x] += red2[threadIdx.x + st];
         __syncthreads();
     }
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    *dipz_out = red[0];
+    if (threadIdx.x != 0) return;
+    *dipz_out = red2[0];
+    dmax = fmax(fmax(ext[0][0], ext[0][1]), fmax(ext[0][2], ext[0][3]));
+    zlo = fmin(fmin(ext[1][0], ext[1][1]), fmin(ext[1][2], ext[1][3]));
+    zhi = fmax(fmax(ext[2][0], ext[2][1]), fmax(ext[2][2], ext[2][3]));
     DevFlags *f = S.flags;
-    double aux = __longlong_as_double((long long)f->max_disp2_bits);
-    if (__dmul_rn(4.0, aux) > S.skin2) f->rebuild = 1;
+    f->overflow_mm = 0;
+    f->overflow_mw = 0;
+    f->rebuild = (force_rebuild || __dmul_rn(4.0, dmax) > S.skin2) ? 1 : 0;
     if (f->rebuild) {
         // pair counts describe the standing list (ipack/wpack): restart them for this build
         f->n_builds += 1;
@@ -95,7 +114,6 @@ __global__ __launch_bounds__(256) void k_decide(DevSys S, const double *dip_part
         f->z0 = 0.0;
         f->csz = S.g.csz_fixed;
     } else {
-        double zlo = dunkey(f->zmin_key), zhi = dunkey(f->zmax_key);
         if (!(zlo <= zhi)) { zlo = 0.0; zhi = 0.0; }
         double rs = sqrt(S.rskinsq) * (1.0 + 1e-9);
         double range = zhi - zlo;
@@ -126,43 +144,52 @@ __device__ __forceinline__ int lead_cell(const DevSys &S, int i, int ncz, double
     return (cz * S.g.ncy + cy) * S.g.ncx + cx;
 }
 
-__global__ void k_cell_clear(DevSys S) {
-    if (!S.flags->rebuild) return;
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < S.n_cells_cap; c += gridDim.x * blockDim.x)
-        S.cell_cnt[c] = 0;
-}
-
-__global__ void k_cell_count(DevSys S, int *lead_cell_out) {
-    const DevFlags *f = S.flags;
-    if (!f->rebuild) return;
-    int l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= S.n_lead) return;
-    int c = lead_cell(S, S.lead[l], f->ncz, f->z0, f->csz);
-    lead_cell_out[l] = c;
-    atomicAdd(&S.cell_cnt[c], 1);
-}
-
-// exclusive scan of cell counts, one block (cells per build are at most n_cells_cap)
-__global__ __launch_bounds__(1024) void k_cell_scan(int *cnt, int *start, int *cursor, const DevFlags *f,
-                                                    int ncells_xy, int use_flag_ncz, int ncz_given) {
-    if (use_flag_ncz && !f->rebuild) return;
+// exclusive scan of n cell counts by one block (any size); with zero_cnt the counts are
+// cleared behind it, ready for the next build's counting
+__device__ void scan_cells(int *cnt, int *start, int *cursor, int n, bool zero_cnt) {
     __shared__ int part[1024];
-    int ncz = use_flag_ncz ? f->ncz : ncz_given;
-    int n = ncells_xy * ncz;
-    int per = (n + blockDim.x - 1) / blockDim.x;
-    int b = threadIdx.x * per, e = min(n, b + per);
+    const int nt = blockDim.x;
+    const int per = (n + nt - 1) / nt;
+    const int b = threadIdx.x * per, e = min(n, b + per);
     int s = 0;
     for (int c = b; c < e; c++) s += cnt[c];
     part[threadIdx.x] = s;
     __syncthreads();
     if (threadIdx.x == 0) {
         int acc = 0;
-        for (int t = 0; t < (int)blockDim.x; t++) { int v = part[t]; part[t] = acc; acc += v; }
+        for (int t = 0; t < nt; t++) { int v = part[t]; part[t] = acc; acc += v; }
     }
     __syncthreads();
     int acc = part[threadIdx.x];
-    for (int c = b; c < e; c++) { start[c] = acc; cursor[c] = acc; acc += cnt[c]; }
-    if (threadIdx.x == blockDim.x - 1) start[n] = acc;
+    for (int c = b; c < e; c++) {
+        const int v = cnt[c];
+        start[c] = acc; cursor[c] = acc; acc += v;
+        if (zero_cnt) cnt[c] = 0;
+    }
+    if (threadIdx.x == nt - 1) start[n] = acc;
+}
+
+// cell of every lead and the cell counts
+__global__ __launch_bounds__(256) void k_cell_count(DevSys S, int *lead_cell_out) {
+    const DevFlags *f = S.flags;
+    if (!f->rebuild) return;
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= S.n_lead) return;
+    const int c = lead_cell(S, S.lead[l], f->ncz, f->z0, f->csz);
+    lead_cell_out[l] = c;
+    atomicAdd(&S.cell_cnt[c], 1);
+}
+
+// cell starts of this build; the counts are cleared behind the scan for the next build
+__global__ __launch_bounds__(1024) void k_cell_scan_build(DevSys S) {
+    const DevFlags *f = S.flags;
+    if (!f->rebuild) return;
+    scan_cells(S.cell_cnt, S.cell_start, S.cell_cursor, S.g.ncx * S.g.ncy * f->ncz, true);
+}
+
+// exclusive scan of cell counts, one block (frozen-site cells at creation)
+__global__ __launch_bounds__(1024) void k_cell_scan(int *cnt, int *start, int *cursor, int n) {
+    scan_cells(cnt, start, cursor, n, false);
 }
 
 __global__ void k_cell_fill(const int *item_cell, int n_items, int *cursor, int *items, const DevFlags *f,
@@ -256,8 +283,8 @@ __global__ __launch_bounds__(256) void k_build(DevSys S) {
     for (int o = 32; o > 0; o >>= 1) lower += __shfl_xor(lower, o, 64);
     if (lane == 0) {
         S.cnt_mm[l] = cnt;
+        S.cnt_lo[l] = lower;   // i < j pairs of this row; summed after the build
         if (cnt > S.cap_mm) atomicMax(&S.flags->overflow_mm, cnt);
-        if (lower) atomicAdd(&S.flags->n_pairs_mm, lower);
     }
     // frozen partners: same x,y grid, frozen z grid built once
     int cntw = 0;
@@ -304,16 +331,16 @@ __global__ __launch_bounds__(256) void k_build(DevSys S) {
     if (lane == 0) {
         S.cnt_mw[l] = cntw;
         if (cntw > S.cap_mw) atomicMax(&S.flags->overflow_mw, cntw);
-        if (cntw) atomicAdd(&S.flags->n_pairs_mw, cntw);
     }
 }
 
 // snapshot for the next skin test (src/nnlist.f90:99-119)
+// reference positions for the next skin tests (src/nnlist.f90:99-119)
 __global__ void k_snapshot(DevSys S) {
     if (!S.flags->rebuild) return;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < S.n_rig) { S.ocx[t] = S.cx[t]; S.ocy[t] = S.cy[t]; S.ocz[t] = S.cz[t]; }
-    int i = S.n_rigid_sites + t;
+    const int i = S.n_rigid_sites + t;
     if (i < S.n_move) { S.ox[i] = S.x[i]; S.oy[i] = S.y[i]; S.oz[i] = S.z[i]; }
 }
 

@@ -193,35 +193,48 @@ template __global__ void k_sf<float, 4>(const double *, const double *, const do
                                        double, double, const KRow *, int, int, double2 *, int);
 
 // S(k) = sum over site blocks + static frozen part; T(k) = A_k conj(S(k)); per-block
-// partial energies and virial:  eK = A|S|^2, eLL = A|S_moving|^2, vir_ab = 1/2 eK (d_ab - g k_a k_b)
+// partial energies and virial:  eK = A|S|^2, eLL = A|S_moving|^2, vir_ab = 1/2 eK (d_ab - g k_a k_b).
+// A block owns SFR_K k-vectors; its four waves each sum every fourth site block (fixed
+// order), the first wave combines the four slices and finishes the k-vectors.
 __global__ __launch_bounds__(256) void k_sf_reduce(const double2 *__restrict__ partial, int n_sb, int nk,
                                                    const double2 *__restrict__ s_wall, const double *__restrict__ ak,
                                                    const int *__restrict__ kvec, double pipix, double pipiy,
                                                    double pipiz, double lamb, double2 *__restrict__ s_out,
                                                    double4 *__restrict__ t_out, double *__restrict__ epart) {
-    __shared__ double red[RE_N][256];
+    constexpr int SL = 256 / SFR_K;
+    __shared__ double2 slice[SL][SFR_K];
+    const int kk = threadIdx.x % SFR_K, sl = threadIdx.x / SFR_K;
+    const int k = blockIdx.x * SFR_K + kk;
+    double sr = 0.0, si = 0.0;
+    if (k < nk) {
+#pragma unroll 4
+        for (int b = sl; b < n_sb; b += SL) {
+            const double2 v = partial[(size_t)b * nk + k];
+            sr += v.x; si += v.y;
+        }
+    }
+    slice[sl][kk] = make_double2(sr, si);
+    __syncthreads();
+    if (sl != 0) return;
     double acc[RE_N];
 #pragma unroll
     for (int c = 0; c < RE_N; c++) acc[c] = 0.0;
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < nk) {
-        double sr = 0.0, si = 0.0;
-        for (int b = 0; b < n_sb; b++) {
-            double2 v = partial[(size_t)b * nk + k];
-            sr += v.x; si += v.y;
-        }
-        double2 w = s_wall[k];
-        double qr = sr + w.x, qi = si + w.y;
+        sr = 0.0; si = 0.0;
+#pragma unroll
+        for (int j = 0; j < SL; j++) { sr += slice[j][kk].x; si += slice[j][kk].y; }
+        const double2 w = s_wall[k];
+        const double qr = sr + w.x, qi = si + w.y;
         s_out[k] = make_double2(qr, qi);
-        double a = ak[k];
+        const double a = ak[k];
         const double kn = (double)kvec[3 * k + 2];
         t_out[k] = make_double4(a * qr, -a * qi, kn * (a * qr), kn * (-a * qi));
-        double ek = a * (qr * qr + qi * qi);
+        const double ek = a * (qr * qr + qi * qi);
         acc[0] = ek;
         acc[1] = a * (sr * sr + si * si);
-        double kx = pipix * kvec[3 * k], ky = pipiy * kvec[3 * k + 1], kz = pipiz * kvec[3 * k + 2];
-        double g = 2.0 * (1.0 / (kx * kx + ky * ky + kz * kz) + lamb);
-        double h = 0.5 * ek;
+        const double kx = pipix * kvec[3 * k], ky = pipiy * kvec[3 * k + 1], kz = pipiz * kvec[3 * k + 2];
+        const double g = 2.0 * (1.0 / (kx * kx + ky * ky + kz * kz) + lamb);
+        const double h = 0.5 * ek;
         acc[2] = h * (1.0 - g * kx * kx);
         acc[3] = h * (1.0 - g * ky * ky);
         acc[4] = h * (1.0 - g * kz * kz);
@@ -229,16 +242,13 @@ __global__ __launch_bounds__(256) void k_sf_reduce(const double2 *__restrict__ p
         acc[6] = -h * g * kx * kz;
         acc[7] = -h * g * ky * kz;
     }
+    // fixed butterfly over the wave: the same additions in the same order on every run
 #pragma unroll
-    for (int c = 0; c < RE_N; c++) red[c][threadIdx.x] = acc[c];
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s)
+    for (int c = 0; c < RE_N; c++)
+        for (int o = 32; o > 0; o >>= 1) acc[c] += __shfl_xor(acc[c], o, 64);
+    if (kk == 0)
 #pragma unroll
-            for (int c = 0; c < RE_N; c++) red[c][threadIdx.x] += red[c][threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x < RE_N) epart[blockIdx.x * RE_N + threadIdx.x] = red[threadIdx.x][0];
+        for (int c = 0; c < RE_N; c++) epart[blockIdx.x * RE_N + c] = acc[c];
 }
 
 // T4[k] = (A conj S, kn A conj S): per k-vector the force pieces are Im(T e) and kn Im(T e),
