@@ -90,16 +90,32 @@ def build_abi(system, precision: int = FP64, const_mode: int = CONST_REFERENCE,
     src/forcem.f90:808-819); angles add k(theta - theta0) bending forces on j and k
     (src/forcem.f90:565-614).  ``n_liquid_sites`` moves the liquid/surface boundary
     (``nliqa``): moving sites from that index on are surface sites."""
+    from .topology import TopologyData
     s = system
-    ijtype, kind, eps4, sig6, A, rhoi, Cc = s.vdw_tables()
-    keep = dict(site_type=np.ascontiguousarray(s.site_types(), np.int32),
-                charge=np.ascontiguousarray(s.charges(), np.float64),
-                nsi=np.ascontiguousarray(s.nsi(), np.int32),
-                ijtype=np.ascontiguousarray(ijtype.ravel(), np.int32),
-                vdw_kind=kind, vdw_eps4=eps4, vdw_sig6=sig6, vdw_a=A, vdw_rhoi=rhoi, vdw_c=Cc)
+    derived = isinstance(s, TopologyData)
+    if derived:
+        keep = dict(site_type=np.ascontiguousarray(s.site_type, np.int32),
+                    charge=np.ascontiguousarray(s.charge, np.float64),
+                    nsi=np.ascontiguousarray(s.nsi, np.int32),
+                    ijtype=np.ascontiguousarray(s.ijtype.ravel(), np.int32),
+                    vdw_kind=np.ascontiguousarray(s.vdw_kind, np.int32))
+        for k in ("vdw_eps4", "vdw_sig6", "vdw_a", "vdw_rhoi", "vdw_c"):
+            keep[k] = np.ascontiguousarray(getattr(s, k), np.float64)
+        if constraints is None:
+            constraints = s.constraints
+        if angles is None:
+            angles = s.angles
+    else:
+        ijtype, kind, eps4, sig6, A, rhoi, Cc = s.vdw_tables()
+        keep = dict(site_type=np.ascontiguousarray(s.site_types(), np.int32),
+                    charge=np.ascontiguousarray(s.charges(), np.float64),
+                    nsi=np.ascontiguousarray(s.nsi(), np.int32),
+                    ijtype=np.ascontiguousarray(ijtype.ravel(), np.int32),
+                    vdw_kind=kind, vdw_eps4=eps4, vdw_sig6=sig6, vdw_a=A, vdw_rhoi=rhoi, vdw_c=Cc)
     top = Topology()
     top.n_total, top.n_move, top.n_rigid_sites = s.ntota, s.nmove, s.nriga
-    top.n_liquid_sites, top.n_rig, top.n_types, top.n_ltype = s.nliqa, s.nrig, s.nit, s.nltype
+    top.n_liquid_sites, top.n_rig, top.n_types = s.nliqa, s.nrig, s.nit
+    top.n_ltype = s.n_ltype if derived else s.nltype
     top.site_type = _p(keep["site_type"], C.c_int32)
     top.charge = _p(keep["charge"], C.c_double)
     top.nsi = _p(keep["nsi"], C.c_int32)
@@ -126,10 +142,15 @@ def build_abi(system, precision: int = FP64, const_mode: int = CONST_REFERENCE,
             keep[k] = np.ascontiguousarray(v, t)
             setattr(top, k, _p(keep[k], C.c_int32 if t is np.int32 else C.c_double))
         top.n_angles = len(keep["ang_i"])
-    cr = s.crystal
-    top.wall_lw = cr.lw
-    top.wall_cell[:] = list(cr.cell)
-    top.wall_nss[:] = [cr.sites[0][1], cr.sites[1][1]]
+    if derived:
+        top.wall_lw = s.wall_lw
+        top.wall_cell[:] = list(s.wall_cell)
+        top.wall_nss[:] = list(s.wall_nss)
+    else:
+        cr = s.crystal
+        top.wall_lw = cr.lw
+        top.wall_cell[:] = list(cr.cell)
+        top.wall_nss[:] = [cr.sites[0][1], cr.sites[1][1]]
     par = Params()
     par.box[:] = list(s.box)
     par.rcut, par.skin = s.rcut, s.skin
@@ -148,6 +169,8 @@ def build_abi(system, precision: int = FP64, const_mode: int = CONST_REFERENCE,
 
 
 def positions(system):
+    if hasattr(system, "positions"):
+        return system.positions()
     pos = system.site_positions()
     x = np.ascontiguousarray(pos[:, 0])
     y = np.ascontiguousarray(pos[:, 1])

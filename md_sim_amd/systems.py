@@ -206,7 +206,8 @@ class System:
                 if k == "LeJo":
                     kind[ijt - 1] = 1
                     eps4[ijt - 1] = 4.0 * p[1]
-                    sig6[ijt - 1] = p[0] ** 6
+                    x = p[0]
+                    sig6[ijt - 1] = x * x * x * x * x * x   # sig**6 as the reference evaluates it
                 elif k == "Buck":
                     kind[ijt - 1] = 2
                     A[ijt - 1] = p[0]

@@ -23,6 +23,7 @@ program ref_harness
     real*8, allocatable :: f1x(:), f1y(:), f1z(:), f2x(:), f2y(:), f2z(:)
     real*8, allocatable :: kak(:), kre(:), kim(:), kvec(:)
     integer*8, allocatable :: ibuf(:)
+    real*8, allocatable :: cbuf(:)
 
     nproc = 1
     me = 0
@@ -124,7 +125,7 @@ program ref_harness
 
     u = 11
     open(u, file=trim(outname), access='stream', form='unformatted', status='replace')
-    allocate(ibuf(max(ntota, int(ipack,8), int(wpack,8), 10_8)))
+    allocate(ibuf(max(ntota, int(ipack,8), int(wpack,8), nit*nit, nijt, 10_8)))
     ibuf(1:10) = (/ ntota, nmove, nriga, nliqa, nrig, nwall, nijt, int(ipack,8), int(wpack,8), nsel /)
     call put_i(u, 'counts', ibuf, 10_8)
     call put_r(u, 'x0', x0, ntota)
@@ -181,6 +182,64 @@ program ref_harness
     call put_r(u, 'fwsz', fwsz, 3_8)
     ibuf(1:3) = (/ int(klmax,8), int(kmmax,8), int(knmax,8) /)
     call put_i(u, 'kmaxes', ibuf, 3_8)
+
+    ! the derived topology the force path reads (after sysdef_init's transforms), so a
+    ! system can be rebuilt from the dump alone
+    ibuf(1:4) = (/ nit, nltype, ntype, nwtype /)
+    call put_i(u, 'types', ibuf, 4_8)
+    ibuf(1:nit) = nsi(1:nit)
+    call put_i(u, 'nsi', ibuf, nit)
+    do i = 1, nit
+        ibuf((i-1)*nit+1:i*nit) = ijtype(i, 1:nit)
+    end do
+    call put_i(u, 'ijtype', ibuf, nit*nit)
+    ibuf(1:nijt) = vdwnum(1:nijt)
+    call put_i(u, 'vdwnum', ibuf, nijt)
+    call put_r(u, 'eps', eps, nijt)
+    call put_r(u, 'sig', sig, nijt)
+    call put_r(u, 'aa', aa, nijt)
+    call put_r(u, 'rhoi', rhoi, nijt)
+    call put_r(u, 'cc', cc, nijt)
+    ! constraints, molecule by molecule (src/sysdef.f90:193-204)
+    n = 0
+    do i = 1, ncons
+        n = n + nconst(ictype(i))
+    end do
+    allocate(cbuf(3*max(n, 1_8)))
+    kcnt = 0
+    do i = 1, ncons
+        do ix = 1, nconst(ictype(i))
+            kcnt = kcnt + 1
+            cbuf(kcnt) = dble(constra(ix, i))
+            cbuf(n + kcnt) = dble(constrb(ix, i))
+            cbuf(2*n + kcnt) = length(ictype(i), ix)
+        end do
+    end do
+    call put_r(u, 'constraints', cbuf, 3*n)
+    deallocate(cbuf)
+    ! angles, group by group (src/forcem.f90:571-576); theta in radians
+    n = 0
+    do i = 1, nang
+        n = n + nangt(iatype(i))
+    end do
+    allocate(cbuf(5*max(n, 1_8)))
+    kcnt = 0
+    do i = 1, nang
+        do ix = 1, nangt(iatype(i))
+            kcnt = kcnt + 1
+            cbuf(kcnt) = dble(ang1(ix, i))
+            cbuf(n + kcnt) = dble(ang2(ix, i))
+            cbuf(2*n + kcnt) = dble(ang3(ix, i))
+            cbuf(3*n + kcnt) = kt(iatype(i), ix)
+            cbuf(4*n + kcnt) = theta(iatype(i), ix)
+        end do
+    end do
+    call put_r(u, 'angles', cbuf, 5*n)
+    deallocate(cbuf)
+    allocate(cbuf(12))
+    cbuf(1:12) = (/ lx, ly, lz, lw, cx, cy, cz, dble(nss(ntype, 1)), dble(nss(ntype, 2)), rcut, skin, dble(kmax) /)
+    call put_r(u, 'geometry', cbuf, 12_8)
+    deallocate(cbuf)
     close(u)
 
     ! timing: repeated force evaluations on the same configuration (no list rebuild)

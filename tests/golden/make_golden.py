@@ -8,7 +8,10 @@ stores its inputs and outputs:
 * pore_small.npz - 96 waters + 8 mobile ions over a 4-layer rock-salt slab, slab variant
                    (src/forcem.f90) with slab correction and wall tail correction;
 * slab2048.npz   - 2048 waters on a 4608-ion slab (the 16k workload's smaller sibling):
-                   energies, pair-list checksums and every 7th site force.
+                   energies, pair-list checksums and every 7th site force;
+* sample_rutile.npz - the reference's own sample input (src/sim/y.*, surface OH groups with
+                   constraints and angles on rutile): input text, the derived topology the
+                   reference builds from it, and its outputs.
 
 Usage:  python tests/golden/make_golden.py
 """
@@ -36,7 +39,55 @@ def pair_digest(i, j):
     return h.hexdigest()
 
 
+SAMPLE_DIR = "/root/reference/src/sim"
+SAMPLE_FILES = ("y.fld", "y.top", "y.cfg", "y.inp")
+TOPOLOGY = ("types", "nsi", "ijtype", "vdwnum", "eps", "sig", "aa", "rhoi", "cc", "constraints", "angles",
+            "geometry")
+
+
+def sample_case():
+    """The reference's own sample system (src/sim/y.*: 258 SPC waters and 54 surface OH groups
+    with constraints and Ti-O-H angles on a 3189-site rutile crystal), run through the
+    harness with the run parameters of its y.inp.  The fixture keeps the input files' text
+    (for the reader tests), the derived topology the reference builds from them, and its
+    outputs."""
+    import re
+    import shutil
+    import subprocess
+    import tempfile
+    from oracle.refrun import REF_DIR, parse_dump
+    d = tempfile.mkdtemp(prefix="mdsample_")
+    for f in SAMPLE_FILES:
+        shutil.copy(os.path.join(SAMPLE_DIR, f), d)
+    inp = open(os.path.join(SAMPLE_DIR, "y.inp")).read()
+    val = {m.group(1): m.group(2).split() for m in re.finditer(r"^\s*(\w+):\s*(.*)$", inp, re.M)}
+    dt = float(val["time"][2])
+    with open(os.path.join(d, "harness.inp"), "w") as f:
+        f.write("y\n%s %s %s %s %s\n" % (dt, val["rcut"][0], val["skin"][0], val["rdel"][0], val["kmax"][0]))
+    r = subprocess.run([os.path.join(REF_DIR, "ref_slab"), os.path.join(d, "dump.bin")], cwd=d,
+                       capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        raise RuntimeError(r.stdout[-2000:] + r.stderr[-2000:])
+    dump = parse_dump(os.path.join(d, "dump.bin"))
+    out = {k: dump[k] for k in SCALARS}
+    for k in ("elj", "eqq", "fwsx", "fwsy", "fwsz", "kmaxes", "counts", "x0", "y0", "z0", "qs", "itype", "cx0",
+              "cy0", "cz0", "f_site_x", "f_site_y", "f_site_z", "fcx", "fcy", "fcz", "tx", "ty", "tz",
+              "k_ak", "k_sre", "k_sim", "k_vec") + TOPOLOGY:
+        out[k] = dump[k]
+    out["list_digest"] = np.array(pair_digest(dump["list1"], dump["list2"]))
+    out["wlist_digest"] = np.array(pair_digest(dump["listw1"], dump["listw2"]))
+    out["n_list"] = np.array([len(dump["list1"]), len(dump["listw1"])])
+    for f in SAMPLE_FILES:
+        out["file_" + f.replace(".", "_")] = np.frombuffer(open(os.path.join(SAMPLE_DIR, f), "rb").read(), np.uint8)
+    shutil.rmtree(d, ignore_errors=True)
+    path = os.path.join(HERE, "sample_rutile.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, os.path.getsize(path), "bytes")
+
+
 def main():
+    if os.path.isdir(SAMPLE_DIR):
+        sample_case()
     for name in ("bulk216", "pore_small", "slab2048"):
         s = systems.config(name)
         d = run_reference(s)

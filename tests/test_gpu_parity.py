@@ -87,3 +87,27 @@ def test_engine_matches_oracle_slab2048(oracle_lib):
     g = golden("slab2048")
     for a, k in enumerate("xyz"):
         assert rel_err(r["f_site"][a][g["sel"]], g["f_site_" + k]) < TOL
+
+
+def test_engine_matches_reference_sample():
+    """The reference's own sample system (surface OH groups: constraints and angles on
+    rutile) through the HIP engine, at the FP64 bar; the pair lists are the reference's."""
+    from md_sim_amd import refio
+    from md_sim_amd.engine import ForceEngine
+    from golden.make_golden import pair_digest
+    print("case sample_rutile", flush=True)
+    g = golden("sample_rutile")
+    # driven from the reference's own input files, read by refio
+    txt = lambda k: bytes(g["file_" + k]).decode()
+    td = refio.load_reference_inputs(txt("y_fld"), txt("y_top"), txt("y_cfg"), txt("y_inp"), name="sample_rutile")
+    e = ForceEngine(td)
+    r = e.force_get(*td.positions())
+    for a, k in enumerate("xyz"):
+        assert rel_err(r["f_site"][a], g["f_site_" + k]) < 1e-10
+    assert rel_err(r["torque"][1], g["ty"]) < 1e-10 and rel_err(r["f_com"][0], g["fcx"]) < 1e-10
+    for k, gk in [("e_recip", "uewa"), ("e_slab", "urf"), ("e_wall_lr", "ulr"), ("e_conf", "uconf"),
+                  ("press_wall", "press")]:
+        assert abs(r[k] - g[gk].item()) <= 1e-10 * max(abs(g[gk].item()), 1.0), k
+    l1, l2, w1, w2 = e.pairs()
+    assert pair_digest(l1 + 1, l2 + 1) == str(g["list_digest"])
+    assert pair_digest(w1 + 1, w2 + 1) == str(g["wlist_digest"])
