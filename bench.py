@@ -51,6 +51,7 @@ def parse():
     ap.add_argument("--cpu-reps", type=int, default=3, help="force evaluations in the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--virial", action="store_true", help="also accumulate the pair virial")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--amplitude", type=float, default=0.25,
                     help="rms per-axis oscillation amplitude of a molecule (A)")
     ap.add_argument("--period", type=float, default=80.0, help="oscillation period (steps)")
@@ -168,6 +169,7 @@ def main():
     prec = abi.FP32_ACC64 if args.precision == "fp32" else abi.FP64
     # the reference computes no virial; it is an opt-in extra of this engine
     eng = ForceEngine(s, precision=prec, virial=args.virial)
+    shapes = eng.autotune(reps=10) if not args.no_autotune else None
     sec, first, last, stages = _run(eng, frames, cframes, args)
     kv, ak, _ = eng.kvectors()
     eng.close()
@@ -177,12 +179,13 @@ def main():
     fp64 = None
     if prec != abi.FP64:
         e64 = ForceEngine(s, precision=abi.FP64, virial=args.virial)
+        shapes64 = e64.autotune(reps=10) if not args.no_autotune else None
         sec64, _, last64, stages64 = _run(e64, frames, cframes, args)
         e64.close()
         f32, f64 = last["f_site"], last64["f_site"]
         err = float(np.sqrt(((f32 - f64) ** 2).mean()) / np.sqrt((f64 ** 2).mean()))
         fp64 = {"steps_per_s": args.steps / sec64, "ms_per_step": sec64 / args.steps * 1e3,
-                "stages_ms": stages64,
+                "stages_ms": stages64, "launch_shapes": shapes64,
                 "force_rms_rel_err_vs_fp64": err,
                 "e_conf_rel_diff_vs_fp64": abs(last["e_conf"] - last64["e_conf"]) / abs(last64["e_conf"])}
 
@@ -243,6 +246,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "stages_ms": stages,
+        "launch_shapes": shapes,
         "list_builds_timed": last["n_list_builds"] - first["n_list_builds"],
         "e_conf_last": last["e_conf"],
     }

@@ -188,6 +188,21 @@ const char *mdf_last_error(void);
    step over the steps since mdf_reset_stage_times: the stages (from steps run with stage
    timing on) followed by "step", the whole step measured in both modes.  cap: capacity
    of names/ms; n out: number of entries.                                              */
+/* Pick the launch shapes of this handle's kernels by timing candidates on its current
+   state: lanes per lead (8/16/32) and loop form of the pair kernel, blocks per launch of the
+   structure-factor and back-projection kernels.  Each candidate runs reps + 1 times; the
+   fastest is kept for later steps.  chosen (optional, 4 entries) receives lanes per lead,
+   rolled (0/1), structure-factor blocks and back-projection blocks.  Device results of the
+   last step are overwritten: call between steps, before reading them again.  No
+   counterpart in the reference (its loops have no launch shape). */
+int mdf_autotune(mdf_system *sys, int32_t reps, int32_t *chosen);
+
+/* Set the launch shapes mdf_autotune chooses, by hand: pair lanes per lead (8, 16, 32),
+   rolled site-pair loop (0/1), structure-factor and back-projection blocks per launch.
+   No counterpart in the reference. */
+int mdf_set_launch_shapes(mdf_system *sys, int32_t pair_lanes, int32_t pair_rolled, int32_t sf_blocks,
+                          int32_t bp_blocks);
+
 int mdf_set_stage_timing(mdf_system *sys, int32_t on);
 int mdf_stage_times(mdf_system *sys, const char **names, float *ms, int32_t cap, int32_t *n);
 int mdf_reset_stage_times(mdf_system *sys);

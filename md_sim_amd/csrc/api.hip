@@ -182,6 +182,7 @@ struct mdf_system {
     hipGraphExec_t graph[NRING] = {};
     bool use_graphs = true;
     bool pair_rolled = false;   // site pairs of a molecule pair as a rolled loop (fewer registers)
+    int pair_g = 16;            // lanes per lead in the pair kernel (8, 16 or 32)
     // launch-shape targets of the reciprocal kernels (blocks per launch; the row groups follow)
     int sf_target_blocks = 4096, bp_target_blocks = 1024;
     double stage_sum[N_STAGES] = {};
@@ -351,6 +352,23 @@ static void geometry(mdf_system *s) {
     s->colper = std::max(1, (s->n_lead + s->n_colblk - 1) / s->n_colblk);
 }
 
+static void drop_graphs(mdf_system *s);
+
+// launch shapes of the reciprocal kernels for the current targets: row groups, their device
+// table and the partial-force buffer they need; captured graphs hold the old shapes
+static int apply_geometry(mdf_system *s) {
+    geometry(s);
+    s->d_bp_row_start.release();
+    HIPCHK(upload(s->d_bp_row_start, s->bp_row_start.data(), s->bp_row_start.size()));
+    const size_t need = (size_t)s->n_rg * 3 * std::max(s->top.n_move, 1);
+    if (s->fpart.n < need) {
+        s->fpart.release();
+        HIPCHK(s->fpart.alloc(need));
+    }
+    drop_graphs(s);
+    return MDF_OK;
+}
+
 static void drop_graphs(mdf_system *s) {
     for (int r = 0; r < mdf_system::NRING; r++)
         if (s->graph[r]) { (void)hipGraphExecDestroy(s->graph[r]); s->graph[r] = nullptr; }
@@ -382,55 +400,96 @@ static void launch_lists(mdf_system *s, int force_rebuild) {
     k_snapshot<<<std::max(1, (std::max(s->top.n_rig, s->top.n_move) + 255) / 256), 256, 0, st>>>(S);
 }
 
+// phase factors of the reciprocal kernels: 2 pi / L (widened pi) and the same over pi
+struct RecipArgs {
+    double cux, cuy, cuz, pipix, pipiy, pipiz;
+};
+static RecipArgs recip_args(const mdf_system *s) {
+    RecipArgs a;
+    a.cux = 2.0 * s->c.pi / s->par.box[0] / M_PI;
+    a.cuy = 2.0 * s->c.pi / s->par.box[1] / M_PI;
+    a.cuz = 2.0 * s->c.pi / s->par.box[2] / M_PI;
+    a.pipix = 2.0 * s->c.pi / s->par.box[0];
+    a.pipiy = 2.0 * s->c.pi / s->par.box[1];
+    a.pipiz = 2.0 * s->c.pi / s->par.box[2];
+    return a;
+}
+
+template <int G>
+static void launch_pair_g(mdf_system *s, bool f32, bool vir, bool rolled) {
+    const int blocks = std::max(1, (s->n_lead + (256 / G) - 1) / (256 / G));
+    hipStream_t st = s->stream;
+    DevSys &S = s->S;
+#define MDF_PAIR(R, V, RO) k_pair<R, G, V, RO><<<blocks, 256, 0, st>>>(S)
+    if (f32) {
+        if (vir) { if (rolled) MDF_PAIR(float, true, true); else MDF_PAIR(float, true, false); }
+        else { if (rolled) MDF_PAIR(float, false, true); else MDF_PAIR(float, false, false); }
+    } else {
+        if (vir) { if (rolled) MDF_PAIR(double, true, true); else MDF_PAIR(double, true, false); }
+        else { if (rolled) MDF_PAIR(double, false, true); else MDF_PAIR(double, false, false); }
+    }
+#undef MDF_PAIR
+}
+
+// real-space pair kernel with the handle's lanes-per-lead and loop form
+static void launch_pair(mdf_system *s) {
+    const bool f32 = s->par.precision == MDF_FP32_ACC64, vir = s->par.virial != 0;
+    switch (s->pair_g) {
+        case 8: launch_pair_g<8>(s, f32, vir, s->pair_rolled); break;
+        case 32: launch_pair_g<32>(s, f32, vir, s->pair_rolled); break;
+        default: launch_pair_g<16>(s, f32, vir, s->pair_rolled); break;
+    }
+}
+
+static void launch_sf(mdf_system *s) {
+    const int nk = (int)s->ak.size(), nrows = (int)s->rows.size(), nm = s->top.n_move;
+    if (nk == 0 || nm == 0) return;
+    const RecipArgs a = recip_args(s);
+    DevSys &S = s->S;
+    dim3 g1(s->n_sb, s->n_cg);
+    if (s->par.precision == MDF_FP32_ACC64)
+        k_sf<float, 4><<<g1, SF_BLOCK, 0, s->stream>>>(S.x, S.y, S.z, S.q, nm, a.cux, a.cuy, a.cuz, s->d_rows.p,
+                                                       nrows, s->cpb, s->sf_partial.p, nk);
+    else
+        k_sf<double, 4><<<g1, SF_BLOCK, 0, s->stream>>>(S.x, S.y, S.z, S.q, nm, a.cux, a.cuy, a.cuz, s->d_rows.p,
+                                                        nrows, s->cpb, s->sf_partial.p, nk);
+}
+
+static void launch_bp(mdf_system *s) {
+    const int nk = (int)s->ak.size(), nm = s->top.n_move;
+    if (nk == 0 || nm == 0) return;
+    const RecipArgs a = recip_args(s);
+    DevSys &S = s->S;
+    dim3 g2(std::max(1, (nm + 256 * BP_NS - 1) / (256 * BP_NS)), s->n_rg);
+    if (s->par.precision == MDF_FP32_ACC64)
+        k_bp<float, BP_NS><<<g2, 256, 0, s->stream>>>(S.x, S.y, S.z, S.q, nm, a.cux, a.cuy, a.cuz, s->d_rows.p,
+                                                      s->d_bp_row_start.p, s->d_t.p, a.pipix, a.pipiy, a.pipiz,
+                                                      s->fpart.p);
+    else
+        k_bp<double, BP_NS><<<g2, 256, 0, s->stream>>>(S.x, S.y, S.z, S.q, nm, a.cux, a.cuy, a.cuz, s->d_rows.p,
+                                                       s->d_bp_row_start.p, s->d_t.p, a.pipix, a.pipiy, a.pipiz,
+                                                       s->fpart.p);
+}
+
 static void launch_forces(mdf_system *s) {
     DevSys &S = s->S;
     hipStream_t st = s->stream;
-    const bool f32 = s->par.precision == MDF_FP32_ACC64;
     const int nm = s->top.n_move;
-    const double cux = 2.0 * s->c.pi / s->par.box[0] / M_PI;
-    const double cuy = 2.0 * s->c.pi / s->par.box[1] / M_PI;
-    const double cuz = 2.0 * s->c.pi / s->par.box[2] / M_PI;
-    const double pipix = 2.0 * s->c.pi / s->par.box[0], pipiy = 2.0 * s->c.pi / s->par.box[1],
-                 pipiz = 2.0 * s->c.pi / s->par.box[2];
+    const RecipArgs ra = recip_args(s);
     if (s->ev) (void)hipEventRecord(s->ev[1], st);
-    {
-        const int G = 16;
-        int blocks = std::max(1, (s->n_lead + (256 / G) - 1) / (256 / G));
-        const bool vir = s->par.virial != 0, rolled = s->pair_rolled;
-#define MDF_PAIR(R, V, RO) k_pair<R, 16, V, RO><<<blocks, 256, 0, st>>>(S)
-        if (f32) {
-            if (vir) { if (rolled) MDF_PAIR(float, true, true); else MDF_PAIR(float, true, false); }
-            else { if (rolled) MDF_PAIR(float, false, true); else MDF_PAIR(float, false, false); }
-        } else {
-            if (vir) { if (rolled) MDF_PAIR(double, true, true); else MDF_PAIR(double, true, false); }
-            else { if (rolled) MDF_PAIR(double, false, true); else MDF_PAIR(double, false, false); }
-        }
-#undef MDF_PAIR
-    }
+    launch_pair(s);
     if (s->ev) (void)hipEventRecord(s->ev[2], st);
-    int nk = (int)s->ak.size(), nrows = (int)s->rows.size();
-    if (nk > 0 && nm > 0) {
-        dim3 g1(s->n_sb, s->n_cg);
-        if (f32) k_sf<float, 4><<<g1, SF_BLOCK, 0, st>>>(S.x, S.y, S.z, S.q, nm, cux, cuy, cuz, s->d_rows.p, nrows,
-                                                           s->cpb, s->sf_partial.p, nk);
-        else k_sf<double, 4><<<g1, SF_BLOCK, 0, st>>>(S.x, S.y, S.z, S.q, nm, cux, cuy, cuz, s->d_rows.p, nrows,
-                                                        s->cpb, s->sf_partial.p, nk);
-    }
+    int nk = (int)s->ak.size();
+    launch_sf(s);
     if (s->ev) (void)hipEventRecord(s->ev[3], st);
     if (nk > 0) {
         double lamb = 1.0 / (4.0 * S.alfa * S.alfa);
         k_sf_reduce<<<s->n_kblk, 256, 0, st>>>(s->sf_partial.p, nm > 0 ? s->n_sb : 0, nk, s->d_swall.p, s->d_ak.p,
-                                               s->d_kvec.p, pipix, pipiy, pipiz, lamb, s->d_s.p, s->d_t.p,
+                                               s->d_kvec.p, ra.pipix, ra.pipiy, ra.pipiz, lamb, s->d_s.p, s->d_t.p,
                                                s->kpart.p);
     }
     if (s->ev) (void)hipEventRecord(s->ev[4], st);
-    if (nk > 0 && nm > 0) {
-        dim3 g2(std::max(1, (nm + 256 * BP_NS - 1) / (256 * BP_NS)), s->n_rg);
-        if (f32) k_bp<float, BP_NS><<<g2, 256, 0, st>>>(S.x, S.y, S.z, S.q, nm, cux, cuy, cuz, s->d_rows.p,
-                                                 s->d_bp_row_start.p, s->d_t.p, pipix, pipiy, pipiz, s->fpart.p);
-        else k_bp<double, BP_NS><<<g2, 256, 0, st>>>(S.x, S.y, S.z, S.q, nm, cux, cuy, cuz, s->d_rows.p,
-                                              s->d_bp_row_start.p, s->d_t.p, pipix, pipiy, pipiz, s->fpart.p);
-    }
+    launch_bp(s);
     if (s->ev) (void)hipEventRecord(s->ev[5], st);
     k_epilogue<<<s->n_epblk, EPI_BLOCK, 0, st>>>(S, s->fpart.p, (nk > 0 && nm > 0) ? s->n_rg : 0, s->d_wl.p,
                                                  s->par.ew3dc, s->crf0, s->dipz.p, s->eppart.p, s->fcom.p,
@@ -1137,6 +1196,90 @@ int mdf_stage_times(mdf_system *s, const char **names, float *ms, int32_t cap, i
         if (ms) ms[N_STAGES] = s->step_steps > 0 ? (float)(s->step_sum / (double)s->step_steps) : 0.f;
     }
     return MDF_OK;
+}
+
+int mdf_autotune(mdf_system *s, int32_t reps, int32_t *chosen) {
+    if (!s) return fail(MDF_ERR_ARG, "null handle");
+    reps = std::max(reps, 1);
+    HIPCHK(hipSetDevice(s->device));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    drain_all(s);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    int rc = MDF_OK;
+    // mean time of one launch of the given stage, after one untimed launch
+    auto timed = [&](auto launch, float *ms) -> int {
+        launch();
+        HIPCHK(hipEventRecord(e0, s->stream));
+        for (int r = 0; r < reps; r++) launch();
+        HIPCHK(hipEventRecord(e1, s->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float t = 0.0f;
+        HIPCHK(hipEventElapsedTime(&t, e0, e1));
+        HIPCHK(hipGetLastError());
+        *ms = t / reps;
+        return MDF_OK;
+    };
+    float best = 1e30f, t = 0.0f;
+    int best_g = s->pair_g;
+    bool best_rolled = s->pair_rolled;
+    for (int g : {8, 16, 32})
+        for (int ro = 0; ro < 2; ro++) {
+            s->pair_g = g;
+            s->pair_rolled = ro != 0;
+            if ((rc = timed([&] { launch_pair(s); }, &t)) != MDF_OK) goto done;
+            if (t < best) { best = t; best_g = g; best_rolled = ro != 0; }
+        }
+    s->pair_g = best_g;
+    s->pair_rolled = best_rolled;
+    {
+        best = 1e30f;
+        int best_sf = s->sf_target_blocks;
+        for (int tb : {1024, 2048, 4096, 8192}) {
+            s->sf_target_blocks = tb;
+            geometry(s);
+            if ((rc = timed([&] { launch_sf(s); }, &t)) != MDF_OK) goto done;
+            if (t < best) { best = t; best_sf = tb; }
+        }
+        s->sf_target_blocks = best_sf;
+        best = 1e30f;
+        int best_bp = s->bp_target_blocks;
+        for (int tb : {512, 1024, 2048}) {
+            s->bp_target_blocks = tb;
+            if ((rc = apply_geometry(s)) != MDF_OK) goto done;
+            if ((rc = timed([&] { launch_bp(s); }, &t)) != MDF_OK) goto done;
+            if (t < best) { best = t; best_bp = tb; }
+        }
+        s->bp_target_blocks = best_bp;
+    }
+    rc = apply_geometry(s);
+    if (chosen) {
+        chosen[0] = s->pair_g;
+        chosen[1] = s->pair_rolled ? 1 : 0;
+        chosen[2] = s->sf_target_blocks;
+        chosen[3] = s->bp_target_blocks;
+    }
+done:
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
+int mdf_set_launch_shapes(mdf_system *s, int32_t pair_lanes, int32_t pair_rolled, int32_t sf_blocks,
+                          int32_t bp_blocks) {
+    if (!s) return fail(MDF_ERR_ARG, "null handle");
+    if (pair_lanes != 8 && pair_lanes != 16 && pair_lanes != 32)
+        return fail(MDF_ERR_ARG, "pair lanes per lead must be 8, 16 or 32 (got %d)", pair_lanes);
+    if (sf_blocks <= 0 || bp_blocks <= 0) return fail(MDF_ERR_ARG, "block targets must be positive");
+    HIPCHK(hipSetDevice(s->device));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    drain_all(s);
+    s->pair_g = pair_lanes;
+    s->pair_rolled = pair_rolled != 0;
+    s->sf_target_blocks = sf_blocks;
+    s->bp_target_blocks = bp_blocks;
+    return apply_geometry(s);
 }
 
 int mdf_set_stage_timing(mdf_system *s, int32_t on) {

@@ -321,14 +321,19 @@ __global__ __launch_bounds__(256) void k_pair(DevSys S) {
     }
 }
 
-template __global__ void k_pair<double, 16, true, false>(DevSys);
-template __global__ void k_pair<float, 16, true, false>(DevSys);
-template __global__ void k_pair<double, 16, false, false>(DevSys);
-template __global__ void k_pair<float, 16, false, false>(DevSys);
-template __global__ void k_pair<double, 16, true, true>(DevSys);
-template __global__ void k_pair<float, 16, true, true>(DevSys);
-template __global__ void k_pair<double, 16, false, true>(DevSys);
-template __global__ void k_pair<float, 16, false, true>(DevSys);
+#define MDF_PAIR_INST(G)                                            \
+    template __global__ void k_pair<double, G, true, false>(DevSys);  \
+    template __global__ void k_pair<float, G, true, false>(DevSys);   \
+    template __global__ void k_pair<double, G, false, false>(DevSys); \
+    template __global__ void k_pair<float, G, false, false>(DevSys);  \
+    template __global__ void k_pair<double, G, true, true>(DevSys);   \
+    template __global__ void k_pair<float, G, true, true>(DevSys);    \
+    template __global__ void k_pair<double, G, false, true>(DevSys);  \
+    template __global__ void k_pair<float, G, false, true>(DevSys);
+MDF_PAIR_INST(8)
+MDF_PAIR_INST(16)
+MDF_PAIR_INST(32)
+#undef MDF_PAIR_INST
 
 // forces of moving sites that are neither leads nor lead-carried are zero before the
 // reciprocal part: clear every moving-site force the pair kernel does not write

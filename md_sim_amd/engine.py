@@ -52,7 +52,8 @@ def lib() -> C.CDLL:
         L.mdf_last_error.restype = C.c_char_p
         for fn in ("mdf_create", "mdf_compute", "mdf_compute_device", "mdf_read_outputs", "mdf_get_pairs",
                    "mdf_get_kvectors", "mdf_get_structure_factor", "mdf_stage_times", "mdf_device_results",
-                   "mdf_zero", "mdf_reset_stage_times", "mdf_set_stage_timing"):
+                   "mdf_zero", "mdf_reset_stage_times", "mdf_set_stage_timing", "mdf_autotune",
+                   "mdf_set_launch_shapes"):
             getattr(L, fn).restype = C.c_int
         L.mdf_destroy.restype = None
         _lib = L
@@ -133,6 +134,19 @@ class ForceEngine:
         n = C.c_int32()
         _check(lib().mdf_stage_times(self._h, names, ms, 16, C.byref(n)))
         return {names[k].decode(): float(ms[k]) for k in range(n.value)}
+
+    def autotune(self, reps: int = 10) -> dict:
+        """Time candidate launch shapes on the current state and keep the fastest; returns
+        the choice (pair lanes per lead, rolled loop, structure-factor and back-projection
+        blocks per launch)."""
+        ch = (C.c_int32 * 4)()
+        _check(lib().mdf_autotune(self._h, int(reps), ch))
+        return dict(pair_lanes=ch[0], pair_rolled=bool(ch[1]), sf_blocks=ch[2], bp_blocks=ch[3])
+
+    def set_launch_shapes(self, pair_lanes: int = 16, pair_rolled: bool = False, sf_blocks: int = 4096,
+                          bp_blocks: int = 1024):
+        _check(lib().mdf_set_launch_shapes(self._h, int(pair_lanes), int(bool(pair_rolled)), int(sf_blocks),
+                                           int(bp_blocks)))
 
     def set_stage_timing(self, on: bool):
         _check(lib().mdf_set_stage_timing(self._h, int(bool(on))))

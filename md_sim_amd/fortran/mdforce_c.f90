@@ -66,6 +66,13 @@ module mdforce_c
             type(c_ptr), value :: sys
         end function mdf_zero
 
+        integer(c_int) function mdf_autotune(sys, reps, chosen) bind(C, name='mdf_autotune')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: sys
+            integer(c_int), value :: reps
+            integer(c_int), intent(out) :: chosen(4)
+        end function mdf_autotune
+
         subroutine mdf_destroy(sys) bind(C, name='mdf_destroy')
             import :: c_ptr
             type(c_ptr), value :: sys

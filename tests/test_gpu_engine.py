@@ -126,3 +126,31 @@ def test_deterministic_repeat():
     b = e.force_get(*pos)
     assert np.array_equal(a["f_site"], b["f_site"]) and a["e_conf"] == b["e_conf"]
     assert np.array_equal(a["e_vdw"], b["e_vdw"]) and np.array_equal(a["virial"], b["virial"])
+
+
+@pytest.mark.parametrize("shape", [(8, False, 1024, 512), (32, False, 8192, 2048), (16, True, 2048, 1024),
+                                   (32, True, 4096, 1024)])
+def test_launch_shapes_keep_parity(oracle_lib, shape):
+    """Every launch shape the autotuner can pick gives the FP64 result."""
+    _say("case launch shape", shape)
+    s = systems.config("slab2048")
+    pos = abi.positions(s)
+    e = _engine(s)
+    e.set_launch_shapes(*shape)
+    r = e.force_get(*pos)
+    ro = oracle_lib.Oracle(s).compute(*pos)
+    assert rel_err(r["f_site"], ro["f_site"]) < 1e-10
+    assert abs(r["e_conf"] - ro["e_conf"]) < 1e-10 * abs(ro["e_conf"])
+
+
+def test_autotune_then_parity(oracle_lib):
+    _say("case autotune slab2048")
+    s = systems.config("slab2048")
+    pos = abi.positions(s)
+    e = _engine(s)
+    ch = e.autotune(reps=3)
+    _say("  chosen", ch)
+    assert ch["pair_lanes"] in (8, 16, 32) and ch["sf_blocks"] > 0 and ch["bp_blocks"] > 0
+    r = e.force_get(*pos)
+    ro = oracle_lib.Oracle(s).compute(*pos)
+    assert rel_err(r["f_site"], ro["f_site"]) < 1e-10
