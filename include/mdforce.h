@@ -149,7 +149,8 @@ int mdf_compute(mdf_system *sys,
 
 /* force_get on device-resident inputs (n_move and n_rig entries).  Enqueued on the
    handle's stream and not waited for; results stay on the device until
-   mdf_read_outputs.  stream may be NULL for the handle's own stream.                  */
+   mdf_read_outputs.  stream is the stream that produced the inputs (a hipStream_t): the
+   step starts after the work enqueued on it so far; NULL means the null stream.      */
 int mdf_compute_device(mdf_system *sys,
                        const double *d_x, const double *d_y, const double *d_z,
                        const double *d_cx, const double *d_cy, const double *d_cz,
@@ -188,6 +189,29 @@ const char *mdf_last_error(void);
    step over the steps since mdf_reset_stage_times: the stages (from steps run with stage
    timing on) followed by "step", the whole step measured in both modes.  cap: capacity
    of names/ms; n out: number of entries.                                              */
+/* ---- batches of independent systems (no counterpart in the reference) ----------------
+   Many systems - replicas of uneven size, each with its own box, lists and k-set - stepped
+   together: every stage of the step is one launch over all of them, and the whole step is
+   one replayed graph.  The systems stay owned by the caller and must not be stepped on their
+   own while the batch is in flight.  All must share device, precision and the virial switch;
+   the batch uses the first system's pair lanes per lead. */
+typedef struct mdf_batch mdf_batch;
+
+int mdf_batch_create(mdf_system *const *systems, int32_t n, mdf_batch **out);
+
+/* Enqueue one step of every system on device-resident positions: x[i] .. cz[i] are system
+   i's arrays, as for mdf_compute_device (cx/cy/cz may be NULL when no system has rigid
+   molecules); stream as for mdf_compute_device.  Returns without waiting. */
+int mdf_batch_compute_device(mdf_batch *b, const double *const *x, const double *const *y,
+                             const double *const *z, const double *const *cx, const double *const *cy,
+                             const double *const *cz, void *stream);
+
+/* Wait for the step and copy every system's results into outs[0 .. n-1] (as
+   mdf_read_outputs).  A system whose pair list overflowed is grown and redone alone. */
+int mdf_batch_read_outputs(mdf_batch *b, mdf_outputs *outs);
+
+void mdf_batch_destroy(mdf_batch *b);
+
 /* Pick the launch shapes of this handle's kernels by timing candidates on its current
    state: lanes per lead (8/16/32) and loop form of the pair kernel, blocks per launch of the
    structure-factor and back-projection kernels.  Each candidate runs reps + 1 times; the

@@ -55,17 +55,51 @@ __global__ void k_pair_totals(DevSys S);
 __global__ void k_final(DevSys S, FinalArgs A, const double *colpart, const double *kpart, const double *eppart,
                         const double *dipz, double *e_vdw, double *e_coul, Scalars *out);
 
+__global__ void k_skin_b(const RepArgs *, const int2 *);
+__global__ void k_decide_b(const RepArgs *);
+__global__ void k_cell_count_b(const RepArgs *, const int2 *);
+__global__ void k_cell_scan_build_b(const RepArgs *);
+__global__ void k_cell_fill_b(const RepArgs *, const int2 *);
+__global__ void k_cell_sort_b(const RepArgs *, const int2 *);
+__global__ void k_build_b(const RepArgs *, const int2 *);
+__global__ void k_snapshot_b(const RepArgs *, const int2 *);
+__global__ void k_flags_b(const RepArgs *, int, DevFlags *);
+template <typename R, int G, bool VIR>
+__global__ void k_pair_b(const RepArgs *, const int2 *);
+template <typename R>
+__global__ void k_sf_b(const RepArgs *, const int2 *);
+__global__ void k_sf_reduce_b(const RepArgs *, const int2 *);
+template <typename R>
+__global__ void k_bp_b(const RepArgs *, const int2 *);
+__global__ void k_epilogue_b(const RepArgs *, const int2 *);
+__global__ void k_colsum_b(const RepArgs *, const int2 *);
+__global__ void k_final_b(const RepArgs *);
+
 // device-resident inputs -> the handle's buffers, one launch
-__global__ void k_gather(DevSys S, const double *__restrict__ x, const double *__restrict__ y,
+__device__ __forceinline__ void gather_body(const DevSys &S, const double *__restrict__ x, const double *__restrict__ y,
                          const double *__restrict__ z, const double *__restrict__ cx, const double *__restrict__ cy,
-                         const double *__restrict__ cz) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+                         const double *__restrict__ cz, int bid, int nblk) {
+    int t = bid * blockDim.x + threadIdx.x;
     if (t < S.n_move) { S.x[t] = x[t]; S.y[t] = y[t]; S.z[t] = z[t]; }
     if (t < S.n_rig) {
         const_cast<double *>(S.cx)[t] = cx[t];
         const_cast<double *>(S.cy)[t] = cy[t];
         const_cast<double *>(S.cz)[t] = cz[t];
     }
+}
+
+__global__ void k_gather(DevSys S, const double *__restrict__ x, const double *__restrict__ y,
+                         const double *__restrict__ z, const double *__restrict__ cx, const double *__restrict__ cy,
+                         const double *__restrict__ cz) {
+    gather_body(S, x, y, z, cx, cy, cz, blockIdx.x, gridDim.x);
+}
+
+// the same for every system of a batch: in[6 * system + c] holds its six input pointers
+__global__ void k_gather_b(const RepArgs *__restrict__ RA, const int2 *__restrict__ map,
+                           const double *const *__restrict__ in) {
+    const int2 m = map[blockIdx.x];
+    const double *const *p = in + 6 * m.x;
+    gather_body(RA[m.x].S, p[0], p[1], p[2], p[3], p[4], p[5], m.y, RA[m.x].gather_blocks);
 }
 }  // namespace mdf
 
@@ -181,6 +215,7 @@ struct mdf_system {
     // one captured step graph per ring slot (each records its slot's events)
     hipGraphExec_t graph[NRING] = {};
     bool use_graphs = true;
+    hipEvent_t in_event = nullptr;   // orders a device-input step after the inputs' producer
     bool pair_rolled = false;   // site pairs of a molecule pair as a rolled loop (fewer registers)
     int pair_g = 16;            // lanes per lead in the pair kernel (8, 16 or 32)
     // launch-shape targets of the reciprocal kernels (blocks per launch; the row groups follow)
@@ -471,6 +506,27 @@ static void launch_bp(mdf_system *s) {
                                                        s->fpart.p);
 }
 
+// the scalars the final reduction needs
+static FinalArgs final_args(const mdf_system *s) {
+    FinalArgs A;
+    memset(&A, 0, sizeof A);
+    A.n_colblk = s->n_colblk;
+    A.n_slots = (int)s->slot_type.size();
+    for (int k = 0; k < A.n_slots; k++) A.slot_type[k] = s->slot_type[k];
+    for (int c = 0; c < NCLS; c++) A.cls_type[c] = s->cls_type[c];
+    A.n_kblk = s->ak.size() > 0 ? s->n_kblk : 0;
+    A.n_epblk = s->n_epblk;
+    A.e_self = s->uself;
+    A.e_recip_ww = s->ewqWW;
+    A.crf0 = s->crf0;
+    A.ew3dc = s->par.ew3dc;
+    A.lx = s->par.box[0];
+    A.ly = s->par.box[1];
+    A.na_scale = s->c.na * s->c.e30;
+    A.bar = s->c.e5;
+    return A;
+}
+
 static void launch_forces(mdf_system *s) {
     DevSys &S = s->S;
     hipStream_t st = s->stream;
@@ -497,22 +553,7 @@ static void launch_forces(mdf_system *s) {
     if (s->ev) (void)hipEventRecord(s->ev[6], st);
     {
         dim3 gc(NCOL, s->n_colblk);
-        FinalArgs A;
-        memset(&A, 0, sizeof A);
-        A.n_colblk = s->n_colblk;
-        A.n_slots = (int)s->slot_type.size();
-        for (int k = 0; k < A.n_slots; k++) A.slot_type[k] = s->slot_type[k];
-        for (int c = 0; c < NCLS; c++) A.cls_type[c] = s->cls_type[c];
-        A.n_kblk = nk > 0 ? s->n_kblk : 0;
-        A.n_epblk = s->n_epblk;
-        A.e_self = s->uself;
-        A.e_recip_ww = s->ewqWW;
-        A.crf0 = s->crf0;
-        A.ew3dc = s->par.ew3dc;
-        A.lx = s->par.box[0];
-        A.ly = s->par.box[1];
-        A.na_scale = s->c.na * s->c.e30;
-        A.bar = s->c.e5;
+        const FinalArgs A = final_args(s);
         k_colsum<<<gc, 256, 0, st>>>(S, s->lead_slot.p, s->colper, s->colpart.p);
         k_final<<<1, 256, 0, st>>>(S, A, s->colpart.p, s->kpart.p, s->eppart.p, s->dipz.p, s->e_vdw.p, s->e_coul.p,
                                    s->d_scal.p);
@@ -715,6 +756,7 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
     } while (0)
     {
         HTRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+        HTRY(hipEventCreateWithFlags(&s->in_event, hipEventDisableTiming));
         for (int r = 0; r < mdf_system::NRING; r++)
             for (int k = 0; k <= N_STAGES; k++) HTRY(hipEventCreate(&s->ring[r][k]));
         // leads, lead-type slots, energy classes
@@ -894,7 +936,9 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
                 double rs = sqrt(S.rskinsq) * (1.0 + 1e-9);
                 wncz = std::max(1, std::min(1024, (int)floor((hi - lo) / rs)));
                 wz0 = lo;
-                wcsz = hi > lo ? (hi - lo) / wncz : 1.0;
+                // cells at least the list radius high, so the three-cell stencil of a lead
+                // reaches every frozen site within it, also for crystals thinner than that
+                wcsz = std::max(hi > lo ? (hi - lo) / wncz : 0.0, rs);
             }
             int nwc = s->ncells_xy * wncz;
             DBuf<int> wcell_of, wcnt, wcur, witems2;
@@ -1097,9 +1141,11 @@ int mdf_compute(mdf_system *s, const double *x, const double *y, const double *z
 int mdf_compute_device(mdf_system *s, const double *d_x, const double *d_y, const double *d_z, const double *d_cx,
                        const double *d_cy, const double *d_cz, void *stream) {
     if (!s || !d_x || !d_y || !d_z) return fail(MDF_ERR_ARG, "null argument");
-    (void)stream;
     if (s->top.n_rig > 0 && (!d_cx || !d_cy || !d_cz)) return fail(MDF_ERR_ARG, "COM arrays required");
     HIPCHK(hipSetDevice(s->device));
+    // start after the producer of the inputs (the handle's stream does not block on others)
+    HIPCHK(hipEventRecord(s->in_event, (hipStream_t)stream));
+    HIPCHK(hipStreamWaitEvent(s->stream, s->in_event, 0));
     int n = std::max(s->top.n_move, s->top.n_rig);
     k_gather<<<std::max(1, (n + 255) / 256), 256, 0, s->stream>>>(s->S, d_x, d_y, d_z, d_cx, d_cy, d_cz);
     return run_step(s, 0);
@@ -1312,8 +1358,315 @@ void mdf_destroy(mdf_system *s) {
     for (int r = 0; r < mdf_system::NRING; r++)
         for (int k = 0; k <= N_STAGES; k++)
             if (s->ring[r][k]) (void)hipEventDestroy(s->ring[r][k]);
+    if (s->in_event) (void)hipEventDestroy(s->in_event);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
+}
+
+}  // extern "C"
+
+// ---- batches --------------------------------------------------------------------------
+// Many independent systems stepped together: every stage is one launch over all of them
+// (block maps give each block its system), so small systems fill the GPU together and a
+// step of the whole batch costs one graph launch.
+enum { BK_GATHER, BK_SKIN, BK_COUNT, BK_FILL, BK_SORT, BK_BUILD, BK_SNAP, BK_PAIR, BK_SF, BK_SFR, BK_BP, BK_EPI,
+       BK_COLSUM, BK_N };
+
+struct mdf_batch {
+    std::vector<mdf_system *> sys;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<RepArgs> h_rep;
+    DBuf<RepArgs> d_rep;
+    DBuf<int2> maps[BK_N];
+    int nmap[BK_N] = {};
+    std::vector<const double *> h_in;
+    DBuf<const double *> d_in;
+    DBuf<DevFlags> d_flags;
+    std::vector<DevFlags> h_flags;
+    hipGraphExec_t graph = nullptr;
+    hipEvent_t in_event = nullptr;
+    bool pending = false;
+    int pair_g = 16;
+    bool f32 = false, vir = false;
+};
+
+static void fill_rep(mdf_system *s, RepArgs &r) {
+    memset(&r, 0, sizeof r);
+    const int nm = s->top.n_move, nr = s->top.n_rig, nk = (int)s->ak.size();
+    r.S = s->S;
+    r.dip_part = s->dip_part.p;
+    r.dipz = s->dipz.p;
+    r.skin_blocks = s->skin_blocks;
+    r.force_rebuild = 0;
+    r.lead_cell = s->lead_cell.p;
+    r.cell_items2 = s->cell_items2.p;
+    r.lead_blocks = std::max(1, (s->n_lead + 255) / 256);
+    r.sort_blocks = 64;
+    r.build_blocks = std::max(1, (s->n_lead + 3) / 4);
+    r.snap_blocks = std::max(1, (std::max(nr, nm) + 255) / 256);
+    r.ncells_xy = s->ncells_xy;
+    r.gather_blocks = std::max(1, (std::max(nm, nr) + 255) / 256);
+    const RecipArgs a = recip_args(s);
+    r.rows = s->d_rows.p;
+    r.nrows = (int)s->rows.size();
+    r.cpb = s->cpb;
+    r.n_sb = s->n_sb;
+    r.n_cg = s->n_cg;
+    r.nk = nk;
+    r.n_kblk = s->n_kblk;
+    r.n_rg = s->n_rg;
+    r.n_bpsb = std::max(1, (nm + 256 * BP_NS - 1) / (256 * BP_NS));
+    r.sf_partial = s->sf_partial.p;
+    r.s_wall = s->d_swall.p;
+    r.ak = s->d_ak.p;
+    r.kvec = s->d_kvec.p;
+    r.s_out = s->d_s.p;
+    r.t_out = s->d_t.p;
+    r.kpart = s->kpart.p;
+    r.bp_row_start = s->d_bp_row_start.p;
+    r.fpart = s->fpart.p;
+    r.cux = a.cux; r.cuy = a.cuy; r.cuz = a.cuz;
+    r.pipix = a.pipix; r.pipiy = a.pipiy; r.pipiz = a.pipiz;
+    r.lamb = 1.0 / (4.0 * s->S.alfa * s->S.alfa);
+    r.wl = s->d_wl.p;
+    r.ew3dc = s->par.ew3dc;
+    r.n_groups = (nk > 0 && nm > 0) ? s->n_rg : 0;
+    r.n_epblk = s->n_epblk;
+    r.colper = s->colper;
+    r.n_colblk = s->n_colblk;
+    r.crf0 = s->crf0;
+    r.eppart = s->eppart.p;
+    r.fcom = s->fcom.p;
+    r.tq = s->torque.p;
+    r.colpart = s->colpart.p;
+    r.e_vdw = s->e_vdw.p;
+    r.e_coul = s->e_coul.p;
+    r.lead_slot = s->lead_slot.p;
+    r.scal = s->d_scal.p;
+    r.A = final_args(s);
+}
+
+// block counts of every batched launch for one system
+static void rep_blocks(const mdf_batch *b, const mdf_system *s, const RepArgs &r, int cnt[BK_N]) {
+    const int nk = r.nk, nm = s->top.n_move;
+    cnt[BK_GATHER] = r.gather_blocks;
+    cnt[BK_SKIN] = r.skin_blocks;
+    cnt[BK_COUNT] = r.lead_blocks;
+    cnt[BK_FILL] = r.lead_blocks;
+    cnt[BK_SORT] = r.sort_blocks;
+    cnt[BK_BUILD] = r.build_blocks;
+    cnt[BK_SNAP] = r.snap_blocks;
+    cnt[BK_PAIR] = std::max(1, (s->n_lead + (256 / b->pair_g) - 1) / (256 / b->pair_g));
+    cnt[BK_SF] = (nk > 0 && nm > 0) ? r.n_sb * r.n_cg : 0;
+    cnt[BK_SFR] = nk > 0 ? r.n_kblk : 0;
+    cnt[BK_BP] = (nk > 0 && nm > 0) ? r.n_bpsb * r.n_rg : 0;
+    cnt[BK_EPI] = r.n_epblk;
+    cnt[BK_COLSUM] = NCOL * r.n_colblk;
+}
+
+// per-system arguments and block maps to the device; the captured graph reads both at run
+// time, so it stays valid when only their contents change
+static int batch_upload(mdf_batch *b, bool maps) {
+    const int n = (int)b->sys.size();
+    b->h_rep.resize(n);
+    std::vector<int2> mp[BK_N];
+    for (int i = 0; i < n; i++) {
+        fill_rep(b->sys[i], b->h_rep[i]);
+        int cnt[BK_N];
+        rep_blocks(b, b->sys[i], b->h_rep[i], cnt);
+        for (int k = 0; k < BK_N; k++)
+            for (int l = 0; l < cnt[k]; l++) mp[k].push_back(make_int2(i, l));
+    }
+    HIPCHK(hipMemcpyAsync(b->d_rep.p, b->h_rep.data(), sizeof(RepArgs) * n, hipMemcpyHostToDevice, b->stream));
+    if (maps) {
+        for (int k = 0; k < BK_N; k++) {
+            b->maps[k].release();
+            b->nmap[k] = (int)mp[k].size();
+            HIPCHK(upload(b->maps[k], mp[k].data(), mp[k].size()));
+        }
+    }
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return MDF_OK;
+}
+
+static void batch_launch(mdf_batch *b) {
+    const int n = (int)b->sys.size();
+    hipStream_t st = b->stream;
+    const RepArgs *R = b->d_rep.p;
+    auto grid = [&](int k) { return b->nmap[k]; };
+    if (grid(BK_GATHER)) k_gather_b<<<grid(BK_GATHER), 256, 0, st>>>(R, b->maps[BK_GATHER].p, b->d_in.p);
+    k_skin_b<<<grid(BK_SKIN), 256, 0, st>>>(R, b->maps[BK_SKIN].p);
+    k_decide_b<<<n, 256, 0, st>>>(R);
+    k_cell_count_b<<<grid(BK_COUNT), 256, 0, st>>>(R, b->maps[BK_COUNT].p);
+    k_cell_scan_build_b<<<n, 1024, 0, st>>>(R);
+    k_cell_fill_b<<<grid(BK_FILL), 256, 0, st>>>(R, b->maps[BK_FILL].p);
+    k_cell_sort_b<<<grid(BK_SORT), 64, 0, st>>>(R, b->maps[BK_SORT].p);
+    k_build_b<<<grid(BK_BUILD), 256, 0, st>>>(R, b->maps[BK_BUILD].p);
+    k_snapshot_b<<<grid(BK_SNAP), 256, 0, st>>>(R, b->maps[BK_SNAP].p);
+    {
+        const int2 *mp = b->maps[BK_PAIR].p;
+        const int g = grid(BK_PAIR);
+#define MDF_PB(RT, G)                                                                    \
+    do {                                                                                 \
+        if (b->vir) k_pair_b<RT, G, true><<<g, 256, 0, st>>>(R, mp);                     \
+        else k_pair_b<RT, G, false><<<g, 256, 0, st>>>(R, mp);                           \
+    } while (0)
+        if (b->f32) {
+            if (b->pair_g == 8) MDF_PB(float, 8); else if (b->pair_g == 32) MDF_PB(float, 32); else MDF_PB(float, 16);
+        } else {
+            if (b->pair_g == 8) MDF_PB(double, 8); else if (b->pair_g == 32) MDF_PB(double, 32); else MDF_PB(double, 16);
+        }
+#undef MDF_PB
+    }
+    if (grid(BK_SF)) {
+        if (b->f32) k_sf_b<float><<<grid(BK_SF), SF_BLOCK, 0, st>>>(R, b->maps[BK_SF].p);
+        else k_sf_b<double><<<grid(BK_SF), SF_BLOCK, 0, st>>>(R, b->maps[BK_SF].p);
+    }
+    if (grid(BK_SFR)) k_sf_reduce_b<<<grid(BK_SFR), 256, 0, st>>>(R, b->maps[BK_SFR].p);
+    if (grid(BK_BP)) {
+        if (b->f32) k_bp_b<float><<<grid(BK_BP), 256, 0, st>>>(R, b->maps[BK_BP].p);
+        else k_bp_b<double><<<grid(BK_BP), 256, 0, st>>>(R, b->maps[BK_BP].p);
+    }
+    k_epilogue_b<<<grid(BK_EPI), EPI_BLOCK, 0, st>>>(R, b->maps[BK_EPI].p);
+    k_colsum_b<<<grid(BK_COLSUM), 256, 0, st>>>(R, b->maps[BK_COLSUM].p);
+    k_final_b<<<n, 256, 0, st>>>(R);
+    k_flags_b<<<(n + 63) / 64, 64, 0, st>>>(R, n, b->d_flags.p);
+}
+
+static void batch_free(mdf_batch *b) {
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    b->d_rep.release();
+    for (int k = 0; k < BK_N; k++) b->maps[k].release();
+    b->d_in.release();
+    b->d_flags.release();
+    if (b->in_event) (void)hipEventDestroy(b->in_event);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+}
+
+extern "C" {
+
+int mdf_batch_create(mdf_system *const *systems, int32_t n, mdf_batch **out) {
+    if (!systems || !out || n <= 0) return fail(MDF_ERR_ARG, "null argument or empty batch");
+    *out = nullptr;
+    for (int i = 0; i < n; i++)
+        if (!systems[i]) return fail(MDF_ERR_ARG, "system %d is null", i);
+    const mdf_system *s0 = systems[0];
+    for (int i = 1; i < n; i++) {
+        const mdf_system *s = systems[i];
+        if (s->device != s0->device || s->par.precision != s0->par.precision || s->par.virial != s0->par.virial)
+            return fail(MDF_ERR_ARG, "system %d differs from system 0 in device, precision or virial", i);
+        for (int j = 0; j < i; j++)
+            if (systems[j] == s) return fail(MDF_ERR_ARG, "system %d appears twice", i);
+    }
+    mdf_batch *b = new mdf_batch;
+    b->sys.assign(systems, systems + n);
+    b->device = s0->device;
+    b->f32 = s0->par.precision == MDF_FP32_ACC64;
+    b->vir = s0->par.virial != 0;
+    b->pair_g = s0->pair_g;
+    int rc = MDF_OK;
+    hipError_t e = hipSetDevice(b->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->in_event, hipEventDisableTiming);
+    for (int i = 0; i < n && e == hipSuccess; i++) e = hipStreamSynchronize(systems[i]->stream);
+    if (e == hipSuccess) e = b->d_rep.alloc(n);
+    if (e == hipSuccess) e = b->d_in.alloc(6 * (size_t)n);
+    if (e == hipSuccess) e = b->d_flags.alloc(n);
+    if (e != hipSuccess) rc = fail(MDF_ERR_HIP, "batch set-up: %s", hipGetErrorString(e));
+    if (rc == MDF_OK) rc = batch_upload(b, true);
+    if (rc != MDF_OK) {
+        batch_free(b);
+        delete b;
+        return rc;
+    }
+    b->h_flags.resize(n);
+    b->h_in.assign(6 * (size_t)n, nullptr);
+    *out = b;
+    return MDF_OK;
+}
+
+int mdf_batch_compute_device(mdf_batch *b, const double *const *x, const double *const *y, const double *const *z,
+                             const double *const *cx, const double *const *cy, const double *const *cz,
+                             void *stream) {
+    if (!b || !x || !y || !z) return fail(MDF_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(b->device));
+    const int n = (int)b->sys.size();
+    if (b->pending) HIPCHK(hipStreamSynchronize(b->stream));   // the pointer table is about to change
+    for (int i = 0; i < n; i++) {
+        if (!x[i] || !y[i] || !z[i]) return fail(MDF_ERR_ARG, "null positions for system %d", i);
+        if (b->sys[i]->top.n_rig > 0 && (!cx || !cy || !cz || !cx[i] || !cy[i] || !cz[i]))
+            return fail(MDF_ERR_ARG, "COM arrays required for system %d", i);
+        const double *p[6] = {x[i], y[i], z[i], cx ? cx[i] : nullptr, cy ? cy[i] : nullptr, cz ? cz[i] : nullptr};
+        for (int c = 0; c < 6; c++) b->h_in[6 * i + c] = p[c];
+    }
+    HIPCHK(hipMemcpyAsync(b->d_in.p, b->h_in.data(), sizeof(double *) * 6 * n, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipEventRecord(b->in_event, (hipStream_t)stream));
+    HIPCHK(hipStreamWaitEvent(b->stream, b->in_event, 0));
+    if (!b->graph) {
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
+        batch_launch(b);
+        hipError_t ec = hipStreamEndCapture(b->stream, &g);
+        if (ec != hipSuccess) return fail(MDF_ERR_HIP, "batch graph capture: %s", hipGetErrorString(ec));
+        ec = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ec != hipSuccess) return fail(MDF_ERR_HIP, "batch graph instantiate: %s", hipGetErrorString(ec));
+    }
+    HIPCHK(hipGraphLaunch(b->graph, b->stream));
+    HIPCHK(hipGetLastError());
+    b->pending = true;
+    return MDF_OK;
+}
+
+int mdf_batch_read_outputs(mdf_batch *b, mdf_outputs *outs) {
+    if (!b || !outs) return fail(MDF_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(b->device));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    b->pending = false;
+    const int n = (int)b->sys.size();
+    HIPCHK(hipMemcpy(b->h_flags.data(), b->d_flags.p, sizeof(DevFlags) * n, hipMemcpyDeviceToHost));
+    bool regrown = false;
+    for (int i = 0; i < n; i++) {
+        mdf_system *s = b->sys[i];
+        s->last_flags = b->h_flags[i];
+        s->last_rebuilt = s->last_flags.rebuild;
+        const DevFlags &f = s->last_flags;
+        if (f.overflow_mm || f.overflow_mw) {
+            // this system's list outgrew its rows: grow them and redo its step on its own
+            // (the batch kernels already gathered its positions)
+            if (s->fixed_cap)
+                return fail(MDF_ERR_LIST_OVERFLOW, "system %d: nnlist too big (need %d / %d partners per lead)", i,
+                            f.overflow_mm, f.overflow_mw);
+            int rc = alloc_lists(s, std::max(s->S.cap_mm, (int)(f.overflow_mm * 1.25) + 16),
+                                 std::max(s->S.cap_mw, (int)(f.overflow_mw * 1.25) + 16));
+            if (rc) return rc;
+            rc = run_step(s, 1);
+            if (rc) return rc;
+            int redo = 0;
+            rc = finish_step(s, &redo);
+            if (rc) return rc;
+            if (redo) return fail(MDF_ERR_LIST_OVERFLOW, "system %d: pair list kept overflowing", i);
+            regrown = true;
+        }
+    }
+    if (regrown) {
+        int rc = batch_upload(b, false);   // new list rows; block counts are unchanged
+        if (rc) return rc;
+    }
+    for (int i = 0; i < n; i++) {
+        int rc = copy_outputs(b->sys[i], &outs[i]);
+        if (rc) return rc;
+    }
+    return MDF_OK;
+}
+
+void mdf_batch_destroy(mdf_batch *b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    batch_free(b);
+    delete b;
 }
 
 }  // extern "C"

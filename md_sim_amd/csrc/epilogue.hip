@@ -47,14 +47,14 @@ __device__ void angle_force(const DevSys &S, int a, int role, double &fx, double
 // One block covers EPI_BLOCK = 3 x 64 consecutive sites, i.e. 64 whole rigid molecules
 // (rigid sites come first, three per molecule), so the block also forms those molecules'
 // COM forces and torques (src/forcem.f90:1003-1024) from the forces it has just finished.
-__global__ __launch_bounds__(EPI_BLOCK) void k_epilogue(DevSys S, const double *__restrict__ fpart, int n_groups,
+__device__ __forceinline__ void epilogue_body(const DevSys &S, const double *__restrict__ fpart, int n_groups,
                                                         const WallLR *__restrict__ wl, int ew3dc, double crf0,
                                                         const double *__restrict__ dipz, double *__restrict__ epart,
-                                                        double *__restrict__ fcom, double *__restrict__ tq) {
+                                                        double *__restrict__ fcom, double *__restrict__ tq, int bid, int nblk) {
     __shared__ double sf[3][EPI_BLOCK];
     __shared__ double wsum[EPI_BLOCK / 64][EP_N];
     double acc[EP_N] = {0, 0, 0, 0, 0, 0};
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = bid * blockDim.x + threadIdx.x;
     const int n = S.n_move;
     sf[0][threadIdx.x] = 0.0; sf[1][threadIdx.x] = 0.0; sf[2][threadIdx.x] = 0.0;
     if (i < n) {
@@ -116,11 +116,11 @@ __global__ __launch_bounds__(EPI_BLOCK) void k_epilogue(DevSys S, const double *
     if (threadIdx.x < EP_N) {
         double v = 0.0;
         for (int w = 0; w < EPI_BLOCK / 64; w++) v += wsum[w][threadIdx.x];
-        epart[blockIdx.x * EP_N + threadIdx.x] = v;
+        epart[bid * EP_N + threadIdx.x] = v;
     }
     // COM force and torque of the block's molecules
     const int nr = S.n_rig;
-    const int m = blockIdx.x * (EPI_BLOCK / 3) + threadIdx.x;
+    const int m = bid * (EPI_BLOCK / 3) + threadIdx.x;
     if (threadIdx.x < EPI_BLOCK / 3 && m < nr) {
         const int j = 3 * m, lj = 3 * threadIdx.x;
         double tx = 0, ty = 0, tz = 0, gx = 0, gy = 0, gz = 0;
@@ -137,11 +137,18 @@ __global__ __launch_bounds__(EPI_BLOCK) void k_epilogue(DevSys S, const double *
     }
 }
 
+__global__ __launch_bounds__(EPI_BLOCK) void k_epilogue(DevSys S, const double *__restrict__ fpart, int n_groups,
+                                                        const WallLR *__restrict__ wl, int ew3dc, double crf0,
+                                                        const double *__restrict__ dipz, double *__restrict__ epart,
+                                                        double *__restrict__ fcom, double *__restrict__ tq) {
+    epilogue_body(S, fpart, n_groups, wl, ew3dc, crf0, dipz, epart, fcom, tq, blockIdx.x, gridDim.x);
+}
+
 // column sums of the per-lead partials, split by the lead's type slot
-__global__ __launch_bounds__(256) void k_colsum(DevSys S, const int *__restrict__ lead_slot, int per_block,
-                                                double *__restrict__ colpart) {
+__device__ __forceinline__ void colsum_body(const DevSys &S, const int *__restrict__ lead_slot, int per_block,
+                                                double *__restrict__ colpart, int bx, int by, int nbx, int nby) {
     __shared__ double red[MAX_LEAD_TYPES][256];
-    const int col = blockIdx.x, blk = blockIdx.y;
+    const int col = bx, blk = by;
     const size_t L = (size_t)S.n_lead;
     double acc[MAX_LEAD_TYPES] = {0, 0, 0, 0};
     const int b = blk * per_block, e = min(S.n_lead, b + per_block);
@@ -161,7 +168,12 @@ __global__ __launch_bounds__(256) void k_colsum(DevSys S, const int *__restrict_
         __syncthreads();
     }
     if (threadIdx.x < MAX_LEAD_TYPES)
-        colpart[((size_t)col * gridDim.y + blk) * MAX_LEAD_TYPES + threadIdx.x] = red[threadIdx.x][0];
+        colpart[((size_t)col * nby + blk) * MAX_LEAD_TYPES + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(256) void k_colsum(DevSys S, const int *__restrict__ lead_slot, int per_block,
+                                                double *__restrict__ colpart) {
+    colsum_body(S, lead_slot, per_block, colpart, blockIdx.x, blockIdx.y, gridDim.x, gridDim.y);
 }
 
 // listed pair counts (ipack, wpack) after a rebuild: the per-row counts summed in fixed
@@ -187,10 +199,10 @@ __device__ void pair_totals(const DevSys &S) {
 // the same after the list build at creation (no force step follows it)
 __global__ __launch_bounds__(256) void k_pair_totals(DevSys S) { pair_totals(S); }
 
-__global__ __launch_bounds__(256) void k_final(DevSys S, FinalArgs A, const double *__restrict__ colpart,
+__device__ __forceinline__ void final_body(const DevSys &S, FinalArgs A, const double *__restrict__ colpart,
                                                const double *__restrict__ kpart, const double *__restrict__ eppart,
                                                const double *__restrict__ dipz, double *__restrict__ e_vdw,
-                                               double *__restrict__ e_coul, Scalars *__restrict__ out) {
+                                               double *__restrict__ e_coul, Scalars *__restrict__ out, int bid, int nblk) {
     __shared__ double colsum[NCOL][MAX_LEAD_TYPES];
     __shared__ double ksum[8];
     __shared__ double epsum[EP_N];
@@ -282,4 +294,30 @@ __global__ __launch_bounds__(256) void k_final(DevSys S, FinalArgs A, const doub
     *out = o;
 }
 
+__global__ __launch_bounds__(256) void k_final(DevSys S, FinalArgs A, const double *__restrict__ colpart,
+                                               const double *__restrict__ kpart, const double *__restrict__ eppart,
+                                               const double *__restrict__ dipz, double *__restrict__ e_vdw,
+                                               double *__restrict__ e_coul, Scalars *__restrict__ out) {
+    final_body(S, A, colpart, kpart, eppart, dipz, e_vdw, e_coul, out, blockIdx.x, gridDim.x);
+}
+
+}  // namespace mdf
+
+namespace mdf {
+// batched forms
+__global__ __launch_bounds__(EPI_BLOCK) void k_epilogue_b(const RepArgs *__restrict__ RA,
+                                                          const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    const RepArgs &r = RA[m.x];
+    epilogue_body(r.S, r.fpart, r.n_groups, r.wl, r.ew3dc, r.crf0, r.dipz, r.eppart, r.fcom, r.tq, m.y, r.n_epblk);
+}
+__global__ __launch_bounds__(256) void k_colsum_b(const RepArgs *__restrict__ RA, const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    const RepArgs &r = RA[m.x];
+    colsum_body(r.S, r.lead_slot, r.colper, r.colpart, m.y % NCOL, m.y / NCOL, NCOL, r.n_colblk);
+}
+__global__ __launch_bounds__(256) void k_final_b(const RepArgs *__restrict__ RA) {
+    const RepArgs &r = RA[blockIdx.x];
+    final_body(r.S, r.A, r.colpart, r.kpart, r.eppart, r.dipz, r.e_vdw, r.e_coul, r.scal, 0, 1);
+}
 }  // namespace mdf

@@ -153,4 +153,39 @@ __device__ __forceinline__ double list_d2(const DevSys &S, int i, int j) {
     return __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
 }
 
+// ---- batches of independent systems ----------------------------------------------------
+// One entry per system: everything its launches take.  A batched kernel covers the blocks of
+// every system in one launch; a block map entry (system, local block) tells each block whose
+// work it does.  Kernels with one block per system index the table by blockIdx.x.
+struct RepArgs {
+    DevSys S;
+    int n_rig_sites_pad;                 // unused; keeps the table entries 8-byte aligned
+    // list stage
+    double *dip_part, *dipz;
+    int skin_blocks, force_rebuild;
+    int *lead_cell, *cell_items2;
+    int lead_blocks, sort_blocks, build_blocks, snap_blocks, ncells_xy, gather_blocks;
+    // reciprocal sum
+    const KRow *rows;
+    int nrows, cpb, n_sb, n_cg, nk, n_kblk, n_rg, n_bpsb;
+    double2 *sf_partial;
+    const double2 *s_wall;
+    const double *ak;
+    const int *kvec;
+    double2 *s_out;
+    double4 *t_out;
+    double *kpart;
+    const int *bp_row_start;
+    double *fpart;
+    double cux, cuy, cuz, pipix, pipiy, pipiz, lamb;
+    // epilogue, column sums, final sums
+    const WallLR *wl;
+    int ew3dc, n_groups, n_epblk, colper, n_colblk, pad2;
+    double crf0;
+    double *eppart, *fcom, *tq, *colpart, *e_vdw, *e_coul;
+    const int *lead_slot;
+    Scalars *scal;
+    FinalArgs A;
+};
+
 }  // namespace mdf

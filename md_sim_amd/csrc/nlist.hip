@@ -15,14 +15,14 @@ namespace mdf {
 
 // max squared displacement since the last build (src/forcem.f90:245-249) and lead z range;
 // per-block partial of the moving dipole sum(q z) (src/forcem.f90:924) into dip_part
-__global__ __launch_bounds__(256) void k_skin(DevSys S, double *dip_part) {
+__device__ __forceinline__ void skin_body(const DevSys &S, double *dip_part, int bid, int nblk) {
     __shared__ double red[256];
     double dmax = 0.0, dip = 0.0;
     double zlo = 1e300, zhi = -1e300;
     const int n_nonrig = S.n_move - S.n_rigid_sites;
     const int total = S.n_rig + n_nonrig;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < max(total, max(S.n_move, S.n_lead));
-         t += gridDim.x * blockDim.x) {
+    for (int t = bid * blockDim.x + threadIdx.x; t < max(total, max(S.n_move, S.n_lead));
+         t += nblk * blockDim.x) {
         if (t < S.n_rig) {
             double a = __dsub_rn(S.ocx[t], S.cx[t]), b = __dsub_rn(S.ocy[t], S.cy[t]), c = __dsub_rn(S.ocz[t], S.cz[t]);
             double d = __dadd_rn(__dadd_rn(__dmul_rn(a, a), __dmul_rn(b, b)), __dmul_rn(c, c));
@@ -58,19 +58,23 @@ __global__ __launch_bounds__(256) void k_skin(DevSys S, double *dip_part) {
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        const int nb = gridDim.x;
-        dip_part[blockIdx.x] = red[0];
-        dip_part[nb + blockIdx.x] = fmax(fmax(ext[0][0], ext[0][1]), fmax(ext[0][2], ext[0][3]));
-        dip_part[2 * nb + blockIdx.x] = fmin(fmin(ext[1][0], ext[1][1]), fmin(ext[1][2], ext[1][3]));
-        dip_part[3 * nb + blockIdx.x] = fmax(fmax(ext[2][0], ext[2][1]), fmax(ext[2][2], ext[2][3]));
+        const int nb = nblk;
+        dip_part[bid] = red[0];
+        dip_part[nb + bid] = fmax(fmax(ext[0][0], ext[0][1]), fmax(ext[0][2], ext[0][3]));
+        dip_part[2 * nb + bid] = fmin(fmin(ext[1][0], ext[1][1]), fmin(ext[1][2], ext[1][3]));
+        dip_part[3 * nb + bid] = fmax(fmax(ext[2][0], ext[2][1]), fmax(ext[2][2], ext[2][3]));
     }
+}
+
+__global__ __launch_bounds__(256) void k_skin(DevSys S, double *dip_part) {
+    skin_body(S, dip_part, blockIdx.x, gridDim.x);
 }
 
 // rebuild decision (src/forcem.f90:250) and z grid of this build; dipole total.  One block,
 // over k_skin's per-block partials.  It also clears the overflow flags for this step's build
 // (so no reset launch is needed).
-__global__ __launch_bounds__(256) void k_decide(DevSys S, const double *dip_part, int n_dip, double *dipz_out,
-                                                int force_rebuild) {
+__device__ __forceinline__ void decide_body(const DevSys &S, const double *dip_part, int n_dip, double *dipz_out,
+                                                int force_rebuild, int bid, int nblk) {
     __shared__ double red2[256];
     __shared__ double ext[3][4];
     double acc = 0.0, dmax = 0.0, zlo = 1e300, zhi = -1e300;
@@ -126,6 +130,11 @@ __global__ __launch_bounds__(256) void k_decide(DevSys S, const double *dip_part
     }
 }
 
+__global__ __launch_bounds__(256) void k_decide(DevSys S, const double *dip_part, int n_dip, double *dipz_out,
+                                                int force_rebuild) {
+    decide_body(S, dip_part, n_dip, dipz_out, force_rebuild, blockIdx.x, gridDim.x);
+}
+
 __device__ __forceinline__ int wrap_cell(double v, double L, double Li, int n) {
     double w = v - L * floor(v * Li);
     int c = (int)(w * (double)n * Li);
@@ -170,21 +179,29 @@ __device__ void scan_cells(int *cnt, int *start, int *cursor, int n, bool zero_c
 }
 
 // cell of every lead and the cell counts
-__global__ __launch_bounds__(256) void k_cell_count(DevSys S, int *lead_cell_out) {
+__device__ __forceinline__ void cell_count_body(const DevSys &S, int *lead_cell_out, int bid, int nblk) {
     const DevFlags *f = S.flags;
     if (!f->rebuild) return;
-    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = bid * blockDim.x + threadIdx.x;
     if (l >= S.n_lead) return;
     const int c = lead_cell(S, S.lead[l], f->ncz, f->z0, f->csz);
     lead_cell_out[l] = c;
     atomicAdd(&S.cell_cnt[c], 1);
 }
 
+__global__ __launch_bounds__(256) void k_cell_count(DevSys S, int *lead_cell_out) {
+    cell_count_body(S, lead_cell_out, blockIdx.x, gridDim.x);
+}
+
 // cell starts of this build; the counts are cleared behind the scan for the next build
-__global__ __launch_bounds__(1024) void k_cell_scan_build(DevSys S) {
+__device__ __forceinline__ void cell_scan_build_body(const DevSys &S, int bid, int nblk) {
     const DevFlags *f = S.flags;
     if (!f->rebuild) return;
     scan_cells(S.cell_cnt, S.cell_start, S.cell_cursor, S.g.ncx * S.g.ncy * f->ncz, true);
+}
+
+__global__ __launch_bounds__(1024) void k_cell_scan_build(DevSys S) {
+    cell_scan_build_body(S, blockIdx.x, gridDim.x);
 }
 
 // exclusive scan of cell counts, one block (frozen-site cells at creation)
@@ -192,22 +209,27 @@ __global__ __launch_bounds__(1024) void k_cell_scan(int *cnt, int *start, int *c
     scan_cells(cnt, start, cursor, n, false);
 }
 
-__global__ void k_cell_fill(const int *item_cell, int n_items, int *cursor, int *items, const DevFlags *f,
-                            int use_flag) {
+__device__ __forceinline__ void cell_fill_body(const int *item_cell, int n_items, int *cursor, int *items, const DevFlags *f,
+                            int use_flag, int bid, int nblk) {
     if (use_flag && !f->rebuild) return;
-    int l = blockIdx.x * blockDim.x + threadIdx.x;
+    int l = bid * blockDim.x + threadIdx.x;
     if (l >= n_items) return;
     int pos = atomicAdd(&cursor[item_cell[l]], 1);
     items[pos] = l;
 }
 
+__global__ void k_cell_fill(const int *item_cell, int n_items, int *cursor, int *items, const DevFlags *f,
+                            int use_flag) {
+    cell_fill_body(item_cell, n_items, cursor, items, f, use_flag, blockIdx.x, gridDim.x);
+}
+
 // ascending order inside every cell (rank sort, one wave per cell), so list rows are
 // deterministic; items_out gets the sorted copy
-__global__ __launch_bounds__(64) void k_cell_sort(const int *start, const int *items_in, int *items_out,
-                                                  const DevFlags *f, int use_flag, int ncells_xy, int ncz_given) {
+__device__ __forceinline__ void cell_sort_body(const int *start, const int *items_in, int *items_out,
+                                                  const DevFlags *f, int use_flag, int ncells_xy, int ncz_given, int bid, int nblk) {
     if (use_flag && !f->rebuild) return;
     int ncz = use_flag ? f->ncz : ncz_given;
-    for (int c = blockIdx.x; c < ncells_xy * ncz; c += gridDim.x) {
+    for (int c = bid; c < ncells_xy * ncz; c += nblk) {
         int b = start[c], n = start[c + 1] - b;
         for (int a = threadIdx.x; a < n; a += 64) {
             int v = items_in[b + a];
@@ -216,6 +238,11 @@ __global__ __launch_bounds__(64) void k_cell_sort(const int *start, const int *i
             items_out[b + rank] = v;
         }
     }
+}
+
+__global__ __launch_bounds__(64) void k_cell_sort(const int *start, const int *items_in, int *items_out,
+                                                  const DevFlags *f, int use_flag, int ncells_xy, int ncz_given) {
+    cell_sort_body(start, items_in, items_out, f, use_flag, ncells_xy, ncz_given, blockIdx.x, gridDim.x);
 }
 
 __device__ __forceinline__ bool constrained(const DevSys &S, int i, int j) {
@@ -228,11 +255,11 @@ __device__ __forceinline__ bool constrained(const DevSys &S, int i, int j) {
 // list rows: one wave per lead.  The wave walks the lead's neighbour cells in a fixed
 // order, tests 64 candidates at a time and appends the hits with a ballot prefix count,
 // so every row holds its partners in exactly the order a serial scan would give.
-__global__ __launch_bounds__(256) void k_build(DevSys S) {
+__device__ __forceinline__ void build_body(const DevSys &S, int bid, int nblk) {
     const DevFlags *f = S.flags;
     if (!f->rebuild) return;
     const int lane = threadIdx.x & 63;
-    const int l = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int l = bid * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (l >= S.n_lead) return;   // whole wave leaves together
     const int i = S.lead[l];
     const int ncx = S.g.ncx, ncy = S.g.ncy;
@@ -334,14 +361,22 @@ __global__ __launch_bounds__(256) void k_build(DevSys S) {
     }
 }
 
+__global__ __launch_bounds__(256) void k_build(DevSys S) {
+    build_body(S, blockIdx.x, gridDim.x);
+}
+
 // snapshot for the next skin test (src/nnlist.f90:99-119)
 // reference positions for the next skin tests (src/nnlist.f90:99-119)
-__global__ void k_snapshot(DevSys S) {
+__device__ __forceinline__ void snapshot_body(const DevSys &S, int bid, int nblk) {
     if (!S.flags->rebuild) return;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = bid * blockDim.x + threadIdx.x;
     if (t < S.n_rig) { S.ocx[t] = S.cx[t]; S.ocy[t] = S.cy[t]; S.ocz[t] = S.cz[t]; }
     const int i = S.n_rigid_sites + t;
     if (i < S.n_move) { S.ox[i] = S.x[i]; S.oy[i] = S.y[i]; S.oz[i] = S.z[i]; }
+}
+
+__global__ void k_snapshot(DevSys S) {
+    snapshot_body(S, blockIdx.x, gridDim.x);
 }
 
 // frozen-site cells, built once at creation
@@ -364,4 +399,48 @@ __global__ void k_zero_int(int *p, int n) {
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) p[t] = 0;
 }
 
+}  // namespace mdf
+
+namespace mdf {
+// ---- batched forms: one launch over every system of a batch ----------------------------
+__global__ __launch_bounds__(256) void k_skin_b(const RepArgs *__restrict__ RA, const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    const RepArgs &r = RA[m.x];
+    skin_body(r.S, r.dip_part, m.y, r.skin_blocks);
+}
+__global__ __launch_bounds__(256) void k_decide_b(const RepArgs *__restrict__ RA) {
+    const RepArgs &r = RA[blockIdx.x];
+    decide_body(r.S, r.dip_part, r.skin_blocks, r.dipz, r.force_rebuild, 0, 1);
+}
+__global__ __launch_bounds__(256) void k_cell_count_b(const RepArgs *__restrict__ RA, const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    const RepArgs &r = RA[m.x];
+    cell_count_body(r.S, r.lead_cell, m.y, r.lead_blocks);
+}
+__global__ __launch_bounds__(1024) void k_cell_scan_build_b(const RepArgs *__restrict__ RA) {
+    cell_scan_build_body(RA[blockIdx.x].S, 0, 1);
+}
+__global__ __launch_bounds__(256) void k_cell_fill_b(const RepArgs *__restrict__ RA, const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    const RepArgs &r = RA[m.x];
+    cell_fill_body(r.lead_cell, r.S.n_lead, r.S.cell_cursor, r.cell_items2, r.S.flags, 1, m.y, r.lead_blocks);
+}
+__global__ __launch_bounds__(64) void k_cell_sort_b(const RepArgs *__restrict__ RA, const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    const RepArgs &r = RA[m.x];
+    cell_sort_body(r.S.cell_start, r.cell_items2, r.S.cell_items, r.S.flags, 1, r.ncells_xy, 0, m.y, r.sort_blocks);
+}
+__global__ __launch_bounds__(256) void k_build_b(const RepArgs *__restrict__ RA, const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    build_body(RA[m.x].S, m.y, RA[m.x].build_blocks);
+}
+__global__ __launch_bounds__(256) void k_snapshot_b(const RepArgs *__restrict__ RA, const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    snapshot_body(RA[m.x].S, m.y, RA[m.x].snap_blocks);
+}
+// every system's flags side by side, for one copy to the host
+__global__ void k_flags_b(const RepArgs *__restrict__ RA, int n, DevFlags *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = *RA[i].S.flags;
+}
 }  // namespace mdf

@@ -98,13 +98,13 @@ __device__ __forceinline__ void vdw_eval(int kind, R rsq, R eps4, R sig6, R A, R
 __device__ __forceinline__ double shfl_xor_d(double v, int m) { return __shfl_xor(v, m, 64); }
 
 template <typename R, int G, bool VIR, bool ROLLED>
-__global__ __launch_bounds__(256) void k_pair(DevSys S) {
+__device__ __forceinline__ void pair_body(const DevSys &S, int bid, int nblk) {
     // per-lane energy-by-class and wall-force accumulators live in LDS slots owned by the
     // lane (no sharing, no atomics); only the site forces and the virial stay in registers
     __shared__ double s_e[2 * NCLS][256];
     __shared__ double s_w[6][256];
     const int tid = threadIdx.x;
-    const int gid = blockIdx.x * (blockDim.x / G) + tid / G;
+    const int gid = bid * (blockDim.x / G) + tid / G;
     const int sub = tid % G;
     const bool active = gid < S.n_lead;
 #pragma unroll
@@ -321,6 +321,11 @@ __global__ __launch_bounds__(256) void k_pair(DevSys S) {
     }
 }
 
+template <typename R, int G, bool VIR, bool ROLLED>
+__global__ __launch_bounds__(256) void k_pair(DevSys S) {
+    pair_body<R, G, VIR, ROLLED>(S, blockIdx.x, gridDim.x);
+}
+
 #define MDF_PAIR_INST(G)                                            \
     template __global__ void k_pair<double, G, true, false>(DevSys);  \
     template __global__ void k_pair<float, G, true, false>(DevSys);   \
@@ -342,4 +347,22 @@ __global__ void k_zero_forces(DevSys S) {
     if (t < S.n_move) { S.fx[t] = 0.0; S.fy[t] = 0.0; S.fz[t] = 0.0; }
 }
 
+}  // namespace mdf
+
+namespace mdf {
+// batched form (one lanes-per-lead choice for the whole batch; unrolled site-pair loop)
+template <typename R, int G, bool VIR>
+__global__ __launch_bounds__(256) void k_pair_b(const RepArgs *__restrict__ RA, const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    pair_body<R, G, VIR, false>(RA[m.x].S, m.y, 0);
+}
+#define MDF_PAIR_B_INST(G)                                                                   \
+    template __global__ void k_pair_b<double, G, true>(const RepArgs *, const int2 *);    \
+    template __global__ void k_pair_b<double, G, false>(const RepArgs *, const int2 *);   \
+    template __global__ void k_pair_b<float, G, true>(const RepArgs *, const int2 *);     \
+    template __global__ void k_pair_b<float, G, false>(const RepArgs *, const int2 *);
+MDF_PAIR_B_INST(8)
+MDF_PAIR_B_INST(16)
+MDF_PAIR_B_INST(32)
+#undef MDF_PAIR_B_INST
 }  // namespace mdf

@@ -103,15 +103,15 @@ __device__ __forceinline__ void row_phase_reload(bool fresh, int kl, int km, con
 constexpr int SF_KC = 8;   // kn per accumulation chunk
 
 template <typename R, int NS>
-__global__ __launch_bounds__(SF_BLOCK) void k_sf(const double *__restrict__ x, const double *__restrict__ y,
+__device__ __forceinline__ void sf_body(const double *__restrict__ x, const double *__restrict__ y,
                                                  const double *__restrict__ z, const double *__restrict__ q, int n,
                                                  double cux, double cuy, double cuz,
                                                  const KRow *__restrict__ rows, int n_rows, int rpb,
-                                                 double2 *__restrict__ partial, int nk) {
+                                                 double2 *__restrict__ partial, int nk, int bx, int by, int nbx, int nby) {
     constexpr int NW = SF_BLOCK / 64;
     __shared__ double red[NW][2 * SF_KC][64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int sb = blockIdx.x;
+    const int sb = bx;
     double su[NS], sv[NS];
     R sq[NS], wc[NS], ws[NS], vc[NS], vs[NS], Pr[NS], Pi[NS];
 #pragma unroll
@@ -123,7 +123,7 @@ __global__ __launch_bounds__(SF_BLOCK) void k_sf(const double *__restrict__ x, c
         sincospi_r<R>(w, ws[m], wc[m]);
         sincospi_r<R>(sv[m], vs[m], vc[m]);
     }
-    const int r_begin = blockIdx.y * rpb, r_end = min(n_rows, r_begin + rpb);
+    const int r_begin = by * rpb, r_end = min(n_rows, r_begin + rpb);
     int pkl = -1, pkm = 0;
     for (int r = r_begin; r < r_end; r++) {
         const KRow row = rows[r];
@@ -187,6 +187,15 @@ __global__ __launch_bounds__(SF_BLOCK) void k_sf(const double *__restrict__ x, c
     }
 }
 
+template <typename R, int NS>
+__global__ __launch_bounds__(SF_BLOCK) void k_sf(const double *__restrict__ x, const double *__restrict__ y,
+                                                 const double *__restrict__ z, const double *__restrict__ q, int n,
+                                                 double cux, double cuy, double cuz,
+                                                 const KRow *__restrict__ rows, int n_rows, int rpb,
+                                                 double2 *__restrict__ partial, int nk) {
+    sf_body<R, NS>(x, y, z, q, n, cux, cuy, cuz, rows, n_rows, rpb, partial, nk, blockIdx.x, blockIdx.y, gridDim.x, gridDim.y);
+}
+
 template __global__ void k_sf<double, 4>(const double *, const double *, const double *, const double *, int, double,
                                         double, double, const KRow *, int, int, double2 *, int);
 template __global__ void k_sf<float, 4>(const double *, const double *, const double *, const double *, int, double,
@@ -196,15 +205,15 @@ template __global__ void k_sf<float, 4>(const double *, const double *, const do
 // partial energies and virial:  eK = A|S|^2, eLL = A|S_moving|^2, vir_ab = 1/2 eK (d_ab - g k_a k_b).
 // A block owns SFR_K k-vectors; its four waves each sum every fourth site block (fixed
 // order), the first wave combines the four slices and finishes the k-vectors.
-__global__ __launch_bounds__(256) void k_sf_reduce(const double2 *__restrict__ partial, int n_sb, int nk,
+__device__ __forceinline__ void sf_reduce_body(const double2 *__restrict__ partial, int n_sb, int nk,
                                                    const double2 *__restrict__ s_wall, const double *__restrict__ ak,
                                                    const int *__restrict__ kvec, double pipix, double pipiy,
                                                    double pipiz, double lamb, double2 *__restrict__ s_out,
-                                                   double4 *__restrict__ t_out, double *__restrict__ epart) {
+                                                   double4 *__restrict__ t_out, double *__restrict__ epart, int bid, int nblk) {
     constexpr int SL = 256 / SFR_K;
     __shared__ double2 slice[SL][SFR_K];
     const int kk = threadIdx.x % SFR_K, sl = threadIdx.x / SFR_K;
-    const int k = blockIdx.x * SFR_K + kk;
+    const int k = bid * SFR_K + kk;
     double sr = 0.0, si = 0.0;
     if (k < nk) {
 #pragma unroll 4
@@ -248,7 +257,15 @@ __global__ __launch_bounds__(256) void k_sf_reduce(const double2 *__restrict__ p
         for (int o = 32; o > 0; o >>= 1) acc[c] += __shfl_xor(acc[c], o, 64);
     if (kk == 0)
 #pragma unroll
-        for (int c = 0; c < RE_N; c++) epart[blockIdx.x * RE_N + c] = acc[c];
+        for (int c = 0; c < RE_N; c++) epart[bid * RE_N + c] = acc[c];
+}
+
+__global__ __launch_bounds__(256) void k_sf_reduce(const double2 *__restrict__ partial, int n_sb, int nk,
+                                                   const double2 *__restrict__ s_wall, const double *__restrict__ ak,
+                                                   const int *__restrict__ kvec, double pipix, double pipiy,
+                                                   double pipiz, double lamb, double2 *__restrict__ s_out,
+                                                   double4 *__restrict__ t_out, double *__restrict__ epart) {
+    sf_reduce_body(partial, n_sb, nk, s_wall, ak, kvec, pipix, pipiy, pipiz, lamb, s_out, t_out, epart, blockIdx.x, gridDim.x);
 }
 
 // T4[k] = (A conj S, kn A conj S): per k-vector the force pieces are Im(T e) and kn Im(T e),
@@ -267,14 +284,14 @@ __device__ __forceinline__ void bp_acc(const double4 &t, R er, R ei, R &a1, R &a
 constexpr int BP_LDS_K = 1024;
 
 template <typename R, int NS>
-__global__ __launch_bounds__(256) void k_bp(const double *__restrict__ x, const double *__restrict__ y,
+__device__ __forceinline__ void bp_body(const double *__restrict__ x, const double *__restrict__ y,
                                             const double *__restrict__ z, const double *__restrict__ q, int n,
                                             double cux, double cuy, double cuz, const KRow *__restrict__ rows,
                                             const int *__restrict__ row_start, const double4 *__restrict__ T,
-                                            double pipix, double pipiy, double pipiz, double *__restrict__ fpart) {
+                                            double pipix, double pipiy, double pipiz, double *__restrict__ fpart, int bx, int by, int nbx, int nby) {
     // NS sites per thread: every k datum read from LDS serves NS sites
     __shared__ double4 sT[BP_LDS_K + 2];
-    const int g = blockIdx.y;
+    const int g = by;
     const int r_begin = row_start[g], r_end = row_start[g + 1];
     const int k_begin = rows[r_begin].koff;
     const int k_count = rows[r_end - 1].koff + (rows[r_end - 1].kn1 - rows[r_end - 1].kn0 + 1) - k_begin;
@@ -286,7 +303,7 @@ __global__ __launch_bounds__(256) void k_bp(const double *__restrict__ x, const 
     double fx[NS], fy[NS], fz[NS];
 #pragma unroll
     for (int m = 0; m < NS; m++) {
-        idx[m] = blockIdx.x * blockDim.x * NS + m * blockDim.x + threadIdx.x;
+        idx[m] = bx * blockDim.x * NS + m * blockDim.x + threadIdx.x;
         const bool ok = idx[m] < n;
         const double w = ok ? z[idx[m]] * cuz : 0.0, v = ok ? y[idx[m]] * cuy : 0.0;
         sincospi_r<R>(w, ws[m], wc[m]);
@@ -348,6 +365,15 @@ __global__ __launch_bounds__(256) void k_bp(const double *__restrict__ x, const 
     }
 }
 
+template <typename R, int NS>
+__global__ __launch_bounds__(256) void k_bp(const double *__restrict__ x, const double *__restrict__ y,
+                                            const double *__restrict__ z, const double *__restrict__ q, int n,
+                                            double cux, double cuy, double cuz, const KRow *__restrict__ rows,
+                                            const int *__restrict__ row_start, const double4 *__restrict__ T,
+                                            double pipix, double pipiy, double pipiz, double *__restrict__ fpart) {
+    bp_body<R, NS>(x, y, z, q, n, cux, cuy, cuz, rows, row_start, T, pipix, pipiy, pipiz, fpart, blockIdx.x, blockIdx.y, gridDim.x, gridDim.y);
+}
+
 template __global__ void k_bp<double, 4>(const double *, const double *, const double *, const double *, int, double,
                                          double, double, const KRow *, const int *, const double4 *, double, double,
                                          double, double *);
@@ -355,4 +381,36 @@ template __global__ void k_bp<float, 4>(const double *, const double *, const do
                                         double, double, const KRow *, const int *, const double4 *, double, double,
                                         double, double *);
 
+}  // namespace mdf
+
+namespace mdf {
+// batched forms
+template <typename R>
+__global__ __launch_bounds__(SF_BLOCK) void k_sf_b(const RepArgs *__restrict__ RA, const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    const RepArgs &r = RA[m.x];
+    const int bx = m.y % r.n_sb, by = m.y / r.n_sb;
+    sf_body<R, 4>(r.S.x, r.S.y, r.S.z, r.S.q, r.S.n_move, r.cux, r.cuy, r.cuz, r.rows, r.nrows, r.cpb, r.sf_partial,
+                  r.nk, bx, by, r.n_sb, r.n_cg);
+}
+template __global__ void k_sf_b<double>(const RepArgs *, const int2 *);
+template __global__ void k_sf_b<float>(const RepArgs *, const int2 *);
+
+__global__ __launch_bounds__(256) void k_sf_reduce_b(const RepArgs *__restrict__ RA, const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    const RepArgs &r = RA[m.x];
+    sf_reduce_body(r.sf_partial, r.S.n_move > 0 ? r.n_sb : 0, r.nk, r.s_wall, r.ak, r.kvec, r.pipix, r.pipiy, r.pipiz,
+                   r.lamb, r.s_out, r.t_out, r.kpart, m.y, r.n_kblk);
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_bp_b(const RepArgs *__restrict__ RA, const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    const RepArgs &r = RA[m.x];
+    const int bx = m.y % r.n_bpsb, by = m.y / r.n_bpsb;
+    bp_body<R, 4>(r.S.x, r.S.y, r.S.z, r.S.q, r.S.n_move, r.cux, r.cuy, r.cuz, r.rows, r.bp_row_start, r.t_out,
+                  r.pipix, r.pipiy, r.pipiz, r.fpart, bx, by, r.n_bpsb, r.n_rg);
+}
+template __global__ void k_bp_b<double>(const RepArgs *, const int2 *);
+template __global__ void k_bp_b<float>(const RepArgs *, const int2 *);
 }  // namespace mdf

@@ -53,9 +53,11 @@ def lib() -> C.CDLL:
         for fn in ("mdf_create", "mdf_compute", "mdf_compute_device", "mdf_read_outputs", "mdf_get_pairs",
                    "mdf_get_kvectors", "mdf_get_structure_factor", "mdf_stage_times", "mdf_device_results",
                    "mdf_zero", "mdf_reset_stage_times", "mdf_set_stage_timing", "mdf_autotune",
-                   "mdf_set_launch_shapes"):
+                   "mdf_set_launch_shapes", "mdf_batch_create", "mdf_batch_compute_device",
+                   "mdf_batch_read_outputs"):
             getattr(L, fn).restype = C.c_int
         L.mdf_destroy.restype = None
+        L.mdf_batch_destroy.restype = None
         _lib = L
     return _lib
 
@@ -63,6 +65,16 @@ def lib() -> C.CDLL:
 def _check(rc: int):
     if rc != abi.MDF_OK:
         raise MDForceError(rc, lib().mdf_last_error().decode())
+
+
+def _producer_stream(stream):
+    if stream is not None:
+        return int(stream)
+    import sys as _sys
+    torch = _sys.modules.get("torch")
+    if torch is not None and torch.cuda.is_available():
+        return int(torch.cuda.current_stream().cuda_stream) or None
+    return None
 
 
 def _dp(a):
@@ -95,10 +107,13 @@ class ForceEngine:
         _check(lib().mdf_zero(self._h))
 
     # -- device-resident path ------------------------------------------------------------
-    def compute_device(self, d_x: int, d_y: int, d_z: int, d_cx: int, d_cy: int, d_cz: int):
-        """Enqueue a step on device pointers (ints, e.g. torch tensor.data_ptr())."""
+    def compute_device(self, d_x: int, d_y: int, d_z: int, d_cx: int, d_cy: int, d_cz: int, stream=None):
+        """Enqueue a step on device pointers (ints, e.g. torch tensor.data_ptr()); it starts
+        after the work already on ``stream`` (a hipStream_t as int; default: torch's current
+        stream when torch is loaded, else the null stream)."""
         vp = C.c_void_p
-        _check(lib().mdf_compute_device(self._h, vp(d_x), vp(d_y), vp(d_z), vp(d_cx), vp(d_cy), vp(d_cz), None))
+        _check(lib().mdf_compute_device(self._h, vp(d_x), vp(d_y), vp(d_z), vp(d_cx), vp(d_cy), vp(d_cz),
+                                        vp(_producer_stream(stream))))
 
     def read_outputs(self) -> abi.ForceResult:
         o, arrs = abi.make_outputs(self.n_move, self.n_rig, self.n_pairs)
@@ -157,6 +172,47 @@ class ForceEngine:
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
             lib().mdf_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ForceBatch:
+    """Independent systems (``ForceEngine`` handles) stepped together: one launch per stage
+    over all of them, one graph per step."""
+
+    def __init__(self, engines):
+        self.engines = list(engines)
+        n = len(self.engines)
+        arr = (C.c_void_p * n)(*[e._h.value for e in self.engines])
+        self._h = C.c_void_p()
+        _check(lib().mdf_batch_create(arr, n, C.byref(self._h)))
+
+    def compute_device(self, ptrs, stream=None):
+        """ptrs[i] = (x, y, z, cx, cy, cz) device pointers (ints) of system i; ordered after
+        ``stream`` as ForceEngine.compute_device."""
+        n = len(self.engines)
+        cols = [(C.c_void_p * n)(*[int(p[c]) if p[c] else None for p in ptrs]) for c in range(6)]
+        _check(lib().mdf_batch_compute_device(self._h, *cols, C.c_void_p(_producer_stream(stream))))
+
+    def read_outputs(self):
+        n = len(self.engines)
+        outs = (abi.Outputs * n)()
+        keep = []
+        for i, e in enumerate(self.engines):
+            o, arrs = abi.make_outputs(e.n_move, e.n_rig, e.n_pairs)
+            outs[i] = o
+            keep.append(arrs)
+        _check(lib().mdf_batch_read_outputs(self._h, outs))
+        return [abi.collect(outs[i], keep[i], e.n_move, e.n_rig) for i, e in enumerate(self.engines)]
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().mdf_batch_destroy(self._h)
             self._h = C.c_void_p()
 
     def __del__(self):

@@ -440,6 +440,34 @@ def rocksalt_bulk(ncell: int = 4, a: float = 5.64, rcut: float = 11.0, kmax: int
     return sysm
 
 
+def replica_set(n: int = 64, seed: int = 3000, a: float = 5.64):
+    """Independent interface systems of uneven size for batched stepping: per replica (seeded
+    with seed + r) a rock-salt slab of 6-12 cells laterally (4 ncell^2 ions per layer, 256-2048
+    ions in all), 512-4096 waters (fewer when the film would not fit the box), Lz = 3 Lx,
+    |k| range of the 2048-water system (k-index 8 at 12 cells), rc 10 A, a temperature in
+    280-360 K for the velocities."""
+    out = []
+    for r in range(n):
+        rng = np.random.default_rng(seed + r)
+        ncell = int(rng.integers(6, 13))
+        lx = ncell * a
+        per_layer_ions = 4 * ncell * ncell
+        nlayers = int(np.clip(round(rng.uniform(256, 2048) / per_layer_ions), 2, 8))
+        while per_layer_ions * nlayers > 2048 and nlayers > 1:
+            nlayers -= 1
+        per_layer_w = max(1, int(round(lx / 3.1))) ** 2
+        lz = 3.0 * lx
+        max_w = per_layer_w * int((0.5 * lz - 8.0) / 3.1)
+        n_water = int(min(rng.integers(512, 4097), max_w))
+        height = -(-n_water // per_layer_w) * 3.1
+        lw = min(0.6 * lz, height + 20.0)
+        kmax = max(4, int(round(8 * lx / (12 * a))))
+        T = float(rng.uniform(280.0, 360.0))
+        out.append(water_on_slab(n_water, ncell, nlayers, seed=seed + r, rcut=10.0, kmax=kmax, lw=lw, T=T,
+                                 name="replica%02d" % r))
+    return out
+
+
 # ---------------------------------------------------------------------------
 # the named workloads
 # ---------------------------------------------------------------------------

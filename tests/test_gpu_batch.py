@@ -1,0 +1,115 @@
+"""Batches of independent systems: one launch per stage over every system of the batch."""
+import sys
+
+import numpy as np
+import pytest
+
+from md_sim_amd import abi, systems
+from helpers import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _dev(s, pos):
+    import torch
+    return [torch.tensor(a[:s.nmove] if k < 3 else (a if len(a) else np.zeros(1)), dtype=torch.float64,
+                         device="cuda:0") for k, a in enumerate(pos)]
+
+
+def _ptrs(ts, s):
+    return tuple(t.data_ptr() for t in ts[:3]) + (tuple(t.data_ptr() for t in ts[3:]) if s.nrig else (0, 0, 0))
+
+
+@pytest.mark.parametrize("precision", [abi.FP64, abi.FP32_ACC64])
+def test_batch_equals_single_handles(precision):
+    """A mixed batch (slab, bulk 3-D form, water-free crystal, replicas of uneven size) gives
+    bit for bit what each handle gives alone."""
+    from md_sim_amd.engine import ForceBatch, ForceEngine
+    _say("case batch vs single, precision", precision)
+    syss = [systems.config("pore_small"), systems.config("bulk216"), systems.rocksalt_bulk(2, rcut=5.5, kmax=5)]
+    syss += systems.replica_set(3, seed=4100)
+    kws = [{}, {}, dict(z_mode=abi.Z_PERIODIC)] + [{}] * 3
+    engines, singles, dev = [], [], []
+    for s, kw in zip(syss, kws):
+        e = ForceEngine(s, precision=precision, **kw)
+        pos = abi.positions(s)
+        singles.append(e.force_get(*pos))
+        engines.append(e)
+        dev.append(_dev(s, pos))
+    b = ForceBatch(engines)
+    for rep in range(3):
+        b.compute_device([_ptrs(d, s) for d, s in zip(dev, syss)])
+        outs = b.read_outputs()
+        for i, (o, r) in enumerate(zip(outs, singles)):
+            assert np.array_equal(o["f_site"], r["f_site"]), (rep, i)
+            assert np.array_equal(o["torque"], r["torque"]), (rep, i)
+            assert o["e_conf"] == r["e_conf"] and o["e_recip"] == r["e_recip"], (rep, i)
+            assert np.array_equal(o["e_vdw"], r["e_vdw"]) and np.array_equal(o["e_coul_real"], r["e_coul_real"])
+            assert o["n_pairs_listed"] == r["n_pairs_listed"], (rep, i)
+    b.close()
+
+
+def test_batch_trajectory_matches_oracle(oracle_lib):
+    """Replicas moving independently: rebuilds fire per replica as the reference rule says and
+    every replica's forces stay at the FP64 bar."""
+    import torch
+    from md_sim_amd.engine import ForceBatch, ForceEngine
+    _say("case batch trajectory")
+    syss = systems.replica_set(4, seed=4200)
+    engines = [ForceEngine(s) for s in syss]
+    oracles = [oracle_lib.Oracle(s) for s in syss]
+    b = ForceBatch(engines)
+    rng = np.random.default_rng(5)
+    vel = [rng.normal(0, 0.06, size=(s.nrig, 3)) for s in syss]
+    builds = np.zeros(len(syss), int)
+    for t in range(1, 7):
+        devs, hosts = [], []
+        for s, v in zip(syss, vel):
+            x, y, z, cx, cy, cz = abi.positions(s)
+            d = np.repeat(v * t, 3, axis=0)
+            x, y, z = x.copy(), y.copy(), z.copy()
+            x[:s.nriga] += d[:, 0]; y[:s.nriga] += d[:, 1]; z[:s.nriga] += d[:, 2]
+            c = (cx + v[:, 0] * t, cy + v[:, 1] * t, cz + v[:, 2] * t)
+            hosts.append((x, y, z) + c)
+            devs.append(_dev(s, hosts[-1]))
+        b.compute_device([_ptrs(d, s) for d, s in zip(devs, syss)])
+        outs = b.read_outputs()
+        for i, (o, s) in enumerate(zip(outs, syss)):
+            ro = oracles[i].compute(*hosts[i])
+            assert o["list_rebuilt"] == ro["list_rebuilt"], (t, i)
+            builds[i] += o["list_rebuilt"]
+            assert rel_err(o["f_site"], ro["f_site"]) < 1e-10, (t, i)
+            assert abs(o["e_conf"] - ro["e_conf"]) < 1e-10 * abs(ro["e_conf"]), (t, i)
+        torch.cuda.synchronize()
+    assert builds.sum() > 0
+    b.close()
+
+
+def test_batch_list_overflow_reports_error():
+    """A system of the batch with a fixed, too small list capacity stops the batch read with
+    the list-overflow status."""
+    from md_sim_amd.engine import ForceBatch, ForceEngine, MDForceError
+    _say("case batch overflow")
+    s_ok = systems.config("pore_small")
+    e_ok = ForceEngine(s_ok)
+    s_bad = systems.replica_set(1, seed=4300)[0]
+    e_bad = ForceEngine(s_bad, max_neighbors=1024)   # ample at creation ...
+    b = ForceBatch([e_ok, e_bad])
+    # ... then squeeze the second system's molecules together so its rows outgrow the cap
+    x, y, z, cx, cy, cz = abi.positions(s_bad)
+    f = 0.45
+    xs, ys, zs = x.copy(), y.copy(), z.copy()
+    off = np.repeat(np.stack([cx, cy, cz], 1) * (f - 1.0), 3, axis=0)
+    xs[:s_bad.nriga] += off[:, 0]; ys[:s_bad.nriga] += off[:, 1]; zs[:s_bad.nriga] += off[:, 2]
+    bad = _dev(s_bad, (xs, ys, zs, cx * f, cy * f, cz * f))
+    ok = _dev(s_ok, abi.positions(s_ok))
+    b.compute_device([_ptrs(ok, s_ok), _ptrs(bad, s_bad)])
+    with pytest.raises(MDForceError) as ei:
+        b.read_outputs()
+    assert ei.value.code == 3
+    b.close()

@@ -4,7 +4,7 @@ import sys
 import numpy as np
 import pytest
 
-from md_sim_amd import systems
+from md_sim_amd import abi, systems
 from helpers import golden, golden_positions, rel_err, rms_rel
 
 pytestmark = pytest.mark.gpu
@@ -111,3 +111,21 @@ def test_engine_matches_reference_sample():
     l1, l2, w1, w2 = e.pairs()
     assert pair_digest(l1 + 1, l2 + 1) == str(g["list_digest"])
     assert pair_digest(w1 + 1, w2 + 1) == str(g["wlist_digest"])
+
+
+def test_engine_thin_crystal_wall_pairs(oracle_lib):
+    """A two-layer crystal thinner than the list radius: every lead-frozen pair within the
+    radius is listed (the frozen-site cells are at least that high)."""
+    print("case thin crystal", flush=True)
+    s = systems.replica_set(1, seed=4200)[0]
+    assert s.wall_pos[:, 2].max() - s.wall_pos[:, 2].min() < s.rcut
+    pos = abi.positions(s)
+    e = _engine(s)
+    r = e.force_get(*pos)
+    o = oracle_lib.Oracle(s)
+    ro = o.compute(*pos)
+    assert r["n_wall_pairs_listed"] == ro["n_wall_pairs_listed"]
+    l1, l2, w1, w2 = e.pairs()
+    m1, m2, u1, u2 = o.pairs()
+    assert np.array_equal(w1, u1) and np.array_equal(w2, u2)
+    assert rel_err(r["f_site"], ro["f_site"]) < TOL
