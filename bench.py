@@ -55,6 +55,11 @@ def parse():
     ap.add_argument("--amplitude", type=float, default=0.25,
                     help="rms per-axis oscillation amplitude of a molecule (A)")
     ap.add_argument("--period", type=float, default=80.0, help="oscillation period (steps)")
+    ap.add_argument("--replicas", type=int, default=64,
+                    help="replica batch line: this many independent interface systems of uneven size "
+                         "(0: skip)")
+    ap.add_argument("--replica-steps", type=int, default=50)
+    ap.add_argument("--replica-warmup", type=int, default=5)
     return ap.parse_args()
 
 
@@ -82,7 +87,7 @@ def cpu_baseline(system, reps: int):
                        "reference (oracle/oracle.c), one thread" % (reps, system.name))
 
 
-def _trajectory(s, args, dev):
+def _trajectory(s, args, dev, n_frames=None):
     """Device frames of the synthetic trajectory: every molecule (and mobile ion) oscillates
     rigidly about its start, x(t) = x0 + a sin(2 pi t / P), a ~ N(0, amp) per axis, so the
     skin test fires and the list is rebuilt repeatedly while molecules never collide."""
@@ -101,7 +106,7 @@ def _trajectory(s, args, dev):
     amp = torch.tensor(site_a.T.copy(), dtype=torch.float64, device=dev)
     cbase = torch.tensor(np.stack([cx, cy, cz]), dtype=torch.float64, device=dev)
     camp = torch.tensor(mol_a.T.copy(), dtype=torch.float64, device=dev)
-    n_frames = args.warmup + args.steps
+    n_frames = n_frames or args.warmup + args.steps
     frames = torch.empty((n_frames, 3, nm), dtype=torch.float64, device=dev)
     cframes = torch.empty((n_frames, 3, max(nr, 1)), dtype=torch.float64, device=dev)
     for t in range(n_frames):
@@ -147,6 +152,87 @@ def _run(eng, frames, cframes, args):
     eng.set_stage_timing(False)
     stages["step_graph"] = step_gpu_ms
     return sec, first, last, stages
+
+
+def replica_bench(args, prec):
+    """Replica batch: independent interface systems of uneven size (512-4096 waters on
+    256-2048 frozen ions, each its own box, list and k-set, seeded per replica) stepped by one
+    batch (one launch per stage over all of them, one graph per step), against the same
+    systems stepped as independent handles (one graph per system per step, each on its own
+    stream).  Both run the same oscillating trajectories; each picks its launch shapes with
+    its autotuner first."""
+    import numpy as np
+    import torch
+    from md_sim_amd import systems
+    from md_sim_amd.engine import ForceBatch, ForceEngine
+    dev = torch.device("cuda:0")
+    syss = systems.replica_set(args.replicas, seed=3000)
+    nf = args.replica_warmup + args.replica_steps
+    traj = [_trajectory(s, args, dev, nf) for s in syss]
+    engines = [ForceEngine(s, precision=prec) for s in syss]
+
+    def ptrs(t):
+        return [(f[t][0].data_ptr(), f[t][1].data_ptr(), f[t][2].data_ptr(),
+                 c[t][0].data_ptr(), c[t][1].data_ptr(), c[t][2].data_ptr()) for f, c in traj]
+
+    # independent handles
+    for e, p in zip(engines, ptrs(0)):
+        e.compute_device(*p)
+        e.read_outputs()
+        e.autotune(reps=5)
+    for t in range(args.replica_warmup):
+        for e, p in zip(engines, ptrs(t)):
+            e.compute_device(*p)
+    for e in engines:
+        e.read_outputs()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for t in range(args.replica_warmup, nf):
+        for e, p in zip(engines, ptrs(t)):
+            e.compute_device(*p)
+    single = [e.read_outputs() for e in engines]
+    torch.cuda.synchronize()
+    sec_single = time.perf_counter() - t0
+
+    # one batch
+    b = ForceBatch(engines)
+    b.compute_device(ptrs(0))
+    b.read_outputs()
+    shapes = b.autotune(reps=5)
+    for t in range(args.replica_warmup):
+        b.compute_device(ptrs(t))
+    b.read_outputs()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for t in range(args.replica_warmup, nf):
+        b.compute_device(ptrs(t))
+    batch = b.read_outputs()
+    torch.cuda.synchronize()
+    sec_batch = time.perf_counter() - t0
+    b.close()
+    dif = max(abs(o["e_conf"] - r["e_conf"]) / abs(r["e_conf"]) for o, r in zip(batch, single))
+    ferr = max(float(np.sqrt(((o["f_site"] - r["f_site"]) ** 2).mean() / (r["f_site"] ** 2).mean()))
+               for o, r in zip(batch, single))
+    nw = [s.nrig for s in syss]
+    ni = [s.nwall for s in syss]
+    out = {
+        "workload": "%d replicas: %d-%d SPC/E waters on %d-%d frozen ions (%d waters, %d ions in all), "
+                    "T 280-360 K, own box, list and k-set each" % (len(syss), min(nw), max(nw), min(ni), max(ni),
+                                                                   sum(nw), sum(ni)),
+        "steps": args.replica_steps,
+        "batch_steps_per_s": args.replica_steps / sec_batch,
+        "batch_system_steps_per_s": args.replica_steps * len(syss) / sec_batch,
+        "independent_handles_steps_per_s": args.replica_steps / sec_single,
+        "batch_speedup_vs_independent_handles": sec_single / sec_batch,
+        "batch_launch_shapes": shapes,
+        "max_e_conf_rel_diff_batch_vs_handles": dif,
+        "max_force_rms_rel_diff_batch_vs_handles": ferr,
+    }
+    for e in engines:
+        e.close()
+    del traj
+    torch.cuda.empty_cache()
+    return out
 
 
 def main():
@@ -252,6 +338,8 @@ def main():
     }
     if fp64 is not None:
         out["fp64"] = fp64
+    if args.replicas > 0:
+        out["replica_batch"] = replica_bench(args, prec)
     if cpu and "value" in cpu:
         out["speedup_vs_cpu"] = out["value"] / cpu["value"]
         if fp64 is not None:

@@ -183,18 +183,12 @@ void mdf_destroy(mdf_system *sys);
 /* Message for the last failing call (per thread).                                    */
 const char *mdf_last_error(void);
 
-/* Timing.  Steps normally replay one captured HIP graph; with stage timing on they use
-   direct launches with HIP events between the stages instead (events recorded during
-   graph capture cannot be timed).  mdf_stage_times gives names and mean milliseconds per
-   step over the steps since mdf_reset_stage_times: the stages (from steps run with stage
-   timing on) followed by "step", the whole step measured in both modes.  cap: capacity
-   of names/ms; n out: number of entries.                                              */
 /* ---- batches of independent systems (no counterpart in the reference) ----------------
    Many systems - replicas of uneven size, each with its own box, lists and k-set - stepped
    together: every stage of the step is one launch over all of them, and the whole step is
    one replayed graph.  The systems stay owned by the caller and must not be stepped on their
    own while the batch is in flight.  All must share device, precision and the virial switch;
-   the batch uses the first system's pair lanes per lead. */
+   the batch uses the first system's pair lanes per lead until mdf_batch_autotune picks its own. */
 typedef struct mdf_batch mdf_batch;
 
 int mdf_batch_create(mdf_system *const *systems, int32_t n, mdf_batch **out);
@@ -209,6 +203,15 @@ int mdf_batch_compute_device(mdf_batch *b, const double *const *x, const double 
 /* Wait for the step and copy every system's results into outs[0 .. n-1] (as
    mdf_read_outputs).  A system whose pair list overflowed is grown and redone alone. */
 int mdf_batch_read_outputs(mdf_batch *b, mdf_outputs *outs);
+
+/* Pick the batch's launch shapes by timing whole batch steps on the inputs of the last
+   mdf_batch_compute_device (those arrays must still hold valid positions): lanes per lead of
+   the pair kernel, and the structure-factor and back-projection blocks per launch of the
+   whole batch, shared out to the systems in proportion to their k-vectors x moving sites.
+   Each candidate runs reps + 1 times.  chosen (optional, 4 entries) receives lanes per lead,
+   0, structure-factor and back-projection blocks of the batch.  Results of the last step are
+   overwritten: call it between steps.  No counterpart in the reference. */
+int mdf_batch_autotune(mdf_batch *b, int32_t reps, int32_t *chosen);
 
 void mdf_batch_destroy(mdf_batch *b);
 
@@ -227,6 +230,12 @@ int mdf_autotune(mdf_system *sys, int32_t reps, int32_t *chosen);
 int mdf_set_launch_shapes(mdf_system *sys, int32_t pair_lanes, int32_t pair_rolled, int32_t sf_blocks,
                           int32_t bp_blocks);
 
+/* Timing.  Steps normally replay one captured HIP graph; with stage timing on they use
+   direct launches with HIP events between the stages instead (events recorded during
+   graph capture cannot be timed).  mdf_stage_times gives names and mean milliseconds per
+   step over the steps since mdf_reset_stage_times: the stages (from steps run with stage
+   timing on) followed by "step", the whole step measured in both modes.  cap: capacity
+   of names/ms; n out: number of entries.                                              */
 int mdf_set_stage_timing(mdf_system *sys, int32_t on);
 int mdf_stage_times(mdf_system *sys, const char **names, float *ms, int32_t cap, int32_t *n);
 int mdf_reset_stage_times(mdf_system *sys);

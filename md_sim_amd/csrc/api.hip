@@ -1661,6 +1661,89 @@ int mdf_batch_read_outputs(mdf_batch *b, mdf_outputs *outs) {
     return MDF_OK;
 }
 
+int mdf_batch_autotune(mdf_batch *b, int32_t reps, int32_t *chosen) {
+    if (!b) return fail(MDF_ERR_ARG, "null batch");
+    if (reps <= 0) reps = 5;
+    const int n = (int)b->sys.size();
+    for (int i = 0; i < n; i++)
+        if (!b->h_in[6 * i])
+            return fail(MDF_ERR_ARG, "tuning times steps on the last inputs: call mdf_batch_compute_device first");
+    HIPCHK(hipSetDevice(b->device));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    b->pending = false;
+    // reciprocal work share of each system: k-vectors x moving sites
+    std::vector<double> w(n);
+    double wt = 0.0;
+    for (int i = 0; i < n; i++) {
+        HIPCHK(hipStreamSynchronize(b->sys[i]->stream));
+        w[i] = (double)b->sys[i]->ak.size() * b->sys[i]->top.n_move;
+        wt += w[i];
+    }
+    wt = std::max(wt, 1.0);
+    // new block maps and per-system tables; the captured step holds the old maps
+    auto reshape = [&](int sf_total, int bp_total) -> int {
+        if (b->graph) { (void)hipGraphExecDestroy(b->graph); b->graph = nullptr; }
+        for (int i = 0; i < n; i++) {
+            mdf_system *s = b->sys[i];
+            s->sf_target_blocks = std::max(1, (int)llround(sf_total * w[i] / wt));
+            s->bp_target_blocks = std::max(1, (int)llround(bp_total * w[i] / wt));
+            int rc = apply_geometry(s);
+            if (rc) return rc;
+        }
+        return batch_upload(b, true);
+    };
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    int rc = MDF_OK;
+    // mean time of one whole batch step (direct launches), after one untimed step
+    auto timed = [&](float *ms) -> int {
+        batch_launch(b);
+        HIPCHK(hipEventRecord(e0, b->stream));
+        for (int r = 0; r < reps; r++) batch_launch(b);
+        HIPCHK(hipEventRecord(e1, b->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        HIPCHK(hipGetLastError());
+        float t = 0.0f;
+        HIPCHK(hipEventElapsedTime(&t, e0, e1));
+        *ms = t / reps;
+        return MDF_OK;
+    };
+    float best = 1e30f, t = 0.0f;
+    int sf_best = 8192, bp_best = 2048, g_best = b->pair_g;
+    if ((rc = reshape(sf_best, bp_best)) != MDF_OK) goto done;
+    for (int g : {8, 16, 32}) {
+        b->pair_g = g;
+        if ((rc = batch_upload(b, true)) != MDF_OK) goto done;
+        if ((rc = timed(&t)) != MDF_OK) goto done;
+        if (t < best) { best = t; g_best = g; }
+    }
+    b->pair_g = g_best;
+    best = 1e30f;
+    for (int tb : {4096, 8192, 16384, 32768}) {
+        if ((rc = reshape(tb, bp_best)) != MDF_OK) goto done;
+        if ((rc = timed(&t)) != MDF_OK) goto done;
+        if (t < best) { best = t; sf_best = tb; }
+    }
+    best = 1e30f;
+    for (int tb : {1024, 2048, 4096, 8192}) {
+        if ((rc = reshape(sf_best, tb)) != MDF_OK) goto done;
+        if ((rc = timed(&t)) != MDF_OK) goto done;
+        if (t < best) { best = t; bp_best = tb; }
+    }
+    rc = reshape(sf_best, bp_best);
+    if (rc == MDF_OK && chosen) {
+        chosen[0] = b->pair_g;
+        chosen[1] = 0;
+        chosen[2] = sf_best;
+        chosen[3] = bp_best;
+    }
+done:
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
 void mdf_batch_destroy(mdf_batch *b) {
     if (!b) return;
     (void)hipSetDevice(b->device);

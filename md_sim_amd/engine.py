@@ -54,7 +54,7 @@ def lib() -> C.CDLL:
                    "mdf_get_kvectors", "mdf_get_structure_factor", "mdf_stage_times", "mdf_device_results",
                    "mdf_zero", "mdf_reset_stage_times", "mdf_set_stage_timing", "mdf_autotune",
                    "mdf_set_launch_shapes", "mdf_batch_create", "mdf_batch_compute_device",
-                   "mdf_batch_read_outputs"):
+                   "mdf_batch_read_outputs", "mdf_batch_autotune"):
             getattr(L, fn).restype = C.c_int
         L.mdf_destroy.restype = None
         L.mdf_batch_destroy.restype = None
@@ -209,6 +209,15 @@ class ForceBatch:
             keep.append(arrs)
         _check(lib().mdf_batch_read_outputs(self._h, outs))
         return [abi.collect(outs[i], keep[i], e.n_move, e.n_rig) for i, e in enumerate(self.engines)]
+
+    def autotune(self, reps: int = 5) -> dict:
+        """Time whole batch steps on the last step's inputs (still valid on the device) for
+        candidate launch shapes and keep the fastest: pair lanes per lead, and the batch's
+        structure-factor and back-projection blocks per launch, shared out to the systems by
+        their reciprocal work."""
+        ch = (C.c_int32 * 4)()
+        _check(lib().mdf_batch_autotune(self._h, int(reps), ch))
+        return dict(pair_lanes=ch[0], sf_blocks=ch[2], bp_blocks=ch[3])
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:

@@ -113,3 +113,27 @@ def test_batch_list_overflow_reports_error():
         b.read_outputs()
     assert ei.value.code == 3
     b.close()
+
+
+def test_batch_autotune_keeps_results():
+    """The batch tuner changes launch shapes only: after it, the batch still gives each
+    system's forces and energies at the FP64 bar."""
+    from md_sim_amd.engine import ForceBatch, ForceEngine
+    _say("case batch autotune")
+    syss = systems.replica_set(4, seed=4400)
+    engines = [ForceEngine(s) for s in syss]
+    dev = [_dev(s, abi.positions(s)) for s in syss]
+    ref = [e.force_get(*abi.positions(s)) for e, s in zip(engines, syss)]
+    b = ForceBatch(engines)
+    b.compute_device([_ptrs(d, s) for d, s in zip(dev, syss)])
+    b.read_outputs()
+    ch = b.autotune(reps=2)
+    assert ch["pair_lanes"] in (8, 16, 32) and ch["sf_blocks"] > 0 and ch["bp_blocks"] > 0
+    for _ in range(2):
+        b.compute_device([_ptrs(d, s) for d, s in zip(dev, syss)])
+        outs = b.read_outputs()
+        for i, (o, r) in enumerate(zip(outs, ref)):
+            assert rel_err(o["f_site"], r["f_site"]) < 1e-12, i
+            assert abs(o["e_conf"] - r["e_conf"]) < 1e-12 * abs(r["e_conf"]), i
+            assert o["n_pairs_listed"] == r["n_pairs_listed"], i
+    b.close()
