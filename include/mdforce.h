@@ -230,6 +230,68 @@ int mdf_autotune(mdf_system *sys, int32_t reps, int32_t *chosen);
 int mdf_set_launch_shapes(mdf_system *sys, int32_t pair_lanes, int32_t pair_rolled, int32_t sf_blocks,
                           int32_t bp_blocks);
 
+/* ---- reciprocal-force surrogate with multiple time steps (no counterpart in the reference,
+   whose force_get evaluates the full k-sum every step, src/forcem.f90:827-918) ----------
+   Between full k-sums the k-space site force is predicted by a small per-site network of
+   the surroundings within 6 A plus the correction [full - predicted] from the last full
+   sum, which runs every `interval`-th step and also judges the prediction: when the
+   relative force RMS error it finds exceeds `tol`, the scheme falls back to full sums
+   every step (training again) for `train_frames` steps before it resumes.  The network is
+   trained on the GPU, with Adam, on the full-sum frames of the run itself: the first
+   `train_frames` steps run the full sum and train (`train_iters` Adam steps per frame).
+   model MDF_MTS_HELD uses no network: the last full sum's force is held in between.
+   On steps without the full sum, e_recip (and e_conf, e_avex) is the last full sum's value
+   plus the work of the k-space force along the path moved since (trapezoid rule), and the
+   k-space virial is the last full sum's.  Needs every moving site in a lead's molecule and
+   6 A + twice the largest site-to-lead distance below rc; not available in batches. */
+enum { MDF_MTS_HELD = 0, MDF_MTS_NETWORK = 1 };
+
+typedef struct {
+    int32_t interval;       /* full k-sum every interval-th step (>= 1)                   */
+    int32_t model;          /* MDF_MTS_NETWORK or MDF_MTS_HELD                             */
+    int32_t train_frames;   /* full-sum steps before multiple time steps start, and after a fallback */
+    int32_t train_iters;    /* Adam steps per full-sum frame while collecting (0..16)      */
+    double tol;             /* judged relative force RMS error that triggers the fallback  */
+    float lr;               /* Adam learning rate                                          */
+    uint32_t seed;          /* initial weights                                             */
+} mdf_mts_params;
+
+typedef struct {
+    int32_t state;          /* 0 collecting (full sum every step), 1 multiple time steps   */
+    int32_t last_exact;     /* the last step ran the full sum                              */
+    int64_t n_exact, n_surrogate, n_fallback, n_judged, adam_steps;
+    double err_last, err_max, err_mean;   /* judged relative force RMS errors              */
+    double loss_last;       /* training loss of the last Adam step                         */
+} mdf_mts_stats;
+
+/* Switch the scheme on (p non-NULL: fresh network unless weights are set afterwards) or off
+   (p NULL).  Takes effect from the next step. */
+int mdf_mts_enable(mdf_system *sys, const mdf_mts_params *p);
+int mdf_mts_get_stats(mdf_system *sys, mdf_mts_stats *out);
+
+/* Network state as one float array: parameters (W1 [64][48], b1 [64], W2 [64][64], b2 [64],
+   W3 [16][64], b3 [16]), then input means [48], input deviations [48] and vector scales [16]
+   (MDF_MTS_WEIGHTS floats).  Setting weights restarts the collect phase with them. */
+#define MDF_MTS_WEIGHTS 8448
+int mdf_mts_get_weights(mdf_system *sys, float *w, int32_t cap);
+int mdf_mts_set_weights(mdf_system *sys, const float *w, int32_t n);
+
+/* The surrogate's inputs [n_move][48] and vector features [n_move][16][3] at the positions of
+   the last step (scheme on, network model). */
+int mdf_mts_features(mdf_system *sys, float *feat, float *vec);
+
+/* One evaluation of the network on given inputs, outside any handle: predicted forces
+   f_out [3][n] of frame a (feat [n][48], vec [n][16][3]); with targets of frame a and of a
+   second frame b (feat_b, vec_b, target_b), the gradient of the loss on their change,
+   sum |(S_a - S_b) - (T_a - T_b)|^2 / (n mean |T_a - T_b|^2), in grad_out
+   [MDF_MTS_WEIGHTS - 112 + 1] (the last entry is the loss) and, for lr > 0, the parameters
+   after one Adam step from zero moments (step number adam_t + 1).  weights: as
+   mdf_mts_get_weights.  Outputs may be NULL; target NULL: forward only.  Runs the scheme's
+   own kernels, on the current device. */
+int mdf_mts_eval(const float *weights, int32_t n, const float *feat, const float *vec, const double *target,
+                 const float *feat_b, const float *vec_b, const double *target_b, float lr, int32_t adam_t,
+                 float *f_out, float *grad_out, float *params_out);
+
 /* Timing.  Steps normally replay one captured HIP graph; with stage timing on they use
    direct launches with HIP events between the stages instead (events recorded during
    graph capture cannot be timed).  mdf_stage_times gives names and mean milliseconds per

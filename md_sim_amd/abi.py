@@ -56,6 +56,23 @@ class Outputs(C.Structure):
                 ("n_wall_pairs_listed", C.c_int32), ("n_kvec", C.c_int32), ("n_list_builds", C.c_int32)]
 
 
+# reciprocal-force surrogate with multiple time steps (mdf_mts_params / mdf_mts_stats)
+MTS_MODELS = {"held": 0, "network": 1}
+MTS_WEIGHTS = 8448
+
+
+class MtsParams(C.Structure):
+    _fields_ = [("interval", C.c_int32), ("model", C.c_int32), ("train_frames", C.c_int32),
+                ("train_iters", C.c_int32), ("tol", C.c_double), ("lr", C.c_float), ("seed", C.c_uint32)]
+
+
+class MtsStats(C.Structure):
+    _fields_ = [("state", C.c_int32), ("last_exact", C.c_int32), ("n_exact", C.c_int64),
+                ("n_surrogate", C.c_int64), ("n_fallback", C.c_int64), ("n_judged", C.c_int64),
+                ("adam_steps", C.c_int64), ("err_last", C.c_double), ("err_max", C.c_double),
+                ("err_mean", C.c_double), ("loss_last", C.c_double)]
+
+
 SCALAR_OUTPUTS = ("e_recip", "e_recip_ll", "e_recip_wl", "e_recip_ww", "e_self", "e_slab",
                   "e_wall_lr", "e_conf", "e_avex", "press_wall")
 

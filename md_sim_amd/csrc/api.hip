@@ -41,15 +41,30 @@ template <typename R, int G, bool VIR, bool ROLLED>
 __global__ void k_pair(DevSys S);
 template <typename R, int NS>
 __global__ void k_sf(const double *, const double *, const double *, const double *, int, double, double, double,
-                     const KRow *, int, int, double2 *, int);
+                     const KRow *, int, int, double2 *, int, const MtsState *);
 __global__ void k_sf_reduce(const double2 *, int, int, const double2 *, const double *, const int *, double, double,
-                            double, double, double2 *, double4 *, double *);
+                            double, double, double2 *, double4 *, double *, const MtsState *);
 template <typename R, int NS>
 __global__ void k_bp(const double *, const double *, const double *, const double *, int, double, double, double,
-                     const KRow *, const int *, const double4 *, double, double, double, double *);
+                     const KRow *, const int *, const double4 *, double, double, double, double *, const MtsState *);
+__global__ void k_sg_control(MtsArgs a, int again);
+template <int G>
+__global__ void k_sg_features(DevSys S, MtsArgs a);
+__global__ void k_sg_norm(MtsArgs a, int n);
+__global__ void k_sg_infer(MtsArgs a, int n, int after_training, const DevFlags *flags);
+__global__ void k_sg_train(MtsArgs a, int n, const DevFlags *flags);
+__global__ void k_sg_adam(MtsArgs a, int nb, const DevFlags *flags);
+__global__ void k_sg_combine(DevSys S, MtsArgs a);
+__global__ void k_sg_finish(DevSys S, MtsArgs a, Scalars *scal);
+__global__ void k_sg_exact(DevSys S, MtsArgs a, const double *fpart, int n_groups);
+__global__ void k_sg_judge(DevSys S, MtsArgs a);
+__global__ void k_sg_delta(DevSys S, MtsArgs a);
+__global__ void k_sg_post(DevSys S, MtsArgs a);
+constexpr int SG_FEAT_G = 8;   // lanes per lead in the input kernel
 constexpr int BP_NS = 4;   // sites per back-projection thread
 __global__ void k_epilogue(DevSys S, const double *fpart, int n_groups, const WallLR *wl, int ew3dc, double crf0,
-                           const double *dipz, double *epart, double *fcom, double *tq);
+                           const double *dipz, double *epart, double *fcom, double *tq, const MtsState *mts,
+                           const double *fmts);
 __global__ void k_colsum(DevSys S, const int *lead_slot, int per_block, double *colpart);
 __global__ void k_pair_totals(DevSys S);
 __global__ void k_final(DevSys S, FinalArgs A, const double *colpart, const double *kpart, const double *eppart,
@@ -161,8 +176,9 @@ struct DBuf {
     }
 };
 
-static const char *STAGE_NAMES[] = {"list", "pair", "sf", "sf_reduce", "backproj", "epilogue", "reduce", "step"};
-constexpr int N_STAGES = 7;  // timed stages; entry N_STAGES of the names is the whole step
+static const char *STAGE_NAMES[] = {"list", "pair", "surrogate", "sf", "sf_reduce", "backproj", "epilogue", "reduce",
+                                    "step"};
+constexpr int N_STAGES = 8;  // timed stages; entry N_STAGES of the names is the whole step
 
 struct mdf_system {
     mdf_topology top;
@@ -231,7 +247,31 @@ struct mdf_system {
     bool pending = false;
     int last_rebuilt = 0;
     DevFlags last_flags;
+    // reciprocal-force surrogate and multiple time steps (off unless mdf_mts_enable)
+    bool mts_on = false;
+    int mts_model = 1, mts_interval = 4, mts_train_frames = 0, mts_train_iters = 1;
+    double mts_tol = 1e-3;
+    float mts_lr = 1e-3f;
+    int mts_again = 0;                   // the step being launched redoes an overflowed one
+    int sg_n_sblk = 1, sg_n_tblk = 1, sg_n_iblk = 1;
+    DBuf<MtsState> mts_st;
+    DBuf<float> sg_feat, sg_vec, sg_out, sg_params, sg_m, sg_v, sg_gsum, sg_norm, sg_gpart, sg_feat_prev,
+        sg_vec_prev;
+    DBuf<unsigned short> sg_wb;
+    DBuf<double> sg_delta, sg_frec, sg_fmts, sg_rprev, sg_fprev, sg_jpart, sg_epart, sg_frec_prev, sg_delta_prev;
 };
+
+static void mts_free(mdf_system *s) {
+    s->mts_st.release();
+    DBuf<float> *fb[] = {&s->sg_feat, &s->sg_vec, &s->sg_out, &s->sg_params, &s->sg_m, &s->sg_v, &s->sg_gsum,
+                         &s->sg_norm, &s->sg_gpart, &s->sg_feat_prev, &s->sg_vec_prev};
+    for (auto *b : fb) b->release();
+    s->sg_wb.release();
+    DBuf<double> *db[] = {&s->sg_delta, &s->sg_frec, &s->sg_fmts, &s->sg_rprev, &s->sg_fprev, &s->sg_jpart,
+                          &s->sg_epart, &s->sg_frec_prev, &s->sg_delta_prev};
+    for (auto *b : db) b->release();
+    s->mts_on = false;
+}
 
 static void free_all(mdf_system *s) {
     DBuf<int> *ib[] = {&s->itype, &s->nsi, &s->ijtype, &s->vkind, &s->cls, &s->lead, &s->lead_slot, &s->con_ptr,
@@ -246,6 +286,7 @@ static void free_all(mdf_system *s) {
     for (auto *b : db) b->release();
     s->flags.release(); s->d_bp_row_start.release(); s->d_chunks.release(); s->d_rows.release(); s->d_swall.release(); s->d_s.release();
     s->d_t.release(); s->sf_partial.release(); s->d_wl.release(); s->d_scal.release();
+    mts_free(s);
 }
 
 template <typename T>
@@ -476,6 +517,32 @@ static void launch_pair(mdf_system *s) {
     }
 }
 
+// the scheme's device state when multiple time steps are on (the k-sum kernels skip the
+// steps it serves without the full sum), else null
+static const MtsState *mts_state(const mdf_system *s) { return s->mts_on ? s->mts_st.p : nullptr; }
+
+static MtsArgs mts_args(mdf_system *s) {
+    MtsArgs a;
+    memset(&a, 0, sizeof a);
+    a.st = s->mts_st.p;
+    a.interval = s->mts_interval;
+    a.train_frames = s->mts_train_frames;
+    a.model = s->mts_model;
+    a.n_sblk = s->sg_n_sblk;
+    a.n_tblk = s->sg_n_tblk;
+    a.tol = s->mts_tol;
+    a.lr = s->mts_lr;
+    a.feat = s->sg_feat.p; a.vec = s->sg_vec.p; a.sg = s->sg_out.p;
+    a.feat_prev = s->sg_feat_prev.p; a.vec_prev = s->sg_vec_prev.p; a.frec_prev = s->sg_frec_prev.p;
+    a.delta_prev = s->sg_delta_prev.p;
+    a.delta = s->sg_delta.p; a.frec = s->sg_frec.p; a.fmts = s->sg_fmts.p;
+    a.rprev = s->sg_rprev.p; a.fprev = s->sg_fprev.p;
+    a.params = s->sg_params.p; a.adam_m = s->sg_m.p; a.adam_v = s->sg_v.p; a.gsum = s->sg_gsum.p;
+    a.wb = s->sg_wb.p; a.norm = s->sg_norm.p; a.gpart = s->sg_gpart.p;
+    a.jpart = s->sg_jpart.p; a.epart = s->sg_epart.p;
+    return a;
+}
+
 static void launch_sf(mdf_system *s) {
     const int nk = (int)s->ak.size(), nrows = (int)s->rows.size(), nm = s->top.n_move;
     if (nk == 0 || nm == 0) return;
@@ -484,10 +551,10 @@ static void launch_sf(mdf_system *s) {
     dim3 g1(s->n_sb, s->n_cg);
     if (s->par.precision == MDF_FP32_ACC64)
         k_sf<float, 4><<<g1, SF_BLOCK, 0, s->stream>>>(S.x, S.y, S.z, S.q, nm, a.cux, a.cuy, a.cuz, s->d_rows.p,
-                                                       nrows, s->cpb, s->sf_partial.p, nk);
+                                                       nrows, s->cpb, s->sf_partial.p, nk, mts_state(s));
     else
         k_sf<double, 4><<<g1, SF_BLOCK, 0, s->stream>>>(S.x, S.y, S.z, S.q, nm, a.cux, a.cuy, a.cuz, s->d_rows.p,
-                                                        nrows, s->cpb, s->sf_partial.p, nk);
+                                                        nrows, s->cpb, s->sf_partial.p, nk, mts_state(s));
 }
 
 static void launch_bp(mdf_system *s) {
@@ -499,11 +566,11 @@ static void launch_bp(mdf_system *s) {
     if (s->par.precision == MDF_FP32_ACC64)
         k_bp<float, BP_NS><<<g2, 256, 0, s->stream>>>(S.x, S.y, S.z, S.q, nm, a.cux, a.cuy, a.cuz, s->d_rows.p,
                                                       s->d_bp_row_start.p, s->d_t.p, a.pipix, a.pipiy, a.pipiz,
-                                                      s->fpart.p);
+                                                      s->fpart.p, mts_state(s));
     else
         k_bp<double, BP_NS><<<g2, 256, 0, s->stream>>>(S.x, S.y, S.z, S.q, nm, a.cux, a.cuy, a.cuz, s->d_rows.p,
                                                        s->d_bp_row_start.p, s->d_t.p, a.pipix, a.pipiy, a.pipiz,
-                                                       s->fpart.p);
+                                                       s->fpart.p, mts_state(s));
 }
 
 // the scalars the final reduction needs
@@ -532,25 +599,39 @@ static void launch_forces(mdf_system *s) {
     hipStream_t st = s->stream;
     const int nm = s->top.n_move;
     const RecipArgs ra = recip_args(s);
+    const bool mts = s->mts_on;
+    const MtsArgs ma = mts ? mts_args(s) : MtsArgs{};
     if (s->ev) (void)hipEventRecord(s->ev[1], st);
+    if (mts) k_sg_control<<<1, 64, 0, st>>>(ma, s->mts_again);
     launch_pair(s);
     if (s->ev) (void)hipEventRecord(s->ev[2], st);
+    if (mts) {
+        if (s->mts_model) {
+            k_sg_features<SG_FEAT_G><<<std::max(1, (s->n_lead + 256 / SG_FEAT_G - 1) / (256 / SG_FEAT_G)), 256, 0,
+                                       st>>>(S, ma);
+            k_sg_norm<<<1, 1024, 0, st>>>(ma, nm);
+            k_sg_infer<<<s->sg_n_iblk, 256, 0, st>>>(ma, nm, 0, s->flags.p);
+        }
+        k_sg_combine<<<s->sg_n_sblk, 256, 0, st>>>(S, ma);
+    }
+    if (s->ev) (void)hipEventRecord(s->ev[3], st);
     int nk = (int)s->ak.size();
     launch_sf(s);
-    if (s->ev) (void)hipEventRecord(s->ev[3], st);
+    if (s->ev) (void)hipEventRecord(s->ev[4], st);
     if (nk > 0) {
         double lamb = 1.0 / (4.0 * S.alfa * S.alfa);
         k_sf_reduce<<<s->n_kblk, 256, 0, st>>>(s->sf_partial.p, nm > 0 ? s->n_sb : 0, nk, s->d_swall.p, s->d_ak.p,
                                                s->d_kvec.p, ra.pipix, ra.pipiy, ra.pipiz, lamb, s->d_s.p, s->d_t.p,
-                                               s->kpart.p);
+                                               s->kpart.p, mts_state(s));
     }
-    if (s->ev) (void)hipEventRecord(s->ev[4], st);
-    launch_bp(s);
     if (s->ev) (void)hipEventRecord(s->ev[5], st);
-    k_epilogue<<<s->n_epblk, EPI_BLOCK, 0, st>>>(S, s->fpart.p, (nk > 0 && nm > 0) ? s->n_rg : 0, s->d_wl.p,
-                                                 s->par.ew3dc, s->crf0, s->dipz.p, s->eppart.p, s->fcom.p,
-                                                 s->torque.p);
+    launch_bp(s);
     if (s->ev) (void)hipEventRecord(s->ev[6], st);
+    const int n_groups = (nk > 0 && nm > 0) ? s->n_rg : 0;
+    k_epilogue<<<s->n_epblk, EPI_BLOCK, 0, st>>>(S, s->fpart.p, n_groups, s->d_wl.p, s->par.ew3dc, s->crf0,
+                                                 s->dipz.p, s->eppart.p, s->fcom.p, s->torque.p, mts_state(s),
+                                                 s->sg_fmts.p);
+    if (s->ev) (void)hipEventRecord(s->ev[7], st);
     {
         dim3 gc(NCOL, s->n_colblk);
         const FinalArgs A = final_args(s);
@@ -558,7 +639,22 @@ static void launch_forces(mdf_system *s) {
         k_final<<<1, 256, 0, st>>>(S, A, s->colpart.p, s->kpart.p, s->eppart.p, s->dipz.p, s->e_vdw.p, s->e_coul.p,
                                    s->d_scal.p);
     }
-    if (s->ev) (void)hipEventRecord(s->ev[7], st);
+    if (mts) {
+        // energies of a surrogate step; on a full-sum step: target, judge, training, correction
+        k_sg_finish<<<1, 64, 0, st>>>(S, ma, s->d_scal.p);
+        k_sg_exact<<<s->sg_n_sblk, 256, 0, st>>>(S, ma, s->fpart.p, n_groups);
+        k_sg_judge<<<1, 256, 0, st>>>(S, ma);
+        if (s->mts_model) {
+            for (int it = 0; it < s->mts_train_iters; it++) {
+                k_sg_train<<<s->sg_n_tblk, 256, 0, st>>>(ma, nm, s->flags.p);
+                k_sg_adam<<<(SG_GP + 255) / 256, 256, 0, st>>>(ma, s->sg_n_tblk, s->flags.p);
+            }
+            k_sg_infer<<<s->sg_n_iblk, 256, 0, st>>>(ma, nm, 1, s->flags.p);
+        }
+        k_sg_delta<<<s->sg_n_sblk, 256, 0, st>>>(S, ma);
+        k_sg_post<<<1, 64, 0, st>>>(S, ma);
+    }
+    if (s->ev) (void)hipEventRecord(s->ev[8], st);
 }
 
 static void drain_slot(mdf_system *s, int slot) {
@@ -603,6 +699,7 @@ static int run_step(mdf_system *s, int force_rebuild) {
     s->ring_used[slot] = true;
     s->step_no++;
     hipEvent_t *ev = s->ev;
+    s->mts_again = force_rebuild;   // a forced rebuild only ever redoes an overflowed step
     if (s->use_graphs && !force_rebuild && !s->stage_timing) {
         // the graph holds no events (HIP cannot time events recorded during capture); the
         // step is bracketed by events recorded outside it
@@ -760,6 +857,7 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
         for (int r = 0; r < mdf_system::NRING; r++)
             for (int k = 0; k <= N_STAGES; k++) HTRY(hipEventCreate(&s->ring[r][k]));
         // leads, lead-type slots, energy classes
+        s->h_itype.assign(t.site_type, t.site_type + t.n_total);
         std::vector<int> lead_types;
         for (int i = 0; i < t.n_move; i++) {
             int ti = t.site_type[i];
@@ -1026,10 +1124,10 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
                 ncg = (nrow + rpb - 1) / rpb;
                 k_sf<double, 4><<<dim3(n_sb_w, ncg), SF_BLOCK, 0, s->stream>>>(
                     S.x + t.n_move, S.y + t.n_move, S.z + t.n_move, S.q + t.n_move, nw, cux, cuy, cuz,
-                    s->d_rows.p, nrow, rpb, s->sf_partial.p, nk);
+                    s->d_rows.p, nrow, rpb, s->sf_partial.p, nk, nullptr);
                 k_sf_reduce<<<s->n_kblk, 256, 0, s->stream>>>(s->sf_partial.p, n_sb_w, nk, s->d_swall.p, s->d_ak.p,
                                                               s->d_kvec.p, 1.0, 1.0, 1.0, 0.0, s->d_s.p, s->d_t.p,
-                                                              s->kpart.p);
+                                                              s->kpart.p, nullptr);
                 HTRY(hipStreamSynchronize(s->stream));
                 HTRY(hipMemcpy(s->s_wall.data(), s->d_s.p, sizeof(double2) * nk, hipMemcpyDeviceToHost));
                 HTRY(hipMemcpy(s->d_swall.p, s->d_s.p, sizeof(double2) * nk, hipMemcpyDeviceToDevice));
@@ -1365,6 +1463,321 @@ void mdf_destroy(mdf_system *s) {
 
 }  // extern "C"
 
+// ---- reciprocal-force surrogate and multiple time steps: host side ----------------------
+static unsigned short host_f2bf(float f) {   // round to nearest even, as the kernels
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
+// bfloat16 copies (and the transposes the backward pass uses) of the weights
+static void host_weight_copies(const float *p, unsigned short *wb) {
+    for (int k = 0; k < SG_H * SG_D; k++) wb[SG_WB_W1 + k] = host_f2bf(p[SG_OFF_W1 + k]);
+    for (int u2 = 0; u2 < SG_H; u2++)
+        for (int u1 = 0; u1 < SG_H; u1++) {
+            const unsigned short v = host_f2bf(p[SG_OFF_W2 + u2 * SG_H + u1]);
+            wb[SG_WB_W2 + u2 * SG_H + u1] = v;
+            wb[SG_WB_W2T + u1 * SG_H + u2] = v;
+        }
+    for (int m = 0; m < SG_M; m++)
+        for (int u = 0; u < SG_H; u++) {
+            const unsigned short v = host_f2bf(p[SG_OFF_W3 + m * SG_H + u]);
+            wb[SG_WB_W3 + m * SG_H + u] = v;
+            wb[SG_WB_W3T + u * SG_M + m] = v;
+        }
+}
+
+// initial weights: W1, W2 normal with variance 1/fan-in (splitmix64 + Box-Muller), W3 and
+// the biases zero; the normalisation kernel then sets c_0 = su_0 (the physics prior)
+static void host_init_params(std::vector<float> &p, uint32_t seed) {
+    p.assign(SG_NP, 0.0f);
+    uint64_t st = 0x9E3779B97F4A7C15ull ^ seed;
+    auto next = [&]() {
+        uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return ((z >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    };
+    auto normal = [&]() { return sqrt(-2.0 * log(next())) * cos(2.0 * M_PI * next()); };
+    for (int k = 0; k < SG_H * SG_D; k++) p[SG_OFF_W1 + k] = (float)(normal() / sqrt((double)SG_D));
+    for (int k = 0; k < SG_H * SG_H; k++) p[SG_OFF_W2 + k] = (float)(normal() / sqrt((double)SG_H));
+}
+
+static void mts_geometry(int n, int &n_sblk, int &n_iblk, int &n_tblk) {
+    const int ntiles = (n + 15) / 16;
+    n_sblk = std::max(1, (n + 255) / 256);
+    n_iblk = std::min(256, std::max(1, (ntiles + 3) / 4));
+    n_tblk = std::min(SG_TRAIN_BLOCKS, std::max(1, (ntiles + 3) / 4));
+}
+
+static int mts_alloc(mdf_system *s) {
+    const size_t nm = (size_t)std::max(s->top.n_move, 1);
+    mts_geometry(s->top.n_move, s->sg_n_sblk, s->sg_n_iblk, s->sg_n_tblk);
+    HIPCHK(s->mts_st.alloc(1));
+    HIPCHK(s->sg_feat.alloc(nm * SG_D));
+    HIPCHK(s->sg_vec.alloc(nm * SG_M * 3));
+    HIPCHK(s->sg_feat_prev.alloc(nm * SG_D));
+    HIPCHK(s->sg_vec_prev.alloc(nm * SG_M * 3));
+    HIPCHK(s->sg_out.alloc(3 * nm));
+    HIPCHK(s->sg_params.alloc(SG_NP));
+    HIPCHK(s->sg_m.alloc(SG_NP));
+    HIPCHK(s->sg_v.alloc(SG_NP));
+    HIPCHK(s->sg_gsum.alloc(SG_GP));
+    HIPCHK(s->sg_norm.alloc(SG_NORM_N));
+    HIPCHK(s->sg_gpart.alloc((size_t)SG_TRAIN_BLOCKS * SG_GP));
+    HIPCHK(s->sg_wb.alloc(SG_WB_N));
+    DBuf<double> *db[] = {&s->sg_delta, &s->sg_frec, &s->sg_fmts, &s->sg_rprev, &s->sg_fprev, &s->sg_frec_prev,
+                          &s->sg_delta_prev};
+    for (auto *b : db) {
+        HIPCHK(b->alloc(3 * nm));
+        HIPCHK(hipMemset(b->p, 0, sizeof(double) * 3 * nm));
+    }
+    HIPCHK(hipMemset(s->sg_out.p, 0, sizeof(float) * 3 * nm));
+    HIPCHK(s->sg_jpart.alloc(4 * (size_t)s->sg_n_sblk));
+    HIPCHK(s->sg_epart.alloc((size_t)s->sg_n_sblk));
+    return MDF_OK;
+}
+
+// parameters, moments, copies and normalisation to the device; the state restarts collecting
+static int mts_load(mdf_system *s, const float *params, const float *norm, bool norm_ready) {
+    std::vector<unsigned short> wb(SG_WB_N);
+    host_weight_copies(params, wb.data());
+    HIPCHK(hipMemcpy(s->sg_params.p, params, sizeof(float) * SG_NP, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(s->sg_wb.p, wb.data(), sizeof(unsigned short) * SG_WB_N, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(s->sg_norm.p, norm, sizeof(float) * SG_NORM_N, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(s->sg_m.p, 0, sizeof(float) * SG_NP));
+    HIPCHK(hipMemset(s->sg_v.p, 0, sizeof(float) * SG_NP));
+    MtsState st;
+    memset(&st, 0, sizeof st);
+    st.exact = 1;
+    st.norm_ready = norm_ready ? 1 : 0;
+    st.prior = norm_ready ? 0 : 1;
+    HIPCHK(hipMemcpy(s->mts_st.p, &st, sizeof st, hipMemcpyHostToDevice));
+    return MDF_OK;
+}
+
+extern "C" {
+
+int mdf_mts_enable(mdf_system *s, const mdf_mts_params *p) {
+    if (!s) return fail(MDF_ERR_ARG, "null handle");
+    HIPCHK(hipSetDevice(s->device));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    drain_all(s);
+    drop_graphs(s);
+    if (!p) {
+        mts_free(s);
+        return MDF_OK;
+    }
+    if (p->interval < 1) return fail(MDF_ERR_ARG, "interval %d < 1", p->interval);
+    if (p->model != MDF_MTS_HELD && p->model != MDF_MTS_NETWORK) return fail(MDF_ERR_ARG, "unknown model %d", p->model);
+    if (p->train_frames < 0 || p->train_iters < 0 || p->train_iters > 16)
+        return fail(MDF_ERR_ARG, "train_frames %d / train_iters %d out of range", p->train_frames, p->train_iters);
+    if (!(p->tol > 0.0) || !(p->lr >= 0.0f)) return fail(MDF_ERR_ARG, "tol must be > 0 and lr >= 0");
+    const int nm = s->top.n_move;
+    if (nm <= 0 || s->ak.empty()) return fail(MDF_ERR_ARG, "multiple time steps need moving sites and a k-set");
+    if (p->model == MDF_MTS_NETWORK) {
+        // every moving site in a lead's molecule, and the lists reach every site within 6 A
+        std::vector<int> nsi(s->top.n_types + 1);
+        HIPCHK(hipMemcpy(nsi.data(), s->nsi.p, sizeof(int) * nsi.size(), hipMemcpyDeviceToHost));
+        std::vector<double> hx(nm), hy(nm), hz(nm);
+        HIPCHK(hipMemcpy(hx.data(), s->S.x, sizeof(double) * nm, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(hy.data(), s->S.y, sizeof(double) * nm, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(hz.data(), s->S.z, sizeof(double) * nm, hipMemcpyDeviceToHost));
+        std::vector<char> cover(nm, 0);
+        double off = 0.0;
+        for (int l = 0; l < s->n_lead; l++) {
+            const int i = s->h_lead[l];
+            const int ns = nsi[s->h_itype[i]];
+            if (i + ns > nm) return fail(MDF_ERR_ARG, "surrogate: the molecule of lead site %d is not all moving", i);
+            for (int a = 0; a < ns; a++) {
+                cover[i + a] = 1;
+                const double dx = hx[i + a] - hx[i], dy = hy[i + a] - hy[i], dz = hz[i + a] - hz[i];
+                off = std::max(off, sqrt(dx * dx + dy * dy + dz * dz));
+            }
+        }
+        for (int i = 0; i < nm; i++)
+            if (!cover[i]) return fail(MDF_ERR_ARG, "surrogate: moving site %d is in no molecule with a lead site", i);
+        if (SG_RC + 2.0 * off >= sqrt(s->S.rcutsq))
+            return fail(MDF_ERR_ARG, "surrogate: 6 A + 2 x %.3f A (site-to-lead) must be below rc %.3f", off,
+                        sqrt(s->S.rcutsq));
+        const double lmin = std::min(s->par.box[0], s->par.box[1]);
+        if (2.0 * SG_RC >= lmin || (s->S.zper_force && 2.0 * SG_RC >= s->par.box[2]))
+            return fail(MDF_ERR_ARG, "surrogate: box shorter than 12 A");
+    }
+    mts_free(s);
+    int rc = mts_alloc(s);
+    if (rc) { mts_free(s); return rc; }
+    std::vector<float> params;
+    host_init_params(params, p->seed);
+    std::vector<float> norm(SG_NORM_N, 1.0f);
+    for (int k = 0; k < SG_D; k++) norm[k] = 0.0f;
+    rc = mts_load(s, params.data(), norm.data(), false);
+    if (rc) { mts_free(s); return rc; }
+    s->mts_interval = p->interval;
+    s->mts_model = p->model;
+    s->mts_train_frames = p->train_frames;
+    s->mts_train_iters = p->model == MDF_MTS_NETWORK ? p->train_iters : 0;
+    s->mts_tol = p->tol;
+    s->mts_lr = p->lr;
+    s->mts_on = true;
+    return MDF_OK;
+}
+
+int mdf_mts_get_stats(mdf_system *s, mdf_mts_stats *out) {
+    if (!s || !out) return fail(MDF_ERR_ARG, "null argument");
+    if (!s->mts_on) return fail(MDF_ERR_ARG, "multiple time steps are off");
+    HIPCHK(hipSetDevice(s->device));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    MtsState st;
+    HIPCHK(hipMemcpy(&st, s->mts_st.p, sizeof st, hipMemcpyDeviceToHost));
+    out->state = st.state;
+    out->last_exact = st.exact;
+    out->n_exact = st.n_exact;
+    out->n_surrogate = st.n_surrogate;
+    out->n_fallback = st.n_fallback;
+    out->n_judged = st.n_judged;
+    out->adam_steps = st.adam_t;
+    out->err_last = st.err_last;
+    out->err_max = st.err_max;
+    out->err_mean = st.n_judged ? st.err_sum / st.n_judged : 0.0;
+    out->loss_last = st.loss_last;
+    return MDF_OK;
+}
+
+int mdf_mts_get_weights(mdf_system *s, float *w, int32_t cap) {
+    if (!s || !w) return fail(MDF_ERR_ARG, "null argument");
+    if (!s->mts_on) return fail(MDF_ERR_ARG, "multiple time steps are off");
+    if (cap < MDF_MTS_WEIGHTS) return fail(MDF_ERR_ARG, "capacity %d < %d", cap, MDF_MTS_WEIGHTS);
+    HIPCHK(hipSetDevice(s->device));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    HIPCHK(hipMemcpy(w, s->sg_params.p, sizeof(float) * SG_NP, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(w + SG_NP, s->sg_norm.p, sizeof(float) * SG_NORM_N, hipMemcpyDeviceToHost));
+    return MDF_OK;
+}
+
+static int check_weights(const float *w) {
+    for (int k = 0; k < MDF_MTS_WEIGHTS; k++)
+        if (!std::isfinite(w[k])) return fail(MDF_ERR_ARG, "weights: entry %d is not finite", k);
+    for (int k = SG_D; k < SG_NORM_N; k++)
+        if (!(w[SG_NP + k] > 0.0f)) return fail(MDF_ERR_ARG, "weights: scale %d must be positive", k);
+    return MDF_OK;
+}
+
+int mdf_mts_set_weights(mdf_system *s, const float *w, int32_t n) {
+    if (!s || !w) return fail(MDF_ERR_ARG, "null argument");
+    if (!s->mts_on) return fail(MDF_ERR_ARG, "multiple time steps are off");
+    if (n != MDF_MTS_WEIGHTS) return fail(MDF_ERR_ARG, "expected %d floats, got %d", MDF_MTS_WEIGHTS, n);
+    int rc = check_weights(w);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(s->device));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    drain_all(s);
+    return mts_load(s, w, w + SG_NP, true);
+}
+
+int mdf_mts_features(mdf_system *s, float *feat, float *vec) {
+    if (!s || !feat || !vec) return fail(MDF_ERR_ARG, "null argument");
+    if (!s->mts_on || s->mts_model != MDF_MTS_NETWORK)
+        return fail(MDF_ERR_ARG, "the surrogate's inputs need the scheme on with the network model");
+    HIPCHK(hipSetDevice(s->device));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    const MtsArgs ma = mts_args(s);
+    k_sg_features<SG_FEAT_G><<<std::max(1, (s->n_lead + 256 / SG_FEAT_G - 1) / (256 / SG_FEAT_G)), 256, 0,
+                               s->stream>>>(s->S, ma);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->stream));
+    const size_t nm = s->top.n_move;
+    HIPCHK(hipMemcpy(feat, s->sg_feat.p, sizeof(float) * nm * SG_D, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(vec, s->sg_vec.p, sizeof(float) * nm * SG_M * 3, hipMemcpyDeviceToHost));
+    return MDF_OK;
+}
+
+int mdf_mts_eval(const float *weights, int32_t n, const float *feat, const float *vec, const double *target,
+                 const float *feat_b, const float *vec_b, const double *target_b, float lr, int32_t adam_t,
+                 float *f_out, float *grad_out, float *params_out) {
+    if (!weights || !feat || !vec || n <= 0) return fail(MDF_ERR_ARG, "null argument or n <= 0");
+    if (adam_t < 0 || !(lr >= 0.0f)) return fail(MDF_ERR_ARG, "adam_t >= 0 and lr >= 0 required");
+    const bool train = target != nullptr;
+    if (train && (!feat_b || !vec_b || !target_b)) return fail(MDF_ERR_ARG, "training needs the second frame");
+    int rc = check_weights(weights);
+    if (rc) return rc;
+    double t2 = 0.0;
+    if (train) {
+        for (size_t k = 0; k < 3 * (size_t)n; k++) t2 += (target[k] - target_b[k]) * (target[k] - target_b[k]);
+        t2 /= n;
+        if (!(t2 > 0.0)) return fail(MDF_ERR_ARG, "the two targets are equal");
+    }
+    int n_sblk, n_iblk, n_tblk;
+    mts_geometry(n, n_sblk, n_iblk, n_tblk);
+    DBuf<MtsState> st;
+    DBuf<float> d_feat, d_vec, d_feat_b, d_vec_b, d_sg, d_params, d_m, d_v, d_gsum, d_norm, d_gpart;
+    DBuf<unsigned short> d_wb;
+    DBuf<double> d_frec, d_frec_b;
+    hipStream_t stream = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
+    std::vector<unsigned short> wb(SG_WB_N);
+    host_weight_copies(weights, wb.data());
+    MtsState hs;
+    memset(&hs, 0, sizeof hs);
+    hs.norm_ready = 1;
+    hs.have_prev = train ? 1 : 0;
+    hs.train_now = train ? 1 : 0;
+    hs.t2 = t2;
+    hs.adam_t = adam_t;
+    const size_t nn = (size_t)n;
+#define MTRY(x) do { if (e == hipSuccess) e = (x); } while (0)
+    MTRY(st.alloc(1));
+    MTRY(hipMemcpy(st.p, &hs, sizeof hs, hipMemcpyHostToDevice));
+    MTRY(upload(d_feat, feat, nn * SG_D));
+    MTRY(upload(d_vec, vec, nn * SG_M * 3));
+    MTRY(d_sg.alloc(3 * nn));
+    MTRY(upload(d_params, weights, SG_NP));
+    MTRY(upload(d_norm, weights + SG_NP, SG_NORM_N));
+    MTRY(upload(d_wb, wb.data(), SG_WB_N));
+    MTRY(d_m.alloc(SG_NP));
+    MTRY(d_v.alloc(SG_NP));
+    MTRY(hipMemset(d_m.p, 0, sizeof(float) * SG_NP));
+    MTRY(hipMemset(d_v.p, 0, sizeof(float) * SG_NP));
+    MTRY(d_gsum.alloc(SG_GP));
+    MTRY(d_gpart.alloc((size_t)SG_TRAIN_BLOCKS * SG_GP));
+    if (train) {
+        MTRY(upload(d_frec, target, 3 * nn));
+        MTRY(upload(d_frec_b, target_b, 3 * nn));
+        MTRY(upload(d_feat_b, feat_b, nn * SG_D));
+        MTRY(upload(d_vec_b, vec_b, nn * SG_M * 3));
+    }
+    if (e == hipSuccess) {
+        MtsArgs a;
+        memset(&a, 0, sizeof a);
+        a.st = st.p; a.model = 1; a.n_sblk = n_sblk; a.n_tblk = n_tblk; a.lr = lr;
+        a.feat = d_feat.p; a.vec = d_vec.p; a.sg = d_sg.p; a.frec = d_frec.p;
+        a.feat_prev = d_feat_b.p; a.vec_prev = d_vec_b.p; a.frec_prev = d_frec_b.p;
+        a.params = d_params.p; a.adam_m = d_m.p; a.adam_v = d_v.p; a.gsum = d_gsum.p;
+        a.wb = d_wb.p; a.norm = d_norm.p; a.gpart = d_gpart.p;
+        k_sg_infer<<<n_iblk, 256, 0, stream>>>(a, n, 0, nullptr);
+        if (train) {
+            k_sg_train<<<n_tblk, 256, 0, stream>>>(a, n, nullptr);
+            k_sg_adam<<<(SG_GP + 255) / 256, 256, 0, stream>>>(a, n_tblk, nullptr);
+        }
+        MTRY(hipGetLastError());
+        MTRY(hipStreamSynchronize(stream));
+        if (f_out) MTRY(hipMemcpy(f_out, d_sg.p, sizeof(float) * 3 * nn, hipMemcpyDeviceToHost));
+        if (train && grad_out) MTRY(hipMemcpy(grad_out, d_gsum.p, sizeof(float) * SG_GP, hipMemcpyDeviceToHost));
+        if (train && params_out) MTRY(hipMemcpy(params_out, d_params.p, sizeof(float) * SG_NP, hipMemcpyDeviceToHost));
+    }
+#undef MTRY
+    st.release(); d_feat.release(); d_vec.release(); d_feat_b.release(); d_vec_b.release(); d_sg.release();
+    d_params.release(); d_m.release(); d_v.release(); d_gsum.release(); d_norm.release(); d_gpart.release();
+    d_wb.release(); d_frec.release(); d_frec_b.release();
+    if (stream) (void)hipStreamDestroy(stream);
+    if (e != hipSuccess) return fail(MDF_ERR_HIP, "surrogate evaluation: %s", hipGetErrorString(e));
+    return MDF_OK;
+}
+
+}  // extern "C"
+
 // ---- batches --------------------------------------------------------------------------
 // Many independent systems stepped together: every stage is one launch over all of them
 // (block maps give each block its system), so small systems fill the GPU together and a
@@ -1559,6 +1972,8 @@ int mdf_batch_create(mdf_system *const *systems, int32_t n, mdf_batch **out) {
         for (int j = 0; j < i; j++)
             if (systems[j] == s) return fail(MDF_ERR_ARG, "system %d appears twice", i);
     }
+    for (int i = 0; i < n; i++)
+        if (systems[i]->mts_on) return fail(MDF_ERR_ARG, "system %d has multiple time steps on (not in batches)", i);
     mdf_batch *b = new mdf_batch;
     b->sys.assign(systems, systems + n);
     b->device = s0->device;

@@ -47,11 +47,18 @@ __device__ void angle_force(const DevSys &S, int a, int role, double &fx, double
 // One block covers EPI_BLOCK = 3 x 64 consecutive sites, i.e. 64 whole rigid molecules
 // (rigid sites come first, three per molecule), so the block also forms those molecules'
 // COM forces and torques (src/forcem.f90:1003-1024) from the forces it has just finished.
-__device__ __forceinline__ void epilogue_body(const DevSys &S, const double *__restrict__ fpart, int n_groups,
+// With multiple time steps on (mts non-null), a step without the full k-sum takes the
+// reciprocal forces from fmts [3][n] (surrogate plus correction) instead of the row groups.
+__device__ __forceinline__ void epilogue_body(const DevSys &S, const double *__restrict__ fpart_k, int n_groups_k,
                                                         const WallLR *__restrict__ wl, int ew3dc, double crf0,
                                                         const double *__restrict__ dipz, double *__restrict__ epart,
-                                                        double *__restrict__ fcom, double *__restrict__ tq, int bid, int nblk) {
+                                                        double *__restrict__ fcom, double *__restrict__ tq, int bid, int nblk,
+                                                        const MtsState *__restrict__ mts = nullptr,
+                                                        const double *__restrict__ fmts = nullptr) {
     __shared__ double sf[3][EPI_BLOCK];
+    const bool surrogate = mts != nullptr && !mts->exact;
+    const double *__restrict__ fpart = surrogate ? fmts : fpart_k;
+    const int n_groups = surrogate ? 1 : n_groups_k;
     __shared__ double wsum[EPI_BLOCK / 64][EP_N];
     double acc[EP_N] = {0, 0, 0, 0, 0, 0};
     const int i = bid * blockDim.x + threadIdx.x;
@@ -140,8 +147,9 @@ __device__ __forceinline__ void epilogue_body(const DevSys &S, const double *__r
 __global__ __launch_bounds__(EPI_BLOCK) void k_epilogue(DevSys S, const double *__restrict__ fpart, int n_groups,
                                                         const WallLR *__restrict__ wl, int ew3dc, double crf0,
                                                         const double *__restrict__ dipz, double *__restrict__ epart,
-                                                        double *__restrict__ fcom, double *__restrict__ tq) {
-    epilogue_body(S, fpart, n_groups, wl, ew3dc, crf0, dipz, epart, fcom, tq, blockIdx.x, gridDim.x);
+                                                        double *__restrict__ fcom, double *__restrict__ tq,
+                                                        const MtsState *__restrict__ mts, const double *__restrict__ fmts) {
+    epilogue_body(S, fpart, n_groups, wl, ew3dc, crf0, dipz, epart, fcom, tq, blockIdx.x, gridDim.x, mts, fmts);
 }
 
 // column sums of the per-lead partials, split by the lead's type slot

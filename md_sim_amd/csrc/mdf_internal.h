@@ -153,6 +153,61 @@ __device__ __forceinline__ double list_d2(const DevSys &S, int i, int j) {
     return __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
 }
 
+// ---- reciprocal-force surrogate and multiple time steps --------------------------------
+// Per moving site: 48 invariant inputs and 16 vector features from the sites within
+// SG_RC; a 48-64-64-16 SiLU network gives the coefficients of the vector features.
+constexpr int SG_D = 48, SG_H = 64, SG_M = 16;
+constexpr float SG_RC = 6.0f;
+constexpr int SG_OFF_W1 = 0, SG_OFF_B1 = 3072, SG_OFF_W2 = 3136, SG_OFF_B2 = 7232, SG_OFF_W3 = 7296,
+              SG_OFF_B3 = 8320, SG_NP = 8336;
+// bfloat16 copies of the weights: W1 [64][48], W2 [64][64], W3 [16][64], and the transposes
+// W2T [64][64], W3T [64][16] that the backward pass multiplies by
+constexpr int SG_WB_W1 = 0, SG_WB_W2 = 3072, SG_WB_W3 = 7168, SG_WB_W2T = 8192, SG_WB_W3T = 12288, SG_WB_N = 13312;
+constexpr int SG_NORM_N = 2 * SG_D + SG_M;   // mu[48], sigma[48], su[16]
+constexpr int SG_TRAIN_BLOCKS = 128;         // most blocks of one training launch
+constexpr int SG_GP = SG_NP + 1;             // gradient partial row: parameters, then the loss
+
+// device-resident state of the scheme, one per handle
+struct MtsState {
+    int state;        // 0: collect (full k-sum every step, training), 1: multiple time steps
+    int exact;        // this step runs the full k-sum
+    int phase;        // steps since the last exact step
+    int frames;       // exact frames in the current collect run
+    int train_now;    // this exact step trains
+    int have_delta;   // a correction from an exact step with the current weights exists
+    int norm_ready;   // input normalisation fixed (first frame, or given weights)
+    int prior;        // set the physics prior (c_0 = su_0) when the normalisation is fixed
+    int adam_t;       // Adam steps taken
+    int have_prev;    // a previous full-sum frame is stored (training pairs)
+    int have_slope;   // the last two corrections came from the same weights: extrapolate
+    int pad;
+    long long step;                      // steps since the scheme was switched on
+    long long delta_step, delta_prev_step;  // steps of the last two corrections
+    long long n_exact, n_surrogate, n_fallback, n_judged;
+    double err_last, err_max, err_sum;   // judged relative force RMS errors
+    double t2;        // mean |F_k - F_k(previous full sum)|^2 of the last full sum (loss scale)
+    double loss_last;
+    double de_acc;    // reciprocal energy change since the last exact step (path integral)
+};
+
+struct MtsArgs {
+    MtsState *st;
+    int interval, train_frames, model, n_sblk, n_tblk, pad;   // model 1: network, 0: held force
+    double tol;
+    float lr;
+    float *feat, *vec, *sg;                    // [n][48], [n][16][3], [3][n]
+    float *feat_prev, *vec_prev;               // the previous full-sum frame's
+    double *delta, *frec, *fmts, *rprev, *fprev; // [3][n]
+    double *frec_prev;                         // [3][n]
+    double *delta_prev;                        // [3][n] the correction before the last
+    float *params, *adam_m, *adam_v, *gsum;    // [SG_NP] (gsum: [SG_GP])
+    unsigned short *wb;                        // [SG_WB_N]
+    float *norm;                               // [SG_NORM_N]
+    float *gpart;                              // [SG_TRAIN_BLOCKS][SG_GP]
+    double *jpart;                             // [n_sblk][4] judge partials
+    double *epart;                             // [n_sblk] energy path partials
+};
+
 // ---- batches of independent systems ----------------------------------------------------
 // One entry per system: everything its launches take.  A batched kernel covers the blocks of
 // every system in one launch; a block map entry (system, local block) tells each block whose

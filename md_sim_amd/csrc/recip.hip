@@ -192,14 +192,16 @@ __global__ __launch_bounds__(SF_BLOCK) void k_sf(const double *__restrict__ x, c
                                                  const double *__restrict__ z, const double *__restrict__ q, int n,
                                                  double cux, double cuy, double cuz,
                                                  const KRow *__restrict__ rows, int n_rows, int rpb,
-                                                 double2 *__restrict__ partial, int nk) {
+                                                 double2 *__restrict__ partial, int nk,
+                                                 const MtsState *__restrict__ mts) {
+    if (mts && !mts->exact) return;   // a surrogate step: no k-sum
     sf_body<R, NS>(x, y, z, q, n, cux, cuy, cuz, rows, n_rows, rpb, partial, nk, blockIdx.x, blockIdx.y, gridDim.x, gridDim.y);
 }
 
 template __global__ void k_sf<double, 4>(const double *, const double *, const double *, const double *, int, double,
-                                        double, double, const KRow *, int, int, double2 *, int);
+                                        double, double, const KRow *, int, int, double2 *, int, const MtsState *);
 template __global__ void k_sf<float, 4>(const double *, const double *, const double *, const double *, int, double,
-                                       double, double, const KRow *, int, int, double2 *, int);
+                                       double, double, const KRow *, int, int, double2 *, int, const MtsState *);
 
 // S(k) = sum over site blocks + static frozen part; T(k) = A_k conj(S(k)); per-block
 // partial energies and virial:  eK = A|S|^2, eLL = A|S_moving|^2, vir_ab = 1/2 eK (d_ab - g k_a k_b).
@@ -264,7 +266,9 @@ __global__ __launch_bounds__(256) void k_sf_reduce(const double2 *__restrict__ p
                                                    const double2 *__restrict__ s_wall, const double *__restrict__ ak,
                                                    const int *__restrict__ kvec, double pipix, double pipiy,
                                                    double pipiz, double lamb, double2 *__restrict__ s_out,
-                                                   double4 *__restrict__ t_out, double *__restrict__ epart) {
+                                                   double4 *__restrict__ t_out, double *__restrict__ epart,
+                                                   const MtsState *__restrict__ mts) {
+    if (mts && !mts->exact) return;   // k partials keep the last full sum's values
     sf_reduce_body(partial, n_sb, nk, s_wall, ak, kvec, pipix, pipiy, pipiz, lamb, s_out, t_out, epart, blockIdx.x, gridDim.x);
 }
 
@@ -370,16 +374,18 @@ __global__ __launch_bounds__(256) void k_bp(const double *__restrict__ x, const 
                                             const double *__restrict__ z, const double *__restrict__ q, int n,
                                             double cux, double cuy, double cuz, const KRow *__restrict__ rows,
                                             const int *__restrict__ row_start, const double4 *__restrict__ T,
-                                            double pipix, double pipiy, double pipiz, double *__restrict__ fpart) {
+                                            double pipix, double pipiy, double pipiz, double *__restrict__ fpart,
+                                            const MtsState *__restrict__ mts) {
+    if (mts && !mts->exact) return;
     bp_body<R, NS>(x, y, z, q, n, cux, cuy, cuz, rows, row_start, T, pipix, pipiy, pipiz, fpart, blockIdx.x, blockIdx.y, gridDim.x, gridDim.y);
 }
 
 template __global__ void k_bp<double, 4>(const double *, const double *, const double *, const double *, int, double,
                                          double, double, const KRow *, const int *, const double4 *, double, double,
-                                         double, double *);
+                                         double, double *, const MtsState *);
 template __global__ void k_bp<float, 4>(const double *, const double *, const double *, const double *, int, double,
                                         double, double, const KRow *, const int *, const double4 *, double, double,
-                                        double, double *);
+                                        double, double *, const MtsState *);
 
 }  // namespace mdf
 

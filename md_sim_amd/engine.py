@@ -54,7 +54,8 @@ def lib() -> C.CDLL:
                    "mdf_get_kvectors", "mdf_get_structure_factor", "mdf_stage_times", "mdf_device_results",
                    "mdf_zero", "mdf_reset_stage_times", "mdf_set_stage_timing", "mdf_autotune",
                    "mdf_set_launch_shapes", "mdf_batch_create", "mdf_batch_compute_device",
-                   "mdf_batch_read_outputs", "mdf_batch_autotune"):
+                   "mdf_batch_read_outputs", "mdf_batch_autotune", "mdf_mts_enable", "mdf_mts_get_stats",
+                   "mdf_mts_get_weights", "mdf_mts_set_weights", "mdf_mts_features", "mdf_mts_eval"):
             getattr(L, fn).restype = C.c_int
         L.mdf_destroy.restype = None
         L.mdf_batch_destroy.restype = None
@@ -143,6 +144,43 @@ class ForceEngine:
         _check(lib().mdf_get_structure_factor(self._h, _dp(sf), n.value))
         return kv.reshape(-1, 3), ak, sf[0:2 * n.value:2] + 1j * sf[1:2 * n.value:2]
 
+    # -- reciprocal-force surrogate with multiple time steps ------------------------------
+    def mts_enable(self, interval: int = 4, model: str = "network", train_frames: int = 200, train_iters: int = 1,
+                   tol: float = 1e-3, lr: float = 5e-4, seed: int = 1):
+        """Full k-sum every ``interval``-th step; in between the k-space force is the
+        surrogate's (``model="network"``) or the last full sum's (``"held"``) plus the
+        correction from the last full sum.  The first ``train_frames`` steps (and as many
+        after each fallback) run the full sum and train the network."""
+        p = abi.MtsParams(interval=interval, model=abi.MTS_MODELS[model], train_frames=train_frames,
+                          train_iters=train_iters, tol=tol, lr=lr, seed=seed)
+        _check(lib().mdf_mts_enable(self._h, C.byref(p)))
+
+    def mts_disable(self):
+        _check(lib().mdf_mts_enable(self._h, None))
+
+    def mts_stats(self) -> dict:
+        s = abi.MtsStats()
+        _check(lib().mdf_mts_get_stats(self._h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in abi.MtsStats._fields_}
+
+    def mts_weights(self) -> np.ndarray:
+        w = np.zeros(abi.MTS_WEIGHTS, np.float32)
+        _check(lib().mdf_mts_get_weights(self._h, w.ctypes.data_as(C.POINTER(C.c_float)), len(w)))
+        return w
+
+    def set_mts_weights(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        _check(lib().mdf_mts_set_weights(self._h, w.ctypes.data_as(C.POINTER(C.c_float)), len(w)))
+
+    def mts_features(self):
+        """The surrogate's inputs [n_move, 48] and vector features [n_move, 16, 3] at the
+        positions of the last step."""
+        feat = np.zeros((self.n_move, 48), np.float32)
+        vec = np.zeros((self.n_move, 16, 3), np.float32)
+        fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+        _check(lib().mdf_mts_features(self._h, fp(feat), fp(vec)))
+        return feat, vec
+
     def stage_times(self):
         names = (C.c_char_p * 16)()
         ms = (C.c_float * 16)()
@@ -179,6 +217,42 @@ class ForceEngine:
             self.close()
         except Exception:
             pass
+
+
+def mts_eval(weights, feat, vec, target=None, feat_b=None, vec_b=None, target_b=None, lr: float = 0.0,
+             adam_t: int = 0):
+    """The surrogate network on given inputs, on the current GPU, with the scheme's own
+    kernels: predicted forces [n, 3] of frame a (feat [n, 48], vec [n, 16, 3]); with targets
+    [n, 3] of frame a and of a second frame b, also the gradient of the loss on their change
+    (flat parameters, then the loss) and, for lr > 0, the parameters after one Adam step."""
+    fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+    w = np.ascontiguousarray(weights, np.float32)
+    feat = np.ascontiguousarray(feat, np.float32)
+    vec = np.ascontiguousarray(vec, np.float32)
+    n = len(feat)
+    f = np.zeros(3 * n, np.float32)
+    grad = np.zeros(abi.MTS_WEIGHTS - 112 + 1, np.float32)
+    params = np.zeros(abi.MTS_WEIGHTS - 112, np.float32)
+    args = [None] * 5
+    if target is not None:
+        fb = np.ascontiguousarray(feat_b, np.float32)
+        vb = np.ascontiguousarray(vec_b, np.float32)
+        ta = np.ascontiguousarray(np.asarray(target, np.float64).T)
+        tb = np.ascontiguousarray(np.asarray(target_b, np.float64).T)
+        args = [ta, fb, vb, tb]
+        _check(lib().mdf_mts_eval(fp(w), n, fp(feat), fp(vec), _dp(ta), fp(fb), fp(vb), _dp(tb), C.c_float(lr),
+                                  int(adam_t), fp(f), fp(grad), fp(params)))
+    else:
+        _check(lib().mdf_mts_eval(fp(w), n, fp(feat), fp(vec), None, None, None, None, C.c_float(0.0), 0, fp(f),
+                                  None, None))
+    del args
+    out = {"force": f.reshape(3, n).T.copy()}
+    if target is not None:
+        out["grad"] = grad[:-1].copy()
+        out["loss"] = float(grad[-1])
+        if lr > 0:
+            out["params"] = params
+    return out
 
 
 class ForceBatch:

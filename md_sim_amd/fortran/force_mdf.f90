@@ -18,13 +18,22 @@
 !   MDF_PRECISION      fp64 | fp32   FP32 pair/structure-factor arithmetic with FP64
 !                                    accumulation; default fp64
 !   MDF_DEVICE         HIP device index; default 0
+!   MDF_MTS            k-space force by multiple time steps: the full sum every MDF_MTS-th
+!                      step, the surrogate plus the last correction in between (no
+!                      counterpart in the reference); unset or 1: full sum every step
+!   MDF_MTS_MODEL      network | held; default network
+!   MDF_MTS_TRAIN      full-sum training steps before the scheme starts; default 200
+!   MDF_MTS_TOL        judged relative force RMS error that triggers the fallback; default 1e-3
+!   MDF_MTS_LOG        file for one line of the scheme's counters per step
 !----------------------------------------------------------------------------
 module force
     use nnlist
     use measure
     use mdforce_c, only: mdf_topology, mdf_params, mdf_outputs, mdf_create, mdf_zero, mdf_destroy, &
                          mdf_error_message, MDF_OK, MDF_FP64, MDF_FP32_ACC64, MDF_CONST_REFERENCE, &
-                         MDF_ERFC_AS7126, MDF_ALPHA_REFERENCE, MDF_Z_OPEN, MDF_Z_PERIODIC_OPEN_LIST
+                         MDF_ERFC_AS7126, MDF_ALPHA_REFERENCE, MDF_Z_OPEN, MDF_Z_PERIODIC_OPEN_LIST, &
+                         mdf_mts_params, mdf_mts_stats, mdf_mts_enable, mdf_mts_get_stats, MDF_MTS_HELD, &
+                         MDF_MTS_NETWORK
     use, intrinsic :: iso_c_binding
     implicit none
 
@@ -40,6 +49,7 @@ module force
 
     ! engine handle and the arrays the C side points into
     type(c_ptr), private :: sys = c_null_ptr
+    integer, private :: mts_log = 0                ! unit of the scheme's log (0: none)
     type(mdf_topology), private :: top
     type(mdf_params), private :: par
     type(mdf_outputs), private :: out
@@ -218,9 +228,44 @@ module force
             print *, 'force_init(): ', trim(mdf_error_message())
             stop 'force_init(): force engine could not be created'
         end if
+        call mts_init
         totpres = 0.0
         call force_zero
     end subroutine force_init
+
+    ! multiple time steps for the k-space force, when MDF_MTS asks for them
+    subroutine mts_init
+        type(mdf_mts_params) :: mp
+        character(len=64) :: v
+        integer :: interval, ios
+        integer(c_int) :: rc
+        v = env_value('MDF_MTS')
+        interval = 1
+        if (len_trim(v) > 0) read(v, *, iostat=ios) interval
+        if (interval <= 1) return
+        mp%interval = int(interval, c_int)
+        mp%model = MDF_MTS_NETWORK
+        if (trim(env_value('MDF_MTS_MODEL')) == 'held') mp%model = MDF_MTS_HELD
+        mp%train_frames = 200
+        v = env_value('MDF_MTS_TRAIN')
+        if (len_trim(v) > 0) read(v, *, iostat=ios) mp%train_frames
+        mp%train_iters = 1
+        mp%tol = 1.0d-3
+        v = env_value('MDF_MTS_TOL')
+        if (len_trim(v) > 0) read(v, *, iostat=ios) mp%tol
+        mp%lr = 5.0e-4
+        mp%seed = 1
+        rc = mdf_mts_enable(sys, mp)
+        if (rc /= MDF_OK) then
+            print *, 'force_init(): ', trim(mdf_error_message())
+            stop 'force_init(): multiple time steps could not be set up'
+        end if
+        v = env_value('MDF_MTS_LOG')
+        if (len_trim(v) > 0) then
+            mts_log = 97
+            open(mts_log, file=trim(v), status='replace')
+        end if
+    end subroutine mts_init
 
     ! address of a real*8 module array for mdf_create, which copies what it needs
     function c_loc_or_null(a) result(p)
@@ -279,6 +324,18 @@ module force
                 tefipos(it - 2, l) = tefipos(it - 2, l) + ftz(i)
             end do
         end if
+        if (mts_log > 0) call mts_log_line
     end subroutine force_get
+
+    ! one line per step: state, last step's kind, full sums, steps without, fallbacks,
+    ! judgements, last and largest judged error
+    subroutine mts_log_line
+        type(mdf_mts_stats) :: st
+        integer(c_int) :: rc
+        rc = mdf_mts_get_stats(sys, st)
+        if (rc /= MDF_OK) return
+        write(mts_log, '(2i3,4i9,2es14.5)') st%state, st%last_exact, st%n_exact, st%n_surrogate, st%n_fallback, &
+            st%n_judged, st%err_last, st%err_max
+    end subroutine mts_log_line
 
 end module force

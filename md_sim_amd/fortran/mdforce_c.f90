@@ -38,6 +38,21 @@ module mdforce_c
                           erfc_mode, max_neighbors, device, virial
     end type mdf_params
 
+    integer(c_int), parameter :: MDF_MTS_HELD = 0, MDF_MTS_NETWORK = 1
+
+    type, bind(C) :: mdf_mts_params
+        integer(c_int) :: interval, model, train_frames, train_iters
+        real(c_double) :: tol
+        real(c_float) :: lr
+        integer(c_int32_t) :: seed
+    end type mdf_mts_params
+
+    type, bind(C) :: mdf_mts_stats
+        integer(c_int) :: state, last_exact
+        integer(c_int64_t) :: n_exact, n_surrogate, n_fallback, n_judged, adam_steps
+        real(c_double) :: err_last, err_max, err_mean, loss_last
+    end type mdf_mts_stats
+
     type, bind(C) :: mdf_outputs
         type(c_ptr) :: f_site, f_com, torque, e_vdw, e_coul_real
         real(c_double) :: e_recip, e_recip_ll, e_recip_wl, e_recip_ww, e_self, e_slab, &
@@ -72,6 +87,18 @@ module mdforce_c
             integer(c_int), value :: reps
             integer(c_int), intent(out) :: chosen(4)
         end function mdf_autotune
+
+        integer(c_int) function mdf_mts_enable(sys, p) bind(C, name='mdf_mts_enable')
+            import :: c_int, c_ptr, mdf_mts_params
+            type(c_ptr), value :: sys
+            type(mdf_mts_params), intent(in) :: p
+        end function mdf_mts_enable
+
+        integer(c_int) function mdf_mts_get_stats(sys, st) bind(C, name='mdf_mts_get_stats')
+            import :: c_int, c_ptr, mdf_mts_stats
+            type(c_ptr), value :: sys
+            type(mdf_mts_stats), intent(out) :: st
+        end function mdf_mts_get_stats
 
         subroutine mdf_destroy(sys) bind(C, name='mdf_destroy')
             import :: c_ptr

@@ -483,7 +483,11 @@ def config(name: str) -> System:
       k 11/11/33 (|k|max of slab2048 kept);
     * ``slab16k_k8`` - the same with k 8/8/24;
     * ``nacl4``    - 512 mobile ions in a periodic 4x4x4-cell rock-salt crystal (closed-form
-      Madelung energy).
+      Madelung energy);
+    * ``slab512``  - 512 waters on a 6x6x(4 layers) slab (576 ions), rc 10, k 6/6/18 (small
+      case of the surrogate with multiple time steps);
+    * ``slab32k``  - 32768 waters on a 32x32x(3 layers) slab (12288 ions), box 180.48 x 180.48
+      x 541.44 A, rc 10, k 21/21/63 (the surrogate's large interface).
     """
     if name == "bulk216":
         return bulk_water()
@@ -500,4 +504,8 @@ def config(name: str) -> System:
         return water_on_slab(16384, 16, 6, seed=1002, rcut=10.0, kmax=8, lw=80.0, name=name)
     if name == "nacl4":
         return rocksalt_bulk(4)
+    if name == "slab512":
+        return water_on_slab(512, 6, 4, seed=1201, rcut=10.0, kmax=6, lw=40.0, name=name)
+    if name == "slab32k":
+        return water_on_slab(32768, 32, 3, seed=1004, rcut=10.0, kmax=21, lw=80.0, name=name)
     raise KeyError(name)

@@ -32,3 +32,16 @@ int mdf_zero(mdf_system *sys) { (void)sys; return MDF_OK; }
 void mdf_destroy(mdf_system *sys) { orc_destroy((orc_system *)sys); }
 
 const char *mdf_last_error(void) { return orc_last_error(); }
+
+/* The oracle computes every step with the full k-sum: multiple time steps are refused
+   (switching them off is accepted), and there are no counters to report. */
+int mdf_mts_enable(mdf_system *sys, const mdf_mts_params *p) {
+    (void)sys;
+    return p ? MDF_ERR_ARG : MDF_OK;
+}
+
+int mdf_mts_get_stats(mdf_system *sys, mdf_mts_stats *out) {
+    (void)sys;
+    (void)out;
+    return MDF_ERR_ARG;
+}
