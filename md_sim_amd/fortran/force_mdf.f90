@@ -76,7 +76,7 @@ module force
     ! value of an environment variable ('' when unset)
     function env_value(name) result(v)
         character(len=*), intent(in) :: name
-        character(len=64) :: v
+        character(len=512) :: v
         integer :: st
         v = ''
         call get_environment_variable(name, v, status=st)
@@ -236,7 +236,7 @@ module force
     ! multiple time steps for the k-space force, when MDF_MTS asks for them
     subroutine mts_init
         type(mdf_mts_params) :: mp
-        character(len=64) :: v
+        character(len=512) :: v
         integer :: interval, ios
         integer(c_int) :: rc
         v = env_value('MDF_MTS')
