@@ -51,6 +51,7 @@ __global__ void k_sg_control(MtsArgs a, int again);
 template <int G>
 __global__ void k_sg_features(DevSys S, MtsArgs a);
 __global__ void k_sg_norm(MtsArgs a, int n);
+__global__ void k_sg_pack(DevSys S, MtsArgs a);
 __global__ void k_sg_infer(MtsArgs a, int n, int after_training, const DevFlags *flags);
 __global__ void k_sg_train(MtsArgs a, int n, const DevFlags *flags);
 __global__ void k_sg_adam(MtsArgs a, int nb, const DevFlags *flags);
@@ -60,7 +61,7 @@ __global__ void k_sg_exact(DevSys S, MtsArgs a, const double *fpart, int n_group
 __global__ void k_sg_judge(DevSys S, MtsArgs a);
 __global__ void k_sg_delta(DevSys S, MtsArgs a);
 __global__ void k_sg_post(DevSys S, MtsArgs a);
-constexpr int SG_FEAT_G = 8;   // lanes per lead in the input kernel
+constexpr int SG_FEAT_G = 16;  // lanes per lead in the input kernel
 constexpr int BP_NS = 4;   // sites per back-projection thread
 __global__ void k_epilogue(DevSys S, const double *fpart, int n_groups, const WallLR *wl, int ew3dc, double crf0,
                            const double *dipz, double *epart, double *fcom, double *tq, const MtsState *mts,
@@ -259,6 +260,8 @@ struct mdf_system {
         sg_vec_prev;
     DBuf<unsigned short> sg_wb;
     DBuf<double> sg_delta, sg_frec, sg_fmts, sg_rprev, sg_fprev, sg_jpart, sg_epart, sg_frec_prev, sg_delta_prev;
+    DBuf<float4> sg_pos4;
+    DBuf<int> sg_msize;
 };
 
 static void mts_free(mdf_system *s) {
@@ -267,6 +270,8 @@ static void mts_free(mdf_system *s) {
                          &s->sg_norm, &s->sg_gpart, &s->sg_feat_prev, &s->sg_vec_prev};
     for (auto *b : fb) b->release();
     s->sg_wb.release();
+    s->sg_pos4.release();
+    s->sg_msize.release();
     DBuf<double> *db[] = {&s->sg_delta, &s->sg_frec, &s->sg_fmts, &s->sg_rprev, &s->sg_fprev, &s->sg_jpart,
                           &s->sg_epart, &s->sg_frec_prev, &s->sg_delta_prev};
     for (auto *b : db) b->release();
@@ -535,6 +540,8 @@ static MtsArgs mts_args(mdf_system *s) {
     a.feat = s->sg_feat.p; a.vec = s->sg_vec.p; a.sg = s->sg_out.p;
     a.feat_prev = s->sg_feat_prev.p; a.vec_prev = s->sg_vec_prev.p; a.frec_prev = s->sg_frec_prev.p;
     a.delta_prev = s->sg_delta_prev.p;
+    a.pos4 = s->sg_pos4.p;
+    a.msize = s->sg_msize.p;
     a.delta = s->sg_delta.p; a.frec = s->sg_frec.p; a.fmts = s->sg_fmts.p;
     a.rprev = s->sg_rprev.p; a.fprev = s->sg_fprev.p;
     a.params = s->sg_params.p; a.adam_m = s->sg_m.p; a.adam_v = s->sg_v.p; a.gsum = s->sg_gsum.p;
@@ -607,6 +614,7 @@ static void launch_forces(mdf_system *s) {
     if (s->ev) (void)hipEventRecord(s->ev[2], st);
     if (mts) {
         if (s->mts_model) {
+            k_sg_pack<<<std::max(1, (s->top.n_total + 255) / 256), 256, 0, st>>>(S, ma);
             k_sg_features<SG_FEAT_G><<<std::max(1, (s->n_lead + 256 / SG_FEAT_G - 1) / (256 / SG_FEAT_G)), 256, 0,
                                        st>>>(S, ma);
             k_sg_norm<<<1, 1024, 0, st>>>(ma, nm);
@@ -1519,6 +1527,13 @@ static int mts_alloc(mdf_system *s) {
     HIPCHK(s->sg_feat.alloc(nm * SG_D));
     HIPCHK(s->sg_vec.alloc(nm * SG_M * 3));
     HIPCHK(s->sg_feat_prev.alloc(nm * SG_D));
+    HIPCHK(s->sg_pos4.alloc((size_t)std::max(s->top.n_total, 1)));
+    {
+        std::vector<int> nsi(s->top.n_types + 1), msize(std::max(s->top.n_total, 1), 0);
+        HIPCHK(hipMemcpy(nsi.data(), s->nsi.p, sizeof(int) * nsi.size(), hipMemcpyDeviceToHost));
+        for (int l = 0; l < s->n_lead; l++) msize[s->h_lead[l]] = nsi[s->h_itype[s->h_lead[l]]];
+        HIPCHK(upload(s->sg_msize, msize.data(), msize.size()));
+    }
     HIPCHK(s->sg_vec_prev.alloc(nm * SG_M * 3));
     HIPCHK(s->sg_out.alloc(3 * nm));
     HIPCHK(s->sg_params.alloc(SG_NP));
@@ -1684,6 +1699,7 @@ int mdf_mts_features(mdf_system *s, float *feat, float *vec) {
     HIPCHK(hipSetDevice(s->device));
     HIPCHK(hipStreamSynchronize(s->stream));
     const MtsArgs ma = mts_args(s);
+    k_sg_pack<<<std::max(1, (s->top.n_total + 255) / 256), 256, 0, s->stream>>>(s->S, ma);
     k_sg_features<SG_FEAT_G><<<std::max(1, (s->n_lead + 256 / SG_FEAT_G - 1) / (256 / SG_FEAT_G)), 256, 0,
                                s->stream>>>(s->S, ma);
     HIPCHK(hipGetLastError());

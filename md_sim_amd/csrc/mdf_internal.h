@@ -157,6 +157,7 @@ __device__ __forceinline__ double list_d2(const DevSys &S, int i, int j) {
 // Per moving site: 48 invariant inputs and 16 vector features from the sites within
 // SG_RC; a 48-64-64-16 SiLU network gives the coefficients of the vector features.
 constexpr int SG_D = 48, SG_H = 64, SG_M = 16;
+constexpr int SG_MV = 8;   // vector features computed (the rest of the 16 outputs' partners are zero)
 constexpr float SG_RC = 6.0f;
 constexpr int SG_OFF_W1 = 0, SG_OFF_B1 = 3072, SG_OFF_W2 = 3136, SG_OFF_B2 = 7232, SG_OFF_W3 = 7296,
               SG_OFF_B3 = 8320, SG_NP = 8336;
@@ -197,6 +198,8 @@ struct MtsArgs {
     float lr;
     float *feat, *vec, *sg;                    // [n][48], [n][16][3], [3][n]
     float *feat_prev, *vec_prev;               // the previous full-sum frame's
+    float4 *pos4;                              // [n_total] x, y, z, q as float
+    const int *msize;                          // [n_total] sites of the molecule a lead carries (0: not a lead)
     double *delta, *frec, *fmts, *rprev, *fprev; // [3][n]
     double *frec_prev;                         // [3][n]
     double *delta_prev;                        // [3][n] the correction before the last

@@ -23,7 +23,8 @@
 // the smooth step wg(R) = 1 - s^2 (3 - 2 s), s = clamp((R - rc + 2)/2, 0, 1), of that lead
 // distance, and the other sites of i's own molecule with weight 1:
 //   U_0               facq q_i sum_j q_j wg (erf(a r)/r^2 - 2a/sqrt(pi) exp(-a^2 r^2)/r) r_ij/r
-//   U_m (m = 1..15)   facq q_i sum_j q_j wg T_{m-1}(2 r/rc - 1) r_ij/r   (Chebyshev T)
+//   U_m (m = 1..7)    facq q_i sum_j q_j wg T_{m-1}(2 r/rc - 1) r_ij/r   (Chebyshev T)
+//   U_8 .. U_15       0 (the network's 16 outputs fill one matrix-core tile)
 // Network (bfloat16 weights, FP32 accumulation on the matrix cores): h1 = silu(W1 xn + b1),
 // h2 = silu(W2 h1 + b2), c = W3 h2 + b3, S_i = sum_m c_m U_i,m / su_m, xn = (x - mu)/sigma.
 // Training: Adam on the change between two full-sum frames a, b (what the scheme uses),
@@ -83,67 +84,91 @@ __global__ void k_sg_control(MtsArgs a, int again) {
 }
 
 // ---- inputs and vector features ----------------------------------------------------------
+// per-site accumulators as one array: radial [0, 16), moments [16, 34), vectors [34, 58)
+constexpr int SA_A = 16, SA_U = 34, SA_N = 34 + 3 * SG_MV;
 struct SiteAcc {
-    float rad[16];
-    float A[6][3];
-    float U[SG_M][3];
+    float v[SA_N];
 };
 
-// invariant inputs: one site j within SG_RC of site i
+// invariant inputs: one site j within SG_RC of site i.  The Gaussians of equally spaced
+// centres follow from one exponential each by the ratio of neighbours,
+// g_{m+1} / g_m = exp(2 eta dc r) exp(-eta (c_{m+1}^2 - c_m^2)), with eta dc = 1.5 for both
+// families (radial: eta 2, centres 0.75 (m+1); angular: eta 1, centres 1.5 (s+1)).
 __device__ __forceinline__ void sg_inputs(SiteAcc &s, float dx, float dy, float dz, float r2, float qj) {
-    const float r = sqrtf(r2), ri = 1.0f / r;
-    const float fc = 0.5f * (cosf(3.14159265358979f / SG_RC * r) + 1.0f);
+    const float ri = __builtin_amdgcn_rsqf(r2), r = r2 * ri;
+    const float fc = 0.5f * (__cosf(3.14159265358979f / SG_RC * r) + 1.0f);
     const float ux = dx * ri, uy = dy * ri, uz = dz * ri;
     const float aq = fabsf(qj) * fc;
     const float wp = qj > 0.0f ? aq : 0.0f, wn = qj < 0.0f ? aq : 0.0f;
+    // exp(-2 (c_{m+1}^2 - c_m^2)), c_m = 0.75 (m + 1), and exp(-(c_{s+1}^2 - c_s^2)), c_s = 1.5 (s + 1)
+    constexpr float KR[8] = {3.421811831e-02f, 3.606563136e-03f, 3.801289579e-04f, 4.006529739e-05f, 4.222851277e-06f, 4.450852500e-07f, 4.691164022e-08f, 4.944450502e-09f};
+    constexpr float KA[3] = {1.170879621e-03f, 1.300729765e-05f, 1.444980246e-07f};
+    const float e3 = __expf(3.0f * r);
+    float g = __expf(-2.0f * (r - 0.75f) * (r - 0.75f));
 #pragma unroll
     for (int m = 0; m < 8; m++) {
-        const float d = r - 0.75f * (m + 1);
-        const float g = expf(-2.0f * d * d);
-        s.rad[m] += wp * g;
-        s.rad[8 + m] += wn * g;
+        s.v[m] += wp * g;
+        s.v[8 + m] += wn * g;
+        g *= e3 * KR[m];
     }
+    float h = __expf(-(r - 1.5f) * (r - 1.5f));
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-        const float d = r - 1.5f * (c + 1);
-        const float g = expf(-d * d);
-        s.A[c][0] += wp * g * ux; s.A[c][1] += wp * g * uy; s.A[c][2] += wp * g * uz;
-        s.A[3 + c][0] += wn * g * ux; s.A[3 + c][1] += wn * g * uy; s.A[3 + c][2] += wn * g * uz;
+        const float hp = wp * h, hn = wn * h;
+        s.v[SA_A + 3 * (c) + (0)] += hp * ux; s.v[SA_A + 3 * (c) + (1)] += hp * uy; s.v[SA_A + 3 * (c) + (2)] += hp * uz;
+        s.v[SA_A + 3 * (3 + c) + (0)] += hn * ux; s.v[SA_A + 3 * (3 + c) + (1)] += hn * uy; s.v[SA_A + 3 * (3 + c) + (2)] += hn * uz;
+        h *= e3 * KA[c];
     }
+}
+
+// erf by Abramowitz & Stegun 7.1.26 (|error| < 1.5e-7), sharing exp(-x^2) with the caller
+__device__ __forceinline__ float erf_as(float x, float ex) {
+    const float t = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * x);
+    const float p = ((((1.061405429f * t - 1.453152027f) * t + 1.421413741f) * t - 0.284496736f) * t + 0.254829592f) * t;
+    return 1.0f - p * ex;
 }
 
 // vector features: one site j of a molecule inside the group cut-off, group weight wg
 __device__ __forceinline__ void sg_vectors(SiteAcc &s, float dx, float dy, float dz, float r2, float qj, float wg,
                                            float alpha, float x_scale) {
-    const float r = sqrtf(r2), ri = 1.0f / r;
+    const float ri = __builtin_amdgcn_rsqf(r2), r = r2 * ri;
     const float ux = dx * ri, uy = dy * ri, uz = dz * ri;
     const float w = qj * wg;
     const float ar = alpha * r;
-    const float kew = w * (erff(ar) * ri * ri - 1.12837916709551f * alpha * expf(-ar * ar) * ri);
-    s.U[0][0] += kew * ux; s.U[0][1] += kew * uy; s.U[0][2] += kew * uz;
+    const float ex = __expf(-ar * ar);
+    const float kew = w * (erf_as(ar, ex) * ri * ri - 1.12837916709551f * alpha * ex * ri);
+    s.v[SA_U + 3 * (0) + (0)] += kew * ux; s.v[SA_U + 3 * (0) + (1)] += kew * uy; s.v[SA_U + 3 * (0) + (2)] += kew * uz;
     const float xr = x_scale * r - 1.0f;     // 2 r / rc - 1
+    const float wx = w * ux, wy = w * uy, wz = w * uz;
     float t0 = 1.0f, t1 = xr;
-    s.U[1][0] += w * ux; s.U[1][1] += w * uy; s.U[1][2] += w * uz;
-    s.U[2][0] += w * t1 * ux; s.U[2][1] += w * t1 * uy; s.U[2][2] += w * t1 * uz;
+    s.v[SA_U + 3 * (1) + (0)] += wx; s.v[SA_U + 3 * (1) + (1)] += wy; s.v[SA_U + 3 * (1) + (2)] += wz;
+    s.v[SA_U + 3 * (2) + (0)] += wx * t1; s.v[SA_U + 3 * (2) + (1)] += wy * t1; s.v[SA_U + 3 * (2) + (2)] += wz * t1;
 #pragma unroll
-    for (int m = 3; m < SG_M; m++) {
+    for (int m = 3; m < SG_MV; m++) {
         const float t2 = 2.0f * xr * t1 - t0;
         t0 = t1;
         t1 = t2;
-        s.U[m][0] += w * t2 * ux; s.U[m][1] += w * t2 * uy; s.U[m][2] += w * t2 * uz;
+        s.v[SA_U + 3 * (m) + (0)] += wx * t2; s.v[SA_U + 3 * (m) + (1)] += wy * t2; s.v[SA_U + 3 * (m) + (2)] += wz * t2;
     }
 }
 
-__device__ __forceinline__ float group_weight(double R, double rg) {
-    const float sx = fminf(fmaxf((float)((R - rg + 2.0) * 0.5), 0.0f), 1.0f);
+__device__ __forceinline__ float group_weight(float R, float rg) {
+    const float sx = fminf(fmaxf((R - rg + 2.0f) * 0.5f, 0.0f), 1.0f);
     return 1.0f - sx * sx * (3.0f - 2.0f * sx);
+}
+
+// positions and charges of every site as float4 for the input kernel
+__global__ __launch_bounds__(256) void k_sg_pack(DevSys S, MtsArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < S.n_total) a.pos4[i] = make_float4((float)S.x[i], (float)S.y[i], (float)S.z[i], (float)S.q[i]);
 }
 
 // G lanes per lead walk its list rows (moving partners: every site of the partner
 // molecule; frozen partners: the listed site) once per site of the lead's molecule, then
 // reduce over the group.  The rows hold every lead within rc (the skin rule) and every
 // frozen site within rc of the lead, which covers the group cut-off at rc; the host checks
-// SG_RC + twice the largest site-to-lead distance < rc for the inputs.
+// SG_RC + twice the largest site-to-lead distance < rc for the inputs.  FP32 throughout:
+// the inputs need relative vectors to ~1e-5 A, not the list's bit-exact predicate.
 template <int G>
 __global__ __launch_bounds__(256) void k_sg_features(DevSys S, MtsArgs a) {
     const int tid = threadIdx.x;
@@ -151,101 +176,116 @@ __global__ __launch_bounds__(256) void k_sg_features(DevSys S, MtsArgs a) {
     const int sub = tid % G;
     const bool active = gid < S.n_lead;
     int i = 0, nsi_i = 0;
-    double xi = 0.0, yi = 0.0, zi = 0.0;
+    float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
     if (active) {
         i = S.lead[gid];
         nsi_i = S.nsi[S.itype[i]];
-        xi = S.x[i]; yi = S.y[i]; zi = S.z[i];
+        pi = a.pos4[i];
     }
     const float alpha = (float)S.alfa;
-    const double rg2 = S.rcutsq, rg = sqrt(S.rcutsq);
-    const float x_scale = (float)(2.0 / rg);
+    const float rg = (float)sqrt(S.rcutsq), rg2 = rg * rg;
+    const float x_scale = 2.0f / rg;
+    const float lx = (float)S.lx, ly = (float)S.ly, lz = (float)S.lz;
+    const float lxi = 1.0f / lx, lyi = 1.0f / ly, lzi = 1.0f / lz;
+    const bool zp = S.zper_force != 0;
     constexpr float RC2 = SG_RC * SG_RC;
+#pragma unroll 1
     for (int as = 0; as < 3; as++) {
         const bool on = active && as < nsi_i;
         SiteAcc acc;
 #pragma unroll
-        for (int m = 0; m < 16; m++) acc.rad[m] = 0.0f;
-#pragma unroll
-        for (int m = 0; m < 6; m++) acc.A[m][0] = acc.A[m][1] = acc.A[m][2] = 0.0f;
-#pragma unroll
-        for (int m = 0; m < SG_M; m++) acc.U[m][0] = acc.U[m][1] = acc.U[m][2] = 0.0f;
+        for (int m = 0; m < SA_N; m++) acc.v[m] = 0.0f;
         const int ia = i + as;
         if (on) {
-            const double oxa = S.x[ia] - xi, oya = S.y[ia] - yi, oza = S.z[ia] - zi;
+            const float4 pa = a.pos4[ia];
             // the other sites of the molecule
             if (sub == 0)
                 for (int b = 0; b < nsi_i; b++)
                     if (b != as) {
-                        const float dx = (float)(S.x[ia] - S.x[i + b]), dy = (float)(S.y[ia] - S.y[i + b]),
-                                    dz = (float)(S.z[ia] - S.z[i + b]);
-                        const float r2 = dx * dx + dy * dy + dz * dz, qb = (float)S.q[i + b];
-                        if (r2 < RC2) sg_inputs(acc, dx, dy, dz, r2, qb);
-                        sg_vectors(acc, dx, dy, dz, r2, qb, 1.0f, alpha, x_scale);
+                        const float4 pb = a.pos4[i + b];
+                        const float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+                        const float r2 = dx * dx + dy * dy + dz * dz;
+                        if (r2 < RC2) sg_inputs(acc, dx, dy, dz, r2, pb.w);
+                        sg_vectors(acc, dx, dy, dz, r2, pb.w, 1.0f, alpha, x_scale);
                     }
             const int n = min(S.cnt_mm[gid], S.cap_mm);
             const int *row = S.nbr_mm + (size_t)gid * S.cap_mm;
+            const int last = S.n_total - 1;
+            // two partner molecules per pass: all their loads are issued before either is used
 #pragma unroll 1
-            for (int k = sub; k < n; k += G) {
-                const int j = row[k];
-                const double xj = S.x[j], yj = S.y[j], zj = S.z[j];
-                double dx = xi - xj; dx -= S.lx * anint_d(dx * S.lxi);
-                double dy = yi - yj; dy -= S.ly * anint_d(dy * S.lyi);
-                double dz = zi - zj;
-                if (S.zper_force) dz -= S.lz * anint_d(dz * S.lzi);
-                const double R2 = dx * dx + dy * dy + dz * dz;
-                if (R2 >= rg2) continue;
-                const float wg = group_weight(sqrt(R2), rg);
-                const int nsj = S.nsi[S.itype[j]];
-                for (int b = 0; b < nsj; b++) {
-                    const double ox = S.x[j + b] - xj, oy = S.y[j + b] - yj, oz = S.z[j + b] - zj;
-                    const float ex = (float)(dx + oxa - ox), ey = (float)(dy + oya - oy), ez = (float)(dz + oza - oz);
-                    const float r2 = ex * ex + ey * ey + ez * ez, qb = (float)S.q[j + b];
-                    if (r2 < RC2) sg_inputs(acc, ex, ey, ez, r2, qb);
-                    if (wg > 0.0f) sg_vectors(acc, ex, ey, ez, r2, qb, wg, alpha, x_scale);
+            for (int k = sub; k < n; k += 2 * G) {
+                const bool two = k + G < n;
+                const int j0 = row[k], j1 = two ? row[k + G] : j0;
+                float4 p0[3], p1[3];
+                int ns0 = a.msize[j0], ns1 = a.msize[j1];
+#pragma unroll
+                for (int b = 0; b < 3; b++) {
+                    p0[b] = a.pos4[min(j0 + b, last)];
+                    p1[b] = a.pos4[min(j1 + b, last)];
+                }
+                if (!two) ns1 = 0;
+#pragma unroll
+                for (int u = 0; u < 2; u++) {
+                    const float4 *pm = u == 0 ? p0 : p1;
+                    const int nsj = u == 0 ? ns0 : ns1;
+                    if (nsj == 0) continue;
+                    // lead-lead image shift, applied to every site pair of the two molecules
+                    float dx = pi.x - pm[0].x, dy = pi.y - pm[0].y, dz = pi.z - pm[0].z;
+                    const float sx = -lx * rintf(dx * lxi), sy = -ly * rintf(dy * lyi),
+                                sz = zp ? -lz * rintf(dz * lzi) : 0.0f;
+                    dx += sx; dy += sy; dz += sz;
+                    const float R2 = dx * dx + dy * dy + dz * dz;
+                    if (R2 >= rg2) continue;
+                    const float wg = group_weight(sqrtf(R2), rg);
+#pragma unroll
+                    for (int b = 0; b < 3; b++) {
+                        if (b >= nsj) break;
+                        const float4 pb = pm[b];
+                        const float ex = pa.x - pb.x + sx, ey = pa.y - pb.y + sy, ez = pa.z - pb.z + sz;
+                        const float r2 = ex * ex + ey * ey + ez * ez;
+                        if (r2 < RC2) sg_inputs(acc, ex, ey, ez, r2, pb.w);
+                        if (wg > 0.0f) sg_vectors(acc, ex, ey, ez, r2, pb.w, wg, alpha, x_scale);
+                    }
                 }
             }
             const int nw = min(S.cnt_mw[gid], S.cap_mw);
             const int *wrow = S.nbr_mw + (size_t)gid * S.cap_mw;
-#pragma unroll 1
+#pragma unroll 2
             for (int k = sub; k < nw; k += G) {
                 const int j = wrow[k];
-                double dx = xi - S.x[j]; dx -= S.lx * anint_d(dx * S.lxi);
-                double dy = yi - S.y[j]; dy -= S.ly * anint_d(dy * S.lyi);
-                double dz = zi - S.z[j];
-                if (S.zper_force) dz -= S.lz * anint_d(dz * S.lzi);
-                const double R2 = dx * dx + dy * dy + dz * dz;
+                const float4 pj = a.pos4[j];
+                float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                const float sx = -lx * rintf(dx * lxi), sy = -ly * rintf(dy * lyi), sz = zp ? -lz * rintf(dz * lzi) : 0.0f;
+                dx += sx; dy += sy; dz += sz;
+                const float R2 = dx * dx + dy * dy + dz * dz;
                 if (R2 >= rg2) continue;
-                const float wg = group_weight(sqrt(R2), rg);
-                const float ex = (float)(dx + oxa), ey = (float)(dy + oya), ez = (float)(dz + oza);
-                const float r2 = ex * ex + ey * ey + ez * ez, qb = (float)S.q[j];
-                if (r2 < RC2) sg_inputs(acc, ex, ey, ez, r2, qb);
-                if (wg > 0.0f) sg_vectors(acc, ex, ey, ez, r2, qb, wg, alpha, x_scale);
+                const float wg = group_weight(sqrtf(R2), rg);
+                const float ex = pa.x - pj.x + sx, ey = pa.y - pj.y + sy, ez = pa.z - pj.z + sz;
+                const float r2 = ex * ex + ey * ey + ez * ez;
+                if (r2 < RC2) sg_inputs(acc, ex, ey, ez, r2, pj.w);
+                if (wg > 0.0f) sg_vectors(acc, ex, ey, ez, r2, pj.w, wg, alpha, x_scale);
             }
         }
+        // reduce over the group in chunks of 6 values: the scheduling barriers keep the
+        // compiler from overlapping all 82 shuffles (and their temporaries) at once
 #pragma unroll
-        for (int o = G / 2; o > 0; o >>= 1) {
+        for (int c0 = 0; c0 < SA_N; c0 += 6) {
 #pragma unroll
-            for (int m = 0; m < 16; m++) acc.rad[m] += __shfl_xor(acc.rad[m], o, 64);
+            for (int o = G / 2; o > 0; o >>= 1)
 #pragma unroll
-            for (int m = 0; m < 6; m++)
-#pragma unroll
-                for (int c = 0; c < 3; c++) acc.A[m][c] += __shfl_xor(acc.A[m][c], o, 64);
-#pragma unroll
-            for (int m = 0; m < SG_M; m++)
-#pragma unroll
-                for (int c = 0; c < 3; c++) acc.U[m][c] += __shfl_xor(acc.U[m][c], o, 64);
+                for (int c = c0; c < c0 + 6 && c < SA_N; c++) acc.v[c] += __shfl_xor(acc.v[c], o, 64);
+            __builtin_amdgcn_sched_barrier(0);
         }
         if (on && sub == 0) {
             float *f = a.feat + (size_t)ia * SG_D;
 #pragma unroll
-            for (int m = 0; m < 16; m++) f[m] = acc.rad[m];
+            for (int m = 0; m < 16; m++) f[m] = acc.v[m];
             int k = 16;
 #pragma unroll
             for (int p = 0; p < 6; p++)
 #pragma unroll
                 for (int q = p; q < 6; q++, k++)
-                    f[k] = acc.A[p][0] * acc.A[q][0] + acc.A[p][1] * acc.A[q][1] + acc.A[p][2] * acc.A[q][2];
+                    f[k] = acc.v[SA_A + 3 * (p) + (0)] * acc.v[SA_A + 3 * (q) + (0)] + acc.v[SA_A + 3 * (p) + (1)] * acc.v[SA_A + 3 * (q) + (1)] + acc.v[SA_A + 3 * (p) + (2)] * acc.v[SA_A + 3 * (q) + (2)];
             const float qi = (float)S.q[ia];
             f[37] = qi;
             f[38] = qi * qi;
@@ -254,13 +294,18 @@ __global__ __launch_bounds__(256) void k_sg_features(DevSys S, MtsArgs a) {
             const float sc = (float)S.facq * qi;
             float *u = a.vec + (size_t)ia * (SG_M * 3);
 #pragma unroll
-            for (int m = 0; m < SG_M; m++) {
-                u[3 * m + 0] = sc * acc.U[m][0]; u[3 * m + 1] = sc * acc.U[m][1]; u[3 * m + 2] = sc * acc.U[m][2];
+            for (int m = 0; m < SG_MV; m++) {
+                u[3 * m + 0] = sc * acc.v[SA_U + 3 * m + 0];
+                u[3 * m + 1] = sc * acc.v[SA_U + 3 * m + 1];
+                u[3 * m + 2] = sc * acc.v[SA_U + 3 * m + 2];
             }
+#pragma unroll
+            for (int m = 3 * SG_MV; m < 3 * SG_M; m++) u[m] = 0.0f;
         }
     }
 }
 template __global__ void k_sg_features<8>(DevSys, MtsArgs);
+template __global__ void k_sg_features<16>(DevSys, MtsArgs);
 
 // ---- input normalisation from the first frame -------------------------------------------
 // mu, sigma per input (sigma 1 where an input does not vary) and su_m = rms |U_m|; sets the

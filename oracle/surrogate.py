@@ -23,8 +23,9 @@ when z is periodic), with taper fc(r) = (cos(pi r / RC) + 1) / 2 and r_ij = r_i 
   keeps neighbour molecules neutral), plus the other sites of i's own molecule (wg = 1):
     U_0 = facq q_i sum_j q_j wg [erf(a r)/r^2 - 2a/sqrt(pi) exp(-a^2 r^2)/r] r_ij / r
           (the field of j's Gaussian-screened charge: the near part of the k-space force)
-    U_m = facq q_i sum_j q_j wg T_{m-1}(2 r / RG - 1) r_ij / r,  m = 1..15 (Chebyshev
-          polynomials of the first kind: a smooth radial basis the network weighs)
+    U_m = facq q_i sum_j q_j wg T_{m-1}(2 r / RG - 1) r_ij / r,  m = 1..7 (Chebyshev
+          polynomials of the first kind: a smooth radial basis the network weighs);
+    U_8 .. U_15 = 0 (the network's 16 outputs fill one matrix-core tile)
 Network: xn = (x - mu) / sigma, h1 = silu(W1 xn + b1), h2 = silu(W2 h1 + b2),
 c = W3 h2 + b3 (16 outputs), predicted force F_i = sum_m c_m U_i,m / su_m.  The force is
 a combination of vectors, so it turns with the system; the inputs do not.
@@ -50,6 +51,7 @@ D_IN = 48
 N_USED = 39
 H = 64
 NV = 16
+NV_USED = 8
 # flat parameter layout
 OFF_W1, OFF_B1 = 0, 64 * 48
 OFF_W2 = OFF_B1 + 64
@@ -124,7 +126,7 @@ def features(pos, q, n_move, box, z_periodic, alpha, facq, lead, rg, chunk=256):
         U[i0:i1, 0] = ((w * kew)[..., None] * rhat).sum(1)
         xr = 2.0 * r / rg - 1.0
         tm1, tm = np.ones_like(r), xr
-        for m in range(NV - 1):
+        for m in range(NV_USED - 1):
             g = tm1 if m == 0 else tm
             if m >= 1:
                 tm1, tm = tm, 2.0 * xr * tm - tm1
