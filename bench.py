@@ -282,7 +282,7 @@ def mts_bench(args, workload):
     e.close()
     runs = []
     for interval, model in ((4, "network"), (2, "network"), (4, "held")):
-        cfg = dict(interval=interval, model=model, train_frames=args.mts_train, train_iters=1, lr=5e-4, tol=0.05)
+        cfg = dict(interval=interval, model=model, train_frames=args.mts_train, train_iters=2, lr=1e-3, tol=0.05)
         e = ForceEngine(s, precision=abi.FP32_ACC64, virial=False)
         e.force_get(*frames[0])
         e.set_launch_shapes(**shapes)

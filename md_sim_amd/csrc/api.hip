@@ -261,6 +261,7 @@ struct mdf_system {
     DBuf<unsigned short> sg_wb;
     DBuf<double> sg_delta, sg_frec, sg_fmts, sg_rprev, sg_fprev, sg_jpart, sg_epart, sg_frec_prev, sg_delta_prev;
     DBuf<float4> sg_pos4;
+    DBuf<float> sg_out_prev;
     DBuf<int> sg_msize;
 };
 
@@ -271,6 +272,7 @@ static void mts_free(mdf_system *s) {
     for (auto *b : fb) b->release();
     s->sg_wb.release();
     s->sg_pos4.release();
+    s->sg_out_prev.release();
     s->sg_msize.release();
     DBuf<double> *db[] = {&s->sg_delta, &s->sg_frec, &s->sg_fmts, &s->sg_rprev, &s->sg_fprev, &s->sg_jpart,
                           &s->sg_epart, &s->sg_frec_prev, &s->sg_delta_prev};
@@ -535,9 +537,10 @@ static MtsArgs mts_args(mdf_system *s) {
     a.model = s->mts_model;
     a.n_sblk = s->sg_n_sblk;
     a.n_tblk = s->sg_n_tblk;
+    a.train_iters = s->mts_train_iters;
     a.tol = s->mts_tol;
     a.lr = s->mts_lr;
-    a.feat = s->sg_feat.p; a.vec = s->sg_vec.p; a.sg = s->sg_out.p;
+    a.feat = s->sg_feat.p; a.vec = s->sg_vec.p; a.sg = s->sg_out.p; a.sg_prev = s->sg_out_prev.p;
     a.feat_prev = s->sg_feat_prev.p; a.vec_prev = s->sg_vec_prev.p; a.frec_prev = s->sg_frec_prev.p;
     a.delta_prev = s->sg_delta_prev.p;
     a.pos4 = s->sg_pos4.p;
@@ -658,6 +661,7 @@ static void launch_forces(mdf_system *s) {
                 k_sg_adam<<<(SG_GP + 255) / 256, 256, 0, st>>>(ma, s->sg_n_tblk, s->flags.p);
             }
             k_sg_infer<<<s->sg_n_iblk, 256, 0, st>>>(ma, nm, 1, s->flags.p);
+            k_sg_infer<<<s->sg_n_iblk, 256, 0, st>>>(ma, nm, 2, s->flags.p);
         }
         k_sg_delta<<<s->sg_n_sblk, 256, 0, st>>>(S, ma);
         k_sg_post<<<1, 64, 0, st>>>(S, ma);
@@ -1536,6 +1540,7 @@ static int mts_alloc(mdf_system *s) {
     }
     HIPCHK(s->sg_vec_prev.alloc(nm * SG_M * 3));
     HIPCHK(s->sg_out.alloc(3 * nm));
+    HIPCHK(s->sg_out_prev.alloc(3 * nm));
     HIPCHK(s->sg_params.alloc(SG_NP));
     HIPCHK(s->sg_m.alloc(SG_NP));
     HIPCHK(s->sg_v.alloc(SG_NP));
@@ -1767,7 +1772,7 @@ int mdf_mts_eval(const float *weights, int32_t n, const float *feat, const float
     if (e == hipSuccess) {
         MtsArgs a;
         memset(&a, 0, sizeof a);
-        a.st = st.p; a.model = 1; a.n_sblk = n_sblk; a.n_tblk = n_tblk; a.lr = lr;
+        a.st = st.p; a.model = 1; a.n_sblk = n_sblk; a.n_tblk = n_tblk; a.lr = lr; a.train_iters = 1;
         a.feat = d_feat.p; a.vec = d_vec.p; a.sg = d_sg.p; a.frec = d_frec.p;
         a.feat_prev = d_feat_b.p; a.vec_prev = d_vec_b.p; a.frec_prev = d_frec_b.p;
         a.params = d_params.p; a.adam_m = d_m.p; a.adam_v = d_v.p; a.gsum = d_gsum.p;

@@ -193,10 +193,11 @@ struct MtsState {
 
 struct MtsArgs {
     MtsState *st;
-    int interval, train_frames, model, n_sblk, n_tblk, pad;   // model 1: network, 0: held force
+    int interval, train_frames, model, n_sblk, n_tblk, train_iters;   // model 1: network, 0: held force
     double tol;
     float lr;
     float *feat, *vec, *sg;                    // [n][48], [n][16][3], [3][n]
+    float *sg_prev;                            // [3][n] the previous full-sum frame, new weights
     float *feat_prev, *vec_prev;               // the previous full-sum frame's
     float4 *pos4;                              // [n_total] x, y, z, q as float
     const int *msize;                          // [n_total] sites of the molecule a lead carries (0: not a lead)

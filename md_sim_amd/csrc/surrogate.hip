@@ -71,6 +71,7 @@ __global__ void k_sg_control(MtsArgs a, int again) {
     if (threadIdx.x != 0 || again) return;   // a redone step keeps its decision
     MtsState &st = *a.st;
     st.step += 1;
+    st.train_now = 0;   // only a full-sum step's judge may set it (training needs that step's target)
     if (st.state == 1) {
         st.phase += 1;
         st.exact = st.phase >= a.interval;
@@ -447,7 +448,9 @@ __device__ __forceinline__ void tile_force(const InferSmem &L, const float *__re
     fx += __shfl_xor(fx, 32, 64); fy += __shfl_xor(fy, 32, 64); fz += __shfl_xor(fz, 32, 64);
 }
 
-// after_training = 1: the second pass of an exact step, only when the weights changed
+// after_training = 0: this step's frame; 1: this frame again after training, 2: the
+// previous full-sum frame after training (into sg_prev, for its correction with the new
+// weights); 1 and 2 only on steps that trained
 __global__ __launch_bounds__(256) void k_sg_infer(MtsArgs a, int n, int after_training, const DevFlags *flags) {
     const MtsState &st = *a.st;
     if (!st.norm_ready) return;
@@ -463,13 +466,15 @@ __global__ __launch_bounds__(256) void k_sg_infer(MtsArgs a, int n, int after_tr
         const int s = tile * 16 + col;
         const bool valid = s < n;
         TileFwd T;
-        tile_forward(L, a.feat, s, valid, col, kg, T);
+        const bool prev = after_training == 2;
+        tile_forward(L, prev ? a.feat_prev : a.feat, s, valid, col, kg, T);
         float fx, fy, fz;
-        tile_force(L, a.vec, s, valid, kg, T.c, fx, fy, fz);
+        tile_force(L, prev ? a.vec_prev : a.vec, s, valid, kg, T.c, fx, fy, fz);
+        float *out = prev ? a.sg_prev : a.sg;
         if (kg == 0 && valid) {
-            a.sg[s] = fx;
-            a.sg[n + s] = fy;
-            a.sg[2 * (size_t)n + s] = fz;
+            out[s] = fx;
+            out[n + s] = fy;
+            out[2 * (size_t)n + s] = fz;
         }
     }
 }
@@ -835,7 +840,9 @@ __global__ __launch_bounds__(256) void k_sg_judge(DevSys S, MtsArgs a) {
             st.n_fallback++;
         }
     }
-    st.train_now = (st.state == 0 && a.model != 0 && st.norm_ready && st.have_prev && st.t2 > 0.0) ? 1 : 0;
+    // every full-sum step trains on the pair (this frame, the previous full-sum frame): every
+    // step while collecting, and online at each full sum of the scheme
+    st.train_now = (a.model != 0 && a.train_iters > 0 && st.norm_ready && st.have_prev && st.t2 > 0.0) ? 1 : 0;
 }
 
 // correction for the following steps, the path start, and this frame kept as the previous
@@ -851,7 +858,8 @@ __global__ __launch_bounds__(256) void k_sg_delta(DevSys S, MtsArgs a) {
     for (int c = 0; c < 3; c++) {
         const size_t k = (size_t)c * n + i;
         const double fr = a.frec[k];
-        a.delta_prev[k] = a.delta[k];
+        // the previous correction with the weights the next steps use
+        a.delta_prev[k] = (net && st.train_now) ? a.frec_prev[k] - (double)a.sg_prev[k] : a.delta[k];
         a.delta[k] = fr - (net ? (double)a.sg[k] : 0.0);
         a.rprev[k] = r[c];
         a.fprev[k] = fr;
@@ -878,8 +886,8 @@ __global__ void k_sg_post(DevSys S, MtsArgs a) {
             st.phase = 0;
         }
     }
-    // the slope needs two corrections from the same weights (no training at this step)
-    st.have_slope = st.have_delta && !st.train_now;
+    // two corrections (the previous one redone with new weights after training) give the slope
+    st.have_slope = st.have_delta;
     st.delta_prev_step = st.delta_step;
     st.delta_step = st.step;
     st.have_delta = 1;

@@ -145,8 +145,8 @@ class ForceEngine:
         return kv.reshape(-1, 3), ak, sf[0:2 * n.value:2] + 1j * sf[1:2 * n.value:2]
 
     # -- reciprocal-force surrogate with multiple time steps ------------------------------
-    def mts_enable(self, interval: int = 4, model: str = "network", train_frames: int = 200, train_iters: int = 1,
-                   tol: float = 1e-3, lr: float = 5e-4, seed: int = 1):
+    def mts_enable(self, interval: int = 4, model: str = "network", train_frames: int = 200, train_iters: int = 2,
+                   tol: float = 1e-3, lr: float = 1e-3, seed: int = 1):
         """Full k-sum every ``interval``-th step; in between the k-space force is the
         surrogate's (``model="network"``) or the last full sum's (``"held"``) plus the
         correction from the last full sum.  The first ``train_frames`` steps (and as many

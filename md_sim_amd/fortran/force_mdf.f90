@@ -249,11 +249,11 @@ module force
         mp%train_frames = 200
         v = env_value('MDF_MTS_TRAIN')
         if (len_trim(v) > 0) read(v, *, iostat=ios) mp%train_frames
-        mp%train_iters = 1
+        mp%train_iters = 2
         mp%tol = 1.0d-3
         v = env_value('MDF_MTS_TOL')
         if (len_trim(v) > 0) read(v, *, iostat=ios) mp%tol
-        mp%lr = 5.0e-4
+        mp%lr = 1.0e-3
         mp%seed = 1
         rc = mdf_mts_enable(sys, mp)
         if (rc /= MDF_OK) then

@@ -6,6 +6,10 @@ runs ``--steps`` force evaluations on device-resident positions with direct kern
 device-to-device copy of ``--copy-mib`` MiB.  The copy is the calibration for byte
 counters: it fetches and writes exactly that many bytes, so the ratio of its measured
 FETCH_SIZE / WRITE_SIZE to the true size scales the counters of the engine's kernels.
+
+``--mts N`` runs the k-space surrogate with the full sum every N-th step instead, on a
+librating water trajectory: ``--mts-train`` full-sum training steps, then ``--steps`` steps
+of the scheme.
 """
 from __future__ import annotations
 
@@ -20,6 +24,9 @@ def main(argv=None):
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--copy-mib", type=int, default=256)
+    ap.add_argument("--mts", type=int, default=0, help="full k-sum every N-th step with the surrogate between")
+    ap.add_argument("--mts-model", choices=["network", "held"], default="network")
+    ap.add_argument("--mts-train", type=int, default=8)
     a = ap.parse_args(argv)
     os.environ["MDF_GRAPHS"] = "0"
     import torch
@@ -31,11 +38,19 @@ def main(argv=None):
     prec = abi.FP32_ACC64 if a.precision == "fp32" else abi.FP64
     pos = abi.positions(s)
     eng = ForceEngine(s, precision=prec, virial=False)
-    dev = [torch.tensor(v[:s.nmove] if k < 3 else v, dtype=torch.float64, device="cuda:0")
-           for k, v in enumerate(pos)]
-    for _ in range(a.steps):
-        eng.compute_device(*[t.data_ptr() for t in dev])
+    if a.mts > 1:
+        frames = systems.libration_frames(s, a.mts_train + a.steps, seed=7)
+        eng.force_get(*frames[0])
+        eng.mts_enable(interval=a.mts, model=a.mts_model, train_frames=a.mts_train, tol=0.5)
+    else:
+        frames = [pos] * a.steps
+    devf = [[torch.tensor(v[:s.nmove] if k < 3 else v, dtype=torch.float64, device="cuda:0")
+             for k, v in enumerate(f)] for f in frames]
+    for d in devf:
+        eng.compute_device(*[t.data_ptr() for t in d])
     r = eng.read_outputs()
+    if a.mts > 1:
+        print("scheme: %s" % eng.mts_stats(), flush=True)
     print("steps done: e_conf %.6f" % r["e_conf"], flush=True)
     n = a.copy_mib * (1 << 20) // 4
     src = torch.ones(n, dtype=torch.float32, device="cuda:0")

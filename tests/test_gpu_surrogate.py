@@ -161,7 +161,7 @@ def test_network_gains_over_held_force():
     kw = dict(interval=4, train_frames=n_train, tol=0.2)
     held, st_h = _run(s, frames, model="held", **kw)
     prior, st_p = _run(s, frames, model="network", train_iters=0, **kw)
-    net, st_n = _run(s, frames, model="network", train_iters=1, lr=5e-4, **kw)
+    net, st_n = _run(s, frames, model="network", train_iters=2, lr=1e-3, **kw)
     for k, st in (("held", st_h), ("prior", st_p), ("network", st_n)):
         _say(k, st)
     steps = [t for t in range(n_train, n_steps) if (t - n_train + 1) % 4]   # steps without the full sum
@@ -274,3 +274,17 @@ def test_nve_drift_with_multiple_time_steps(tmp_path):
     assert abs(d["network/4 tol 1e-3"]) <= 2.0 * abs(d["fp64"]) + 1.0
     log = np.loadtxt(tmp_path / "mts.log", ndmin=2)
     assert log[-1, 4] >= 1          # the judge fell back at least once
+
+
+def test_training_only_on_full_sum_steps():
+    """Training happens on full-sum steps only, train_iters Adam steps each: every collected
+    frame from the second on, then every full sum of the scheme (online), none on the steps
+    in between (whose targets would be stale)."""
+    _say("case surrogate training count")
+    s = systems.config("slab512")
+    frames, _ = _frames_and_exact(s, 30, seed=11)
+    _, st = _run(s, frames, model="network", interval=4, train_frames=10, train_iters=2, tol=0.5)
+    # frames 0..9 collect (9 pairs), then full sums at steps 13, 17, 21, 25, 29 (5 pairs)
+    assert st["n_exact"] == 15 and st["n_surrogate"] == 15
+    assert st["adam_steps"] == 2 * (9 + 5)
+    assert st["loss_last"] < 1.0
