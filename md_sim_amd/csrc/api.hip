@@ -53,8 +53,8 @@ __global__ void k_sg_features(DevSys S, MtsArgs a);
 __global__ void k_sg_norm(MtsArgs a, int n);
 __global__ void k_sg_pack(DevSys S, MtsArgs a);
 __global__ void k_sg_infer(MtsArgs a, int n, int after_training, const DevFlags *flags);
-__global__ void k_sg_train(MtsArgs a, int n, const DevFlags *flags);
-__global__ void k_sg_adam(MtsArgs a, int nb, const DevFlags *flags);
+__global__ void k_sg_train(MtsArgs a, int n, int it, const DevFlags *flags);
+__global__ void k_sg_adam(MtsArgs a, int nb, int it, const DevFlags *flags);
 __global__ void k_sg_combine(DevSys S, MtsArgs a);
 __global__ void k_sg_finish(DevSys S, MtsArgs a, Scalars *scal);
 __global__ void k_sg_exact(DevSys S, MtsArgs a, const double *fpart, int n_groups);
@@ -254,7 +254,7 @@ struct mdf_system {
     double mts_tol = 1e-3;
     float mts_lr = 1e-3f;
     int mts_again = 0;                   // the step being launched redoes an overflowed one
-    int sg_n_sblk = 1, sg_n_tblk = 1, sg_n_iblk = 1;
+    int sg_n_sblk = 1, sg_n_tblk = 1, sg_n_iblk = 1, sg_train_stride = 1;
     DBuf<MtsState> mts_st;
     DBuf<float> sg_feat, sg_vec, sg_out, sg_params, sg_m, sg_v, sg_gsum, sg_norm, sg_gpart, sg_feat_prev,
         sg_vec_prev;
@@ -538,6 +538,7 @@ static MtsArgs mts_args(mdf_system *s) {
     a.n_sblk = s->sg_n_sblk;
     a.n_tblk = s->sg_n_tblk;
     a.train_iters = s->mts_train_iters;
+    a.train_stride = s->sg_train_stride;
     a.tol = s->mts_tol;
     a.lr = s->mts_lr;
     a.feat = s->sg_feat.p; a.vec = s->sg_vec.p; a.sg = s->sg_out.p; a.sg_prev = s->sg_out_prev.p;
@@ -657,8 +658,8 @@ static void launch_forces(mdf_system *s) {
         k_sg_judge<<<1, 256, 0, st>>>(S, ma);
         if (s->mts_model) {
             for (int it = 0; it < s->mts_train_iters; it++) {
-                k_sg_train<<<s->sg_n_tblk, 256, 0, st>>>(ma, nm, s->flags.p);
-                k_sg_adam<<<(SG_GP + 255) / 256, 256, 0, st>>>(ma, s->sg_n_tblk, s->flags.p);
+                k_sg_train<<<s->sg_n_tblk, 256, 0, st>>>(ma, nm, it, s->flags.p);
+                k_sg_adam<<<(SG_GP + 255) / 256, 256, 0, st>>>(ma, s->sg_n_tblk, it, s->flags.p);
             }
             k_sg_infer<<<s->sg_n_iblk, 256, 0, st>>>(ma, nm, 1, s->flags.p);
             k_sg_infer<<<s->sg_n_iblk, 256, 0, st>>>(ma, nm, 2, s->flags.p);
@@ -1524,9 +1525,14 @@ static void mts_geometry(int n, int &n_sblk, int &n_iblk, int &n_tblk) {
     n_tblk = std::min(SG_TRAIN_BLOCKS, std::max(1, (ntiles + 3) / 4));
 }
 
+// an Adam step of the scheme trains on about SG_TRAIN_TILES tiles of 16 sites
+constexpr int SG_TRAIN_TILES = 1024;
+static int mts_train_stride(int n) { return std::max(1, ((n + 15) / 16 + SG_TRAIN_TILES - 1) / SG_TRAIN_TILES); }
+
 static int mts_alloc(mdf_system *s) {
     const size_t nm = (size_t)std::max(s->top.n_move, 1);
     mts_geometry(s->top.n_move, s->sg_n_sblk, s->sg_n_iblk, s->sg_n_tblk);
+    s->sg_train_stride = mts_train_stride(s->top.n_move);
     HIPCHK(s->mts_st.alloc(1));
     HIPCHK(s->sg_feat.alloc(nm * SG_D));
     HIPCHK(s->sg_vec.alloc(nm * SG_M * 3));
@@ -1746,7 +1752,8 @@ int mdf_mts_eval(const float *weights, int32_t n, const float *feat, const float
     hs.have_prev = train ? 1 : 0;
     hs.train_now = train ? 1 : 0;
     hs.t2 = t2;
-    hs.adam_t = adam_t;
+    hs.adam_base = adam_t;
+    hs.adam_t = adam_t + (train ? 1 : 0);
     const size_t nn = (size_t)n;
 #define MTRY(x) do { if (e == hipSuccess) e = (x); } while (0)
     MTRY(st.alloc(1));
@@ -1773,14 +1780,15 @@ int mdf_mts_eval(const float *weights, int32_t n, const float *feat, const float
         MtsArgs a;
         memset(&a, 0, sizeof a);
         a.st = st.p; a.model = 1; a.n_sblk = n_sblk; a.n_tblk = n_tblk; a.lr = lr; a.train_iters = 1;
+        a.train_stride = 1;
         a.feat = d_feat.p; a.vec = d_vec.p; a.sg = d_sg.p; a.frec = d_frec.p;
         a.feat_prev = d_feat_b.p; a.vec_prev = d_vec_b.p; a.frec_prev = d_frec_b.p;
         a.params = d_params.p; a.adam_m = d_m.p; a.adam_v = d_v.p; a.gsum = d_gsum.p;
         a.wb = d_wb.p; a.norm = d_norm.p; a.gpart = d_gpart.p;
         k_sg_infer<<<n_iblk, 256, 0, stream>>>(a, n, 0, nullptr);
         if (train) {
-            k_sg_train<<<n_tblk, 256, 0, stream>>>(a, n, nullptr);
-            k_sg_adam<<<(SG_GP + 255) / 256, 256, 0, stream>>>(a, n_tblk, nullptr);
+            k_sg_train<<<n_tblk, 256, 0, stream>>>(a, n, 0, nullptr);
+            k_sg_adam<<<(SG_GP + 255) / 256, 256, 0, stream>>>(a, n_tblk, 0, nullptr);
         }
         MTRY(hipGetLastError());
         MTRY(hipStreamSynchronize(stream));

@@ -181,7 +181,7 @@ struct MtsState {
     int adam_t;       // Adam steps taken
     int have_prev;    // a previous full-sum frame is stored (training pairs)
     int have_slope;   // the last two corrections came from the same weights: extrapolate
-    int pad;
+    int adam_base;    // Adam steps taken before this step's training iterations
     long long step;                      // steps since the scheme was switched on
     long long delta_step, delta_prev_step;  // steps of the last two corrections
     long long n_exact, n_surrogate, n_fallback, n_judged;
@@ -194,6 +194,7 @@ struct MtsState {
 struct MtsArgs {
     MtsState *st;
     int interval, train_frames, model, n_sblk, n_tblk, train_iters;   // model 1: network, 0: held force
+    int train_stride, pad2;                    // an Adam step trains on one 16-site tile in train_stride
     double tol;
     float lr;
     float *feat, *vec, *sg;                    // [n][48], [n][16][3], [3][n]

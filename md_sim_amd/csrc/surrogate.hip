@@ -574,12 +574,15 @@ __device__ __forceinline__ void tile_backward(TrainSmem &M, int wave, const Tile
 
 // Frame a = (feat, vec, frec), frame b = (feat_prev, vec_prev, frec_prev):
 // e = (S_a - S_b) - (F_a - F_b); frame a back-propagates +2e/(n t2), frame b -2e/(n t2).
-__global__ __launch_bounds__(256) void k_sg_train(MtsArgs a, int n, const DevFlags *flags) {
-    MtsState &st = *a.st;
+// Iteration `it` of a step trains on the tiles t with t = adam_base + it (mod train_stride),
+// a different subset each Adam step (the gradient of a random-like subset of the frame).
+__global__ __launch_bounds__(256) void k_sg_train(MtsArgs a, int n, int it, const DevFlags *flags) {
+    const MtsState &st = *a.st;
     if (!st.train_now) return;
     if (flags && (flags->overflow_mm || flags->overflow_mw)) return;
     __shared__ TrainSmem M;
-    if (blockIdx.x == 0 && threadIdx.x == 0) st.adam_t += 1;   // read by the Adam launch that follows
+    const int stride = max(a.train_stride, 1);
+    const int phase = (st.adam_base + it) % stride;
     load_infer_smem(M.L, a);
     for (int k = threadIdx.x; k < SG_H * SG_H; k += blockDim.x) M.w2t[k] = a.wb[SG_WB_W2T + k];
     for (int k = threadIdx.x; k < SG_H * SG_M; k += blockDim.x) M.w3t[k] = a.wb[SG_WB_W3T + k];
@@ -589,7 +592,9 @@ __global__ __launch_bounds__(256) void k_sg_train(MtsArgs a, int n, const DevFla
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int col = lane & 15, kg = lane >> 4;
     const int ntiles = (n + 15) / 16;
-    const float gscale = (float)(2.0 / ((double)n * st.t2));
+    const int nsel = (ntiles - phase + stride - 1) / stride;   // tiles of this subset
+    const int nsites = min(n, 16 * nsel);
+    const float gscale = (float)(2.0 / ((double)nsites * st.t2));
     GradAcc G;
 #pragma unroll
     for (int p = 0; p < 4; p++) {
@@ -603,7 +608,8 @@ __global__ __launch_bounds__(256) void k_sg_train(MtsArgs a, int n, const DevFla
         for (int r = 0; r < 4; r++) G.b1[p][r] = G.b2[p][r] = 0.0f;
     }
     float lacc = 0.0f;
-    for (int tile = blockIdx.x * 4 + wave; tile < ntiles; tile += gridDim.x * 4) {
+    for (int q = blockIdx.x * 4 + wave; q < nsel; q += gridDim.x * 4) {
+        const int tile = phase + q * stride;
         const int s = tile * 16 + col;
         const bool valid = s < n;
         TileFwd T;
@@ -668,7 +674,7 @@ __global__ __launch_bounds__(256) void k_sg_train(MtsArgs a, int n, const DevFla
                     }
                 }
             }
-            if (lane == 0) M.gacc[SG_NP] += lacc * (float)(1.0 / ((double)n * st.t2));
+            if (lane == 0) M.gacc[SG_NP] += lacc * (float)(1.0 / ((double)nsites * st.t2));
         }
         __syncthreads();
     }
@@ -677,7 +683,7 @@ __global__ __launch_bounds__(256) void k_sg_train(MtsArgs a, int n, const DevFla
 
 // Sum of the block partials (fixed order), Adam step (bias-corrected moments) on the FP32
 // parameters, and fresh bfloat16 copies.  lr = 0: the gradient only (gsum), no update.
-__global__ __launch_bounds__(256) void k_sg_adam(MtsArgs a, int nb, const DevFlags *flags) {
+__global__ __launch_bounds__(256) void k_sg_adam(MtsArgs a, int nb, int it, const DevFlags *flags) {
     MtsState &st = *a.st;
     if (!st.train_now) return;
     if (flags && (flags->overflow_mm || flags->overflow_mw)) return;
@@ -691,7 +697,7 @@ __global__ __launch_bounds__(256) void k_sg_adam(MtsArgs a, int nb, const DevFla
         return;
     }
     if (!(a.lr > 0.0f)) return;
-    const int t = st.adam_t;
+    const int t = st.adam_base + it + 1;
     const float gf = (float)g;
     const float m = 0.9f * a.adam_m[p] + 0.1f * gf;
     const float v = 0.999f * a.adam_v[p] + 0.001f * gf * gf;
@@ -843,6 +849,8 @@ __global__ __launch_bounds__(256) void k_sg_judge(DevSys S, MtsArgs a) {
     // every full-sum step trains on the pair (this frame, the previous full-sum frame): every
     // step while collecting, and online at each full sum of the scheme
     st.train_now = (a.model != 0 && a.train_iters > 0 && st.norm_ready && st.have_prev && st.t2 > 0.0) ? 1 : 0;
+    st.adam_base = st.adam_t;
+    if (st.train_now) st.adam_t += a.train_iters;
 }
 
 // correction for the following steps, the path start, and this frame kept as the previous

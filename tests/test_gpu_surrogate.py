@@ -245,9 +245,9 @@ def _relaxed_slab512(n_iter=300):
 def test_nve_drift_with_multiple_time_steps(tmp_path):
     """The reference's own time loop (Gear predictor-corrector, quaternions) on the engine's
     drop-in force module, 1000 steps of 1 fs from a relaxed slab512: with the network and the
-    full sum every 2nd step the htot drift stays within twice the FP64 run's; every 4th step
-    it stays below the held force's; with the 1e-3 tolerance the judge falls back to full
-    sums and the drift is the full sum's."""
+    full sum every 2nd or every 4th step the htot drift stays within twice the FP64 run's and
+    below the held force's; with the 1e-3 tolerance the judge falls back to full sums and the
+    drift is the full sum's."""
     import os
     from golden.make_nve import drift
     from oracle import refrun
@@ -270,6 +270,7 @@ def test_nve_drift_with_multiple_time_steps(tmp_path):
         d[name] = drift(tr)
     _say("  htot drift over %d steps (kJ/mol): %s" % (n, {k: round(v, 3) for k, v in d.items()}))
     assert abs(d["network/2"]) <= 2.0 * abs(d["fp64"]) + 1.0
+    assert abs(d["network/4"]) <= 2.0 * abs(d["fp64"]) + 1.0
     assert abs(d["network/4"]) < abs(d["held/4"])
     assert abs(d["network/4 tol 1e-3"]) <= 2.0 * abs(d["fp64"]) + 1.0
     log = np.loadtxt(tmp_path / "mts.log", ndmin=2)
