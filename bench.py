@@ -21,7 +21,8 @@ over the stage's mean kernel time, measured with HIP events on the engine's stre
 second pass of the same steps run with per-stage events (the timed pass replays a captured
 HIP graph, inside which events cannot be timed).  ``cpu_baseline`` times the reference's
 own force path (built from its sources into oracle/_ref) on a few force evaluations of the
-same system on one core, or the CPU oracle when that build is absent.
+same system on the host's cores (MPI ranks, the reference's own decomposition; 16 by
+default) and on one core, or the CPU oracle when that build is absent.
 """
 from __future__ import annotations
 
@@ -48,7 +49,9 @@ def parse():
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp32",
                     help="fp32: FP32 pair and structure-factor arithmetic with FP64 accumulation (the "
                          "16k-water configuration's form; the FP64 form is timed beside it); fp64: FP64 only")
-    ap.add_argument("--cpu-reps", type=int, default=3, help="force evaluations in the CPU baseline sample")
+    ap.add_argument("--cpu-reps", type=int, default=3, help="force evaluations in the one-core CPU sample")
+    ap.add_argument("--cpu-cores", type=int, default=16,
+                    help="MPI ranks of the host-core CPU baseline (the reference's own decomposition)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--virial", action="store_true", help="also accumulate the pair virial")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
@@ -67,16 +70,20 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(system, reps: int):
-    """Reference force path on the host (one core): mean seconds per force evaluation."""
+def cpu_baseline(system, reps: int, cores: int = 1):
+    """Reference force path on the host: force evaluations per second, on `cores` MPI ranks
+    (the reference's own decomposition: leads and k-vectors over ranks, force_sum's
+    all-reduce), else on one core."""
     from oracle import refrun
     if refrun.ref_available():
-        d = refrun.run_reference(system, nrep=reps, timeout=1200)
+        nproc = cores if cores > 1 and refrun.mpi_available() else 1
+        d = refrun.run_reference(system, nrep=reps, nproc=nproc, timeout=1200)
         n, sec = d["_timing"]
-        return dict(value=n / sec, unit="steps/s", cores=1, kind="reference",
+        return dict(value=n / sec, unit="steps/s", cores=nproc, kind="reference",
                     sample="%d force evaluations (no list rebuild) of %s by the reference's own "
                            "force_real/force_fouri/force_intra/force_sum, built from its Fortran "
-                           "sources with amdflang -O2, one MPI rank" % (int(n), system.name))
+                           "sources with amdflang -O2, %d MPI rank%s" % (int(n), system.name, nproc,
+                                                                        "s" if nproc > 1 else ""))
     from md_sim_amd import abi
     from oracle import pyoracle
     o = pyoracle.Oracle(system)
@@ -361,12 +368,18 @@ def main():
         except Exception:
             traffic = None
 
-    cpu = None
+    cpu = cpu1 = None
     if not args.no_cpu_baseline:
+        cores = max(1, min(args.cpu_cores, os.cpu_count() or 1))
         try:
-            cpu = cpu_baseline(s, args.cpu_reps)
+            cpu = cpu_baseline(s, 4 * args.cpu_reps if cores > 1 else args.cpu_reps, cores)
         except Exception as e:  # the GPU figure stands on its own
             cpu = {"error": str(e)[:200]}
+        if cores > 1:
+            try:
+                cpu1 = cpu_baseline(s, args.cpu_reps, 1)
+            except Exception as e:
+                cpu1 = {"error": str(e)[:200]}
 
     prec_txt = "FP64" if args.precision == "fp64" else "FP32 arithmetic with FP64 accumulation"
     out = {
@@ -404,6 +417,7 @@ def main():
             "mean_ms": t_recip_ms,
         },
         "cpu_baseline": cpu,
+        "cpu_baseline_one_core": cpu1,
         "stages_ms": stages,
         "launch_shapes": shapes,
         "list_builds_timed": last["n_list_builds"] - first["n_list_builds"],

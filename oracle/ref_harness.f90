@@ -2,17 +2,19 @@
 ! Golden-vector harness for the reference force path (test infrastructure).
 !
 ! Replaces the reference's program unit (src/main.f90:19-108): it performs the
-! same set-up as `input`/`init` (src/main.f90:113-315) for one MPI rank, sets the
+! same set-up as `input`/`init` (src/main.f90:113-315), sets the
 ! site offsets the way move_predict does (src/move.f90:157-161), runs the pieces
 ! of one force_get (src/forcem.f90:93-129) and dumps inputs and outputs to a
 ! binary stream file.  Site forces are dumped before force_sum, which overwrites
-! fx(1:nrig) with COM sums (src/forcem.f90:1019-1024).
+! fx(1:nrig) with COM sums (src/forcem.f90:1019-1024).  On several MPI ranks it only times
+! repeated force evaluations (the reference's own decomposition of the force path).
 !
 ! Dump record: name (16 chars), kind (int64: 1 real*8, 2 int64), count (int64),
 ! data.  Built only in the development container, from the reference sources.
 !----------------------------------------------------------------------------
 program ref_harness
     use move
+    use comm
     implicit none
     include 'mpif.h'
     integer :: ierr, u
@@ -30,13 +32,15 @@ program ref_harness
     call mpi_init(ierr)
     call mpi_comm_size(mpi_comm_world, nproc, ierr)
     call mpi_comm_rank(mpi_comm_world, me, ierr)
-    if (nproc /= 1) stop 'ref_harness: run on one rank'
 
     call get_command_argument(1, outname)
     ! optional second argument: number of extra timed force evaluations
     nrep = 0
     call get_command_argument(2, arg2)
     if (len_trim(arg2) > 0) read(arg2, *) nrep
+    ! several ranks: the reference's own decomposition of the force path (leads and
+    ! k-vectors over ranks, force_sum's all-reduce), for timing only - the dump is one rank's
+    if (nproc /= 1 .and. nrep <= 0) stop 'ref_harness: several ranks only time (give nrep)'
     open(7, file='harness.inp', status='old')
     read(7, *) base
     read(7, *) dt, rcut, skin, rdel, kmax
@@ -65,7 +69,20 @@ program ref_harness
     call sysdef_input
     call sysdef_alloc
     call config_zero
-    call config_input
+    ! as src/main.f90:249-300: rank 0 reads the configuration, the others receive it
+    if (me == 0) call config_input
+    if (nproc > 1) then
+        call bcast_dbl(lx) ; call bcast_dbl(ly) ; call bcast_dbl(lz)
+        call bcast_int1d(itype)
+        call bcast_dbl1d(x0) ; call bcast_dbl1d(y0) ; call bcast_dbl1d(z0)
+        call bcast_dbl1d(cx0) ; call bcast_dbl1d(cy0) ; call bcast_dbl1d(cz0)
+        call bcast_dbl1d(wx0) ; call bcast_dbl1d(wy0) ; call bcast_dbl1d(wz0)
+        call bcast_dbl1d(q10) ; call bcast_dbl1d(q20) ; call bcast_dbl1d(q30) ; call bcast_dbl1d(q40)
+        call bcast_dbl1d(a11) ; call bcast_dbl1d(a12) ; call bcast_dbl1d(a13)
+        call bcast_dbl1d(a21) ; call bcast_dbl1d(a22) ; call bcast_dbl1d(a23)
+        call bcast_dbl1d(a31) ; call bcast_dbl1d(a32) ; call bcast_dbl1d(a33)
+        call bcast_dbl1d(rsx) ; call bcast_dbl1d(rsy) ; call bcast_dbl1d(rsz)
+    end if
     call sysdef_init
     call config_init
     call force_init
@@ -123,6 +140,7 @@ program ref_harness
 
     call force_sum
 
+    if (nproc == 1) then
     u = 11
     open(u, file=trim(outname), access='stream', form='unformatted', status='replace')
     allocate(ibuf(max(ntota, int(ipack,8), int(wpack,8), nit*nit, nijt, 10_8)))
@@ -241,6 +259,7 @@ program ref_harness
     call put_r(u, 'geometry', cbuf, 12_8)
     deallocate(cbuf)
     close(u)
+    end if
 
     ! timing: repeated force evaluations on the same configuration (no list rebuild)
     if (nrep > 0) then
@@ -255,7 +274,7 @@ program ref_harness
             call force_sum
         end do
         call system_clock(c1)
-        write(*, '(a,i8,a,es16.8)') 'TIMING nrep=', nrep, ' seconds=', dble(c1 - c0)/dble(crate)
+        if (me == 0) write(*, '(a,i8,a,es16.8)') 'TIMING nrep=', nrep, ' seconds=', dble(c1 - c0)/dble(crate)
     end if
 
     call mpi_finalize(ierr)

@@ -34,29 +34,45 @@ def parse_dump(path: str) -> Dict[str, np.ndarray]:
     return out
 
 
-def run_reference(system, workdir: str = None, timeout: float = 600.0, nrep: int = 0) -> Dict[str, np.ndarray]:
+MPIEXEC = os.environ.get("MDF_MPIEXEC", "/opt/conda/bin/mpiexec")
+
+
+def mpi_available() -> bool:
+    return os.access(MPIEXEC, os.X_OK)
+
+
+def run_reference(system, workdir: str = None, timeout: float = 600.0, nrep: int = 0,
+                  nproc: int = 1) -> Dict[str, np.ndarray]:
     """Write the reference's input files for `system`, run the matching harness variant
-    (bulk = 3-D periodic, slab otherwise) and return the parsed dump."""
+    (bulk = 3-D periodic, slab otherwise) and return the parsed dump.  nproc > 1 runs the
+    harness on that many MPI ranks (the reference's own decomposition of the force path)
+    for timing only: the result then holds the timing and no dump."""
     from md_sim_amd.refio import write_reference_inputs
 
     variant = "bulk" if system.z_periodic else "slab"
     exe = os.path.join(REF_DIR, "ref_" + variant)
     if not os.access(exe, os.X_OK):
         raise FileNotFoundError(exe)
+    if nproc > 1 and nrep <= 0:
+        raise ValueError("several ranks only time: give nrep")
     own = workdir is None
     d = tempfile.mkdtemp(prefix="mdref_") if own else workdir
     write_reference_inputs(system, d)
     out = os.path.join(d, "dump.bin")
     args = [exe, out] + ([str(nrep)] if nrep > 0 else [])
+    if nproc > 1:
+        args = [MPIEXEC, "-n", str(int(nproc))] + args
     r = subprocess.run(args, cwd=d, capture_output=True, text=True, timeout=timeout)
-    if r.returncode != 0 or not os.path.exists(out):
+    if r.returncode != 0 or (nproc == 1 and not os.path.exists(out)):
         raise RuntimeError("reference harness failed (%d):\n%s\n%s" % (r.returncode, r.stdout[-2000:], r.stderr[-2000:]))
-    dump = parse_dump(out)
+    dump = parse_dump(out) if nproc == 1 else {}
     dump["_stdout"] = np.array(r.stdout)
     for line in r.stdout.splitlines():
         m = re.match(r"\s*TIMING nrep=\s*(\d+)\s+seconds=\s*(\S+)", line)
         if m:
             dump["_timing"] = np.array([float(m.group(1)), float(m.group(2))])
+    if own:
+        shutil.rmtree(d, ignore_errors=True)
     return dump
 
 
