@@ -230,6 +230,21 @@ int mdf_autotune(mdf_system *sys, int32_t reps, int32_t *chosen);
 int mdf_set_launch_shapes(mdf_system *sys, int32_t pair_lanes, int32_t pair_rolled, int32_t sf_blocks,
                           int32_t bp_blocks);
 
+/* ---- observables of the reference's hot loops (src/forcem.f90:347-354,388-395,486-495,
+   930-934; arrays of src/measure.f90:18-22,80-86) -------------------------------------
+   Accumulated on the device over the steps since they were switched on (or last reset):
+   ngc, the site-pair distance histogram per type pair, bin int(r / rdel + 0.5), as int64
+   [nbins][n_pairs] (the memory of the reference's ngc(nijt, 0:lmax)), over every site of a
+   lead's molecule against each frozen site inside the cut-off and, with a periodic z mode,
+   also every site pair of the moving molecule pairs inside the cut-off
+   (src/forcem_back.f90:348-349,380-382, counts the slab form leaves commented out at
+   src/forcem.f90:357,399); efipos, the reference's z-force profile of
+   the mobile ions of types 3 and 4, bin int((z + lw/2) / rdel + 0.5), as double [nbins][2]
+   (efipos(1:2, 0:lmax)).  nbins = int(lw / rdel + 0.5) + 2.  rdel <= 0 switches them off. */
+int mdf_observables_enable(mdf_system *sys, double rdel);
+/* Copy the accumulated observables (either pointer may be NULL); reset non-zero zeroes them. */
+int mdf_get_observables(mdf_system *sys, int64_t *ngc, double *efipos, int32_t *nbins, int32_t reset);
+
 /* ---- reciprocal-force surrogate with multiple time steps (no counterpart in the reference,
    whose force_get evaluates the full k-sum every step, src/forcem.f90:827-918) ----------
    Between full k-sums the k-space site force is predicted by a small per-site network of

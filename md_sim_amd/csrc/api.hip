@@ -65,7 +65,9 @@ constexpr int SG_FEAT_G = 16;  // lanes per lead in the input kernel
 constexpr int BP_NS = 4;   // sites per back-projection thread
 __global__ void k_epilogue(DevSys S, const double *fpart, int n_groups, const WallLR *wl, int ew3dc, double crf0,
                            const double *dipz, double *epart, double *fcom, double *tq, const MtsState *mts,
-                           const double *fmts);
+                           const double *fmts, double *efi);
+__global__ void k_obs_ngc(DevSys S, ObsArgs o);
+__global__ void k_obs_efipos(DevSys S, ObsArgs o);
 __global__ void k_colsum(DevSys S, const int *lead_slot, int per_block, double *colpart);
 __global__ void k_pair_totals(DevSys S);
 __global__ void k_final(DevSys S, FinalArgs A, const double *colpart, const double *kpart, const double *eppart,
@@ -262,6 +264,11 @@ struct mdf_system {
     DBuf<double> sg_delta, sg_frec, sg_fmts, sg_rprev, sg_fprev, sg_jpart, sg_epart, sg_frec_prev, sg_delta_prev;
     DBuf<float4> sg_pos4;
     DBuf<float> sg_out_prev;
+    // observables of the reference's hot loops (off unless mdf_observables_enable)
+    bool obs_on = false;
+    ObsArgs obs;
+    DBuf<unsigned long long> obs_ngc;
+    DBuf<double> obs_efi, obs_efipos;
     DBuf<int> sg_msize;
 };
 
@@ -294,6 +301,8 @@ static void free_all(mdf_system *s) {
     s->flags.release(); s->d_bp_row_start.release(); s->d_chunks.release(); s->d_rows.release(); s->d_swall.release(); s->d_s.release();
     s->d_t.release(); s->sf_partial.release(); s->d_wl.release(); s->d_scal.release();
     mts_free(s);
+    s->obs_ngc.release(); s->obs_efi.release(); s->obs_efipos.release();
+    s->obs_on = false;
 }
 
 template <typename T>
@@ -642,7 +651,11 @@ static void launch_forces(mdf_system *s) {
     const int n_groups = (nk > 0 && nm > 0) ? s->n_rg : 0;
     k_epilogue<<<s->n_epblk, EPI_BLOCK, 0, st>>>(S, s->fpart.p, n_groups, s->d_wl.p, s->par.ew3dc, s->crf0,
                                                  s->dipz.p, s->eppart.p, s->fcom.p, s->torque.p, mts_state(s),
-                                                 s->sg_fmts.p);
+                                                 s->sg_fmts.p, s->obs_on ? s->obs_efi.p : nullptr);
+    if (s->obs_on) {
+        k_obs_ngc<<<std::max(1, (s->n_lead + 15) / 16), 256, 0, st>>>(S, s->obs);
+        k_obs_efipos<<<1, 64, 0, st>>>(S, s->obs);
+    }
     if (s->ev) (void)hipEventRecord(s->ev[7], st);
     {
         dim3 gc(NCOL, s->n_colblk);
@@ -1651,6 +1664,52 @@ int mdf_mts_enable(mdf_system *s, const mdf_mts_params *p) {
     return MDF_OK;
 }
 
+int mdf_observables_enable(mdf_system *s, double rdel) {
+    if (!s) return fail(MDF_ERR_ARG, "null handle");
+    HIPCHK(hipSetDevice(s->device));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    drain_all(s);
+    drop_graphs(s);
+    s->obs_ngc.release(); s->obs_efi.release(); s->obs_efipos.release();
+    s->obs_on = false;
+    if (!(rdel > 0.0)) return MDF_OK;
+    // bins 0..lmax, lmax = int(lw rdeli + 0.5) + 1 (src/measure.f90:72,80)
+    const double rdeli = 1.0 / rdel;
+    const int nbins = (int)(s->top.wall_lw * rdeli + 0.5) + 2;
+    if (nbins < 1 || nbins > (1 << 24)) return fail(MDF_ERR_ARG, "rdel %g gives %d bins", rdel, nbins);
+    HIPCHK(s->obs_ngc.alloc((size_t)nbins * std::max(s->nijt, 1)));
+    HIPCHK(s->obs_efipos.alloc(2 * (size_t)nbins));
+    HIPCHK(s->obs_efi.alloc((size_t)std::max(s->top.n_move, 1)));
+    HIPCHK(hipMemset(s->obs_ngc.p, 0, sizeof(unsigned long long) * s->obs_ngc.n));
+    HIPCHK(hipMemset(s->obs_efipos.p, 0, sizeof(double) * s->obs_efipos.n));
+    HIPCHK(hipMemset(s->obs_efi.p, 0, sizeof(double) * s->obs_efi.n));
+    memset(&s->obs, 0, sizeof s->obs);
+    s->obs.ngc = s->obs_ngc.p;
+    s->obs.efi = s->obs_efi.p;
+    s->obs.efipos = s->obs_efipos.p;
+    s->obs.rdeli = rdeli;
+    s->obs.lwh = 0.5 * s->top.wall_lw;
+    s->obs.nbins = nbins;
+    s->obs.count_mm = s->S.zper_force ? 1 : 0;
+    s->obs_on = true;
+    return MDF_OK;
+}
+
+int mdf_get_observables(mdf_system *s, int64_t *ngc, double *efipos, int32_t *nbins, int32_t reset) {
+    if (!s) return fail(MDF_ERR_ARG, "null handle");
+    if (!s->obs_on) return fail(MDF_ERR_ARG, "observables are off");
+    HIPCHK(hipSetDevice(s->device));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    if (nbins) *nbins = s->obs.nbins;
+    if (ngc) HIPCHK(hipMemcpy(ngc, s->obs_ngc.p, sizeof(int64_t) * s->obs_ngc.n, hipMemcpyDeviceToHost));
+    if (efipos) HIPCHK(hipMemcpy(efipos, s->obs_efipos.p, sizeof(double) * s->obs_efipos.n, hipMemcpyDeviceToHost));
+    if (reset) {
+        HIPCHK(hipMemset(s->obs_ngc.p, 0, sizeof(unsigned long long) * s->obs_ngc.n));
+        HIPCHK(hipMemset(s->obs_efipos.p, 0, sizeof(double) * s->obs_efipos.n));
+    }
+    return MDF_OK;
+}
+
 int mdf_mts_get_stats(mdf_system *s, mdf_mts_stats *out) {
     if (!s || !out) return fail(MDF_ERR_ARG, "null argument");
     if (!s->mts_on) return fail(MDF_ERR_ARG, "multiple time steps are off");
@@ -2001,8 +2060,10 @@ int mdf_batch_create(mdf_system *const *systems, int32_t n, mdf_batch **out) {
         for (int j = 0; j < i; j++)
             if (systems[j] == s) return fail(MDF_ERR_ARG, "system %d appears twice", i);
     }
-    for (int i = 0; i < n; i++)
+    for (int i = 0; i < n; i++) {
         if (systems[i]->mts_on) return fail(MDF_ERR_ARG, "system %d has multiple time steps on (not in batches)", i);
+        if (systems[i]->obs_on) return fail(MDF_ERR_ARG, "system %d accumulates observables (not in batches)", i);
+    }
     mdf_batch *b = new mdf_batch;
     b->sys.assign(systems, systems + n);
     b->device = s0->device;

@@ -54,7 +54,8 @@ __device__ __forceinline__ void epilogue_body(const DevSys &S, const double *__r
                                                         const double *__restrict__ dipz, double *__restrict__ epart,
                                                         double *__restrict__ fcom, double *__restrict__ tq, int bid, int nblk,
                                                         const MtsState *__restrict__ mts = nullptr,
-                                                        const double *__restrict__ fmts = nullptr) {
+                                                        const double *__restrict__ fmts = nullptr,
+                                                        double *__restrict__ efi = nullptr) {
     __shared__ double sf[3][EPI_BLOCK];
     const bool surrogate = mts != nullptr && !mts->exact;
     const double *__restrict__ fpart = surrogate ? fmts : fpart_k;
@@ -66,6 +67,7 @@ __device__ __forceinline__ void epilogue_body(const DevSys &S, const double *__r
     sf[0][threadIdx.x] = 0.0; sf[1][threadIdx.x] = 0.0; sf[2][threadIdx.x] = 0.0;
     if (i < n) {
         double fx = S.fx[i], fy = S.fy[i], fz = S.fz[i];
+        const double fz_pair = fz;   // the pair kernel's part (loops A and B)
         // wall long-range tail correction, sites with |q| >= 0.7
         if (wl->on) {
             const int it = S.itype[i];
@@ -102,6 +104,13 @@ __device__ __forceinline__ void epilogue_body(const DevSys &S, const double *__r
         if (ew3dc) {  // crf = 2 crf0 facq dipz; fz += crf q  (src/forcem.f90:927-928)
             const double crf = 2.0 * crf0 * S.facq * *dipz;
             fz += crf * S.q[i];
+        }
+        // efipos of the reference: each pair's z-force on a mobile ion of type 3 or 4 in loops A
+        // and B, then the ion's force after the k-sum and the slab correction
+        // (src/forcem.f90:347-354,388-395,486-489,930-934); angles come later there
+        if (efi) {
+            const int it = S.itype[i];
+            efi[i] = (i >= S.n_rigid_sites && i < S.n_liquid_sites && (it == 3 || it == 4)) ? fz_pair + fz : 0.0;
         }
         if (S.n_angles > 0)
             for (int p = S.ang_ptr[i]; p < S.ang_ptr[i + 1]; p++) {
@@ -148,8 +157,96 @@ __global__ __launch_bounds__(EPI_BLOCK) void k_epilogue(DevSys S, const double *
                                                         const WallLR *__restrict__ wl, int ew3dc, double crf0,
                                                         const double *__restrict__ dipz, double *__restrict__ epart,
                                                         double *__restrict__ fcom, double *__restrict__ tq,
-                                                        const MtsState *__restrict__ mts, const double *__restrict__ fmts) {
-    epilogue_body(S, fpart, n_groups, wl, ew3dc, crf0, dipz, epart, fcom, tq, blockIdx.x, gridDim.x, mts, fmts);
+                                                        const MtsState *__restrict__ mts, const double *__restrict__ fmts,
+                                                        double *__restrict__ efi) {
+    epilogue_body(S, fpart, n_groups, wl, ew3dc, crf0, dipz, epart, fcom, tq, blockIdx.x, gridDim.x, mts, fmts, efi);
+}
+
+// ngc of the reference's pair loops, binned by site-pair distance per type pair with the
+// reference's operation order and no contraction.  Integer counts: exact whatever the order
+// of the atomic additions.
+//  - moving-frozen (every form): every site of a lead's molecule against each frozen site of
+//    its row inside the cut-off (src/forcem.f90:418-425,459-466,493-495;
+//    src/forcem_back.f90:405-412,443-449,472-474);
+//  - moving-moving (3-D form only, count_mm): every site pair of a molecule pair i<j inside
+//    the cut-off, lead pair included, at (xio + rs(ix)) - rs(jx) with rs the site offset from
+//    its lead (src/forcem_back.f90:288-298,348-349,353-382).  The slab form has these counts
+//    commented out (src/forcem.f90:357,399).
+__global__ __launch_bounds__(256) void k_obs_ngc(DevSys S, ObsArgs o) {
+    const int lane = threadIdx.x & 15;
+    const int gid = blockIdx.x * 16 + (threadIdx.x >> 4);
+    if (gid >= S.n_lead) return;
+    const int i = S.lead[gid];
+    const int ti = S.itype[i], nsi_i = S.nsi[ti];
+    const double xi = S.x[i], yi = S.y[i], zi = S.z[i];
+    const int nt1 = S.n_types + 1;
+    if (o.count_mm) {
+        const int nm = min(S.cnt_mm[gid], S.cap_mm);
+        const int *mrow = S.nbr_mm + (size_t)gid * S.cap_mm;
+        for (int k = lane; k < nm; k += 16) {
+            const int j = mrow[k];
+            if (j <= i) continue;   // each pair once, from the row of its lower index (list1 < list2)
+            double dx = __dsub_rn(xi, S.x[j]); dx = __dsub_rn(dx, __dmul_rn(S.lx, anint_d(__dmul_rn(dx, S.lxi))));
+            double dy = __dsub_rn(yi, S.y[j]); dy = __dsub_rn(dy, __dmul_rn(S.ly, anint_d(__dmul_rn(dy, S.lyi))));
+            double dz = __dsub_rn(zi, S.z[j]);
+            if (S.zper_force) dz = __dsub_rn(dz, __dmul_rn(S.lz, anint_d(__dmul_rn(dz, S.lzi))));
+            const double rsq0 = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
+            if (rsq0 > S.rcutsq) continue;
+            const int nsi_j = S.nsi[S.itype[j]];
+            const double xj = S.x[j], yj = S.y[j], zj = S.z[j];
+            for (int a = 0; a < nsi_i && a < 3; a++) {
+                const int ix = i + a;
+                const double xa = __dadd_rn(dx, __dsub_rn(S.x[ix], xi));
+                const double ya = __dadd_rn(dy, __dsub_rn(S.y[ix], yi));
+                const double za = __dadd_rn(dz, __dsub_rn(S.z[ix], zi));
+                const int ta = S.itype[ix];
+                for (int b = 0; b < nsi_j && b < 3; b++) {
+                    const int jx = j + b;
+                    const double xd = __dsub_rn(xa, __dsub_rn(S.x[jx], xj));
+                    const double yd = __dsub_rn(ya, __dsub_rn(S.y[jx], yj));
+                    const double zd = __dsub_rn(za, __dsub_rn(S.z[jx], zj));
+                    const double rsq = __dadd_rn(__dadd_rn(__dmul_rn(xd, xd), __dmul_rn(yd, yd)), __dmul_rn(zd, zd));
+                    const int l = (int)__dadd_rn(__dmul_rn(sqrt(rsq), o.rdeli), 0.5);
+                    const int ii = S.ijtype[ta * nt1 + S.itype[jx]];
+                    if (l >= 0 && l < o.nbins && ii >= 1) atomicAdd(&o.ngc[(size_t)l * S.nijt + (ii - 1)], 1ull);
+                }
+            }
+        }
+    }
+    const int nw = min(S.cnt_mw[gid], S.cap_mw);
+    const int *wrow = S.nbr_mw + (size_t)gid * S.cap_mw;
+    for (int k = lane; k < nw; k += 16) {
+        const int j = wrow[k];
+        double dx = __dsub_rn(xi, S.x[j]); dx = __dsub_rn(dx, __dmul_rn(S.lx, anint_d(__dmul_rn(dx, S.lxi))));
+        double dy = __dsub_rn(yi, S.y[j]); dy = __dsub_rn(dy, __dmul_rn(S.ly, anint_d(__dmul_rn(dy, S.lyi))));
+        double dz = __dsub_rn(zi, S.z[j]);
+        if (S.zper_force) dz = __dsub_rn(dz, __dmul_rn(S.lz, anint_d(__dmul_rn(dz, S.lzi))));
+        const double rsq0 = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
+        if (rsq0 > S.rcutsq) continue;
+        const int tj = S.itype[j];
+        for (int a = 0; a < nsi_i && a < 3; a++) {
+            const int ix = i + a;
+            const double xd = __dadd_rn(dx, __dsub_rn(S.x[ix], xi));
+            const double yd = __dadd_rn(dy, __dsub_rn(S.y[ix], yi));
+            const double zd = __dadd_rn(dz, __dsub_rn(S.z[ix], zi));
+            const double rsq = __dadd_rn(__dadd_rn(__dmul_rn(xd, xd), __dmul_rn(yd, yd)), __dmul_rn(zd, zd));
+            const double rij = sqrt(rsq);
+            const int l = (int)__dadd_rn(__dmul_rn(rij, o.rdeli), 0.5);
+            const int ii = S.ijtype[S.itype[ix] * nt1 + tj];
+            if (l >= 0 && l < o.nbins && ii >= 1) atomicAdd(&o.ngc[(size_t)l * S.nijt + (ii - 1)], 1ull);
+        }
+    }
+}
+
+// efipos bins of the step's ion z-forces, in site order (one thread: a fixed summation order)
+__global__ void k_obs_efipos(DevSys S, ObsArgs o) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int i = S.n_rigid_sites; i < S.n_liquid_sites; i++) {
+        const int it = S.itype[i];
+        if (it != 3 && it != 4) continue;
+        const int l = (int)__dadd_rn(__dmul_rn(__dadd_rn(S.z[i], o.lwh), o.rdeli), 0.5);
+        if (l >= 0 && l < o.nbins) o.efipos[2 * l + (it - 3)] += o.efi[i];
+    }
 }
 
 // column sums of the per-lead partials, split by the lead's type slot

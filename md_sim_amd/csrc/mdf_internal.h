@@ -213,6 +213,16 @@ struct MtsArgs {
     double *epart;                             // [n_sblk] energy path partials
 };
 
+// ---- observables of the reference's hot loops (measurement only) -----------------------
+struct ObsArgs {
+    unsigned long long *ngc;   // [nbins][nijt] site-pair distance histogram of the pair loops
+    double *efi;               // [n_move] this step's z-force of each mobile ion of types 3, 4
+    double *efipos;            // [nbins][2] accumulated z-force profile of those ions
+    double rdeli, lwh;
+    int nbins;
+    int count_mm;              // 1: moving-moving site pairs count too (3-D pair loops)
+};
+
 // ---- batches of independent systems ----------------------------------------------------
 // One entry per system: everything its launches take.  A batched kernel covers the blocks of
 // every system in one launch; a block map entry (system, local block) tells each block whose

@@ -55,7 +55,8 @@ def lib() -> C.CDLL:
                    "mdf_zero", "mdf_reset_stage_times", "mdf_set_stage_timing", "mdf_autotune",
                    "mdf_set_launch_shapes", "mdf_batch_create", "mdf_batch_compute_device",
                    "mdf_batch_read_outputs", "mdf_batch_autotune", "mdf_mts_enable", "mdf_mts_get_stats",
-                   "mdf_mts_get_weights", "mdf_mts_set_weights", "mdf_mts_features", "mdf_mts_eval"):
+                   "mdf_mts_get_weights", "mdf_mts_set_weights", "mdf_mts_features", "mdf_mts_eval",
+                   "mdf_observables_enable", "mdf_get_observables"):
             getattr(L, fn).restype = C.c_int
         L.mdf_destroy.restype = None
         L.mdf_batch_destroy.restype = None
@@ -143,6 +144,22 @@ class ForceEngine:
         sf = np.zeros(2 * max(n.value, 1))
         _check(lib().mdf_get_structure_factor(self._h, _dp(sf), n.value))
         return kv.reshape(-1, 3), ak, sf[0:2 * n.value:2] + 1j * sf[1:2 * n.value:2]
+
+    # -- observables of the reference's hot loops ---------------------------------------
+    def observables_enable(self, rdel: float):
+        """Accumulate ngc and efipos (the reference's measure arrays) from the next step on,
+        with bin width rdel; rdel <= 0 switches them off."""
+        _check(lib().mdf_observables_enable(self._h, C.c_double(rdel)))
+
+    def observables(self, reset: bool = False):
+        """(ngc [nbins, n_pairs] int64, efipos [nbins, 2]) accumulated so far."""
+        nb = C.c_int32()
+        _check(lib().mdf_get_observables(self._h, None, None, C.byref(nb), 0))
+        ngc = np.zeros((nb.value, self.n_pairs), np.int64)
+        efp = np.zeros((nb.value, 2))
+        _check(lib().mdf_get_observables(self._h, ngc.ctypes.data_as(C.POINTER(C.c_int64)), _dp(efp), C.byref(nb),
+                                         int(bool(reset))))
+        return ngc, efp
 
     # -- reciprocal-force surrogate with multiple time steps ------------------------------
     def mts_enable(self, interval: int = 4, model: str = "network", train_frames: int = 200, train_iters: int = 2,

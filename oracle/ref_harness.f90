@@ -14,6 +14,7 @@
 !----------------------------------------------------------------------------
 program ref_harness
     use move
+    use measure
     use comm
     implicit none
     include 'mpif.h'
@@ -97,7 +98,9 @@ program ref_harness
         rsz(ix+2:ix+3) = z0(ix+2:ix+3) - z0(ix+1)
     end do
 
-    ! accumulators as force_get zeroes them
+    ! accumulators as force_get zeroes them; the hot-loop observables as measure_init does
+    ngc = 0
+    efipos = 0.0
     fx = 0.0 ; fy = 0.0 ; fz = 0.0
     fwsx = 0.0 ; fwsy = 0.0 ; fwsz = 0.0
     vir = 0.0 ; elj = 0.0 ; eqq = 0.0 ; uewa = 0.0
@@ -177,6 +180,10 @@ program ref_harness
     call put_r(u, 'tz', tz, nrig)
     call put_r(u, 'elj', elj, nijt)
     call put_r(u, 'eqq', eqq, nijt)
+    ! observables of the hot loops (src/forcem.f90:347-354,388-395,486-495,930-934):
+    ! ngc(nijt, 0:lmax) and efipos(2, 0:lmax), column-major
+    call put_i(u, 'ngc', ngc, int(size(ngc), 8))
+    call put_r(u, 'efipos', efipos, int(size(efipos), 8))
     call put_r(u, 'k_ak', kak, nsel)
     call put_r(u, 'k_sre', kre, nsel)
     call put_r(u, 'k_sim', kim, nsel)

@@ -94,6 +94,11 @@ def main():
         out = {k: d[k] for k in SCALARS}
         for k in ("elj", "eqq", "fwsx", "fwsy", "fwsz", "kmaxes", "counts"):
             out[k] = d[k]
+        if name != "slab2048":
+            # observables of the hot loops: ngc(nijt, 0:lmax), efipos(2, 0:lmax), rdel
+            out["ngc"] = d["ngc"].reshape(-1, s.nijt)
+            out["efipos"] = d["efipos"].reshape(-1, 2)
+            out["rdel"] = np.array(s.rdel)
         out["list_digest"] = np.array(pair_digest(d["list1"], d["list2"]))
         out["wlist_digest"] = np.array(pair_digest(d["listw1"], d["listw2"]))
         if name == "slab2048":

@@ -129,3 +129,32 @@ def test_engine_thin_crystal_wall_pairs(oracle_lib):
     m1, m2, u1, u2 = o.pairs()
     assert np.array_equal(w1, u1) and np.array_equal(w2, u2)
     assert rel_err(r["f_site"], ro["f_site"]) < TOL
+
+
+@pytest.mark.parametrize("name", ["pore_small", "bulk216"])
+def test_observables_match_reference_golden(name):
+    """ngc and efipos, the reference's hot-loop observables (src/forcem.f90:347-354,388-395,
+    486-495,930-934), against its own dump for the golden configuration: the distance
+    histogram bit for bit, the ions' z-force profile at 1e-10; two steps accumulate twice."""
+    from md_sim_amd.engine import ForceEngine
+    _say("case observables", name)
+    g = golden(name)
+    s = systems.config(name)
+    e = ForceEngine(s)
+    e.observables_enable(float(g["rdel"]))
+    pos = golden_positions(g)
+    e.force_get(*pos)
+    ngc, efp = e.observables()
+    assert ngc.shape == g["ngc"].shape and efp.shape == g["efipos"].shape
+    assert np.array_equal(ngc, g["ngc"])
+    scale = max(np.abs(g["efipos"]).max(), 1e-30)
+    assert np.abs(efp - g["efipos"]).max() <= 1e-10 * scale
+    if name == "pore_small":
+        assert g["ngc"].sum() > 0 and np.abs(g["efipos"]).max() > 0
+    e.force_get(*pos)
+    ngc2, efp2 = e.observables(reset=True)
+    assert np.array_equal(ngc2, 2 * g["ngc"])
+    assert np.abs(efp2 - 2 * g["efipos"]).max() <= 2e-10 * scale
+    ngc3, _ = e.observables()
+    assert ngc3.sum() == 0
+    e.close()
