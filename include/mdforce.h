@@ -245,6 +245,43 @@ int mdf_observables_enable(mdf_system *sys, double rdel);
 /* Copy the accumulated observables (either pointer may be NULL); reset non-zero zeroes them. */
 int mdf_get_observables(mdf_system *sys, int64_t *ngc, double *efipos, int32_t *nbins, int32_t reset);
 
+/* ---- Widom test-particle insertions: the reference's chempot (src/forcem.f90:1226-1507,
+   called at the end of force_get when lchempot is set, :126; inputs src/main.f90:148-152,
+   set-up src/measure.f90:149-166, output src/measure.f90:700-705) --------------------------
+   After every step, up to n_insert SPC/E waters are inserted with the reference's generator
+   (src/rndmod.f90:28-35): oxygen uniform in x, y and at a distance from the pore center in
+   (lwh - wall_max, lwh - wall_min); 20 orientations in bins above bin_mid, 1 in the others.
+   An insertion whose oxygen comes within 2.35 A of an oxygen or 2 A of another non-hydrogen
+   site ends the step's insertions, as in the reference.  pave(bin) accumulates
+   exp(-u / RT) over the orientations and inave(bin) their number; the excess chemical
+   potential is -RT ln(sum pave / sum inave) (src/measure.f90:701).  The insertion energy
+   is the reference's: the oxygen's vdW and real-space Coulomb with every non-hydrogen site
+   within rcut and whole partner molecules for the hydrogens, 0.5 sum A_k Re(rho_ins conj S)
+   with this step's S(k), the wall tail terms of type 1, and the slab term below.
+   ew3dc: the reference's chempot multiplies a local `crf` that nothing assigns
+   (src/forcem.f90:1236,1477); 1 uses the step's slab-correction coefficient
+   2 crf0 facq sum(q z) (the module variable it shadows, :925-927), 0 leaves the term out.
+   Runs in FP64 on the handle's stream after each step (not in batches, not with the
+   k-space surrogate).  The host draws each step's geometry, so a step with insertions
+   waits for the previous step's insertions to finish. */
+typedef struct {
+    int32_t n_insert;        /* ichmax                                                   */
+    int32_t n_bins;          /* lchmax                                                   */
+    int32_t bin_mid;         /* ilmid                                                    */
+    int32_t ew3dc;           /* 1: slab term with the step's coefficient; 0: none        */
+    double wall_min;         /* lwmin as tot.inp gives it: distance from the wall        */
+    double wall_max;         /* lwmax                                                    */
+    double temperature;      /* chtemper (K); <= 0: 300 K as the reference without temper */
+    double seed;             /* generator state; <= 0: the reference's rank-0 start      */
+    int32_t max_partners;    /* partner list per insertion; 0: min(non-H sites, 4096)    */
+    int32_t pad;
+} mdf_chempot_params;
+/* NULL params switch the insertions off. */
+int mdf_chempot_enable(mdf_system *sys, const mdf_chempot_params *params);
+/* Copy pave [n_bins], inave [n_bins] and the generator state (any may be NULL); reset
+   non-zero zeroes pave and inave (not the generator). */
+int mdf_get_chempot(mdf_system *sys, double *pave, int64_t *inave, double *seed, int32_t reset);
+
 /* ---- reciprocal-force surrogate with multiple time steps (no counterpart in the reference,
    whose force_get evaluates the full k-sum every step, src/forcem.f90:827-918) ----------
    Between full k-sums the k-space site force is predicted by a small per-site network of

@@ -58,12 +58,19 @@ class Outputs(C.Structure):
 
 # reciprocal-force surrogate with multiple time steps (mdf_mts_params / mdf_mts_stats)
 MTS_MODELS = {"held": 0, "network": 1}
+PH_R = float(np.float32(8.31441))   # the reference's gas constant, a default-real literal (src/global.f90:21)
 MTS_WEIGHTS = 8448
 
 
 class MtsParams(C.Structure):
     _fields_ = [("interval", C.c_int32), ("model", C.c_int32), ("train_frames", C.c_int32),
                 ("train_iters", C.c_int32), ("tol", C.c_double), ("lr", C.c_float), ("seed", C.c_uint32)]
+
+
+class ChempotParams(C.Structure):
+    _fields_ = [("n_insert", C.c_int32), ("n_bins", C.c_int32), ("bin_mid", C.c_int32), ("ew3dc", C.c_int32),
+                ("wall_min", C.c_double), ("wall_max", C.c_double), ("temperature", C.c_double),
+                ("seed", C.c_double), ("max_partners", C.c_int32), ("pad", C.c_int32)]
 
 
 class MtsStats(C.Structure):

@@ -8,7 +8,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmdforce.so")
-SOURCES = ["nlist.hip", "pair.hip", "recip.hip", "epilogue.hip", "surrogate.hip", "api.hip"]
+SOURCES = ["nlist.hip", "pair.hip", "recip.hip", "epilogue.hip", "surrogate.hip", "chempot.hip", "api.hip"]
 ARCH = os.environ.get("MDF_OFFLOAD_ARCH", "gfx950")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
          # no packed FP32 instructions anywhere in the device code

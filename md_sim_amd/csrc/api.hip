@@ -68,6 +68,10 @@ __global__ void k_epilogue(DevSys S, const double *fpart, int n_groups, const Wa
                            const double *fmts, double *efi);
 __global__ void k_obs_ngc(DevSys S, ObsArgs o);
 __global__ void k_obs_efipos(DevSys S, ObsArgs o);
+__global__ void k_chem_scan(DevSys S, ChemArgs c);
+__global__ void k_chem_first(DevSys S, ChemArgs c);
+__global__ void k_chem_rot(DevSys S, ChemArgs c);
+__global__ void k_chem_accum(ChemArgs c);
 __global__ void k_colsum(DevSys S, const int *lead_slot, int per_block, double *colpart);
 __global__ void k_pair_totals(DevSys S);
 __global__ void k_final(DevSys S, FinalArgs A, const double *colpart, const double *kpart, const double *eppart,
@@ -270,7 +274,40 @@ struct mdf_system {
     DBuf<unsigned long long> obs_ngc;
     DBuf<double> obs_efi, obs_efipos;
     DBuf<int> sg_msize;
+    // Widom insertions (off unless mdf_chempot_enable)
+    bool chem_on = false;
+    mdf_chempot_params chem_p;
+    ChemArgs chem;
+    double chem_seed = 0, chem_lwmin = 0, chem_lwmax = 0, chem_facbin = 0;
+    double chem_aco = 0, chem_asi = 0, chem_lo = 0;
+    size_t chem_lds = 0;
+    DBuf<double> ch_ins, ch_rot, ch_uox, ch_nbr_d, ch_boltz, ch_pave;
+    DBuf<int> ch_il, ch_insrot, ch_rotins, ch_over, ch_nnbr, ch_nbr, ch_status;
+    DBuf<unsigned long long> ch_inave;
+    double *h_ins = nullptr, *h_rot = nullptr;   // pinned staging of one call's draws
+    int *h_il = nullptr, *h_insrot = nullptr, *h_rotins = nullptr, *h_status = nullptr;
+    std::vector<double> ch_seed_pos;             // generator state after each insertion's position
+    double ch_seed_end = 0;
+    bool ch_pending = false;
+    hipEvent_t ch_done = nullptr;
 };
+
+static void chem_free(mdf_system *s) {
+    DBuf<double> *db[] = {&s->ch_ins, &s->ch_rot, &s->ch_uox, &s->ch_nbr_d, &s->ch_boltz, &s->ch_pave};
+    for (auto *b : db) b->release();
+    DBuf<int> *ib[] = {&s->ch_il, &s->ch_insrot, &s->ch_rotins, &s->ch_over, &s->ch_nnbr, &s->ch_nbr, &s->ch_status};
+    for (auto *b : ib) b->release();
+    s->ch_inave.release();
+    void *hb[] = {s->h_ins, s->h_rot, s->h_il, s->h_insrot, s->h_rotins, s->h_status};
+    for (void *b : hb)
+        if (b) (void)hipHostFree(b);
+    s->h_ins = s->h_rot = nullptr;
+    s->h_il = s->h_insrot = s->h_rotins = s->h_status = nullptr;
+    if (s->ch_done) (void)hipEventDestroy(s->ch_done);
+    s->ch_done = nullptr;
+    s->ch_pending = false;
+    s->chem_on = false;
+}
 
 static void mts_free(mdf_system *s) {
     s->mts_st.release();
@@ -303,6 +340,7 @@ static void free_all(mdf_system *s) {
     mts_free(s);
     s->obs_ngc.release(); s->obs_efi.release(); s->obs_efipos.release();
     s->obs_on = false;
+    chem_free(s);
 }
 
 template <typename T>
@@ -716,6 +754,133 @@ static void drain_all(mdf_system *s) {
     }
 }
 
+// ---- Widom insertions -------------------------------------------------------------------
+
+// the reference's generator: seed = mod(8192 seed, 67099547), r = seed 1.4903230270690e-8
+// (src/rndmod.f90:28-35); the product stays below 2^53, so fmod is exact
+struct ChemRng {
+    double seed;
+    double next() {
+        seed = std::fmod(8192.0 * seed, 67099547.0);
+        return seed * 1.4903230270690e-8;
+    }
+};
+
+// one call's geometry in the reference's draw order (src/forcem.f90:1258-1272,1338-1369);
+// the generator state after each insertion's position is kept, since an overlap ends the
+// call right there
+static int chem_draw(mdf_system *s, int *n_rot) {
+    const int n = s->chem_p.n_insert;
+    const double lx = s->par.box[0], ly = s->par.box[1];
+    const double lwmin = s->chem_lwmin, lwmax = s->chem_lwmax, facbin = s->chem_facbin;
+    const double aco = s->chem_aco, asi = s->chem_asi, lo = s->chem_lo, pi = s->c.pi;
+    ChemRng g{s->chem_seed};
+    int nr = 0;
+    for (int ic = 0; ic < n; ic++) {
+        const double ox = (g.next() - 0.5) * lx;
+        const double oy = (g.next() - 0.5) * ly;
+        double oz;
+        long guard = 0;
+        do {
+            oz = ((g.next() - 0.5) * lwmax) * 2.0;
+            if (++guard > 100000000L) return fail(MDF_ERR_ARG, "chempot: no insertion height found");
+        } while (!(std::fabs(oz) > lwmin));
+        s->ch_seed_pos[ic] = g.seed;
+        const int il = (int)((std::fabs(oz) - lwmin) * facbin) + 1;
+        const int irot = il > s->chem_p.bin_mid ? CHEM_ROT_MAX : 1;
+        s->h_ins[3 * ic] = ox; s->h_ins[3 * ic + 1] = oy; s->h_ins[3 * ic + 2] = oz;
+        s->h_il[ic] = il;
+        s->h_insrot[ic] = nr;
+        const double o[3] = {ox, oy, oz};
+        for (int ir = 0; ir < irot; ir++) {
+            double h1[3], ri;
+            do {
+                h1[0] = g.next() - 0.5;
+                h1[1] = g.next() - 0.5;
+                h1[2] = g.next() - 0.5;
+                ri = h1[0] * h1[0] + h1[1] * h1[1] + h1[2] * h1[2];
+            } while (!(ri < 0.25 && ri > lo));
+            ri = 1.0 / std::sqrt(ri);
+            for (int d = 0; d < 3; d++) h1[d] = h1[d] * ri;
+            ri = 1.0 / std::sqrt(h1[0] * h1[0] + h1[1] * h1[1]);
+            const double h2[3] = {-h1[1] * ri, h1[0] * ri, 0.0};
+            const double mr[3] = {h2[1] * h1[2] - h2[2] * h1[1], h2[2] * h1[0] - h2[0] * h1[2],
+                                  h2[0] * h1[1] - h2[1] * h1[0]};
+            const double alph = 2.0 * pi * g.next();
+            const double ca = std::cos(alph), sa = std::sin(alph);
+            for (int d = 0; d < 3; d++) {
+                s->h_rot[6 * nr + d] = (o[d] + aco * h1[d]) + asi * (ca * h2[d] + sa * mr[d]);
+                s->h_rot[6 * nr + 3 + d] = o[d] + h1[d];
+            }
+            s->h_rotins[nr] = ic;
+            nr++;
+        }
+    }
+    s->h_insrot[n] = nr;
+    s->ch_seed_end = g.seed;
+    *n_rot = nr;
+    return MDF_OK;
+}
+
+// the last call's outcome: the generator continues after the first overlapping insertion's
+// position, or after the whole call
+static int chem_resolve(mdf_system *s) {
+    if (!s->ch_pending) return MDF_OK;
+    HIPCHK(hipEventSynchronize(s->ch_done));
+    s->ch_pending = false;
+    const int *st = s->h_status;
+    if (st[2]) return MDF_OK;   // dropped with its step, whose list overflowed: the redo inserts again
+    if (st[1]) {
+        const int had = s->chem.cap;
+        int rc = MDF_OK;
+        const int cap = std::max(had, (int)(st[3] * 1.25) + 16);
+        DBuf<int> nb;
+        DBuf<double> nd;
+        if (nb.alloc((size_t)s->chem_p.n_insert * cap) != hipSuccess ||
+            nd.alloc((size_t)3 * s->chem_p.n_insert * cap) != hipSuccess) {
+            nb.release(); nd.release();
+            return fail(MDF_ERR_HIP, "chempot: partner lists of %d", cap);
+        }
+        s->ch_nbr.release(); s->ch_nbr_d.release();
+        s->ch_nbr = nb; s->ch_nbr_d = nd;
+        s->chem.nbr = nb.p; s->chem.nbr_d = nd.p; s->chem.cap = cap;
+        rc = fail(MDF_ERR_LIST_OVERFLOW, "chempot: an insertion had %d partners within rcut, more than %d: that "
+                  "step's insertions were dropped (capacity now %d)", st[3], had, cap);
+        return rc;
+    }
+    const int first = st[0];
+    s->chem_seed = first < s->chem_p.n_insert ? s->ch_seed_pos[first] : s->ch_seed_end;
+    return MDF_OK;
+}
+
+static int chem_enqueue(mdf_system *s) {
+    int rc = chem_resolve(s);
+    if (rc) return rc;
+    int nr = 0;
+    rc = chem_draw(s, &nr);
+    if (rc) return rc;
+    const int n = s->chem_p.n_insert;
+    hipStream_t st = s->stream;
+    HIPCHK(hipMemcpyAsync(s->ch_ins.p, s->h_ins, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(s->ch_il.p, s->h_il, sizeof(int) * n, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(s->ch_insrot.p, s->h_insrot, sizeof(int) * (n + 1), hipMemcpyHostToDevice, st));
+    if (nr > 0) {
+        HIPCHK(hipMemcpyAsync(s->ch_rotins.p, s->h_rotins, sizeof(int) * nr, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(s->ch_rot.p, s->h_rot, sizeof(double) * 6 * nr, hipMemcpyHostToDevice, st));
+    }
+    ChemArgs c = s->chem;
+    c.n_rot = nr;
+    k_chem_scan<<<n, 256, 0, st>>>(s->S, c);
+    k_chem_first<<<1, 256, 0, st>>>(s->S, c);
+    if (nr > 0) k_chem_rot<<<nr, 256, s->chem_lds, st>>>(s->S, c);
+    k_chem_accum<<<1, 256, 0, st>>>(c);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(s->h_status, s->ch_status.p, sizeof(int) * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipEventRecord(s->ch_done, st));
+    s->ch_pending = true;
+    return MDF_OK;
+}
+
 // one step: the fixed launch sequence, replayed from a captured graph when possible (a
 // forced rebuild goes through direct launches)
 static int run_step(mdf_system *s, int force_rebuild) {
@@ -754,6 +919,11 @@ static int run_step(mdf_system *s, int force_rebuild) {
     }
     HIPCHK(hipGetLastError());
     s->pending = true;
+    if (s->chem_on) {
+        // after the step, outside its graph: the host draws each call's geometry
+        int rc = chem_enqueue(s);
+        if (rc) return rc;
+    }
     return MDF_OK;
 }
 
@@ -1609,6 +1779,7 @@ int mdf_mts_enable(mdf_system *s, const mdf_mts_params *p) {
         mts_free(s);
         return MDF_OK;
     }
+    if (s->chem_on) return fail(MDF_ERR_ARG, "Widom insertions read every step's S(k): switch them off first");
     if (p->interval < 1) return fail(MDF_ERR_ARG, "interval %d < 1", p->interval);
     if (p->model != MDF_MTS_HELD && p->model != MDF_MTS_NETWORK) return fail(MDF_ERR_ARG, "unknown model %d", p->model);
     if (p->train_frames < 0 || p->train_iters < 0 || p->train_iters > 16)
@@ -1706,6 +1877,124 @@ int mdf_get_observables(mdf_system *s, int64_t *ngc, double *efipos, int32_t *nb
     if (reset) {
         HIPCHK(hipMemset(s->obs_ngc.p, 0, sizeof(unsigned long long) * s->obs_ngc.n));
         HIPCHK(hipMemset(s->obs_efipos.p, 0, sizeof(double) * s->obs_efipos.n));
+    }
+    return MDF_OK;
+}
+
+int mdf_chempot_enable(mdf_system *s, const mdf_chempot_params *p) {
+    if (!s) return fail(MDF_ERR_ARG, "null handle");
+    HIPCHK(hipSetDevice(s->device));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    chem_free(s);
+    if (!p) return MDF_OK;
+    if (s->mts_on) return fail(MDF_ERR_ARG, "insertions read every step's S(k): switch the k-space surrogate off first");
+    if (p->n_insert < 1 || p->n_insert > (1 << 22)) return fail(MDF_ERR_ARG, "n_insert %d outside 1..2^22", p->n_insert);
+    if (p->n_bins < 1 || p->n_bins > (1 << 24)) return fail(MDF_ERR_ARG, "n_bins %d outside 1..2^24", p->n_bins);
+    if (p->ew3dc != 0 && p->ew3dc != 1) return fail(MDF_ERR_ARG, "ew3dc must be 0 or 1");
+    if (p->max_partners < 0) return fail(MDF_ERR_ARG, "max_partners < 0");
+    // wall distances to center distances (src/measure.f90:151-154)
+    const double lwh = 0.5 * s->top.wall_lw;
+    const double lwmin = lwh - p->wall_max, lwmax = lwh - p->wall_min;
+    if (!(lwmax > lwmin) || !(lwmax > 0.0))
+        return fail(MDF_ERR_ARG, "insertion window: wall_min %g must be below wall_max %g and below lw/2 = %g",
+                    p->wall_min, p->wall_max, lwh);
+    int klmax = 0, kmmax = 0, knmax = 0;
+    for (size_t k = 0; k + 2 < s->kvec.size(); k += 3) {
+        klmax = std::max(klmax, std::abs(s->kvec[k]));
+        kmmax = std::max(kmmax, std::abs(s->kvec[k + 1]));
+        knmax = std::max(knmax, std::abs(s->kvec[k + 2]));
+    }
+    const int per = (klmax + 1) + (2 * kmmax + 1) + (2 * knmax + 1);
+    if (per > CHEM_TAB_MAX) return fail(MDF_ERR_ARG, "k-set too wide for the insertion tables (%d > %d)", per, CHEM_TAB_MAX);
+    int n_nonh = 0;
+    for (int t : s->h_itype) n_nonh += t != 2;
+    const int cap = std::max(1, p->max_partners > 0 ? p->max_partners : std::min(n_nonh, 4096));
+    const int n = p->n_insert, nrmax = CHEM_ROT_MAX * n;
+    const bool ref = s->par.const_mode == MDF_CONST_REFERENCE;
+    auto lit = [&](double v) { return ref ? (double)(float)v : v; };
+    s->chem_p = *p;
+    s->chem_seed = p->seed > 0.0 ? p->seed : 14135621769.0 + 3.0;   // rnd_init on rank 0 (src/measure.f90:150)
+    s->chem_lwmin = lwmin;
+    s->chem_lwmax = lwmax;
+    s->chem_facbin = 1.0 / (lwmax - lwmin) * (double)(float)p->n_bins;
+    s->chem_aco = lit(-0.333333);
+    s->chem_asi = lit(0.942809);
+    s->chem_lo = lit(0.001);
+    s->chem_lds = sizeof(double2) * 3 * per;
+#define CTRY(x)                          \
+    do {                                 \
+        hipError_t e_ = (x);             \
+        if (e_ != hipSuccess) {          \
+            chem_free(s);                \
+            return fail(MDF_ERR_HIP, "chempot buffers: %s", hipGetErrorString(e_)); \
+        }                                \
+    } while (0)
+    CTRY(s->ch_ins.alloc(3 * (size_t)n));
+    CTRY(s->ch_il.alloc(n));
+    CTRY(s->ch_insrot.alloc((size_t)n + 1));
+    CTRY(s->ch_rotins.alloc(nrmax));
+    CTRY(s->ch_rot.alloc(6 * (size_t)nrmax));
+    CTRY(s->ch_over.alloc(n));
+    CTRY(s->ch_nnbr.alloc(n));
+    CTRY(s->ch_uox.alloc(n));
+    CTRY(s->ch_nbr.alloc((size_t)n * cap));
+    CTRY(s->ch_nbr_d.alloc(3 * (size_t)n * cap));
+    CTRY(s->ch_boltz.alloc(nrmax));
+    CTRY(s->ch_status.alloc(4));
+    CTRY(s->ch_pave.alloc(p->n_bins));
+    CTRY(s->ch_inave.alloc(p->n_bins));
+    CTRY(hipMemset(s->ch_pave.p, 0, sizeof(double) * p->n_bins));
+    CTRY(hipMemset(s->ch_inave.p, 0, sizeof(unsigned long long) * p->n_bins));
+    CTRY(hipMemset(s->ch_status.p, 0, sizeof(int) * 4));
+    CTRY(hipHostMalloc((void **)&s->h_ins, sizeof(double) * 3 * n, hipHostMallocDefault));
+    CTRY(hipHostMalloc((void **)&s->h_rot, sizeof(double) * 6 * nrmax, hipHostMallocDefault));
+    CTRY(hipHostMalloc((void **)&s->h_il, sizeof(int) * n, hipHostMallocDefault));
+    CTRY(hipHostMalloc((void **)&s->h_insrot, sizeof(int) * (n + 1), hipHostMallocDefault));
+    CTRY(hipHostMalloc((void **)&s->h_rotins, sizeof(int) * nrmax, hipHostMallocDefault));
+    CTRY(hipHostMalloc((void **)&s->h_status, sizeof(int) * 4, hipHostMallocDefault));
+    CTRY(hipEventCreateWithFlags(&s->ch_done, hipEventDisableTiming));
+#undef CTRY
+    s->ch_seed_pos.assign(n, 0.0);
+    ChemArgs &c = s->chem;
+    memset(&c, 0, sizeof c);
+    c.ins = s->ch_ins.p; c.ins_il = s->ch_il.p; c.ins_rot = s->ch_insrot.p; c.rot_ins = s->ch_rotins.p;
+    c.rot = s->ch_rot.p; c.overlap = s->ch_over.p; c.n_nbr = s->ch_nnbr.p; c.u_ox = s->ch_uox.p;
+    c.nbr = s->ch_nbr.p; c.nbr_d = s->ch_nbr_d.p; c.boltz = s->ch_boltz.p; c.status = s->ch_status.p;
+    c.pave = s->ch_pave.p; c.inave = s->ch_inave.p; c.dipz = s->dipz.p;
+    c.kvec = s->d_kvec.p; c.ak = s->d_ak.p; c.sk = s->d_s.p;
+    c.n_ins = n; c.cap = cap; c.n_bins = p->n_bins; c.nk = (int)s->ak.size();
+    c.klmax = klmax; c.kmmax = kmmax; c.knmax = knmax;
+    const double chtemper = p->temperature > 0.0 ? p->temperature : 300.0;
+    c.beta = 1000.0 / (lit(8.31441) * chtemper);
+    c.rcut = s->par.rcut;
+    c.rinsmin = 4.0;
+    c.rwinsmin = ref ? (double)(2.35f * 2.35f) : 2.35 * 2.35;
+    c.qsw[0] = lit(-0.8476); c.qsw[1] = lit(0.4238); c.qsw[2] = lit(0.4238);
+    c.crf_scale = p->ew3dc ? 2.0 * s->crf0 * s->S.facq : 0.0;
+    c.lwh = lwh;
+    c.pipix = 2.0 * s->c.pi / s->par.box[0];
+    c.pipiy = 2.0 * s->c.pi / s->par.box[1];
+    c.pipiz = 2.0 * s->c.pi / s->par.box[2];
+    c.ulr_w = s->wl.ulr_w[1]; c.ulr_l = s->wl.ulr_l[1]; c.lja = s->wl.flr_lja[1]; c.ljc = s->wl.flr_ljc[1];
+    c.bua = s->wl.flr_bua[1]; c.buc = s->wl.flr_buc[1]; c.bur = s->wl.flr_bur[1];
+    s->chem_on = true;
+    return MDF_OK;
+}
+
+int mdf_get_chempot(mdf_system *s, double *pave, int64_t *inave, double *seed, int32_t reset) {
+    if (!s) return fail(MDF_ERR_ARG, "null handle");
+    if (!s->chem_on) return fail(MDF_ERR_ARG, "insertions are off");
+    HIPCHK(hipSetDevice(s->device));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    int rc = chem_resolve(s);
+    if (rc) return rc;
+    const int nb = s->chem_p.n_bins;
+    if (pave) HIPCHK(hipMemcpy(pave, s->ch_pave.p, sizeof(double) * nb, hipMemcpyDeviceToHost));
+    if (inave) HIPCHK(hipMemcpy(inave, s->ch_inave.p, sizeof(int64_t) * nb, hipMemcpyDeviceToHost));
+    if (seed) *seed = s->chem_seed;
+    if (reset) {
+        HIPCHK(hipMemset(s->ch_pave.p, 0, sizeof(double) * nb));
+        HIPCHK(hipMemset(s->ch_inave.p, 0, sizeof(unsigned long long) * nb));
     }
     return MDF_OK;
 }
@@ -2063,6 +2352,7 @@ int mdf_batch_create(mdf_system *const *systems, int32_t n, mdf_batch **out) {
     for (int i = 0; i < n; i++) {
         if (systems[i]->mts_on) return fail(MDF_ERR_ARG, "system %d has multiple time steps on (not in batches)", i);
         if (systems[i]->obs_on) return fail(MDF_ERR_ARG, "system %d accumulates observables (not in batches)", i);
+        if (systems[i]->chem_on) return fail(MDF_ERR_ARG, "system %d makes Widom insertions (not in batches)", i);
     }
     mdf_batch *b = new mdf_batch;
     b->sys.assign(systems, systems + n);

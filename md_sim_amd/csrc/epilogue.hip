@@ -176,6 +176,8 @@ __global__ __launch_bounds__(256) void k_obs_ngc(DevSys S, ObsArgs o) {
     const int lane = threadIdx.x & 15;
     const int gid = blockIdx.x * 16 + (threadIdx.x >> 4);
     if (gid >= S.n_lead) return;
+    // a step whose list overflowed is run again with grown lists: count it only then
+    if (S.flags->overflow_mm || S.flags->overflow_mw) return;
     const int i = S.lead[gid];
     const int ti = S.itype[i], nsi_i = S.nsi[ti];
     const double xi = S.x[i], yi = S.y[i], zi = S.z[i];
@@ -241,6 +243,7 @@ __global__ __launch_bounds__(256) void k_obs_ngc(DevSys S, ObsArgs o) {
 // efipos bins of the step's ion z-forces, in site order (one thread: a fixed summation order)
 __global__ void k_obs_efipos(DevSys S, ObsArgs o) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (S.flags->overflow_mm || S.flags->overflow_mw) return;
     for (int i = S.n_rigid_sites; i < S.n_liquid_sites; i++) {
         const int it = S.itype[i];
         if (it != 3 && it != 4) continue;

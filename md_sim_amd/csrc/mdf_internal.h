@@ -223,6 +223,37 @@ struct ObsArgs {
     int count_mm;              // 1: moving-moving site pairs count too (3-D pair loops)
 };
 
+// ---- Widom test-particle insertions (measurement only) ---------------------------------
+constexpr int CHEM_ROT_MAX = 20;        // orientations per insertion in the far bins
+constexpr int CHEM_TAB_MAX = 1024;      // e^{i k r} table entries per site (LDS)
+struct ChemArgs {
+    // this call's draws, in insertion order
+    const double *ins;          // [n_ins][3] oxygen positions
+    const int *ins_il;          // [n_ins] 1-based bin
+    const int *ins_rot;         // [n_ins + 1] prefix of the orientation counts
+    const int *rot_ins;         // [n_rot] insertion of each orientation
+    const double *rot;          // [n_rot][6] the two hydrogens (x, y, z each)
+    // oxygen scan
+    int *overlap;               // [n_ins]
+    int *n_nbr;                 // [n_ins] partners within rcut (may exceed cap)
+    double *u_ox;               // [n_ins]
+    int *nbr;                   // [n_ins][cap]
+    double *nbr_d;              // [n_ins][cap][3] oxygen - partner, minimum image in x, y
+    double *boltz;              // [n_rot] exp(-beta u)
+    int *status;                // [4]: first overlap (n_ins: none), partner overflow, skipped, max partners
+    double *pave;               // [n_bins]
+    unsigned long long *inave;  // [n_bins]
+    const double *dipz;         // the step's sum q z over moving sites
+    const int *kvec;            // [nk][3]
+    const double *ak;           // [nk]
+    const double2 *sk;          // [nk] this step's S(k)
+    int n_ins, n_rot, cap, n_bins, nk, klmax, kmmax, knmax;
+    double rcut, beta, rinsmin, rwinsmin, qsw[3];
+    double crf_scale;           // crf = crf_scale * dipz (0: no slab term)
+    double lwh, pipix, pipiy, pipiz;
+    double ulr_w, ulr_l, lja, ljc, bua, buc, bur;  // wall tail tables of type 1
+};
+
 // ---- batches of independent systems ----------------------------------------------------
 // One entry per system: everything its launches take.  A batched kernel covers the blocks of
 // every system in one launch; a block map entry (system, local block) tells each block whose

@@ -56,7 +56,7 @@ def lib() -> C.CDLL:
                    "mdf_set_launch_shapes", "mdf_batch_create", "mdf_batch_compute_device",
                    "mdf_batch_read_outputs", "mdf_batch_autotune", "mdf_mts_enable", "mdf_mts_get_stats",
                    "mdf_mts_get_weights", "mdf_mts_set_weights", "mdf_mts_features", "mdf_mts_eval",
-                   "mdf_observables_enable", "mdf_get_observables"):
+                   "mdf_observables_enable", "mdf_get_observables", "mdf_chempot_enable", "mdf_get_chempot"):
             getattr(L, fn).restype = C.c_int
         L.mdf_destroy.restype = None
         L.mdf_batch_destroy.restype = None
@@ -160,6 +160,38 @@ class ForceEngine:
         _check(lib().mdf_get_observables(self._h, ngc.ctypes.data_as(C.POINTER(C.c_int64)), _dp(efp), C.byref(nb),
                                          int(bool(reset))))
         return ngc, efp
+
+    # -- Widom test-particle insertions (the reference's chempot) ------------------------
+    def chempot_enable(self, n_insert: int, wall_min: float, wall_max: float, n_bins: int, bin_mid: int,
+                       temperature: float = 0.0, ew3dc: bool = True, seed: float = 0.0, max_partners: int = 0):
+        """Insert up to n_insert SPC/E waters after every step, as the reference's chempot
+        (src/forcem.f90:1226-1507) with its tot.inp values ichmax, lwmin, lwmax, lchmax, ilmid
+        (src/main.f90:148-152).  temperature <= 0 means 300 K; seed <= 0 the reference's
+        start; ew3dc: the slab term with the step's coefficient (the reference reads an
+        unassigned local there)."""
+        p = abi.ChempotParams(n_insert=n_insert, n_bins=n_bins, bin_mid=bin_mid, ew3dc=int(bool(ew3dc)),
+                              wall_min=wall_min, wall_max=wall_max, temperature=temperature, seed=seed,
+                              max_partners=max_partners)
+        _check(lib().mdf_chempot_enable(self._h, C.byref(p)))
+        self._chem_bins = n_bins
+        self._chem_temper = temperature if temperature > 0 else 300.0
+
+    def chempot_disable(self):
+        _check(lib().mdf_chempot_enable(self._h, None))
+
+    def chempot(self, reset: bool = False):
+        """dict(pave [n_bins], inave [n_bins] int64, seed, mu): the accumulated Boltzmann
+        factors and orientation counts per bin, the generator state, and the excess chemical
+        potential -RT ln(sum pave / sum inave) in kJ/mol (src/measure.f90:701)."""
+        nb = self._chem_bins
+        pave = np.zeros(nb)
+        inave = np.zeros(nb, np.int64)
+        seed = C.c_double()
+        _check(lib().mdf_get_chempot(self._h, _dp(pave), inave.ctypes.data_as(C.POINTER(C.c_int64)),
+                                     C.byref(seed), int(bool(reset))))
+        n, sp = int(inave.sum()), float(pave.sum())
+        mu = -np.log(sp / n) * abi.PH_R * self._chem_temper / 1000.0 if n > 0 and sp > 0 else float("nan")
+        return dict(pave=pave, inave=inave, seed=seed.value, mu=mu)
 
     # -- reciprocal-force surrogate with multiple time steps ------------------------------
     def mts_enable(self, interval: int = 4, model: str = "network", train_frames: int = 200, train_iters: int = 2,

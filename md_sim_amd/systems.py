@@ -477,6 +477,8 @@ def config(name: str) -> System:
     * ``bulk216``  - 216 SPC/E waters, 18.64 A cube, rc 9, kmax 6, 3-D periodic;
     * ``pore_small`` - 96 waters + 8 mobile ions on a 2x2-cell, 4-layer slab (parity case);
     * ``pore_water`` - the same 96 waters without mobile ions (NVE drift case);
+    * ``pore_sparse`` - 8 waters + 4 mobile ions in one layer 2.8 A above the same slab, the
+      rest of the pore empty (Widom insertion case);
     * ``slab2048`` - 2048 waters on a 12x12x(8 layers) rock-salt slab (4608 ions),
       rc 10, k 8/8/24;
     * ``slab16k``  - 16384 waters on a 16x16x(6 layers) slab (6144 ions), rc 10,
@@ -494,6 +496,9 @@ def config(name: str) -> System:
     if name == "pore_small":
         return water_on_slab(96, 2, 4, seed=1101, rcut=5.0, kmax=3, lw=20.0,
                              n_mobile_ions=8, name=name)
+    if name == "pore_sparse":
+        return water_on_slab(8, 2, 4, seed=1103, rcut=5.0, kmax=3, lw=20.0,
+                             n_mobile_ions=4, name=name)
     if name == "pore_water":
         return water_on_slab(96, 2, 4, seed=1101, rcut=5.0, kmax=3, lw=20.0, name=name)
     if name == "slab2048":

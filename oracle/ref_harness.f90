@@ -16,6 +16,7 @@ program ref_harness
     use move
     use measure
     use comm
+    use rndmod
     implicit none
     include 'mpif.h'
     integer :: ierr, u
@@ -27,6 +28,11 @@ program ref_harness
     real*8, allocatable :: kak(:), kre(:), kim(:), kvec(:)
     integer*8, allocatable :: ibuf(:)
     real*8, allocatable :: cbuf(:)
+    character*256 :: chemenv
+    integer*8 :: nchcall, ic
+    logical :: chzero
+    real*8, allocatable :: chp(:), chs(:), chpar(:)
+    integer*8, allocatable :: chi(:)
 
     nproc = 1
     me = 0
@@ -58,6 +64,17 @@ program ref_harness
     lcouett = .false.
     lfext = .false.
     lchempot = .false.
+    ! optional Widom insertions after the step (src/forcem.f90:126): REF_CHEMPOT holds the
+    ! tot.inp values "ichmax lwmin lwmax lchmax ilmid" (src/main.f90:148-152) and the number
+    ! of chempot calls to make on the same configuration
+    nchcall = 0
+    call get_environment_variable('REF_CHEMPOT_ZERO', chemenv)
+    chzero = len_trim(chemenv) > 0
+    call get_environment_variable('REF_CHEMPOT', chemenv)
+    if (len_trim(chemenv) > 0) then
+        read(chemenv, *) ichmax, lwmin, lwmax, lchmax, ilmid, nchcall
+        lchempot = .true.
+    end if
     dshx = 0.0 ; dshy = 0.0 ; dshz = 0.0
     n = len_trim(base)
     filfld = base(1:n)//'.fld'
@@ -143,6 +160,27 @@ program ref_harness
 
     call force_sum
 
+    ! Widom insertions on the step's configuration: cumulative pave/inave and the generator
+    ! state after each call
+    if (lchempot .and. nchcall > 0) then
+        ! chempot's k loop runs over every kcount, also the ones force_fouri_init leaves
+        ! unassigned (myk < 0, src/forcem.f90:757-770): define them as zero, as a fresh
+        ! allocation would read
+        if (chzero) then
+            where (myk(1:kcnt) < 0) Ak(1:kcnt) = 0.0
+            where (myk(1:kcnt) < 0) Qsum(1:kcnt) = (0.0, 0.0)
+        end if
+        allocate(chp(lchmax*nchcall), chi(lchmax*nchcall), chs(nchcall), chpar(13))
+        do ic = 1, nchcall
+            call chempot
+            chp((ic-1)*lchmax+1:ic*lchmax) = pave(1:lchmax)
+            chi((ic-1)*lchmax+1:ic*lchmax) = inave(1:lchmax)
+            chs(ic) = seed
+        end do
+        chpar = (/ lwmin, lwmax, facbin, chtemper, lwh, ulr_w(1), ulr_l(1), flr_lja(1), flr_ljc(1), &
+                   flr_bua(1), flr_buc(1), flr_bur(1), rcut /)
+    end if
+
     if (nproc == 1) then
     u = 11
     open(u, file=trim(outname), access='stream', form='unformatted', status='replace')
@@ -184,6 +222,20 @@ program ref_harness
     ! ngc(nijt, 0:lmax) and efipos(2, 0:lmax), column-major
     call put_i(u, 'ngc', ngc, int(size(ngc), 8))
     call put_r(u, 'efipos', efipos, int(size(efipos), 8))
+    if (lchempot .and. nchcall > 0) then
+        call put_r(u, 'chem_pave', chp, lchmax*nchcall)
+        call put_i(u, 'chem_inave', chi, lchmax*nchcall)
+        call put_r(u, 'chem_seed', chs, nchcall)
+        call put_r(u, 'chem_par', chpar, 13_8)
+        ! the wall tail tables of every type (src/forcem.f90:170-228), for the force path's ulr
+        call put_r(u, 'wall_ulr_w', ulr_w, int(nit, 8))
+        call put_r(u, 'wall_ulr_l', ulr_l, int(nit, 8))
+        call put_r(u, 'wall_lja', flr_lja, int(nit, 8))
+        call put_r(u, 'wall_ljc', flr_ljc, int(nit, 8))
+        call put_r(u, 'wall_bua', flr_bua, int(nit, 8))
+        call put_r(u, 'wall_buc', flr_buc, int(nit, 8))
+        call put_r(u, 'wall_bur', flr_bur, int(nit, 8))
+    end if
     call put_r(u, 'k_ak', kak, nsel)
     call put_r(u, 'k_sre', kre, nsel)
     call put_r(u, 'k_sim', kim, nsel)

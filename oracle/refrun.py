@@ -42,11 +42,13 @@ def mpi_available() -> bool:
 
 
 def run_reference(system, workdir: str = None, timeout: float = 600.0, nrep: int = 0,
-                  nproc: int = 1) -> Dict[str, np.ndarray]:
+                  nproc: int = 1, env=None) -> Dict[str, np.ndarray]:
     """Write the reference's input files for `system`, run the matching harness variant
     (bulk = 3-D periodic, slab otherwise) and return the parsed dump.  nproc > 1 runs the
     harness on that many MPI ranks (the reference's own decomposition of the force path)
-    for timing only: the result then holds the timing and no dump."""
+    for timing only: the result then holds the timing and no dump.  env adds environment
+    variables (REF_CHEMPOT="ichmax lwmin lwmax lchmax ilmid ncall" runs chempot ncall times
+    after the step and dumps chem_pave, chem_inave, chem_seed and chem_par)."""
     from md_sim_amd.refio import write_reference_inputs
 
     variant = "bulk" if system.z_periodic else "slab"
@@ -62,7 +64,10 @@ def run_reference(system, workdir: str = None, timeout: float = 600.0, nrep: int
     args = [exe, out] + ([str(nrep)] if nrep > 0 else [])
     if nproc > 1:
         args = [MPIEXEC, "-n", str(int(nproc))] + args
-    r = subprocess.run(args, cwd=d, capture_output=True, text=True, timeout=timeout)
+    e = dict(os.environ)
+    if env:
+        e.update(env)
+    r = subprocess.run(args, cwd=d, capture_output=True, text=True, timeout=timeout, env=e)
     if r.returncode != 0 or (nproc == 1 and not os.path.exists(out)):
         raise RuntimeError("reference harness failed (%d):\n%s\n%s" % (r.returncode, r.stdout[-2000:], r.stderr[-2000:]))
     dump = parse_dump(out) if nproc == 1 else {}

@@ -9,11 +9,16 @@ stores its inputs and outputs:
                    (src/forcem.f90) with slab correction and wall tail correction;
 * slab2048.npz   - 2048 waters on a 4608-ion slab (the 16k workload's smaller sibling):
                    energies, pair-list checksums and every 7th site force;
+* chempot.npz    - the reference's Widom insertions (chempot) after one step: a dense pore
+                   (pore_small: every call ends at an overlap) and a sparse one (pore_sparse:
+                   insertion windows clear of the water layer, and reaching into it), with
+                   cumulative pave/inave and the generator state after each call, the wall
+                   tail tables, and the state the insertions read (positions, k-set, S(k));
 * sample_rutile.npz - the reference's own sample input (src/sim/y.*, surface OH groups with
                    constraints and angles on rutile): input text, the derived topology the
                    reference builds from it, and its outputs.
 
-Usage:  python tests/golden/make_golden.py
+Usage:  python tests/golden/make_golden.py [chempot]
 """
 import hashlib
 import os
@@ -85,7 +90,35 @@ def sample_case():
     print("wrote", path, os.path.getsize(path), "bytes")
 
 
+# Widom cases: system, REF_CHEMPOT = "ichmax lwmin lwmax lchmax ilmid ncall" (tot.inp's
+# chempot lines, src/main.f90:148-152, and the number of calls)
+CHEMPOT_CASES = (("dense", "pore_small", "40 1.0 9.0 8 4 2"),
+                 ("clear", "pore_sparse", "30 6.0 9.5 6 3 2"),
+                 ("mixed", "pore_sparse", "60 1.0 9.5 10 5 3"))
+CHEMPOT_KEYS = ("chem_pave", "chem_inave", "chem_seed", "chem_par", "wall_ulr_w", "wall_ulr_l", "wall_lja",
+                "wall_ljc", "wall_bua", "wall_buc", "wall_bur", "ulr", "x0", "y0", "z0", "qs", "itype", "cx0",
+                "cy0", "cz0", "k_ak", "k_sre", "k_sim", "k_vec", "types", "nsi", "ijtype", "vdwnum", "eps", "sig",
+                "aa", "rhoi", "cc", "alfa", "facq", "counts", "urf")
+
+
+def chempot_cases():
+    out = {}
+    for case, name, spec in CHEMPOT_CASES:
+        s = systems.config(name)
+        d = run_reference(s, env={"REF_CHEMPOT": spec})
+        out[case + "__spec"] = np.array([float(v) for v in spec.split()])
+        out[case + "__system"] = np.array(name)
+        for k in CHEMPOT_KEYS:
+            out[case + "__" + k] = d[k]
+    path = os.path.join(HERE, "chempot.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, os.path.getsize(path), "bytes")
+
+
 def main():
+    if sys.argv[1:] == ["chempot"]:
+        chempot_cases()
+        return
     if os.path.isdir(SAMPLE_DIR):
         sample_case()
     for name in ("bulk216", "pore_small", "slab2048"):

@@ -1,0 +1,139 @@
+"""Widom insertions on the device against the reference's chempot dumps and the oracle.
+
+The reference's dumps pin the generator's state after each call, the orientation counts
+inave and the bins that received accepted orientations, exactly; the Boltzmann sums pave
+(undefined in the reference, whose chempot multiplies an unassigned local) are checked
+against the oracle, which the CPU tests pin by brute force and Ewald identities.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from md_sim_amd import systems
+from helpers import GOLDEN
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), ".."))
+
+pytestmark = pytest.mark.gpu
+
+CASES = ("dense", "clear", "mixed")
+TOL = 1e-10
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _case(name):
+    g = np.load(os.path.join(GOLDEN, "chempot.npz"), allow_pickle=False)
+    d = {k.split("__", 1)[1]: g[k] for k in g.files if k.startswith(name + "__")}
+    return d, systems.config(str(d["system"]))
+
+
+def _oracle_calls(d, s, crf):
+    from oracle import chempot as C
+    st = C.state_from_dump(d, s.box)
+    spec, p = d["spec"], d["chem_par"]
+    par = dict(ichmax=int(spec[0]), lwmin=p[0], lwmax=p[1], facbin=p[2], lchmax=int(spec[3]),
+               ilmid=int(spec[4]), temper=p[3])
+    rng = C.Lcg()
+    pave = np.zeros(par["lchmax"])
+    inave = np.zeros(par["lchmax"], np.int64)
+    out = []
+    for _ in range(int(spec[5])):
+        C.chempot(st, par, rng, pave, inave, crf=crf)
+        out.append((pave.copy(), inave.copy(), rng.seed))
+    return out
+
+
+def _engine_calls(d, s, ew3dc, max_partners=0):
+    from md_sim_amd.engine import ForceEngine
+    pos = (d["x0"], d["y0"], d["z0"], d["cx0"], d["cy0"], d["cz0"])
+    e = ForceEngine(s, positions=pos)
+    spec = d["spec"]
+    e.chempot_enable(n_insert=int(spec[0]), wall_min=float(spec[1]), wall_max=float(spec[2]),
+                     n_bins=int(spec[3]), bin_mid=int(spec[4]), ew3dc=ew3dc, max_partners=max_partners)
+    out = []
+    for _ in range(int(spec[5])):
+        e.force_get(*pos)
+        out.append(e.chempot())
+    return e, out
+
+
+def _crf_module(d, s):
+    from oracle import chempot as C
+    nm = int(d["counts"][1])
+    dipz = float(np.sum(d["qs"][:nm] * d["z0"][:nm]))
+    crf0 = -2.0 * C.f32(3.1415926535898) * (1.0 / s.box[2]) * (1.0 / s.box[0]) * (1.0 / s.box[1])
+    return 2.0 * crf0 * float(np.ravel(d["facq"])[0]) * dipz
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_insertions_match_reference_and_oracle(case):
+    _say("case chempot", case)
+    d, s = _case(case)
+    _, got = _engine_calls(d, s, ew3dc=False)
+    want = _oracle_calls(d, s, crf=0.0)
+    ncall = int(d["spec"][5])
+    ref_p = d["chem_pave"].reshape(ncall, -1)
+    ref_i = d["chem_inave"].reshape(ncall, -1)
+    for c in range(ncall):
+        r = got[c]
+        assert r["seed"] == d["chem_seed"][c], (c, r["seed"], d["chem_seed"][c])
+        assert np.array_equal(r["inave"], ref_i[c])
+        assert np.array_equal(np.isnan(ref_p[c]), r["pave"] > 0), (r["pave"], ref_p[c])
+        wp, wi, ws = want[c]
+        assert np.array_equal(r["inave"], wi) and r["seed"] == ws
+        scale = max(np.abs(wp).max(), 1e-300)
+        assert np.abs(r["pave"] - wp).max() <= TOL * scale, (r["pave"], wp)
+
+
+def test_slab_term_with_the_step_coefficient():
+    _say("case chempot slab term")
+    d, s = _case("clear")
+    _, got = _engine_calls(d, s, ew3dc=True)
+    want = _oracle_calls(d, s, crf=_crf_module(d, s))
+    plain = _oracle_calls(d, s, crf=0.0)
+    for r, (wp, wi, ws), (pp, _, _) in zip(got, want, plain):
+        assert np.array_equal(r["inave"], wi) and r["seed"] == ws
+        assert np.abs(r["pave"] - wp).max() <= TOL * np.abs(wp).max()
+        assert np.abs(wp - pp).max() > 1e-6 * np.abs(wp).max()   # the term matters here
+
+
+def test_partner_overflow_is_reported_and_recovers():
+    # a partner list of 2 cannot hold the clear case's partners: the step's insertions are
+    # dropped with an error, the capacity grows, and the next steps insert as the reference
+    from md_sim_amd.engine import ForceEngine, MDForceError
+    _say("case chempot partner overflow")
+    d, s = _case("clear")
+    pos = (d["x0"], d["y0"], d["z0"], d["cx0"], d["cy0"], d["cz0"])
+    e = ForceEngine(s, positions=pos)
+    spec = d["spec"]
+    e.chempot_enable(n_insert=int(spec[0]), wall_min=float(spec[1]), wall_max=float(spec[2]),
+                     n_bins=int(spec[3]), bin_mid=int(spec[4]), ew3dc=False, max_partners=2)
+    e.force_get(*pos)
+    with pytest.raises(MDForceError) as err:
+        e.chempot()
+    assert err.value.code == 3
+    r = e.chempot()
+    assert r["inave"].sum() == 0 and r["pave"].sum() == 0
+    want = _oracle_calls(d, s, crf=0.0)
+    for c in range(int(spec[5])):
+        e.force_get(*pos)
+        r = e.chempot()
+        wp, wi, ws = want[c]
+        assert np.array_equal(r["inave"], wi) and r["seed"] == ws
+        assert np.abs(r["pave"] - wp).max() <= TOL * np.abs(wp).max()
+
+
+def test_reset_and_chemical_potential():
+    _say("case chempot reset")
+    d, s = _case("clear")
+    e, got = _engine_calls(d, s, ew3dc=False)
+    r = e.chempot(reset=True)
+    assert np.array_equal(r["inave"], got[-1]["inave"])
+    assert np.isfinite(r["mu"])
+    r2 = e.chempot()
+    assert r2["inave"].sum() == 0 and r2["pave"].sum() == 0 and r2["seed"] == r["seed"]
