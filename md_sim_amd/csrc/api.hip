@@ -71,6 +71,7 @@ __global__ void k_obs_efipos(DevSys S, ObsArgs o);
 __global__ void k_chem_scan(DevSys S, ChemArgs c);
 __global__ void k_chem_first(DevSys S, ChemArgs c);
 __global__ void k_chem_rot(DevSys S, ChemArgs c);
+__global__ void k_chem_bsum(ChemArgs c);
 __global__ void k_chem_accum(ChemArgs c);
 __global__ void k_colsum(DevSys S, const int *lead_slot, int per_block, double *colpart);
 __global__ void k_pair_totals(DevSys S);
@@ -281,7 +282,7 @@ struct mdf_system {
     double chem_seed = 0, chem_lwmin = 0, chem_lwmax = 0, chem_facbin = 0;
     double chem_aco = 0, chem_asi = 0, chem_lo = 0;
     size_t chem_lds = 0;
-    DBuf<double> ch_ins, ch_rot, ch_uox, ch_nbr_d, ch_boltz, ch_pave;
+    DBuf<double> ch_ins, ch_rot, ch_uox, ch_nbr_d, ch_boltz, ch_bsum, ch_pave;
     DBuf<int> ch_il, ch_insrot, ch_rotins, ch_over, ch_nnbr, ch_nbr, ch_status;
     DBuf<unsigned long long> ch_inave;
     double *h_ins = nullptr, *h_rot = nullptr;   // pinned staging of one call's draws
@@ -293,7 +294,7 @@ struct mdf_system {
 };
 
 static void chem_free(mdf_system *s) {
-    DBuf<double> *db[] = {&s->ch_ins, &s->ch_rot, &s->ch_uox, &s->ch_nbr_d, &s->ch_boltz, &s->ch_pave};
+    DBuf<double> *db[] = {&s->ch_ins, &s->ch_rot, &s->ch_uox, &s->ch_nbr_d, &s->ch_boltz, &s->ch_bsum, &s->ch_pave};
     for (auto *b : db) b->release();
     DBuf<int> *ib[] = {&s->ch_il, &s->ch_insrot, &s->ch_rotins, &s->ch_over, &s->ch_nnbr, &s->ch_nbr, &s->ch_status};
     for (auto *b : ib) b->release();
@@ -873,7 +874,8 @@ static int chem_enqueue(mdf_system *s) {
     k_chem_scan<<<n, 256, 0, st>>>(s->S, c);
     k_chem_first<<<1, 256, 0, st>>>(s->S, c);
     if (nr > 0) k_chem_rot<<<nr, 256, s->chem_lds, st>>>(s->S, c);
-    k_chem_accum<<<1, 256, 0, st>>>(c);
+    k_chem_bsum<<<(n + 255) / 256, 256, 0, st>>>(c);
+    k_chem_accum<<<std::min(s->chem_p.n_bins, 1024), 256, 0, st>>>(c);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(s->h_status, s->ch_status.p, sizeof(int) * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipEventRecord(s->ch_done, st));
@@ -1940,6 +1942,7 @@ int mdf_chempot_enable(mdf_system *s, const mdf_chempot_params *p) {
     CTRY(s->ch_nbr.alloc((size_t)n * cap));
     CTRY(s->ch_nbr_d.alloc(3 * (size_t)n * cap));
     CTRY(s->ch_boltz.alloc(nrmax));
+    CTRY(s->ch_bsum.alloc(n));
     CTRY(s->ch_status.alloc(4));
     CTRY(s->ch_pave.alloc(p->n_bins));
     CTRY(s->ch_inave.alloc(p->n_bins));
@@ -1959,7 +1962,7 @@ int mdf_chempot_enable(mdf_system *s, const mdf_chempot_params *p) {
     memset(&c, 0, sizeof c);
     c.ins = s->ch_ins.p; c.ins_il = s->ch_il.p; c.ins_rot = s->ch_insrot.p; c.rot_ins = s->ch_rotins.p;
     c.rot = s->ch_rot.p; c.overlap = s->ch_over.p; c.n_nbr = s->ch_nnbr.p; c.u_ox = s->ch_uox.p;
-    c.nbr = s->ch_nbr.p; c.nbr_d = s->ch_nbr_d.p; c.boltz = s->ch_boltz.p; c.status = s->ch_status.p;
+    c.nbr = s->ch_nbr.p; c.nbr_d = s->ch_nbr_d.p; c.boltz = s->ch_boltz.p; c.bsum = s->ch_bsum.p; c.status = s->ch_status.p;
     c.pave = s->ch_pave.p; c.inave = s->ch_inave.p; c.dipz = s->dipz.p;
     c.kvec = s->d_kvec.p; c.ak = s->d_ak.p; c.sk = s->d_s.p;
     c.n_ins = n; c.cap = cap; c.n_bins = p->n_bins; c.nk = (int)s->ak.size();

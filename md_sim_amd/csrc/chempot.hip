@@ -1,6 +1,6 @@
 // Widom test-particle insertions of an SPC/E water into the step's configuration: the
 // reference's chempot (src/forcem.f90:1226-1507; identical in src/forcem_back.f90:1192-1473)
-// as four kernels over one call's insertions, whose random geometry the host draws with the
+// as five kernels over one call's insertions, whose random geometry the host draws with the
 // reference's generator (the draw sequence depends on where the call ends, so it is serial).
 //   k_chem_scan   one workgroup per insertion: the oxygen against every site but type 2
 //                 within rcut (overlap test, vdW + erfc Coulomb), partners compacted in
@@ -9,8 +9,8 @@
 //   k_chem_rot    one workgroup per orientation: hydrogen real-space terms over the partners,
 //                 0.5 A_k Re(rho_ins conj S(k)) over the k-set from e^{i k r} tables in LDS,
 //                 slab term, wall tail terms; exp(-beta u)               (:1380-1495)
-//   k_chem_accum  pave(il) += exp(-beta u) and inave(il) += irot in the reference's
-//                 insertion and orientation order                        (:1277,1495)
+//   k_chem_bsum   per insertion, its orientations' factors summed in order
+//   k_chem_accum  per bin, pave(il) += those sums and inave(il) += irot  (:1277,1495)
 // Everything in FP64 whatever the force path's precision.
 #include "mdf_internal.h"
 
@@ -213,22 +213,50 @@ __global__ __launch_bounds__(256) void k_chem_rot(DevSys S, ChemArgs c) {
     c.boltz[r] = exp(-c.beta * u);
 }
 
+// per insertion: the sum of its orientations' factors, in orientation order (0 from the
+// first overlap on)
+__global__ __launch_bounds__(256) void k_chem_bsum(ChemArgs c) {
+    const int ic = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ic >= c.n_ins || c.status[1] || c.status[2]) return;
+    double p = 0.0;
+    if (ic < c.status[0]) {
+        const int r0 = c.ins_rot[ic], r1 = c.ins_rot[ic + 1];
+        for (int r = r0; r < r1; r++) p += c.boltz[r];
+    }
+    c.bsum[ic] = p;
+}
+
+// per bin: pave += the insertions' sums, inave += their orientation counts (the overlapping
+// insertion counts, src/forcem.f90:1277); fixed assignment and tree order, so the sums are
+// the same run to run
 __global__ __launch_bounds__(256) void k_chem_accum(ChemArgs c) {
+    __shared__ double red[256];
+    __shared__ unsigned long long redn[256];
     if (c.status[1] || c.status[2]) return;
-    const int first = c.status[0];
-    const int last = min(first, c.n_ins - 1);
-    for (int b = threadIdx.x; b < c.n_bins; b += blockDim.x) {
-        double p = c.pave[b];
-        unsigned long long n = c.inave[b];
-        for (int ic = 0; ic <= last; ic++) {
+    const int last = min(c.status[0], c.n_ins - 1);
+    for (int b = blockIdx.x; b < c.n_bins; b += gridDim.x) {
+        double p = 0.0;
+        unsigned long long n = 0;
+        for (int ic = threadIdx.x; ic <= last; ic += blockDim.x) {
             if (c.ins_il[ic] - 1 != b) continue;
-            const int r0 = c.ins_rot[ic], r1 = c.ins_rot[ic + 1];
-            n += (unsigned long long)(r1 - r0);
-            if (ic < first)
-                for (int r = r0; r < r1; r++) p += c.boltz[r];
+            n += (unsigned long long)(c.ins_rot[ic + 1] - c.ins_rot[ic]);
+            p += c.bsum[ic];
         }
-        c.pave[b] = p;
-        c.inave[b] = n;
+        red[threadIdx.x] = p;
+        redn[threadIdx.x] = n;
+        __syncthreads();
+        for (int st = 128; st > 0; st >>= 1) {
+            if (threadIdx.x < st) {
+                red[threadIdx.x] += red[threadIdx.x + st];
+                redn[threadIdx.x] += redn[threadIdx.x + st];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            c.pave[b] += red[0];
+            c.inave[b] += redn[0];
+        }
+        __syncthreads();
     }
 }
 

@@ -240,6 +240,7 @@ struct ChemArgs {
     int *nbr;                   // [n_ins][cap]
     double *nbr_d;              // [n_ins][cap][3] oxygen - partner, minimum image in x, y
     double *boltz;              // [n_rot] exp(-beta u)
+    double *bsum;               // [n_ins] sum over the insertion's orientations
     int *status;                // [4]: first overlap (n_ins: none), partner overflow, skipped, max partners
     double *pave;               // [n_bins]
     unsigned long long *inave;  // [n_bins]

@@ -25,15 +25,25 @@
 !   MDF_MTS_TRAIN      full-sum training steps before the scheme starts; default 200
 !   MDF_MTS_TOL        judged relative force RMS error that triggers the fallback; default 1e-3
 !   MDF_MTS_LOG        file for one line of the scheme's counters per step
+!   MDF_CHEMPOT_EW3DC  1 | 0: the Widom insertions' slab term with the step's coefficient, or
+!                      none (the reference's chempot reads an unassigned local there);
+!                      default 1
+!
+! The hot-loop observables ngc and efipos (src/forcem.f90:347-354,388-395,493-495,930-934)
+! and, when tot.inp sets lchempot, the Widom insertions (src/forcem.f90:126,1226-1507) run
+! on the device too; each step adds their counts into measure's arrays, and the insertions'
+! generator state is mirrored into rndmod's seed.
 !----------------------------------------------------------------------------
 module force
     use nnlist
     use measure
+    use rndmod, only: seed
     use mdforce_c, only: mdf_topology, mdf_params, mdf_outputs, mdf_create, mdf_zero, mdf_destroy, &
                          mdf_error_message, MDF_OK, MDF_FP64, MDF_FP32_ACC64, MDF_CONST_REFERENCE, &
                          MDF_ERFC_AS7126, MDF_ALPHA_REFERENCE, MDF_Z_OPEN, MDF_Z_PERIODIC_OPEN_LIST, &
                          mdf_mts_params, mdf_mts_stats, mdf_mts_enable, mdf_mts_get_stats, MDF_MTS_HELD, &
-                         MDF_MTS_NETWORK
+                         MDF_MTS_NETWORK, mdf_observables_enable, mdf_get_observables, mdf_chempot_params, &
+                         mdf_chempot_enable, mdf_get_chempot
     use, intrinsic :: iso_c_binding
     implicit none
 
@@ -58,6 +68,10 @@ module force
     real(c_double), allocatable, target, private :: c_q(:), c_eps4(:), c_sig6(:), c_a(:), c_rhoi(:), c_c(:)
     real(c_double), allocatable, target, private :: c_conl(:), c_angkt(:), c_angth(:)
     real(c_double), allocatable, target, private :: o_fsite(:), o_fcom(:), o_tq(:), o_evdw(:), o_ecoul(:)
+    ! one step's observables and insertions, fetched from the device
+    integer(c_int64_t), allocatable, target, private :: o_ngc(:), o_inave(:)
+    real(c_double), allocatable, target, private :: o_efipos(:), o_pave(:)
+    integer(c_int32_t), private :: o_nbins = 0
 
     ! mdf_compute with the positions passed as arrays (no TARGET needed on the driver's arrays)
     interface
@@ -229,9 +243,75 @@ module force
             stop 'force_init(): force engine could not be created'
         end if
         call mts_init
+        call observables_init
         totpres = 0.0
         call force_zero
     end subroutine force_init
+
+    ! the hot-loop observables with the driver's bin width, and the Widom insertions when
+    ! tot.inp asks for them (its values as read: measure_init converts lwmin/lwmax later)
+    subroutine observables_init
+        type(mdf_chempot_params) :: cp
+        integer(c_int) :: rc
+        integer :: ios, ew
+        character(len=512) :: v
+        rc = mdf_observables_enable(sys, real(rdel, c_double))
+        if (rc /= MDF_OK) then
+            print *, 'force_init(): ', trim(mdf_error_message())
+            stop 'force_init(): observables could not be set up'
+        end if
+        rc = mdf_get_observables(sys, c_null_ptr, c_null_ptr, o_nbins, 0_c_int32_t)
+        allocate(o_ngc(max(1, o_nbins*int(nijt))), o_efipos(max(1, 2*o_nbins)))
+        if (.not. lchempot) return
+        ew = 1
+        v = env_value('MDF_CHEMPOT_EW3DC')
+        if (len_trim(v) > 0) read(v, *, iostat=ios) ew
+        cp%n_insert = int(ichmax, c_int)
+        cp%n_bins = int(lchmax, c_int)
+        cp%bin_mid = int(ilmid, c_int)
+        cp%ew3dc = int(ew, c_int)
+        cp%wall_min = lwmin
+        cp%wall_max = lwmax
+        cp%temperature = temper
+        cp%seed = 0.0d0
+        cp%max_partners = 0
+        cp%pad = 0
+        rc = mdf_chempot_enable(sys, cp)
+        if (rc /= MDF_OK) then
+            print *, 'force_init(): ', trim(mdf_error_message())
+            stop 'force_init(): Widom insertions could not be set up'
+        end if
+        allocate(o_pave(max(1_8, lchmax)), o_inave(max(1_8, lchmax)))
+    end subroutine observables_init
+
+    ! add the step's ngc/efipos counts, and its insertions, into measure's arrays (the
+    ! reference adds them inside its loops and chempot, src/forcem.f90:349-354,495,930-934,
+    ! 1277,1495-1506)
+    subroutine observables_step
+        integer(c_int) :: rc
+        integer*8 :: nb
+        real(c_double) :: sd
+        rc = mdf_get_observables(sys, c_loc(o_ngc), c_loc(o_efipos), o_nbins, 1_c_int32_t)
+        if (rc /= MDF_OK) then
+            print *, 'force_get(): ', trim(mdf_error_message())
+            stop 'force_get(): observables could not be read'
+        end if
+        nb = min(int(o_nbins, 8), int(size(ngc, 2), 8))
+        ngc(:, 0:nb-1) = ngc(:, 0:nb-1) + reshape(o_ngc(1:nijt*nb), (/ int(nijt), int(nb) /))
+        nb = min(int(o_nbins, 8), int(size(efipos, 2), 8))
+        efipos(:, 0:nb-1) = efipos(:, 0:nb-1) + reshape(o_efipos(1:2*nb), (/ 2, int(nb) /))
+        if (.not. lchempot) return
+        rc = mdf_get_chempot(sys, c_loc(o_pave), c_loc(o_inave), sd, 1_c_int32_t)
+        if (rc /= MDF_OK) then
+            print *, 'force_get(): ', trim(mdf_error_message())
+            stop 'force_get(): Widom insertions failed'
+        end if
+        pave(1:lchmax) = pave(1:lchmax) + o_pave(1:lchmax)
+        inave(1:lchmax) = inave(1:lchmax) + o_inave(1:lchmax)
+        pavesum(1:lchmax) = pave(1:lchmax)
+        inavesum(1:lchmax) = inave(1:lchmax)
+        seed = sd
+    end subroutine observables_step
 
     ! multiple time steps for the k-space force, when MDF_MTS asks for them
     subroutine mts_init
@@ -283,7 +363,6 @@ module force
         integer(c_int) :: rc
         integer*8 :: i, it, l, m
 
-        if (lchempot) stop 'force_get(): the chemical-potential insertion is not provided by the engine'
         rc = mdf_compute_arr(sys, x0, y0, z0, cx0, cy0, cz0, out)
         if (rc /= MDF_OK) then
             print *, 'force_get(): ', trim(mdf_error_message())
@@ -324,6 +403,7 @@ module force
                 tefipos(it - 2, l) = tefipos(it - 2, l) + ftz(i)
             end do
         end if
+        call observables_step
         if (mts_log > 0) call mts_log_line
     end subroutine force_get
 

@@ -47,6 +47,12 @@ module mdforce_c
         integer(c_int32_t) :: seed
     end type mdf_mts_params
 
+    type, bind(C) :: mdf_chempot_params
+        integer(c_int) :: n_insert, n_bins, bin_mid, ew3dc
+        real(c_double) :: wall_min, wall_max, temperature, seed
+        integer(c_int) :: max_partners, pad
+    end type mdf_chempot_params
+
     type, bind(C) :: mdf_mts_stats
         integer(c_int) :: state, last_exact
         integer(c_int64_t) :: n_exact, n_surrogate, n_fallback, n_judged, adam_steps
@@ -99,6 +105,33 @@ module mdforce_c
             type(c_ptr), value :: sys
             type(mdf_mts_stats), intent(out) :: st
         end function mdf_mts_get_stats
+
+        integer(c_int) function mdf_observables_enable(sys, rdel) bind(C, name='mdf_observables_enable')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: sys
+            real(c_double), value :: rdel
+        end function mdf_observables_enable
+
+        integer(c_int) function mdf_get_observables(sys, ngc, efipos, nbins, reset) &
+                bind(C, name='mdf_get_observables')
+            import :: c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: sys, ngc, efipos
+            integer(c_int32_t), intent(out) :: nbins
+            integer(c_int32_t), value :: reset
+        end function mdf_get_observables
+
+        integer(c_int) function mdf_chempot_enable(sys, p) bind(C, name='mdf_chempot_enable')
+            import :: c_int, c_ptr, mdf_chempot_params
+            type(c_ptr), value :: sys
+            type(mdf_chempot_params), intent(in) :: p
+        end function mdf_chempot_enable
+
+        integer(c_int) function mdf_get_chempot(sys, pave, inave, seed, reset) bind(C, name='mdf_get_chempot')
+            import :: c_int, c_ptr, c_double, c_int32_t
+            type(c_ptr), value :: sys, pave, inave
+            real(c_double), intent(out) :: seed
+            integer(c_int32_t), value :: reset
+        end function mdf_get_chempot
 
         subroutine mdf_destroy(sys) bind(C, name='mdf_destroy')
             import :: c_ptr

@@ -10,6 +10,9 @@ FETCH_SIZE / WRITE_SIZE to the true size scales the counters of the engine's ker
 ``--mts N`` runs the k-space surrogate with the full sum every N-th step instead, on a
 librating water trajectory: ``--mts-train`` full-sum training steps, then ``--steps`` steps
 of the scheme.
+
+``--chempot "N WMIN WMAX BINS MID"`` adds the Widom insertions after every step (tot.inp's
+ichmax, lwmin, lwmax, lchmax, ilmid).
 """
 from __future__ import annotations
 
@@ -27,6 +30,7 @@ def main(argv=None):
     ap.add_argument("--mts", type=int, default=0, help="full k-sum every N-th step with the surrogate between")
     ap.add_argument("--mts-model", choices=["network", "held"], default="network")
     ap.add_argument("--mts-train", type=int, default=8)
+    ap.add_argument("--chempot", default="", help="Widom insertions: 'ichmax lwmin lwmax lchmax ilmid'")
     a = ap.parse_args(argv)
     os.environ["MDF_GRAPHS"] = "0"
     import torch
@@ -44,6 +48,10 @@ def main(argv=None):
         eng.mts_enable(interval=a.mts, model=a.mts_model, train_frames=a.mts_train, tol=0.5)
     else:
         frames = [pos] * a.steps
+    if a.chempot:
+        v = a.chempot.split()
+        eng.chempot_enable(n_insert=int(v[0]), wall_min=float(v[1]), wall_max=float(v[2]), n_bins=int(v[3]),
+                           bin_mid=int(v[4]))
     devf = [[torch.tensor(v[:s.nmove] if k < 3 else v, dtype=torch.float64, device="cuda:0")
              for k, v in enumerate(f)] for f in frames]
     for d in devf:
@@ -51,6 +59,9 @@ def main(argv=None):
     r = eng.read_outputs()
     if a.mts > 1:
         print("scheme: %s" % eng.mts_stats(), flush=True)
+    if a.chempot:
+        c = eng.chempot()
+        print("insertions: %d orientations, mu %.4f kJ/mol" % (int(c["inave"].sum()), c["mu"]), flush=True)
     print("steps done: e_conf %.6f" % r["e_conf"], flush=True)
     n = a.copy_mib * (1 << 20) // 4
     src = torch.ones(n, dtype=torch.float32, device="cuda:0")

@@ -9,16 +9,22 @@
 ! module (md_sim_amd/fortran/force_mdf.f90), so the two runs differ only in the force
 ! evaluation.  Writes one line per step: step, htot, uconf, ke, tempnow, ulr (the wall
 ! tail energy, which the reference keeps out of uconf and htot).
+! Optional: REF_CHEMPOT="ichmax lwmin lwmax lchmax ilmid" switches the Widom insertions on
+! (tot.inp's chempot lines, src/main.f90:148-152); MD_NVE_OBS=<file> writes the measurement
+! arrays the force path fills (ngc, efipos; pave, inave and the generator state with
+! insertions) after the last step.
 ! Usage: md_nve <output> <nsteps>   (harness.inp and the system files in the cwd)
 !----------------------------------------------------------------------------
 program md_nve
     use move
+    use measure
+    use rndmod
     implicit none
     include 'mpif.h'
     integer :: ierr, u
     integer*8 :: n, nsteps, i, ix
     character*40 :: base
-    character*256 :: outname, arg2
+    character*256 :: outname, arg2, chemenv, obsname
 
     nproc = 1
     me = 0
@@ -44,6 +50,11 @@ program md_nve
     lcouett = .false.
     lfext = .false.
     lchempot = .false.
+    call get_environment_variable('REF_CHEMPOT', chemenv)
+    if (len_trim(chemenv) > 0) then
+        read(chemenv, *) ichmax, lwmin, lwmax, lchmax, ilmid
+        lchempot = .true.
+    end if
     dshx = 0.0 ; dshy = 0.0 ; dshz = 0.0
     n = len_trim(base)
     filfld = base(1:n)//'.fld'
@@ -72,5 +83,22 @@ program md_nve
         write(u, '(i8,5(1x,es24.16))') itime, htot, uconf, ke, tempnow, ulr
     end do
     close(u)
+    call get_environment_variable('MD_NVE_OBS', obsname)
+    if (len_trim(obsname) > 0) then
+        open(12, file=trim(obsname), status='replace')
+        write(12, '(a,i12)') 'ngc ', size(ngc, kind=8)
+        write(12, '(i12)') ngc
+        write(12, '(a,i12)') 'efipos ', size(efipos, kind=8)
+        write(12, '(es26.17)') efipos
+        if (lchempot) then
+            write(12, '(a,i12)') 'pave ', lchmax
+            write(12, '(es26.17)') pave(1:lchmax)
+            write(12, '(a,i12)') 'inave ', lchmax
+            write(12, '(i12)') inave(1:lchmax)
+            write(12, '(a,i12)') 'seed ', 1_8
+            write(12, '(es26.17)') seed
+        end if
+        close(12)
+    end if
     call mpi_finalize(ierr)
 end program md_nve

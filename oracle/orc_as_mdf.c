@@ -45,3 +45,34 @@ int mdf_mts_get_stats(mdf_system *sys, mdf_mts_stats *out) {
     (void)out;
     return MDF_ERR_ARG;
 }
+
+/* the CPU statement has no observables or insertions: it reports empty histograms and
+   refuses insertions */
+int mdf_observables_enable(mdf_system *sys, double rdel) {
+    (void)sys;
+    (void)rdel;
+    return MDF_OK;
+}
+
+int mdf_get_observables(mdf_system *sys, int64_t *ngc, double *efipos, int32_t *nbins, int32_t reset) {
+    (void)sys;
+    (void)ngc;
+    (void)efipos;
+    (void)reset;
+    if (nbins) *nbins = 0;
+    return MDF_OK;
+}
+
+int mdf_chempot_enable(mdf_system *sys, const mdf_chempot_params *p) {
+    (void)sys;
+    return p ? MDF_ERR_ARG : MDF_OK;
+}
+
+int mdf_get_chempot(mdf_system *sys, double *pave, int64_t *inave, double *seed, int32_t reset) {
+    (void)sys;
+    (void)pave;
+    (void)inave;
+    (void)seed;
+    (void)reset;
+    return MDF_ERR_ARG;
+}

@@ -81,11 +81,33 @@ def run_reference(system, workdir: str = None, timeout: float = 600.0, nrep: int
     return dump
 
 
+def parse_nve_obs(path: str) -> Dict[str, np.ndarray]:
+    """The measurement arrays md_nve writes with MD_NVE_OBS: name/count headers, one value a
+    line (ngc and inave as int64, the rest float64; NaN kept)."""
+    out = {}
+    lines = open(path).read().split("\n")
+    i = 0
+    while i < len(lines):
+        head = lines[i].split()
+        i += 1
+        if not head:
+            continue
+        name, n = head[0], int(head[1])
+        vals = [v.strip() for v in lines[i:i + n]]
+        i += n
+        if name in ("ngc", "inave"):
+            out[name] = np.array([int(v) for v in vals], np.int64)
+        else:
+            out[name] = np.array([float(v) for v in vals], np.float64)
+    return out
+
+
 def run_nve(system, nsteps: int, variant: str = None, env=None, workdir: str = None,
-            timeout: float = 600.0) -> np.ndarray:
+            timeout: float = 600.0, obs: bool = False):
     """Run an NVE trajectory driver (oracle/md_nve.f90) on `system` for `nsteps` steps of
     system.dt: variant 'slab'/'bulk' is the reference's own force module, 'gpu' the engine's
-    drop-in module.  Returns (nsteps, 6): step, htot, uconf, ke, tempnow, ulr."""
+    drop-in module.  Returns (nsteps, 6): step, htot, uconf, ke, tempnow, ulr; with obs also
+    the measurement arrays after the last step (parse_nve_obs)."""
     from md_sim_amd.refio import write_reference_inputs
 
     if variant is None:
@@ -100,11 +122,14 @@ def run_nve(system, nsteps: int, variant: str = None, env=None, workdir: str = N
     e = dict(os.environ)
     if env:
         e.update(env)
+    if obs:
+        e["MD_NVE_OBS"] = "obs.txt"
     r = subprocess.run([exe, out, str(int(nsteps))], cwd=d, capture_output=True, text=True, timeout=timeout,
                        env=e)
     if r.returncode != 0 or not os.path.exists(out):
         raise RuntimeError("NVE driver failed (%d):\n%s\n%s" % (r.returncode, r.stdout[-2000:], r.stderr[-2000:]))
     t = np.loadtxt(out, ndmin=2)
+    o = parse_nve_obs(os.path.join(d, "obs.txt")) if obs else None
     if own:
         shutil.rmtree(d, ignore_errors=True)
-    return t
+    return (t, o) if obs else t

@@ -14,6 +14,8 @@ stores its inputs and outputs:
                    insertion windows clear of the water layer, and reaching into it), with
                    cumulative pave/inave and the generator state after each call, the wall
                    tail tables, and the state the insertions read (positions, k-set, S(k));
+                   and (dropin__*) the measurement arrays after one step of the reference's
+                   time loop with its own force module and insertions on;
 * sample_rutile.npz - the reference's own sample input (src/sim/y.*, surface OH groups with
                    constraints and angles on rutile): input text, the derived topology the
                    reference builds from it, and its outputs.
@@ -110,6 +112,16 @@ def chempot_cases():
         out[case + "__system"] = np.array(name)
         for k in CHEMPOT_KEYS:
             out[case + "__" + k] = d[k]
+    # the reference's time loop with its own force module for one step (oracle/md_nve.f90;
+    # from step 2 on, the reference's force_sum has overwritten molecule 1's lead offset,
+    # src/forcem.f90:1006-1008, so the trajectories part): the measurement arrays its force
+    # path fills, for the engine's drop-in module to match
+    from oracle.refrun import run_nve
+    spec = "30 1.0 9.5 10 5"
+    _, o = run_nve(systems.config("pore_sparse"), 1, variant="slab", obs=True, env={"REF_CHEMPOT": spec})
+    out["dropin__spec"] = np.array([float(v) for v in spec.split()] + [1.0])
+    for k, v in o.items():
+        out["dropin__" + k] = v
     path = os.path.join(HERE, "chempot.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, os.path.getsize(path), "bytes")

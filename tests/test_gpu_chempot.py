@@ -137,3 +137,32 @@ def test_reset_and_chemical_potential():
     assert np.isfinite(r["mu"])
     r2 = e.chempot()
     assert r2["inave"].sum() == 0 and r2["pave"].sum() == 0 and r2["seed"] == r["seed"]
+
+
+def test_dropin_fills_the_measurement_arrays_as_the_reference():
+    # the reference's time loop (oracle/md_nve.f90) for one step with insertions on, once
+    # with its own force module (the fixture) and once with the engine's drop-in module: ngc,
+    # efipos, inave and the generator state as the reference's, pave finite exactly where the
+    # reference's is NaN.  (Later steps part: the reference's force_sum overwrites molecule
+    # 1's lead offset after the first step, src/forcem.f90:1006-1008.)  A second run checks
+    # that the drop-in keeps adding the steps' counts into measure's arrays.
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    if not os.access(os.path.join(root, "oracle", "_ref", "md_nve_gpu"), os.X_OK):
+        pytest.skip("NVE driver with the engine's force module not built (oracle/build_ref.sh)")
+    from oracle.refrun import run_nve
+    _say("case chempot drop-in")
+    g = np.load(os.path.join(GOLDEN, "chempot.npz"), allow_pickle=False)
+    sp = g["dropin__spec"]
+    spec = "%d %r %r %d %d" % (int(sp[0]), float(sp[1]), float(sp[2]), int(sp[3]), int(sp[4]))
+    _, o = run_nve(systems.config("pore_sparse"), int(sp[5]), variant="gpu", obs=True,
+                   env={"REF_CHEMPOT": spec, "MDF_CHEMPOT_EW3DC": "0"})
+    assert np.array_equal(o["ngc"], g["dropin__ngc"]) and g["dropin__ngc"].sum() > 0
+    ref = g["dropin__efipos"]
+    assert np.abs(o["efipos"] - ref).max() <= 1e-9 * np.abs(ref).max()
+    assert np.array_equal(o["inave"], g["dropin__inave"])
+    assert o["seed"][0] == g["dropin__seed"][0]
+    assert np.array_equal(np.isnan(g["dropin__pave"]), o["pave"] > 0)
+    _, o3 = run_nve(systems.config("pore_sparse"), 3, variant="gpu", obs=True,
+                    env={"REF_CHEMPOT": spec, "MDF_CHEMPOT_EW3DC": "0"})
+    assert o3["ngc"].sum() > 2.5 * o["ngc"].sum() and np.all(o3["inave"] >= o["inave"])
+    assert o3["inave"].sum() > o["inave"].sum() and o3["seed"][0] != o["seed"][0]
