@@ -66,6 +66,12 @@ constexpr int BP_NS = 4;   // sites per back-projection thread
 __global__ void k_epilogue(DevSys S, const double *fpart, int n_groups, const WallLR *wl, int ew3dc, double crf0,
                            const double *dipz, double *epart, double *fcom, double *tq, const MtsState *mts,
                            const double *fmts, double *efi);
+__global__ void k_sf_mma(const double *x, const double *y, const double *z, const double *q, int n, double cux,
+                         double cuy, double cuz, const KRow *rows, int n_rows, int knmax, int n_nt, double2 *partial,
+                         int nk, const MtsState *mts);
+constexpr int SFM_SITES = 512;   // sites per k_sf_mma block
+constexpr int SFM_ROWS_PER_BLOCK = 64;
+constexpr int SFM_TILES_MAX = 8; // kn tiles of 16 (knmax <= 63)
 __global__ void k_obs_ngc(DevSys S, ObsArgs o);
 __global__ void k_obs_efipos(DevSys S, ObsArgs o);
 __global__ void k_chem_scan(DevSys S, ChemArgs c);
@@ -227,6 +233,9 @@ struct mdf_system {
     DBuf<WallLR> d_wl;
     DBuf<Scalars> d_scal;
     // launch geometry
+    // the FP32 form's structure factor on the matrix cores (k_sf_mma)
+    bool sf_mma = false, sf_mma_allowed = true;
+    int n_sb_mma = 1, n_rg_mma = 1, knmax = 0, n_nt = 0;
     int skin_blocks = 1, n_sb = 1, n_cg = 1, cpb = 1, n_kblk = 1, n_rg = 1, rpg = 1, n_epblk = 1, n_colblk = 1,
         colper = 1, ncells_xy = 1, n_cells_cap = 1;
     // timing: a ring of event sets, one per in-flight step; stage durations are accumulated
@@ -454,6 +463,12 @@ static void geometry(mdf_system *s) {
     s->n_cg = std::max(1, (nrow + s->cpb - 1) / s->cpb);
     int nk = (int)s->ak.size();
     s->n_kblk = std::max(1, (nk + SFR_K - 1) / SFR_K);
+    s->knmax = 0;
+    for (size_t k = 2; k < s->kvec.size(); k += 3) s->knmax = std::max(s->knmax, std::abs(s->kvec[k]));
+    s->n_nt = (2 * s->knmax + 1 + 15) / 16;
+    s->sf_mma = s->sf_mma_allowed && s->par.precision == MDF_FP32_ACC64 && nk > 0 && s->n_nt <= SFM_TILES_MAX;
+    s->n_sb_mma = std::max(1, (nm + SFM_SITES - 1) / SFM_SITES);
+    s->n_rg_mma = std::max(1, (nrow + SFM_ROWS_PER_BLOCK - 1) / SFM_ROWS_PER_BLOCK);
     // back-projection row groups: about equal k counts, each at most BP_GROUP_K k-vectors (the
     // LDS stage), and enough groups that sites x groups fill the GPU
     int nsb_bp = std::max(1, (nm + 256 * BP_NS - 1) / (256 * BP_NS));
@@ -607,6 +622,12 @@ static void launch_sf(mdf_system *s) {
     if (nk == 0 || nm == 0) return;
     const RecipArgs a = recip_args(s);
     DevSys &S = s->S;
+    if (s->sf_mma) {
+        k_sf_mma<<<dim3(s->n_sb_mma, s->n_rg_mma), 256, 0, s->stream>>>(S.x, S.y, S.z, S.q, nm, a.cux, a.cuy, a.cuz,
+                                                                       s->d_rows.p, nrows, s->knmax, s->n_nt,
+                                                                       s->sf_partial.p, nk, mts_state(s));
+        return;
+    }
     dim3 g1(s->n_sb, s->n_cg);
     if (s->par.precision == MDF_FP32_ACC64)
         k_sf<float, 4><<<g1, SF_BLOCK, 0, s->stream>>>(S.x, S.y, S.z, S.q, nm, a.cux, a.cuy, a.cuz, s->d_rows.p,
@@ -680,7 +701,8 @@ static void launch_forces(mdf_system *s) {
     if (s->ev) (void)hipEventRecord(s->ev[4], st);
     if (nk > 0) {
         double lamb = 1.0 / (4.0 * S.alfa * S.alfa);
-        k_sf_reduce<<<s->n_kblk, 256, 0, st>>>(s->sf_partial.p, nm > 0 ? s->n_sb : 0, nk, s->d_swall.p, s->d_ak.p,
+        k_sf_reduce<<<s->n_kblk, 256, 0, st>>>(s->sf_partial.p, nm > 0 ? (s->sf_mma ? s->n_sb_mma : s->n_sb) : 0, nk,
+                                               s->d_swall.p, s->d_ak.p,
                                                s->d_kvec.p, ra.pipix, ra.pipiy, ra.pipiz, lamb, s->d_s.p, s->d_t.p,
                                                s->kpart.p, mts_state(s));
     }
@@ -1031,6 +1053,8 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
         if (g && g[0] == '0') s->use_graphs = false;
         const char *pr = getenv("MDF_PAIR_ROLLED");
         if (pr && pr[0] == '1') s->pair_rolled = true;
+        const char *sm = getenv("MDF_SF_MMA");
+        if (sm && sm[0] == '0') s->sf_mma_allowed = false;
         const char *sb = getenv("MDF_SF_BLOCKS");
         if (sb && atoi(sb) > 0) s->sf_target_blocks = atoi(sb);
         const char *bb = getenv("MDF_BP_BLOCKS");
@@ -1309,7 +1333,7 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
             int nw = s->n_wall;
             int per_sb = SF_BLOCK * 4;
             int n_sb_w = std::max(1, (nw + per_sb - 1) / per_sb);
-            HTRY(s->sf_partial.alloc((size_t)std::max(s->n_sb, n_sb_w) * std::max(nk, 1)));
+            HTRY(s->sf_partial.alloc((size_t)std::max(std::max(s->n_sb, s->n_sb_mma), n_sb_w) * std::max(nk, 1)));
             HTRY(s->fpart.alloc((size_t)s->n_rg * 3 * std::max(t.n_move, 1)));
             s->s_wall.assign(2 * std::max(nk, 1), 0.0);
             s->ewqWW = 0.0;

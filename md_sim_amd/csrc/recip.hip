@@ -187,6 +187,202 @@ __device__ __forceinline__ void sf_body(const double *__restrict__ x, const doub
     }
 }
 
+// ---- S(k) on the matrix cores (FP32 form) ------------------------------------------------
+// For a row (kl, km) and kn, S = sum_j P_j E_j with P_j = q_j exp(i pi (kl u_j + km v_j)) and
+// E_j = exp(i pi kn w_j): over a block of sites that is a complex product of a (rows x sites)
+// and a (sites x kn) matrix.  As real operands, A = [Pr Pi] per site pair of k entries and
+// B = [Er -Ei] (real part) or [Ei Er] (imaginary part).  Each FP32 operand is split into two
+// FP16 parts, v = hi + lo, and A B ~ A_hi B_hi + A_hi B_lo + A_lo B_hi (the dropped lo lo
+// term is 2^-22 relative), accumulated in FP32 by v_mfma_f32_16x16x16_f16 over a 512-site
+// block and written as FP64 partials - the same partial layout k_sf_reduce sums.
+// A block takes 512 sites and 64 rows (one 16-row tile per wave); per 32-site stage all
+// threads build the P and E tables (split, in LDS) by recursion along km and kn, then each
+// wave runs its tile over the kn tiles its rows reach.
+typedef _Float16 h16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int SFM_SB = 512;                  // sites per block (one partial row)
+constexpr int SFM_SUB = 32;                  // sites per LDS stage
+constexpr int SFM_ROWS = 64;                 // rows per block: 4 waves x 16
+constexpr int SFM_NT_MAX = 8;                // kn tiles of 16 the E table holds (knmax <= 63)
+// tables are [site][kn] and [site][row]; strides = 8 mod 16 put the two site pairs a half
+// wave reads in one step 16 banks apart
+constexpr int SFM_KP = SFM_NT_MAX * 16 + 8;
+constexpr int SFM_RP = SFM_ROWS + 8;
+
+// (re, im) -> packed FP16 pairs hi and lo with re + i im = hi + lo to about 21 bits: hi
+// rounds toward zero (v_cvt_pkrtz), so re - hi is exact in FP32 and lo carries it
+typedef __fp16 h16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void split_c(float re, float im, unsigned &hi, unsigned &lo) {
+    const h16x2 h = __builtin_amdgcn_cvt_pkrtz(re, im);
+    const h16x2 l = __builtin_amdgcn_cvt_pkrtz(re - (float)h[0], im - (float)h[1]);
+    hi = __builtin_bit_cast(unsigned, h);
+    lo = __builtin_bit_cast(unsigned, l);
+}
+
+__device__ __forceinline__ h16x4 frag2(unsigned a, unsigned b) {
+    uint2 v = make_uint2(a, b);
+    return __builtin_bit_cast(h16x4, v);
+}
+__device__ __forceinline__ unsigned neg_im(unsigned c) { return c ^ 0x80000000u; }          // (re, -im)
+__device__ __forceinline__ unsigned swap_c(unsigned c) { return (c >> 16) | (c << 16); }    // (im, re)
+
+__device__ __forceinline__ f32x4 mfma16h(h16x4 a, h16x4 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ int wave_min(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_sf_mma(const double *__restrict__ x, const double *__restrict__ y,
+                                                const double *__restrict__ z, const double *__restrict__ q, int n,
+                                                double cux, double cuy, double cuz, const KRow *__restrict__ rows,
+                                                int n_rows, int knmax, int n_nt, double2 *__restrict__ partial,
+                                                int nk, const MtsState *__restrict__ mts) {
+    if (mts && !mts->exact) return;   // a surrogate step: no k-sum
+    __shared__ unsigned sEh[SFM_SUB][SFM_KP], sEl[SFM_SUB][SFM_KP];
+    __shared__ unsigned sPh[SFM_SUB][SFM_RP], sPl[SFM_SUB][SFM_RP];
+    __shared__ double sU[SFM_SB], sV[SFM_SB], sW[SFM_SB];   // site phases in half turns
+    __shared__ float4 sStep[SFM_SB];                        // cos, sin of pi v and of pi w
+    __shared__ float sQ[SFM_SB];
+    __shared__ int sKl[SFM_ROWS], sKm[SFM_ROWS], sFresh[SFM_ROWS];
+    const int sb = blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row0 = blockIdx.y * SFM_ROWS;
+    const int wrow0 = row0 + wave * 16;
+    const bool active = wrow0 < n_rows;   // uniform over the wave
+    const int s_begin = sb * SFM_SB, s_end = min(n, s_begin + SFM_SB);
+    // prologue: the block's sites and rows, once
+    for (int t = threadIdx.x; t < SFM_SB; t += blockDim.x) {
+        const int i = s_begin + t;
+        const bool ok = i < s_end;
+        const double u = ok ? x[i] * cux : 0.0, v = ok ? y[i] * cuy : 0.0, w = ok ? z[i] * cuz : 0.0;
+        float vs, vc, ws, wc;
+        sincospi_r<float>(v, vs, vc);
+        sincospi_r<float>(w, ws, wc);
+        sU[t] = u; sV[t] = v; sW[t] = w;
+        sStep[t] = make_float4(vc, vs, wc, ws);
+        sQ[t] = ok ? (float)q[i] : 0.f;
+    }
+    if (threadIdx.x < SFM_ROWS) {
+        const int r = row0 + threadIdx.x;
+        int kl = 0, km = 0, fresh = 1;
+        if (r < n_rows) {
+            const KRow row = rows[r];
+            kl = row.kl; km = row.km;
+            if ((threadIdx.x & 7) != 0) {   // a P segment is 8 rows: its first row always starts fresh
+                const KRow pr = rows[r - 1];
+                fresh = (row.kl != pr.kl || row.km != pr.km + 1) ? 1 : 0;
+            }
+        }
+        sKl[threadIdx.x] = kl; sKm[threadIdx.x] = km; sFresh[threadIdx.x] = fresh;
+    }
+    int lo = 1 << 30, hi = -(1 << 30);
+    if (lane < 16 && wrow0 + lane < n_rows) {
+        const KRow r = rows[wrow0 + lane];
+        lo = r.kn0;
+        hi = r.kn1;
+    }
+    lo = wave_min(lo);
+    hi = wave_max(hi);
+    const int nt_lo = active ? (lo + knmax) >> 4 : 1, nt_hi = active ? (hi + knmax) >> 4 : 0;
+    f32x4 accr[SFM_NT_MAX], acci[SFM_NT_MAX];
+#pragma unroll
+    for (int t = 0; t < SFM_NT_MAX; t++) {
+        accr[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+        acci[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    }
+    __syncthreads();
+    // table roles: site gj of the stage, segment gs of 8 (kn for E, rows for P)
+    const int gj = threadIdx.x % SFM_SUB, gs = threadIdx.x / SFM_SUB;
+    const int per_e = 2 * n_nt;   // n_nt * 16 kn over 8 segments
+    for (int c0 = 0; c0 < s_end - s_begin; c0 += SFM_SUB) {
+        const int c = c0 + gj;
+        const float4 st = sStep[c];
+        {   // E = exp(i pi kn w), kn = -knmax + e
+            float es, ec;
+            const int e0 = gs * per_e;
+            sincospi_r<float>((double)(e0 - knmax) * sW[c], es, ec);
+            for (int e = 0; e < per_e; e++) {
+                unsigned h, l;
+                split_c(ec, es, h, l);
+                sEh[gj][e0 + e] = h;
+                sEl[gj][e0 + e] = l;
+                const float nr = ec * st.z - es * st.w, ni = ec * st.w + es * st.z;
+                ec = nr; es = ni;
+            }
+        }
+        {   // P = q exp(i pi (kl u + km v)) for the block's rows gs*8 .. gs*8+7
+            const float qq = sQ[c];
+            const double u = sU[c], v = sV[c];
+            float pr = 0.f, pi_ = 0.f;
+#pragma unroll
+            for (int rr = 0; rr < 8; rr++) {
+                const int lr = gs * 8 + rr;
+                if (sFresh[lr]) {
+                    float s0, cc;
+                    sincospi_r<float>((double)sKl[lr] * u + (double)sKm[lr] * v, s0, cc);
+                    pr = cc; pi_ = s0;
+                } else {
+                    const float nr = pr * st.x - pi_ * st.y, ni = pr * st.y + pi_ * st.x;
+                    pr = nr; pi_ = ni;
+                }
+                unsigned h, l;
+                split_c(qq * pr, qq * pi_, h, l);
+                sPh[gj][lr] = h;
+                sPl[gj][lr] = l;
+            }
+        }
+        __syncthreads();
+        if (active) {
+            const int ar = wave * 16 + (lane & 15), g2 = 2 * (lane >> 4);
+#pragma unroll 1
+            for (int ks = 0; ks < SFM_SUB / 8; ks++) {
+                const int sa = ks * 8 + g2;
+                const h16x4 Ah = frag2(sPh[sa][ar], sPh[sa + 1][ar]);
+                const h16x4 Al = frag2(sPl[sa][ar], sPl[sa + 1][ar]);
+#pragma unroll
+                for (int t = 0; t < SFM_NT_MAX; t++) {
+                    if (t < nt_lo || t > nt_hi) continue;
+                    const int ec = t * 16 + (lane & 15);
+                    const unsigned h0 = sEh[sa][ec], h1 = sEh[sa + 1][ec];
+                    const unsigned l0 = sEl[sa][ec], l1 = sEl[sa + 1][ec];
+                    const h16x4 Brh = frag2(neg_im(h0), neg_im(h1)), Brl = frag2(neg_im(l0), neg_im(l1));
+                    const h16x4 Bih = frag2(swap_c(h0), swap_c(h1)), Bil = frag2(swap_c(l0), swap_c(l1));
+                    accr[t] = mfma16h(Ah, Brh, accr[t]);
+                    acci[t] = mfma16h(Ah, Bih, acci[t]);
+                    accr[t] = mfma16h(Ah, Brl, accr[t]);
+                    acci[t] = mfma16h(Ah, Bil, acci[t]);
+                    accr[t] = mfma16h(Al, Brh, accr[t]);
+                    acci[t] = mfma16h(Al, Bih, acci[t]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (!active) return;
+    // lane holds rows 4 (lane/16) + e of the tile at kn column lane % 16
+#pragma unroll
+    for (int t = 0; t < SFM_NT_MAX; t++) {
+        if (t < nt_lo || t > nt_hi) continue;
+        const int kn = t * 16 + (lane & 15) - knmax;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int r = wrow0 + 4 * (lane >> 4) + e;
+            if (r >= n_rows) continue;
+            const KRow row = rows[r];
+            if (kn < row.kn0 || kn > row.kn1) continue;
+            partial[(size_t)sb * nk + row.koff + (kn - row.kn0)] = make_double2((double)accr[t][e], (double)acci[t][e]);
+        }
+    }
+}
+
 template <typename R, int NS>
 __global__ __launch_bounds__(SF_BLOCK) void k_sf(const double *__restrict__ x, const double *__restrict__ y,
                                                  const double *__restrict__ z, const double *__restrict__ q, int n,

@@ -54,6 +54,32 @@ def test_fp32_force_error_bound(oracle_lib, name):
     assert abs(r["e_conf"] - ro["e_conf"]) < 1e-4 * abs(ro["e_recip"]) + 1e-4 * abs(ro["e_conf"])
 
 
+@pytest.mark.parametrize("name", ["slab2048", "slab512", "bulk216"])
+def test_fp32_structure_factor_on_matrix_cores(name, monkeypatch):
+    """The FP32 form's S(k) (FP16 hi/lo split products on the matrix cores, FP32 block sums,
+    FP64 across blocks) against the FP64 form's, and against the FP32 vector-ALU kernel:
+    both well inside the FP32 budget; every k-vector is written (no stale partials)."""
+    _say("case fp32 S(k)", name)
+    s = systems.config(name)
+    pos = abi.positions(s)
+    e64 = _engine(s, precision=abi.FP64)
+    e64.force_get(*pos)
+    _, _, s64 = e64.kvectors()
+    e32 = _engine(s, precision=abi.FP32_ACC64)
+    r32 = e32.force_get(*pos)
+    _, _, s32 = e32.kvectors()
+    monkeypatch.setenv("MDF_SF_MMA", "0")
+    ev = _engine(s, precision=abi.FP32_ACC64)
+    rv = ev.force_get(*pos)
+    _, _, sv = ev.kvectors()
+    scale = np.abs(s64).max()
+    err_m = np.abs(s32 - s64).max() / scale
+    err_v = np.abs(sv - s64).max() / scale
+    _say("  max |S32 - S64| / max |S64|: matrix cores %.2e, vector ALU %.2e" % (err_m, err_v))
+    assert err_m < 2e-5 and err_v < 2e-5
+    assert rms_rel(r32["f_site"], rv["f_site"]) < 1e-4
+
+
 def test_rebuilds_along_trajectory_match_oracle(oracle_lib):
     """Rigid translations of every molecule step by step: the skin test fires at the same
     steps as the reference rule and the rebuilt pair sets stay identical."""
