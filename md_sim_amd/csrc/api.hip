@@ -69,6 +69,11 @@ __global__ void k_epilogue(DevSys S, const double *fpart, int n_groups, const Wa
 __global__ void k_sf_mma(const double *x, const double *y, const double *z, const double *q, int n, double cux,
                          double cuy, double cuz, const KRow *rows, int n_rows, int knmax, int n_nt, double2 *partial,
                          int nk, const MtsState *mts);
+__global__ void k_bp_mma(const double *x, const double *y, const double *z, const double *q, int n, double cux,
+                         double cuy, double cuz, const KRow *rows, int n_rows, int knmax, int n_nt, const double4 *T,
+                         double pipix, double pipiy, double pipiz, double *fpart, const MtsState *mts);
+constexpr int BPM_SITES = 128;   // sites per k_bp_mma block (8 waves x 16)
+constexpr int BPM_ROWS = 1024;   // rows k_bp_mma keeps in LDS
 constexpr int SFM_SITES = 512;   // sites per k_sf_mma block
 constexpr int SFM_ROWS_PER_BLOCK = 64;
 constexpr int SFM_TILES_MAX = 8; // kn tiles of 16 (knmax <= 63)
@@ -234,7 +239,7 @@ struct mdf_system {
     DBuf<Scalars> d_scal;
     // launch geometry
     // the FP32 form's structure factor on the matrix cores (k_sf_mma)
-    bool sf_mma = false, sf_mma_allowed = true;
+    bool sf_mma = false, sf_mma_allowed = true, bp_mma = false, bp_mma_allowed = true;
     int n_sb_mma = 1, n_rg_mma = 1, knmax = 0, n_nt = 0;
     int skin_blocks = 1, n_sb = 1, n_cg = 1, cpb = 1, n_kblk = 1, n_rg = 1, rpg = 1, n_epblk = 1, n_colblk = 1,
         colper = 1, ncells_xy = 1, n_cells_cap = 1;
@@ -467,6 +472,8 @@ static void geometry(mdf_system *s) {
     for (size_t k = 2; k < s->kvec.size(); k += 3) s->knmax = std::max(s->knmax, std::abs(s->kvec[k]));
     s->n_nt = (2 * s->knmax + 1 + 15) / 16;
     s->sf_mma = s->sf_mma_allowed && s->par.precision == MDF_FP32_ACC64 && nk > 0 && s->n_nt <= SFM_TILES_MAX;
+    s->bp_mma = s->bp_mma_allowed && s->par.precision == MDF_FP32_ACC64 && nk > 0 && s->n_nt <= SFM_TILES_MAX &&
+                nrow <= BPM_ROWS;
     s->n_sb_mma = std::max(1, (nm + SFM_SITES - 1) / SFM_SITES);
     s->n_rg_mma = std::max(1, (nrow + SFM_ROWS_PER_BLOCK - 1) / SFM_ROWS_PER_BLOCK);
     // back-projection row groups: about equal k counts, each at most BP_GROUP_K k-vectors (the
@@ -642,6 +649,12 @@ static void launch_bp(mdf_system *s) {
     if (nk == 0 || nm == 0) return;
     const RecipArgs a = recip_args(s);
     DevSys &S = s->S;
+    if (s->bp_mma) {
+        k_bp_mma<<<(nm + BPM_SITES - 1) / BPM_SITES, BPM_SITES * 4, 0, s->stream>>>(
+            S.x, S.y, S.z, S.q, nm, a.cux, a.cuy, a.cuz, s->d_rows.p, (int)s->rows.size(), s->knmax, s->n_nt,
+            s->d_t.p, a.pipix, a.pipiy, a.pipiz, s->fpart.p, mts_state(s));
+        return;
+    }
     dim3 g2(std::max(1, (nm + 256 * BP_NS - 1) / (256 * BP_NS)), s->n_rg);
     if (s->par.precision == MDF_FP32_ACC64)
         k_bp<float, BP_NS><<<g2, 256, 0, s->stream>>>(S.x, S.y, S.z, S.q, nm, a.cux, a.cuy, a.cuz, s->d_rows.p,
@@ -709,7 +722,7 @@ static void launch_forces(mdf_system *s) {
     if (s->ev) (void)hipEventRecord(s->ev[5], st);
     launch_bp(s);
     if (s->ev) (void)hipEventRecord(s->ev[6], st);
-    const int n_groups = (nk > 0 && nm > 0) ? s->n_rg : 0;
+    const int n_groups = (nk > 0 && nm > 0) ? (s->bp_mma ? 1 : s->n_rg) : 0;
     k_epilogue<<<s->n_epblk, EPI_BLOCK, 0, st>>>(S, s->fpart.p, n_groups, s->d_wl.p, s->par.ew3dc, s->crf0,
                                                  s->dipz.p, s->eppart.p, s->fcom.p, s->torque.p, mts_state(s),
                                                  s->sg_fmts.p, s->obs_on ? s->obs_efi.p : nullptr);
@@ -1055,6 +1068,8 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
         if (pr && pr[0] == '1') s->pair_rolled = true;
         const char *sm = getenv("MDF_SF_MMA");
         if (sm && sm[0] == '0') s->sf_mma_allowed = false;
+        const char *bm = getenv("MDF_BP_MMA");
+        if (bm && bm[0] == '0') s->bp_mma_allowed = false;
         const char *sb = getenv("MDF_SF_BLOCKS");
         if (sb && atoi(sb) > 0) s->sf_target_blocks = atoi(sb);
         const char *bb = getenv("MDF_BP_BLOCKS");

@@ -193,22 +193,24 @@ __device__ __forceinline__ void sf_body(const double *__restrict__ x, const doub
 // and a (sites x kn) matrix.  As real operands, A = [Pr Pi] per site pair of k entries and
 // B = [Er -Ei] (real part) or [Ei Er] (imaginary part).  Each FP32 operand is split into two
 // FP16 parts, v = hi + lo, and A B ~ A_hi B_hi + A_hi B_lo + A_lo B_hi (the dropped lo lo
-// term is 2^-22 relative), accumulated in FP32 by v_mfma_f32_16x16x16_f16 over a 512-site
-// block and written as FP64 partials - the same partial layout k_sf_reduce sums.
+// term is 2^-22 relative), accumulated in FP32 by v_mfma_f32_16x16x32_f16 over a 512-site
+// block and written as FP64 partials - the same partial layout k_sf_reduce sums.  Operand
+// layout of that instruction (one wave): lane l holds A[l%16][8(l/16)..+7], B[8(l/16)..+7][l%16]
+// and D[4(l/16)+e][l%16]; 8 k entries are 4 complex values.
 // A block takes 512 sites and 64 rows (one 16-row tile per wave); per 32-site stage all
-// threads build the P and E tables (split, in LDS) by recursion along km and kn, then each
-// wave runs its tile over the kn tiles its rows reach.
-typedef _Float16 h16x4 __attribute__((ext_vector_type(4)));
+// threads build the P and E tables (split, in LDS) by recursion in steps of 8 along km and
+// kn, then each wave runs its tile over the kn tiles its rows reach.
+typedef _Float16 h16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int SFM_SB = 512;                  // sites per block (one partial row)
 constexpr int SFM_SUB = 32;                  // sites per LDS stage
 constexpr int SFM_ROWS = 64;                 // rows per block: 4 waves x 16
 constexpr int SFM_NT_MAX = 8;                // kn tiles of 16 the E table holds (knmax <= 63)
-// tables are [site][kn] and [site][row]; strides = 8 mod 16 put the two site pairs a half
+// tables are [site][kn] and [site][row]; strides = 4 mod 8 put the two site quads a half
 // wave reads in one step 16 banks apart
-constexpr int SFM_KP = SFM_NT_MAX * 16 + 8;
-constexpr int SFM_RP = SFM_ROWS + 8;
+constexpr int SFM_KP = SFM_NT_MAX * 16 + 4;
+constexpr int SFM_RP = SFM_ROWS + 4;
 
 // (re, im) -> packed FP16 pairs hi and lo with re + i im = hi + lo to about 21 bits: hi
 // rounds toward zero (v_cvt_pkrtz), so re - hi is exact in FP32 and lo carries it
@@ -220,15 +222,23 @@ __device__ __forceinline__ void split_c(float re, float im, unsigned &hi, unsign
     lo = __builtin_bit_cast(unsigned, l);
 }
 
-__device__ __forceinline__ h16x4 frag2(unsigned a, unsigned b) {
-    uint2 v = make_uint2(a, b);
-    return __builtin_bit_cast(h16x4, v);
+// sin, cos of pi t for the FP16-split tables: reduce to [-1, 1] half turns in FP64, then the
+// hardware v_sin_f32 / v_cos_f32 (arguments in revolutions, absolute error about 2^-21)
+__device__ __forceinline__ void sincospi_hw(double t, float &s, float &c) {
+    const float rev = (float)(0.5 * (t - 2.0 * rint(0.5 * t)));
+    s = __builtin_amdgcn_sinf(rev);
+    c = __builtin_amdgcn_cosf(rev);
+}
+
+__device__ __forceinline__ h16x8 frag4(unsigned a, unsigned b, unsigned c, unsigned d) {
+    uint4 v = make_uint4(a, b, c, d);
+    return __builtin_bit_cast(h16x8, v);
 }
 __device__ __forceinline__ unsigned neg_im(unsigned c) { return c ^ 0x80000000u; }          // (re, -im)
 __device__ __forceinline__ unsigned swap_c(unsigned c) { return (c >> 16) | (c << 16); }    // (im, re)
 
-__device__ __forceinline__ f32x4 mfma16h(h16x4 a, h16x4 b, f32x4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+__device__ __forceinline__ f32x4 mfma16h(h16x8 a, h16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
 }
 
 __device__ __forceinline__ int wave_min(int v) {
@@ -249,7 +259,7 @@ __global__ __launch_bounds__(256) void k_sf_mma(const double *__restrict__ x, co
     __shared__ unsigned sEh[SFM_SUB][SFM_KP], sEl[SFM_SUB][SFM_KP];
     __shared__ unsigned sPh[SFM_SUB][SFM_RP], sPl[SFM_SUB][SFM_RP];
     __shared__ double sU[SFM_SB], sV[SFM_SB], sW[SFM_SB];   // site phases in half turns
-    __shared__ float4 sStep[SFM_SB];                        // cos, sin of pi v and of pi w
+    __shared__ float4 sStep[SFM_SB];                        // cos, sin of 8 pi v and of 8 pi w
     __shared__ float sQ[SFM_SB];
     __shared__ int sKl[SFM_ROWS], sKm[SFM_ROWS], sFresh[SFM_ROWS];
     const int sb = blockIdx.x;
@@ -264,8 +274,8 @@ __global__ __launch_bounds__(256) void k_sf_mma(const double *__restrict__ x, co
         const bool ok = i < s_end;
         const double u = ok ? x[i] * cux : 0.0, v = ok ? y[i] * cuy : 0.0, w = ok ? z[i] * cuz : 0.0;
         float vs, vc, ws, wc;
-        sincospi_r<float>(v, vs, vc);
-        sincospi_r<float>(w, ws, wc);
+        sincospi_hw(8.0 * v, vs, vc);
+        sincospi_hw(8.0 * w, ws, wc);
         sU[t] = u; sV[t] = v; sW[t] = w;
         sStep[t] = make_float4(vc, vs, wc, ws);
         sQ[t] = ok ? (float)q[i] : 0.f;
@@ -276,9 +286,9 @@ __global__ __launch_bounds__(256) void k_sf_mma(const double *__restrict__ x, co
         if (r < n_rows) {
             const KRow row = rows[r];
             kl = row.kl; km = row.km;
-            if ((threadIdx.x & 7) != 0) {   // a P segment is 8 rows: its first row always starts fresh
-                const KRow pr = rows[r - 1];
-                fresh = (row.kl != pr.kl || row.km != pr.km + 1) ? 1 : 0;
+            if (threadIdx.x >= 8) {   // P walks rows in steps of 8: km + 8 in the same kl continues
+                const KRow pr = rows[r - 8];
+                fresh = (row.kl != pr.kl || row.km != pr.km + 8) ? 1 : 0;
             }
         }
         sKl[threadIdx.x] = kl; sKm[threadIdx.x] = km; sFresh[threadIdx.x] = fresh;
@@ -299,35 +309,41 @@ __global__ __launch_bounds__(256) void k_sf_mma(const double *__restrict__ x, co
         acci[t] = f32x4{0.f, 0.f, 0.f, 0.f};
     }
     __syncthreads();
-    // table roles: site gj of the stage, segment gs of 8 (kn for E, rows for P)
-    const int gj = threadIdx.x % SFM_SUB, gs = threadIdx.x / SFM_SUB;
-    const int per_e = 2 * n_nt;   // n_nt * 16 kn over 8 segments
+    // table roles: site gj of the stage and phase gs of 8; a thread walks kn = gs + 8 e >= 0
+    // (and writes E(-kn) = conj E(kn) beside it) and rows gs + 8 rr, so the 8 lanes of a site
+    // write 8 consecutive LDS words
+    const int gj = threadIdx.x >> 3, gs = threadIdx.x & 7;
+    const int per_e = (knmax + 8) >> 3;   // kn = 0 .. knmax over 8 phases
     for (int c0 = 0; c0 < s_end - s_begin; c0 += SFM_SUB) {
         const int c = c0 + gj;
         const float4 st = sStep[c];
-        {   // E = exp(i pi kn w), kn = -knmax + e
+        {   // E = exp(i pi kn w) at table index knmax + kn, kn = gs + 8 e, and its conjugate at knmax - kn
             float es, ec;
-            const int e0 = gs * per_e;
-            sincospi_r<float>((double)(e0 - knmax) * sW[c], es, ec);
+            sincospi_hw((double)gs * sW[c], es, ec);
             for (int e = 0; e < per_e; e++) {
-                unsigned h, l;
-                split_c(ec, es, h, l);
-                sEh[gj][e0 + e] = h;
-                sEl[gj][e0 + e] = l;
+                const int kn = gs + 8 * e;
+                if (kn <= knmax) {
+                    unsigned h, l;
+                    split_c(ec, es, h, l);
+                    sEh[gj][knmax + kn] = h;
+                    sEl[gj][knmax + kn] = l;
+                    sEh[gj][knmax - kn] = neg_im(h);
+                    sEl[gj][knmax - kn] = neg_im(l);
+                }
                 const float nr = ec * st.z - es * st.w, ni = ec * st.w + es * st.z;
                 ec = nr; es = ni;
             }
         }
-        {   // P = q exp(i pi (kl u + km v)) for the block's rows gs*8 .. gs*8+7
+        {   // P = q exp(i pi (kl u + km v)) for the block's rows gs + 8 rr
             const float qq = sQ[c];
             const double u = sU[c], v = sV[c];
             float pr = 0.f, pi_ = 0.f;
 #pragma unroll
             for (int rr = 0; rr < 8; rr++) {
-                const int lr = gs * 8 + rr;
+                const int lr = gs + 8 * rr;
                 if (sFresh[lr]) {
                     float s0, cc;
-                    sincospi_r<float>((double)sKl[lr] * u + (double)sKm[lr] * v, s0, cc);
+                    sincospi_hw((double)sKl[lr] * u + (double)sKm[lr] * v, s0, cc);
                     pr = cc; pi_ = s0;
                 } else {
                     const float nr = pr * st.x - pi_ * st.y, ni = pr * st.y + pi_ * st.x;
@@ -341,26 +357,29 @@ __global__ __launch_bounds__(256) void k_sf_mma(const double *__restrict__ x, co
         }
         __syncthreads();
         if (active) {
-            const int ar = wave * 16 + (lane & 15), g2 = 2 * (lane >> 4);
+            const int ar = wave * 16 + (lane & 15), g4 = 4 * (lane >> 4);
 #pragma unroll 1
-            for (int ks = 0; ks < SFM_SUB / 8; ks++) {
-                const int sa = ks * 8 + g2;
-                const h16x4 Ah = frag2(sPh[sa][ar], sPh[sa + 1][ar]);
-                const h16x4 Al = frag2(sPl[sa][ar], sPl[sa + 1][ar]);
+            for (int ks = 0; ks < SFM_SUB / 16; ks++) {
+                const int sa = ks * 16 + g4;
+                // real part: [Pr -Pi] . [Er Ei]; imaginary part: [Pi Pr] . [Er Ei]
+                const unsigned ph0 = sPh[sa][ar], ph1 = sPh[sa + 1][ar], ph2 = sPh[sa + 2][ar], ph3 = sPh[sa + 3][ar];
+                const unsigned pl0 = sPl[sa][ar], pl1 = sPl[sa + 1][ar], pl2 = sPl[sa + 2][ar], pl3 = sPl[sa + 3][ar];
+                const h16x8 Arh = frag4(neg_im(ph0), neg_im(ph1), neg_im(ph2), neg_im(ph3));
+                const h16x8 Arl = frag4(neg_im(pl0), neg_im(pl1), neg_im(pl2), neg_im(pl3));
+                const h16x8 Aih = frag4(swap_c(ph0), swap_c(ph1), swap_c(ph2), swap_c(ph3));
+                const h16x8 Ail = frag4(swap_c(pl0), swap_c(pl1), swap_c(pl2), swap_c(pl3));
 #pragma unroll
                 for (int t = 0; t < SFM_NT_MAX; t++) {
                     if (t < nt_lo || t > nt_hi) continue;
                     const int ec = t * 16 + (lane & 15);
-                    const unsigned h0 = sEh[sa][ec], h1 = sEh[sa + 1][ec];
-                    const unsigned l0 = sEl[sa][ec], l1 = sEl[sa + 1][ec];
-                    const h16x4 Brh = frag2(neg_im(h0), neg_im(h1)), Brl = frag2(neg_im(l0), neg_im(l1));
-                    const h16x4 Bih = frag2(swap_c(h0), swap_c(h1)), Bil = frag2(swap_c(l0), swap_c(l1));
-                    accr[t] = mfma16h(Ah, Brh, accr[t]);
-                    acci[t] = mfma16h(Ah, Bih, acci[t]);
-                    accr[t] = mfma16h(Ah, Brl, accr[t]);
-                    acci[t] = mfma16h(Ah, Bil, acci[t]);
-                    accr[t] = mfma16h(Al, Brh, accr[t]);
-                    acci[t] = mfma16h(Al, Bih, acci[t]);
+                    const h16x8 Bh = frag4(sEh[sa][ec], sEh[sa + 1][ec], sEh[sa + 2][ec], sEh[sa + 3][ec]);
+                    const h16x8 Bl = frag4(sEl[sa][ec], sEl[sa + 1][ec], sEl[sa + 2][ec], sEl[sa + 3][ec]);
+                    accr[t] = mfma16h(Arh, Bh, accr[t]);
+                    acci[t] = mfma16h(Aih, Bh, acci[t]);
+                    accr[t] = mfma16h(Arh, Bl, accr[t]);
+                    acci[t] = mfma16h(Aih, Bl, acci[t]);
+                    accr[t] = mfma16h(Arl, Bh, accr[t]);
+                    acci[t] = mfma16h(Ail, Bh, acci[t]);
                 }
             }
         }
@@ -380,6 +399,154 @@ __global__ __launch_bounds__(256) void k_sf_mma(const double *__restrict__ x, co
             if (kn < row.kn0 || kn > row.kn1) continue;
             partial[(size_t)sb * nk + row.koff + (kn - row.kn0)] = make_double2((double)accr[t][e], (double)acci[t][e]);
         }
+    }
+}
+
+// ---- back-projection on the matrix cores (FP32 form) ---------------------------------------
+// F_j = q_j sum_rows c(row) Im(P_j(row) G_j(row)) with G_j(row) = sum_kn T(row, kn) E_j(kn),
+// T = A conj S (and kn T for the z component): per 16-row tile, G for 16 sites is a complex
+// product of T (rows x kn) and E (kn x sites) on the matrix cores (FP16 hi/lo split as in
+// k_sf_mma, 12 products per K-step of 16 kn: G and Gz, real and imaginary).  A wave owns 16 sites
+// and keeps their E fragments in registers for the whole kernel; the block stages each T
+// tile in LDS once for its 8 waves.  The contraction with P and the sum over rows run per
+// lane in FP64, and the four lanes that share a site add up at the end: every site's force
+// is complete in one wave (one partial group).
+constexpr int BPM_WAVES = 8;
+constexpr int BPM_KP = SFM_NT_MAX * 16 + 4;  // T tile row stride (words)
+constexpr int BPM_ROWS_MAX = 1024;           // rows whose metadata the block keeps in LDS
+
+__global__ __launch_bounds__(64 * BPM_WAVES) void k_bp_mma(const double *__restrict__ x, const double *__restrict__ y,
+                                                           const double *__restrict__ z, const double *__restrict__ q,
+                                                           int n, double cux, double cuy, double cuz,
+                                                           const KRow *__restrict__ rows, int n_rows, int knmax,
+                                                           int n_nt, const double4 *__restrict__ T, double pipix,
+                                                           double pipiy, double pipiz, double *__restrict__ fpart,
+                                                           const MtsState *__restrict__ mts) {
+    if (mts && !mts->exact) return;
+    __shared__ unsigned sTh[16][BPM_KP], sTl[16][BPM_KP], sZh[16][BPM_KP], sZl[16][BPM_KP];
+    __shared__ int sKl[BPM_ROWS_MAX], sKm[BPM_ROWS_MAX], sKn0[BPM_ROWS_MAX], sKn1[BPM_ROWS_MAX], sKoff[BPM_ROWS_MAX];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane >> 4, col = lane & 15;
+    const int site = (blockIdx.x * BPM_WAVES + wave) * 16 + col;
+    const bool ok = site < n;
+    for (int r = threadIdx.x; r < n_rows; r += blockDim.x) {
+        const KRow row = rows[r];
+        sKl[r] = row.kl; sKm[r] = row.km; sKn0[r] = row.kn0; sKn1[r] = row.kn1; sKoff[r] = row.koff;
+    }
+    const double u = ok ? x[site] * cux : 0.0, v = ok ? y[site] * cuy : 0.0, w = ok ? z[site] * cuz : 0.0;
+    const double qq = ok ? q[site] : 0.0;
+    const int nks = n_nt;                // K-steps of 16 kn
+    const int ncols = 16 * nks;          // staged kn columns: kn = column - knmax
+    // E fragments for kn columns 16 ks + 4 g .. +3 (site col)
+    h16x8 Bh[SFM_NT_MAX], Bl[SFM_NT_MAX];
+    {
+        float s1, c1, s16, c16, as, ac;
+        sincospi_hw(w, s1, c1);
+        sincospi_hw(16.0 * w, s16, c16);
+        sincospi_hw((double)(4 * g - knmax) * w, as, ac);
+#pragma unroll
+        for (int ks = 0; ks < SFM_NT_MAX; ks++) {
+            unsigned h[4], l[4];
+            float er = ac, ei = as;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                split_c(er, ei, h[j], l[j]);
+                const float t0 = er * c1 - ei * s1, t1 = er * s1 + ei * c1;
+                er = t0; ei = t1;
+            }
+            Bh[ks] = frag4(h[0], h[1], h[2], h[3]);
+            Bl[ks] = frag4(l[0], l[1], l[2], l[3]);
+            const float t0 = ac * c16 - as * s16, t1 = ac * s16 + as * c16;
+            ac = t0; as = t1;
+        }
+    }
+    float vs1, vc1;
+    sincospi_hw(v, vs1, vc1);
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    const int n_mt = (n_rows + 15) >> 4;
+    for (int mt = 0; mt < n_mt; mt++) {
+        __syncthreads();   // metadata in place; the previous tile's readers are done
+        for (int e = threadIdx.x; e < 16 * ncols; e += blockDim.x) {
+            const int rr = e / ncols, ki = e - rr * ncols;
+            const int r = mt * 16 + rr, kn = ki - knmax;
+            double4 t = make_double4(0.0, 0.0, 0.0, 0.0);
+            if (r < n_rows && kn >= sKn0[r] && kn <= sKn1[r]) t = T[sKoff[r] + (kn - sKn0[r])];
+            unsigned h, l;
+            split_c((float)t.x, (float)t.y, h, l);
+            sTh[rr][ki] = h; sTl[rr][ki] = l;
+            split_c((float)t.z, (float)t.w, h, l);
+            sZh[rr][ki] = h; sZl[rr][ki] = l;
+        }
+        // kn range of the tile's rows (the same for every wave)
+        int lo = 1 << 30, hi = -(1 << 30);
+        if (lane < 16 && mt * 16 + lane < n_rows) { lo = sKn0[mt * 16 + lane]; hi = sKn1[mt * 16 + lane]; }
+        lo = wave_min(lo);
+        hi = wave_max(hi);
+        const int ks_lo = (lo + knmax) >> 4, ks_hi = min(nks - 1, (hi + knmax) >> 4);
+        __syncthreads();
+        f32x4 gr = {0.f, 0.f, 0.f, 0.f}, gi = gr, zr = gr, zi = gr;
+#pragma unroll
+        for (int ks = 0; ks < SFM_NT_MAX; ks++) {
+            if (ks < ks_lo || ks > ks_hi) continue;
+            const int kc = 16 * ks + 4 * g;
+            const uint4 th = *reinterpret_cast<const uint4 *>(&sTh[col][kc]);
+            const uint4 tl = *reinterpret_cast<const uint4 *>(&sTl[col][kc]);
+            const uint4 zh = *reinterpret_cast<const uint4 *>(&sZh[col][kc]);
+            const uint4 zl = *reinterpret_cast<const uint4 *>(&sZl[col][kc]);
+            // real part: [Tr -Ti] . [Er Ei]; imaginary part: [Ti Tr] . [Er Ei]
+            const h16x8 Arh = frag4(neg_im(th.x), neg_im(th.y), neg_im(th.z), neg_im(th.w));
+            const h16x8 Arl = frag4(neg_im(tl.x), neg_im(tl.y), neg_im(tl.z), neg_im(tl.w));
+            const h16x8 Aih = frag4(swap_c(th.x), swap_c(th.y), swap_c(th.z), swap_c(th.w));
+            const h16x8 Ail = frag4(swap_c(tl.x), swap_c(tl.y), swap_c(tl.z), swap_c(tl.w));
+            const h16x8 Zrh = frag4(neg_im(zh.x), neg_im(zh.y), neg_im(zh.z), neg_im(zh.w));
+            const h16x8 Zrl = frag4(neg_im(zl.x), neg_im(zl.y), neg_im(zl.z), neg_im(zl.w));
+            const h16x8 Zih = frag4(swap_c(zh.x), swap_c(zh.y), swap_c(zh.z), swap_c(zh.w));
+            const h16x8 Zil = frag4(swap_c(zl.x), swap_c(zl.y), swap_c(zl.z), swap_c(zl.w));
+            gr = mfma16h(Arh, Bh[ks], gr);
+            gi = mfma16h(Aih, Bh[ks], gi);
+            zr = mfma16h(Zrh, Bh[ks], zr);
+            zi = mfma16h(Zih, Bh[ks], zi);
+            gr = mfma16h(Arh, Bl[ks], gr);
+            gi = mfma16h(Aih, Bl[ks], gi);
+            zr = mfma16h(Zrh, Bl[ks], zr);
+            zi = mfma16h(Zih, Bl[ks], zi);
+            gr = mfma16h(Arl, Bh[ks], gr);
+            gi = mfma16h(Ail, Bh[ks], gi);
+            zr = mfma16h(Zrl, Bh[ks], zr);
+            zi = mfma16h(Zil, Bh[ks], zi);
+        }
+        // lane holds G for rows 4 g + e of the tile at its site: contract with P(row, site)
+        float pr = 0.f, pi_ = 0.f;
+        int pkl = -1, pkm = 0;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int r = mt * 16 + 4 * g + e;
+            if (r >= n_rows) break;
+            const int kl = sKl[r], km = sKm[r];
+            if (e == 0 || kl != pkl || km != pkm + 1) {
+                float s0, c0;
+                sincospi_hw((double)kl * u + (double)km * v, s0, c0);
+                pr = c0; pi_ = s0;
+            } else {
+                const float t0 = pr * vc1 - pi_ * vs1, t1 = pr * vs1 + pi_ * vc1;
+                pr = t0; pi_ = t1;
+            }
+            pkl = kl; pkm = km;
+            const double im = (double)(pr * gi[e] + pi_ * gr[e]);
+            const double imz = (double)(pr * zi[e] + pi_ * zr[e]);
+            fx += im * (pipix * (double)kl);
+            fy += im * (pipiy * (double)km);
+            fz += imz * pipiz;
+        }
+    }
+    // the four lanes of a site (g = 0..3) hold its row groups
+    fx += __shfl_xor(fx, 16, 64); fx += __shfl_xor(fx, 32, 64);
+    fy += __shfl_xor(fy, 16, 64); fy += __shfl_xor(fy, 32, 64);
+    fz += __shfl_xor(fz, 16, 64); fz += __shfl_xor(fz, 32, 64);
+    if (g == 0 && ok) {
+        fpart[site] = qq * fx;
+        fpart[(size_t)n + site] = qq * fy;
+        fpart[2 * (size_t)n + site] = qq * fz;
     }
 }
 

@@ -56,9 +56,10 @@ def test_fp32_force_error_bound(oracle_lib, name):
 
 @pytest.mark.parametrize("name", ["slab2048", "slab512", "bulk216"])
 def test_fp32_structure_factor_on_matrix_cores(name, monkeypatch):
-    """The FP32 form's S(k) (FP16 hi/lo split products on the matrix cores, FP32 block sums,
-    FP64 across blocks) against the FP64 form's, and against the FP32 vector-ALU kernel:
-    both well inside the FP32 budget; every k-vector is written (no stale partials)."""
+    """The FP32 form's reciprocal sum on the matrix cores (FP16 hi/lo split products, FP32
+    tile sums, FP64 across blocks and rows): S(k) against the FP64 form's and the FP32
+    vector-ALU kernel's, every k-vector written (no stale partials); forces against the
+    vector-ALU kernels."""
     _say("case fp32 S(k)", name)
     s = systems.config(name)
     pos = abi.positions(s)
@@ -69,6 +70,7 @@ def test_fp32_structure_factor_on_matrix_cores(name, monkeypatch):
     r32 = e32.force_get(*pos)
     _, _, s32 = e32.kvectors()
     monkeypatch.setenv("MDF_SF_MMA", "0")
+    monkeypatch.setenv("MDF_BP_MMA", "0")
     ev = _engine(s, precision=abi.FP32_ACC64)
     rv = ev.force_get(*pos)
     _, _, sv = ev.kvectors()
@@ -77,7 +79,12 @@ def test_fp32_structure_factor_on_matrix_cores(name, monkeypatch):
     err_v = np.abs(sv - s64).max() / scale
     _say("  max |S32 - S64| / max |S64|: matrix cores %.2e, vector ALU %.2e" % (err_m, err_v))
     assert err_m < 2e-5 and err_v < 2e-5
-    assert rms_rel(r32["f_site"], rv["f_site"]) < 1e-4
+    # the whole reciprocal path (structure factor and back-projection on the matrix cores)
+    # against the vector-ALU kernels: site forces, COM forces and torques
+    ef = rms_rel(r32["f_site"], rv["f_site"])
+    _say("  force rms rel diff, matrix cores vs vector ALU: %.2e" % ef)
+    assert ef < 1e-5
+    assert rms_rel(r32["f_com"], rv["f_com"]) < 1e-5 and rms_rel(r32["torque"], rv["torque"]) < 1e-5
 
 
 def test_rebuilds_along_trajectory_match_oracle(oracle_lib):
