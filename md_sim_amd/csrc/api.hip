@@ -43,7 +43,8 @@ template <typename R, int NS>
 __global__ void k_sf(const double *, const double *, const double *, const double *, int, double, double, double,
                      const KRow *, int, int, double2 *, int, const MtsState *);
 __global__ void k_sf_reduce(const double2 *, int, int, const double2 *, const double *, const int *, double, double,
-                            double, double, double2 *, double4 *, double *, const MtsState *);
+                            double, double, double2 *, double4 *, double *, const MtsState *, unsigned *, const int *,
+                            int, int);
 template <typename R, int NS>
 __global__ void k_bp(const double *, const double *, const double *, const double *, int, double, double, double,
                      const KRow *, const int *, const double4 *, double, double, double, double *, const MtsState *);
@@ -70,8 +71,9 @@ __global__ void k_sf_mma(const double *x, const double *y, const double *z, cons
                          double cuy, double cuz, const KRow *rows, int n_rows, int knmax, int n_nt, double2 *partial,
                          int nk, const MtsState *mts);
 __global__ void k_bp_mma(const double *x, const double *y, const double *z, const double *q, int n, double cux,
-                         double cuy, double cuz, const KRow *rows, int n_rows, int knmax, int n_nt, const double4 *T,
-                         double pipix, double pipiy, double pipiz, double *fpart, const MtsState *mts);
+                         double cuy, double cuz, const int2 *rowkk, const int2 *tile_ks, int n_rows, int knmax,
+                         int n_nt, const unsigned *tpack, double pipix, double pipiy, double pipiz, double *fpart,
+                         const MtsState *mts);
 constexpr int BPM_SITES = 128;   // sites per k_bp_mma block (8 waves x 16)
 constexpr int BPM_ROWS = 1024;   // rows k_bp_mma keeps in LDS
 constexpr int SFM_SITES = 512;   // sites per k_sf_mma block
@@ -240,6 +242,9 @@ struct mdf_system {
     // launch geometry
     // the FP32 form's structure factor on the matrix cores (k_sf_mma)
     bool sf_mma = false, sf_mma_allowed = true, bp_mma = false, bp_mma_allowed = true;
+    DBuf<int2> d_rowkk, d_tile_ks;   // k_bp_mma: (kl, km) per row, K-step range per 16-row tile
+    DBuf<int> d_krow;                // row of each k-vector
+    DBuf<unsigned> d_tpack;          // T split for k_bp_mma, [row][plane][kn + knmax]
     int n_sb_mma = 1, n_rg_mma = 1, knmax = 0, n_nt = 0;
     int skin_blocks = 1, n_sb = 1, n_cg = 1, cpb = 1, n_kblk = 1, n_rg = 1, rpg = 1, n_epblk = 1, n_colblk = 1,
         colper = 1, ncells_xy = 1, n_cells_cap = 1;
@@ -350,6 +355,7 @@ static void free_all(mdf_system *s) {
                           &s->fz, &s->part, &s->dip_part, &s->dipz, &s->d_ak, &s->kpart, &s->fpart, &s->eppart,
                           &s->colpart, &s->fcom, &s->torque, &s->e_vdw, &s->e_coul, &s->ang_kt, &s->ang_th};
     for (auto *b : db) b->release();
+    s->d_rowkk.release(); s->d_tile_ks.release(); s->d_krow.release(); s->d_tpack.release();
     s->flags.release(); s->d_bp_row_start.release(); s->d_chunks.release(); s->d_rows.release(); s->d_swall.release(); s->d_s.release();
     s->d_t.release(); s->sf_partial.release(); s->d_wl.release(); s->d_scal.release();
     mts_free(s);
@@ -651,8 +657,8 @@ static void launch_bp(mdf_system *s) {
     DevSys &S = s->S;
     if (s->bp_mma) {
         k_bp_mma<<<(nm + BPM_SITES - 1) / BPM_SITES, BPM_SITES * 4, 0, s->stream>>>(
-            S.x, S.y, S.z, S.q, nm, a.cux, a.cuy, a.cuz, s->d_rows.p, (int)s->rows.size(), s->knmax, s->n_nt,
-            s->d_t.p, a.pipix, a.pipiy, a.pipiz, s->fpart.p, mts_state(s));
+            S.x, S.y, S.z, S.q, nm, a.cux, a.cuy, a.cuz, s->d_rowkk.p, s->d_tile_ks.p, (int)s->rows.size(), s->knmax,
+            s->n_nt, s->d_tpack.p, a.pipix, a.pipiy, a.pipiz, s->fpart.p, mts_state(s));
         return;
     }
     dim3 g2(std::max(1, (nm + 256 * BP_NS - 1) / (256 * BP_NS)), s->n_rg);
@@ -717,7 +723,8 @@ static void launch_forces(mdf_system *s) {
         k_sf_reduce<<<s->n_kblk, 256, 0, st>>>(s->sf_partial.p, nm > 0 ? (s->sf_mma ? s->n_sb_mma : s->n_sb) : 0, nk,
                                                s->d_swall.p, s->d_ak.p,
                                                s->d_kvec.p, ra.pipix, ra.pipiy, ra.pipiz, lamb, s->d_s.p, s->d_t.p,
-                                               s->kpart.p, mts_state(s));
+                                               s->kpart.p, mts_state(s), s->bp_mma ? s->d_tpack.p : nullptr,
+                                               s->d_krow.p, 16 * s->n_nt, s->knmax);
     }
     if (s->ev) (void)hipEventRecord(s->ev[5], st);
     launch_bp(s);
@@ -1342,6 +1349,31 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
             HTRY(upload(s->d_ak, s->ak.data(), s->ak.size()));
             HTRY(s->d_s.alloc(std::max(nk, 1)));
             HTRY(s->d_t.alloc(nk + 4));   // padded: the back-projection prefetches one entry past a row
+            if (s->bp_mma) {
+                // k_bp_mma's tables: rows' (kl, km), each 16-row tile's K-step range, each k's row,
+                // and the zero-padded split T that k_sf_reduce fills inside the kn sphere
+                const int nr = (int)s->rows.size(), kn_pad = 16 * s->n_nt;
+                std::vector<int2> rkk(std::max(nr, 1)), tks(std::max((nr + 15) / 16, 1));
+                std::vector<int> krow(std::max(nk, 1), 0);
+                for (int r = 0; r < nr; r++) {
+                    const KRow &w = s->rows[r];
+                    rkk[r] = make_int2(w.kl, w.km);
+                    for (int kn = w.kn0; kn <= w.kn1; kn++) krow[w.koff + kn - w.kn0] = r;
+                }
+                for (int m = 0; m * 16 < nr; m++) {
+                    int lo = 1 << 30, hi = -(1 << 30);
+                    for (int r = 16 * m; r < std::min(nr, 16 * m + 16); r++) {
+                        lo = std::min(lo, s->rows[r].kn0);
+                        hi = std::max(hi, s->rows[r].kn1);
+                    }
+                    tks[m] = make_int2((lo + s->knmax) >> 4, std::min(s->n_nt - 1, (hi + s->knmax) >> 4));
+                }
+                HTRY(upload(s->d_rowkk, rkk.data(), rkk.size()));
+                HTRY(upload(s->d_tile_ks, tks.data(), tks.size()));
+                HTRY(upload(s->d_krow, krow.data(), krow.size()));
+                HTRY(s->d_tpack.alloc((size_t)std::max(nr, 1) * 4 * kn_pad));
+                HTRY(hipMemset(s->d_tpack.p, 0, sizeof(unsigned) * s->d_tpack.n));
+            }
             HTRY(s->kpart.alloc((size_t)8 * s->n_kblk));
             std::vector<double2> zero2(std::max(nk, 1), make_double2(0.0, 0.0));
             HTRY(upload(s->d_swall, zero2.data(), zero2.size()));
@@ -1364,7 +1396,7 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
                     s->d_rows.p, nrow, rpb, s->sf_partial.p, nk, nullptr);
                 k_sf_reduce<<<s->n_kblk, 256, 0, s->stream>>>(s->sf_partial.p, n_sb_w, nk, s->d_swall.p, s->d_ak.p,
                                                               s->d_kvec.p, 1.0, 1.0, 1.0, 0.0, s->d_s.p, s->d_t.p,
-                                                              s->kpart.p, nullptr);
+                                                              s->kpart.p, nullptr, nullptr, nullptr, 0, 0);
                 HTRY(hipStreamSynchronize(s->stream));
                 HTRY(hipMemcpy(s->s_wall.data(), s->d_s.p, sizeof(double2) * nk, hipMemcpyDeviceToHost));
                 HTRY(hipMemcpy(s->d_swall.p, s->d_s.p, sizeof(double2) * nk, hipMemcpyDeviceToDevice));

@@ -406,37 +406,51 @@ __global__ __launch_bounds__(256) void k_sf_mma(const double *__restrict__ x, co
 // F_j = q_j sum_rows c(row) Im(P_j(row) G_j(row)) with G_j(row) = sum_kn T(row, kn) E_j(kn),
 // T = A conj S (and kn T for the z component): per 16-row tile, G for 16 sites is a complex
 // product of T (rows x kn) and E (kn x sites) on the matrix cores (FP16 hi/lo split as in
-// k_sf_mma, 12 products per K-step of 16 kn: G and Gz, real and imaginary).  A wave owns 16 sites
-// and keeps their E fragments in registers for the whole kernel; the block stages each T
-// tile in LDS once for its 8 waves.  The contraction with P and the sum over rows run per
-// lane in FP64, and the four lanes that share a site add up at the end: every site's force
-// is complete in one wave (one partial group).
+// k_sf_mma, 12 products per K-step of 16 kn: G and Gz, real and imaginary).  k_sf_reduce
+// leaves T already split, in a dense zero-padded layout tpack[(row * 4 + plane) * kn_pad +
+// kn + knmax] with planes T hi, T lo, kn T hi, kn T lo.  A wave owns 16 sites and keeps their
+// E fragments in registers for the whole kernel; the block copies each T tile into LDS once
+// for its 8 waves, one tile ahead (two buffers, one barrier per tile).  The contraction with
+// P and the sum over rows run per lane in FP64, and the four lanes that share a site add up
+// at the end: every site's force is complete in one wave (one partial group).
 constexpr int BPM_WAVES = 8;
 constexpr int BPM_KP = SFM_NT_MAX * 16 + 4;  // T tile row stride (words)
-constexpr int BPM_ROWS_MAX = 1024;           // rows whose metadata the block keeps in LDS
+constexpr int BPM_ROWS_MAX = 1024;           // rows whose (kl, km) the block keeps in LDS
 
 __global__ __launch_bounds__(64 * BPM_WAVES) void k_bp_mma(const double *__restrict__ x, const double *__restrict__ y,
                                                            const double *__restrict__ z, const double *__restrict__ q,
                                                            int n, double cux, double cuy, double cuz,
-                                                           const KRow *__restrict__ rows, int n_rows, int knmax,
-                                                           int n_nt, const double4 *__restrict__ T, double pipix,
+                                                           const int2 *__restrict__ rowkk,
+                                                           const int2 *__restrict__ tile_ks, int n_rows, int knmax,
+                                                           int n_nt, const unsigned *__restrict__ tpack, double pipix,
                                                            double pipiy, double pipiz, double *__restrict__ fpart,
                                                            const MtsState *__restrict__ mts) {
     if (mts && !mts->exact) return;
-    __shared__ unsigned sTh[16][BPM_KP], sTl[16][BPM_KP], sZh[16][BPM_KP], sZl[16][BPM_KP];
-    __shared__ int sKl[BPM_ROWS_MAX], sKm[BPM_ROWS_MAX], sKn0[BPM_ROWS_MAX], sKn1[BPM_ROWS_MAX], sKoff[BPM_ROWS_MAX];
+    __shared__ unsigned sT[2][4][16][BPM_KP];
+    __shared__ int sKl[BPM_ROWS_MAX], sKm[BPM_ROWS_MAX];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, col = lane & 15;
     const int site = (blockIdx.x * BPM_WAVES + wave) * 16 + col;
     const bool ok = site < n;
+    const int kn_pad = 16 * n_nt, q4 = kn_pad >> 2;   // uint4 per (row, plane)
+    const int n_mt = (n_rows + 15) >> 4;
     for (int r = threadIdx.x; r < n_rows; r += blockDim.x) {
-        const KRow row = rows[r];
-        sKl[r] = row.kl; sKm[r] = row.km; sKn0[r] = row.kn0; sKn1[r] = row.kn1; sKoff[r] = row.koff;
+        const int2 v = rowkk[r];
+        sKl[r] = v.x; sKm[r] = v.y;
     }
+    // stage tile mt (16 rows x 4 planes x kn_pad) into buffer b
+    auto stage = [&](int mt, int b) {
+        for (int e = threadIdx.x; e < 64 * q4; e += blockDim.x) {
+            const int rp = e / q4, c4 = e - rp * q4;   // rp = row-in-tile * 4 + plane
+            const int rr = rp >> 2, pl = rp & 3, r = mt * 16 + rr;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (r < n_rows) v = *reinterpret_cast<const uint4 *>(&tpack[((size_t)r * 4 + pl) * kn_pad + 4 * c4]);
+            *reinterpret_cast<uint4 *>(&sT[b][pl][rr][4 * c4]) = v;
+        }
+    };
+    stage(0, 0);
     const double u = ok ? x[site] * cux : 0.0, v = ok ? y[site] * cuy : 0.0, w = ok ? z[site] * cuz : 0.0;
     const double qq = ok ? q[site] : 0.0;
-    const int nks = n_nt;                // K-steps of 16 kn
-    const int ncols = 16 * nks;          // staged kn columns: kn = column - knmax
     // E fragments for kn columns 16 ks + 4 g .. +3 (site col)
     h16x8 Bh[SFM_NT_MAX], Bl[SFM_NT_MAX];
     {
@@ -460,39 +474,26 @@ __global__ __launch_bounds__(64 * BPM_WAVES) void k_bp_mma(const double *__restr
             ac = t0; as = t1;
         }
     }
-    float vs1, vc1;
+    float vs1, vc1, vs16, vc16;
     sincospi_hw(v, vs1, vc1);
+    sincospi_hw(16.0 * v, vs16, vc16);
     double fx = 0.0, fy = 0.0, fz = 0.0;
-    const int n_mt = (n_rows + 15) >> 4;
+    float p0r = 0.f, p0i = 0.f;   // P of the lane's first row in the previous tile
+    int p0kl = -1 << 20, p0km = 0;
     for (int mt = 0; mt < n_mt; mt++) {
-        __syncthreads();   // metadata in place; the previous tile's readers are done
-        for (int e = threadIdx.x; e < 16 * ncols; e += blockDim.x) {
-            const int rr = e / ncols, ki = e - rr * ncols;
-            const int r = mt * 16 + rr, kn = ki - knmax;
-            double4 t = make_double4(0.0, 0.0, 0.0, 0.0);
-            if (r < n_rows && kn >= sKn0[r] && kn <= sKn1[r]) t = T[sKoff[r] + (kn - sKn0[r])];
-            unsigned h, l;
-            split_c((float)t.x, (float)t.y, h, l);
-            sTh[rr][ki] = h; sTl[rr][ki] = l;
-            split_c((float)t.z, (float)t.w, h, l);
-            sZh[rr][ki] = h; sZl[rr][ki] = l;
-        }
-        // kn range of the tile's rows (the same for every wave)
-        int lo = 1 << 30, hi = -(1 << 30);
-        if (lane < 16 && mt * 16 + lane < n_rows) { lo = sKn0[mt * 16 + lane]; hi = sKn1[mt * 16 + lane]; }
-        lo = wave_min(lo);
-        hi = wave_max(hi);
-        const int ks_lo = (lo + knmax) >> 4, ks_hi = min(nks - 1, (hi + knmax) >> 4);
-        __syncthreads();
+        __syncthreads();   // tile mt staged; the other buffer's readers are done
+        if (mt + 1 < n_mt) stage(mt + 1, (mt + 1) & 1);
+        const int b = mt & 1;
+        const int2 kr = tile_ks[mt];
         f32x4 gr = {0.f, 0.f, 0.f, 0.f}, gi = gr, zr = gr, zi = gr;
 #pragma unroll
         for (int ks = 0; ks < SFM_NT_MAX; ks++) {
-            if (ks < ks_lo || ks > ks_hi) continue;
+            if (ks < kr.x || ks > kr.y) continue;
             const int kc = 16 * ks + 4 * g;
-            const uint4 th = *reinterpret_cast<const uint4 *>(&sTh[col][kc]);
-            const uint4 tl = *reinterpret_cast<const uint4 *>(&sTl[col][kc]);
-            const uint4 zh = *reinterpret_cast<const uint4 *>(&sZh[col][kc]);
-            const uint4 zl = *reinterpret_cast<const uint4 *>(&sZl[col][kc]);
+            const uint4 th = *reinterpret_cast<const uint4 *>(&sT[b][0][col][kc]);
+            const uint4 tl = *reinterpret_cast<const uint4 *>(&sT[b][1][col][kc]);
+            const uint4 zh = *reinterpret_cast<const uint4 *>(&sT[b][2][col][kc]);
+            const uint4 zl = *reinterpret_cast<const uint4 *>(&sT[b][3][col][kc]);
             // real part: [Tr -Ti] . [Er Ei]; imaginary part: [Ti Tr] . [Er Ei]
             const h16x8 Arh = frag4(neg_im(th.x), neg_im(th.y), neg_im(th.z), neg_im(th.w));
             const h16x8 Arl = frag4(neg_im(tl.x), neg_im(tl.y), neg_im(tl.z), neg_im(tl.w));
@@ -515,21 +516,32 @@ __global__ __launch_bounds__(64 * BPM_WAVES) void k_bp_mma(const double *__restr
             zr = mfma16h(Zrl, Bh[ks], zr);
             zi = mfma16h(Zil, Bh[ks], zi);
         }
-        // lane holds G for rows 4 g + e of the tile at its site: contract with P(row, site)
+        // lane holds G for rows 4 g + e of the tile at its site: contract with P(row, site);
+        // P walks km by 1 inside the tile and by 16 from tile to tile while kl stays
         float pr = 0.f, pi_ = 0.f;
-        int pkl = -1, pkm = 0;
+        int pkl = 0, pkm = 0;
 #pragma unroll
         for (int e = 0; e < 4; e++) {
             const int r = mt * 16 + 4 * g + e;
             if (r >= n_rows) break;
             const int kl = sKl[r], km = sKm[r];
-            if (e == 0 || kl != pkl || km != pkm + 1) {
+            if (e == 0) {
+                if (kl == p0kl && km == p0km + 16) {
+                    const float t0 = p0r * vc16 - p0i * vs16, t1 = p0r * vs16 + p0i * vc16;
+                    pr = t0; pi_ = t1;
+                } else {
+                    float s0, c0;
+                    sincospi_hw((double)kl * u + (double)km * v, s0, c0);
+                    pr = c0; pi_ = s0;
+                }
+                p0r = pr; p0i = pi_; p0kl = kl; p0km = km;
+            } else if (kl == pkl && km == pkm + 1) {
+                const float t0 = pr * vc1 - pi_ * vs1, t1 = pr * vs1 + pi_ * vc1;
+                pr = t0; pi_ = t1;
+            } else {
                 float s0, c0;
                 sincospi_hw((double)kl * u + (double)km * v, s0, c0);
                 pr = c0; pi_ = s0;
-            } else {
-                const float t0 = pr * vc1 - pi_ * vs1, t1 = pr * vs1 + pi_ * vc1;
-                pr = t0; pi_ = t1;
             }
             pkl = kl; pkm = km;
             const double im = (double)(pr * gi[e] + pi_ * gr[e]);
@@ -574,7 +586,9 @@ __device__ __forceinline__ void sf_reduce_body(const double2 *__restrict__ parti
                                                    const double2 *__restrict__ s_wall, const double *__restrict__ ak,
                                                    const int *__restrict__ kvec, double pipix, double pipiy,
                                                    double pipiz, double lamb, double2 *__restrict__ s_out,
-                                                   double4 *__restrict__ t_out, double *__restrict__ epart, int bid, int nblk) {
+                                                   double4 *__restrict__ t_out, double *__restrict__ epart, int bid, int nblk,
+                                                   unsigned *__restrict__ tpack = nullptr, const int *__restrict__ krow = nullptr,
+                                                   int kn_pad = 0, int knmax = 0) {
     constexpr int SL = 256 / SFR_K;
     __shared__ double2 slice[SL][SFR_K];
     const int kk = threadIdx.x % SFR_K, sl = threadIdx.x / SFR_K;
@@ -603,6 +617,16 @@ __device__ __forceinline__ void sf_reduce_body(const double2 *__restrict__ parti
         const double a = ak[k];
         const double kn = (double)kvec[3 * k + 2];
         t_out[k] = make_double4(a * qr, -a * qi, kn * (a * qr), kn * (-a * qi));
+        if (tpack) {   // the same, split for k_bp_mma (planes T hi, T lo, kn T hi, kn T lo)
+            const size_t base = (size_t)krow[k] * 4 * kn_pad + (size_t)(kvec[3 * k + 2] + knmax);
+            unsigned h, l;
+            split_c((float)(a * qr), (float)(-a * qi), h, l);
+            tpack[base] = h;
+            tpack[base + kn_pad] = l;
+            split_c((float)(kn * (a * qr)), (float)(kn * (-a * qi)), h, l);
+            tpack[base + 2 * kn_pad] = h;
+            tpack[base + 3 * kn_pad] = l;
+        }
         const double ek = a * (qr * qr + qi * qi);
         acc[0] = ek;
         acc[1] = a * (sr * sr + si * si);
@@ -630,9 +654,11 @@ __global__ __launch_bounds__(256) void k_sf_reduce(const double2 *__restrict__ p
                                                    const int *__restrict__ kvec, double pipix, double pipiy,
                                                    double pipiz, double lamb, double2 *__restrict__ s_out,
                                                    double4 *__restrict__ t_out, double *__restrict__ epart,
-                                                   const MtsState *__restrict__ mts) {
+                                                   const MtsState *__restrict__ mts, unsigned *__restrict__ tpack,
+                                                   const int *__restrict__ krow, int kn_pad, int knmax) {
     if (mts && !mts->exact) return;   // k partials keep the last full sum's values
-    sf_reduce_body(partial, n_sb, nk, s_wall, ak, kvec, pipix, pipiy, pipiz, lamb, s_out, t_out, epart, blockIdx.x, gridDim.x);
+    sf_reduce_body(partial, n_sb, nk, s_wall, ak, kvec, pipix, pipiy, pipiz, lamb, s_out, t_out, epart, blockIdx.x,
+                   gridDim.x, tpack, krow, kn_pad, knmax);
 }
 
 // T4[k] = (A conj S, kn A conj S): per k-vector the force pieces are Im(T e) and kn Im(T e),
