@@ -70,6 +70,7 @@ __global__ void k_epilogue(DevSys S, const double *fpart, int n_groups, const Wa
 __global__ void k_sf_mma(const double *x, const double *y, const double *z, const double *q, int n, double cux,
                          double cuy, double cuz, const KRow *rows, int n_rows, int knmax, int n_nt, double2 *partial,
                          int nk, const MtsState *mts);
+template <int NT>
 __global__ void k_bp_mma(const double *x, const double *y, const double *z, const double *q, int n, double cux,
                          double cuy, double cuz, const int2 *rowkk, const int2 *tile_ks, int n_rows, int knmax,
                          int n_nt, const unsigned *tpack, double pipix, double pipiy, double pipiz, double *fpart,
@@ -478,8 +479,8 @@ static void geometry(mdf_system *s) {
     for (size_t k = 2; k < s->kvec.size(); k += 3) s->knmax = std::max(s->knmax, std::abs(s->kvec[k]));
     s->n_nt = (2 * s->knmax + 1 + 15) / 16;
     s->sf_mma = s->sf_mma_allowed && s->par.precision == MDF_FP32_ACC64 && nk > 0 && s->n_nt <= SFM_TILES_MAX;
-    s->bp_mma = s->bp_mma_allowed && s->par.precision == MDF_FP32_ACC64 && nk > 0 && s->n_nt <= SFM_TILES_MAX &&
-                nrow <= BPM_ROWS;
+    s->bp_mma = s->bp_mma_allowed && s->par.precision == MDF_FP32_ACC64 && nk > 0 && s->n_nt >= 1 &&
+                s->n_nt <= SFM_TILES_MAX && nrow <= BPM_ROWS;
     s->n_sb_mma = std::max(1, (nm + SFM_SITES - 1) / SFM_SITES);
     s->n_rg_mma = std::max(1, (nrow + SFM_ROWS_PER_BLOCK - 1) / SFM_ROWS_PER_BLOCK);
     // back-projection row groups: about equal k counts, each at most BP_GROUP_K k-vectors (the
@@ -656,9 +657,19 @@ static void launch_bp(mdf_system *s) {
     const RecipArgs a = recip_args(s);
     DevSys &S = s->S;
     if (s->bp_mma) {
-        k_bp_mma<<<(nm + BPM_SITES - 1) / BPM_SITES, BPM_SITES * 4, 0, s->stream>>>(
-            S.x, S.y, S.z, S.q, nm, a.cux, a.cuy, a.cuz, s->d_rowkk.p, s->d_tile_ks.p, (int)s->rows.size(), s->knmax,
-            s->n_nt, s->d_tpack.p, a.pipix, a.pipiy, a.pipiz, s->fpart.p, mts_state(s));
+        const dim3 grid((nm + BPM_SITES - 1) / BPM_SITES), block(BPM_SITES * 4);
+#define BPM_LAUNCH(NT)                                                                                          \
+    case NT:                                                                                                    \
+        k_bp_mma<NT><<<grid, block, 0, s->stream>>>(S.x, S.y, S.z, S.q, nm, a.cux, a.cuy, a.cuz, s->d_rowkk.p,   \
+                                                    s->d_tile_ks.p, (int)s->rows.size(), s->knmax, s->n_nt,    \
+                                                    s->d_tpack.p, a.pipix, a.pipiy, a.pipiz, s->fpart.p,       \
+                                                    mts_state(s));                                              \
+        break;
+        switch (s->n_nt) {
+            BPM_LAUNCH(1) BPM_LAUNCH(2) BPM_LAUNCH(3) BPM_LAUNCH(4) BPM_LAUNCH(5) BPM_LAUNCH(6) BPM_LAUNCH(7)
+            BPM_LAUNCH(8)
+        }
+#undef BPM_LAUNCH
         return;
     }
     dim3 g2(std::max(1, (nm + 256 * BP_NS - 1) / (256 * BP_NS)), s->n_rg);

@@ -411,12 +411,14 @@ __global__ __launch_bounds__(256) void k_sf_mma(const double *__restrict__ x, co
 // kn + knmax] with planes T hi, T lo, kn T hi, kn T lo.  A wave owns 16 sites and keeps their
 // E fragments in registers for the whole kernel; the block copies each T tile into LDS once
 // for its 8 waves, one tile ahead (two buffers, one barrier per tile).  The contraction with
-// P and the sum over rows run per lane in FP64, and the four lanes that share a site add up
-// at the end: every site's force is complete in one wave (one partial group).
+// P runs per lane (a tile's four rows in FP32, the sum over tiles in FP64), and the four
+// lanes that share a site add up at the end: every site's force is complete in one wave
+// (one partial group).
 constexpr int BPM_WAVES = 8;
 constexpr int BPM_KP = SFM_NT_MAX * 16 + 4;  // T tile row stride (words)
 constexpr int BPM_ROWS_MAX = 1024;           // rows whose (kl, km) the block keeps in LDS
 
+template <int NT>
 __global__ __launch_bounds__(64 * BPM_WAVES) void k_bp_mma(const double *__restrict__ x, const double *__restrict__ y,
                                                            const double *__restrict__ z, const double *__restrict__ q,
                                                            int n, double cux, double cuy, double cuz,
@@ -432,7 +434,7 @@ __global__ __launch_bounds__(64 * BPM_WAVES) void k_bp_mma(const double *__restr
     const int g = lane >> 4, col = lane & 15;
     const int site = (blockIdx.x * BPM_WAVES + wave) * 16 + col;
     const bool ok = site < n;
-    const int kn_pad = 16 * n_nt, q4 = kn_pad >> 2;   // uint4 per (row, plane)
+    const int kn_pad = 16 * NT, q4 = kn_pad >> 2;   // uint4 per (row, plane); n_nt == NT
     const int n_mt = (n_rows + 15) >> 4;
     for (int r = threadIdx.x; r < n_rows; r += blockDim.x) {
         const int2 v = rowkk[r];
@@ -451,15 +453,16 @@ __global__ __launch_bounds__(64 * BPM_WAVES) void k_bp_mma(const double *__restr
     stage(0, 0);
     const double u = ok ? x[site] * cux : 0.0, v = ok ? y[site] * cuy : 0.0, w = ok ? z[site] * cuz : 0.0;
     const double qq = ok ? q[site] : 0.0;
-    // E fragments for kn columns 16 ks + 4 g .. +3 (site col)
-    h16x8 Bh[SFM_NT_MAX], Bl[SFM_NT_MAX];
+    // E fragments for kn columns 16 ks + 4 g .. +3 (site col), in the two forms the complex
+    // product needs: [Er -Ei] (real part) and [Ei Er] (imaginary part), hi and lo
+    h16x8 Brh[NT], Brl[NT], Bih[NT], Bil[NT];
     {
         float s1, c1, s16, c16, as, ac;
         sincospi_hw(w, s1, c1);
         sincospi_hw(16.0 * w, s16, c16);
         sincospi_hw((double)(4 * g - knmax) * w, as, ac);
 #pragma unroll
-        for (int ks = 0; ks < SFM_NT_MAX; ks++) {
+        for (int ks = 0; ks < NT; ks++) {
             unsigned h[4], l[4];
             float er = ac, ei = as;
 #pragma unroll
@@ -468,8 +471,10 @@ __global__ __launch_bounds__(64 * BPM_WAVES) void k_bp_mma(const double *__restr
                 const float t0 = er * c1 - ei * s1, t1 = er * s1 + ei * c1;
                 er = t0; ei = t1;
             }
-            Bh[ks] = frag4(h[0], h[1], h[2], h[3]);
-            Bl[ks] = frag4(l[0], l[1], l[2], l[3]);
+            Brh[ks] = frag4(neg_im(h[0]), neg_im(h[1]), neg_im(h[2]), neg_im(h[3]));
+            Brl[ks] = frag4(neg_im(l[0]), neg_im(l[1]), neg_im(l[2]), neg_im(l[3]));
+            Bih[ks] = frag4(swap_c(h[0]), swap_c(h[1]), swap_c(h[2]), swap_c(h[3]));
+            Bil[ks] = frag4(swap_c(l[0]), swap_c(l[1]), swap_c(l[2]), swap_c(l[3]));
             const float t0 = ac * c16 - as * s16, t1 = ac * s16 + as * c16;
             ac = t0; as = t1;
         }
@@ -487,38 +492,30 @@ __global__ __launch_bounds__(64 * BPM_WAVES) void k_bp_mma(const double *__restr
         const int2 kr = tile_ks[mt];
         f32x4 gr = {0.f, 0.f, 0.f, 0.f}, gi = gr, zr = gr, zi = gr;
 #pragma unroll
-        for (int ks = 0; ks < SFM_NT_MAX; ks++) {
+        for (int ks = 0; ks < NT; ks++) {
             if (ks < kr.x || ks > kr.y) continue;
             const int kc = 16 * ks + 4 * g;
-            const uint4 th = *reinterpret_cast<const uint4 *>(&sT[b][0][col][kc]);
-            const uint4 tl = *reinterpret_cast<const uint4 *>(&sT[b][1][col][kc]);
-            const uint4 zh = *reinterpret_cast<const uint4 *>(&sT[b][2][col][kc]);
-            const uint4 zl = *reinterpret_cast<const uint4 *>(&sT[b][3][col][kc]);
-            // real part: [Tr -Ti] . [Er Ei]; imaginary part: [Ti Tr] . [Er Ei]
-            const h16x8 Arh = frag4(neg_im(th.x), neg_im(th.y), neg_im(th.z), neg_im(th.w));
-            const h16x8 Arl = frag4(neg_im(tl.x), neg_im(tl.y), neg_im(tl.z), neg_im(tl.w));
-            const h16x8 Aih = frag4(swap_c(th.x), swap_c(th.y), swap_c(th.z), swap_c(th.w));
-            const h16x8 Ail = frag4(swap_c(tl.x), swap_c(tl.y), swap_c(tl.z), swap_c(tl.w));
-            const h16x8 Zrh = frag4(neg_im(zh.x), neg_im(zh.y), neg_im(zh.z), neg_im(zh.w));
-            const h16x8 Zrl = frag4(neg_im(zl.x), neg_im(zl.y), neg_im(zl.z), neg_im(zl.w));
-            const h16x8 Zih = frag4(swap_c(zh.x), swap_c(zh.y), swap_c(zh.z), swap_c(zh.w));
-            const h16x8 Zil = frag4(swap_c(zl.x), swap_c(zl.y), swap_c(zl.z), swap_c(zl.w));
-            gr = mfma16h(Arh, Bh[ks], gr);
-            gi = mfma16h(Aih, Bh[ks], gi);
-            zr = mfma16h(Zrh, Bh[ks], zr);
-            zi = mfma16h(Zih, Bh[ks], zi);
-            gr = mfma16h(Arh, Bl[ks], gr);
-            gi = mfma16h(Aih, Bl[ks], gi);
-            zr = mfma16h(Zrh, Bl[ks], zr);
-            zi = mfma16h(Zih, Bl[ks], zi);
-            gr = mfma16h(Arl, Bh[ks], gr);
-            gi = mfma16h(Ail, Bh[ks], gi);
-            zr = mfma16h(Zrl, Bh[ks], zr);
-            zi = mfma16h(Zil, Bh[ks], zi);
+            const h16x8 Th = *reinterpret_cast<const h16x8 *>(&sT[b][0][col][kc]);
+            const h16x8 Tl = *reinterpret_cast<const h16x8 *>(&sT[b][1][col][kc]);
+            const h16x8 Zh = *reinterpret_cast<const h16x8 *>(&sT[b][2][col][kc]);
+            const h16x8 Zl = *reinterpret_cast<const h16x8 *>(&sT[b][3][col][kc]);
+            // real part: [Tr Ti] . [Er -Ei]; imaginary part: [Tr Ti] . [Ei Er]
+            gr = mfma16h(Th, Brh[ks], gr);
+            gi = mfma16h(Th, Bih[ks], gi);
+            zr = mfma16h(Zh, Brh[ks], zr);
+            zi = mfma16h(Zh, Bih[ks], zi);
+            gr = mfma16h(Th, Brl[ks], gr);
+            gi = mfma16h(Th, Bil[ks], gi);
+            zr = mfma16h(Zh, Brl[ks], zr);
+            zi = mfma16h(Zh, Bil[ks], zi);
+            gr = mfma16h(Tl, Brh[ks], gr);
+            gi = mfma16h(Tl, Bih[ks], gi);
+            zr = mfma16h(Zl, Brh[ks], zr);
+            zi = mfma16h(Zl, Bih[ks], zi);
         }
         // lane holds G for rows 4 g + e of the tile at its site: contract with P(row, site);
         // P walks km by 1 inside the tile and by 16 from tile to tile while kl stays
-        float pr = 0.f, pi_ = 0.f;
+        float pr = 0.f, pi_ = 0.f, tx = 0.f, ty = 0.f, tz = 0.f;
         int pkl = 0, pkm = 0;
 #pragma unroll
         for (int e = 0; e < 4; e++) {
@@ -544,13 +541,16 @@ __global__ __launch_bounds__(64 * BPM_WAVES) void k_bp_mma(const double *__restr
                 pr = c0; pi_ = s0;
             }
             pkl = kl; pkm = km;
-            const double im = (double)(pr * gi[e] + pi_ * gr[e]);
-            const double imz = (double)(pr * zi[e] + pi_ * zr[e]);
-            fx += im * (pipix * (double)kl);
-            fy += im * (pipiy * (double)km);
-            fz += imz * pipiz;
+            const float im = pr * gi[e] + pi_ * gr[e];
+            tx += im * (float)kl;
+            ty += im * (float)km;
+            tz += pr * zi[e] + pi_ * zr[e];
         }
+        fx += (double)tx;   // the tile's four rows in FP32, the sum over tiles in FP64
+        fy += (double)ty;
+        fz += (double)tz;
     }
+    fx *= pipix; fy *= pipiy; fz *= pipiz;
     // the four lanes of a site (g = 0..3) hold its row groups
     fx += __shfl_xor(fx, 16, 64); fx += __shfl_xor(fx, 32, 64);
     fy += __shfl_xor(fy, 16, 64); fy += __shfl_xor(fy, 32, 64);
@@ -561,6 +561,13 @@ __global__ __launch_bounds__(64 * BPM_WAVES) void k_bp_mma(const double *__restr
         fpart[2 * (size_t)n + site] = qq * fz;
     }
 }
+
+#define BPM_INST(NT)                                                                                     \
+    template __global__ void k_bp_mma<NT>(const double *, const double *, const double *, const double *, int, \
+                                          double, double, double, const int2 *, const int2 *, int, int, int,    \
+                                          const unsigned *, double, double, double, double *, const MtsState *);
+BPM_INST(1) BPM_INST(2) BPM_INST(3) BPM_INST(4) BPM_INST(5) BPM_INST(6) BPM_INST(7) BPM_INST(8)
+#undef BPM_INST
 
 template <typename R, int NS>
 __global__ __launch_bounds__(SF_BLOCK) void k_sf(const double *__restrict__ x, const double *__restrict__ y,
