@@ -17,7 +17,8 @@ against it.  ``--precision fp64`` times the FP64 form alone.
 
 Prints one JSON line.  ``roofline`` is for the dominant stage, the reciprocal sum: its
 algorithmic work 18 flop per (k-vector, moving site) + 8 flop per ((kl,km) row, site)
-over the stage's mean kernel time, measured with HIP events on the engine's stream over a
+(FP32 arithmetic; the FP32 form runs its products on the FP16 matrix cores as hi/lo pairs,
+priced against the FP32 peak) over the stage's mean kernel time, measured with HIP events on the engine's stream over a
 second pass of the same steps run with per-stage events (the timed pass replays a captured
 HIP graph, inside which events cannot be timed).  ``cpu_baseline`` times the reference's
 own force path (built from its sources into oracle/_ref) on a few force evaluations of the
@@ -393,7 +394,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32 (f64 accumulation)",
+        "dtype": "f64" if args.precision == "fp64" else (
+            "f32 (f64 accumulation; reciprocal-sum products as FP16 hi/lo pairs on the matrix cores)"
+            if eng.reciprocal_on_matrix_cores() else "f32 (f64 accumulation)"),
         "data": "synthetic (seeded lattice water on a rock-salt slab, oscillating trajectory)",
         "config": {
             "workload": "%s: %d SPC/E waters + %d frozen ions, box %.2f x %.2f x %.2f A, rc %.1f A, "
@@ -407,7 +410,10 @@ def main():
             "kernel": "reciprocal sum (k_sf + k_sf_reduce + k_bp)",
             "bound": "mfma",
             "pipe": "FP64 vector ALU (MI355X FP64 vector peak = FP64 matrix peak)" if args.precision == "fp64"
-                    else "FP32 vector ALU (peak assumes packed FP32, which this build does not emit)",
+                    else ("FP16 matrix cores (v_mfma_f32_16x16x32_f16), each FP32 operand split into FP16 hi + lo "
+                          "parts, 3 matrix products per FP32 product; peak is MI355X's FP32 peak, the accuracy "
+                          "class the results keep" if eng.reciprocal_on_matrix_cores()
+                          else "FP32 vector ALU (peak assumes packed FP32, which this build does not emit)"),
             "achieved": achieved,
             "peak": peak,
             "unit": "TFLOP/s",

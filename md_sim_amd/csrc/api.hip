@@ -1919,6 +1919,13 @@ int mdf_mts_enable(mdf_system *s, const mdf_mts_params *p) {
     return MDF_OK;
 }
 
+int mdf_reciprocal_kernels(const mdf_system *s, int32_t *sf_matrix, int32_t *bp_matrix) {
+    if (!s) return fail(MDF_ERR_ARG, "null handle");
+    if (sf_matrix) *sf_matrix = s->sf_mma ? 1 : 0;
+    if (bp_matrix) *bp_matrix = s->bp_mma ? 1 : 0;
+    return MDF_OK;
+}
+
 int mdf_observables_enable(mdf_system *s, double rdel) {
     if (!s) return fail(MDF_ERR_ARG, "null handle");
     HIPCHK(hipSetDevice(s->device));

@@ -56,7 +56,8 @@ def lib() -> C.CDLL:
                    "mdf_set_launch_shapes", "mdf_batch_create", "mdf_batch_compute_device",
                    "mdf_batch_read_outputs", "mdf_batch_autotune", "mdf_mts_enable", "mdf_mts_get_stats",
                    "mdf_mts_get_weights", "mdf_mts_set_weights", "mdf_mts_features", "mdf_mts_eval",
-                   "mdf_observables_enable", "mdf_get_observables", "mdf_chempot_enable", "mdf_get_chempot"):
+                   "mdf_observables_enable", "mdf_get_observables", "mdf_chempot_enable", "mdf_get_chempot",
+                   "mdf_reciprocal_kernels"):
             getattr(L, fn).restype = C.c_int
         L.mdf_destroy.restype = None
         L.mdf_batch_destroy.restype = None
@@ -144,6 +145,12 @@ class ForceEngine:
         sf = np.zeros(2 * max(n.value, 1))
         _check(lib().mdf_get_structure_factor(self._h, _dp(sf), n.value))
         return kv.reshape(-1, 3), ak, sf[0:2 * n.value:2] + 1j * sf[1:2 * n.value:2]
+
+    def reciprocal_on_matrix_cores(self) -> bool:
+        """True when both reciprocal-sum kernels run on the matrix cores (the FP32 form)."""
+        a, b = C.c_int32(), C.c_int32()
+        _check(lib().mdf_reciprocal_kernels(self._h, C.byref(a), C.byref(b)))
+        return bool(a.value and b.value)
 
     # -- observables of the reference's hot loops ---------------------------------------
     def observables_enable(self, rdel: float):
