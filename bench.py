@@ -339,6 +339,7 @@ def main():
     shapes = eng.autotune(reps=10) if not args.no_autotune else None
     sec, first, last, stages = _run(eng, frames, cframes, args)
     kv, ak, _ = eng.kvectors()
+    on_mma = eng.reciprocal_on_matrix_cores()
     eng.close()
 
     # the FP64 form on the same trajectory: its throughput, and the reduced-precision
@@ -396,7 +397,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else (
             "f32 (f64 accumulation; reciprocal-sum products as FP16 hi/lo pairs on the matrix cores)"
-            if eng.reciprocal_on_matrix_cores() else "f32 (f64 accumulation)"),
+            if on_mma else "f32 (f64 accumulation)"),
         "data": "synthetic (seeded lattice water on a rock-salt slab, oscillating trajectory)",
         "config": {
             "workload": "%s: %d SPC/E waters + %d frozen ions, box %.2f x %.2f x %.2f A, rc %.1f A, "
@@ -412,7 +413,7 @@ def main():
             "pipe": "FP64 vector ALU (MI355X FP64 vector peak = FP64 matrix peak)" if args.precision == "fp64"
                     else ("FP16 matrix cores (v_mfma_f32_16x16x32_f16), each FP32 operand split into FP16 hi + lo "
                           "parts, 3 matrix products per FP32 product; peak is MI355X's FP32 peak, the accuracy "
-                          "class the results keep" if eng.reciprocal_on_matrix_cores()
+                          "class the results keep" if on_mma
                           else "FP32 vector ALU (peak assumes packed FP32, which this build does not emit)"),
             "achieved": achieved,
             "peak": peak,
