@@ -44,7 +44,7 @@ __global__ void k_sf(const double *, const double *, const double *, const doubl
                      const KRow *, int, int, double2 *, int, const MtsState *);
 __global__ void k_sf_reduce(const double2 *, int, int, const double2 *, const double *, const int *, double, double,
                             double, double, double2 *, double4 *, double *, const MtsState *, unsigned *, const int *,
-                            int, int);
+                            int, int, const float2 *);
 template <typename R, int NS>
 __global__ void k_bp(const double *, const double *, const double *, const double *, int, double, double, double,
                      const KRow *, const int *, const double4 *, double, double, double, double *, const MtsState *);
@@ -67,18 +67,33 @@ constexpr int BP_NS = 4;   // sites per back-projection thread
 __global__ void k_epilogue(DevSys S, const double *fpart, int n_groups, const WallLR *wl, int ew3dc, double crf0,
                            const double *dipz, double *epart, double *fcom, double *tq, const MtsState *mts,
                            const double *fmts, double *efi);
+template <int NT>
 __global__ void k_sf_mma(const double *x, const double *y, const double *z, const double *q, int n, double cux,
-                         double cuy, double cuz, const KRow *rows, int n_rows, int knmax, int n_nt, double2 *partial,
-                         int nk, const MtsState *mts);
+                         double cuy, double cuz, const KRow *rows, int n_rows, int knmax, int n_nt, float2 *partial,
+                         int nk, int sb_sites, const MtsState *mts);
 template <int NT>
 __global__ void k_bp_mma(const double *x, const double *y, const double *z, const double *q, int n, double cux,
                          double cuy, double cuz, const int2 *rowkk, const int2 *tile_ks, int n_rows, int knmax,
                          int n_nt, const unsigned *tpack, double pipix, double pipiy, double pipiz, double *fpart,
                          const MtsState *mts);
-constexpr int BPM_SITES = 128;   // sites per k_bp_mma block (8 waves x 16)
+constexpr int BPM_WAVES_MAX = 8; // waves per k_bp_mma block (16 sites each), at most
 constexpr int BPM_ROWS = 1024;   // rows k_bp_mma keeps in LDS
 constexpr int SFM_SITES = 512;   // sites per k_sf_mma block
 constexpr int SFM_ROWS_PER_BLOCK = 64;
+constexpr size_t SFM_LDS_MAX = 64 * 1024 - 1024;   // dynamic LDS of a k_sf_mma block, at most
+// the k_sf_mma instance for nt kn tiles of 16
+static const void *sfm_kernel(int nt) {
+    switch (nt) {
+        case 1: return reinterpret_cast<const void *>(&k_sf_mma<1>);
+        case 2: return reinterpret_cast<const void *>(&k_sf_mma<2>);
+        case 3: return reinterpret_cast<const void *>(&k_sf_mma<3>);
+        case 4: return reinterpret_cast<const void *>(&k_sf_mma<4>);
+        case 5: return reinterpret_cast<const void *>(&k_sf_mma<5>);
+        case 6: return reinterpret_cast<const void *>(&k_sf_mma<6>);
+        case 7: return reinterpret_cast<const void *>(&k_sf_mma<7>);
+        default: return reinterpret_cast<const void *>(&k_sf_mma<8>);
+    }
+}
 constexpr int SFM_TILES_MAX = 8; // kn tiles of 16 (knmax <= 63)
 __global__ void k_obs_ngc(DevSys S, ObsArgs o);
 __global__ void k_obs_efipos(DevSys S, ObsArgs o);
@@ -247,6 +262,9 @@ struct mdf_system {
     DBuf<int> d_krow;                // row of each k-vector
     DBuf<unsigned> d_tpack;          // T split for k_bp_mma, [row][plane][kn + knmax]
     int n_sb_mma = 1, n_rg_mma = 1, knmax = 0, n_nt = 0;
+    int sfm_sites = SFM_SITES, bpm_waves = BPM_WAVES_MAX;   // k_sf_mma sites per block, k_bp_mma waves
+    int sfm_sites_env = 0, bpm_waves_env = 0;                // fixed by MDF_SFM_SITES / MDF_BPM_WAVES
+    int n_cu = 256;                                          // compute units of the device
     int skin_blocks = 1, n_sb = 1, n_cg = 1, cpb = 1, n_kblk = 1, n_rg = 1, rpg = 1, n_epblk = 1, n_colblk = 1,
         colper = 1, ncells_xy = 1, n_cells_cap = 1;
     // timing: a ring of event sets, one per in-flight step; stage durations are accumulated
@@ -481,8 +499,41 @@ static void geometry(mdf_system *s) {
     s->sf_mma = s->sf_mma_allowed && s->par.precision == MDF_FP32_ACC64 && nk > 0 && s->n_nt <= SFM_TILES_MAX;
     s->bp_mma = s->bp_mma_allowed && s->par.precision == MDF_FP32_ACC64 && nk > 0 && s->n_nt >= 1 &&
                 s->n_nt <= SFM_TILES_MAX && nrow <= BPM_ROWS;
-    s->n_sb_mma = std::max(1, (nm + SFM_SITES - 1) / SFM_SITES);
-    s->n_rg_mma = std::max(1, (nrow + SFM_ROWS_PER_BLOCK - 1) / SFM_ROWS_PER_BLOCK);
+    // k_sf_mma: row groups of at most four 16-row tiles (spread evenly by the kernel); then as
+    // many site blocks as fill the resident block slots in whole rounds, 32 sites a stage
+    {
+        const int n_mt = (nrow + 15) / 16;
+        s->n_rg_mma = std::max(1, (n_mt + SFM_ROWS_PER_BLOCK / 16 - 1) / (SFM_ROWS_PER_BLOCK / 16));
+        // sites per block: the least time on the busiest compute unit, its blocks running in
+        // rounds of as many as stay resident (dynamic LDS sets that), a round of c blocks at a
+        // throughput that grows slower than c (latency hiding)
+        int sites = s->sfm_sites_env > 0 ? s->sfm_sites_env : SFM_SITES;
+        while (sites > 32 && sfm_lds_bytes(sites, s->n_nt) > SFM_LDS_MAX) sites -= 32;
+        if (s->sfm_sites_env <= 0 && nm > 0 && s->sf_mma) {
+            auto thr = [](long c) { return c >= 4 ? 1.5 : c == 3 ? 1.3 : c == 2 ? 1.0 : 0.6; };
+            double best = 1e300;
+            for (int cand : {512, 384, 256, 128}) {
+                if (sfm_lds_bytes(cand, s->n_nt) > SFM_LDS_MAX) continue;
+                int occ = 0;
+                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, sfm_kernel(s->n_nt), 256,
+                                                                 sfm_lds_bytes(cand, s->n_nt)) != hipSuccess ||
+                    occ < 1) {
+                    (void)hipGetLastError();
+                    continue;
+                }
+                const long blocks = (long)((nm + cand - 1) / cand) * s->n_rg_mma;
+                const long per_cu = (blocks + s->n_cu - 1) / s->n_cu;
+                const long full = per_cu / occ, rem = per_cu % occ;
+                const double cost = full * occ * cand / thr(occ) + (rem ? rem * cand / thr(rem) : 0.0);
+                if (cost < best) { best = cost; sites = cand; }
+            }
+        }
+        s->sfm_sites = sites;
+        s->n_sb_mma = std::max(1, (nm + sites - 1) / sites);
+        // k_bp_mma: the most waves per block, over which the block's staging of each T tile
+        // spreads (measured faster than balancing blocks over the compute units)
+        s->bpm_waves = s->bpm_waves_env > 0 ? s->bpm_waves_env : BPM_WAVES_MAX;
+    }
     // back-projection row groups: about equal k counts, each at most BP_GROUP_K k-vectors (the
     // LDS stage), and enough groups that sites x groups fill the GPU
     int nsb_bp = std::max(1, (nm + 256 * BP_NS - 1) / (256 * BP_NS));
@@ -637,9 +688,20 @@ static void launch_sf(mdf_system *s) {
     const RecipArgs a = recip_args(s);
     DevSys &S = s->S;
     if (s->sf_mma) {
-        k_sf_mma<<<dim3(s->n_sb_mma, s->n_rg_mma), 256, 0, s->stream>>>(S.x, S.y, S.z, S.q, nm, a.cux, a.cuy, a.cuz,
-                                                                       s->d_rows.p, nrows, s->knmax, s->n_nt,
-                                                                       s->sf_partial.p, nk, mts_state(s));
+        const dim3 grid(s->n_sb_mma, s->n_rg_mma);
+        const size_t lds = sfm_lds_bytes(s->sfm_sites, s->n_nt);
+        float2 *part = reinterpret_cast<float2 *>(s->sf_partial.p);
+#define SFM_LAUNCH(NT)                                                                                          \
+    case NT:                                                                                                    \
+        k_sf_mma<NT><<<grid, 256, lds, s->stream>>>(S.x, S.y, S.z, S.q, nm, a.cux, a.cuy, a.cuz, s->d_rows.p,     \
+                                                    nrows, s->knmax, s->n_nt, part, nk, s->sfm_sites,          \
+                                                    mts_state(s));                                              \
+        break;
+        switch (s->n_nt) {
+            SFM_LAUNCH(1) SFM_LAUNCH(2) SFM_LAUNCH(3) SFM_LAUNCH(4) SFM_LAUNCH(5) SFM_LAUNCH(6) SFM_LAUNCH(7)
+            SFM_LAUNCH(8)
+        }
+#undef SFM_LAUNCH
         return;
     }
     dim3 g1(s->n_sb, s->n_cg);
@@ -657,7 +719,8 @@ static void launch_bp(mdf_system *s) {
     const RecipArgs a = recip_args(s);
     DevSys &S = s->S;
     if (s->bp_mma) {
-        const dim3 grid((nm + BPM_SITES - 1) / BPM_SITES), block(BPM_SITES * 4);
+        const int ns = 16 * s->bpm_waves;
+        const dim3 grid((nm + ns - 1) / ns), block(64 * s->bpm_waves);
 #define BPM_LAUNCH(NT)                                                                                          \
     case NT:                                                                                                    \
         k_bp_mma<NT><<<grid, block, 0, s->stream>>>(S.x, S.y, S.z, S.q, nm, a.cux, a.cuy, a.cuz, s->d_rowkk.p,   \
@@ -731,11 +794,13 @@ static void launch_forces(mdf_system *s) {
     if (s->ev) (void)hipEventRecord(s->ev[4], st);
     if (nk > 0) {
         double lamb = 1.0 / (4.0 * S.alfa * S.alfa);
-        k_sf_reduce<<<s->n_kblk, 256, 0, st>>>(s->sf_partial.p, nm > 0 ? (s->sf_mma ? s->n_sb_mma : s->n_sb) : 0, nk,
+        k_sf_reduce<<<s->n_kblk, SFR_THREADS, 0, st>>>(s->sf_partial.p, nm > 0 ? (s->sf_mma ? s->n_sb_mma : s->n_sb) : 0, nk,
                                                s->d_swall.p, s->d_ak.p,
                                                s->d_kvec.p, ra.pipix, ra.pipiy, ra.pipiz, lamb, s->d_s.p, s->d_t.p,
                                                s->kpart.p, mts_state(s), s->bp_mma ? s->d_tpack.p : nullptr,
-                                               s->d_krow.p, 16 * s->n_nt, s->knmax);
+                                               s->d_krow.p, 16 * s->n_nt, s->knmax,
+                                               s->sf_mma && nm > 0 ? reinterpret_cast<const float2 *>(s->sf_partial.p)
+                                                                   : nullptr);
     }
     if (s->ev) (void)hipEventRecord(s->ev[5], st);
     launch_bp(s);
@@ -1092,9 +1157,19 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
         if (sb && atoi(sb) > 0) s->sf_target_blocks = atoi(sb);
         const char *bb = getenv("MDF_BP_BLOCKS");
         if (bb && atoi(bb) > 0) s->bp_target_blocks = atoi(bb);
+        const char *ss = getenv("MDF_SFM_SITES");
+        if (ss && atoi(ss) > 0) s->sfm_sites_env = std::min(SFM_SITES, 32 * ((atoi(ss) + 31) / 32));
+        const char *bw = getenv("MDF_BPM_WAVES");
+        if (bw && atoi(bw) > 0) s->bpm_waves_env = std::min(BPM_WAVES_MAX, atoi(bw));
     }
     hipError_t he = hipSetDevice(par->device);
     if (he != hipSuccess) { delete s; return fail(MDF_ERR_HIP, "hipSetDevice: %s", hipGetErrorString(he)); }
+    {   // compute units size the matrix-core launches
+        int cu = 0;
+        if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, par->device) == hipSuccess && cu > 0)
+            s->n_cu = cu;
+        (void)hipGetLastError();
+    }
     int rc = MDF_OK;
 #define TRY(x)                        \
     do {                              \
@@ -1405,9 +1480,9 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
                 k_sf<double, 4><<<dim3(n_sb_w, ncg), SF_BLOCK, 0, s->stream>>>(
                     S.x + t.n_move, S.y + t.n_move, S.z + t.n_move, S.q + t.n_move, nw, cux, cuy, cuz,
                     s->d_rows.p, nrow, rpb, s->sf_partial.p, nk, nullptr);
-                k_sf_reduce<<<s->n_kblk, 256, 0, s->stream>>>(s->sf_partial.p, n_sb_w, nk, s->d_swall.p, s->d_ak.p,
+                k_sf_reduce<<<s->n_kblk, SFR_THREADS, 0, s->stream>>>(s->sf_partial.p, n_sb_w, nk, s->d_swall.p, s->d_ak.p,
                                                               s->d_kvec.p, 1.0, 1.0, 1.0, 0.0, s->d_s.p, s->d_t.p,
-                                                              s->kpart.p, nullptr, nullptr, nullptr, 0, 0);
+                                                              s->kpart.p, nullptr, nullptr, nullptr, 0, 0, nullptr);
                 HTRY(hipStreamSynchronize(s->stream));
                 HTRY(hipMemcpy(s->s_wall.data(), s->d_s.p, sizeof(double2) * nk, hipMemcpyDeviceToHost));
                 HTRY(hipMemcpy(s->d_swall.p, s->d_s.p, sizeof(double2) * nk, hipMemcpyDeviceToDevice));
@@ -2405,7 +2480,7 @@ static void batch_launch(mdf_batch *b) {
         if (b->f32) k_sf_b<float><<<grid(BK_SF), SF_BLOCK, 0, st>>>(R, b->maps[BK_SF].p);
         else k_sf_b<double><<<grid(BK_SF), SF_BLOCK, 0, st>>>(R, b->maps[BK_SF].p);
     }
-    if (grid(BK_SFR)) k_sf_reduce_b<<<grid(BK_SFR), 256, 0, st>>>(R, b->maps[BK_SFR].p);
+    if (grid(BK_SFR)) k_sf_reduce_b<<<grid(BK_SFR), SFR_THREADS, 0, st>>>(R, b->maps[BK_SFR].p);
     if (grid(BK_BP)) {
         if (b->f32) k_bp_b<float><<<grid(BK_BP), 256, 0, st>>>(R, b->maps[BK_BP].p);
         else k_bp_b<double><<<grid(BK_BP), 256, 0, st>>>(R, b->maps[BK_BP].p);

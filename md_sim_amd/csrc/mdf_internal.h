@@ -26,7 +26,14 @@ constexpr int KC = 16;           // k-vectors per structure-factor chunk held in
 constexpr int SF_SITES = 4;      // sites per thread in the structure-factor kernel
 constexpr int SF_BLOCK = 256;
 constexpr int SFR_K = 64;        // k-vectors per structure-factor reduction block (one wave wide)
+constexpr int SFR_THREADS = 1024; // threads of a reduction block: 16 slices over the site blocks
 constexpr int BP_ROWGROUPS_MAX = 64;
+// dynamic LDS of k_sf_mma for a block of `sites` sites and n_nt kn tiles: cos/sin steps
+// (float4), site phases (3 doubles) and charges per site, then one 32-site stage's split E
+// tables (kn stride 16 n_nt + 4 words) and P tables (row stride 68 words), hi and lo each
+__host__ __device__ inline size_t sfm_lds_bytes(int sites, int n_nt) {
+    return (size_t)sites * 44 + (size_t)2 * 32 * (16 * n_nt + 4) * 4 + (size_t)2 * 32 * 68 * 4;
+}
 constexpr int BP_GROUP_K = 1024;     // k-vectors per back-projection row group (its LDS stage)
 
 // k-set rows (contiguous kn) and structure-factor chunks (<= KC consecutive kn of a row)

@@ -193,13 +193,14 @@ __device__ __forceinline__ void sf_body(const double *__restrict__ x, const doub
 // and a (sites x kn) matrix.  As real operands, A = [Pr Pi] per site pair of k entries and
 // B = [Er -Ei] (real part) or [Ei Er] (imaginary part).  Each FP32 operand is split into two
 // FP16 parts, v = hi + lo, and A B ~ A_hi B_hi + A_hi B_lo + A_lo B_hi (the dropped lo lo
-// term is 2^-22 relative), accumulated in FP32 by v_mfma_f32_16x16x32_f16 over a 512-site
-// block and written as FP64 partials - the same partial layout k_sf_reduce sums.  Operand
+// term is 2^-22 relative), accumulated in FP32 by v_mfma_f32_16x16x32_f16 over a block of
+// sites and written as FP32 partial rows, which k_sf_reduce sums in FP64.  Operand
 // layout of that instruction (one wave): lane l holds A[l%16][8(l/16)..+7], B[8(l/16)..+7][l%16]
 // and D[4(l/16)+e][l%16]; 8 k entries are 4 complex values.
-// A block takes 512 sites and 64 rows (one 16-row tile per wave); per 32-site stage all
-// threads build the P and E tables (split, in LDS) by recursion in steps of 8 along km and
-// kn, then each wave runs its tile over the kn tiles its rows reach.
+// A block takes up to 512 sites (sb_sites, a multiple of 32) and a row group of up to four
+// 16-row tiles, one per wave: the row tiles are spread evenly over the gridDim.y groups.  Per
+// 32-site stage all threads build the P and E tables (split, in LDS) by recursion in steps of
+// 8 along km and kn, then each wave runs its tile over the kn tiles its rows reach.
 typedef _Float16 h16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
@@ -250,26 +251,38 @@ __device__ __forceinline__ int wave_max(int v) {
     return v;
 }
 
+template <int NT>
 __global__ __launch_bounds__(256) void k_sf_mma(const double *__restrict__ x, const double *__restrict__ y,
                                                 const double *__restrict__ z, const double *__restrict__ q, int n,
                                                 double cux, double cuy, double cuz, const KRow *__restrict__ rows,
-                                                int n_rows, int knmax, int n_nt, double2 *__restrict__ partial,
-                                                int nk, const MtsState *__restrict__ mts) {
+                                                int n_rows, int knmax, int n_nt, float2 *__restrict__ partial,
+                                                int nk, int sb_sites, const MtsState *__restrict__ mts) {
     if (mts && !mts->exact) return;   // a surrogate step: no k-sum
-    __shared__ unsigned sEh[SFM_SUB][SFM_KP], sEl[SFM_SUB][SFM_KP];
-    __shared__ unsigned sPh[SFM_SUB][SFM_RP], sPl[SFM_SUB][SFM_RP];
-    __shared__ double sU[SFM_SB], sV[SFM_SB], sW[SFM_SB];   // site phases in half turns
-    __shared__ float4 sStep[SFM_SB];                        // cos, sin of 8 pi v and of 8 pi w
-    __shared__ float sQ[SFM_SB];
+    // dynamic LDS, laid out as sfm_lds_bytes(sb_sites, n_nt) counts it
+    extern __shared__ __align__(16) unsigned char sfm_smem[];
+    const int kp = 16 * n_nt + 4;   // E row stride: = 4 mod 8 like the P stride
+    float4 *sStep = reinterpret_cast<float4 *>(sfm_smem);              // cos, sin of 8 pi v and of 8 pi w
+    double *sU = reinterpret_cast<double *>(sfm_smem + 16 * sb_sites);   // site phases in half turns
+    double *sV = sU + sb_sites, *sW = sV + sb_sites;
+    float *sQ = reinterpret_cast<float *>(sW + sb_sites);
+    unsigned *sEh = reinterpret_cast<unsigned *>(sQ + sb_sites);
+    unsigned *sEl = sEh + SFM_SUB * kp;
+    unsigned *sPh = sEl + SFM_SUB * kp;
+    unsigned *sPl = sPh + SFM_SUB * SFM_RP;
     __shared__ int sKl[SFM_ROWS], sKm[SFM_ROWS], sFresh[SFM_ROWS];
     const int sb = blockIdx.x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int row0 = blockIdx.y * SFM_ROWS;
+    // the group's row tiles [t0, t1) of n_mt, at most SFM_ROWS / 16 of them
+    const int n_mt = (n_rows + 15) >> 4;
+    const int t0 = (int)(((long long)blockIdx.y * n_mt) / gridDim.y);
+    const int t1 = (int)(((long long)(blockIdx.y + 1) * n_mt) / gridDim.y);
+    const int row0 = t0 * 16;
+    const int nr_g = min(16 * (t1 - t0), n_rows - row0);   // rows of the group
     const int wrow0 = row0 + wave * 16;
-    const bool active = wrow0 < n_rows;   // uniform over the wave
-    const int s_begin = sb * SFM_SB, s_end = min(n, s_begin + SFM_SB);
+    const bool active = wave < t1 - t0;   // uniform over the wave
+    const int s_begin = sb * sb_sites, s_end = min(n, s_begin + sb_sites);
     // prologue: the block's sites and rows, once
-    for (int t = threadIdx.x; t < SFM_SB; t += blockDim.x) {
+    for (int t = threadIdx.x; t < sb_sites; t += blockDim.x) {
         const int i = s_begin + t;
         const bool ok = i < s_end;
         const double u = ok ? x[i] * cux : 0.0, v = ok ? y[i] * cuy : 0.0, w = ok ? z[i] * cuz : 0.0;
@@ -283,7 +296,7 @@ __global__ __launch_bounds__(256) void k_sf_mma(const double *__restrict__ x, co
     if (threadIdx.x < SFM_ROWS) {
         const int r = row0 + threadIdx.x;
         int kl = 0, km = 0, fresh = 1;
-        if (r < n_rows) {
+        if ((int)threadIdx.x < nr_g) {
             const KRow row = rows[r];
             kl = row.kl; km = row.km;
             if (threadIdx.x >= 8) {   // P walks rows in steps of 8: km + 8 in the same kl continues
@@ -302,9 +315,9 @@ __global__ __launch_bounds__(256) void k_sf_mma(const double *__restrict__ x, co
     lo = wave_min(lo);
     hi = wave_max(hi);
     const int nt_lo = active ? (lo + knmax) >> 4 : 1, nt_hi = active ? (hi + knmax) >> 4 : 0;
-    f32x4 accr[SFM_NT_MAX], acci[SFM_NT_MAX];
+    f32x4 accr[NT], acci[NT];
 #pragma unroll
-    for (int t = 0; t < SFM_NT_MAX; t++) {
+    for (int t = 0; t < NT; t++) {
         accr[t] = f32x4{0.f, 0.f, 0.f, 0.f};
         acci[t] = f32x4{0.f, 0.f, 0.f, 0.f};
     }
@@ -325,10 +338,10 @@ __global__ __launch_bounds__(256) void k_sf_mma(const double *__restrict__ x, co
                 if (kn <= knmax) {
                     unsigned h, l;
                     split_c(ec, es, h, l);
-                    sEh[gj][knmax + kn] = h;
-                    sEl[gj][knmax + kn] = l;
-                    sEh[gj][knmax - kn] = neg_im(h);
-                    sEl[gj][knmax - kn] = neg_im(l);
+                    sEh[gj * kp + knmax + kn] = h;
+                    sEl[gj * kp + knmax + kn] = l;
+                    sEh[gj * kp + knmax - kn] = neg_im(h);
+                    sEl[gj * kp + knmax - kn] = neg_im(l);
                 }
                 const float nr = ec * st.z - es * st.w, ni = ec * st.w + es * st.z;
                 ec = nr; es = ni;
@@ -341,6 +354,7 @@ __global__ __launch_bounds__(256) void k_sf_mma(const double *__restrict__ x, co
 #pragma unroll
             for (int rr = 0; rr < 8; rr++) {
                 const int lr = gs + 8 * rr;
+                if (8 * rr >= nr_g) break;   // uniform: no row of the group from here on
                 if (sFresh[lr]) {
                     float s0, cc;
                     sincospi_hw((double)sKl[lr] * u + (double)sKm[lr] * v, s0, cc);
@@ -351,8 +365,8 @@ __global__ __launch_bounds__(256) void k_sf_mma(const double *__restrict__ x, co
                 }
                 unsigned h, l;
                 split_c(qq * pr, qq * pi_, h, l);
-                sPh[gj][lr] = h;
-                sPl[gj][lr] = l;
+                sPh[gj * SFM_RP + lr] = h;
+                sPl[gj * SFM_RP + lr] = l;
             }
         }
         __syncthreads();
@@ -362,18 +376,20 @@ __global__ __launch_bounds__(256) void k_sf_mma(const double *__restrict__ x, co
             for (int ks = 0; ks < SFM_SUB / 16; ks++) {
                 const int sa = ks * 16 + g4;
                 // real part: [Pr -Pi] . [Er Ei]; imaginary part: [Pi Pr] . [Er Ei]
-                const unsigned ph0 = sPh[sa][ar], ph1 = sPh[sa + 1][ar], ph2 = sPh[sa + 2][ar], ph3 = sPh[sa + 3][ar];
-                const unsigned pl0 = sPl[sa][ar], pl1 = sPl[sa + 1][ar], pl2 = sPl[sa + 2][ar], pl3 = sPl[sa + 3][ar];
+                const unsigned *pph = sPh + sa * SFM_RP + ar, *ppl = sPl + sa * SFM_RP + ar;
+                const unsigned ph0 = pph[0], ph1 = pph[SFM_RP], ph2 = pph[2 * SFM_RP], ph3 = pph[3 * SFM_RP];
+                const unsigned pl0 = ppl[0], pl1 = ppl[SFM_RP], pl2 = ppl[2 * SFM_RP], pl3 = ppl[3 * SFM_RP];
                 const h16x8 Arh = frag4(neg_im(ph0), neg_im(ph1), neg_im(ph2), neg_im(ph3));
                 const h16x8 Arl = frag4(neg_im(pl0), neg_im(pl1), neg_im(pl2), neg_im(pl3));
                 const h16x8 Aih = frag4(swap_c(ph0), swap_c(ph1), swap_c(ph2), swap_c(ph3));
                 const h16x8 Ail = frag4(swap_c(pl0), swap_c(pl1), swap_c(pl2), swap_c(pl3));
 #pragma unroll
-                for (int t = 0; t < SFM_NT_MAX; t++) {
+                for (int t = 0; t < NT; t++) {
                     if (t < nt_lo || t > nt_hi) continue;
                     const int ec = t * 16 + (lane & 15);
-                    const h16x8 Bh = frag4(sEh[sa][ec], sEh[sa + 1][ec], sEh[sa + 2][ec], sEh[sa + 3][ec]);
-                    const h16x8 Bl = frag4(sEl[sa][ec], sEl[sa + 1][ec], sEl[sa + 2][ec], sEl[sa + 3][ec]);
+                    const unsigned *peh = sEh + sa * kp + ec, *pel = sEl + sa * kp + ec;
+                    const h16x8 Bh = frag4(peh[0], peh[kp], peh[2 * kp], peh[3 * kp]);
+                    const h16x8 Bl = frag4(pel[0], pel[kp], pel[2 * kp], pel[3 * kp]);
                     accr[t] = mfma16h(Arh, Bh, accr[t]);
                     acci[t] = mfma16h(Aih, Bh, acci[t]);
                     accr[t] = mfma16h(Arh, Bl, accr[t]);
@@ -388,7 +404,7 @@ __global__ __launch_bounds__(256) void k_sf_mma(const double *__restrict__ x, co
     if (!active) return;
     // lane holds rows 4 (lane/16) + e of the tile at kn column lane % 16
 #pragma unroll
-    for (int t = 0; t < SFM_NT_MAX; t++) {
+    for (int t = 0; t < NT; t++) {
         if (t < nt_lo || t > nt_hi) continue;
         const int kn = t * 16 + (lane & 15) - knmax;
 #pragma unroll
@@ -397,10 +413,17 @@ __global__ __launch_bounds__(256) void k_sf_mma(const double *__restrict__ x, co
             if (r >= n_rows) continue;
             const KRow row = rows[r];
             if (kn < row.kn0 || kn > row.kn1) continue;
-            partial[(size_t)sb * nk + row.koff + (kn - row.kn0)] = make_double2((double)accr[t][e], (double)acci[t][e]);
+            partial[(size_t)sb * nk + row.koff + (kn - row.kn0)] = make_float2(accr[t][e], acci[t][e]);
         }
     }
 }
+
+#define SFM_INST(NT)                                                                                            \
+    template __global__ void k_sf_mma<NT>(const double *, const double *, const double *, const double *, int,   \
+                                          double, double, double, const KRow *, int, int, int, float2 *, int, int, \
+                                          const MtsState *);
+SFM_INST(1) SFM_INST(2) SFM_INST(3) SFM_INST(4) SFM_INST(5) SFM_INST(6) SFM_INST(7) SFM_INST(8)
+#undef SFM_INST
 
 // ---- back-projection on the matrix cores (FP32 form) ---------------------------------------
 // F_j = q_j sum_rows c(row) Im(P_j(row) G_j(row)) with G_j(row) = sum_kn T(row, kn) E_j(kn),
@@ -410,11 +433,11 @@ __global__ __launch_bounds__(256) void k_sf_mma(const double *__restrict__ x, co
 // leaves T already split, in a dense zero-padded layout tpack[(row * 4 + plane) * kn_pad +
 // kn + knmax] with planes T hi, T lo, kn T hi, kn T lo.  A wave owns 16 sites and keeps their
 // E fragments in registers for the whole kernel; the block copies each T tile into LDS once
-// for its 8 waves, one tile ahead (two buffers, one barrier per tile).  The contraction with
+// for its waves, one tile ahead (two buffers, one barrier per tile).  The contraction with
 // P runs per lane (a tile's four rows in FP32, the sum over tiles in FP64), and the four
 // lanes that share a site add up at the end: every site's force is complete in one wave
 // (one partial group).
-constexpr int BPM_WAVES = 8;
+constexpr int BPM_WAVES = 8;                 // most waves per block (blockDim.x / 64 may be fewer)
 constexpr int BPM_KP = SFM_NT_MAX * 16 + 4;  // T tile row stride (words)
 constexpr int BPM_ROWS_MAX = 1024;           // rows whose (kl, km) the block keeps in LDS
 
@@ -432,7 +455,7 @@ __global__ __launch_bounds__(64 * BPM_WAVES) void k_bp_mma(const double *__restr
     __shared__ int sKl[BPM_ROWS_MAX], sKm[BPM_ROWS_MAX];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, col = lane & 15;
-    const int site = (blockIdx.x * BPM_WAVES + wave) * 16 + col;
+    const int site = (blockIdx.x * (int)(blockDim.x >> 6) + wave) * 16 + col;
     const bool ok = site < n;
     const int kn_pad = 16 * NT, q4 = kn_pad >> 2;   // uint4 per (row, plane); n_nt == NT
     const int n_mt = (n_rows + 15) >> 4;
@@ -587,21 +610,29 @@ template __global__ void k_sf<float, 4>(const double *, const double *, const do
 
 // S(k) = sum over site blocks + static frozen part; T(k) = A_k conj(S(k)); per-block
 // partial energies and virial:  eK = A|S|^2, eLL = A|S_moving|^2, vir_ab = 1/2 eK (d_ab - g k_a k_b).
-// A block owns SFR_K k-vectors; its four waves each sum every fourth site block (fixed
-// order), the first wave combines the four slices and finishes the k-vectors.
+// A block of SFR_THREADS owns SFR_K k-vectors; each of its waves sums every SFR_THREADS /
+// SFR_K-th site block (fixed order, many loads in flight), the first wave combines the
+// slices and finishes the k-vectors.
 __device__ __forceinline__ void sf_reduce_body(const double2 *__restrict__ partial, int n_sb, int nk,
                                                    const double2 *__restrict__ s_wall, const double *__restrict__ ak,
                                                    const int *__restrict__ kvec, double pipix, double pipiy,
                                                    double pipiz, double lamb, double2 *__restrict__ s_out,
                                                    double4 *__restrict__ t_out, double *__restrict__ epart, int bid, int nblk,
                                                    unsigned *__restrict__ tpack = nullptr, const int *__restrict__ krow = nullptr,
-                                                   int kn_pad = 0, int knmax = 0) {
-    constexpr int SL = 256 / SFR_K;
+                                                   int kn_pad = 0, int knmax = 0,
+                                                   const float2 *__restrict__ partial32 = nullptr) {
+    constexpr int SL = SFR_THREADS / SFR_K;
     __shared__ double2 slice[SL][SFR_K];
     const int kk = threadIdx.x % SFR_K, sl = threadIdx.x / SFR_K;
     const int k = bid * SFR_K + kk;
     double sr = 0.0, si = 0.0;
-    if (k < nk) {
+    if (k < nk && partial32) {   // FP32 partial rows (the matrix-core structure factor)
+#pragma unroll 4
+        for (int b = sl; b < n_sb; b += SL) {
+            const float2 v = partial32[(size_t)b * nk + k];
+            sr += (double)v.x; si += (double)v.y;
+        }
+    } else if (k < nk) {
 #pragma unroll 4
         for (int b = sl; b < n_sb; b += SL) {
             const double2 v = partial[(size_t)b * nk + k];
@@ -656,16 +687,17 @@ __device__ __forceinline__ void sf_reduce_body(const double2 *__restrict__ parti
         for (int c = 0; c < RE_N; c++) epart[bid * RE_N + c] = acc[c];
 }
 
-__global__ __launch_bounds__(256) void k_sf_reduce(const double2 *__restrict__ partial, int n_sb, int nk,
+__global__ __launch_bounds__(SFR_THREADS) void k_sf_reduce(const double2 *__restrict__ partial, int n_sb, int nk,
                                                    const double2 *__restrict__ s_wall, const double *__restrict__ ak,
                                                    const int *__restrict__ kvec, double pipix, double pipiy,
                                                    double pipiz, double lamb, double2 *__restrict__ s_out,
                                                    double4 *__restrict__ t_out, double *__restrict__ epart,
                                                    const MtsState *__restrict__ mts, unsigned *__restrict__ tpack,
-                                                   const int *__restrict__ krow, int kn_pad, int knmax) {
+                                                   const int *__restrict__ krow, int kn_pad, int knmax,
+                                                   const float2 *__restrict__ partial32) {
     if (mts && !mts->exact) return;   // k partials keep the last full sum's values
     sf_reduce_body(partial, n_sb, nk, s_wall, ak, kvec, pipix, pipiy, pipiz, lamb, s_out, t_out, epart, blockIdx.x,
-                   gridDim.x, tpack, krow, kn_pad, knmax);
+                   gridDim.x, tpack, krow, kn_pad, knmax, partial32);
 }
 
 // T4[k] = (A conj S, kn A conj S): per k-vector the force pieces are Im(T e) and kn Im(T e),
@@ -798,7 +830,7 @@ __global__ __launch_bounds__(SF_BLOCK) void k_sf_b(const RepArgs *__restrict__ R
 template __global__ void k_sf_b<double>(const RepArgs *, const int2 *);
 template __global__ void k_sf_b<float>(const RepArgs *, const int2 *);
 
-__global__ __launch_bounds__(256) void k_sf_reduce_b(const RepArgs *__restrict__ RA, const int2 *__restrict__ map) {
+__global__ __launch_bounds__(SFR_THREADS) void k_sf_reduce_b(const RepArgs *__restrict__ RA, const int2 *__restrict__ map) {
     const int2 m = map[blockIdx.x];
     const RepArgs &r = RA[m.x];
     sf_reduce_body(r.sf_partial, r.S.n_move > 0 ? r.n_sb : 0, r.nk, r.s_wall, r.ak, r.kvec, r.pipix, r.pipiy, r.pipiz,
