@@ -530,9 +530,10 @@ static void geometry(mdf_system *s) {
         }
         s->sfm_sites = sites;
         s->n_sb_mma = std::max(1, (nm + sites - 1) / sites);
-        // k_bp_mma: the most waves per block, over which the block's staging of each T tile
-        // spreads (measured faster than balancing blocks over the compute units)
-        s->bpm_waves = s->bpm_waves_env > 0 ? s->bpm_waves_env : BPM_WAVES_MAX;
+        // k_bp_mma: many waves per block, over which the block's staging of each T tile spreads
+        // (measured faster than balancing blocks over the compute units), two blocks resident
+        // per unit: 4 waves per SIMD up to 5 kn tiles, 3 beyond (its registers)
+        s->bpm_waves = s->bpm_waves_env > 0 ? s->bpm_waves_env : (s->n_nt <= 5 ? BPM_WAVES_MAX : 6);
     }
     // back-projection row groups: about equal k counts, each at most BP_GROUP_K k-vectors (the
     // LDS stage), and enough groups that sites x groups fill the GPU

@@ -237,6 +237,20 @@ __device__ __forceinline__ h16x8 frag4(unsigned a, unsigned b, unsigned c, unsig
 }
 __device__ __forceinline__ unsigned neg_im(unsigned c) { return c ^ 0x80000000u; }          // (re, -im)
 __device__ __forceinline__ unsigned swap_c(unsigned c) { return (c >> 16) | (c << 16); }    // (im, re)
+// four packed (re, -im) pairs -> (im, re): swap the halves, then flip the new low half's
+// sign.  Two VALU instructions per pair, written as volatile asm so that the compiler
+// recomputes them where they are used instead of keeping a loop-invariant copy in registers.
+__device__ __forceinline__ unsigned conj_swap1(unsigned c) {
+    unsigned r;
+    asm volatile("v_alignbit_b32 %0, %1, %1, 16\n\tv_xor_b32 %0, 0x8000, %0" : "=&v"(r) : "v"(c));
+    return r;
+}
+__device__ __forceinline__ h16x8 conj_swap8(h16x8 v) {
+    uint4 u = __builtin_bit_cast(uint4, v);
+    u.x = conj_swap1(u.x); u.y = conj_swap1(u.y);
+    u.z = conj_swap1(u.z); u.w = conj_swap1(u.w);
+    return __builtin_bit_cast(h16x8, u);
+}
 
 __device__ __forceinline__ f32x4 mfma16h(h16x8 a, h16x8 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
@@ -438,11 +452,10 @@ SFM_INST(1) SFM_INST(2) SFM_INST(3) SFM_INST(4) SFM_INST(5) SFM_INST(6) SFM_INST
 // lanes that share a site add up at the end: every site's force is complete in one wave
 // (one partial group).
 constexpr int BPM_WAVES = 8;                 // most waves per block (blockDim.x / 64 may be fewer)
-constexpr int BPM_KP = SFM_NT_MAX * 16 + 4;  // T tile row stride (words)
 constexpr int BPM_ROWS_MAX = 1024;           // rows whose (kl, km) the block keeps in LDS
 
 template <int NT>
-__global__ __launch_bounds__(64 * BPM_WAVES) void k_bp_mma(const double *__restrict__ x, const double *__restrict__ y,
+__global__ __launch_bounds__(64 * BPM_WAVES) __attribute__((amdgpu_waves_per_eu(NT <= 5 ? 4 : 1))) void k_bp_mma(const double *__restrict__ x, const double *__restrict__ y,
                                                            const double *__restrict__ z, const double *__restrict__ q,
                                                            int n, double cux, double cuy, double cuz,
                                                            const int2 *__restrict__ rowkk,
@@ -451,7 +464,7 @@ __global__ __launch_bounds__(64 * BPM_WAVES) void k_bp_mma(const double *__restr
                                                            double pipiy, double pipiz, double *__restrict__ fpart,
                                                            const MtsState *__restrict__ mts) {
     if (mts && !mts->exact) return;
-    __shared__ unsigned sT[2][4][16][BPM_KP];
+    __shared__ unsigned sT[2][4][16][16 * NT + 4];   // row stride = 4 mod 8 words
     __shared__ int sKl[BPM_ROWS_MAX], sKm[BPM_ROWS_MAX];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, col = lane & 15;
@@ -478,7 +491,7 @@ __global__ __launch_bounds__(64 * BPM_WAVES) void k_bp_mma(const double *__restr
     const double qq = ok ? q[site] : 0.0;
     // E fragments for kn columns 16 ks + 4 g .. +3 (site col), in the two forms the complex
     // product needs: [Er -Ei] (real part) and [Ei Er] (imaginary part), hi and lo
-    h16x8 Brh[NT], Brl[NT], Bih[NT], Bil[NT];
+    h16x8 Brh[NT], Brl[NT], Bih[NT];   // [Ei Er] lo is derived from [Er -Ei] lo where it is used
     {
         float s1, c1, s16, c16, as, ac;
         sincospi_hw(w, s1, c1);
@@ -497,7 +510,6 @@ __global__ __launch_bounds__(64 * BPM_WAVES) void k_bp_mma(const double *__restr
             Brh[ks] = frag4(neg_im(h[0]), neg_im(h[1]), neg_im(h[2]), neg_im(h[3]));
             Brl[ks] = frag4(neg_im(l[0]), neg_im(l[1]), neg_im(l[2]), neg_im(l[3]));
             Bih[ks] = frag4(swap_c(h[0]), swap_c(h[1]), swap_c(h[2]), swap_c(h[3]));
-            Bil[ks] = frag4(swap_c(l[0]), swap_c(l[1]), swap_c(l[2]), swap_c(l[3]));
             const float t0 = ac * c16 - as * s16, t1 = ac * s16 + as * c16;
             ac = t0; as = t1;
         }
@@ -528,9 +540,10 @@ __global__ __launch_bounds__(64 * BPM_WAVES) void k_bp_mma(const double *__restr
             zr = mfma16h(Zh, Brh[ks], zr);
             zi = mfma16h(Zh, Bih[ks], zi);
             gr = mfma16h(Th, Brl[ks], gr);
-            gi = mfma16h(Th, Bil[ks], gi);
+            const h16x8 Bil = conj_swap8(Brl[ks]);
+            gi = mfma16h(Th, Bil, gi);
             zr = mfma16h(Zh, Brl[ks], zr);
-            zi = mfma16h(Zh, Bil[ks], zi);
+            zi = mfma16h(Zh, Bil, zi);
             gr = mfma16h(Tl, Brh[ks], gr);
             gi = mfma16h(Tl, Bih[ks], gi);
             zr = mfma16h(Zl, Brh[ks], zr);
