@@ -345,19 +345,27 @@ __device__ __forceinline__ void final_body(const DevSys &S, FinalArgs A, const d
         }
     }
     pair_totals(S);
-    if (t != 0) return;
+    // energies per pair type: thread k adds the (slot, class) columns of type k in (slot,
+    // class) order, in LDS, then writes them once
+    constexpr int NIJM = MAX_TYPES * (MAX_TYPES + 1) / 2 + 1;
+    __shared__ double sev[NIJM], sec[NIJM];
     const int nt1 = S.n_types + 1;
-    for (int k = 0; k <= S.nijt; k++) { e_vdw[k] = 0.0; e_coul[k] = 0.0; }
-    for (int s = 0; s < A.n_slots; s++) {
-        int ti = A.slot_type[s];
-        for (int c = 0; c < NCLS; c++) {
-            int tc = A.cls_type[c];
-            if (tc <= 0) continue;
-            int ijt = S.ijtype[ti * nt1 + tc];
-            e_vdw[ijt] += colsum[COL_EV + c][s];
-            e_coul[ijt] += colsum[COL_EC + c][s];
+    for (int k = t; k <= S.nijt; k += blockDim.x) {
+        double ev = 0.0, ec = 0.0;
+        for (int s = 0; s < A.n_slots; s++) {
+            const int ti = A.slot_type[s];
+            for (int c = 0; c < NCLS; c++) {
+                const int tc = A.cls_type[c];
+                if (tc <= 0 || S.ijtype[ti * nt1 + tc] != k) continue;
+                ev += colsum[COL_EV + c][s];
+                ec += colsum[COL_EC + c][s];
+            }
         }
+        sev[k] = ev; sec[k] = ec;
+        e_vdw[k] = ev; e_coul[k] = ec;
     }
+    __syncthreads();
+    if (t != 0) return;
     Scalars o;
     o.e_recip = 0.5 * ksum[0];
     o.e_recip_ll = 0.5 * ksum[1];
@@ -368,8 +376,8 @@ __device__ __forceinline__ void final_body(const DevSys &S, FinalArgs A, const d
     o.e_slab = A.ew3dc ? -A.crf0 * S.facq * (dz * dz) : 0.0;
     o.e_wall_lr = epsum[0];
     double sum = 0.0;
-    for (int k = 1; k <= S.nijt; k++) sum += e_vdw[k];
-    for (int k = 1; k <= S.nijt; k++) sum += e_coul[k];
+    for (int k = 1; k <= S.nijt; k++) sum += sev[k];
+    for (int k = 1; k <= S.nijt; k++) sum += sec[k];
     sum += o.e_recip;
     double uconf = sum + o.e_self + o.e_slab;
     o.e_conf = uconf - o.e_self;
