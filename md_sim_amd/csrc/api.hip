@@ -532,8 +532,8 @@ static void geometry(mdf_system *s) {
         s->n_sb_mma = std::max(1, (nm + sites - 1) / sites);
         // k_bp_mma: many waves per block, over which the block's staging of each T tile spreads
         // (measured faster than balancing blocks over the compute units), two blocks resident
-        // per unit: 4 waves per SIMD up to 5 kn tiles, 3 beyond (its registers)
-        s->bpm_waves = s->bpm_waves_env > 0 ? s->bpm_waves_env : (s->n_nt <= 5 ? BPM_WAVES_MAX : 6);
+        // per unit: 4 waves per SIMD up to 7 kn tiles, 3 at 8 (its registers)
+        s->bpm_waves = s->bpm_waves_env > 0 ? s->bpm_waves_env : (s->n_nt <= 7 ? BPM_WAVES_MAX : 6);
     }
     // back-projection row groups: about equal k counts, each at most BP_GROUP_K k-vectors (the
     // LDS stage), and enough groups that sites x groups fill the GPU

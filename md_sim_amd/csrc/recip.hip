@@ -455,7 +455,7 @@ constexpr int BPM_WAVES = 8;                 // most waves per block (blockDim.x
 constexpr int BPM_ROWS_MAX = 1024;           // rows whose (kl, km) the block keeps in LDS
 
 template <int NT>
-__global__ __launch_bounds__(64 * BPM_WAVES) __attribute__((amdgpu_waves_per_eu(NT <= 5 ? 4 : 1))) void k_bp_mma(const double *__restrict__ x, const double *__restrict__ y,
+__global__ __launch_bounds__(64 * BPM_WAVES) __attribute__((amdgpu_waves_per_eu(NT <= 7 ? 4 : 1))) void k_bp_mma(const double *__restrict__ x, const double *__restrict__ y,
                                                            const double *__restrict__ z, const double *__restrict__ q,
                                                            int n, double cux, double cuy, double cuz,
                                                            const int2 *__restrict__ rowkk,
@@ -491,7 +491,10 @@ __global__ __launch_bounds__(64 * BPM_WAVES) __attribute__((amdgpu_waves_per_eu(
     const double qq = ok ? q[site] : 0.0;
     // E fragments for kn columns 16 ks + 4 g .. +3 (site col), in the two forms the complex
     // product needs: [Er -Ei] (real part) and [Ei Er] (imaginary part), hi and lo
-    h16x8 Brh[NT], Brl[NT], Bih[NT];   // [Ei Er] lo is derived from [Er -Ei] lo where it is used
+    // [Ei Er] lo is derived from [Er -Ei] lo where it is used; at 6 and 7 kn tiles [Ei Er] hi
+    // too, which brings them to 128 VGPRs (at 8 tiles that still leaves 3 waves per SIMD)
+    constexpr bool IH_KEEP = NT <= 5 || NT >= 8;
+    h16x8 Brh[NT], Brl[NT], Bih[IH_KEEP ? NT : 1];
     {
         float s1, c1, s16, c16, as, ac;
         sincospi_hw(w, s1, c1);
@@ -509,7 +512,7 @@ __global__ __launch_bounds__(64 * BPM_WAVES) __attribute__((amdgpu_waves_per_eu(
             }
             Brh[ks] = frag4(neg_im(h[0]), neg_im(h[1]), neg_im(h[2]), neg_im(h[3]));
             Brl[ks] = frag4(neg_im(l[0]), neg_im(l[1]), neg_im(l[2]), neg_im(l[3]));
-            Bih[ks] = frag4(swap_c(h[0]), swap_c(h[1]), swap_c(h[2]), swap_c(h[3]));
+            if constexpr (IH_KEEP) Bih[ks] = frag4(swap_c(h[0]), swap_c(h[1]), swap_c(h[2]), swap_c(h[3]));
             const float t0 = ac * c16 - as * s16, t1 = ac * s16 + as * c16;
             ac = t0; as = t1;
         }
@@ -536,18 +539,20 @@ __global__ __launch_bounds__(64 * BPM_WAVES) __attribute__((amdgpu_waves_per_eu(
             const h16x8 Zl = *reinterpret_cast<const h16x8 *>(&sT[b][3][col][kc]);
             // real part: [Tr Ti] . [Er -Ei]; imaginary part: [Tr Ti] . [Ei Er]
             gr = mfma16h(Th, Brh[ks], gr);
-            gi = mfma16h(Th, Bih[ks], gi);
+            h16x8 Bhi;
+            if constexpr (IH_KEEP) Bhi = Bih[ks]; else Bhi = conj_swap8(Brh[ks]);
+            gi = mfma16h(Th, Bhi, gi);
             zr = mfma16h(Zh, Brh[ks], zr);
-            zi = mfma16h(Zh, Bih[ks], zi);
+            zi = mfma16h(Zh, Bhi, zi);
             gr = mfma16h(Th, Brl[ks], gr);
             const h16x8 Bil = conj_swap8(Brl[ks]);
             gi = mfma16h(Th, Bil, gi);
             zr = mfma16h(Zh, Brl[ks], zr);
             zi = mfma16h(Zh, Bil, zi);
             gr = mfma16h(Tl, Brh[ks], gr);
-            gi = mfma16h(Tl, Bih[ks], gi);
+            gi = mfma16h(Tl, Bhi, gi);
             zr = mfma16h(Zl, Brh[ks], zr);
-            zi = mfma16h(Zl, Bih[ks], zi);
+            zi = mfma16h(Zl, Bhi, zi);
         }
         // lane holds G for rows 4 g + e of the tile at its site: contract with P(row, site);
         // P walks km by 1 inside the tile and by 16 from tile to tile while kl stays

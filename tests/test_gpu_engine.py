@@ -54,19 +54,28 @@ def test_fp32_force_error_bound(oracle_lib, name):
     assert abs(r["e_conf"] - ro["e_conf"]) < 1e-4 * abs(ro["e_recip"]) + 1e-4 * abs(ro["e_conf"])
 
 
-@pytest.mark.parametrize("name", ["slab2048", "slab512", "bulk216"])
+def _mma_system(name):
+    # slab512 at k-index 14 and 18: knmax 42 and 54, 6 and 7 kn tiles of 16 (the kernels'
+    # register layouts change with the tile count)
+    if name.startswith("slab512_k"):
+        return systems.water_on_slab(512, 6, 4, seed=1201, rcut=10.0, kmax=int(name[9:]), lw=40.0, name=name)
+    return systems.config(name)
+
+
+@pytest.mark.parametrize("name", ["slab2048", "slab512", "bulk216", "slab512_k14", "slab512_k18"])
 def test_fp32_structure_factor_on_matrix_cores(name, monkeypatch):
     """The FP32 form's reciprocal sum on the matrix cores (FP16 hi/lo split products, FP32
     tile sums, FP64 across blocks and rows): S(k) against the FP64 form's and the FP32
     vector-ALU kernel's, every k-vector written (no stale partials); forces against the
     vector-ALU kernels."""
     _say("case fp32 S(k)", name)
-    s = systems.config(name)
+    s = _mma_system(name)
     pos = abi.positions(s)
     e64 = _engine(s, precision=abi.FP64)
     e64.force_get(*pos)
     _, _, s64 = e64.kvectors()
     e32 = _engine(s, precision=abi.FP32_ACC64)
+    assert e32.reciprocal_on_matrix_cores()
     r32 = e32.force_get(*pos)
     _, _, s32 = e32.kvectors()
     monkeypatch.setenv("MDF_SF_MMA", "0")
