@@ -319,6 +319,7 @@ __device__ __forceinline__ void final_body(const DevSys &S, FinalArgs A, const d
     for (int idx = t; idx < NCOL * MAX_LEAD_TYPES; idx += blockDim.x) {
         int col = idx / MAX_LEAD_TYPES, s = idx % MAX_LEAD_TYPES;
         double acc = 0.0;
+#pragma unroll 8
         for (int b = 0; b < A.n_colblk; b++) acc += colpart[((size_t)col * A.n_colblk + b) * MAX_LEAD_TYPES + s];
         colsum[col][s] = acc;
     }
@@ -330,8 +331,10 @@ __device__ __forceinline__ void final_body(const DevSys &S, FinalArgs A, const d
             const int qq = q + 8 * pass;
             double acc = 0.0;
             if (qq < RE_N) {
+#pragma unroll 8
                 for (int b = l; b < A.n_kblk; b += 32) acc += kpart[b * RE_N + qq];
             } else if (qq < RE_N + EP_N) {
+#pragma unroll 8
                 for (int b = l; b < A.n_epblk; b += 32) acc += eppart[b * EP_N + (qq - RE_N)];
             }
             if (qq < RE_N + EP_N) tmp[qq][l] = acc;
