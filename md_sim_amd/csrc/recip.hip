@@ -643,6 +643,17 @@ __device__ __forceinline__ void sf_reduce_body(const double2 *__restrict__ parti
     __shared__ double2 slice[SL][SFR_K];
     const int kk = threadIdx.x % SFR_K, sl = threadIdx.x / SFR_K;
     const int k = bid * SFR_K + kk;
+    // the finishing wave's k data, fetched before the partial rows so the loads overlap
+    double2 w = make_double2(0.0, 0.0);
+    double a = 0.0;
+    int3 kv = make_int3(0, 0, 0);
+    int kr = 0;
+    if (sl == 0 && k < nk) {
+        w = s_wall[k];
+        a = ak[k];
+        kv = make_int3(kvec[3 * k], kvec[3 * k + 1], kvec[3 * k + 2]);
+        if (tpack) kr = krow[k];
+    }
     double sr = 0.0, si = 0.0;
     if (k < nk && partial32) {   // FP32 partial rows (the matrix-core structure factor)
 #pragma unroll 4
@@ -667,14 +678,12 @@ __device__ __forceinline__ void sf_reduce_body(const double2 *__restrict__ parti
         sr = 0.0; si = 0.0;
 #pragma unroll
         for (int j = 0; j < SL; j++) { sr += slice[j][kk].x; si += slice[j][kk].y; }
-        const double2 w = s_wall[k];
         const double qr = sr + w.x, qi = si + w.y;
         s_out[k] = make_double2(qr, qi);
-        const double a = ak[k];
-        const double kn = (double)kvec[3 * k + 2];
+        const double kn = (double)kv.z;
         t_out[k] = make_double4(a * qr, -a * qi, kn * (a * qr), kn * (-a * qi));
         if (tpack) {   // the same, split for k_bp_mma (planes T hi, T lo, kn T hi, kn T lo)
-            const size_t base = (size_t)krow[k] * 4 * kn_pad + (size_t)(kvec[3 * k + 2] + knmax);
+            const size_t base = (size_t)kr * 4 * kn_pad + (size_t)(kv.z + knmax);
             unsigned h, l;
             split_c((float)(a * qr), (float)(-a * qi), h, l);
             tpack[base] = h;
@@ -686,7 +695,7 @@ __device__ __forceinline__ void sf_reduce_body(const double2 *__restrict__ parti
         const double ek = a * (qr * qr + qi * qi);
         acc[0] = ek;
         acc[1] = a * (sr * sr + si * si);
-        const double kx = pipix * kvec[3 * k], ky = pipiy * kvec[3 * k + 1], kz = pipiz * kvec[3 * k + 2];
+        const double kx = pipix * kv.x, ky = pipiy * kv.y, kz = pipiz * kv.z;
         const double g = 2.0 * (1.0 / (kx * kx + ky * ky + kz * kz) + lamb);
         const double h = 0.5 * ek;
         acc[2] = h * (1.0 - g * kx * kx);
