@@ -385,6 +385,9 @@ __global__ __launch_bounds__(256) void k_sf_mma(const double *__restrict__ x, co
         }
         __syncthreads();
         if (active) {
+            // the matrix products at raised wave priority: they go first while the other
+            // resident blocks build their tables
+            __builtin_amdgcn_s_setprio(1);
             const int ar = wave * 16 + (lane & 15), g4 = 4 * (lane >> 4);
 #pragma unroll 1
             for (int ks = 0; ks < SFM_SUB / 16; ks++) {
@@ -412,6 +415,7 @@ __global__ __launch_bounds__(256) void k_sf_mma(const double *__restrict__ x, co
                     acci[t] = mfma16h(Ail, Bh, acci[t]);
                 }
             }
+            __builtin_amdgcn_s_setprio(0);
         }
         __syncthreads();
     }
@@ -529,6 +533,7 @@ __global__ __launch_bounds__(64 * BPM_WAVES) __attribute__((amdgpu_waves_per_eu(
         const int b = mt & 1;
         const int2 kr = tile_ks[mt];
         f32x4 gr = {0.f, 0.f, 0.f, 0.f}, gi = gr, zr = gr, zi = gr;
+        __builtin_amdgcn_s_setprio(1);   // the matrix products first, as in k_sf_mma
 #pragma unroll
         for (int ks = 0; ks < NT; ks++) {
             if (ks < kr.x || ks > kr.y) continue;
@@ -554,6 +559,7 @@ __global__ __launch_bounds__(64 * BPM_WAVES) __attribute__((amdgpu_waves_per_eu(
             zr = mfma16h(Zl, Brh[ks], zr);
             zi = mfma16h(Zl, Bhi, zi);
         }
+        __builtin_amdgcn_s_setprio(0);
         // lane holds G for rows 4 g + e of the tile at its site: contract with P(row, site);
         // P walks km by 1 inside the tile and by 16 from tile to tile while kl stays
         float pr = 0.f, pi_ = 0.f, tx = 0.f, ty = 0.f, tz = 0.f;
