@@ -13,6 +13,7 @@ from typing import Dict, Optional
 import numpy as np
 
 MDF_OK = 0
+MDF_ERR_LIST_OVERFLOW = 3
 STATUS_NAMES = {0: "OK", 1: "ERR_ARG", 2: "ERR_HIP", 3: "ERR_LIST_OVERFLOW", 4: "ERR_VDW_TYPE",
                 5: "ERR_KSET", 6: "ERR_NONFINITE", 7: "ERR_NO_DEVICE"}
 FP64, FP32_ACC64 = 0, 1

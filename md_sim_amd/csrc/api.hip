@@ -71,10 +71,14 @@ template <int NT>
 __global__ void k_sf_mma(const double *x, const double *y, const double *z, const double *q, int n, double cux,
                          double cuy, double cuz, const KRow *rows, int n_rows, int knmax, int n_nt, float2 *partial,
                          int nk, int sb_sites, const MtsState *mts);
+__global__ void k_sf_reduce_rows(const double2 *, const float2 *, int, int, const KRow *, const double2 *,
+                                 const double *, double, double, double, double, double2 *, double4 *, double *,
+                                 const MtsState *, unsigned *, int *, int, int);
 template <int NT>
 __global__ void k_bp_mma(const double *x, const double *y, const double *z, const double *q, int n, double cux,
                          double cuy, double cuz, const int2 *rowkk, const int2 *tile_ks, int n_rows, int knmax,
-                         int n_nt, const unsigned *tpack, double pipix, double pipiy, double pipiz, double *fpart,
+                         int n_nt, const unsigned *tpack, const int *tscale, double pipix, double pipiy, double pipiz,
+                         double *fpart,
                          const MtsState *mts);
 constexpr int BPM_WAVES_MAX = 8; // waves per k_bp_mma block (16 sites each), at most
 constexpr int BPM_ROWS = 1024;   // rows k_bp_mma keeps in LDS
@@ -261,6 +265,7 @@ struct mdf_system {
     DBuf<int2> d_rowkk, d_tile_ks;   // k_bp_mma: (kl, km) per row, K-step range per 16-row tile
     DBuf<int> d_krow;                // row of each k-vector
     DBuf<unsigned> d_tpack;          // T split for k_bp_mma, [row][plane][kn + knmax]
+    DBuf<int> d_tscale;              // each row's power-of-two scales of the split T
     int n_sb_mma = 1, n_rg_mma = 1, knmax = 0, n_nt = 0;
     int sfm_sites = SFM_SITES, bpm_waves = BPM_WAVES_MAX;   // k_sf_mma sites per block, k_bp_mma waves
     int sfm_sites_env = 0, bpm_waves_env = 0;                // fixed by MDF_SFM_SITES / MDF_BPM_WAVES
@@ -293,6 +298,8 @@ struct mdf_system {
     bool pending = false;
     int last_rebuilt = 0;
     DevFlags last_flags;
+    int *h_mirror = nullptr;      // host-mapped flag mirror (DevSys::flag_mirror)
+    bool rebuild_next = false;    // a fixed-capacity overflow left truncated rows: rebuild next step
     // reciprocal-force surrogate and multiple time steps (off unless mdf_mts_enable)
     bool mts_on = false;
     int mts_model = 1, mts_interval = 4, mts_train_frames = 0, mts_train_iters = 1;
@@ -374,13 +381,16 @@ static void free_all(mdf_system *s) {
                           &s->fz, &s->part, &s->dip_part, &s->dipz, &s->d_ak, &s->kpart, &s->fpart, &s->eppart,
                           &s->colpart, &s->fcom, &s->torque, &s->e_vdw, &s->e_coul, &s->ang_kt, &s->ang_th};
     for (auto *b : db) b->release();
-    s->d_rowkk.release(); s->d_tile_ks.release(); s->d_krow.release(); s->d_tpack.release();
+    s->d_rowkk.release(); s->d_tile_ks.release(); s->d_krow.release(); s->d_tpack.release(); s->d_tscale.release();
     s->flags.release(); s->d_bp_row_start.release(); s->d_chunks.release(); s->d_rows.release(); s->d_swall.release(); s->d_s.release();
     s->d_t.release(); s->sf_partial.release(); s->d_wl.release(); s->d_scal.release();
     mts_free(s);
     s->obs_ngc.release(); s->obs_efi.release(); s->obs_efipos.release();
     s->obs_on = false;
     chem_free(s);
+    if (s->h_mirror) (void)hipHostFree(s->h_mirror);
+    s->h_mirror = nullptr;
+    s->S.flag_mirror = nullptr;
 }
 
 template <typename T>
@@ -726,7 +736,8 @@ static void launch_bp(mdf_system *s) {
     case NT:                                                                                                    \
         k_bp_mma<NT><<<grid, block, 0, s->stream>>>(S.x, S.y, S.z, S.q, nm, a.cux, a.cuy, a.cuz, s->d_rowkk.p,   \
                                                     s->d_tile_ks.p, (int)s->rows.size(), s->knmax, s->n_nt,    \
-                                                    s->d_tpack.p, a.pipix, a.pipiy, a.pipiz, s->fpart.p,       \
+                                                    s->d_tpack.p, s->d_tscale.p, a.pipix, a.pipiy, a.pipiz,    \
+                                                    s->fpart.p,                                                 \
                                                     mts_state(s));                                              \
         break;
         switch (s->n_nt) {
@@ -755,7 +766,7 @@ static FinalArgs final_args(const mdf_system *s) {
     A.n_slots = (int)s->slot_type.size();
     for (int k = 0; k < A.n_slots; k++) A.slot_type[k] = s->slot_type[k];
     for (int c = 0; c < NCLS; c++) A.cls_type[c] = s->cls_type[c];
-    A.n_kblk = s->ak.size() > 0 ? s->n_kblk : 0;
+    A.n_kblk = s->ak.size() > 0 ? (s->bp_mma ? (int)s->rows.size() : s->n_kblk) : 0;
     A.n_epblk = s->n_epblk;
     A.e_self = s->uself;
     A.e_recip_ww = s->ewqWW;
@@ -795,13 +806,18 @@ static void launch_forces(mdf_system *s) {
     if (s->ev) (void)hipEventRecord(s->ev[4], st);
     if (nk > 0) {
         double lamb = 1.0 / (4.0 * S.alfa * S.alfa);
-        k_sf_reduce<<<s->n_kblk, SFR_THREADS, 0, st>>>(s->sf_partial.p, nm > 0 ? (s->sf_mma ? s->n_sb_mma : s->n_sb) : 0, nk,
-                                               s->d_swall.p, s->d_ak.p,
-                                               s->d_kvec.p, ra.pipix, ra.pipiy, ra.pipiz, lamb, s->d_s.p, s->d_t.p,
-                                               s->kpart.p, mts_state(s), s->bp_mma ? s->d_tpack.p : nullptr,
-                                               s->d_krow.p, 16 * s->n_nt, s->knmax,
-                                               s->sf_mma && nm > 0 ? reinterpret_cast<const float2 *>(s->sf_partial.p)
-                                                                   : nullptr);
+        const int n_sb = nm > 0 ? (s->sf_mma ? s->n_sb_mma : s->n_sb) : 0;
+        const float2 *p32 = s->sf_mma && nm > 0 ? reinterpret_cast<const float2 *>(s->sf_partial.p) : nullptr;
+        if (s->bp_mma)   // one block per row: T split with per-row scales for k_bp_mma
+            k_sf_reduce_rows<<<(int)s->rows.size(), SFR_THREADS, 0, st>>>(
+                s->sf_partial.p, p32, n_sb, nk, s->d_rows.p, s->d_swall.p, s->d_ak.p, ra.pipix, ra.pipiy, ra.pipiz,
+                lamb, s->d_s.p, s->d_t.p, s->kpart.p, mts_state(s), s->d_tpack.p, s->d_tscale.p, 16 * s->n_nt,
+                s->knmax);
+        else
+            k_sf_reduce<<<s->n_kblk, SFR_THREADS, 0, st>>>(s->sf_partial.p, n_sb, nk, s->d_swall.p, s->d_ak.p,
+                                                           s->d_kvec.p, ra.pipix, ra.pipiy, ra.pipiz, lamb, s->d_s.p,
+                                                           s->d_t.p, s->kpart.p, mts_state(s), nullptr, nullptr, 0, 0,
+                                                           p32);
     }
     if (s->ev) (void)hipEventRecord(s->ev[5], st);
     launch_bp(s);
@@ -1005,6 +1021,7 @@ static int chem_enqueue(mdf_system *s) {
 // one step: the fixed launch sequence, replayed from a captured graph when possible (a
 // forced rebuild goes through direct launches)
 static int run_step(mdf_system *s, int force_rebuild) {
+    if (s->rebuild_next) { force_rebuild = 1; s->rebuild_next = false; }
     int slot = (int)(s->step_no % mdf_system::NRING);
     drain_slot(s, slot);
     s->ev = s->ring[slot];
@@ -1048,26 +1065,50 @@ static int run_step(mdf_system *s, int force_rebuild) {
     return MDF_OK;
 }
 
-// wait, read flags, grow lists and redo the step on overflow (unless the capacity was fixed)
+// the overflow flags are sticky on the device: the host clears them once it has read them
+// (no kernel of this handle is in flight then)
+static int clear_overflow(mdf_system *s) {
+    const int zero[3] = {0, 0, 0};
+    HIPCHK(hipMemcpy(&s->flags.p->overflow_mm, zero, sizeof zero, hipMemcpyHostToDevice));
+    if (s->h_mirror) s->h_mirror[0] = 0;
+    return MDF_OK;
+}
+
+// wait, read flags, grow lists and redo the step on overflow (unless the capacity was fixed).
+// redo: 0 none, 1 redo the last step, 2 the same and the overflow came from an earlier step
+// whose successors (enqueued with mdf_compute_device) also ran on the truncated rows.
 static int finish_step(mdf_system *s, int *redo) {
     *redo = 0;
     HIPCHK(hipStreamSynchronize(s->stream));
     s->pending = false;
     HIPCHK(hipMemcpy(&s->last_flags, s->flags.p, sizeof(DevFlags), hipMemcpyDeviceToHost));
-    const DevFlags &f = s->last_flags;
+    const DevFlags f = s->last_flags;
     s->last_rebuilt = f.rebuild;
     if (f.overflow_mm || f.overflow_mw) {
-        if (s->fixed_cap)
+        int rc = clear_overflow(s);
+        if (rc) return rc;
+        const int late = f.steps - f.overflow_step;   // steps run on the truncated rows after it
+        if (s->fixed_cap) {
+            s->rebuild_next = true;   // the rows stay truncated: the next step builds them again
             return fail(MDF_ERR_LIST_OVERFLOW, "nnlist_make(): nnlist too big! need %d moving / %d frozen partners per "
-                        "lead, capacity %d / %d", f.overflow_mm, f.overflow_mw, s->S.cap_mm, s->S.cap_mw);
+                        "lead, capacity %d / %d%s", f.overflow_mm, f.overflow_mw, s->S.cap_mm, s->S.cap_mw,
+                        late > 0 ? " (and later steps ran on the truncated rows)" : "");
+        }
         int cmm = std::max(s->S.cap_mm, (int)(f.overflow_mm * 1.25) + 16);
         int cmw = std::max(s->S.cap_mw, (int)(f.overflow_mw * 1.25) + 16);
-        int rc = alloc_lists(s, cmm, cmw);
+        rc = alloc_lists(s, cmm, cmw);
         if (rc) return rc;
-        *redo = 1;
+        *redo = late > 0 ? 2 : 1;
     }
     drain_all(s);
     return MDF_OK;
+}
+
+static int late_overflow_error(const mdf_system *s) {
+    return fail(MDF_ERR_LIST_OVERFLOW, "a pair-list row overflowed %d step(s) before the last one; the steps since ran "
+                "on truncated rows and their device results are wrong (capacity grown to %d / %d, the last step "
+                "redone: the outputs are valid)", s->last_flags.steps - s->last_flags.overflow_step, s->S.cap_mm,
+                s->S.cap_mw);
 }
 
 static int copy_outputs(mdf_system *s, mdf_outputs *o) {
@@ -1154,6 +1195,11 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
         if (sm && sm[0] == '0') s->sf_mma_allowed = false;
         const char *bm = getenv("MDF_BP_MMA");
         if (bm && bm[0] == '0') s->bp_mma_allowed = false;
+        // k_sf_mma splits q exp(i phi) of every moving site into FP16 hi/lo parts, which hold
+        // magnitudes below 65504: far larger charges than any physical one (|q| > 2^14 e) take
+        // the vector-ALU structure factor instead
+        for (int i = 0; i < t.n_move; i++)
+            if (!(std::fabs(t.charge[i]) < 16384.0)) { s->sf_mma_allowed = false; break; }
         const char *sb = getenv("MDF_SF_BLOCKS");
         if (sb && atoi(sb) > 0) s->sf_target_blocks = atoi(sb);
         const char *bb = getenv("MDF_BP_BLOCKS");
@@ -1351,6 +1397,10 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
             HTRY(hipMemcpy(s->flags.p, &f0, sizeof f0, hipMemcpyHostToDevice));
         }
         S.flags = s->flags.p;
+        // host-mapped mirror of a pending list overflow, polled by mdf_compute_device
+        HTRY(hipHostMalloc((void **)&s->h_mirror, sizeof(int) * 4, hipHostMallocMapped));
+        memset(s->h_mirror, 0, sizeof(int) * 4);
+        HTRY(hipHostGetDevicePointer((void **)&S.flag_mirror, s->h_mirror, 0));
         HTRY(hipMemset(s->cell_cnt.p, 0, sizeof(int) * (s->n_cells_cap + 1)));
         // frozen-site cells, built once
         {
@@ -1460,8 +1510,10 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
                 HTRY(upload(s->d_krow, krow.data(), krow.size()));
                 HTRY(s->d_tpack.alloc((size_t)std::max(nr, 1) * 4 * kn_pad));
                 HTRY(hipMemset(s->d_tpack.p, 0, sizeof(unsigned) * s->d_tpack.n));
+                HTRY(s->d_tscale.alloc(std::max(nr, 1)));
+                HTRY(hipMemset(s->d_tscale.p, 0, sizeof(int) * s->d_tscale.n));
             }
-            HTRY(s->kpart.alloc((size_t)8 * s->n_kblk));
+            HTRY(s->kpart.alloc((size_t)RE_N * std::max(s->n_kblk, (int)s->rows.size())));
             std::vector<double2> zero2(std::max(nk, 1), make_double2(0.0, 0.0));
             HTRY(upload(s->d_swall, zero2.data(), zero2.size()));
             int nw = s->n_wall;
@@ -1533,6 +1585,7 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
             HTRY(hipStreamSynchronize(s->stream));
             HTRY(hipMemcpy(&s->last_flags, s->flags.p, sizeof(DevFlags), hipMemcpyDeviceToHost));
             if (s->last_flags.overflow_mm || s->last_flags.overflow_mw) {
+                TRY(clear_overflow(s));
                 if (s->fixed_cap) {
                     rc = fail(MDF_ERR_LIST_OVERFLOW, "nnlist_make(): nnlist too big! need %d / %d partners per lead",
                               s->last_flags.overflow_mm, s->last_flags.overflow_mw);
@@ -1579,6 +1632,7 @@ int mdf_compute(mdf_system *s, const double *x, const double *y, const double *z
     if (!s || !x || !y || !z || !out) return fail(MDF_ERR_ARG, "null argument");
     if (s->top.n_rig > 0 && (!cx || !cy || !cz)) return fail(MDF_ERR_ARG, "COM arrays required");
     HIPCHK(hipSetDevice(s->device));
+    bool late = false;   // a step enqueued earlier by mdf_compute_device overflowed
     for (int attempt = 0; attempt < 4; attempt++) {
         int rc = set_inputs(s, x, y, z, cx, cy, cz, hipMemcpyHostToDevice);
         if (rc) return rc;
@@ -1587,7 +1641,11 @@ int mdf_compute(mdf_system *s, const double *x, const double *y, const double *z
         int redo = 0;
         rc = finish_step(s, &redo);
         if (rc) return rc;
-        if (!redo) return copy_outputs(s, out);
+        if (redo == 2) late = true;
+        if (!redo) {
+            rc = copy_outputs(s, out);
+            return (rc == MDF_OK && late) ? late_overflow_error(s) : rc;
+        }
     }
     return fail(MDF_ERR_LIST_OVERFLOW, "pair list kept overflowing");
 }
@@ -1597,6 +1655,9 @@ int mdf_compute_device(mdf_system *s, const double *d_x, const double *d_y, cons
     if (!s || !d_x || !d_y || !d_z) return fail(MDF_ERR_ARG, "null argument");
     if (s->top.n_rig > 0 && (!d_cx || !d_cy || !d_cz)) return fail(MDF_ERR_ARG, "COM arrays required");
     HIPCHK(hipSetDevice(s->device));
+    if (s->h_mirror && *(volatile int *)s->h_mirror)
+        return fail(MDF_ERR_LIST_OVERFLOW, "an earlier step's pair list overflowed its rows (capacity %d / %d); the "
+                    "step was not enqueued: call mdf_read_outputs", s->S.cap_mm, s->S.cap_mw);
     // start after the producer of the inputs (the handle's stream does not block on others)
     HIPCHK(hipEventRecord(s->in_event, (hipStream_t)stream));
     HIPCHK(hipStreamWaitEvent(s->stream, s->in_event, 0));
@@ -1611,6 +1672,7 @@ int mdf_read_outputs(mdf_system *s, mdf_outputs *out) {
     int redo = 0;
     int rc = finish_step(s, &redo);
     if (rc) return rc;
+    const bool late = redo == 2;
     if (redo) {
         // the inputs are still on the device: run the step again with the grown lists
         rc = run_step(s, 1);
@@ -1619,7 +1681,8 @@ int mdf_read_outputs(mdf_system *s, mdf_outputs *out) {
         if (rc) return rc;
         if (redo) return fail(MDF_ERR_LIST_OVERFLOW, "pair list kept overflowing");
     }
-    return copy_outputs(s, out);
+    rc = copy_outputs(s, out);
+    return (rc == MDF_OK && late) ? late_overflow_error(s) : rc;
 }
 
 int mdf_device_results(mdf_system *s, const double **d_f_site, const double **d_f_com, const double **d_torque) {
@@ -2403,6 +2466,7 @@ static void fill_rep(mdf_system *s, RepArgs &r) {
     r.lead_slot = s->lead_slot.p;
     r.scal = s->d_scal.p;
     r.A = final_args(s);
+    r.A.n_kblk = nk > 0 ? s->n_kblk : 0;   // the batch reduces k partials in blocks of SFR_K k-vectors
 }
 
 // block counts of every batched launch for one system
@@ -2560,6 +2624,10 @@ int mdf_batch_compute_device(mdf_batch *b, const double *const *x, const double 
         if (!x[i] || !y[i] || !z[i]) return fail(MDF_ERR_ARG, "null positions for system %d", i);
         if (b->sys[i]->top.n_rig > 0 && (!cx || !cy || !cz || !cx[i] || !cy[i] || !cz[i]))
             return fail(MDF_ERR_ARG, "COM arrays required for system %d", i);
+        const mdf_system *si = b->sys[i];
+        if (si->h_mirror && *(volatile int *)si->h_mirror)
+            return fail(MDF_ERR_LIST_OVERFLOW, "system %d: an earlier step's pair list overflowed its rows; the step "
+                        "was not enqueued: call mdf_batch_read_outputs", i);
         const double *p[6] = {x[i], y[i], z[i], cx ? cx[i] : nullptr, cy ? cy[i] : nullptr, cz ? cz[i] : nullptr};
         for (int c = 0; c < 6; c++) b->h_in[6 * i + c] = p[c];
     }
@@ -2590,19 +2658,26 @@ int mdf_batch_read_outputs(mdf_batch *b, mdf_outputs *outs) {
     const int n = (int)b->sys.size();
     HIPCHK(hipMemcpy(b->h_flags.data(), b->d_flags.p, sizeof(DevFlags) * n, hipMemcpyDeviceToHost));
     bool regrown = false;
+    int late = -1;   // a system whose overflow came from an earlier step (-1: none)
     for (int i = 0; i < n; i++) {
         mdf_system *s = b->sys[i];
         s->last_flags = b->h_flags[i];
         s->last_rebuilt = s->last_flags.rebuild;
-        const DevFlags &f = s->last_flags;
+        const DevFlags f = s->last_flags;
         if (f.overflow_mm || f.overflow_mw) {
             // this system's list outgrew its rows: grow them and redo its step on its own
             // (the batch kernels already gathered its positions)
+            // with a fixed capacity the flags stay set: the batch keeps reporting the overflow
+            // (the reference stops, nnlist.f90:136-139) since its graph cannot rebuild one system
             if (s->fixed_cap)
-                return fail(MDF_ERR_LIST_OVERFLOW, "system %d: nnlist too big (need %d / %d partners per lead)", i,
-                            f.overflow_mm, f.overflow_mw);
-            int rc = alloc_lists(s, std::max(s->S.cap_mm, (int)(f.overflow_mm * 1.25) + 16),
-                                 std::max(s->S.cap_mw, (int)(f.overflow_mw * 1.25) + 16));
+                return fail(MDF_ERR_LIST_OVERFLOW, "system %d: nnlist too big (need %d / %d partners per lead)%s", i,
+                            f.overflow_mm, f.overflow_mw,
+                            f.steps != f.overflow_step ? "; later steps ran on the truncated rows" : "");
+            int rc = clear_overflow(s);
+            if (rc) return rc;
+            if (f.steps != f.overflow_step && late < 0) late = i;
+            rc = alloc_lists(s, std::max(s->S.cap_mm, (int)(f.overflow_mm * 1.25) + 16),
+                             std::max(s->S.cap_mw, (int)(f.overflow_mw * 1.25) + 16));
             if (rc) return rc;
             rc = run_step(s, 1);
             if (rc) return rc;
@@ -2621,6 +2696,9 @@ int mdf_batch_read_outputs(mdf_batch *b, mdf_outputs *outs) {
         int rc = copy_outputs(b->sys[i], &outs[i]);
         if (rc) return rc;
     }
+    if (late >= 0)
+        return fail(MDF_ERR_LIST_OVERFLOW, "system %d: a pair-list row overflowed before the last batch step; the "
+                    "steps since ran on truncated rows (lists grown, the last step redone: the outputs are valid)", late);
     return MDF_OK;
 }
 

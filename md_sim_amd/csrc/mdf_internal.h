@@ -76,9 +76,14 @@ struct FinalArgs {
 };
 
 // flags written by device kernels, read back by the host after a step
+// The overflow fields are sticky: a build that outgrows a row sets them, and only the host
+// clears them after reading them, so an overflow is never lost behind later steps that did
+// not rebuild (they ran on the same truncated rows).
 struct DevFlags {
     int rebuild;                         // 1: the list is rebuilt this step
     int overflow_mm, overflow_mw;        // max count seen when a list row overflowed
+    int overflow_step;                   // value of `steps` at the step whose build overflowed
+    int steps;                           // steps since creation (counted by the rebuild decision)
     int ncz;                             // cells in z for this build
     double z0, csz;                      // z origin and cell size for this build
     int n_pairs_mm, n_pairs_mw;          // listed pairs (i<j count and wall count)
@@ -141,6 +146,7 @@ struct DevSys {
     double *fx, *fy, *fz;        // [n_move] site forces
     double *part;                // [NCOL][n_lead]
     DevFlags *flags;
+    int *flag_mirror;            // host-mapped: [0] = 1 once a list overflow is pending (or null)
 };
 
 

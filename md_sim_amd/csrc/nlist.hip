@@ -71,8 +71,9 @@ __global__ __launch_bounds__(256) void k_skin(DevSys S, double *dip_part) {
 }
 
 // rebuild decision (src/forcem.f90:250) and z grid of this build; dipole total.  One block,
-// over k_skin's per-block partials.  It also clears the overflow flags for this step's build
-// (so no reset launch is needed).
+// over k_skin's per-block partials.  It counts the step and publishes a pending list
+// overflow (left by an earlier step's build) to the host-mapped mirror; the overflow flags
+// themselves stay set until the host has read them.
 __device__ __forceinline__ void decide_body(const DevSys &S, const double *dip_part, int n_dip, double *dipz_out,
                                                 int force_rebuild, int bid, int nblk) {
     __shared__ double red2[256];
@@ -104,8 +105,8 @@ __device__ __forceinline__ void decide_body(const DevSys &S, const double *dip_p
     zlo = fmin(fmin(ext[1][0], ext[1][1]), fmin(ext[1][2], ext[1][3]));
     zhi = fmax(fmax(ext[2][0], ext[2][1]), fmax(ext[2][2], ext[2][3]));
     DevFlags *f = S.flags;
-    f->overflow_mm = 0;
-    f->overflow_mw = 0;
+    f->steps += 1;
+    if (S.flag_mirror) S.flag_mirror[0] = (f->overflow_mm | f->overflow_mw) != 0 ? 1 : 0;
     f->rebuild = (force_rebuild || __dmul_rn(4.0, dmax) > S.skin2) ? 1 : 0;
     if (f->rebuild) {
         // pair counts describe the standing list (ipack/wpack): restart them for this build
@@ -311,7 +312,10 @@ __device__ __forceinline__ void build_body(const DevSys &S, int bid, int nblk) {
     if (lane == 0) {
         S.cnt_mm[l] = cnt;
         S.cnt_lo[l] = lower;   // i < j pairs of this row; summed after the build
-        if (cnt > S.cap_mm) atomicMax(&S.flags->overflow_mm, cnt);
+        if (cnt > S.cap_mm) {
+            atomicMax(&S.flags->overflow_mm, cnt);
+            S.flags->overflow_step = S.flags->steps;   // the same value from every writer
+        }
     }
     // frozen partners: same x,y grid, frozen z grid built once
     int cntw = 0;
@@ -357,7 +361,10 @@ __device__ __forceinline__ void build_body(const DevSys &S, int bid, int nblk) {
     }
     if (lane == 0) {
         S.cnt_mw[l] = cntw;
-        if (cntw > S.cap_mw) atomicMax(&S.flags->overflow_mw, cntw);
+        if (cntw > S.cap_mw) {
+            atomicMax(&S.flags->overflow_mw, cntw);
+            S.flags->overflow_step = S.flags->steps;
+        }
     }
 }
 

@@ -447,9 +447,10 @@ SFM_INST(1) SFM_INST(2) SFM_INST(3) SFM_INST(4) SFM_INST(5) SFM_INST(6) SFM_INST
 // F_j = q_j sum_rows c(row) Im(P_j(row) G_j(row)) with G_j(row) = sum_kn T(row, kn) E_j(kn),
 // T = A conj S (and kn T for the z component): per 16-row tile, G for 16 sites is a complex
 // product of T (rows x kn) and E (kn x sites) on the matrix cores (FP16 hi/lo split as in
-// k_sf_mma, 12 products per K-step of 16 kn: G and Gz, real and imaginary).  k_sf_reduce
+// k_sf_mma, 12 products per K-step of 16 kn: G and Gz, real and imaginary).  k_sf_reduce_rows
 // leaves T already split, in a dense zero-padded layout tpack[(row * 4 + plane) * kn_pad +
-// kn + knmax] with planes T hi, T lo, kn T hi, kn T lo.  A wave owns 16 sites and keeps their
+// kn + knmax] with planes T hi, T lo, kn T hi, kn T lo, each row's planes scaled by powers of
+// two (tscale) that the contraction takes out again.  A wave owns 16 sites and keeps their
 // E fragments in registers for the whole kernel; the block copies each T tile into LDS once
 // for its waves, one tile ahead (two buffers, one barrier per tile).  The contraction with
 // P runs per lane (a tile's four rows in FP32, the sum over tiles in FP64), and the four
@@ -464,12 +465,14 @@ __global__ __launch_bounds__(64 * BPM_WAVES) __attribute__((amdgpu_waves_per_eu(
                                                            int n, double cux, double cuy, double cuz,
                                                            const int2 *__restrict__ rowkk,
                                                            const int2 *__restrict__ tile_ks, int n_rows, int knmax,
-                                                           int n_nt, const unsigned *__restrict__ tpack, double pipix,
+                                                           int n_nt, const unsigned *__restrict__ tpack,
+                                                           const int *__restrict__ tscale, double pipix,
                                                            double pipiy, double pipiz, double *__restrict__ fpart,
                                                            const MtsState *__restrict__ mts) {
     if (mts && !mts->exact) return;
     __shared__ unsigned sT[2][4][16][16 * NT + 4];   // row stride = 4 mod 8 words
-    __shared__ int sKl[BPM_ROWS_MAX], sKm[BPM_ROWS_MAX];
+    // per row: kl, km and the exponents of its two planes' scales, one signed byte each
+    __shared__ int sInfo[BPM_ROWS_MAX];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, col = lane & 15;
     const int site = (blockIdx.x * (int)(blockDim.x >> 6) + wave) * 16 + col;
@@ -478,7 +481,7 @@ __global__ __launch_bounds__(64 * BPM_WAVES) __attribute__((amdgpu_waves_per_eu(
     const int n_mt = (n_rows + 15) >> 4;
     for (int r = threadIdx.x; r < n_rows; r += blockDim.x) {
         const int2 v = rowkk[r];
-        sKl[r] = v.x; sKm[r] = v.y;
+        sInfo[r] = (v.x & 0xff) | ((v.y & 0xff) << 8) | (tscale[r] << 16);
     }
     // stage tile mt (16 rows x 4 planes x kn_pad) into buffer b
     auto stage = [&](int mt, int b) {
@@ -568,7 +571,11 @@ __global__ __launch_bounds__(64 * BPM_WAVES) __attribute__((amdgpu_waves_per_eu(
         for (int e = 0; e < 4; e++) {
             const int r = mt * 16 + 4 * g + e;
             if (r >= n_rows) break;
-            const int kl = sKl[r], km = sKm[r];
+            const int inf = sInfo[r];
+            const int kl = (signed char)inf, km = (signed char)(inf >> 8);
+            // 2^-sT and 2^-sZ undo the row's scaling (exponent bits of a normal float)
+            const float it = __int_as_float((127 - (signed char)(inf >> 16)) << 23);
+            const float iz = __int_as_float((127 - (signed char)(inf >> 24)) << 23);
             if (e == 0) {
                 if (kl == p0kl && km == p0km + 16) {
                     const float t0 = p0r * vc16 - p0i * vs16, t1 = p0r * vs16 + p0i * vc16;
@@ -588,10 +595,10 @@ __global__ __launch_bounds__(64 * BPM_WAVES) __attribute__((amdgpu_waves_per_eu(
                 pr = c0; pi_ = s0;
             }
             pkl = kl; pkm = km;
-            const float im = pr * gi[e] + pi_ * gr[e];
+            const float im = (pr * gi[e] + pi_ * gr[e]) * it;
             tx += im * (float)kl;
             ty += im * (float)km;
-            tz += pr * zi[e] + pi_ * zr[e];
+            tz += (pr * zi[e] + pi_ * zr[e]) * iz;
         }
         fx += (double)tx;   // the tile's four rows in FP32, the sum over tiles in FP64
         fy += (double)ty;
@@ -612,7 +619,8 @@ __global__ __launch_bounds__(64 * BPM_WAVES) __attribute__((amdgpu_waves_per_eu(
 #define BPM_INST(NT)                                                                                     \
     template __global__ void k_bp_mma<NT>(const double *, const double *, const double *, const double *, int, \
                                           double, double, double, const int2 *, const int2 *, int, int, int,    \
-                                          const unsigned *, double, double, double, double *, const MtsState *);
+                                          const unsigned *, const int *, double, double, double, double *,      \
+                                          const MtsState *);
 BPM_INST(1) BPM_INST(2) BPM_INST(3) BPM_INST(4) BPM_INST(5) BPM_INST(6) BPM_INST(7) BPM_INST(8)
 #undef BPM_INST
 
@@ -731,6 +739,130 @@ __global__ __launch_bounds__(SFR_THREADS) void k_sf_reduce(const double2 *__rest
     if (mts && !mts->exact) return;   // k partials keep the last full sum's values
     sf_reduce_body(partial, n_sb, nk, s_wall, ak, kvec, pipix, pipiy, pipiz, lamb, s_out, t_out, epart, blockIdx.x,
                    gridDim.x, tpack, krow, kn_pad, knmax, partial32);
+}
+
+// The same reduction with one block per k-set row, for the matrix-core back-projection: it
+// also splits T = A conj S and kn T into FP16 hi/lo pairs for k_bp_mma.  FP16 holds
+// magnitudes below 65504 only, so each row's two planes are first scaled by exact powers of
+// two, 2^sT and 2^sZ, that bring the row's largest |T| (|kn T|) into [2^14, 2^15): hi never
+// saturates, and lo keeps 11 bits below hi down to 2^-29 of the row's largest entry, beneath
+// which entries contribute less than FP32 rounding.  k_bp_mma multiplies the row's products
+// by 2^-sT and 2^-sZ.  tscale[row] = (sT & 0xff) | (sZ & 0xff) << 8, each clamped to
+// [-100, 100] (|T| <= A_k sum |q| stays many decades inside that range).
+constexpr int SFRR_CHUNKS = 2;   // 64-wide chunks of a row: rows hold at most 2 * 63 + 1 k-vectors
+
+__device__ __forceinline__ int pow2_scale(float m) {
+    if (!(m > 0.f) || !(m < 3.0e38f)) return 0;   // zero or non-finite row: no scaling
+    int ex;
+    (void)frexpf(m, &ex);                         // m = f 2^ex, f in [0.5, 1)
+    return min(100, max(-100, 15 - ex));          // m 2^(15 - ex) in [2^14, 2^15)
+}
+
+__global__ __launch_bounds__(SFR_THREADS) void k_sf_reduce_rows(
+    const double2 *__restrict__ partial, const float2 *__restrict__ partial32, int n_sb, int nk,
+    const KRow *__restrict__ rows, const double2 *__restrict__ s_wall, const double *__restrict__ ak,
+    double pipix, double pipiy, double pipiz, double lamb, double2 *__restrict__ s_out, double4 *__restrict__ t_out,
+    double *__restrict__ epart, const MtsState *__restrict__ mts, unsigned *__restrict__ tpack,
+    int *__restrict__ tscale, int kn_pad, int knmax) {
+    if (mts && !mts->exact) return;
+    constexpr int SL = SFR_THREADS / SFR_K;
+    __shared__ double2 slice[SL][SFR_K];
+    const int kk = threadIdx.x % SFR_K, sl = threadIdx.x / SFR_K;
+    const KRow row = rows[blockIdx.x];
+    const int len = row.kn1 - row.kn0 + 1;
+    double acc[RE_N];
+#pragma unroll
+    for (int c = 0; c < RE_N; c++) acc[c] = 0.0;
+    double tr[SFRR_CHUNKS], ti[SFRR_CHUNKS];
+#pragma unroll
+    for (int ch = 0; ch < SFRR_CHUNKS; ch++) {
+        tr[ch] = 0.0; ti[ch] = 0.0;
+        const int j = ch * SFR_K + kk;
+        const bool ok = j < len;
+        const int k = row.koff + j;
+        double sr = 0.0, si = 0.0;
+        if (ok && partial32) {
+#pragma unroll 4
+            for (int b = sl; b < n_sb; b += SL) {
+                const float2 v = partial32[(size_t)b * nk + k];
+                sr += (double)v.x; si += (double)v.y;
+            }
+        } else if (ok) {
+#pragma unroll 4
+            for (int b = sl; b < n_sb; b += SL) {
+                const double2 v = partial[(size_t)b * nk + k];
+                sr += v.x; si += v.y;
+            }
+        }
+        slice[sl][kk] = make_double2(sr, si);
+        __syncthreads();
+        if (sl == 0 && ok) {
+            sr = 0.0; si = 0.0;
+#pragma unroll
+            for (int q = 0; q < SL; q++) { sr += slice[q][kk].x; si += slice[q][kk].y; }
+            const double2 w = s_wall[k];
+            const double a = ak[k];
+            const int kn = row.kn0 + j;
+            const double qr = sr + w.x, qi = si + w.y;
+            s_out[k] = make_double2(qr, qi);
+            tr[ch] = a * qr; ti[ch] = -a * qi;
+            t_out[k] = make_double4(tr[ch], ti[ch], kn * tr[ch], kn * ti[ch]);
+            const double ek = a * (qr * qr + qi * qi);
+            acc[0] += ek;
+            acc[1] += a * (sr * sr + si * si);
+            const double kx = pipix * row.kl, ky = pipiy * row.km, kz = pipiz * kn;
+            const double g = 2.0 * (1.0 / (kx * kx + ky * ky + kz * kz) + lamb);
+            const double h = 0.5 * ek;
+            acc[2] += h * (1.0 - g * kx * kx);
+            acc[3] += h * (1.0 - g * ky * ky);
+            acc[4] += h * (1.0 - g * kz * kz);
+            acc[5] += -h * g * kx * ky;
+            acc[6] += -h * g * kx * kz;
+            acc[7] += -h * g * ky * kz;
+        }
+        if ((ch + 1) * SFR_K >= len) break;   // uniform over the block
+        __syncthreads();
+    }
+    if (sl != 0) return;
+    // the row's largest |T| and |kn T| (max is exact and order-free: the same bits every run)
+    float mt = 0.f, mz = 0.f;
+#pragma unroll
+    for (int ch = 0; ch < SFRR_CHUNKS; ch++) {
+        const int j = ch * SFR_K + kk;
+        if (j < len) {
+            const float m = (float)fmax(fabs(tr[ch]), fabs(ti[ch]));
+            mt = fmaxf(mt, m);
+            mz = fmaxf(mz, m * (float)abs(row.kn0 + j));
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        mt = fmaxf(mt, __shfl_xor(mt, o, 64));
+        mz = fmaxf(mz, __shfl_xor(mz, o, 64));
+    }
+    // the scaled maxima stay below 2^15 after FP32 rounding of the scaled doubles
+    const int st = pow2_scale(mt), sz = pow2_scale(mz);
+#pragma unroll
+    for (int ch = 0; ch < SFRR_CHUNKS; ch++) {
+        const int j = ch * SFR_K + kk;
+        if (j < len) {
+            const int kn = row.kn0 + j;
+            const size_t base = (size_t)blockIdx.x * 4 * kn_pad + (size_t)(kn + knmax);
+            unsigned h, l;
+            split_c((float)ldexp(tr[ch], st), (float)ldexp(ti[ch], st), h, l);
+            tpack[base] = h;
+            tpack[base + kn_pad] = l;
+            split_c((float)ldexp(kn * tr[ch], sz), (float)ldexp(kn * ti[ch], sz), h, l);
+            tpack[base + 2 * kn_pad] = h;
+            tpack[base + 3 * kn_pad] = l;
+        }
+    }
+    if (kk == 0) tscale[blockIdx.x] = (st & 0xff) | ((sz & 0xff) << 8);
+#pragma unroll
+    for (int c = 0; c < RE_N; c++)
+        for (int o = 32; o > 0; o >>= 1) acc[c] += __shfl_xor(acc[c], o, 64);
+    if (kk == 0)
+#pragma unroll
+        for (int c = 0; c < RE_N; c++) epart[blockIdx.x * RE_N + c] = acc[c];
 }
 
 // T4[k] = (A conj S, kn A conj S): per k-vector the force pieces are Im(T e) and kn Im(T e),

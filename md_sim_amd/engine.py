@@ -119,8 +119,17 @@ class ForceEngine:
                                         vp(_producer_stream(stream))))
 
     def read_outputs(self) -> abi.ForceResult:
+        """Wait for the last ``compute_device`` step and return its results.  When a list row
+        overflowed before that step, the steps since ran on truncated rows: this raises
+        MDForceError (code MDF_ERR_LIST_OVERFLOW); if the capacity could grow, the last step
+        was redone and the error carries its valid results as ``.result``."""
         o, arrs = abi.make_outputs(self.n_move, self.n_rig, self.n_pairs)
-        _check(lib().mdf_read_outputs(self._h, C.byref(o)))
+        rc = lib().mdf_read_outputs(self._h, C.byref(o))
+        if rc != abi.MDF_OK:
+            err = MDForceError(rc, lib().mdf_last_error().decode())
+            if rc == abi.MDF_ERR_LIST_OVERFLOW and o.n_list_builds > 0:   # outputs were written
+                err.result = abi.collect(o, arrs, self.n_move, self.n_rig)
+            raise err
         return abi.collect(o, arrs, self.n_move, self.n_rig)
 
     # -- inspection ------------------------------------------------------------------------
@@ -337,7 +346,12 @@ class ForceBatch:
             o, arrs = abi.make_outputs(e.n_move, e.n_rig, e.n_pairs)
             outs[i] = o
             keep.append(arrs)
-        _check(lib().mdf_batch_read_outputs(self._h, outs))
+        rc = lib().mdf_batch_read_outputs(self._h, outs)
+        if rc != abi.MDF_OK:
+            err = MDForceError(rc, lib().mdf_last_error().decode())
+            if rc == abi.MDF_ERR_LIST_OVERFLOW and all(outs[i].n_list_builds > 0 for i in range(n)):
+                err.result = [abi.collect(outs[i], keep[i], e.n_move, e.n_rig) for i, e in enumerate(self.engines)]
+            raise err
         return [abi.collect(outs[i], keep[i], e.n_move, e.n_rig) for i, e in enumerate(self.engines)]
 
     def autotune(self, reps: int = 5) -> dict:

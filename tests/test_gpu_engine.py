@@ -55,14 +55,16 @@ def test_fp32_force_error_bound(oracle_lib, name):
 
 
 def _mma_system(name):
-    # slab512 at k-index 14 and 18: knmax 42 and 54, 6 and 7 kn tiles of 16 (the kernels'
-    # register layouts change with the tile count)
+    # slab512 at k-index 11, 14, 18 and 21: knmax 33, 42, 54 and 63, 5, 6, 7 and 8 kn tiles of
+    # 16 (the kernels' register layouts and launch attributes change with the tile count; 11
+    # and 21 are the k-sets of slab16k and slab32k)
     if name.startswith("slab512_k"):
         return systems.water_on_slab(512, 6, 4, seed=1201, rcut=10.0, kmax=int(name[9:]), lw=40.0, name=name)
     return systems.config(name)
 
 
-@pytest.mark.parametrize("name", ["slab2048", "slab512", "bulk216", "slab512_k14", "slab512_k18"])
+@pytest.mark.parametrize("name", ["slab2048", "slab512", "bulk216", "slab512_k11", "slab512_k14", "slab512_k18",
+                                  "slab512_k21"])
 def test_fp32_structure_factor_on_matrix_cores(name, monkeypatch):
     """The FP32 form's reciprocal sum on the matrix cores (FP16 hi/lo split products, FP32
     tile sums, FP64 across blocks and rows): S(k) against the FP64 form's and the FP32
@@ -196,3 +198,31 @@ def test_autotune_then_parity(oracle_lib):
     r = e.force_get(*pos)
     ro = oracle_lib.Oracle(s).compute(*pos)
     assert rel_err(r["f_site"], ro["f_site"]) < 1e-10
+
+
+def test_fp16_split_range_of_large_structure_factors():
+    """The matrix-core back-projection splits T = A_k conj S(k) and kn T into FP16 hi/lo
+    parts; FP16 stops at 65504.  With every charge of slab512 scaled by 5000, max |T| and
+    max |kn T| are about 1.6e5 and 2.5e5 (32 and 50 unscaled); the per-row power-of-two scaling keeps the FP32 form
+    within its usual error of the FP64 form (unscaled, hi would saturate and the forces
+    would be wrong by far more)."""
+    _say("case fp16 range slab512 x5000")
+    s = systems.config("slab512")
+    q0 = s.charges
+    s.charges = lambda: q0() * 5000.0
+    pos = abi.positions(s)
+    e64 = _engine(s, precision=abi.FP64)
+    r64 = e64.force_get(*pos)
+    kv, ak, s64 = e64.kvectors()
+    t = ak * np.abs(s64)
+    kn = np.abs(np.asarray(kv).reshape(-1, 3)[:, 2])
+    _say("  max |T| %.3e, max |kn T| %.3e" % (t.max(), (kn * t).max()))
+    assert t.max() > 2 * 65504 and (kn * t).max() > 2 * 65504
+    e32 = _engine(s, precision=abi.FP32_ACC64)
+    assert e32.reciprocal_on_matrix_cores()
+    r32 = e32.force_get(*pos)
+    err = rms_rel(r32["f_site"], r64["f_site"])
+    _say("  fp32 (matrix cores) force rms rel err vs fp64 %.3e" % err)
+    assert err < 1e-5
+    assert rms_rel(r32["torque"], r64["torque"]) < 1e-5
+    assert abs(r32["e_recip"] - r64["e_recip"]) < 1e-5 * abs(r64["e_recip"])
