@@ -350,6 +350,25 @@ int mdf_mts_eval(const float *weights, int32_t n, const float *feat, const float
                  const float *feat_b, const float *vec_b, const double *target_b, float lr, int32_t adam_t,
                  float *f_out, float *grad_out, float *params_out);
 
+/* mdf_mts_eval on device-resident arrays, allocating nothing.  No counterpart in the
+   reference (nor an allocating one there): for callers that cannot allocate device memory -
+   code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.  Every
+   pointer is a device pointer: weights [MDF_MTS_WEIGHTS], feat [n][48], vec [n][16][3], targets
+   [3][n] (FP64), f_out [3][n] (required), grad_out [MDF_MTS_WEIGHTS - 112 + 1] and params_out
+   [MDF_MTS_WEIGHTS - 112] as in mdf_mts_eval (written only with targets).  The weights are not
+   checked (mdf_mts_eval checks its host copy); equal targets make the gradient non-finite.
+   Enqueued on `stream` (a hipStream_t; NULL: the null stream) and not waited for.  The scratch
+   (the scheme state, bfloat16 weight copies, Adam moments, per-block gradient partials:
+   MDF_MTS_EVAL_SCRATCH_BYTES) is one static array per device, which every call on that device
+   shares and which keeps the last call's contents (each call sets it up again): calls that
+   overlap on one device race.  n <= MDF_MTS_EVAL_MAX_SITES (the kernels index sites in 32-bit
+   arithmetic); larger n is refused with MDF_ERR_ARG. */
+#define MDF_MTS_EVAL_SCRATCH_BYTES 4430080   /* 4.2 MiB */
+#define MDF_MTS_EVAL_MAX_SITES (1 << 24)
+int mdf_mts_eval_device(const float *weights, int32_t n, const float *feat, const float *vec, const double *target,
+                        const float *feat_b, const float *vec_b, const double *target_b, float lr, int32_t adam_t,
+                        float *f_out, float *grad_out, float *params_out, void *stream);
+
 /* Timing.  Steps normally replay one captured HIP graph; with stage timing on they use
    direct launches with HIP events between the stages instead (events recorded during
    graph capture cannot be timed).  mdf_stage_times gives names and mean milliseconds per

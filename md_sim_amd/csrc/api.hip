@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -56,6 +57,8 @@ __global__ void k_sg_pack(DevSys S, MtsArgs a);
 __global__ void k_sg_infer(MtsArgs a, int n, int after_training, const DevFlags *flags);
 __global__ void k_sg_train(MtsArgs a, int n, int it, const DevFlags *flags);
 __global__ void k_sg_adam(MtsArgs a, int nb, int it, const DevFlags *flags);
+__global__ void k_sg_eval_prep(MtsArgs a, const float *weights, int n, const double *target, const double *target_b,
+                               int adam_t);
 __global__ void k_sg_combine(DevSys S, MtsArgs a);
 __global__ void k_sg_finish(DevSys S, MtsArgs a, Scalars *scal);
 __global__ void k_sg_exact(DevSys S, MtsArgs a, const double *fpart, int n_groups);
@@ -2381,6 +2384,88 @@ int mdf_mts_eval(const float *weights, int32_t n, const float *feat, const float
     d_wb.release(); d_frec.release(); d_frec_b.release();
     if (stream) (void)hipStreamDestroy(stream);
     if (e != hipSuccess) return fail(MDF_ERR_HIP, "surrogate evaluation: %s", hipGetErrorString(e));
+    return MDF_OK;
+}
+
+// ---- the same evaluation allocating nothing (mdf_mts_eval_device) ------------------------
+// Its scratch - the scheme state, the FP32 parameters being trained, input normalisation,
+// bfloat16 weight copies, Adam moments, the gradient sum and the training kernel's per-block
+// gradient partials - is one static device array.  Each device holds its own copy from the
+// load of this library to the end of the process; every call on a device shares it.
+static constexpr size_t ev_align(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr size_t EV_OFF_ST = 0;
+constexpr size_t EV_OFF_PARAMS = EV_OFF_ST + ev_align(sizeof(MtsState));
+constexpr size_t EV_OFF_NORM = EV_OFF_PARAMS + ev_align(sizeof(float) * SG_NP);
+constexpr size_t EV_OFF_WB = EV_OFF_NORM + ev_align(sizeof(float) * SG_NORM_N);
+constexpr size_t EV_OFF_M = EV_OFF_WB + ev_align(sizeof(unsigned short) * SG_WB_N);
+constexpr size_t EV_OFF_V = EV_OFF_M + ev_align(sizeof(float) * SG_NP);
+constexpr size_t EV_OFF_GSUM = EV_OFF_V + ev_align(sizeof(float) * SG_NP);
+constexpr size_t EV_OFF_GPART = EV_OFF_GSUM + ev_align(sizeof(float) * SG_GP);
+constexpr size_t EV_SCRATCH_BYTES = EV_OFF_GPART + ev_align(sizeof(float) * SG_TRAIN_BLOCKS * SG_GP);
+static_assert(EV_SCRATCH_BYTES == MDF_MTS_EVAL_SCRATCH_BYTES, "the header states the scratch size");
+
+static __device__ __align__(256) unsigned char scratch[EV_SCRATCH_BYTES];
+
+constexpr int EV_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static unsigned char *g_scratch_dev[EV_MAX_DEVICES] = {};
+
+// the address of this device's copy of `scratch` (the host-side name is no device address),
+// looked up once per device with that device current
+static int scratch_on_device(int dev, unsigned char **out) {
+    if (dev < 0 || dev >= EV_MAX_DEVICES) return fail(MDF_ERR_ARG, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(MDF_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+        g_scratch_dev[dev] = static_cast<unsigned char *>(p);
+    }
+    *out = g_scratch_dev[dev];
+    return MDF_OK;
+}
+
+int mdf_mts_eval_device(const float *weights, int32_t n, const float *feat, const float *vec, const double *target,
+                        const float *feat_b, const float *vec_b, const double *target_b, float lr, int32_t adam_t,
+                        float *f_out, float *grad_out, float *params_out, void *stream) {
+    if (!weights || !feat || !vec || !f_out) return fail(MDF_ERR_ARG, "null argument");
+    if (n <= 0 || n > MDF_MTS_EVAL_MAX_SITES)
+        return fail(MDF_ERR_ARG, "n = %d sites: 1 .. %d supported", n, MDF_MTS_EVAL_MAX_SITES);
+    if (adam_t < 0 || !(lr >= 0.0f)) return fail(MDF_ERR_ARG, "adam_t >= 0 and lr >= 0 required");
+    const bool train = target != nullptr;
+    if (train && (!feat_b || !vec_b || !target_b)) return fail(MDF_ERR_ARG, "training needs the second frame");
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    unsigned char *sp = nullptr;
+    int rc = scratch_on_device(dev, &sp);
+    if (rc) return rc;
+    int n_sblk, n_iblk, n_tblk;
+    mts_geometry(n, n_sblk, n_iblk, n_tblk);
+    MtsArgs a;
+    memset(&a, 0, sizeof a);
+    a.st = reinterpret_cast<MtsState *>(sp + EV_OFF_ST);
+    a.model = 1; a.n_sblk = n_sblk; a.n_tblk = n_tblk; a.lr = lr; a.train_iters = 1; a.train_stride = 1;
+    a.feat = const_cast<float *>(feat); a.vec = const_cast<float *>(vec); a.sg = f_out;
+    a.frec = const_cast<double *>(target);
+    a.feat_prev = const_cast<float *>(feat_b); a.vec_prev = const_cast<float *>(vec_b);
+    a.frec_prev = const_cast<double *>(target_b);
+    // trained parameters go straight to params_out; a forward pass leaves it alone
+    a.params = (train && params_out) ? params_out : reinterpret_cast<float *>(sp + EV_OFF_PARAMS);
+    a.norm = reinterpret_cast<float *>(sp + EV_OFF_NORM);
+    a.wb = reinterpret_cast<unsigned short *>(sp + EV_OFF_WB);
+    a.adam_m = reinterpret_cast<float *>(sp + EV_OFF_M);
+    a.adam_v = reinterpret_cast<float *>(sp + EV_OFF_V);
+    a.gsum = (train && grad_out) ? grad_out : reinterpret_cast<float *>(sp + EV_OFF_GSUM);
+    a.gpart = reinterpret_cast<float *>(sp + EV_OFF_GPART);
+    hipStream_t st = (hipStream_t)stream;
+    k_sg_eval_prep<<<1, 1024, 0, st>>>(a, weights, n, target, target_b, adam_t);
+    k_sg_infer<<<n_iblk, 256, 0, st>>>(a, n, 0, nullptr);
+    if (train) {
+        k_sg_train<<<n_tblk, 256, 0, st>>>(a, n, 0, nullptr);
+        k_sg_adam<<<(SG_GP + 255) / 256, 256, 0, st>>>(a, n_tblk, 0, nullptr);
+    }
+    HIPCHK(hipGetLastError());
     return MDF_OK;
 }
 

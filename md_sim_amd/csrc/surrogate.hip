@@ -721,6 +721,59 @@ __global__ __launch_bounds__(256) void k_sg_adam(MtsArgs a, int nb, int it, cons
     }
 }
 
+// Set-up of one evaluation outside any handle that allocates nothing (mdf_mts_eval_device):
+// the FP32 parameters and input normalisation from the caller's weights, their bfloat16
+// copies (and transposes) as k_sg_adam makes them, zero Adam moments, and the scheme state
+// with t2 = mean over sites of |T_a - T_b|^2 (the loss scale).  One block; every buffer the
+// later kernels read is written here, so nothing of an earlier call survives in them.
+__global__ __launch_bounds__(1024) void k_sg_eval_prep(MtsArgs a, const float *__restrict__ weights, int n,
+                                                       const double *__restrict__ target,
+                                                       const double *__restrict__ target_b, int adam_t) {
+    __shared__ double red[1024];
+    const bool train = target != nullptr;
+    double acc = 0.0;
+    if (train)
+        for (size_t k = threadIdx.x; k < 3 * (size_t)n; k += blockDim.x) {
+            const double d = target[k] - target_b[k];
+            acc += d * d;
+        }
+    red[threadIdx.x] = acc;
+    for (int p = threadIdx.x; p < SG_NP; p += blockDim.x) {
+        a.params[p] = weights[p];
+        a.adam_m[p] = 0.0f;
+        a.adam_v[p] = 0.0f;
+    }
+    for (int k = threadIdx.x; k < SG_NORM_N; k += blockDim.x) a.norm[k] = weights[SG_NP + k];
+    for (int k = threadIdx.x; k < SG_H * SG_D; k += blockDim.x) a.wb[SG_WB_W1 + k] = f2bf(weights[SG_OFF_W1 + k]);
+    for (int k = threadIdx.x; k < SG_H * SG_H; k += blockDim.x) {
+        const int u2 = k / SG_H, u1 = k % SG_H;
+        const unsigned short v = f2bf(weights[SG_OFF_W2 + k]);
+        a.wb[SG_WB_W2 + k] = v;
+        a.wb[SG_WB_W2T + u1 * SG_H + u2] = v;
+    }
+    for (int k = threadIdx.x; k < SG_M * SG_H; k += blockDim.x) {
+        const int m3 = k / SG_H, u = k % SG_H;
+        const unsigned short v = f2bf(weights[SG_OFF_W3 + k]);
+        a.wb[SG_WB_W3 + k] = v;
+        a.wb[SG_WB_W3T + u * SG_M + m3] = v;
+    }
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        MtsState st = {};
+        st.norm_ready = 1;
+        st.have_prev = train ? 1 : 0;
+        st.train_now = train ? 1 : 0;
+        st.t2 = train ? red[0] / n : 0.0;
+        st.adam_base = adam_t;
+        st.adam_t = adam_t + (train ? 1 : 0);
+        *a.st = st;
+    }
+}
+
 // weight of the correction's slope at this step: (t - t0) / (t0 - t1), or 0
 __device__ __forceinline__ double slope_weight(const MtsState &st) {
     if (!st.have_slope || st.delta_step <= st.delta_prev_step) return 0.0;

@@ -57,7 +57,7 @@ def lib() -> C.CDLL:
                    "mdf_batch_read_outputs", "mdf_batch_autotune", "mdf_mts_enable", "mdf_mts_get_stats",
                    "mdf_mts_get_weights", "mdf_mts_set_weights", "mdf_mts_features", "mdf_mts_eval",
                    "mdf_observables_enable", "mdf_get_observables", "mdf_chempot_enable", "mdf_get_chempot",
-                   "mdf_reciprocal_kernels"):
+                   "mdf_reciprocal_kernels", "mdf_mts_eval_device"):
             getattr(L, fn).restype = C.c_int
         L.mdf_destroy.restype = None
         L.mdf_batch_destroy.restype = None
@@ -318,6 +318,18 @@ def mts_eval(weights, feat, vec, target=None, feat_b=None, vec_b=None, target_b=
         if lr > 0:
             out["params"] = params
     return out
+
+
+def mts_eval_device(weights, n: int, feat, vec, f_out, target=None, feat_b=None, vec_b=None, target_b=None,
+                    lr: float = 0.0, adam_t: int = 0, grad_out=None, params_out=None, stream=None):
+    """mts_eval on device-resident arrays (torch tensors on the GPU, or device pointers as
+    ints), allocating nothing: weights [8448] float32, feat [n, 48], vec [n, 16, 3] float32,
+    targets [3, n] float64, outputs f_out [3, n], grad_out [8337], params_out [8336] float32.
+    Enqueued on ``stream`` (default: torch's current stream) and not waited for."""
+    ptr = lambda t: None if t is None else C.c_void_p(t if isinstance(t, int) else t.data_ptr())
+    _check(lib().mdf_mts_eval_device(ptr(weights), int(n), ptr(feat), ptr(vec), ptr(target), ptr(feat_b),
+                                     ptr(vec_b), ptr(target_b), C.c_float(lr), int(adam_t), ptr(f_out),
+                                     ptr(grad_out), ptr(params_out), C.c_void_p(_producer_stream(stream))))
 
 
 class ForceBatch:

@@ -1,6 +1,8 @@
 """Run the reference golden-vector harness (oracle/_ref, built by oracle/build_ref.sh) on a
-System and parse its dump.  Test infrastructure: used only to make and check golden
-vectors in the development container, never by the product path."""
+System and parse its dump.  Test infrastructure, never part of the product path: it makes
+and checks golden vectors in the development container, and bench.py's cpu_baseline leg
+times it (the harness binaries built here travel with the tree; the reference sources do
+not) on the host cores wherever it runs, the GPU box included."""
 from __future__ import annotations
 
 import os
