@@ -230,9 +230,10 @@ int mdf_autotune(mdf_system *sys, int32_t reps, int32_t *chosen);
 int mdf_set_launch_shapes(mdf_system *sys, int32_t pair_lanes, int32_t pair_rolled, int32_t sf_blocks,
                           int32_t bp_blocks);
 
-/* Which reciprocal-sum kernels the handle runs: 1 = on the matrix cores (the FP32 form's
-   structure factor and back-projection as FP16 hi/lo split products, v_mfma_f32_16x16x32_f16;
-   environment MDF_SF_MMA=0 / MDF_BP_MMA=0 at creation select the vector-ALU kernels), 0 = the
+/* Which reciprocal-sum kernels the handle runs: 1 = on the matrix cores (the FP64 form's
+   structure factor and back-projection on v_mfma_f64_16x16x4_f64, environment MDF_MMA64=0 at
+   creation selecting the vector-ALU kernels; the FP32 form's as FP16 hi/lo split products on
+   v_mfma_f32_16x16x32_f16, MDF_SF_MMA=0 / MDF_BP_MMA=0 selecting the vector ALU), 0 = the
    vector ALU.  No counterpart in the reference (its loops have no kernel choice). */
 int mdf_reciprocal_kernels(const mdf_system *sys, int32_t *sf_matrix, int32_t *bp_matrix);
 

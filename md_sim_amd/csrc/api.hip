@@ -78,6 +78,17 @@ __global__ void k_sf_reduce_rows(const double2 *, const float2 *, int, int, cons
                                  const double *, double, double, double, double, double2 *, double4 *, double *,
                                  const MtsState *, unsigned *, int *, int, int);
 template <int NT>
+__global__ void k_sf64(const double *, const double *, const double *, const double *, int, double, double, double,
+                       const int2 *, const int *, const int2 *, int, int, int, const int *, int, double2 *, int,
+                       const MtsState *);
+__global__ void k_sf64_reduce(const double2 *, int, int, const int *, const int *, const KRow *, const double2 *,
+                              const double *, double, double, double, double, double2 *, double4 *, double *,
+                              const int2 *, double *, const MtsState *);
+template <int NKS, int NG>
+__global__ void k_bp64(const double *, const double *, const double *, const double *, int, double, double, double,
+                       const int4 *, int, const int2 *, const int2 *, const double *, double, double, double, double *,
+                       const MtsState *);
+template <int NT>
 __global__ void k_bp_mma(const double *x, const double *y, const double *z, const double *q, int n, double cux,
                          double cuy, double cuz, const int2 *rowkk, const int2 *tile_ks, int n_rows, int knmax,
                          int n_nt, const unsigned *tpack, const int *tscale, double pipix, double pipiy, double pipiz,
@@ -270,6 +281,16 @@ struct mdf_system {
     DBuf<unsigned> d_tpack;          // T split for k_bp_mma, [row][plane][kn + knmax]
     DBuf<int> d_tscale;              // each row's power-of-two scales of the split T
     int n_sb_mma = 1, n_rg_mma = 1, knmax = 0, n_nt = 0;
+    // the FP64 form's reciprocal sum on the matrix cores (k_sf64, k_sf64_reduce, k_bp64)
+    bool mma64 = false, mma64_allowed = true;
+    int klmax = 0, kmmax = 0;
+    int s64_nt = 1, s64_ngrp = 1, s64_nsb = 1, s64_sites = SF64_SUB, s64_pcols = 1, s64_sites_env = 0;
+    int b64_nks = 4, b64_ng = 2, b64_nrg = 1, b64_nmt = 1, b64_sites = 128, b64_rg_env = 0, b64_ng_env = 0;
+    DBuf<int2> d_s64_rowkk, d_s64_groups, d_b64_tile, d_b64_rg;
+    DBuf<int> d_s64_tilent, d_s64_pcoff, d_s64_sorted_of;
+    DBuf<int4> d_b64_rinfo;
+    DBuf<double2> d_s64_part;
+    DBuf<double> d_b64_tpk;
     int sfm_sites = SFM_SITES, bpm_waves = BPM_WAVES_MAX;   // k_sf_mma sites per block, k_bp_mma waves
     int sfm_sites_env = 0, bpm_waves_env = 0;                // fixed by MDF_SFM_SITES / MDF_BPM_WAVES
     int n_cu = 256;                                          // compute units of the device
@@ -385,6 +406,9 @@ static void free_all(mdf_system *s) {
                           &s->colpart, &s->fcom, &s->torque, &s->e_vdw, &s->e_coul, &s->ang_kt, &s->ang_th};
     for (auto *b : db) b->release();
     s->d_rowkk.release(); s->d_tile_ks.release(); s->d_krow.release(); s->d_tpack.release(); s->d_tscale.release();
+    s->d_s64_rowkk.release(); s->d_s64_groups.release(); s->d_b64_tile.release(); s->d_b64_rg.release();
+    s->d_s64_tilent.release(); s->d_s64_pcoff.release(); s->d_s64_sorted_of.release(); s->d_b64_rinfo.release();
+    s->d_s64_part.release(); s->d_b64_tpk.release();
     s->flags.release(); s->d_bp_row_start.release(); s->d_chunks.release(); s->d_rows.release(); s->d_swall.release(); s->d_s.release();
     s->d_t.release(); s->sf_partial.release(); s->d_wl.release(); s->d_scal.release();
     mts_free(s);
@@ -509,6 +533,14 @@ static void geometry(mdf_system *s) {
     s->knmax = 0;
     for (size_t k = 2; k < s->kvec.size(); k += 3) s->knmax = std::max(s->knmax, std::abs(s->kvec[k]));
     s->n_nt = (2 * s->knmax + 1 + 15) / 16;
+    s->klmax = 0; s->kmmax = 0;
+    for (size_t k = 0; k + 2 < s->kvec.size(); k += 3) {
+        s->klmax = std::max(s->klmax, std::abs(s->kvec[k]));
+        s->kmmax = std::max(s->kmmax, std::abs(s->kvec[k + 1]));
+    }
+    // the FP64 form on the matrix cores: tables index kl, km and kn up to 63
+    s->mma64 = s->mma64_allowed && s->par.precision == MDF_FP64 && nk > 0 && s->knmax <= 63 && s->klmax <= 63 &&
+               s->kmmax <= 63 && sf64_lds_bytes(s->klmax, s->kmmax, (2 * s->knmax + 1 + 15) / 16) <= SF64_LDS_MAX;
     s->sf_mma = s->sf_mma_allowed && s->par.precision == MDF_FP32_ACC64 && nk > 0 && s->n_nt <= SFM_TILES_MAX;
     s->bp_mma = s->bp_mma_allowed && s->par.precision == MDF_FP32_ACC64 && nk > 0 && s->n_nt >= 1 &&
                 s->n_nt <= SFM_TILES_MAX && nrow <= BPM_ROWS;
@@ -578,6 +610,115 @@ static void geometry(mdf_system *s) {
 }
 
 static void drop_graphs(mdf_system *s);
+
+// Tables and launch shapes of the FP64 matrix-core reciprocal kernels.
+// Structure factor: rows sorted by kn extent E (widest first, stable), 16-row tiles with their
+// column tiles ceil((2 E + 1) / 16), groups of up to 4 consecutive tiles per block, partial
+// columns per row 2 E + 1 in sorted order.  Back-projection: M-tiles of 4 rows in the
+// reference order, K-steps ceil((2 E_max + 1) / 4) each, row groups of about equal K-steps.
+template <int NT>
+static const void *sf64_kernel_nt() { return reinterpret_cast<const void *>(&k_sf64<NT>); }
+static const void *sf64_kernel(int nt) {
+    switch (nt) {
+        case 1: return sf64_kernel_nt<1>(); case 2: return sf64_kernel_nt<2>(); case 3: return sf64_kernel_nt<3>();
+        case 4: return sf64_kernel_nt<4>(); case 5: return sf64_kernel_nt<5>(); case 6: return sf64_kernel_nt<6>();
+        case 7: return sf64_kernel_nt<7>(); default: return sf64_kernel_nt<8>();
+    }
+}
+
+static int setup_mma64(mdf_system *s) {
+    const int nr = (int)s->rows.size(), nm = s->top.n_move;
+    std::vector<int> E(nr);
+    for (int r = 0; r < nr; r++) E[r] = std::max(std::abs(s->rows[r].kn0), std::abs(s->rows[r].kn1));
+    // structure factor tables
+    std::vector<int> sorted(nr), sorted_of(nr);
+    for (int r = 0; r < nr; r++) sorted[r] = r;
+    std::stable_sort(sorted.begin(), sorted.end(), [&](int a, int b) { return E[a] > E[b]; });
+    for (int i = 0; i < nr; i++) sorted_of[sorted[i]] = i;
+    const int n_rt = (nr + 15) / 16;
+    std::vector<int2> rowkk((size_t)n_rt * 16, make_int2(0, 0));
+    std::vector<int> tile_nt(n_rt), pcoff(nr + 1, 0);
+    for (int i = 0; i < nr; i++) {
+        const KRow &w = s->rows[sorted[i]];
+        rowkk[i] = make_int2(w.kl, w.km);
+        pcoff[i + 1] = pcoff[i] + 2 * E[sorted[i]] + 1;
+    }
+    for (int t = 0; t < n_rt; t++) tile_nt[t] = (2 * E[sorted[16 * t]] + 1 + 15) / 16;
+    std::vector<int2> groups;
+    for (int t = 0; t < n_rt; t += 4) groups.push_back(make_int2(t, std::min(n_rt, t + 4)));
+    s->s64_nt = tile_nt[0];
+    s->s64_ngrp = (int)groups.size();
+    s->s64_pcols = pcoff[nr];
+    // sites per block: about one resident round of blocks over the compute units
+    {
+        int occ = 0;
+        const size_t lds = sf64_lds_bytes(s->klmax, s->kmmax, s->s64_nt);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, sf64_kernel(s->s64_nt), 256, lds) != hipSuccess ||
+            occ < 1) {
+            (void)hipGetLastError();
+            occ = 1;
+        }
+        int sites = s->s64_sites_env;
+        if (sites <= 0) {
+            const long want = (long)s->n_cu * occ;
+            const long nsb = std::max(1L, (want + s->s64_ngrp - 1) / s->s64_ngrp);
+            sites = (int)((nm + nsb - 1) / nsb);
+            sites = std::max(SF64_SUB, SF64_SUB * ((sites + SF64_SUB - 1) / SF64_SUB));
+        }
+        s->s64_sites = sites;
+        s->s64_nsb = std::max(1, (nm + sites - 1) / sites);
+    }
+    // back-projection tables
+    const int n_mt = (nr + 3) / 4;
+    std::vector<int2> bptile(n_mt);
+    int koff = 0, nks_max = 1;
+    for (int m = 0; m < n_mt; m++) {
+        int emax = 0;
+        for (int r = 4 * m; r < std::min(nr, 4 * m + 4); r++) emax = std::max(emax, E[r]);
+        const int nks = (2 * emax + 1 + 3) / 4;
+        bptile[m] = make_int2(koff, nks);
+        koff += nks;
+        nks_max = std::max(nks_max, nks);
+    }
+    std::vector<int4> rinfo(nr);
+    for (int r = 0; r < nr; r++) {
+        const KRow &w = s->rows[r];
+        const bool cont = r >= 4 && s->rows[r - 4].kl == w.kl && s->rows[r - 4].km + 4 == w.km;
+        rinfo[r] = make_int4(w.kl, w.km, cont ? 0 : 1, 0);
+    }
+    s->b64_nks = 4 * ((nks_max + 3) / 4);
+    s->b64_ng = s->b64_ng_env > 0 ? s->b64_ng_env : (s->b64_nks <= 20 ? 2 : 1);
+    s->b64_sites = 4 * 16 * s->b64_ng;   // BP64_WAVES waves of 16 NG sites
+    s->b64_nmt = n_mt;
+    const int nsb_bp = std::max(1, (nm + s->b64_sites - 1) / s->b64_sites);
+    int nrg = s->b64_rg_env > 0 ? s->b64_rg_env : std::max(1, std::min(8, (2 * s->n_cu + nsb_bp - 1) / nsb_bp));
+    nrg = std::min(nrg, n_mt);
+    std::vector<int2> rg;
+    {
+        int m = 0;
+        for (int gi = 0; gi < nrg; gi++) {
+            const long target = ((long)(gi + 1) * koff + nrg - 1) / nrg;
+            int m1 = m;
+            while (m1 < n_mt && (bptile[m1].x < target || m1 == m)) m1++;
+            if (gi == nrg - 1) m1 = n_mt;
+            rg.push_back(make_int2(m, m1));
+            m = m1;
+        }
+    }
+    s->b64_nrg = (int)rg.size();
+    HIPCHK(upload(s->d_s64_rowkk, rowkk.data(), rowkk.size()));
+    HIPCHK(upload(s->d_s64_tilent, tile_nt.data(), tile_nt.size()));
+    HIPCHK(upload(s->d_s64_groups, groups.data(), groups.size()));
+    HIPCHK(upload(s->d_s64_pcoff, pcoff.data(), pcoff.size()));
+    HIPCHK(upload(s->d_s64_sorted_of, sorted_of.data(), sorted_of.size()));
+    HIPCHK(upload(s->d_b64_tile, bptile.data(), bptile.size()));
+    HIPCHK(upload(s->d_b64_rinfo, rinfo.data(), rinfo.size()));
+    HIPCHK(upload(s->d_b64_rg, rg.data(), rg.size()));
+    HIPCHK(s->d_s64_part.alloc((size_t)s->s64_nsb * s->s64_pcols));
+    HIPCHK(s->d_b64_tpk.alloc((size_t)std::max(koff, 1) * 64));
+    HIPCHK(hipMemset(s->d_b64_tpk.p, 0, sizeof(double) * s->d_b64_tpk.n));   // rows past the last stay zero
+    return MDF_OK;
+}
 
 // launch shapes of the reciprocal kernels for the current targets: row groups, their device
 // table and the partial-force buffer they need; captured graphs hold the old shapes
@@ -701,6 +842,23 @@ static void launch_sf(mdf_system *s) {
     if (nk == 0 || nm == 0) return;
     const RecipArgs a = recip_args(s);
     DevSys &S = s->S;
+    if (s->mma64) {
+        const dim3 grid(s->s64_nsb, s->s64_ngrp);
+        const size_t lds = sf64_lds_bytes(s->klmax, s->kmmax, s->s64_nt);
+#define SF64_LAUNCH(NT)                                                                                         \
+    case NT:                                                                                                    \
+        k_sf64<NT><<<grid, 256, lds, s->stream>>>(S.x, S.y, S.z, S.q, nm, a.cux, a.cuy, a.cuz, s->d_s64_rowkk.p,  \
+                                                  s->d_s64_tilent.p, s->d_s64_groups.p, nrows, s->klmax, s->kmmax, \
+                                                  s->d_s64_pcoff.p, s->s64_pcols, s->d_s64_part.p, s->s64_sites,   \
+                                                  mts_state(s));                                                 \
+        break;
+        switch (s->s64_nt) {
+            SF64_LAUNCH(1) SF64_LAUNCH(2) SF64_LAUNCH(3) SF64_LAUNCH(4) SF64_LAUNCH(5) SF64_LAUNCH(6) SF64_LAUNCH(7)
+            SF64_LAUNCH(8)
+        }
+#undef SF64_LAUNCH
+        return;
+    }
     if (s->sf_mma) {
         const dim3 grid(s->n_sb_mma, s->n_rg_mma);
         const size_t lds = sfm_lds_bytes(s->sfm_sites, s->n_nt);
@@ -732,6 +890,24 @@ static void launch_bp(mdf_system *s) {
     if (nk == 0 || nm == 0) return;
     const RecipArgs a = recip_args(s);
     DevSys &S = s->S;
+    if (s->mma64) {
+        const dim3 grid((nm + s->b64_sites - 1) / s->b64_sites, s->b64_nrg);
+#define BP64_LAUNCH(NKS, NG)                                                                                    \
+    case NKS * 4 + NG:                                                                                          \
+        k_bp64<NKS, NG><<<grid, 256, 0, s->stream>>>(S.x, S.y, S.z, S.q, nm, a.cux, a.cuy, a.cuz, s->d_b64_rinfo.p, \
+                                                     (int)s->rows.size(), s->d_b64_tile.p, s->d_b64_rg.p,        \
+                                                     s->d_b64_tpk.p, a.pipix, a.pipiy, a.pipiz, s->fpart.p,      \
+                                                     mts_state(s));                                               \
+        break;
+        switch (s->b64_nks * 4 + s->b64_ng) {
+            BP64_LAUNCH(4, 1) BP64_LAUNCH(8, 1) BP64_LAUNCH(12, 1) BP64_LAUNCH(16, 1) BP64_LAUNCH(20, 1)
+            BP64_LAUNCH(24, 1) BP64_LAUNCH(28, 1) BP64_LAUNCH(32, 1) BP64_LAUNCH(4, 2) BP64_LAUNCH(8, 2)
+            BP64_LAUNCH(12, 2) BP64_LAUNCH(16, 2) BP64_LAUNCH(20, 2) BP64_LAUNCH(24, 2) BP64_LAUNCH(28, 2)
+            BP64_LAUNCH(32, 2)
+        }
+#undef BP64_LAUNCH
+        return;
+    }
     if (s->bp_mma) {
         const int ns = 16 * s->bpm_waves;
         const dim3 grid((nm + ns - 1) / ns), block(64 * s->bpm_waves);
@@ -769,7 +945,7 @@ static FinalArgs final_args(const mdf_system *s) {
     A.n_slots = (int)s->slot_type.size();
     for (int k = 0; k < A.n_slots; k++) A.slot_type[k] = s->slot_type[k];
     for (int c = 0; c < NCLS; c++) A.cls_type[c] = s->cls_type[c];
-    A.n_kblk = s->ak.size() > 0 ? (s->bp_mma ? (int)s->rows.size() : s->n_kblk) : 0;
+    A.n_kblk = s->ak.size() > 0 ? ((s->bp_mma || s->mma64) ? (int)s->rows.size() : s->n_kblk) : 0;
     A.n_epblk = s->n_epblk;
     A.e_self = s->uself;
     A.e_recip_ww = s->ewqWW;
@@ -811,7 +987,12 @@ static void launch_forces(mdf_system *s) {
         double lamb = 1.0 / (4.0 * S.alfa * S.alfa);
         const int n_sb = nm > 0 ? (s->sf_mma ? s->n_sb_mma : s->n_sb) : 0;
         const float2 *p32 = s->sf_mma && nm > 0 ? reinterpret_cast<const float2 *>(s->sf_partial.p) : nullptr;
-        if (s->bp_mma)   // one block per row: T split with per-row scales for k_bp_mma
+        if (s->mma64)    // one block per row: S, T, energies and the back-projection operand
+            k_sf64_reduce<<<(int)s->rows.size(), SFR_THREADS, 0, st>>>(
+                s->d_s64_part.p, nm > 0 ? s->s64_nsb : 0, s->s64_pcols, s->d_s64_pcoff.p, s->d_s64_sorted_of.p,
+                s->d_rows.p, s->d_swall.p, s->d_ak.p, ra.pipix, ra.pipiy, ra.pipiz, lamb, s->d_s.p, s->d_t.p,
+                s->kpart.p, s->d_b64_tile.p, s->d_b64_tpk.p, mts_state(s));
+        else if (s->bp_mma)   // one block per row: T split with per-row scales for k_bp_mma
             k_sf_reduce_rows<<<(int)s->rows.size(), SFR_THREADS, 0, st>>>(
                 s->sf_partial.p, p32, n_sb, nk, s->d_rows.p, s->d_swall.p, s->d_ak.p, ra.pipix, ra.pipiy, ra.pipiz,
                 lamb, s->d_s.p, s->d_t.p, s->kpart.p, mts_state(s), s->d_tpack.p, s->d_tscale.p, 16 * s->n_nt,
@@ -825,7 +1006,7 @@ static void launch_forces(mdf_system *s) {
     if (s->ev) (void)hipEventRecord(s->ev[5], st);
     launch_bp(s);
     if (s->ev) (void)hipEventRecord(s->ev[6], st);
-    const int n_groups = (nk > 0 && nm > 0) ? (s->bp_mma ? 1 : s->n_rg) : 0;
+    const int n_groups = (nk > 0 && nm > 0) ? (s->mma64 ? s->b64_nrg : s->bp_mma ? 1 : s->n_rg) : 0;
     k_epilogue<<<s->n_epblk, EPI_BLOCK, 0, st>>>(S, s->fpart.p, n_groups, s->d_wl.p, s->par.ew3dc, s->crf0,
                                                  s->dipz.p, s->eppart.p, s->fcom.p, s->torque.p, mts_state(s),
                                                  s->sg_fmts.p, s->obs_on ? s->obs_efi.p : nullptr);
@@ -1198,6 +1379,14 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
         if (sm && sm[0] == '0') s->sf_mma_allowed = false;
         const char *bm = getenv("MDF_BP_MMA");
         if (bm && bm[0] == '0') s->bp_mma_allowed = false;
+        const char *m64 = getenv("MDF_MMA64");
+        if (m64 && m64[0] == '0') s->mma64_allowed = false;
+        const char *s64 = getenv("MDF_SF64_SITES");
+        if (s64 && atoi(s64) > 0) s->s64_sites_env = SF64_SUB * ((atoi(s64) + SF64_SUB - 1) / SF64_SUB);
+        const char *g64 = getenv("MDF_BP64_NG");
+        if (g64 && (atoi(g64) == 1 || atoi(g64) == 2)) s->b64_ng_env = atoi(g64);
+        const char *r64 = getenv("MDF_BP64_GROUPS");
+        if (r64 && atoi(r64) > 0) s->b64_rg_env = std::min(64, atoi(r64));
         // k_sf_mma splits q exp(i phi) of every moving site into FP16 hi/lo parts, which hold
         // magnitudes below 65504: far larger charges than any physical one (|q| > 2^14 e) take
         // the vector-ALU structure factor instead
@@ -1516,6 +1705,7 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
                 HTRY(s->d_tscale.alloc(std::max(nr, 1)));
                 HTRY(hipMemset(s->d_tscale.p, 0, sizeof(int) * s->d_tscale.n));
             }
+            if (s->mma64) TRY(setup_mma64(s));
             HTRY(s->kpart.alloc((size_t)RE_N * std::max(s->n_kblk, (int)s->rows.size())));
             std::vector<double2> zero2(std::max(nk, 1), make_double2(0.0, 0.0));
             HTRY(upload(s->d_swall, zero2.data(), zero2.size()));
@@ -1523,7 +1713,7 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
             int per_sb = SF_BLOCK * 4;
             int n_sb_w = std::max(1, (nw + per_sb - 1) / per_sb);
             HTRY(s->sf_partial.alloc((size_t)std::max(std::max(s->n_sb, s->n_sb_mma), n_sb_w) * std::max(nk, 1)));
-            HTRY(s->fpart.alloc((size_t)s->n_rg * 3 * std::max(t.n_move, 1)));
+            HTRY(s->fpart.alloc((size_t)std::max(s->n_rg, s->b64_nrg) * 3 * std::max(t.n_move, 1)));
             s->s_wall.assign(2 * std::max(nk, 1), 0.0);
             s->ewqWW = 0.0;
             if (nw > 0 && nk > 0) {
@@ -2063,8 +2253,8 @@ int mdf_mts_enable(mdf_system *s, const mdf_mts_params *p) {
 
 int mdf_reciprocal_kernels(const mdf_system *s, int32_t *sf_matrix, int32_t *bp_matrix) {
     if (!s) return fail(MDF_ERR_ARG, "null handle");
-    if (sf_matrix) *sf_matrix = s->sf_mma ? 1 : 0;
-    if (bp_matrix) *bp_matrix = s->bp_mma ? 1 : 0;
+    if (sf_matrix) *sf_matrix = (s->sf_mma || s->mma64) ? 1 : 0;
+    if (bp_matrix) *bp_matrix = (s->bp_mma || s->mma64) ? 1 : 0;
     return MDF_OK;
 }
 

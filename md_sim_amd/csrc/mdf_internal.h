@@ -35,6 +35,17 @@ __host__ __device__ inline size_t sfm_lds_bytes(int sites, int n_nt) {
     return (size_t)sites * 44 + (size_t)2 * 32 * (16 * n_nt + 4) * 4 + (size_t)2 * 32 * 68 * 4;
 }
 constexpr int BP_GROUP_K = 1024;     // k-vectors per back-projection row group (its LDS stage)
+// FP64 reciprocal sum on the matrix cores: sites per LDS stage of the structure factor, and
+// the powers e, e^2, e^4, e^8 of a site's base phases kept for the tables (a table entry
+// n = g + 8 i starts from e^g, g < 8, and steps by e^8)
+constexpr int SF64_SUB = 32;
+constexpr int SF64_POWS = 4;
+// dynamic LDS of k_sf64: base-phase powers, q e^{i pi kl u}, e^{i pi km v} for km >= 0
+// (complex) and the cos/sin columns (16 n_nt + 2 per site) of one stage
+__host__ __device__ inline size_t sf64_lds_bytes(int klmax, int kmmax, int n_nt) {
+    return (size_t)SF64_SUB * (16 * (3 * SF64_POWS + (klmax + 1) + (kmmax + 1)) + 8 * (16 * n_nt + 2));
+}
+constexpr size_t SF64_LDS_MAX = 64 * 1024 - 512;   // dynamic LDS of a k_sf64 block, at most
 
 // k-set rows (contiguous kn) and structure-factor chunks (<= KC consecutive kn of a row)
 struct KChunk { int kl, km, kn0, len, koff; };

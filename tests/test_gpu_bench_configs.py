@@ -116,14 +116,16 @@ def _batch_step(syss, prec):
 
 
 def test_replica_batch_64_fp64_against_oracle_and_handles(oracle_lib, replicas):
-    """The benchmark's 64-replica batch in FP64: every replica equals its own handle bit for
-    bit, and a sample of replicas (smallest, largest, one between) matches the oracle."""
+    """The benchmark's 64-replica batch in FP64: every replica agrees with its own handle to
+    FP64 rounding (the handle runs its reciprocal sum on the FP64 matrix cores, the batch on
+    the vector ALU), and a sample of replicas (smallest, largest, one between) matches the
+    oracle."""
     _say("case 64-replica batch fp64")
     engines, out = _batch_step(replicas, abi.FP64)
     for i, (s, e) in enumerate(zip(replicas, engines)):
         r = e.force_get(*abi.positions(s))
-        assert np.array_equal(out[i]["f_site"], r["f_site"]), i
-        assert out[i]["e_conf"] == r["e_conf"], i
+        assert rel_err(out[i]["f_site"], r["f_site"]) < 1e-12, i
+        assert abs(out[i]["e_conf"] - r["e_conf"]) <= 1e-12 * abs(r["e_conf"]), i
     sizes = [s.nrig for s in replicas]
     sample = sorted({int(np.argmin(sizes)), int(np.argmax(sizes)), 17})
     for i in sample:

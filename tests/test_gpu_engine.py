@@ -226,3 +226,33 @@ def test_fp16_split_range_of_large_structure_factors():
     assert err < 1e-5
     assert rms_rel(r32["torque"], r64["torque"]) < 1e-5
     assert abs(r32["e_recip"] - r64["e_recip"]) < 1e-5 * abs(r64["e_recip"])
+
+
+@pytest.mark.parametrize("name", ["bulk216", "pore_small", "slab2048", "slab512_k11", "slab512_k21"])
+def test_fp64_matrix_cores_against_vector_alu(oracle_lib, name, monkeypatch):
+    """The FP64 form's reciprocal sum on the FP64 matrix cores (k_sf64, k_sf64_reduce, k_bp64:
+    kn symmetry, sorted rows, branch-free product sequences per tile width) against the
+    vector-ALU kernels and the oracle: S(k), site forces, energies and virial at FP64
+    rounding; kn tiles 5 and 8 cover the widths of slab16k and slab32k."""
+    _say("case fp64 matrix cores", name)
+    s = _mma_system(name)
+    pos = abi.positions(s)
+    e = _engine(s, precision=abi.FP64)
+    assert e.reciprocal_on_matrix_cores()
+    r = e.force_get(*pos)
+    _, _, sm = e.kvectors()
+    monkeypatch.setenv("MDF_MMA64", "0")
+    ev = _engine(s, precision=abi.FP64)
+    assert not ev.reciprocal_on_matrix_cores()
+    rv = ev.force_get(*pos)
+    _, _, sv = ev.kvectors()
+    es = np.abs(sm - sv).max() / np.abs(sv).max()
+    ef = rel_err(r["f_site"], rv["f_site"])
+    _say("  matrix vs vector: S(k) %.2e, forces %.2e" % (es, ef))
+    assert es < 1e-13 and ef < 1e-12
+    assert abs(r["e_recip"] - rv["e_recip"]) < 1e-13 * abs(rv["e_recip"])
+    assert np.abs(r["virial"] - rv["virial"]).max() < 1e-12 * np.abs(rv["virial"]).max()
+    if name in ("bulk216", "pore_small", "slab512_k11"):
+        ro = oracle_lib.Oracle(s).compute(*pos)
+        assert rel_err(r["f_site"], ro["f_site"]) < 1e-10
+        assert abs(r["e_conf"] - ro["e_conf"]) < 1e-10 * abs(ro["e_conf"])
