@@ -187,7 +187,7 @@ const char *mdf_last_error(void);
    Many systems - replicas of uneven size, each with its own box, lists and k-set - stepped
    together: every stage of the step is one launch over all of them, and the whole step is
    one replayed graph.  The systems stay owned by the caller and must not be stepped on their
-   own while the batch is in flight.  All must share device, precision and the virial switch;
+   own while the batch is in flight.  All must share device, precision, erfc form and the virial switch;
    the batch uses the first system's pair lanes per lead until mdf_batch_autotune picks its own. */
 typedef struct mdf_batch mdf_batch;
 

@@ -38,7 +38,7 @@ __global__ void k_snapshot(DevSys S);
 __global__ void k_wall_cells(DevSys S, int *wcell_of, int ncz, double z0, double csz);
 __global__ void k_count_cells(const int *item_cell, int n_items, int *cnt);
 __global__ void k_zero_int(int *p, int n);
-template <typename R, int G, bool VIR, bool ROLLED>
+template <typename R, int G, bool VIR, bool ACC>
 __global__ void k_pair(DevSys S);
 template <typename R, int NS>
 __global__ void k_sf(const double *, const double *, const double *, const double *, int, double, double, double,
@@ -134,7 +134,7 @@ __global__ void k_cell_sort_b(const RepArgs *, const int2 *);
 __global__ void k_build_b(const RepArgs *, const int2 *);
 __global__ void k_snapshot_b(const RepArgs *, const int2 *);
 __global__ void k_flags_b(const RepArgs *, int, DevFlags *);
-template <typename R, int G, bool VIR>
+template <typename R, int G, bool VIR, bool ACC>
 __global__ void k_pair_b(const RepArgs *, const int2 *);
 template <typename R>
 __global__ void k_sf_b(const RepArgs *, const int2 *);
@@ -257,6 +257,9 @@ struct mdf_system {
     bool fixed_cap = false;
     // device buffers
     DBuf<int> itype, nsi, ijtype, vkind, cls, lead, lead_slot, con_ptr, con_idx, ang_ptr, ang_inc, ang_ijk;
+    DBuf<int> pmeta, wcls;           // pair kernel: per moving site slot | nsi << 4, per frozen site class
+    DBuf<double> prec, ptab;         // pair kernel: molecule records, vdW table per (slot, class)
+    DBuf<double4> wrec;              // pair kernel: frozen sites (x, y, z, q)
     DBuf<double> q, eps4, sig6, aa, rhoi, cc, ang_kt, ang_th;
     DBuf<double> x, y, z, cx, cy, cz, ocx, ocy, ocz, ox, oy, oz, fx, fy, fz, part;
     DBuf<int> nbr_mm, cnt_mm, nbr_mw, cnt_mw, cnt_lo;
@@ -409,6 +412,7 @@ static void free_all(mdf_system *s) {
     s->d_s64_rowkk.release(); s->d_s64_groups.release(); s->d_b64_tile.release(); s->d_b64_rg.release();
     s->d_s64_tilent.release(); s->d_s64_pcoff.release(); s->d_s64_sorted_of.release(); s->d_b64_rinfo.release();
     s->d_s64_part.release(); s->d_b64_tpk.release();
+    s->pmeta.release(); s->wcls.release(); s->prec.release(); s->ptab.release(); s->wrec.release();
     s->flags.release(); s->d_bp_row_start.release(); s->d_chunks.release(); s->d_rows.release(); s->d_swall.release(); s->d_s.release();
     s->d_t.release(); s->sf_partial.release(); s->d_wl.release(); s->d_scal.release();
     mts_free(s);
@@ -782,28 +786,30 @@ static RecipArgs recip_args(const mdf_system *s) {
 }
 
 template <int G>
-static void launch_pair_g(mdf_system *s, bool f32, bool vir, bool rolled) {
+static void launch_pair_g(mdf_system *s, bool f32, bool vir, bool acc) {
     const int blocks = std::max(1, (s->n_lead + (256 / G) - 1) / (256 / G));
     hipStream_t st = s->stream;
     DevSys &S = s->S;
-#define MDF_PAIR(R, V, RO) k_pair<R, G, V, RO><<<blocks, 256, 0, st>>>(S)
+#define MDF_PAIR(R, V, A) k_pair<R, G, V, A><<<blocks, 256, 0, st>>>(S)
     if (f32) {
-        if (vir) { if (rolled) MDF_PAIR(float, true, true); else MDF_PAIR(float, true, false); }
-        else { if (rolled) MDF_PAIR(float, false, true); else MDF_PAIR(float, false, false); }
+        if (vir) { if (acc) MDF_PAIR(float, true, true); else MDF_PAIR(float, true, false); }
+        else { if (acc) MDF_PAIR(float, false, true); else MDF_PAIR(float, false, false); }
     } else {
-        if (vir) { if (rolled) MDF_PAIR(double, true, true); else MDF_PAIR(double, true, false); }
-        else { if (rolled) MDF_PAIR(double, false, true); else MDF_PAIR(double, false, false); }
+        if (vir) { if (acc) MDF_PAIR(double, true, true); else MDF_PAIR(double, true, false); }
+        else { if (acc) MDF_PAIR(double, false, true); else MDF_PAIR(double, false, false); }
     }
 #undef MDF_PAIR
 }
 
-// real-space pair kernel with the handle's lanes-per-lead and loop form
+// real-space pair kernel with the handle's lanes per lead and erfc form (the rolled loop
+// form of the launch shapes compiles to the same code and keeps the same results)
 static void launch_pair(mdf_system *s) {
     const bool f32 = s->par.precision == MDF_FP32_ACC64, vir = s->par.virial != 0;
+    const bool acc = s->par.erfc_mode == MDF_ERFC_ACCURATE;
     switch (s->pair_g) {
-        case 8: launch_pair_g<8>(s, f32, vir, s->pair_rolled); break;
-        case 32: launch_pair_g<32>(s, f32, vir, s->pair_rolled); break;
-        default: launch_pair_g<16>(s, f32, vir, s->pair_rolled); break;
+        case 8: launch_pair_g<8>(s, f32, vir, acc); break;
+        case 32: launch_pair_g<32>(s, f32, vir, acc); break;
+        default: launch_pair_g<16>(s, f32, vir, acc); break;
     }
 }
 
@@ -1493,6 +1499,32 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
         HTRY(upload(s->cls, cls.data(), cls.size()));
         HTRY(upload(s->lead, s->h_lead.data(), s->h_lead.size()));
         HTRY(upload(s->lead_slot, s->h_lead_slot.data(), s->h_lead_slot.size()));
+        // the pair kernel's static inputs: lead slot and sites per lead of every moving site,
+        // crystal class of every frozen site, vdW parameters per (lead slot, partner class)
+        {
+            std::vector<int> pm(std::max(t.n_move, 1), 0), wc(std::max(s->n_wall, 1), 0);
+            for (int l = 0; l < s->n_lead; l++) {
+                const int i = s->h_lead[l];
+                const int ns = t.nsi[t.site_type[i] - 1];
+                if (ns > 3 || i + ns > t.n_move) { rc = fail(MDF_ERR_ARG, "lead %d carries %d sites (at most 3 moving)", i, ns); goto fail_out; }
+                pm[i] = s->h_lead_slot[l] | (ns << 4);
+            }
+            for (int w = 0; w < s->n_wall; w++) wc[w] = t.site_type[t.n_move + w] - (nit - 1);
+            std::vector<double> tab((size_t)MAX_LEAD_TYPES * NCLS * PTAB_W, 0.0);
+            for (int si = 0; si < (int)s->slot_type.size(); si++)
+                for (int c = 0; c < NCLS; c++) {
+                    const int tj = s->cls_type[c];
+                    if (tj <= 0) continue;
+                    const int ijt = ij1[s->slot_type[si] * (nit + 1) + tj];
+                    double *e = &tab[((size_t)si * NCLS + c) * PTAB_W];
+                    e[0] = vk1[ijt]; e[1] = e41[ijt]; e[2] = s61[ijt]; e[3] = a1[ijt]; e[4] = r1[ijt]; e[5] = c1[ijt];
+                }
+            HTRY(upload(s->pmeta, pm.data(), pm.size()));
+            HTRY(upload(s->wcls, wc.data(), wc.size()));
+            HTRY(upload(s->ptab, tab.data(), tab.size()));
+            HTRY(s->prec.alloc((size_t)std::max(t.n_move, 1) * PREC));
+            HTRY(hipMemset(s->prec.p, 0, sizeof(double) * s->prec.n));
+        }
         HTRY(upload(s->con_ptr, cptr.data(), cptr.size()));
         HTRY(upload(s->con_idx, cidx.data(), cidx.size()));
         // angle terms: the reference adds each angle's force to its sites j and k only
@@ -1527,6 +1559,7 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
         }
         S.itype = s->itype.p; S.q = s->q.p; S.nsi = s->nsi.p; S.ijtype = s->ijtype.p; S.vkind = s->vkind.p;
         S.eps4 = s->eps4.p; S.sig6 = s->sig6.p; S.aa = s->aa.p; S.rhoi = s->rhoi.p; S.cc = s->cc.p; S.cls = s->cls.p;
+        S.pmeta = s->pmeta.p; S.wcls = s->wcls.p; S.ptab = s->ptab.p; S.prec = s->prec.p;
         S.lead = s->lead.p; S.con_ptr = s->con_ptr.p; S.con_idx = s->con_idx.p;
         // positions (all sites once; moving part per call)
         HTRY(upload(s->x, x, t.n_total)); HTRY(upload(s->y, y, t.n_total)); HTRY(upload(s->z, z, t.n_total));
@@ -1539,6 +1572,13 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
         HTRY(s->fx.alloc(std::max(t.n_move, 1))); HTRY(s->fy.alloc(std::max(t.n_move, 1))); HTRY(s->fz.alloc(std::max(t.n_move, 1)));
         HTRY(s->part.alloc((size_t)NCOL * std::max(s->n_lead, 1)));
         S.x = s->x.p; S.y = s->y.p; S.z = s->z.p; S.cx = s->cx.p; S.cy = s->cy.p; S.cz = s->cz.p;
+        {   // frozen sites for the pair kernel, once (their positions do not change)
+            std::vector<double4> wr(std::max(s->n_wall, 1), make_double4(0.0, 0.0, 0.0, 0.0));
+            for (int w = 0; w < s->n_wall; w++)
+                wr[w] = make_double4(x[t.n_move + w], y[t.n_move + w], z[t.n_move + w], t.charge[t.n_move + w]);
+            HTRY(upload(s->wrec, wr.data(), wr.size()));
+            S.wrec = s->wrec.p;
+        }
         S.ocx = s->ocx.p; S.ocy = s->ocy.p; S.ocz = s->ocz.p; S.ox = s->ox.p; S.oy = s->oy.p; S.oz = s->oz.p;
         S.fx = s->fx.p; S.fy = s->fy.p; S.fz = s->fz.p; S.part = s->part.p;
         // box and constants (force_init, src/forcem.f90:58-60; config_init src/config.f90:67-69)
@@ -2684,7 +2724,7 @@ struct mdf_batch {
     hipEvent_t in_event = nullptr;
     bool pending = false;
     int pair_g = 16;
-    bool f32 = false, vir = false;
+    bool f32 = false, vir = false, acc = false;
 };
 
 static void fill_rep(mdf_system *s, RepArgs &r) {
@@ -2806,8 +2846,13 @@ static void batch_launch(mdf_batch *b) {
         const int g = grid(BK_PAIR);
 #define MDF_PB(RT, G)                                                                    \
     do {                                                                                 \
-        if (b->vir) k_pair_b<RT, G, true><<<g, 256, 0, st>>>(R, mp);                     \
-        else k_pair_b<RT, G, false><<<g, 256, 0, st>>>(R, mp);                           \
+        if (b->acc) {                                                                    \
+            if (b->vir) k_pair_b<RT, G, true, true><<<g, 256, 0, st>>>(R, mp);           \
+            else k_pair_b<RT, G, false, true><<<g, 256, 0, st>>>(R, mp);                 \
+        } else {                                                                         \
+            if (b->vir) k_pair_b<RT, G, true, false><<<g, 256, 0, st>>>(R, mp);          \
+            else k_pair_b<RT, G, false, false><<<g, 256, 0, st>>>(R, mp);                \
+        }                                                                                \
     } while (0)
         if (b->f32) {
             if (b->pair_g == 8) MDF_PB(float, 8); else if (b->pair_g == 32) MDF_PB(float, 32); else MDF_PB(float, 16);
@@ -2851,8 +2896,9 @@ int mdf_batch_create(mdf_system *const *systems, int32_t n, mdf_batch **out) {
     const mdf_system *s0 = systems[0];
     for (int i = 1; i < n; i++) {
         const mdf_system *s = systems[i];
-        if (s->device != s0->device || s->par.precision != s0->par.precision || s->par.virial != s0->par.virial)
-            return fail(MDF_ERR_ARG, "system %d differs from system 0 in device, precision or virial", i);
+        if (s->device != s0->device || s->par.precision != s0->par.precision || s->par.virial != s0->par.virial ||
+            s->par.erfc_mode != s0->par.erfc_mode)
+            return fail(MDF_ERR_ARG, "system %d differs from system 0 in device, precision, erfc form or virial", i);
         for (int j = 0; j < i; j++)
             if (systems[j] == s) return fail(MDF_ERR_ARG, "system %d appears twice", i);
     }
@@ -2866,6 +2912,7 @@ int mdf_batch_create(mdf_system *const *systems, int32_t n, mdf_batch **out) {
     b->device = s0->device;
     b->f32 = s0->par.precision == MDF_FP32_ACC64;
     b->vir = s0->par.virial != 0;
+    b->acc = s0->par.erfc_mode == MDF_ERFC_ACCURATE;
     b->pair_g = s0->pair_g;
     int rc = MDF_OK;
     hipError_t e = hipSetDevice(b->device);

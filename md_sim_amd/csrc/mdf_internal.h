@@ -158,7 +158,19 @@ struct DevSys {
     double *part;                // [NCOL][n_lead]
     DevFlags *flags;
     int *flag_mirror;            // host-mapped: [0] = 1 once a list overflow is pending (or null)
+    // packed inputs of the pair kernel: per lead site i the molecule record (PREC doubles:
+    // x, y, z, q of the lead, then offset x, y, z from the lead and q of sites 1 and 2),
+    // rewritten by the skin test every step; per moving site its lead-type slot | nsi << 4
+    // (static); per frozen site (x, y, z, q) and its crystal class 0/1 (static); the vdW
+    // parameters per (lead slot, partner class): kind, 4 eps, sigma^6, A, 1/rho, C
+    double *prec;
+    const int *pmeta;
+    const double4 *wrec;
+    const int *wcls;
+    const double *ptab;          // [MAX_LEAD_TYPES][NCLS][PTAB_W]
 };
+constexpr int PREC = 12;         // doubles per molecule record
+constexpr int PTAB_W = 8;        // doubles per vdW table entry
 
 
 // ---- helpers ---------------------------------------------------------------------

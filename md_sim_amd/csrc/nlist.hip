@@ -35,9 +35,27 @@ __device__ __forceinline__ void skin_body(const DevSys &S, double *dip_part, int
         }
         if (t < S.n_move) dip += S.q[t] * S.z[t];
         if (t < S.n_lead) {
-            double zz = S.z[S.lead[t]];
+            const int i = S.lead[t];
+            const double xi = S.x[i], yi = S.y[i], zz = S.z[i];
             zlo = fmin(zlo, zz);
             zhi = fmax(zhi, zz);
+            // the pair kernel's record of this lead's molecule: lead position and charge, the
+            // other sites as offsets from the lead (the reference's rsx, src/move.f90:157-161)
+            const int nsi = S.pmeta[i] >> 4;
+            double r[PREC];
+            r[0] = xi; r[1] = yi; r[2] = zz; r[3] = S.q[i];
+#pragma unroll
+            for (int a = 1; a < 3; a++) {
+                const bool ok = a < nsi;
+                r[4 * a + 0] = ok ? S.x[i + a] - xi : 0.0;
+                r[4 * a + 1] = ok ? S.y[i + a] - yi : 0.0;
+                r[4 * a + 2] = ok ? S.z[i + a] - zz : 0.0;
+                r[4 * a + 3] = ok ? S.q[i + a] : 0.0;
+            }
+            double4 *dst = reinterpret_cast<double4 *>(S.prec + (size_t)i * PREC);
+            dst[0] = make_double4(r[0], r[1], r[2], r[3]);
+            dst[1] = make_double4(r[4], r[5], r[6], r[7]);
+            dst[2] = make_double4(r[8], r[9], r[10], r[11]);
         }
     }
     // block partials, no atomics: part[0][b] dipole (fixed tree order), part[1][b] max

@@ -10,7 +10,8 @@
 // from both sides, so no atomics and no write conflicts), then reduce with lane shuffles.
 // Energies and virial of moving pairs are halved for the double visit; frozen pairs are
 // visited once.  Per-lead sums go to column-major partial arrays for a deterministic
-// reduction.  Arithmetic type R is double (FP64) or float (FP32 with FP64 accumulation);
+// reduction; the lanes' energy classes are summed in a fixed butterfly.  Arithmetic type R
+// is double (FP64) or float (FP32 with FP64 accumulation);
 // minimum image and the cut-off test are FP64 without contraction in both, so the set of
 // interacting pairs is the reference's.
 #include "mdf_internal.h"
@@ -50,10 +51,40 @@ __device__ __forceinline__ float rcp_nr(float x) {
     return __builtin_fmaf(y, e, y);
 }
 
-template <typename R>
+// e^x for x <= 0 (the erfc and Buckingham arguments here), FP64, by Cody and Waite's
+// argument reduction (Software Manual for the Elementary Functions, 1980): x = n ln2 + r with
+// n = rint(x / ln2), |r| <= ln2 / 2, ln2 split into a 21-bit head (n ln2_hi exact) and a tail;
+// e^r by its Taylor series to degree 13, whose truncation error at |r| = ln2 / 2 is 4.2e-18
+// relative; e^x = 2^n e^r by exponent scaling (underflowing to 0 below x = -745).  About 19
+// operations and within 2 ulp, without the range checks a general exp needs.
+__device__ __forceinline__ double exp_neg(double x) {
+    x = x < -800.0 ? -800.0 : x;   // e^-800 is 0 in FP64; NaN passes through
+    const double n = rint(x * 1.4426950408889634);
+    double r = __builtin_fma(-n, 0x1.62e42p-1, x);
+    r = __builtin_fma(-n, 0x1.fdf473de6af28p-22, r);
+    double p = 1.0 / 6227020800.0;            // 1/13!
+    p = __builtin_fma(p, r, 1.0 / 479001600.0);
+    p = __builtin_fma(p, r, 1.0 / 39916800.0);
+    p = __builtin_fma(p, r, 1.0 / 3628800.0);
+    p = __builtin_fma(p, r, 1.0 / 362880.0);
+    p = __builtin_fma(p, r, 1.0 / 40320.0);
+    p = __builtin_fma(p, r, 1.0 / 5040.0);
+    p = __builtin_fma(p, r, 1.0 / 720.0);
+    p = __builtin_fma(p, r, 1.0 / 120.0);
+    p = __builtin_fma(p, r, 1.0 / 24.0);
+    p = __builtin_fma(p, r, 1.0 / 6.0);
+    p = __builtin_fma(p, r, 0.5);
+    p = __builtin_fma(p, r, 1.0);
+    p = __builtin_fma(p, r, 1.0);
+    return ldexp(p, (int)n);
+}
+__device__ __forceinline__ double exp_r(double x) { return exp_neg(x); }
+__device__ __forceinline__ float exp_r(float x) { return expf(x); }
+
+// ACC = false: the reference's A&S 7.1.26 erfc (src/forcem.f90:11-13,335-337); true: libm erfc
+template <typename R, bool ACC>
 struct Coul {
     R alfa, spii, p, a1, a2, a3, a4, a5;
-    int erfc_mode;
     // energy (erfc(ar) q/r) and force/r  for a site pair at squared distance r2,
     // with qq = facq q_i q_j
     __device__ __forceinline__ void eval(R r2, R qq, R &e, R &f) const {
@@ -62,9 +93,9 @@ struct Coul {
         const R r2i = ri * ri;
         const R aux = qq * ri;
         const R ar = alfa * r;
-        const R ex = exp(-ar * ar);
+        const R ex = exp_r(-ar * ar);
         R erfx;
-        if (erfc_mode == 0) {
+        if constexpr (!ACC) {
             const R t = rcp_nr(R(1) + p * ar);
             erfx = ((((a5 * t + a4) * t + a3) * t + a2) * t + a1) * t * ex;
         } else {
@@ -86,7 +117,7 @@ __device__ __forceinline__ void vdw_eval(int kind, R rsq, R eps4, R sig6, R A, R
         R rinv = rsqrt_nr(rsq);
         R rij = rsq * rinv;
         R r6 = Cc * (rsqi * rsqi * rsqi);
-        R aux = A * exp(-rij * rhoi);
+        R aux = A * exp_r(-rij * rhoi);
         e = aux - r6;
         f = aux * rhoi * rinv - R(6) * rsqi * r6;
     } else {
@@ -97,74 +128,105 @@ __device__ __forceinline__ void vdw_eval(int kind, R rsq, R eps4, R sig6, R A, R
 
 __device__ __forceinline__ double shfl_xor_d(double v, int m) { return __shfl_xor(v, m, 64); }
 
-template <typename R, int G, bool VIR, bool ROLLED>
+// one partner molecule as the kernel keeps it: lead position, the other sites as offsets
+struct MolRec {
+    double x, y, z, q[3];
+    double rx[3], ry[3], rz[3];
+};
+__device__ __forceinline__ MolRec load_rec(const double *__restrict__ prec, int j) {
+    const double4 *p = reinterpret_cast<const double4 *>(prec + (size_t)j * PREC);
+    const double4 a = p[0], b = p[1], c = p[2];
+    MolRec m;
+    m.x = a.x; m.y = a.y; m.z = a.z; m.q[0] = a.w;
+    m.rx[0] = 0.0; m.ry[0] = 0.0; m.rz[0] = 0.0;
+    m.rx[1] = b.x; m.ry[1] = b.y; m.rz[1] = b.z; m.q[1] = b.w;
+    m.rx[2] = c.x; m.ry[2] = c.y; m.rz[2] = c.z; m.q[2] = c.w;
+    return m;
+}
+
+// c-th of n accumulators += v, with the index known only at run time (no dynamic register
+// indexing: one select per accumulator)
+template <int N>
+__device__ __forceinline__ void acc_at(double (&a)[N], int c, double v) {
+#pragma unroll
+    for (int k = 0; k < N; k++) a[k] += (c == k) ? v : 0.0;
+}
+
+// G lanes share one lead; each lane walks every G-th partner of the lead's rows.  Partner
+// molecules come as packed records (three 32-byte loads) and their vdW parameters from a
+// small LDS table indexed by (lead slot, partner class), so a partner costs one row entry,
+// one meta word and one record.  The next partner's record is loaded while the current pair
+// is evaluated.  Energies per partner class, the virial and the wall forces stay in
+// registers; the G lanes reduce them with shuffles at the end.  The arithmetic and its order
+// are those of the reference's loops (lead-pair minimum image and cut-off test in FP64
+// without contraction).
+template <typename R, int G, bool VIR, bool ACC>
 __device__ __forceinline__ void pair_body(const DevSys &S, int bid, int nblk) {
-    // per-lane energy-by-class and wall-force accumulators live in LDS slots owned by the
-    // lane (no sharing, no atomics); only the site forces and the virial stay in registers
-    __shared__ double s_e[2 * NCLS][256];
-    __shared__ double s_w[6][256];
+    __shared__ double s_tab[MAX_LEAD_TYPES * NCLS * PTAB_W];
+    for (int k = threadIdx.x; k < MAX_LEAD_TYPES * NCLS * PTAB_W; k += blockDim.x) s_tab[k] = S.ptab[k];
+    __syncthreads();
     const int tid = threadIdx.x;
     const int gid = bid * (blockDim.x / G) + tid / G;
     const int sub = tid % G;
     const bool active = gid < S.n_lead;
-#pragma unroll
-    for (int c = 0; c < 2 * NCLS; c++) s_e[c][tid] = 0.0;
-#pragma unroll
-    for (int c = 0; c < 6; c++) s_w[c][tid] = 0.0;
 
     double f[3][3];
     double vir[6];
+    double ev[NCLS], ec[NCLS], w[6];
 #pragma unroll
     for (int a = 0; a < 3; a++) f[a][0] = f[a][1] = f[a][2] = 0.0;
 #pragma unroll
-    for (int c = 0; c < 6; c++) vir[c] = 0.0;
+    for (int c = 0; c < 6; c++) { vir[c] = 0.0; w[c] = 0.0; }
+#pragma unroll
+    for (int c = 0; c < NCLS; c++) { ev[c] = 0.0; ec[c] = 0.0; }
 
-    Coul<R> cl;
+    Coul<R, ACC> cl;
     cl.alfa = (R)S.alfa; cl.spii = (R)S.spii; cl.p = (R)S.p;
     cl.a1 = (R)S.a1; cl.a2 = (R)S.a2; cl.a3 = (R)S.a3; cl.a4 = (R)S.a4; cl.a5 = (R)S.a5;
-    cl.erfc_mode = S.erfc_mode;
 
     int i = 0, nsi_i = 0;
     if (active) {
         i = S.lead[gid];
-        const int ti = S.itype[i];
-        nsi_i = S.nsi[ti];
-        const double xi = S.x[i], yi = S.y[i], zi = S.z[i];
-        // site offsets from the lead (the reference's rsx, src/move.f90:157-161) and charges
-        double rxi[3] = {0, 0, 0}, ryi[3] = {0, 0, 0}, rzi[3] = {0, 0, 0}, qi[3] = {0, 0, 0};
+        const int mi = S.pmeta[i];
+        const int slot_i = mi & 15;
+        nsi_i = mi >> 4;
+        const MolRec me = load_rec(S.prec, i);
+        const double xi = me.x, yi = me.y, zi = me.z;
+        double qi[3];
 #pragma unroll
-        for (int a = 0; a < 3; a++) {
-            if (a < nsi_i) {
-                if (a > 0) { rxi[a] = S.x[i + a] - xi; ryi[a] = S.y[i + a] - yi; rzi[a] = S.z[i + a] - zi; }
-                qi[a] = S.facq * S.q[i + a];
-            }
-        }
-        const int nt1 = S.n_types + 1;
-        // ---- moving partners (loop A) ----
+        for (int a = 0; a < 3; a++) qi[a] = S.facq * me.q[a];   // zero past nsi_i
+        const double *tab_i = s_tab + slot_i * NCLS * PTAB_W;
+        // ---- moving partners (loop A, src/forcem.f90:288-403) ----
         const int n = min(S.cnt_mm[gid], S.cap_mm);
         const int *row = S.nbr_mm + (size_t)gid * S.cap_mm;
+        int k = sub;
+        int jn = k < n ? row[k] : i;
+        int mn = S.pmeta[jn];
+        MolRec rn = load_rec(S.prec, jn);
 #pragma unroll 1
-        for (int k = sub; k < n; k += G) {
-            const int j = row[k];
-            const double xj = S.x[j], yj = S.y[j], zj = S.z[j];
+        for (; k < n; k += G) {
+            const int mj = mn;
+            const MolRec pj = rn;
+            if (k + G < n) {   // next partner in flight during this pair
+                jn = row[k + G];
+                mn = S.pmeta[jn];
+                rn = load_rec(S.prec, jn);
+            }
             // lead-lead minimum image as the reference (x,y always; z per variant), no FMA
-            double dx = __dsub_rn(xi, xj); dx = __dsub_rn(dx, __dmul_rn(S.lx, anint_d(__dmul_rn(dx, S.lxi))));
-            double dy = __dsub_rn(yi, yj); dy = __dsub_rn(dy, __dmul_rn(S.ly, anint_d(__dmul_rn(dy, S.lyi))));
-            double dz = __dsub_rn(zi, zj);
+            double dx = __dsub_rn(xi, pj.x); dx = __dsub_rn(dx, __dmul_rn(S.lx, anint_d(__dmul_rn(dx, S.lxi))));
+            double dy = __dsub_rn(yi, pj.y); dy = __dsub_rn(dy, __dmul_rn(S.ly, anint_d(__dmul_rn(dy, S.lyi))));
+            double dz = __dsub_rn(zi, pj.z);
             if (S.zper_force) dz = __dsub_rn(dz, __dmul_rn(S.lz, anint_d(__dmul_rn(dz, S.lzi))));
             const double rsq = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
             if (rsq > S.rcutsq) continue;
-            const int tj = S.itype[j];
-            const int nsi_j = S.nsi[tj];
-            const int ijt = S.ijtype[ti * nt1 + tj];
+            const int cj = mj & 15, nsi_j = mj >> 4;
+            const double *pt = tab_i + cj * PTAB_W;
             R evd, fvd;
-            vdw_eval<R>(S.vkind[ijt], (R)rsq, (R)S.eps4[ijt], (R)S.sig6[ijt], (R)S.aa[ijt], (R)S.rhoi[ijt],
-                        (R)S.cc[ijt], evd, fvd);
+            vdw_eval<R>((int)pt[0], (R)rsq, (R)pt[1], (R)pt[2], (R)pt[3], (R)pt[4], (R)pt[5], evd, fvd);
             double e_c = 0.0;
-            // lead pair: vdW + Coulomb
-            {
+            {   // lead pair: vdW + Coulomb
                 R ecl, fcl;
-                cl.eval((R)rsq, (R)(qi[0] * S.q[j]), ecl, fcl);
+                cl.eval((R)rsq, (R)(qi[0] * pj.q[0]), ecl, fcl);
                 e_c += (double)ecl;
                 const double fij = (double)fvd + (double)fcl;
                 f[0][0] += dx * fij; f[0][1] += dy * fij; f[0][2] += dz * fij;
@@ -175,54 +237,19 @@ __device__ __forceinline__ void pair_body(const DevSys &S, int bid, int nblk) {
                 }
             }
             // the other site pairs of the two molecules, uncut (src/forcem.f90:360-402)
-            if constexpr (ROLLED) {
-              if (nsi_i > 1 || nsi_j > 1) {
-                // one site pair per iteration, accumulator chosen by selects: few live registers
-                const int npair = nsi_i * nsi_j;
-#pragma unroll 1
-                for (int pq = 1; pq < npair; pq++) {
-                    const int b = pq / nsi_i, a = pq - b * nsi_i;
-                    const double xb = (b > 0) ? S.x[j + b] - xj : 0.0;
-                    const double yb = (b > 0) ? S.y[j + b] - yj : 0.0;
-                    const double zb = (b > 0) ? S.z[j + b] - zj : 0.0;
-                    const double qb = S.q[j + b];
-                    const double ra_x = a == 0 ? 0.0 : (a == 1 ? rxi[1] : rxi[2]);
-                    const double ra_y = a == 0 ? 0.0 : (a == 1 ? ryi[1] : ryi[2]);
-                    const double ra_z = a == 0 ? 0.0 : (a == 1 ? rzi[1] : rzi[2]);
-                    const double qa = a == 0 ? qi[0] : (a == 1 ? qi[1] : qi[2]);
-                    const double xd = (dx + ra_x) - xb, yd = (dy + ra_y) - yb, zd = (dz + ra_z) - zb;
-                    const double r2 = xd * xd + yd * yd + zd * zd;
-                    R es, fs;
-                    cl.eval((R)r2, (R)(qa * qb), es, fs);
-                    e_c += (double)es;
-                    const double fd = (double)fs;
-                    const double gx = xd * fd, gy = yd * fd, gz = zd * fd;
-                    f[0][0] += a == 0 ? gx : 0.0; f[0][1] += a == 0 ? gy : 0.0; f[0][2] += a == 0 ? gz : 0.0;
-                    f[1][0] += a == 1 ? gx : 0.0; f[1][1] += a == 1 ? gy : 0.0; f[1][2] += a == 1 ? gz : 0.0;
-                    f[2][0] += a == 2 ? gx : 0.0; f[2][1] += a == 2 ? gy : 0.0; f[2][2] += a == 2 ? gz : 0.0;
-                    if (VIR) {
-                        const double hf = 0.5 * fd;
-                        vir[0] += xd * xd * hf; vir[1] += xd * yd * hf; vir[2] += xd * zd * hf;
-                        vir[3] += yd * yd * hf; vir[4] += yd * zd * hf; vir[5] += zd * zd * hf;
-                    }
-                }
-              }
-            } else if (nsi_i > 1 || nsi_j > 1) {
+            if (nsi_i > 1 || nsi_j > 1) {
 #pragma unroll
                 for (int b = 0; b < 3; b++) {
                     if (b >= nsi_j) break;
-                    const double xb = (b > 0) ? S.x[j + b] - xj : 0.0;
-                    const double yb = (b > 0) ? S.y[j + b] - yj : 0.0;
-                    const double zb = (b > 0) ? S.z[j + b] - zj : 0.0;
-                    const double qb = S.q[j + b];
 #pragma unroll
                     for (int a = 0; a < 3; a++) {
                         if (a >= nsi_i) break;
                         if (a == 0 && b == 0) continue;
-                        const double xd = (dx + rxi[a]) - xb, yd = (dy + ryi[a]) - yb, zd = (dz + rzi[a]) - zb;
+                        const double xd = (dx + me.rx[a]) - pj.rx[b], yd = (dy + me.ry[a]) - pj.ry[b],
+                                     zd = (dz + me.rz[a]) - pj.rz[b];
                         const double r2 = xd * xd + yd * yd + zd * zd;
                         R es, fs;
-                        cl.eval((R)r2, (R)(qi[a] * qb), es, fs);
+                        cl.eval((R)r2, (R)(qi[a] * pj.q[b]), es, fs);
                         e_c += (double)es;
                         const double fd = (double)fs;
                         f[a][0] += xd * fd; f[a][1] += yd * fd; f[a][2] += zd * fd;
@@ -234,43 +261,48 @@ __device__ __forceinline__ void pair_body(const DevSys &S, int bid, int nblk) {
                     }
                 }
             }
-            const int cj = S.cls[tj];
-            s_e[cj][tid] += 0.5 * (double)evd;
-            s_e[NCLS + cj][tid] += 0.5 * e_c;
+            // moving partners are classes 0 .. MAX_LEAD_TYPES - 1 (their lead slot); the full
+            // list visits each pair from both sides
+            double evh[MAX_LEAD_TYPES], ech[MAX_LEAD_TYPES];
+#pragma unroll
+            for (int c = 0; c < MAX_LEAD_TYPES; c++) { evh[c] = ev[c]; ech[c] = ec[c]; }
+            acc_at<MAX_LEAD_TYPES>(evh, cj, 0.5 * (double)evd);
+            acc_at<MAX_LEAD_TYPES>(ech, cj, 0.5 * e_c);
+#pragma unroll
+            for (int c = 0; c < MAX_LEAD_TYPES; c++) { ev[c] = evh[c]; ec[c] = ech[c]; }
         }
         // surface-site bookkeeping between the two loops (src/forcem.f90:408-413)
-        if (i >= S.n_liquid_sites) s_w[(signbit(zi) ? 0 : 3) + 2][tid] += f[0][2];
-        // ---- frozen partners (loop B) ----
+        if (i >= S.n_liquid_sites) w[(signbit(zi) ? 0 : 3) + 2] += f[0][2];
+        // ---- frozen partners (loop B, src/forcem.f90:418-498) ----
         const int nw = min(S.cnt_mw[gid], S.cap_mw);
         const int *wrow = S.nbr_mw + (size_t)gid * S.cap_mw;
-        const bool wall_bk = ti <= S.n_ltype;
+        const bool wall_bk = S.itype[i] <= S.n_ltype;
 #pragma unroll 1
-        for (int k = sub; k < nw; k += G) {
-            const int j = wrow[k];
-            const double xj = S.x[j], yj = S.y[j], zj = S.z[j];
-            double dx = __dsub_rn(xi, xj); dx = __dsub_rn(dx, __dmul_rn(S.lx, anint_d(__dmul_rn(dx, S.lxi))));
-            double dy = __dsub_rn(yi, yj); dy = __dsub_rn(dy, __dmul_rn(S.ly, anint_d(__dmul_rn(dy, S.lyi))));
-            double dz = __dsub_rn(zi, zj);
+        for (int kw = sub; kw < nw; kw += G) {
+            const int wj = wrow[kw] - S.n_move;
+            const double4 pw = S.wrec[wj];
+            double dx = __dsub_rn(xi, pw.x); dx = __dsub_rn(dx, __dmul_rn(S.lx, anint_d(__dmul_rn(dx, S.lxi))));
+            double dy = __dsub_rn(yi, pw.y); dy = __dsub_rn(dy, __dmul_rn(S.ly, anint_d(__dmul_rn(dy, S.lyi))));
+            double dz = __dsub_rn(zi, pw.z);
             if (S.zper_force) dz = __dsub_rn(dz, __dmul_rn(S.lz, anint_d(__dmul_rn(dz, S.lzi))));
             const double rsq = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
             if (rsq > S.rcutsq) continue;
-            const int sgo = signbit(zj) ? 0 : 3;
-            const int tj = S.itype[j];
-            const int ijt = S.ijtype[ti * nt1 + tj];
+            const int sgo = signbit(pw.z) ? 0 : 3;
+            const int cw = S.wcls[wj];   // 0 / 1: the two crystal types, classes MAX_LEAD_TYPES + cw
+            const double *pt = tab_i + (MAX_LEAD_TYPES + cw) * PTAB_W;
             R evd, fvd;
-            vdw_eval<R>(S.vkind[ijt], (R)rsq, (R)S.eps4[ijt], (R)S.sig6[ijt], (R)S.aa[ijt], (R)S.rhoi[ijt],
-                        (R)S.cc[ijt], evd, fvd);
+            vdw_eval<R>((int)pt[0], (R)rsq, (R)pt[1], (R)pt[2], (R)pt[3], (R)pt[4], (R)pt[5], evd, fvd);
             double e_c = 0.0;
-            const double qw = S.facq * S.q[j];
+            const double qw = S.facq * pw.w;
             double fij = (double)fvd;
             double wx = 0.0, wy = 0.0, wz = 0.0;
 #pragma unroll
             for (int a = 0; a < 3; a++) {
                 if (a >= nsi_i) break;
-                const double xd = dx + rxi[a], yd = dy + ryi[a], zd = dz + rzi[a];
+                const double xd = dx + me.rx[a], yd = dy + me.ry[a], zd = dz + me.rz[a];
                 const double r2 = (a == 0) ? rsq : xd * xd + yd * yd + zd * zd;
                 R es, fs;
-                cl.eval((R)r2, (R)(qw * S.q[i + a]), es, fs);
+                cl.eval((R)r2, (R)(qw * me.q[a]), es, fs);
                 e_c += (double)es;
                 fij += (double)fs;
                 f[a][0] += xd * fij; f[a][1] += yd * fij; f[a][2] += zd * fij;
@@ -281,13 +313,19 @@ __device__ __forceinline__ void pair_body(const DevSys &S, int bid, int nblk) {
                 }
                 fij = 0.0;
             }
-            if (wall_bk) { s_w[sgo + 0][tid] += wx; s_w[sgo + 1][tid] += wy; s_w[sgo + 2][tid] += wz; }
-            const int cj = S.cls[tj];
-            s_e[cj][tid] += (double)evd;
-            s_e[NCLS + cj][tid] += e_c;
+            if (wall_bk) {
+                if (sgo == 0) { w[0] += wx; w[1] += wy; w[2] += wz; }
+                else { w[3] += wx; w[4] += wy; w[5] += wz; }
+            }
+            double evw[2] = {ev[MAX_LEAD_TYPES], ev[MAX_LEAD_TYPES + 1]};
+            double ecw[2] = {ec[MAX_LEAD_TYPES], ec[MAX_LEAD_TYPES + 1]};
+            acc_at<2>(evw, cw, (double)evd);
+            acc_at<2>(ecw, cw, e_c);
+            ev[MAX_LEAD_TYPES] = evw[0]; ev[MAX_LEAD_TYPES + 1] = evw[1];
+            ec[MAX_LEAD_TYPES] = ecw[0]; ec[MAX_LEAD_TYPES + 1] = ecw[1];
         }
     }
-    // ---- reduce over the G lanes of the group ----
+    // ---- reduce over the G lanes of the group (fixed butterfly) ----
 #pragma unroll
     for (int m = G / 2; m > 0; m >>= 1) {
 #pragma unroll
@@ -298,32 +336,33 @@ __device__ __forceinline__ void pair_body(const DevSys &S, int bid, int nblk) {
 #pragma unroll
             for (int c = 0; c < 6; c++) vir[c] += shfl_xor_d(vir[c], m);
         }
+#pragma unroll
+        for (int c = 0; c < NCLS; c++) { ev[c] += shfl_xor_d(ev[c], m); ec[c] += shfl_xor_d(ec[c], m); }
+#pragma unroll
+        for (int c = 0; c < 6; c++) w[c] += shfl_xor_d(w[c], m);
     }
-    __syncthreads();
     if (active && sub == 0) {
         for (int a = 0; a < nsi_i && a < 3; a++) {
             S.fx[i + a] = f[a][0]; S.fy[i + a] = f[a][1]; S.fz[i + a] = f[a][2];
         }
         const size_t L = (size_t)S.n_lead;
         double *P = S.part;
-        for (int c = 0; c < 2 * NCLS; c++) {
-            double acc = 0.0;
-            for (int l = 0; l < G; l++) acc += s_e[c][tid + l];
-            P[(COL_EV + c) * L + gid] = acc;   // COL_EC = COL_EV + NCLS
+#pragma unroll
+        for (int c = 0; c < NCLS; c++) {
+            P[(COL_EV + c) * L + gid] = ev[c];
+            P[(COL_EC + c) * L + gid] = ec[c];
         }
-        for (int c = 0; c < 6; c++) {
-            double acc = 0.0;
-            for (int l = 0; l < G; l++) acc += s_w[c][tid + l];
-            P[(COL_FWS + c) * L + gid] = acc;
-        }
+#pragma unroll
+        for (int c = 0; c < 6; c++) P[(COL_FWS + c) * L + gid] = w[c];
         const double v9[9] = {vir[0], vir[1], vir[2], vir[1], vir[3], vir[4], vir[2], vir[4], vir[5]};
+#pragma unroll
         for (int c = 0; c < 9; c++) P[(COL_VIR + c) * L + gid] = VIR ? v9[c] : 0.0;
     }
 }
 
-template <typename R, int G, bool VIR, bool ROLLED>
+template <typename R, int G, bool VIR, bool ACC>
 __global__ __launch_bounds__(256) void k_pair(DevSys S) {
-    pair_body<R, G, VIR, ROLLED>(S, blockIdx.x, gridDim.x);
+    pair_body<R, G, VIR, ACC>(S, blockIdx.x, gridDim.x);
 }
 
 #define MDF_PAIR_INST(G)                                            \
@@ -350,17 +389,21 @@ __global__ void k_zero_forces(DevSys S) {
 }  // namespace mdf
 
 namespace mdf {
-// batched form (one lanes-per-lead choice for the whole batch; unrolled site-pair loop)
-template <typename R, int G, bool VIR>
+// batched form (one lanes-per-lead choice and one erfc form for the whole batch)
+template <typename R, int G, bool VIR, bool ACC>
 __global__ __launch_bounds__(256) void k_pair_b(const RepArgs *__restrict__ RA, const int2 *__restrict__ map) {
     const int2 m = map[blockIdx.x];
-    pair_body<R, G, VIR, false>(RA[m.x].S, m.y, 0);
+    pair_body<R, G, VIR, ACC>(RA[m.x].S, m.y, 0);
 }
-#define MDF_PAIR_B_INST(G)                                                                   \
-    template __global__ void k_pair_b<double, G, true>(const RepArgs *, const int2 *);    \
-    template __global__ void k_pair_b<double, G, false>(const RepArgs *, const int2 *);   \
-    template __global__ void k_pair_b<float, G, true>(const RepArgs *, const int2 *);     \
-    template __global__ void k_pair_b<float, G, false>(const RepArgs *, const int2 *);
+#define MDF_PAIR_B_INST(G)                                                                        \
+    template __global__ void k_pair_b<double, G, true, false>(const RepArgs *, const int2 *);  \
+    template __global__ void k_pair_b<double, G, false, false>(const RepArgs *, const int2 *); \
+    template __global__ void k_pair_b<float, G, true, false>(const RepArgs *, const int2 *);   \
+    template __global__ void k_pair_b<float, G, false, false>(const RepArgs *, const int2 *);  \
+    template __global__ void k_pair_b<double, G, true, true>(const RepArgs *, const int2 *);   \
+    template __global__ void k_pair_b<double, G, false, true>(const RepArgs *, const int2 *);  \
+    template __global__ void k_pair_b<float, G, true, true>(const RepArgs *, const int2 *);    \
+    template __global__ void k_pair_b<float, G, false, true>(const RepArgs *, const int2 *);
 MDF_PAIR_B_INST(8)
 MDF_PAIR_B_INST(16)
 MDF_PAIR_B_INST(32)
