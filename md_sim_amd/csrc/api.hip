@@ -231,9 +231,11 @@ struct DBuf {
     }
 };
 
-static const char *STAGE_NAMES[] = {"list", "pair", "surrogate", "sf", "sf_reduce", "backproj", "epilogue", "reduce",
-                                    "step"};
-constexpr int N_STAGES = 8;  // timed stages; entry N_STAGES of the names is the whole step
+// list: skin test and rebuild decision; build: cell binning and list build (early exits
+// on the steps that keep the list)
+static const char *STAGE_NAMES[] = {"list", "build", "pair", "surrogate", "sf", "sf_reduce", "backproj", "epilogue",
+                                    "reduce", "step"};
+constexpr int N_STAGES = 9;  // timed stages; entry N_STAGES of the names is the whole step
 
 struct mdf_system {
     mdf_topology top;
@@ -320,7 +322,8 @@ struct mdf_system {
     long long step_steps = 0;        // steps with start/end events (any mode)
     bool slot_staged[NRING] = {};
     float stage_ms[N_STAGES] = {};   // last completed step
-    bool stage_timing = false;       // per-stage events: direct launches instead of graphs
+    int stage_timing = 0;            // per-stage events: direct launches instead of graphs;
+                                     // 2: and a list build on every step
     long long timing_failures = 0;
     bool pending = false;
     int last_rebuilt = 0;
@@ -761,6 +764,7 @@ static void launch_lists(mdf_system *s, int force_rebuild) {
     hipStream_t st = s->stream;
     k_skin<<<s->skin_blocks, 256, 0, st>>>(S, s->dip_part.p);
     k_decide<<<1, 256, 0, st>>>(S, s->dip_part.p, s->skin_blocks, s->dipz.p, force_rebuild);
+    if (s->ev) (void)hipEventRecord(s->ev[1], st);
     int lb = std::max(1, (s->n_lead + 255) / 256);
     k_cell_count<<<lb, 256, 0, st>>>(S, s->lead_cell.p);
     k_cell_scan_build<<<1, 1024, 0, st>>>(S);
@@ -971,10 +975,10 @@ static void launch_forces(mdf_system *s) {
     const RecipArgs ra = recip_args(s);
     const bool mts = s->mts_on;
     const MtsArgs ma = mts ? mts_args(s) : MtsArgs{};
-    if (s->ev) (void)hipEventRecord(s->ev[1], st);
+    if (s->ev) (void)hipEventRecord(s->ev[2], st);
     if (mts) k_sg_control<<<1, 64, 0, st>>>(ma, s->mts_again);
     launch_pair(s);
-    if (s->ev) (void)hipEventRecord(s->ev[2], st);
+    if (s->ev) (void)hipEventRecord(s->ev[3], st);
     if (mts) {
         if (s->mts_model) {
             k_sg_pack<<<std::max(1, (s->top.n_total + 255) / 256), 256, 0, st>>>(S, ma);
@@ -985,10 +989,10 @@ static void launch_forces(mdf_system *s) {
         }
         k_sg_combine<<<s->sg_n_sblk, 256, 0, st>>>(S, ma);
     }
-    if (s->ev) (void)hipEventRecord(s->ev[3], st);
+    if (s->ev) (void)hipEventRecord(s->ev[4], st);
     int nk = (int)s->ak.size();
     launch_sf(s);
-    if (s->ev) (void)hipEventRecord(s->ev[4], st);
+    if (s->ev) (void)hipEventRecord(s->ev[5], st);
     if (nk > 0) {
         double lamb = 1.0 / (4.0 * S.alfa * S.alfa);
         const int n_sb = nm > 0 ? (s->sf_mma ? s->n_sb_mma : s->n_sb) : 0;
@@ -1009,9 +1013,9 @@ static void launch_forces(mdf_system *s) {
                                                            s->d_t.p, s->kpart.p, mts_state(s), nullptr, nullptr, 0, 0,
                                                            p32);
     }
-    if (s->ev) (void)hipEventRecord(s->ev[5], st);
-    launch_bp(s);
     if (s->ev) (void)hipEventRecord(s->ev[6], st);
+    launch_bp(s);
+    if (s->ev) (void)hipEventRecord(s->ev[7], st);
     const int n_groups = (nk > 0 && nm > 0) ? (s->mma64 ? s->b64_nrg : s->bp_mma ? 1 : s->n_rg) : 0;
     k_epilogue<<<s->n_epblk, EPI_BLOCK, 0, st>>>(S, s->fpart.p, n_groups, s->d_wl.p, s->par.ew3dc, s->crf0,
                                                  s->dipz.p, s->eppart.p, s->fcom.p, s->torque.p, mts_state(s),
@@ -1020,7 +1024,7 @@ static void launch_forces(mdf_system *s) {
         k_obs_ngc<<<std::max(1, (s->n_lead + 15) / 16), 256, 0, st>>>(S, s->obs);
         k_obs_efipos<<<1, 64, 0, st>>>(S, s->obs);
     }
-    if (s->ev) (void)hipEventRecord(s->ev[7], st);
+    if (s->ev) (void)hipEventRecord(s->ev[8], st);
     {
         dim3 gc(NCOL, s->n_colblk);
         const FinalArgs A = final_args(s);
@@ -1044,7 +1048,7 @@ static void launch_forces(mdf_system *s) {
         k_sg_delta<<<s->sg_n_sblk, 256, 0, st>>>(S, ma);
         k_sg_post<<<1, 64, 0, st>>>(S, ma);
     }
-    if (s->ev) (void)hipEventRecord(s->ev[8], st);
+    if (s->ev) (void)hipEventRecord(s->ev[9], st);
 }
 
 static void drain_slot(mdf_system *s, int slot) {
@@ -1242,7 +1246,7 @@ static int run_step(mdf_system *s, int force_rebuild) {
     } else {
         s->slot_staged[slot] = true;
         (void)hipEventRecord(ev[0], s->stream);
-        launch_lists(s, force_rebuild);
+        launch_lists(s, force_rebuild || s->stage_timing == 2);
         launch_forces(s);
     }
     HIPCHK(hipGetLastError());
@@ -2082,8 +2086,9 @@ int mdf_set_stage_timing(mdf_system *s, int32_t on) {
     if (!s) return fail(MDF_ERR_ARG, "null handle");
     HIPCHK(hipSetDevice(s->device));
     HIPCHK(hipStreamSynchronize(s->stream));
+    if (on < 0 || on > 2) return fail(MDF_ERR_ARG, "stage timing mode %d (0 off, 1 on, 2 on with a list build every step)", on);
     drain_all(s);
-    s->stage_timing = on != 0;
+    s->stage_timing = on;
     return MDF_OK;
 }
 

@@ -156,7 +156,7 @@ class ForceEngine:
         return kv.reshape(-1, 3), ak, sf[0:2 * n.value:2] + 1j * sf[1:2 * n.value:2]
 
     def reciprocal_on_matrix_cores(self) -> bool:
-        """True when both reciprocal-sum kernels run on the matrix cores (the FP32 form)."""
+        """True when both reciprocal-sum kernels run on the matrix cores (FP64 or FP16 hi/lo MFMA)."""
         a, b = C.c_int32(), C.c_int32()
         _check(lib().mdf_reciprocal_kernels(self._h, C.byref(a), C.byref(b)))
         return bool(a.value and b.value)
@@ -266,8 +266,10 @@ class ForceEngine:
         _check(lib().mdf_set_launch_shapes(self._h, int(pair_lanes), int(bool(pair_rolled)), int(sf_blocks),
                                            int(bp_blocks)))
 
-    def set_stage_timing(self, on: bool):
-        _check(lib().mdf_set_stage_timing(self._h, int(bool(on))))
+    def set_stage_timing(self, on, rebuild_every_step: bool = False):
+        """Per-stage HIP events on (direct launches); rebuild_every_step also builds the
+        pair list on every step, so that the "build" stage times full builds."""
+        _check(lib().mdf_set_stage_timing(self._h, (2 if rebuild_every_step else 1) if on else 0))
 
     def reset_stage_times(self):
         _check(lib().mdf_reset_stage_times(self._h))

@@ -11,6 +11,9 @@ FETCH_SIZE / WRITE_SIZE to the true size scales the counters of the engine's ker
 librating water trajectory: ``--mts-train`` full-sum training steps, then ``--steps`` steps
 of the scheme.
 
+``--rebuild-every-step`` builds the pair list on every step (per-stage timing mode 2), so
+that every dispatch of the list-build kernels is a full build.
+
 ``--chempot "N WMIN WMAX BINS MID"`` adds the Widom insertions after every step (tot.inp's
 ichmax, lwmin, lwmax, lchmax, ilmid).
 """
@@ -30,6 +33,7 @@ def main(argv=None):
     ap.add_argument("--mts", type=int, default=0, help="full k-sum every N-th step with the surrogate between")
     ap.add_argument("--mts-model", choices=["network", "held"], default="network")
     ap.add_argument("--mts-train", type=int, default=8)
+    ap.add_argument("--rebuild-every-step", action="store_true")
     ap.add_argument("--chempot", default="", help="Widom insertions: 'ichmax lwmin lwmax lchmax ilmid'")
     a = ap.parse_args(argv)
     os.environ["MDF_GRAPHS"] = "0"
@@ -42,6 +46,8 @@ def main(argv=None):
     prec = abi.FP32_ACC64 if a.precision == "fp32" else abi.FP64
     pos = abi.positions(s)
     eng = ForceEngine(s, precision=prec, virial=False)
+    if a.rebuild_every_step:
+        eng.set_stage_timing(True, rebuild_every_step=True)
     if a.mts > 1:
         frames = systems.libration_frames(s, a.mts_train + a.steps, seed=7)
         eng.force_get(*frames[0])

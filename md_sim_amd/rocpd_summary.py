@@ -1,0 +1,128 @@
+"""Summaries of rocprofv3 databases (the rocpd SQLite files its runs write).
+
+``python -m md_sim_amd.rocpd_summary kernels DB [--match REGEX] [--top N]``
+    per kernel: calls, mean / min / max duration and total time, as a markdown table;
+
+``python -m md_sim_amd.rocpd_summary traffic --fetch DB --write DB [--copy-mib 256]
+[--match REGEX] [--json OUT [--key KEY]]``
+    HBM bytes per dispatch of each kernel from a ``--pmc FETCH_SIZE`` run and a
+    ``--pmc WRITE_SIZE`` run (KiB per dispatch, summed over the counter's hardware instances,
+    averaged over the kernel's dispatches).  The device-to-device copy that
+    ``md_sim_amd.profile_step`` runs last moves a known number of bytes, so each counter is
+    scaled by (true bytes) / (what the counter reads for that copy).  --json writes the
+    figures to OUT, under KEY when given (other keys of an existing OUT are kept).
+"""
+from __future__ import annotations
+
+import argparse
+import collections
+import json
+import re
+import sqlite3
+import sys
+
+COPY_KERNEL = "__amd_rocclr_copyBuffer"
+
+
+def kernel_durations(db):
+    """name -> list of durations (ns), one per dispatch."""
+    out = collections.defaultdict(list)
+    with sqlite3.connect(db) as c:
+        for name, dur in c.execute("select name, duration from kernels"):
+            out[name].append(dur)
+    return out
+
+
+def counter_per_dispatch(db, counter):
+    """name -> list of the counter's value per dispatch (hardware instances summed)."""
+    per = collections.defaultdict(float)
+    names = {}
+    with sqlite3.connect(db) as c:
+        for name, disp, cn, cv in c.execute(
+                "select name, dispatch_id, counter_name, counter_value from pmc_events where counter_name = ?",
+                (counter,)):
+            per[disp] += cv
+            names[disp] = name
+    out = collections.defaultdict(list)
+    for disp, v in per.items():
+        out[names[disp]].append(v)
+    return out
+
+
+def short(name, n=90):
+    return name if len(name) <= n else name[:n - 3] + "..."
+
+
+def cmd_kernels(a):
+    d = kernel_durations(a.db)
+    total = sum(sum(v) for v in d.values()) or 1.0
+    rows = sorted(d.items(), key=lambda kv: -sum(kv[1]))
+    if a.match:
+        rows = [r for r in rows if re.search(a.match, r[0])]
+    print("| kernel | calls | mean (us) | min (us) | max (us) | total (ms) | share |")
+    print("|---|---:|---:|---:|---:|---:|---:|")
+    for name, v in rows[:a.top]:
+        print("| `%s` | %d | %.1f | %.1f | %.1f | %.2f | %.1f%% |" % (
+            short(name), len(v), sum(v) / len(v) / 1e3, min(v) / 1e3, max(v) / 1e3, sum(v) / 1e6,
+            100.0 * sum(v) / total))
+
+
+def cmd_traffic(a):
+    true_kib = a.copy_mib * 1024.0
+    f = counter_per_dispatch(a.fetch, "FETCH_SIZE")
+    w = counter_per_dispatch(a.write, "WRITE_SIZE")
+    if COPY_KERNEL not in f or COPY_KERNEL not in w:
+        sys.exit("the calibration copy (%s) is missing from a database" % COPY_KERNEL)
+    # the calibration copy is the last (largest) one
+    fs = true_kib / max(f[COPY_KERNEL])
+    ws = true_kib / max(w[COPY_KERNEL])
+    out = {"calibration": {"copy_mib": a.copy_mib, "fetch_kib_read": max(f[COPY_KERNEL]),
+                           "write_kib_read": max(w[COPY_KERNEL]), "fetch_scale": fs, "write_scale": ws},
+           "kernels": {}}
+    print("Calibration: the %d MiB copy reads %.0f KiB by FETCH_SIZE and %.0f KiB by WRITE_SIZE: scales %.3f and "
+          "%.3f.\n" % (a.copy_mib, max(f[COPY_KERNEL]), max(w[COPY_KERNEL]), fs, ws))
+    print("| kernel | dispatches | fetch (MB) | write (MB) |")
+    print("|---|---:|---:|---:|")
+    for name in sorted(set(f) | set(w)):
+        if name == COPY_KERNEL or (a.match and not re.search(a.match, name)):
+            continue
+        fv, wv = f.get(name, [0.0]), w.get(name, [0.0])
+        fb = sum(fv) / len(fv) * fs * 1024.0
+        wb = sum(wv) / len(wv) * ws * 1024.0
+        out["kernels"][name] = {"dispatches": len(fv), "fetch_bytes": fb, "write_bytes": wb,
+                                "bytes": fb + wb}
+        print("| `%s` | %d | %.3f | %.3f |" % (short(name), len(fv), fb / 1e6, wb / 1e6))
+    if a.json:
+        doc = out
+        if a.key:
+            try:
+                with open(a.json) as fh:
+                    doc = json.load(fh)
+            except (OSError, ValueError):
+                doc = {}
+            doc[a.key] = out
+        with open(a.json, "w") as fh:
+            json.dump(doc, fh, indent=1)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    k = sub.add_parser("kernels")
+    k.add_argument("db")
+    k.add_argument("--match", default="")
+    k.add_argument("--top", type=int, default=30)
+    t = sub.add_parser("traffic")
+    t.add_argument("--fetch", required=True)
+    t.add_argument("--write", required=True)
+    t.add_argument("--copy-mib", type=int, default=256)
+    t.add_argument("--match", default="")
+    t.add_argument("--json", default="")
+    t.add_argument("--key", default="")
+    a = ap.parse_args(argv)
+    {"kernels": cmd_kernels, "traffic": cmd_traffic}[a.cmd](a)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

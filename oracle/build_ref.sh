@@ -3,7 +3,8 @@
 # development container only.  Outputs go to oracle/_ref/ (never committed).
 #   slab: src/forcem.f90      (open z, slab correction, wall tail correction)
 #   bulk: src/forcem_back.f90 (3-D periodic pair loops, no slab/wall terms)
-# No fast-math and no FMA contraction, so the vectors are the plain FP64 path.
+# No fast-math and no FMA contraction, so the vectors are the plain FP64 path; the _fast
+# timing builds at the end use the reference's own flags.
 set -euo pipefail
 REF=${MD_SIM_REF_SRC:-/root/reference/src}
 HERE=$(cd "$(dirname "$0")" && pwd)
@@ -64,4 +65,21 @@ if [ -f "$LIBDIR/libmdforce.so" ]; then
     $FC -o "$OUT/md_nve_gpu" "$D/md_nve.o" "${objs[@]}" -L"$LIBDIR" -lmdforce -Wl,-rpath,'$ORIGIN/../../md_sim_amd' \
         -L"$MPI_LIB" -lmpifort -lmpi -Wl,-rpath,"$MPI_LIB"
 fi
-echo "built $OUT/ref_slab $OUT/ref_bulk $OUT/md_nve_slab $OUT/md_nve_bulk $OUT/md_nve_oracle $OUT/md_nve_gpu"
+# timing builds for the CPU baseline: the same harness with the reference's own compiler
+# flags (src/Makefile:2, -O2 -ffast-math), ref_slab_fast / ref_bulk_fast
+FAST="-O2 -ffast-math -w -I$MPI_INC"
+for variant in slab bulk; do
+    if [ $variant = slab ]; then FORCE=forcem.f90; else FORCE=forcem_back.f90; fi
+    D="$OUT/${variant}_fast"
+    mkdir -p "$D"
+    objs=()
+    for f in global.f90 comm.f90 rndmod.f90 sysdef.f90 config.f90 nnlist.f90 measure.f90 $FORCE move.f90; do
+        o="$D/${f%.f90}.o"
+        $FC $FAST -module-dir "$D" -c "$REF/$f" -o "$o"
+        objs+=("$o")
+    done
+    $FC $FAST -module-dir "$D" -I"$D" -c "$HERE/ref_harness.f90" -o "$D/ref_harness.o"
+    $FC -o "$OUT/ref_${variant}_fast" "$D/ref_harness.o" "${objs[@]}" \
+        -L"$MPI_LIB" -lmpifort -lmpi -Wl,-rpath,"$MPI_LIB"
+done
+echo "built $OUT/ref_slab $OUT/ref_bulk $OUT/ref_slab_fast $OUT/ref_bulk_fast $OUT/md_nve_slab $OUT/md_nve_bulk $OUT/md_nve_oracle $OUT/md_nve_gpu"

@@ -18,8 +18,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REF_DIR = os.path.join(HERE, "_ref")
 
 
-def ref_available() -> bool:
-    return all(os.access(os.path.join(REF_DIR, "ref_" + v), os.X_OK) for v in ("slab", "bulk"))
+def ref_available(fast: bool = False) -> bool:
+    sfx = "_fast" if fast else ""
+    return all(os.access(os.path.join(REF_DIR, "ref_" + v + sfx), os.X_OK) for v in ("slab", "bulk"))
 
 
 def parse_dump(path: str) -> Dict[str, np.ndarray]:
@@ -44,17 +45,18 @@ def mpi_available() -> bool:
 
 
 def run_reference(system, workdir: str = None, timeout: float = 600.0, nrep: int = 0,
-                  nproc: int = 1, env=None) -> Dict[str, np.ndarray]:
+                  nproc: int = 1, env=None, fast: bool = False) -> Dict[str, np.ndarray]:
     """Write the reference's input files for `system`, run the matching harness variant
     (bulk = 3-D periodic, slab otherwise) and return the parsed dump.  nproc > 1 runs the
     harness on that many MPI ranks (the reference's own decomposition of the force path)
     for timing only: the result then holds the timing and no dump.  env adds environment
     variables (REF_CHEMPOT="ichmax lwmin lwmax lchmax ilmid ncall" runs chempot ncall times
-    after the step and dumps chem_pave, chem_inave, chem_seed and chem_par)."""
+    after the step and dumps chem_pave, chem_inave, chem_seed and chem_par).  fast runs the
+    timing build with the reference's own compiler flags (-O2 -ffast-math)."""
     from md_sim_amd.refio import write_reference_inputs
 
     variant = "bulk" if system.z_periodic else "slab"
-    exe = os.path.join(REF_DIR, "ref_" + variant)
+    exe = os.path.join(REF_DIR, "ref_" + variant + ("_fast" if fast else ""))
     if not os.access(exe, os.X_OK):
         raise FileNotFoundError(exe)
     if nproc > 1 and nrep <= 0:
