@@ -23,10 +23,12 @@ of both forms.  Algorithmic work per launch (one launch per step):
 * structure factor: 8 flop per (k-vector, moving site) + 8 per ((kl, km) row, moving site);
 * back-projection: 10 flop per (k-vector, moving site);
 * list build: 24 B per site (positions) + 8 B per site (cell keys) + 4 B per listed pair.
-Each is divided by the kernel's mean duration, measured with HIP events recorded on the
-engine's stream around each stage in a second pass over the same steps (direct launches;
-the timed pass replays a captured HIP graph, inside which events cannot be timed), and the
-list build in a third pass that rebuilds the list on every step.  ``traffic`` is the HBM
+Each is divided by the kernel's mean duration, measured in a second pass over the same steps
+with direct launches (the timed pass replays a captured HIP graph, inside which events
+cannot be timed): the pair and reciprocal-sum kernels by start and stop events that their
+own dispatch records on the engine's stream (hipExtLaunchKernelGGL), the list build (six
+kernels) by HIP events around the stage in a third pass that rebuilds the list on every
+step.  ``traffic`` is the HBM
 bytes per launch that rocprofv3's FETCH_SIZE and WRITE_SIZE counters give for the kernel
 (profiles/traffic_<workload>.json), when that file covers the workload.
 
@@ -275,6 +277,10 @@ def roofline(kernel, bound, pipe, work, ms, peak, unit, traffic=None, **extra):
 def form_rooflines(form, run, s, pc, n_k, n_rows, kernels, traffic):
     """Rooflines of one form's kernels from its per-stage means."""
     st, bs = run["stages"], run["build_stages"]
+    # kernel means from the kernels' own dispatch events, the stage means where those are absent
+    km = {k: (st.get(kt) or st[stg]) for k, kt, stg in (("pair", "k_pair", "pair"), ("sf", "k_sf", "sf"),
+                                                        ("sf_reduce", "k_sf_reduce", "sf_reduce"),
+                                                        ("backproj", "k_bp", "backproj"))}
     nm = s.nmove
     f64 = form == "fp64"
     vpeak = PEAK[form]
@@ -283,7 +289,7 @@ def form_rooflines(form, run, s, pc, n_k, n_rows, kernels, traffic):
     out["pair"] = roofline(
         "k_pair<%s>" % ("double" if f64 else "float"), "valu",
         "%s vector ALU" % ("FP64" if f64 else "FP32 (no packed FP32 in this build)"),
-        pair_flops(pc), st["pair"], vpeak, "TFLOP/s", traffic_of(traffic, form, re.escape(pk)))
+        pair_flops(pc), km["pair"], vpeak, "TFLOP/s", traffic_of(traffic, form, re.escape(pk)))
     sf_work = 8.0 * n_k * nm + 8.0 * n_rows * nm
     bp_work = 10.0 * n_k * nm
     if kernels == "mma64":
@@ -304,14 +310,14 @@ def form_rooflines(form, run, s, pc, n_k, n_rows, kernels, traffic):
         mpeak, bound = vpeak, "valu"
     extra_sf, extra_bp = {}, {}
     if kernels == "mma16":   # the accuracy class the results keep
-        extra_sf["frac_of_fp32_peak"] = sf_work / (st["sf"] * 1e-3) / 1e12 / PEAK["fp32"] if st["sf"] > 0 else 0.0
-        extra_bp["frac_of_fp32_peak"] = bp_work / (st["backproj"] * 1e-3) / 1e12 / PEAK["fp32"] \
-            if st["backproj"] > 0 else 0.0
-    out["structure_factor"] = roofline(names[0], bound, sf_pipe, sf_work, st["sf"], mpeak, "TFLOP/s",
+        extra_sf["frac_of_fp32_peak"] = sf_work / (km["sf"] * 1e-3) / 1e12 / PEAK["fp32"] if km["sf"] > 0 else 0.0
+        extra_bp["frac_of_fp32_peak"] = bp_work / (km["backproj"] * 1e-3) / 1e12 / PEAK["fp32"] \
+            if km["backproj"] > 0 else 0.0
+    out["structure_factor"] = roofline(names[0], bound, sf_pipe, sf_work, km["sf"], mpeak, "TFLOP/s",
                                        traffic_of(traffic, form, sf_pat), **extra_sf)
-    out["back_projection"] = roofline(names[1], bound, bp_pipe, bp_work, st["backproj"], mpeak, "TFLOP/s",
+    out["back_projection"] = roofline(names[1], bound, bp_pipe, bp_work, km["backproj"], mpeak, "TFLOP/s",
                                       traffic_of(traffic, form, bp_pat), **extra_bp)
-    t_rec = st["sf"] + st["sf_reduce"] + st["backproj"]
+    t_rec = km["sf"] + km["sf_reduce"] + km["backproj"]
     tr = [traffic_of(traffic, form, p) for p in (sf_pat, red_pat, bp_pat)]
     out["reciprocal_stage"] = roofline(
         "%s + reduce + %s" % names, bound, sf_pipe, sf_work + bp_work, t_rec, mpeak, "TFLOP/s",

@@ -374,8 +374,11 @@ int mdf_mts_eval_device(const float *weights, int32_t n, const float *feat, cons
    direct launches with HIP events between the stages instead (events recorded during
    graph capture cannot be timed).  mdf_stage_times gives names and mean milliseconds per
    step over the steps since mdf_reset_stage_times: the stages (from steps run with stage
-   timing on) followed by "step", the whole step measured in both modes.  cap: capacity
-   of names/ms; n out: number of entries.  on = 2 also rebuilds the pair list on every
+   timing on) followed by "step", the whole step measured in both modes, then "k_pair",
+   "k_sf", "k_sf_reduce" and "k_bp": mean durations per launch of the pair kernel and
+   the three reciprocal-sum kernels, from start and stop events their own dispatches
+   record in the steps with stage timing on.  cap: capacity of names/ms; n out: number
+   of entries.  on = 2 also rebuilds the pair list on every
    step (the "build" stage then times full list builds; the forces do not change: the
    list holds every pair within the cutoff either way).                                */
 int mdf_set_stage_timing(mdf_system *sys, int32_t on);

@@ -9,6 +9,7 @@
 //   mdf_compute  force_get (src/forcem.f90:93-129)
 //   mdf_zero     force_zero (src/forcem.f90:84-90)
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <cmath>
@@ -29,10 +30,12 @@ __global__ void k_decide(DevSys S, const double *dip_part, int n_dip, double *di
 __global__ void k_cell_scan_build(DevSys S);
 __global__ void k_cell_count(DevSys S, int *lead_cell_out);
 __global__ void k_cell_scan(int *cnt, int *start, int *cursor, int n);
-__global__ void k_cell_fill(const int *item_cell, int n_items, int *cursor, int *items, const DevFlags *f,
-                            int use_flag);
-__global__ void k_cell_sort(const int *start, const int *items_in, int *items_out, const DevFlags *f, int use_flag,
-                            int ncells_xy, int ncz_given);
+__global__ void k_cell_fill(DevSys S, const int *lead_cell, int *items_tmp, double4 *xyz_tmp);
+__global__ void k_cell_order(DevSys S, const int *items_tmp, const double4 *xyz_tmp);
+__global__ void k_cell_fill_static(const int *item_cell, int n_items, int *cursor, int *items);
+__global__ void k_cell_sort(const int *start, const int *items_in, int *items_out, int n_cells);
+__global__ void k_cell_xyz(const int *items, int n, int offset, const double *x, const double *y, const double *z,
+                           double4 *out);
 __global__ void k_build(DevSys S);
 __global__ void k_snapshot(DevSys S);
 __global__ void k_wall_cells(DevSys S, int *wcell_of, int ncz, double z0, double csz);
@@ -130,7 +133,7 @@ __global__ void k_decide_b(const RepArgs *);
 __global__ void k_cell_count_b(const RepArgs *, const int2 *);
 __global__ void k_cell_scan_build_b(const RepArgs *);
 __global__ void k_cell_fill_b(const RepArgs *, const int2 *);
-__global__ void k_cell_sort_b(const RepArgs *, const int2 *);
+__global__ void k_cell_order_b(const RepArgs *, const int2 *);
 __global__ void k_build_b(const RepArgs *, const int2 *);
 __global__ void k_snapshot_b(const RepArgs *, const int2 *);
 __global__ void k_flags_b(const RepArgs *, int, DevFlags *);
@@ -236,6 +239,11 @@ struct DBuf {
 static const char *STAGE_NAMES[] = {"list", "build", "pair", "surrogate", "sf", "sf_reduce", "backproj", "epilogue",
                                     "reduce", "step"};
 constexpr int N_STAGES = 9;  // timed stages; entry N_STAGES of the names is the whole step
+// kernels timed by their own dispatch (start and stop events of hipExtLaunchKernelGGL) in the
+// steps with stage timing on: the pair kernel and the three reciprocal-sum kernels
+static const char *KT_NAMES[] = {"k_pair", "k_sf", "k_sf_reduce", "k_bp"};
+constexpr int N_KT = 4;
+enum { KT_PAIR = 0, KT_SF = 1, KT_SFRED = 2, KT_BP = 3 };
 
 struct mdf_system {
     mdf_topology top;
@@ -265,7 +273,8 @@ struct mdf_system {
     DBuf<double> q, eps4, sig6, aa, rhoi, cc, ang_kt, ang_th;
     DBuf<double> x, y, z, cx, cy, cz, ocx, ocy, ocz, ox, oy, oz, fx, fy, fz, part;
     DBuf<int> nbr_mm, cnt_mm, nbr_mw, cnt_mw, cnt_lo;
-    DBuf<int> cell_cnt, cell_start, cell_items, cell_items2, cell_cursor, lead_cell;
+    DBuf<int> cell_cnt, cell_start, cell_items, cell_cursor, lead_cell, cell_items_tmp;
+    DBuf<double4> cell_xyz, wcell_xyz, cell_xyz_tmp;
     DBuf<int> wcell_start, wcell_items;
     DBuf<DevFlags> flags;
     DBuf<double> dip_part, dipz;
@@ -301,13 +310,18 @@ struct mdf_system {
     int n_cu = 256;                                          // compute units of the device
     int skin_blocks = 1, n_sb = 1, n_cg = 1, cpb = 1, n_kblk = 1, n_rg = 1, rpg = 1, n_epblk = 1, n_colblk = 1,
         colper = 1, ncells_xy = 1, n_cells_cap = 1;
+    int build_blocks = 1, order_blocks = 1;   // list build: blocks (cells at a time), order pass
     // timing: a ring of event sets, one per in-flight step; stage durations are accumulated
     // when a set is reused or at the next synchronisation
     static constexpr int NRING = 32;
     hipEvent_t ring[NRING][N_STAGES + 1] = {};
+    hipEvent_t kring[NRING][N_KT][2] = {};   // kernel start/stop events per slot
+    bool kring_used[NRING][N_KT] = {};
     bool ring_used[NRING] = {};
     long long step_no = 0;
     hipEvent_t *ev = nullptr;       // events of the step being launched
+    hipEvent_t (*kev)[2] = nullptr;  // kernel events of the step being launched (stage timing only)
+    bool *kev_used = nullptr;
     // one captured step graph per ring slot (each records its slot's events)
     hipGraphExec_t graph[NRING] = {};
     bool use_graphs = true;
@@ -317,6 +331,8 @@ struct mdf_system {
     // launch-shape targets of the reciprocal kernels (blocks per launch; the row groups follow)
     int sf_target_blocks = 4096, bp_target_blocks = 1024;
     double stage_sum[N_STAGES] = {};
+    double kt_sum[N_KT] = {};
+    long long kt_n[N_KT] = {};
     long long stage_steps = 0;       // steps with per-stage events (direct launches)
     double step_sum = 0.0;
     long long step_steps = 0;        // steps with start/end events (any mode)
@@ -403,9 +419,10 @@ static void mts_free(mdf_system *s) {
 static void free_all(mdf_system *s) {
     DBuf<int> *ib[] = {&s->itype, &s->nsi, &s->ijtype, &s->vkind, &s->cls, &s->lead, &s->lead_slot, &s->con_ptr,
                        &s->con_idx, &s->ang_ptr, &s->ang_inc, &s->ang_ijk, &s->nbr_mm, &s->cnt_mm, &s->nbr_mw, &s->cnt_mw, &s->cnt_lo, &s->cell_cnt, &s->cell_start,
-                       &s->cell_items, &s->cell_items2, &s->cell_cursor, &s->lead_cell, &s->wcell_start,
+                       &s->cell_items, &s->cell_cursor, &s->lead_cell, &s->cell_items_tmp, &s->wcell_start,
                        &s->wcell_items, &s->d_kvec};
     for (auto *b : ib) b->release();
+    s->cell_xyz.release(); s->cell_xyz_tmp.release(); s->wcell_xyz.release();
     DBuf<double> *db[] = {&s->q, &s->eps4, &s->sig6, &s->aa, &s->rhoi, &s->cc, &s->x, &s->y, &s->z, &s->cx,
                           &s->cy, &s->cz, &s->ocx, &s->ocy, &s->ocz, &s->ox, &s->oy, &s->oz, &s->fx, &s->fy,
                           &s->fz, &s->part, &s->dip_part, &s->dipz, &s->d_ak, &s->kpart, &s->fpart, &s->eppart,
@@ -742,6 +759,16 @@ static int apply_geometry(mdf_system *s) {
     return MDF_OK;
 }
 
+static void destroy_events(mdf_system *s) {
+    for (int r = 0; r < mdf_system::NRING; r++) {
+        for (int k = 0; k <= N_STAGES; k++)
+            if (s->ring[r][k]) (void)hipEventDestroy(s->ring[r][k]);
+        for (int k = 0; k < N_KT; k++)
+            for (int e = 0; e < 2; e++)
+                if (s->kring[r][k][e]) (void)hipEventDestroy(s->kring[r][k][e]);
+    }
+}
+
 static void drop_graphs(mdf_system *s) {
     for (int r = 0; r < mdf_system::NRING; r++)
         if (s->graph[r]) { (void)hipGraphExecDestroy(s->graph[r]); s->graph[r] = nullptr; }
@@ -768,9 +795,9 @@ static void launch_lists(mdf_system *s, int force_rebuild) {
     int lb = std::max(1, (s->n_lead + 255) / 256);
     k_cell_count<<<lb, 256, 0, st>>>(S, s->lead_cell.p);
     k_cell_scan_build<<<1, 1024, 0, st>>>(S);
-    k_cell_fill<<<lb, 256, 0, st>>>(s->lead_cell.p, s->n_lead, s->cell_cursor.p, s->cell_items2.p, s->flags.p, 1);
-    k_cell_sort<<<512, 64, 0, st>>>(s->cell_start.p, s->cell_items2.p, s->cell_items.p, s->flags.p, 1, s->ncells_xy, 0);
-    k_build<<<std::max(1, (s->n_lead + 3) / 4), 256, 0, st>>>(S);  // one wave per lead
+    k_cell_fill<<<lb, 256, 0, st>>>(S, s->lead_cell.p, s->cell_items_tmp.p, s->cell_xyz_tmp.p);
+    k_cell_order<<<s->order_blocks, 256, 0, st>>>(S, s->cell_items_tmp.p, s->cell_xyz_tmp.p);
+    k_build<<<s->build_blocks, BUILD_THREADS, 0, st>>>(S);   // one block per cell
     k_snapshot<<<std::max(1, (std::max(s->top.n_rig, s->top.n_move) + 255) / 256), 256, 0, st>>>(S);
 }
 
@@ -789,12 +816,27 @@ static RecipArgs recip_args(const mdf_system *s) {
     return a;
 }
 
+// launch of a kernel timed by its own dispatch: with stage timing on, hipExtLaunchKernelGGL
+// records the slot's start and stop events with the kernel itself (no launch gaps in the
+// interval); otherwise an ordinary launch.  KERN is parenthesised when it has template
+// arguments.
+#define MDF_TLAUNCH(s_, slot, KERN, grid, block, shm, st, ...)                                                  \
+    do {                                                                                                        \
+        if ((s_)->kev) {                                                                                        \
+            (s_)->kev_used[slot] = true;                                                                        \
+            hipExtLaunchKernelGGL(KERN, dim3(grid), dim3(block), (uint32_t)(shm), st, (s_)->kev[slot][0],      \
+                                  (s_)->kev[slot][1], 0u, __VA_ARGS__);                                         \
+        } else {                                                                                                \
+            KERN<<<grid, block, shm, st>>>(__VA_ARGS__);                                                        \
+        }                                                                                                       \
+    } while (0)
+
 template <int G>
 static void launch_pair_g(mdf_system *s, bool f32, bool vir, bool acc) {
     const int blocks = std::max(1, (s->n_lead + (256 / G) - 1) / (256 / G));
     hipStream_t st = s->stream;
     DevSys &S = s->S;
-#define MDF_PAIR(R, V, A) k_pair<R, G, V, A><<<blocks, 256, 0, st>>>(S)
+#define MDF_PAIR(R, V, A) MDF_TLAUNCH(s, KT_PAIR, (k_pair<R, G, V, A>), blocks, 256, 0, st, S)
     if (f32) {
         if (vir) { if (acc) MDF_PAIR(float, true, true); else MDF_PAIR(float, true, false); }
         else { if (acc) MDF_PAIR(float, false, true); else MDF_PAIR(float, false, false); }
@@ -857,10 +899,9 @@ static void launch_sf(mdf_system *s) {
         const size_t lds = sf64_lds_bytes(s->klmax, s->kmmax, s->s64_nt);
 #define SF64_LAUNCH(NT)                                                                                         \
     case NT:                                                                                                    \
-        k_sf64<NT><<<grid, 256, lds, s->stream>>>(S.x, S.y, S.z, S.q, nm, a.cux, a.cuy, a.cuz, s->d_s64_rowkk.p,  \
-                                                  s->d_s64_tilent.p, s->d_s64_groups.p, nrows, s->klmax, s->kmmax, \
-                                                  s->d_s64_pcoff.p, s->s64_pcols, s->d_s64_part.p, s->s64_sites,   \
-                                                  mts_state(s));                                                 \
+        MDF_TLAUNCH(s, KT_SF, k_sf64<NT>, grid, 256, lds, s->stream, S.x, S.y, S.z, S.q, nm, a.cux, a.cuy, a.cuz, \
+                    s->d_s64_rowkk.p, s->d_s64_tilent.p, s->d_s64_groups.p, nrows, s->klmax, s->kmmax,          \
+                    s->d_s64_pcoff.p, s->s64_pcols, s->d_s64_part.p, s->s64_sites, mts_state(s));                \
         break;
         switch (s->s64_nt) {
             SF64_LAUNCH(1) SF64_LAUNCH(2) SF64_LAUNCH(3) SF64_LAUNCH(4) SF64_LAUNCH(5) SF64_LAUNCH(6) SF64_LAUNCH(7)
@@ -875,9 +916,8 @@ static void launch_sf(mdf_system *s) {
         float2 *part = reinterpret_cast<float2 *>(s->sf_partial.p);
 #define SFM_LAUNCH(NT)                                                                                          \
     case NT:                                                                                                    \
-        k_sf_mma<NT><<<grid, 256, lds, s->stream>>>(S.x, S.y, S.z, S.q, nm, a.cux, a.cuy, a.cuz, s->d_rows.p,     \
-                                                    nrows, s->knmax, s->n_nt, part, nk, s->sfm_sites,          \
-                                                    mts_state(s));                                              \
+        MDF_TLAUNCH(s, KT_SF, k_sf_mma<NT>, grid, 256, lds, s->stream, S.x, S.y, S.z, S.q, nm, a.cux, a.cuy,     \
+                    a.cuz, s->d_rows.p, nrows, s->knmax, s->n_nt, part, nk, s->sfm_sites, mts_state(s));          \
         break;
         switch (s->n_nt) {
             SFM_LAUNCH(1) SFM_LAUNCH(2) SFM_LAUNCH(3) SFM_LAUNCH(4) SFM_LAUNCH(5) SFM_LAUNCH(6) SFM_LAUNCH(7)
@@ -888,11 +928,11 @@ static void launch_sf(mdf_system *s) {
     }
     dim3 g1(s->n_sb, s->n_cg);
     if (s->par.precision == MDF_FP32_ACC64)
-        k_sf<float, 4><<<g1, SF_BLOCK, 0, s->stream>>>(S.x, S.y, S.z, S.q, nm, a.cux, a.cuy, a.cuz, s->d_rows.p,
-                                                       nrows, s->cpb, s->sf_partial.p, nk, mts_state(s));
+        MDF_TLAUNCH(s, KT_SF, (k_sf<float, 4>), g1, SF_BLOCK, 0, s->stream, S.x, S.y, S.z, S.q, nm, a.cux, a.cuy,
+                    a.cuz, s->d_rows.p, nrows, s->cpb, s->sf_partial.p, nk, mts_state(s));
     else
-        k_sf<double, 4><<<g1, SF_BLOCK, 0, s->stream>>>(S.x, S.y, S.z, S.q, nm, a.cux, a.cuy, a.cuz, s->d_rows.p,
-                                                        nrows, s->cpb, s->sf_partial.p, nk, mts_state(s));
+        MDF_TLAUNCH(s, KT_SF, (k_sf<double, 4>), g1, SF_BLOCK, 0, s->stream, S.x, S.y, S.z, S.q, nm, a.cux, a.cuy,
+                    a.cuz, s->d_rows.p, nrows, s->cpb, s->sf_partial.p, nk, mts_state(s));
 }
 
 static void launch_bp(mdf_system *s) {
@@ -904,10 +944,9 @@ static void launch_bp(mdf_system *s) {
         const dim3 grid((nm + s->b64_sites - 1) / s->b64_sites, s->b64_nrg);
 #define BP64_LAUNCH(NKS, NG)                                                                                    \
     case NKS * 4 + NG:                                                                                          \
-        k_bp64<NKS, NG><<<grid, 256, 0, s->stream>>>(S.x, S.y, S.z, S.q, nm, a.cux, a.cuy, a.cuz, s->d_b64_rinfo.p, \
-                                                     (int)s->rows.size(), s->d_b64_tile.p, s->d_b64_rg.p,        \
-                                                     s->d_b64_tpk.p, a.pipix, a.pipiy, a.pipiz, s->fpart.p,      \
-                                                     mts_state(s));                                               \
+        MDF_TLAUNCH(s, KT_BP, (k_bp64<NKS, NG>), grid, 256, 0, s->stream, S.x, S.y, S.z, S.q, nm, a.cux, a.cuy,  \
+                    a.cuz, s->d_b64_rinfo.p, (int)s->rows.size(), s->d_b64_tile.p, s->d_b64_rg.p, s->d_b64_tpk.p, \
+                    a.pipix, a.pipiy, a.pipiz, s->fpart.p, mts_state(s));                                        \
         break;
         switch (s->b64_nks * 4 + s->b64_ng) {
             BP64_LAUNCH(4, 1) BP64_LAUNCH(8, 1) BP64_LAUNCH(12, 1) BP64_LAUNCH(16, 1) BP64_LAUNCH(20, 1)
@@ -923,11 +962,9 @@ static void launch_bp(mdf_system *s) {
         const dim3 grid((nm + ns - 1) / ns), block(64 * s->bpm_waves);
 #define BPM_LAUNCH(NT)                                                                                          \
     case NT:                                                                                                    \
-        k_bp_mma<NT><<<grid, block, 0, s->stream>>>(S.x, S.y, S.z, S.q, nm, a.cux, a.cuy, a.cuz, s->d_rowkk.p,   \
-                                                    s->d_tile_ks.p, (int)s->rows.size(), s->knmax, s->n_nt,    \
-                                                    s->d_tpack.p, s->d_tscale.p, a.pipix, a.pipiy, a.pipiz,    \
-                                                    s->fpart.p,                                                 \
-                                                    mts_state(s));                                              \
+        MDF_TLAUNCH(s, KT_BP, k_bp_mma<NT>, grid, block, 0, s->stream, S.x, S.y, S.z, S.q, nm, a.cux, a.cuy,     \
+                    a.cuz, s->d_rowkk.p, s->d_tile_ks.p, (int)s->rows.size(), s->knmax, s->n_nt, s->d_tpack.p,   \
+                    s->d_tscale.p, a.pipix, a.pipiy, a.pipiz, s->fpart.p, mts_state(s));                         \
         break;
         switch (s->n_nt) {
             BPM_LAUNCH(1) BPM_LAUNCH(2) BPM_LAUNCH(3) BPM_LAUNCH(4) BPM_LAUNCH(5) BPM_LAUNCH(6) BPM_LAUNCH(7)
@@ -938,13 +975,13 @@ static void launch_bp(mdf_system *s) {
     }
     dim3 g2(std::max(1, (nm + 256 * BP_NS - 1) / (256 * BP_NS)), s->n_rg);
     if (s->par.precision == MDF_FP32_ACC64)
-        k_bp<float, BP_NS><<<g2, 256, 0, s->stream>>>(S.x, S.y, S.z, S.q, nm, a.cux, a.cuy, a.cuz, s->d_rows.p,
-                                                      s->d_bp_row_start.p, s->d_t.p, a.pipix, a.pipiy, a.pipiz,
-                                                      s->fpart.p, mts_state(s));
+        MDF_TLAUNCH(s, KT_BP, (k_bp<float, BP_NS>), g2, 256, 0, s->stream, S.x, S.y, S.z, S.q, nm, a.cux, a.cuy,
+                    a.cuz, s->d_rows.p, s->d_bp_row_start.p, s->d_t.p, a.pipix, a.pipiy, a.pipiz, s->fpart.p,
+                    mts_state(s));
     else
-        k_bp<double, BP_NS><<<g2, 256, 0, s->stream>>>(S.x, S.y, S.z, S.q, nm, a.cux, a.cuy, a.cuz, s->d_rows.p,
-                                                       s->d_bp_row_start.p, s->d_t.p, a.pipix, a.pipiy, a.pipiz,
-                                                       s->fpart.p, mts_state(s));
+        MDF_TLAUNCH(s, KT_BP, (k_bp<double, BP_NS>), g2, 256, 0, s->stream, S.x, S.y, S.z, S.q, nm, a.cux, a.cuy,
+                    a.cuz, s->d_rows.p, s->d_bp_row_start.p, s->d_t.p, a.pipix, a.pipiy, a.pipiz, s->fpart.p,
+                    mts_state(s));
 }
 
 // the scalars the final reduction needs
@@ -998,20 +1035,19 @@ static void launch_forces(mdf_system *s) {
         const int n_sb = nm > 0 ? (s->sf_mma ? s->n_sb_mma : s->n_sb) : 0;
         const float2 *p32 = s->sf_mma && nm > 0 ? reinterpret_cast<const float2 *>(s->sf_partial.p) : nullptr;
         if (s->mma64)    // one block per row: S, T, energies and the back-projection operand
-            k_sf64_reduce<<<(int)s->rows.size(), SFR_THREADS, 0, st>>>(
-                s->d_s64_part.p, nm > 0 ? s->s64_nsb : 0, s->s64_pcols, s->d_s64_pcoff.p, s->d_s64_sorted_of.p,
-                s->d_rows.p, s->d_swall.p, s->d_ak.p, ra.pipix, ra.pipiy, ra.pipiz, lamb, s->d_s.p, s->d_t.p,
-                s->kpart.p, s->d_b64_tile.p, s->d_b64_tpk.p, mts_state(s));
+            MDF_TLAUNCH(s, KT_SFRED, k_sf64_reduce, (int)s->rows.size(), SFR_THREADS, 0, st, s->d_s64_part.p,
+                        nm > 0 ? s->s64_nsb : 0, s->s64_pcols, s->d_s64_pcoff.p, s->d_s64_sorted_of.p, s->d_rows.p,
+                        s->d_swall.p, s->d_ak.p, ra.pipix, ra.pipiy, ra.pipiz, lamb, s->d_s.p, s->d_t.p,
+                        s->kpart.p, s->d_b64_tile.p, s->d_b64_tpk.p, mts_state(s));
         else if (s->bp_mma)   // one block per row: T split with per-row scales for k_bp_mma
-            k_sf_reduce_rows<<<(int)s->rows.size(), SFR_THREADS, 0, st>>>(
-                s->sf_partial.p, p32, n_sb, nk, s->d_rows.p, s->d_swall.p, s->d_ak.p, ra.pipix, ra.pipiy, ra.pipiz,
-                lamb, s->d_s.p, s->d_t.p, s->kpart.p, mts_state(s), s->d_tpack.p, s->d_tscale.p, 16 * s->n_nt,
-                s->knmax);
+            MDF_TLAUNCH(s, KT_SFRED, k_sf_reduce_rows, (int)s->rows.size(), SFR_THREADS, 0, st, s->sf_partial.p,
+                        p32, n_sb, nk, s->d_rows.p, s->d_swall.p, s->d_ak.p, ra.pipix, ra.pipiy, ra.pipiz, lamb,
+                        s->d_s.p, s->d_t.p, s->kpart.p, mts_state(s), s->d_tpack.p, s->d_tscale.p, 16 * s->n_nt,
+                        s->knmax);
         else
-            k_sf_reduce<<<s->n_kblk, SFR_THREADS, 0, st>>>(s->sf_partial.p, n_sb, nk, s->d_swall.p, s->d_ak.p,
-                                                           s->d_kvec.p, ra.pipix, ra.pipiy, ra.pipiz, lamb, s->d_s.p,
-                                                           s->d_t.p, s->kpart.p, mts_state(s), nullptr, nullptr, 0, 0,
-                                                           p32);
+            MDF_TLAUNCH(s, KT_SFRED, k_sf_reduce, s->n_kblk, SFR_THREADS, 0, st, s->sf_partial.p, n_sb, nk,
+                        s->d_swall.p, s->d_ak.p, s->d_kvec.p, ra.pipix, ra.pipiy, ra.pipiz, lamb, s->d_s.p, s->d_t.p,
+                        s->kpart.p, mts_state(s), nullptr, nullptr, 0, 0, p32);
     }
     if (s->ev) (void)hipEventRecord(s->ev[6], st);
     launch_bp(s);
@@ -1064,6 +1100,17 @@ static void drain_slot(mdf_system *s, int slot) {
         staged = hipEventElapsedTime(&ms[k], e[k], e[k + 1]) == hipSuccess;
     }
     if (ok) { s->step_sum += tot; s->step_steps++; }
+    for (int k = 0; k < N_KT; k++) {
+        if (!s->kring_used[slot][k]) continue;
+        s->kring_used[slot][k] = false;
+        float km = 0.f;
+        if (ok && hipEventElapsedTime(&km, s->kring[slot][k][0], s->kring[slot][k][1]) == hipSuccess) {
+            s->kt_sum[k] += km;
+            s->kt_n[k]++;
+        } else {
+            (void)hipGetLastError();
+        }
+    }
     if (staged) {
         for (int k = 0; k < N_STAGES; k++) { s->stage_sum[k] += ms[k]; s->stage_ms[k] = ms[k]; }
         s->stage_steps++;
@@ -1245,9 +1292,15 @@ static int run_step(mdf_system *s, int force_rebuild) {
         (void)hipEventRecord(ev[N_STAGES], s->stream);
     } else {
         s->slot_staged[slot] = true;
+        if (s->stage_timing) {
+            s->kev = s->kring[slot];
+            s->kev_used = s->kring_used[slot];
+        }
         (void)hipEventRecord(ev[0], s->stream);
         launch_lists(s, force_rebuild || s->stage_timing == 2);
         launch_forces(s);
+        s->kev = nullptr;
+        s->kev_used = nullptr;
     }
     HIPCHK(hipGetLastError());
     s->pending = true;
@@ -1435,6 +1488,9 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
         HTRY(hipEventCreateWithFlags(&s->in_event, hipEventDisableTiming));
         for (int r = 0; r < mdf_system::NRING; r++)
             for (int k = 0; k <= N_STAGES; k++) HTRY(hipEventCreate(&s->ring[r][k]));
+        for (int r = 0; r < mdf_system::NRING; r++)
+            for (int k = 0; k < N_KT; k++)
+                for (int e = 0; e < 2; e++) HTRY(hipEventCreate(&s->kring[r][k][e]));
         // leads, lead-type slots, energy classes
         s->h_itype.assign(t.site_type, t.site_type + t.n_total);
         std::vector<int> lead_types;
@@ -1621,9 +1677,22 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
             S.n_cells_cap = s->n_cells_cap;
             HTRY(s->cell_cnt.alloc(s->n_cells_cap + 1)); HTRY(s->cell_start.alloc(s->n_cells_cap + 1));
             HTRY(s->cell_cursor.alloc(s->n_cells_cap + 1));
-            HTRY(s->cell_items.alloc(std::max(s->n_lead, 1))); HTRY(s->cell_items2.alloc(std::max(s->n_lead, 1)));
+            HTRY(s->cell_items.alloc(std::max(s->n_lead, 1))); HTRY(s->cell_xyz.alloc(std::max(s->n_lead, 1)));
+            HTRY(s->cell_items_tmp.alloc(std::max(s->n_lead, 1)));
+            HTRY(s->cell_xyz_tmp.alloc(std::max(s->n_lead, 1)));
             HTRY(s->lead_cell.alloc(std::max(s->n_lead, 1)));
             S.cell_cnt = s->cell_cnt.p; S.cell_start = s->cell_start.p; S.cell_items = s->cell_items.p;
+            S.cell_xyz = s->cell_xyz.p;
+            // blocks of the list build and the order pass: the cells of the initial lead z
+            // range (the blocks stride over the cells a later build has)
+            int ncz0 = S.g.ncz_fixed;
+            if (!S.g.zper && s->n_lead > 0) {
+                double lo = z[s->h_lead[0]], hi = lo;
+                for (int l = 0; l < s->n_lead; l++) { lo = std::min(lo, z[s->h_lead[l]]); hi = std::max(hi, z[s->h_lead[l]]); }
+                ncz0 = std::max(1, std::min(S.g.ncz_max, (int)floor((hi - lo) / rs)));
+            }
+            s->build_blocks = std::max(1, s->ncells_xy * ncz0 * BUILD_ITEMS_PER_CELL);
+            s->order_blocks = std::max(1, (s->build_blocks + 3) / 4);
             S.cell_cursor = s->cell_cursor.p;
         }
         HTRY(s->flags.alloc(1));
@@ -1660,6 +1729,7 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
             HTRY(wcell_of.alloc(std::max(nw, 1))); HTRY(wcnt.alloc(nwc + 1)); HTRY(wcur.alloc(nwc + 1));
             HTRY(witems2.alloc(std::max(nw, 1)));
             HTRY(s->wcell_start.alloc(nwc + 1)); HTRY(s->wcell_items.alloc(std::max(nw, 1)));
+            HTRY(s->wcell_xyz.alloc(std::max(nw, 1)));
             k_zero_int<<<64, 256, 0, s->stream>>>(wcnt.p, nwc + 1);
             if (nw > 0) {
                 k_wall_cells<<<(nw + 255) / 256, 256, 0, s->stream>>>(S, wcell_of.p, wncz, wz0, wcsz);
@@ -1667,13 +1737,14 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
             }
             k_cell_scan<<<1, 1024, 0, s->stream>>>(wcnt.p, s->wcell_start.p, wcur.p, s->ncells_xy * wncz);
             if (nw > 0) {
-                k_cell_fill<<<(nw + 255) / 256, 256, 0, s->stream>>>(wcell_of.p, nw, wcur.p, witems2.p, s->flags.p, 0);
-                k_cell_sort<<<512, 64, 0, s->stream>>>(s->wcell_start.p, witems2.p, s->wcell_items.p, s->flags.p, 0,
-                                                       s->ncells_xy, wncz);
+                k_cell_fill_static<<<(nw + 255) / 256, 256, 0, s->stream>>>(wcell_of.p, nw, wcur.p, witems2.p);
+                k_cell_sort<<<512, 64, 0, s->stream>>>(s->wcell_start.p, witems2.p, s->wcell_items.p, nwc);
+                k_cell_xyz<<<(nw + 255) / 256, 256, 0, s->stream>>>(s->wcell_items.p, nw, t.n_move, S.x, S.y, S.z,
+                                                                    s->wcell_xyz.p);
             }
             HTRY(hipStreamSynchronize(s->stream));
             wcell_of.release(); wcnt.release(); wcur.release(); witems2.release();
-            S.wcell_start = s->wcell_start.p; S.wcell_items = s->wcell_items.p;
+            S.wcell_start = s->wcell_start.p; S.wcell_items = s->wcell_items.p; S.wcell_xyz = s->wcell_xyz.p;
             S.wncz = wncz; S.wz0 = wz0; S.wcsz = wcsz;
         }
         // lists
@@ -1840,9 +1911,7 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
 fail_out:
     free_all(s);
     if (s->stream) (void)hipStreamDestroy(s->stream);
-    for (int r = 0; r < mdf_system::NRING; r++)
-        for (int k = 0; k <= N_STAGES; k++)
-            if (s->ring[r][k]) (void)hipEventDestroy(s->ring[r][k]);
+    destroy_events(s);
     delete s;
     return rc;
 #undef TRY
@@ -1986,7 +2055,7 @@ int mdf_stage_times(mdf_system *s, const char **names, float *ms, int32_t cap, i
     HIPCHK(hipSetDevice(s->device));
     HIPCHK(hipStreamSynchronize(s->stream));
     drain_all(s);
-    *n = N_STAGES + 1;
+    *n = N_STAGES + 1 + N_KT;
     for (int k = 0; k < N_STAGES && k < cap; k++) {
         if (names) names[k] = STAGE_NAMES[k];
         if (ms) ms[k] = s->stage_steps > 0 ? (float)(s->stage_sum[k] / (double)s->stage_steps) : 0.f;
@@ -1994,6 +2063,10 @@ int mdf_stage_times(mdf_system *s, const char **names, float *ms, int32_t cap, i
     if (N_STAGES < cap) {
         if (names) names[N_STAGES] = STAGE_NAMES[N_STAGES];
         if (ms) ms[N_STAGES] = s->step_steps > 0 ? (float)(s->step_sum / (double)s->step_steps) : 0.f;
+    }
+    for (int k = 0; k < N_KT && N_STAGES + 1 + k < cap; k++) {
+        if (names) names[N_STAGES + 1 + k] = KT_NAMES[k];
+        if (ms) ms[N_STAGES + 1 + k] = s->kt_n[k] > 0 ? (float)(s->kt_sum[k] / (double)s->kt_n[k]) : 0.f;
     }
     return MDF_OK;
 }
@@ -2098,6 +2171,7 @@ int mdf_reset_stage_times(mdf_system *s) {
     HIPCHK(hipStreamSynchronize(s->stream));
     drain_all(s);
     for (int k = 0; k < N_STAGES; k++) s->stage_sum[k] = 0.0;
+    for (int k = 0; k < N_KT; k++) { s->kt_sum[k] = 0.0; s->kt_n[k] = 0; }
     s->stage_steps = 0;
     s->step_sum = 0.0;
     s->step_steps = 0;
@@ -2110,9 +2184,7 @@ void mdf_destroy(mdf_system *s) {
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     drop_graphs(s);
     free_all(s);
-    for (int r = 0; r < mdf_system::NRING; r++)
-        for (int k = 0; k <= N_STAGES; k++)
-            if (s->ring[r][k]) (void)hipEventDestroy(s->ring[r][k]);
+    destroy_events(s);
     if (s->in_event) (void)hipEventDestroy(s->in_event);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
@@ -2710,7 +2782,7 @@ int mdf_mts_eval_device(const float *weights, int32_t n, const float *feat, cons
 // Many independent systems stepped together: every stage is one launch over all of them
 // (block maps give each block its system), so small systems fill the GPU together and a
 // step of the whole batch costs one graph launch.
-enum { BK_GATHER, BK_SKIN, BK_COUNT, BK_FILL, BK_SORT, BK_BUILD, BK_SNAP, BK_PAIR, BK_SF, BK_SFR, BK_BP, BK_EPI,
+enum { BK_GATHER, BK_SKIN, BK_COUNT, BK_FILL, BK_ORDER, BK_BUILD, BK_SNAP, BK_PAIR, BK_SF, BK_SFR, BK_BP, BK_EPI,
        BK_COLSUM, BK_N };
 
 struct mdf_batch {
@@ -2741,10 +2813,11 @@ static void fill_rep(mdf_system *s, RepArgs &r) {
     r.skin_blocks = s->skin_blocks;
     r.force_rebuild = 0;
     r.lead_cell = s->lead_cell.p;
-    r.cell_items2 = s->cell_items2.p;
     r.lead_blocks = std::max(1, (s->n_lead + 255) / 256);
-    r.sort_blocks = 64;
-    r.build_blocks = std::max(1, (s->n_lead + 3) / 4);
+    r.build_blocks = s->build_blocks;
+    r.order_blocks = s->order_blocks;
+    r.items_tmp = s->cell_items_tmp.p;
+    r.xyz_tmp = s->cell_xyz_tmp.p;
     r.snap_blocks = std::max(1, (std::max(nr, nm) + 255) / 256);
     r.ncells_xy = s->ncells_xy;
     r.gather_blocks = std::max(1, (std::max(nm, nr) + 255) / 256);
@@ -2796,7 +2869,7 @@ static void rep_blocks(const mdf_batch *b, const mdf_system *s, const RepArgs &r
     cnt[BK_SKIN] = r.skin_blocks;
     cnt[BK_COUNT] = r.lead_blocks;
     cnt[BK_FILL] = r.lead_blocks;
-    cnt[BK_SORT] = r.sort_blocks;
+    cnt[BK_ORDER] = r.order_blocks;
     cnt[BK_BUILD] = r.build_blocks;
     cnt[BK_SNAP] = r.snap_blocks;
     cnt[BK_PAIR] = std::max(1, (s->n_lead + (256 / b->pair_g) - 1) / (256 / b->pair_g));
@@ -2843,8 +2916,8 @@ static void batch_launch(mdf_batch *b) {
     k_cell_count_b<<<grid(BK_COUNT), 256, 0, st>>>(R, b->maps[BK_COUNT].p);
     k_cell_scan_build_b<<<n, 1024, 0, st>>>(R);
     k_cell_fill_b<<<grid(BK_FILL), 256, 0, st>>>(R, b->maps[BK_FILL].p);
-    k_cell_sort_b<<<grid(BK_SORT), 64, 0, st>>>(R, b->maps[BK_SORT].p);
-    k_build_b<<<grid(BK_BUILD), 256, 0, st>>>(R, b->maps[BK_BUILD].p);
+    k_cell_order_b<<<grid(BK_ORDER), 256, 0, st>>>(R, b->maps[BK_ORDER].p);
+    k_build_b<<<grid(BK_BUILD), BUILD_THREADS, 0, st>>>(R, b->maps[BK_BUILD].p);
     k_snapshot_b<<<grid(BK_SNAP), 256, 0, st>>>(R, b->maps[BK_SNAP].p);
     {
         const int2 *mp = b->maps[BK_PAIR].p;

@@ -149,9 +149,11 @@ struct DevSys {
     // cells
     Grid g;
     int *cell_cnt, *cell_start, *cell_items;   // moving leads
+    double4 *cell_xyz;                         // the cell-ordered leads' x, y, z and site index
     int *cell_cursor;
     int n_cells_cap;
     const int *wcell_start, *wcell_items;      // frozen sites (built once)
+    const double4 *wcell_xyz;                  // and their x, y, z and site index
     int wncz; double wz0, wcsz;
     // outputs
     double *fx, *fy, *fz;        // [n_move] site forces
@@ -291,6 +293,11 @@ struct ChemArgs {
     double ulr_w, ulr_l, lja, ljc, bua, buc, bur;  // wall tail tables of type 1
 };
 
+// list build: blocks of BUILD_THREADS over (cell, split) work items, candidates staged
+// BUILD_CH at a time, leads in rounds of BUILD_LEADS; BUILD_ITEMS_PER_CELL items per cell
+constexpr int BUILD_THREADS = 256, BUILD_WAVES = BUILD_THREADS / 64, BUILD_CH = 512, BUILD_LEADS = 16;
+constexpr int BUILD_ITEMS_PER_CELL = 4;
+
 // ---- batches of independent systems ----------------------------------------------------
 // One entry per system: everything its launches take.  A batched kernel covers the blocks of
 // every system in one launch; a block map entry (system, local block) tells each block whose
@@ -301,8 +308,9 @@ struct RepArgs {
     // list stage
     double *dip_part, *dipz;
     int skin_blocks, force_rebuild;
-    int *lead_cell, *cell_items2;
-    int lead_blocks, sort_blocks, build_blocks, snap_blocks, ncells_xy, gather_blocks;
+    int *lead_cell, *items_tmp;
+    double4 *xyz_tmp;
+    int lead_blocks, order_blocks, build_blocks, snap_blocks, ncells_xy, gather_blocks;
     // reciprocal sum
     const KRow *rows;
     int nrows, cpb, n_sb, n_cg, nk, n_kblk, n_rg, n_bpsb;

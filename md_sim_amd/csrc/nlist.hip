@@ -6,8 +6,9 @@
 // non-rigid sites (src/forcem.f90:245-250).  Here the candidates come from a cell grid
 // whose cells are at least sqrt(rskinsq) wide, and the same predicate - same operation
 // order, no FMA contraction - decides membership, so the pair set is bit-identical.
-// Lists are ELL rows per lead: moving partners in both directions (full list) and frozen
-// partners.  All kernels after the skin test return at once when no rebuild is due, so the
+// Lists are ELL rows per lead: moving partners in both directions (full list, each row in
+// ascending partner index) and frozen partners (in the fixed order of the static frozen-site
+// cells).  All kernels after the skin test return at once when no rebuild is due, so the
 // step's launch sequence is fixed.
 #include "mdf_internal.h"
 
@@ -172,23 +173,26 @@ __device__ __forceinline__ int lead_cell(const DevSys &S, int i, int ncz, double
     return (cz * S.g.ncy + cy) * S.g.ncx + cx;
 }
 
-// exclusive scan of n cell counts by one block (any size); with zero_cnt the counts are
-// cleared behind it, ready for the next build's counting
+// exclusive scan of n cell counts by one block (any size, a multiple of 64 threads); with
+// zero_cnt the counts are cleared behind it, ready for the next build's counting
 __device__ void scan_cells(int *cnt, int *start, int *cursor, int n, bool zero_cnt) {
-    __shared__ int part[1024];
-    const int nt = blockDim.x;
+    __shared__ int wsum[16];
+    const int nt = blockDim.x, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int per = (n + nt - 1) / nt;
     const int b = threadIdx.x * per, e = min(n, b + per);
     int s = 0;
     for (int c = b; c < e; c++) s += cnt[c];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int acc = 0;
-        for (int t = 0; t < nt; t++) { int v = part[t]; part[t] = acc; acc += v; }
+    // exclusive prefix of the per-thread sums: inclusive scan in the wave, then over waves
+    int inc = s;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += v;
     }
+    if (lane == 63) wsum[w] = inc;
     __syncthreads();
-    int acc = part[threadIdx.x];
+    int base = 0;
+    for (int k = 0; k < w; k++) base += wsum[k];
+    int acc = base + inc - s;
     for (int c = b; c < e; c++) {
         const int v = cnt[c];
         start[c] = acc; cursor[c] = acc; acc += v;
@@ -197,15 +201,34 @@ __device__ void scan_cells(int *cnt, int *start, int *cursor, int n, bool zero_c
     if (threadIdx.x == nt - 1) start[n] = acc;
 }
 
+// lanes of a wave that add to the same counter share one atomic: returns, for an active
+// lane, the old value of ctr[key] plus the number of lower lanes with the same key
+__device__ __forceinline__ int wave_agg_add(int *ctr, int key, bool active) {
+    const int lane = threadIdx.x & 63;
+    unsigned long long todo = __ballot(active);
+    int mine = 0;
+    while (todo) {
+        const int leader = __ffsll((long long)todo) - 1;
+        const int k = __shfl(key, leader, 64);
+        const unsigned long long m = __ballot(active && key == k);
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&ctr[k], __popcll(m));
+        base = __shfl(base, leader, 64);
+        if ((m >> lane) & 1ull) mine = base + __popcll(m & ((1ull << lane) - 1ull));
+        todo &= ~m;
+    }
+    return mine;
+}
+
 // cell of every lead and the cell counts
 __device__ __forceinline__ void cell_count_body(const DevSys &S, int *lead_cell_out, int bid, int nblk) {
     const DevFlags *f = S.flags;
     if (!f->rebuild) return;
     const int l = bid * blockDim.x + threadIdx.x;
-    if (l >= S.n_lead) return;
-    const int c = lead_cell(S, S.lead[l], f->ncz, f->z0, f->csz);
-    lead_cell_out[l] = c;
-    atomicAdd(&S.cell_cnt[c], 1);
+    const bool on = l < S.n_lead;
+    const int c = on ? lead_cell(S, S.lead[l], f->ncz, f->z0, f->csz) : 0;
+    if (on) lead_cell_out[l] = c;
+    (void)wave_agg_add(S.cell_cnt, c, on);
 }
 
 __global__ __launch_bounds__(256) void k_cell_count(DevSys S, int *lead_cell_out) {
@@ -228,40 +251,89 @@ __global__ __launch_bounds__(1024) void k_cell_scan(int *cnt, int *start, int *c
     scan_cells(cnt, start, cursor, n, false);
 }
 
-__device__ __forceinline__ void cell_fill_body(const int *item_cell, int n_items, int *cursor, int *items, const DevFlags *f,
-                            int use_flag, int bid, int nblk) {
-    if (use_flag && !f->rebuild) return;
-    int l = bid * blockDim.x + threadIdx.x;
+// leads into their cells in atomic order (items_tmp, xyz_tmp: the lead and its position and
+// site index); cell_order_body then puts every cell in ascending lead order
+__device__ __forceinline__ void cell_fill_body(const DevSys &S, const int *lead_cell, int *items_tmp,
+                                               double4 *xyz_tmp, int bid, int nblk) {
+    if (!S.flags->rebuild) return;
+    const int l = bid * blockDim.x + threadIdx.x;
+    const bool on = l < S.n_lead;
+    const int c = on ? lead_cell[l] : 0;
+    const int pos = wave_agg_add(S.cell_cursor, c, on);
+    if (on) {
+        const int i = S.lead[l];
+        items_tmp[pos] = l;
+        xyz_tmp[pos] = make_double4(S.x[i], S.y[i], S.z[i], (double)i);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cell_fill(DevSys S, const int *lead_cell, int *items_tmp, double4 *xyz_tmp) {
+    cell_fill_body(S, lead_cell, items_tmp, xyz_tmp, blockIdx.x, gridDim.x);
+}
+
+// ascending lead order inside every cell (cell_items, cell_xyz), one wave per cell: a lane's
+// rank is the number of smaller leads in its cell, from the other lanes' values (cells of
+// more than 64 leads rank from memory)
+__device__ __forceinline__ void cell_order_body(const DevSys &S, const int *items_tmp, const double4 *xyz_tmp,
+                                                int bid, int nblk) {
+    const DevFlags *f = S.flags;
+    if (!f->rebuild) return;
+    const int nc = S.g.ncx * S.g.ncy * f->ncz;
+    const int lane = threadIdx.x & 63, wpb = blockDim.x >> 6;
+    for (int c = bid * wpb + (threadIdx.x >> 6); c < nc; c += nblk * wpb) {
+        const int b = S.cell_start[c], n = S.cell_start[c + 1] - b;
+        if (n <= 64) {
+            const int v = lane < n ? items_tmp[b + lane] : 0x7fffffff;
+            int rank = 0;
+            for (int k = 0; k < n; k++) rank += __shfl(v, k, 64) < v;
+            if (lane < n) {
+                S.cell_items[b + rank] = v;
+                S.cell_xyz[b + rank] = xyz_tmp[b + lane];
+            }
+        } else {
+            for (int a = lane; a < n; a += 64) {
+                const int v = items_tmp[b + a];
+                int rank = 0;
+                for (int k = 0; k < n; k++) rank += items_tmp[b + k] < v;
+                S.cell_items[b + rank] = v;
+                S.cell_xyz[b + rank] = xyz_tmp[b + a];
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cell_order(DevSys S, const int *items_tmp, const double4 *xyz_tmp) {
+    cell_order_body(S, items_tmp, xyz_tmp, blockIdx.x, gridDim.x);
+}
+
+// positions and site indices of cell-ordered items (the frozen sites' cells, once)
+__global__ void k_cell_xyz(const int *items, int n, int offset, const double *x, const double *y, const double *z,
+                           double4 *out) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int j = offset + items[p];
+    out[p] = make_double4(x[j], y[j], z[j], (double)j);
+}
+
+// fill of static items (the frozen sites, at creation): atomic order, sorted after
+__global__ void k_cell_fill_static(const int *item_cell, int n_items, int *cursor, int *items) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= n_items) return;
-    int pos = atomicAdd(&cursor[item_cell[l]], 1);
-    items[pos] = l;
+    items[atomicAdd(&cursor[item_cell[l]], 1)] = l;
 }
 
-__global__ void k_cell_fill(const int *item_cell, int n_items, int *cursor, int *items, const DevFlags *f,
-                            int use_flag) {
-    cell_fill_body(item_cell, n_items, cursor, items, f, use_flag, blockIdx.x, gridDim.x);
-}
-
-// ascending order inside every cell (rank sort, one wave per cell), so list rows are
-// deterministic; items_out gets the sorted copy
-__device__ __forceinline__ void cell_sort_body(const int *start, const int *items_in, int *items_out,
-                                                  const DevFlags *f, int use_flag, int ncells_xy, int ncz_given, int bid, int nblk) {
-    if (use_flag && !f->rebuild) return;
-    int ncz = use_flag ? f->ncz : ncz_given;
-    for (int c = bid; c < ncells_xy * ncz; c += nblk) {
-        int b = start[c], n = start[c + 1] - b;
+// ascending order inside every cell (rank sort, one wave per cell) of static items, so the
+// frozen partners of a list row come in a fixed order; items_out gets the sorted copy
+__global__ __launch_bounds__(64) void k_cell_sort(const int *start, const int *items_in, int *items_out, int n_cells) {
+    for (int c = blockIdx.x; c < n_cells; c += gridDim.x) {
+        const int b = start[c], n = start[c + 1] - b;
         for (int a = threadIdx.x; a < n; a += 64) {
-            int v = items_in[b + a];
+            const int v = items_in[b + a];
             int rank = 0;
             for (int k = 0; k < n; k++) rank += (items_in[b + k] < v);
             items_out[b + rank] = v;
         }
     }
-}
-
-__global__ __launch_bounds__(64) void k_cell_sort(const int *start, const int *items_in, int *items_out,
-                                                  const DevFlags *f, int use_flag, int ncells_xy, int ncz_given) {
-    cell_sort_body(start, items_in, items_out, f, use_flag, ncells_xy, ncz_given, blockIdx.x, gridDim.x);
 }
 
 __device__ __forceinline__ bool constrained(const DevSys &S, int i, int j) {
@@ -271,122 +343,400 @@ __device__ __forceinline__ bool constrained(const DevSys &S, int i, int j) {
     return false;
 }
 
-// list rows: one wave per lead.  The wave walks the lead's neighbour cells in a fixed
-// order, tests 64 candidates at a time and appends the hits with a ballot prefix count,
-// so every row holds its partners in exactly the order a serial scan would give.
+// the list predicate's squared distance from coordinates in hand: list_d2's operation
+// order, with the difference taken as (site with the lower index) - (the other), as the
+// reference's i < j loop has it
+__device__ __forceinline__ double list_d2_xyz(const DevSys &S, double xa, double ya, double za, double xb, double yb,
+                                              double zb) {
+    double dx = __dsub_rn(xa, xb);
+    dx = __dsub_rn(dx, __dmul_rn(S.lx, anint_d(__dmul_rn(dx, S.lxi))));
+    double dy = __dsub_rn(ya, yb);
+    dy = __dsub_rn(dy, __dmul_rn(S.ly, anint_d(__dmul_rn(dy, S.lyi))));
+    double dz = __dsub_rn(za, zb);
+    if (S.zper_list) dz = __dsub_rn(dz, __dmul_rn(S.lz, anint_d(__dmul_rn(dz, S.lzi))));
+    return __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
+}
+
+// List rows.  Work items are (cell, split): the leads of a cell go in rounds of BUILD_LEADS,
+// and item (c, s) takes the rounds r = s, s + BUILD_SPLIT, ... of cell c (grid-stride over
+// the items).  The leads of a cell share its 27-cell stencil, so a round stages the
+// stencil's candidates (cell-ordered positions, ascending lead order inside each cell) in
+// LDS, BUILD_CH at a time, and every wave tests two of the round's leads at a time against
+// them, 64 candidates a step, appending the hits with a ballot prefix count: a row holds its
+// partners in the stencil's order, the order a serial scan of the same cells gives.  Frozen
+// partners: the frozen-site cells that the round's leads reach (their frozen-grid layers
+// differ from lead to lead; one run of layers covers them) are staged the same way with
+// their layer, and each lead keeps the candidates of its own three layers.
+//
+// The predicate is decided in FP32 away from the list radius.  Positions are wrapped into
+// the box in FP64, rounded to FP32 and, where the stencil wraps round a periodic axis,
+// shifted by the box length, so a lead and a candidate within the list radius of each other
+// are at their nearest image (a grid of three or more cells on the axis: cells are at least
+// the list radius wide); grids of fewer cells take the minimum image per pair.  Coordinates
+// are then below 2048 A in magnitude (rounding <= 1.2e-4 A), a difference is within 3e-4 A
+// per axis of the FP64 one and a squared distance near the list radius (|d| < 11.2 A) within
+// 1.2e-2 A^2: outside a band of BUILD_BAND = 0.05 A^2 around rskinsq the FP32 decision is the
+// FP64 one.  Inside the band (about 1e-4 of the candidates), for boxes over 1024 A and for
+// open-z coordinates beyond 1024 A (staged as NaN), the reference's FP64 predicate decides,
+// on the FP64 positions, with its operation order (list_d2_xyz).
+constexpr int BUILD_SPLIT = BUILD_ITEMS_PER_CELL;
+constexpr float BUILD_BAND = 0.05f;
+
+// a run of stencil cells: start and size of each cell in the cell-ordered arrays, inclusive
+// prefix of the sizes, frozen-grid layer (frozen stencils), periodic image shift
+struct Stencil {
+    int b[64], n[64], inc[64], layer[64];
+    float sx[64], sy[64], sz[64];
+    int n_st;
+};
+
+struct BuildSmem {
+    float4 cand[BUILD_CH];           // staged candidates: x, y, z in FP32, site index (int bits)
+    int cand_layer[BUILD_CH];        // frozen candidates: frozen-grid z layer
+    double4 lead_xyz[BUILD_LEADS];   // the round's leads: x, y, z, site index
+    int lead_l[BUILD_LEADS], lead_cnt[BUILD_LEADS], lead_low[BUILD_LEADS], lead_wz[BUILD_LEADS];
+    Stencil mv, fz;                  // the cell's moving stencil (all rounds), a round's frozen one
+    int wlo, whi;
+};
+
+// the stencil run's candidate q: cell k holds it (first inclusive prefix above q)
+__device__ __forceinline__ int stencil_locate(const Stencil &st, int q, int &k) {
+    int lo = 0, hi = st.n_st - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (st.inc[mid] <= q) lo = mid + 1; else hi = mid;
+    }
+    k = lo;
+    return st.b[lo] + q - (st.inc[lo] - st.n[lo]);
+}
+
+// inclusive prefix of the stencil sizes (wave 0; n_st <= 64), then a barrier
+__device__ __forceinline__ void stencil_prefix(Stencil &st) {
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        int v = lane < st.n_st ? st.n[lane] : 0;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int u = __shfl_up(v, o, 64);
+            if (lane >= o) v += u;
+        }
+        if (lane < st.n_st) st.inc[lane] = v;
+    }
+    __syncthreads();
+}
+
+// a coordinate wrapped into [0, L) in FP64, rounded to FP32
+__device__ __forceinline__ float wrap_f32(double v, double L, double Li) { return (float)(v - L * floor(v * Li)); }
+
+// a periodic axis' neighbour cell: index and the image shift (in box lengths) of a wrap
+__device__ __forceinline__ int nbr_cell(int c, int o, int n, int no, float &shift, float L) {
+    if (no < 3) { shift = 0.f; return o; }
+    const int r = c + o - 1;
+    shift = r < 0 ? -L : (r >= n ? L : 0.f);
+    return (r + n) % n;
+}
+
+// the round's lead state in the candidate loop
+struct BuildLead {
+    double4 me;            // FP64 position and site index
+    float x, y, z;         // FP32 staged position (NaN z: no prefilter)
+    int i, ld, wz, cnt, lower;
+    bool surf;
+    int *row;
+};
+
 __device__ __forceinline__ void build_body(const DevSys &S, int bid, int nblk) {
+    __shared__ BuildSmem sm;
     const DevFlags *f = S.flags;
     if (!f->rebuild) return;
-    const int lane = threadIdx.x & 63;
-    const int l = bid * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (l >= S.n_lead) return;   // whole wave leaves together
-    const int i = S.lead[l];
-    const int ncx = S.g.ncx, ncy = S.g.ncy;
-    const int ncz = f->ncz;
-    const double xi = S.x[i], yi = S.y[i], zi = S.z[i];
-    const int cx = wrap_cell(xi, S.lx, S.lxi, ncx);
-    const int cy = wrap_cell(yi, S.ly, S.lyi, ncy);
-    const int cz = S.g.zper ? wrap_cell(zi, S.lz, S.lzi, ncz) : zcell_lin(zi, f->z0, f->csz, ncz);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int ncx = S.g.ncx, ncy = S.g.ncy, ncz = f->ncz;
+    const int nc = ncx * ncy * ncz;
     const int nxo = ncx >= 3 ? 3 : ncx, nyo = ncy >= 3 ? 3 : ncy;
     const int nzo = S.g.zper ? (ncz >= 3 ? 3 : ncz) : 3;
-    int *row = S.nbr_mm + (size_t)l * S.cap_mm;
-    const bool i_surf = i >= S.n_liquid_sites;
-    int cnt = 0, lower = 0;
-    for (int oz = 0; oz < nzo; oz++) {
-        int z2;
-        if (S.g.zper) { z2 = (nzo == 3) ? (cz + oz - 1 + ncz) % ncz : oz; }
-        else { z2 = cz + oz - 1; if (z2 < 0 || z2 >= ncz) continue; }
-        for (int oy = 0; oy < nyo; oy++) {
-            const int y2 = (nyo == 3) ? (cy + oy - 1 + ncy) % ncy : oy;
-            for (int ox = 0; ox < nxo; ox++) {
-                const int x2 = (nxo == 3) ? (cx + ox - 1 + ncx) % ncx : ox;
-                const int c = (z2 * ncy + y2) * ncx + x2;
-                const int b = S.cell_start[c], e = S.cell_start[c + 1];
-                for (int p0 = b; p0 < e; p0 += 64) {
-                    const int p = p0 + lane;
-                    bool hit = false;
-                    int j = 0;
-                    if (p < e) {
-                        const int m = S.cell_items[p];
-                        j = S.lead[m];
-                        if (m != l && !(i_surf && j >= S.n_liquid_sites && constrained(S, i, j))) {
-                            const int a = min(i, j), bb = max(i, j);
-                            hit = list_d2(S, a, bb) < S.rskinsq;
-                        }
-                    }
-                    const unsigned long long msk = __ballot(hit);
-                    if (hit) {
-                        const int pos = cnt + __popcll(msk & ((1ull << lane) - 1ull));
-                        if (pos < S.cap_mm) row[pos] = j;
-                        lower += (j > i);
-                    }
-                    cnt += __popcll(msk);
-                }
+    const bool zper = S.g.zper != 0;
+    const bool zper_d = S.zper_list != 0;   // the predicate's z image
+    const float lxf = (float)S.lx, lyf = (float)S.ly, lzf = (float)S.lz;
+    const float lxif = (float)S.lxi, lyif = (float)S.lyi, lzif = (float)S.lzi;
+    const float r2 = (float)S.rskinsq;
+    const float r2_in = r2 - BUILD_BAND, r2_out = r2 + BUILD_BAND;
+    const bool fast_box = fmax(S.lx, fmax(S.ly, zper_d ? S.lz : 0.0)) <= 1024.0;
+    // shifted images stand for the minimum image when every periodic axis has 3+ cells
+    const bool mv_shift = nxo == 3 && nyo == 3 && (!zper || nzo == 3);
+    const float fnan = __int_as_float(0x7fc00000);
+    auto stage = [&](const double4 &v, float sx, float sy, float sz) {   // FP64 position -> FP32 staged one
+        float z = zper_d ? wrap_f32(v.z, S.lz, S.lzi) + sz : (float)v.z;
+        if (!fast_box || !(fabsf(z) <= 1024.f)) z = fnan;
+        return make_float4(wrap_f32(v.x, S.lx, S.lxi) + sx, wrap_f32(v.y, S.ly, S.lyi) + sy, z,
+                           __int_as_float((int)v.w));
+    };
+    // FP32 squared distance: staged images, or the minimum image per pair
+    auto d2f = [&](const BuildLead &L, const float4 &v, bool mi) {
+        float dx = L.x - v.x, dy = L.y - v.y, dz = L.z - v.z;
+        if (mi) {
+            dx = fmaf(-lxf, rintf(dx * lxif), dx);
+            dy = fmaf(-lyf, rintf(dy * lyif), dy);
+            if (zper_d) dz = fmaf(-lzf, rintf(dz * lzif), dz);
+        }
+        return fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    };
+    auto load_lead = [&](int ld, bool frozen) {
+        BuildLead L;
+        L.ld = ld;
+        L.me = sm.lead_xyz[ld];
+        const float4 m = stage(L.me, 0.f, 0.f, 0.f);
+        L.x = m.x; L.y = m.y; L.z = m.z;
+        L.i = (int)L.me.w;
+        L.surf = L.i >= S.n_liquid_sites;
+        L.wz = sm.lead_wz[ld];
+        L.cnt = sm.lead_cnt[ld];
+        L.lower = 0;
+        const int l = sm.lead_l[ld];
+        L.row = frozen ? S.nbr_mw + (size_t)l * S.cap_mw : S.nbr_mm + (size_t)l * S.cap_mm;
+        return L;
+    };
+    auto append = [&](BuildLead &L, bool hit, int j, int cap) {
+        const unsigned long long msk = __ballot(hit);
+        if (hit) {
+            const int pos = L.cnt + __popcll(msk & ((1ull << lane) - 1ull));
+            if (pos < cap) L.row[pos] = j;
+            L.lower += j > L.i;
+        }
+        L.cnt += __popcll(msk);
+    };
+    // moving candidate v (site j) against lead L: FP32 test, FP64 predicate near the radius
+    auto test_mv = [&](const BuildLead &L, const float4 &v, int j, bool ok, bool mi) {
+        const float d2 = d2f(L, v, mi);
+        bool hit = ok && d2 < r2_in;
+        const bool exact = ok && !(d2 < r2_in) && !(d2 >= r2_out);   // in the band, or NaN
+        if (__ballot(exact)) {
+            if (exact) {
+                const double xj = S.x[j], yj = S.y[j], zj = S.z[j];
+                const bool ifirst = L.i < j;
+                hit = list_d2_xyz(S, ifirst ? L.me.x : xj, ifirst ? L.me.y : yj, ifirst ? L.me.z : zj,
+                                  ifirst ? xj : L.me.x, ifirst ? yj : L.me.y, ifirst ? zj : L.me.z) < S.rskinsq;
             }
         }
-    }
-    // wave totals
-    for (int o = 32; o > 0; o >>= 1) lower += __shfl_xor(lower, o, 64);
-    if (lane == 0) {
-        S.cnt_mm[l] = cnt;
-        S.cnt_lo[l] = lower;   // i < j pairs of this row; summed after the build
-        if (cnt > S.cap_mm) {
-            atomicMax(&S.flags->overflow_mm, cnt);
-            S.flags->overflow_step = S.flags->steps;   // the same value from every writer
+        hit = hit && j != L.i;
+        if (L.surf && hit && j >= S.n_liquid_sites) hit = !constrained(S, L.i, j);
+        return hit;
+    };
+    auto test_fz = [&](const BuildLead &L, const float4 &v, int j, int lay, bool ok, bool mi, int wn) {
+        int dz = lay - L.wz;
+        if (zper) dz = (dz % wn + wn) % wn;
+        const bool mine = ok && (zper ? (wn <= 3 || dz <= 1 || dz == wn - 1) : (dz >= -1 && dz <= 1));
+        const float d2 = d2f(L, v, mi);
+        bool hit = mine && d2 < r2_in;
+        const bool exact = mine && !(d2 < r2_in) && !(d2 >= r2_out);
+        if (__ballot(exact)) {
+            if (exact) hit = list_d2_xyz(S, L.me.x, L.me.y, L.me.z, S.x[j], S.y[j], S.z[j]) < S.rskinsq;
         }
-    }
-    // frozen partners: same x,y grid, frozen z grid built once
-    int cntw = 0;
-    if (S.n_wall > 0) {
-        int *wrow = S.nbr_mw + (size_t)l * S.cap_mw;
-        int wcz;
-        if (S.g.zper) {
-            wcz = wrap_cell(zi, S.lz, S.lzi, S.wncz);
-        } else {
-            double czd = floor((zi - S.wz0) / S.wcsz);
-            czd = fmin(fmax(czd, -2.0), (double)S.wncz + 1.0);  // far from the crystal: no cell in reach
-            wcz = (int)czd;
+        if (L.surf && hit) hit = !constrained(S, L.i, j);
+        return hit;
+    };
+    for (int item = bid; item < nc * BUILD_SPLIT; item += nblk) {
+        const int c = item / BUILD_SPLIT, split = item - c * BUILD_SPLIT;
+        const int b = S.cell_start[c], e = S.cell_start[c + 1];
+        const int nrounds = (e - b + BUILD_LEADS - 1) / BUILD_LEADS;
+        if (split >= nrounds) continue;   // uniform over the block
+        const int cx = c % ncx, cy = (c / ncx) % ncy, cz = c / (ncx * ncy);
+        // moving stencil, order (oz, oy, ox)
+        if (threadIdx.x == 0) sm.mv.n_st = 0;
+        __syncthreads();
+        if (threadIdx.x < 32) {
+            const int k = threadIdx.x, per = nyo * nxo;
+            bool ok = k < nzo * per;
+            int cc = 0;
+            float sx = 0.f, sy = 0.f, sz = 0.f;
+            if (ok) {
+                const int oz = k / per, oy = (k / nxo) % nyo, ox = k % nxo;
+                int z2;
+                if (zper) z2 = nbr_cell(cz, oz, ncz, nzo, sz, lzf);
+                else { z2 = cz + oz - 1; ok = z2 >= 0 && z2 < ncz; }
+                const int y2 = nbr_cell(cy, oy, ncy, nyo, sy, lyf);
+                const int x2 = nbr_cell(cx, ox, ncx, nxo, sx, lxf);
+                cc = (z2 * ncy + y2) * ncx + x2;
+            }
+            // compact the valid cells in order
+            const unsigned long long m = __ballot(ok);
+            if (ok) {
+                const int slot = __popcll(m & ((1ull << k) - 1ull));
+                const int cb = S.cell_start[cc];
+                sm.mv.b[slot] = cb;
+                sm.mv.n[slot] = S.cell_start[cc + 1] - cb;
+                sm.mv.layer[slot] = 0;
+                sm.mv.sx[slot] = mv_shift ? sx : 0.f;
+                sm.mv.sy[slot] = mv_shift ? sy : 0.f;
+                sm.mv.sz[slot] = mv_shift ? sz : 0.f;
+            }
+            if (k == 0) sm.mv.n_st = __popcll(m);
         }
-        const int nwz = S.g.zper ? min(3, S.wncz) : 3;
-        for (int oz = 0; oz < nwz; oz++) {
-            int z2;
-            if (S.g.zper) z2 = (S.wncz >= 3) ? (wcz + oz - 1 + S.wncz) % S.wncz : oz;
-            else { z2 = wcz + oz - 1; if (z2 < 0 || z2 >= S.wncz) continue; }
-            for (int oy = 0; oy < nyo; oy++) {
-                const int y2 = (nyo == 3) ? (cy + oy - 1 + ncy) % ncy : oy;
-                for (int ox = 0; ox < nxo; ox++) {
-                    const int x2 = (nxo == 3) ? (cx + ox - 1 + ncx) % ncx : ox;
-                    const int c = (z2 * ncy + y2) * ncx + x2;
-                    const int b = S.wcell_start[c], e = S.wcell_start[c + 1];
-                    for (int p0 = b; p0 < e; p0 += 64) {
-                        const int p = p0 + lane;
-                        bool hit = false;
-                        int j = 0;
-                        if (p < e) {
-                            j = S.n_move + S.wcell_items[p];
-                            if (!(i_surf && constrained(S, i, j))) hit = list_d2(S, i, j) < S.rskinsq;
+        __syncthreads();
+        stencil_prefix(sm.mv);
+        const int n_st = sm.mv.n_st;
+        const int total = n_st > 0 ? sm.mv.inc[n_st - 1] : 0;
+        for (int round = split; round < nrounds; round += BUILD_SPLIT) {
+            const int r0 = b + round * BUILD_LEADS;
+            const int nl = min(BUILD_LEADS, e - r0);
+            if (threadIdx.x == 0) { sm.wlo = 0x7fffffff; sm.whi = -0x7fffffff; }
+            __syncthreads();
+            if (threadIdx.x < nl) {
+                const double4 v = S.cell_xyz[r0 + threadIdx.x];
+                sm.lead_xyz[threadIdx.x] = v;
+                sm.lead_l[threadIdx.x] = S.cell_items[r0 + threadIdx.x];
+                sm.lead_cnt[threadIdx.x] = 0;
+                sm.lead_low[threadIdx.x] = 0;
+                int wcz = 0;
+                if (S.n_wall > 0) {
+                    if (zper) {
+                        wcz = wrap_cell(v.z, S.lz, S.lzi, S.wncz);
+                    } else {
+                        double czd = floor((v.z - S.wz0) / S.wcsz);
+                        czd = fmin(fmax(czd, -2.0), (double)S.wncz + 1.0);  // far from the crystal: no cell in reach
+                        wcz = (int)czd;
+                    }
+                    atomicMin(&sm.wlo, wcz);
+                    atomicMax(&sm.whi, wcz);
+                }
+                sm.lead_wz[threadIdx.x] = wcz;
+            }
+            __syncthreads();
+            // moving candidates
+            for (int q0 = 0; q0 < total; q0 += BUILD_CH) {
+                const int nch = min(BUILD_CH, total - q0);
+                for (int t = threadIdx.x; t < nch; t += BUILD_THREADS) {
+                    int k;
+                    const int p = stencil_locate(sm.mv, q0 + t, k);
+                    sm.cand[t] = stage(S.cell_xyz[p], sm.mv.sx[k], sm.mv.sy[k], sm.mv.sz[k]);
+                }
+                __syncthreads();
+                for (int ld0 = w; ld0 < nl; ld0 += 2 * BUILD_WAVES) {
+                    const int ld1 = ld0 + BUILD_WAVES;
+                    const bool two = ld1 < nl;
+                    BuildLead A = load_lead(ld0, false);
+                    BuildLead B = load_lead(two ? ld1 : ld0, false);
+                    for (int s0 = 0; s0 < nch; s0 += 64) {
+                        const int a = s0 + lane;
+                        const bool ok = a < nch;
+                        const float4 v = sm.cand[ok ? a : 0];
+                        const int j = __float_as_int(v.w);
+                        const bool ha = test_mv(A, v, j, ok, !mv_shift);
+                        append(A, ha, j, S.cap_mm);
+                        if (two) {
+                            const bool hb = test_mv(B, v, j, ok, !mv_shift);
+                            append(B, hb, j, S.cap_mm);
                         }
-                        const unsigned long long msk = __ballot(hit);
-                        if (hit) {
-                            const int pos = cntw + __popcll(msk & ((1ull << lane) - 1ull));
-                            if (pos < S.cap_mw) wrow[pos] = j;
-                        }
-                        cntw += __popcll(msk);
+                    }
+                    for (int o = 32; o > 0; o >>= 1) A.lower += __shfl_xor(A.lower, o, 64);
+                    for (int o = 32; o > 0; o >>= 1) B.lower += __shfl_xor(B.lower, o, 64);
+                    if (lane == 0) {
+                        sm.lead_cnt[ld0] = A.cnt; sm.lead_low[ld0] += A.lower;
+                        if (two) { sm.lead_cnt[ld1] = B.cnt; sm.lead_low[ld1] += B.lower; }
                     }
                 }
+                __syncthreads();
             }
-        }
-    }
-    if (lane == 0) {
-        S.cnt_mw[l] = cntw;
-        if (cntw > S.cap_mw) {
-            atomicMax(&S.flags->overflow_mw, cntw);
-            S.flags->overflow_step = S.flags->steps;
+            // moving rows done: counts and overflow
+            if (threadIdx.x < nl) {
+                const int l = sm.lead_l[threadIdx.x], cnt = sm.lead_cnt[threadIdx.x];
+                S.cnt_mm[l] = cnt;
+                S.cnt_lo[l] = sm.lead_low[threadIdx.x];   // i < j pairs of this row; summed after the build
+                if (cnt > S.cap_mm) {
+                    atomicMax(&S.flags->overflow_mm, cnt);
+                    S.flags->overflow_step = S.flags->steps;   // the same value from every writer
+                }
+                sm.lead_cnt[threadIdx.x] = 0;   // frozen partners next
+            }
+            if (S.n_wall == 0) {
+                if (threadIdx.x < nl) S.cnt_mw[sm.lead_l[threadIdx.x]] = 0;
+                __syncthreads();
+                continue;
+            }
+            // frozen stencil: the layers the round's leads reach, then (oy, ox)
+            const int wlo = sm.wlo, whi = sm.whi, wn = S.wncz;
+            int zl, nlay;
+            bool all_layers = false;
+            if (zper) {
+                all_layers = wn <= 3 || whi - wlo + 3 >= wn;
+                zl = all_layers ? 0 : wlo - 1;
+                nlay = all_layers ? wn : whi - wlo + 3;
+            } else {
+                zl = max(0, wlo - 1);
+                nlay = max(0, min(wn - 1, whi + 1) - zl + 1);
+            }
+            // frozen z images are per pair with a periodic z; x, y as the moving stencil
+            const bool fz_shift = nxo == 3 && nyo == 3 && !zper;
+            __syncthreads();
+            if (threadIdx.x < 64) {
+                const int k = threadIdx.x, per = nyo * nxo;
+                const bool ok = k < nlay * per;
+                if (ok) {
+                    const int oz = k / per, oy = (k / nxo) % nyo, ox = k % nxo;
+                    float sx, sy;
+                    const int z2 = zper ? ((zl + oz) % wn + wn) % wn : zl + oz;
+                    const int y2 = nbr_cell(cy, oy, ncy, nyo, sy, lyf);
+                    const int x2 = nbr_cell(cx, ox, ncx, nxo, sx, lxf);
+                    const int cc = (z2 * ncy + y2) * ncx + x2;
+                    const int cb = S.wcell_start[cc];
+                    sm.fz.b[k] = cb;
+                    sm.fz.n[k] = S.wcell_start[cc + 1] - cb;
+                    sm.fz.layer[k] = z2;
+                    sm.fz.sx[k] = fz_shift ? sx : 0.f;
+                    sm.fz.sy[k] = fz_shift ? sy : 0.f;
+                    sm.fz.sz[k] = 0.f;
+                }
+                if (k == 0) sm.fz.n_st = min(64, nlay * per);
+            }
+            __syncthreads();
+            stencil_prefix(sm.fz);
+            const int n_wst = sm.fz.n_st;
+            const int wtotal = n_wst > 0 ? sm.fz.inc[n_wst - 1] : 0;
+            for (int q0 = 0; q0 < wtotal; q0 += BUILD_CH) {
+                const int nch = min(BUILD_CH, wtotal - q0);
+                for (int t = threadIdx.x; t < nch; t += BUILD_THREADS) {
+                    int k;
+                    const int p = stencil_locate(sm.fz, q0 + t, k);
+                    sm.cand[t] = stage(S.wcell_xyz[p], sm.fz.sx[k], sm.fz.sy[k], sm.fz.sz[k]);
+                    sm.cand_layer[t] = sm.fz.layer[k];
+                }
+                __syncthreads();
+                for (int ld0 = w; ld0 < nl; ld0 += 2 * BUILD_WAVES) {
+                    const int ld1 = ld0 + BUILD_WAVES;
+                    const bool two = ld1 < nl;
+                    BuildLead A = load_lead(ld0, true);
+                    BuildLead B = load_lead(two ? ld1 : ld0, true);
+                    for (int s0 = 0; s0 < nch; s0 += 64) {
+                        const int a = s0 + lane;
+                        const bool ok = a < nch;
+                        const float4 v = sm.cand[ok ? a : 0];
+                        const int j = __float_as_int(v.w), lay = sm.cand_layer[ok ? a : 0];
+                        const bool ha = test_fz(A, v, j, lay, ok, !fz_shift, wn);
+                        append(A, ha, j, S.cap_mw);
+                        if (two) {
+                            const bool hb = test_fz(B, v, j, lay, ok, !fz_shift, wn);
+                            append(B, hb, j, S.cap_mw);
+                        }
+                    }
+                    if (lane == 0) {
+                        sm.lead_cnt[ld0] = A.cnt;
+                        if (two) sm.lead_cnt[ld1] = B.cnt;
+                    }
+                }
+                __syncthreads();
+            }
+            if (threadIdx.x < nl) {
+                const int cntw = sm.lead_cnt[threadIdx.x];
+                S.cnt_mw[sm.lead_l[threadIdx.x]] = cntw;
+                if (cntw > S.cap_mw) {
+                    atomicMax(&S.flags->overflow_mw, cntw);
+                    S.flags->overflow_step = S.flags->steps;
+                }
+            }
+            __syncthreads();
         }
     }
 }
 
-__global__ __launch_bounds__(256) void k_build(DevSys S) {
+__global__ __launch_bounds__(BUILD_THREADS) void k_build(DevSys S) {
     build_body(S, blockIdx.x, gridDim.x);
 }
 
@@ -448,14 +798,14 @@ __global__ __launch_bounds__(1024) void k_cell_scan_build_b(const RepArgs *__res
 __global__ __launch_bounds__(256) void k_cell_fill_b(const RepArgs *__restrict__ RA, const int2 *__restrict__ map) {
     const int2 m = map[blockIdx.x];
     const RepArgs &r = RA[m.x];
-    cell_fill_body(r.lead_cell, r.S.n_lead, r.S.cell_cursor, r.cell_items2, r.S.flags, 1, m.y, r.lead_blocks);
+    cell_fill_body(r.S, r.lead_cell, r.items_tmp, r.xyz_tmp, m.y, r.lead_blocks);
 }
-__global__ __launch_bounds__(64) void k_cell_sort_b(const RepArgs *__restrict__ RA, const int2 *__restrict__ map) {
+__global__ __launch_bounds__(256) void k_cell_order_b(const RepArgs *__restrict__ RA, const int2 *__restrict__ map) {
     const int2 m = map[blockIdx.x];
     const RepArgs &r = RA[m.x];
-    cell_sort_body(r.S.cell_start, r.cell_items2, r.S.cell_items, r.S.flags, 1, r.ncells_xy, 0, m.y, r.sort_blocks);
+    cell_order_body(r.S, r.items_tmp, r.xyz_tmp, m.y, r.order_blocks);
 }
-__global__ __launch_bounds__(256) void k_build_b(const RepArgs *__restrict__ RA, const int2 *__restrict__ map) {
+__global__ __launch_bounds__(BUILD_THREADS) void k_build_b(const RepArgs *__restrict__ RA, const int2 *__restrict__ map) {
     const int2 m = map[blockIdx.x];
     build_body(RA[m.x].S, m.y, RA[m.x].build_blocks);
 }
