@@ -219,15 +219,16 @@ void mdf_batch_destroy(mdf_batch *b);
    state: lanes per lead (8/16/32) and loop form of the pair kernel, blocks per launch of the
    structure-factor and back-projection kernels.  Each candidate runs reps + 1 times; the
    fastest is kept for later steps.  chosen (optional, 4 entries) receives lanes per lead,
-   rolled (0/1), structure-factor blocks and back-projection blocks.  Device results of the
+   pair loop form (0: each lead's partners inside the cut-off compacted before the pair
+   arithmetic, 1: row-strided), structure-factor blocks and back-projection blocks.  Device results of the
    last step are overwritten: call between steps, before reading them again.  No
    counterpart in the reference (its loops have no launch shape). */
 int mdf_autotune(mdf_system *sys, int32_t reps, int32_t *chosen);
 
 /* Set the launch shapes mdf_autotune chooses, by hand: pair lanes per lead (8, 16, 32),
-   rolled site-pair loop (0/1), structure-factor and back-projection blocks per launch.
-   No counterpart in the reference. */
-int mdf_set_launch_shapes(mdf_system *sys, int32_t pair_lanes, int32_t pair_rolled, int32_t sf_blocks,
+   pair loop form (0 compacted, 1 row-strided), structure-factor and back-projection blocks
+   per launch.  No counterpart in the reference. */
+int mdf_set_launch_shapes(mdf_system *sys, int32_t pair_lanes, int32_t pair_strided, int32_t sf_blocks,
                           int32_t bp_blocks);
 
 /* Which reciprocal-sum kernels the handle runs: 1 = on the matrix cores (the FP64 form's

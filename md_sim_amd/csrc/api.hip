@@ -41,7 +41,7 @@ __global__ void k_snapshot(DevSys S);
 __global__ void k_wall_cells(DevSys S, int *wcell_of, int ncz, double z0, double csz);
 __global__ void k_count_cells(const int *item_cell, int n_items, int *cnt);
 __global__ void k_zero_int(int *p, int n);
-template <typename R, int G, bool VIR, bool ACC>
+template <typename R, int G, bool VIR, bool ACC, bool STRIDED>
 __global__ void k_pair(DevSys S);
 template <typename R, int NS>
 __global__ void k_sf(const double *, const double *, const double *, const double *, int, double, double, double,
@@ -137,7 +137,7 @@ __global__ void k_cell_order_b(const RepArgs *, const int2 *);
 __global__ void k_build_b(const RepArgs *, const int2 *);
 __global__ void k_snapshot_b(const RepArgs *, const int2 *);
 __global__ void k_flags_b(const RepArgs *, int, DevFlags *);
-template <typename R, int G, bool VIR, bool ACC>
+template <typename R, int G, bool VIR, bool ACC, bool STRIDED>
 __global__ void k_pair_b(const RepArgs *, const int2 *);
 template <typename R>
 __global__ void k_sf_b(const RepArgs *, const int2 *);
@@ -326,7 +326,7 @@ struct mdf_system {
     hipGraphExec_t graph[NRING] = {};
     bool use_graphs = true;
     hipEvent_t in_event = nullptr;   // orders a device-input step after the inputs' producer
-    bool pair_rolled = false;   // site pairs of a molecule pair as a rolled loop (fewer registers)
+    bool pair_strided = false;  // pair loop form: row-strided (true) or cut-off compacted (false)
     int pair_g = 16;            // lanes per lead in the pair kernel (8, 16 or 32)
     // launch-shape targets of the reciprocal kernels (blocks per launch; the row groups follow)
     int sf_target_blocks = 4096, bp_target_blocks = 1024;
@@ -831,12 +831,12 @@ static RecipArgs recip_args(const mdf_system *s) {
         }                                                                                                       \
     } while (0)
 
-template <int G>
+template <int G, bool ST>
 static void launch_pair_g(mdf_system *s, bool f32, bool vir, bool acc) {
     const int blocks = std::max(1, (s->n_lead + (256 / G) - 1) / (256 / G));
     hipStream_t st = s->stream;
     DevSys &S = s->S;
-#define MDF_PAIR(R, V, A) MDF_TLAUNCH(s, KT_PAIR, (k_pair<R, G, V, A>), blocks, 256, 0, st, S)
+#define MDF_PAIR(R, V, A) MDF_TLAUNCH(s, KT_PAIR, (k_pair<R, G, V, A, ST>), blocks, 256, 0, st, S)
     if (f32) {
         if (vir) { if (acc) MDF_PAIR(float, true, true); else MDF_PAIR(float, true, false); }
         else { if (acc) MDF_PAIR(float, false, true); else MDF_PAIR(float, false, false); }
@@ -847,15 +847,15 @@ static void launch_pair_g(mdf_system *s, bool f32, bool vir, bool acc) {
 #undef MDF_PAIR
 }
 
-// real-space pair kernel with the handle's lanes per lead and erfc form (the rolled loop
-// form of the launch shapes compiles to the same code and keeps the same results)
+// real-space pair kernel with the handle's lanes per lead, loop form and erfc form
 static void launch_pair(mdf_system *s) {
     const bool f32 = s->par.precision == MDF_FP32_ACC64, vir = s->par.virial != 0;
     const bool acc = s->par.erfc_mode == MDF_ERFC_ACCURATE;
+    const bool st = s->pair_strided;
     switch (s->pair_g) {
-        case 8: launch_pair_g<8>(s, f32, vir, acc); break;
-        case 32: launch_pair_g<32>(s, f32, vir, acc); break;
-        default: launch_pair_g<16>(s, f32, vir, acc); break;
+        case 8: if (st) launch_pair_g<8, true>(s, f32, vir, acc); else launch_pair_g<8, false>(s, f32, vir, acc); break;
+        case 32: if (st) launch_pair_g<32, true>(s, f32, vir, acc); else launch_pair_g<32, false>(s, f32, vir, acc); break;
+        default: if (st) launch_pair_g<16, true>(s, f32, vir, acc); else launch_pair_g<16, false>(s, f32, vir, acc); break;
     }
 }
 
@@ -1436,8 +1436,8 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
     {
         const char *g = getenv("MDF_GRAPHS");
         if (g && g[0] == '0') s->use_graphs = false;
-        const char *pr = getenv("MDF_PAIR_ROLLED");
-        if (pr && pr[0] == '1') s->pair_rolled = true;
+        const char *pr = getenv("MDF_PAIR_STRIDED");
+        if (pr && pr[0] == '1') s->pair_strided = true;
         const char *sm = getenv("MDF_SF_MMA");
         if (sm && sm[0] == '0') s->sf_mma_allowed = false;
         const char *bm = getenv("MDF_BP_MMA");
@@ -2096,16 +2096,16 @@ int mdf_autotune(mdf_system *s, int32_t reps, int32_t *chosen) {
     };
     float best = 1e30f, t = 0.0f;
     int best_g = s->pair_g;
-    bool best_rolled = s->pair_rolled;
+    bool best_strided = s->pair_strided;
     for (int g : {8, 16, 32})
         for (int ro = 0; ro < 2; ro++) {
             s->pair_g = g;
-            s->pair_rolled = ro != 0;
+            s->pair_strided = ro != 0;
             if ((rc = timed([&] { launch_pair(s); }, &t)) != MDF_OK) goto done;
-            if (t < best) { best = t; best_g = g; best_rolled = ro != 0; }
+            if (t < best) { best = t; best_g = g; best_strided = ro != 0; }
         }
     s->pair_g = best_g;
-    s->pair_rolled = best_rolled;
+    s->pair_strided = best_strided;
     {
         best = 1e30f;
         int best_sf = s->sf_target_blocks;
@@ -2129,7 +2129,7 @@ int mdf_autotune(mdf_system *s, int32_t reps, int32_t *chosen) {
     rc = apply_geometry(s);
     if (chosen) {
         chosen[0] = s->pair_g;
-        chosen[1] = s->pair_rolled ? 1 : 0;
+        chosen[1] = s->pair_strided ? 1 : 0;
         chosen[2] = s->sf_target_blocks;
         chosen[3] = s->bp_target_blocks;
     }
@@ -2139,7 +2139,7 @@ done:
     return rc;
 }
 
-int mdf_set_launch_shapes(mdf_system *s, int32_t pair_lanes, int32_t pair_rolled, int32_t sf_blocks,
+int mdf_set_launch_shapes(mdf_system *s, int32_t pair_lanes, int32_t pair_strided, int32_t sf_blocks,
                           int32_t bp_blocks) {
     if (!s) return fail(MDF_ERR_ARG, "null handle");
     if (pair_lanes != 8 && pair_lanes != 16 && pair_lanes != 32)
@@ -2149,7 +2149,7 @@ int mdf_set_launch_shapes(mdf_system *s, int32_t pair_lanes, int32_t pair_rolled
     HIPCHK(hipStreamSynchronize(s->stream));
     drain_all(s);
     s->pair_g = pair_lanes;
-    s->pair_rolled = pair_rolled != 0;
+    s->pair_strided = pair_strided != 0;
     s->sf_target_blocks = sf_blocks;
     s->bp_target_blocks = bp_blocks;
     return apply_geometry(s);
@@ -2801,6 +2801,7 @@ struct mdf_batch {
     hipEvent_t in_event = nullptr;
     bool pending = false;
     int pair_g = 16;
+    bool pair_strided = false;
     bool f32 = false, vir = false, acc = false;
 };
 
@@ -2922,15 +2923,20 @@ static void batch_launch(mdf_batch *b) {
     {
         const int2 *mp = b->maps[BK_PAIR].p;
         const int g = grid(BK_PAIR);
-#define MDF_PB(RT, G)                                                                    \
+#define MDF_PB2(RT, G, ST)                                                               \
     do {                                                                                 \
         if (b->acc) {                                                                    \
-            if (b->vir) k_pair_b<RT, G, true, true><<<g, 256, 0, st>>>(R, mp);           \
-            else k_pair_b<RT, G, false, true><<<g, 256, 0, st>>>(R, mp);                 \
+            if (b->vir) k_pair_b<RT, G, true, true, ST><<<g, 256, 0, st>>>(R, mp);       \
+            else k_pair_b<RT, G, false, true, ST><<<g, 256, 0, st>>>(R, mp);             \
         } else {                                                                         \
-            if (b->vir) k_pair_b<RT, G, true, false><<<g, 256, 0, st>>>(R, mp);          \
-            else k_pair_b<RT, G, false, false><<<g, 256, 0, st>>>(R, mp);                \
+            if (b->vir) k_pair_b<RT, G, true, false, ST><<<g, 256, 0, st>>>(R, mp);      \
+            else k_pair_b<RT, G, false, false, ST><<<g, 256, 0, st>>>(R, mp);            \
         }                                                                                \
+    } while (0)
+#define MDF_PB(RT, G)                                  \
+    do {                                               \
+        if (b->pair_strided) MDF_PB2(RT, G, true);     \
+        else MDF_PB2(RT, G, false);                    \
     } while (0)
         if (b->f32) {
             if (b->pair_g == 8) MDF_PB(float, 8); else if (b->pair_g == 32) MDF_PB(float, 32); else MDF_PB(float, 16);
@@ -2938,6 +2944,7 @@ static void batch_launch(mdf_batch *b) {
             if (b->pair_g == 8) MDF_PB(double, 8); else if (b->pair_g == 32) MDF_PB(double, 32); else MDF_PB(double, 16);
         }
 #undef MDF_PB
+#undef MDF_PB2
     }
     if (grid(BK_SF)) {
         if (b->f32) k_sf_b<float><<<grid(BK_SF), SF_BLOCK, 0, st>>>(R, b->maps[BK_SF].p);
@@ -2992,6 +2999,7 @@ int mdf_batch_create(mdf_system *const *systems, int32_t n, mdf_batch **out) {
     b->vir = s0->par.virial != 0;
     b->acc = s0->par.erfc_mode == MDF_ERFC_ACCURATE;
     b->pair_g = s0->pair_g;
+    b->pair_strided = s0->pair_strided;
     int rc = MDF_OK;
     hipError_t e = hipSetDevice(b->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
@@ -3152,14 +3160,19 @@ int mdf_batch_autotune(mdf_batch *b, int32_t reps, int32_t *chosen) {
     };
     float best = 1e30f, t = 0.0f;
     int sf_best = 8192, bp_best = 2048, g_best = b->pair_g;
+    bool st_best = b->pair_strided;
     if ((rc = reshape(sf_best, bp_best)) != MDF_OK) goto done;
     for (int g : {8, 16, 32}) {
         b->pair_g = g;
         if ((rc = batch_upload(b, true)) != MDF_OK) goto done;
-        if ((rc = timed(&t)) != MDF_OK) goto done;
-        if (t < best) { best = t; g_best = g; }
+        for (int ro = 0; ro < 2; ro++) {
+            b->pair_strided = ro != 0;
+            if ((rc = timed(&t)) != MDF_OK) goto done;
+            if (t < best) { best = t; g_best = g; st_best = ro != 0; }
+        }
     }
     b->pair_g = g_best;
+    b->pair_strided = st_best;
     best = 1e30f;
     for (int tb : {4096, 8192, 16384, 32768}) {
         if ((rc = reshape(tb, bp_best)) != MDF_OK) goto done;

@@ -25,7 +25,7 @@ __device__ __forceinline__ double rsqrt_nr(double x) {
     double y = __builtin_amdgcn_rsq(x);
     double h = 0.5 * x;
 #pragma unroll
-    for (int it = 0; it < 3; it++) {
+    for (int it = 0; it < 2; it++) {
         double r = __builtin_fma(-h * y, y, 0.5);   // 0.5 - x y^2 / 2
         y = __builtin_fma(y, r, y);                 // y (1.5 - x y^2 / 2)
     }
@@ -152,23 +152,45 @@ __device__ __forceinline__ void acc_at(double (&a)[N], int c, double v) {
     for (int k = 0; k < N; k++) a[k] += (c == k) ? v : 0.0;
 }
 
-// G lanes share one lead; each lane walks every G-th partner of the lead's rows.  Partner
-// molecules come as packed records (three 32-byte loads) and their vdW parameters from a
-// small LDS table indexed by (lead slot, partner class), so a partner costs one row entry,
-// one meta word and one record.  The next partner's record is loaded while the current pair
-// is evaluated.  Energies per partner class, the virial and the wall forces stay in
-// registers; the G lanes reduce them with shuffles at the end.  The arithmetic and its order
-// are those of the reference's loops (lead-pair minimum image and cut-off test in FP64
-// without contraction).
-template <typename R, int G, bool VIR, bool ACC>
+// the lead-lead minimum image and squared distance of the reference's loops (x, y always;
+// z per variant), FP64 without contraction
+__device__ __forceinline__ double lead_d2(const DevSys &S, double xi, double yi, double zi, double xj, double yj,
+                                          double zj, double &dx, double &dy, double &dz) {
+    dx = __dsub_rn(xi, xj); dx = __dsub_rn(dx, __dmul_rn(S.lx, anint_d(__dmul_rn(dx, S.lxi))));
+    dy = __dsub_rn(yi, yj); dy = __dsub_rn(dy, __dmul_rn(S.ly, anint_d(__dmul_rn(dy, S.lyi))));
+    dz = __dsub_rn(zi, zj);
+    if (S.zper_force) dz = __dsub_rn(dz, __dmul_rn(S.lz, anint_d(__dmul_rn(dz, S.lzi))));
+    return __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
+}
+
+constexpr int PAIR_SEG = 128;   // row entries per compaction segment
+
+// G lanes share one lead.  The lead's moving row goes in segments of PAIR_SEG entries: first
+// the G lanes test the lead-lead cut-off of the segment's partners (one 32-byte record load
+// each) and append the partners inside it to the group's queue in LDS (a ballot prefix
+// count, so the queue keeps the row's order); then the G lanes evaluate the queue, every
+// lane busy, each partner's record loaded while the one before is evaluated.  STRIDED
+// skips the compaction: lane sub takes row entries sub, sub + G, ... directly, and a lane
+// whose partner is outside the cut-off idles through that pair.  All nine site
+// pairs of the two molecules are evaluated without branches: records carry zero charges
+// (and zero offsets) past a molecule's sites, which add exact zeros.  Partner vdW parameters
+// come from a small LDS table indexed by (lead slot, partner class).  Energies per partner
+// class, the virial and the wall forces stay in registers; the G lanes reduce them with
+// shuffles at the end.  The cut-off test and its arithmetic order are the reference's
+// (lead-pair minimum image and cut-off test in FP64 without contraction).
+template <typename R, int G, bool VIR, bool ACC, bool STRIDED>
 __device__ __forceinline__ void pair_body(const DevSys &S, int bid, int nblk) {
     __shared__ double s_tab[MAX_LEAD_TYPES * NCLS * PTAB_W];
+    __shared__ int s_q[256 / G][PAIR_SEG];
     for (int k = threadIdx.x; k < MAX_LEAD_TYPES * NCLS * PTAB_W; k += blockDim.x) s_tab[k] = S.ptab[k];
     __syncthreads();
     const int tid = threadIdx.x;
-    const int gid = bid * (blockDim.x / G) + tid / G;
+    const int grp = tid / G;
+    const int gid = bid * (blockDim.x / G) + grp;
     const int sub = tid % G;
+    const int gbase = (tid & 63) - sub;   // the group's first lane in the wave
     const bool active = gid < S.n_lead;
+    int *queue = s_q[grp];
 
     double f[3][3];
     double vir[6];
@@ -199,65 +221,37 @@ __device__ __forceinline__ void pair_body(const DevSys &S, int bid, int nblk) {
         // ---- moving partners (loop A, src/forcem.f90:288-403) ----
         const int n = min(S.cnt_mm[gid], S.cap_mm);
         const int *row = S.nbr_mm + (size_t)gid * S.cap_mm;
-        int k = sub;
-        int jn = k < n ? row[k] : i;
-        int mn = S.pmeta[jn];
-        MolRec rn = load_rec(S.prec, jn);
-#pragma unroll 1
-        for (; k < n; k += G) {
-            const int mj = mn;
-            const MolRec pj = rn;
-            if (k + G < n) {   // next partner in flight during this pair
-                jn = row[k + G];
-                mn = S.pmeta[jn];
-                rn = load_rec(S.prec, jn);
-            }
-            // lead-lead minimum image as the reference (x,y always; z per variant), no FMA
-            double dx = __dsub_rn(xi, pj.x); dx = __dsub_rn(dx, __dmul_rn(S.lx, anint_d(__dmul_rn(dx, S.lxi))));
-            double dy = __dsub_rn(yi, pj.y); dy = __dsub_rn(dy, __dmul_rn(S.ly, anint_d(__dmul_rn(dy, S.lyi))));
-            double dz = __dsub_rn(zi, pj.z);
-            if (S.zper_force) dz = __dsub_rn(dz, __dmul_rn(S.lz, anint_d(__dmul_rn(dz, S.lzi))));
-            const double rsq = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
-            if (rsq > S.rcutsq) continue;
-            const int cj = mj & 15, nsi_j = mj >> 4;
+        const double4 *prec4 = reinterpret_cast<const double4 *>(S.prec);
+        // one molecule pair: lead pair (vdW + Coulomb) and the other eight site pairs
+        auto mol_pair = [&](int mj, const MolRec &pj) {
+            double dx, dy, dz;
+            const double rsq = lead_d2(S, xi, yi, zi, pj.x, pj.y, pj.z, dx, dy, dz);
+            if (STRIDED && rsq > S.rcutsq) return;
+            const int cj = mj & 15;
             const double *pt = tab_i + cj * PTAB_W;
             R evd, fvd;
             vdw_eval<R>((int)pt[0], (R)rsq, (R)pt[1], (R)pt[2], (R)pt[3], (R)pt[4], (R)pt[5], evd, fvd);
             double e_c = 0.0;
-            {   // lead pair: vdW + Coulomb
-                R ecl, fcl;
-                cl.eval((R)rsq, (R)(qi[0] * pj.q[0]), ecl, fcl);
-                e_c += (double)ecl;
-                const double fij = (double)fvd + (double)fcl;
-                f[0][0] += dx * fij; f[0][1] += dy * fij; f[0][2] += dz * fij;
-                if (VIR) {
-                    const double hf = 0.5 * fij;
-                    vir[0] += dx * dx * hf; vir[1] += dx * dy * hf; vir[2] += dx * dz * hf;
-                    vir[3] += dy * dy * hf; vir[4] += dy * dz * hf; vir[5] += dz * dz * hf;
-                }
-            }
-            // the other site pairs of the two molecules, uncut (src/forcem.f90:360-402)
-            if (nsi_i > 1 || nsi_j > 1) {
 #pragma unroll
-                for (int b = 0; b < 3; b++) {
-                    if (b >= nsi_j) break;
+            for (int b = 0; b < 3; b++) {
 #pragma unroll
-                    for (int a = 0; a < 3; a++) {
-                        if (a >= nsi_i) break;
-                        if (a == 0 && b == 0) continue;
-                        const double xd = (dx + me.rx[a]) - pj.rx[b], yd = (dy + me.ry[a]) - pj.ry[b],
-                                     zd = (dz + me.rz[a]) - pj.rz[b];
-                        const double r2 = xd * xd + yd * yd + zd * zd;
-                        R es, fs;
-                        cl.eval((R)r2, (R)(qi[a] * pj.q[b]), es, fs);
-                        e_c += (double)es;
-                        const double fd = (double)fs;
-                        f[a][0] += xd * fd; f[a][1] += yd * fd; f[a][2] += zd * fd;
-                        if (VIR) {
-                            const double hf = 0.5 * fd;
-                            vir[0] += xd * xd * hf; vir[1] += xd * yd * hf; vir[2] += xd * zd * hf;
-                            vir[3] += yd * yd * hf; vir[4] += yd * zd * hf; vir[5] += zd * zd * hf;
-                        }
+                for (int a = 0; a < 3; a++) {
+                    double xd = dx, yd = dy, zd = dz, r2 = rsq;
+                    if (a != 0 || b != 0) {   // (0, 0) is the lead pair, at the cut-off test's rsq
+                        xd = (dx + me.rx[a]) - pj.rx[b];
+                        yd = (dy + me.ry[a]) - pj.ry[b];
+                        zd = (dz + me.rz[a]) - pj.rz[b];
+                        r2 = xd * xd + yd * yd + zd * zd;
+                    }
+                    R es, fs;
+                    cl.eval((R)r2, (R)(qi[a] * pj.q[b]), es, fs);
+                    e_c += (double)es;
+                    const double fd = (a == 0 && b == 0) ? (double)fvd + (double)fs : (double)fs;
+                    f[a][0] += xd * fd; f[a][1] += yd * fd; f[a][2] += zd * fd;
+                    if (VIR) {
+                        const double hf = 0.5 * fd;
+                        vir[0] += xd * xd * hf; vir[1] += xd * yd * hf; vir[2] += xd * zd * hf;
+                        vir[3] += yd * yd * hf; vir[4] += yd * zd * hf; vir[5] += zd * zd * hf;
                     }
                 }
             }
@@ -270,6 +264,77 @@ __device__ __forceinline__ void pair_body(const DevSys &S, int bid, int nblk) {
             acc_at<MAX_LEAD_TYPES>(ech, cj, 0.5 * e_c);
 #pragma unroll
             for (int c = 0; c < MAX_LEAD_TYPES; c++) { ev[c] = evh[c]; ec[c] = ech[c]; }
+        };
+        if constexpr (STRIDED) {
+            // lane sub takes row entries sub, sub + G, ...; the next partner's record in flight
+            int k = sub;
+            int jn = k < n ? row[k] : i;
+            int mn = S.pmeta[jn];
+            MolRec rn = load_rec(S.prec, jn);
+#pragma unroll 1
+            for (; k < n; k += G) {
+                const int mj = mn;
+                const MolRec pj = rn;
+                if (k + G < n) {
+                    jn = row[k + G];
+                    mn = S.pmeta[jn];
+                    rn = load_rec(S.prec, jn);
+                }
+                mol_pair(mj, pj);
+            }
+        } else {
+#pragma unroll 1
+            for (int s0 = 0; s0 < n; s0 += PAIR_SEG) {
+                const int se = min(n, s0 + PAIR_SEG);
+                // compaction: the segment's partners inside the cut-off, in row order
+                // every load of the segment is issued before the tests: the lane's row entries, then
+                // their partners' lead positions
+                int qn = 0;
+                constexpr int PER = PAIR_SEG / G;
+                int jj[PER];
+                double4 hh[PER];
+#pragma unroll
+                for (int u = 0; u < PER; u++) {
+                    const int k = s0 + u * G + sub;
+                    jj[u] = k < se ? row[k] : -1;
+                }
+#pragma unroll
+                for (int u = 0; u < PER; u++)
+                    hh[u] = jj[u] >= 0 ? prec4[(size_t)jj[u] * (PREC / 4)] : make_double4(0.0, 0.0, 0.0, 0.0);
+#pragma unroll
+                for (int u = 0; u < PER; u++) {
+                    if (s0 + u * G >= se) break;   // uniform over the group
+                    double dx, dy, dz;
+                    const bool in = jj[u] >= 0 &&
+                                    (STRIDED || !(lead_d2(S, xi, yi, zi, hh[u].x, hh[u].y, hh[u].z, dx, dy, dz) > S.rcutsq));
+                    const unsigned long long m = (__ballot(in) >> gbase) & ((G == 64) ? ~0ull : ((1ull << G) - 1ull));
+                    if (in) queue[qn + __popcll(m & ((1ull << sub) - 1ull))] = jj[u];
+                    qn += __popcll(m);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                // evaluation of the queue, the next partner's record in flight
+                int q = sub;
+                int jn = q < qn ? queue[q] : i;
+                int mn = S.pmeta[jn];
+                MolRec rn = load_rec(S.prec, jn);
+#pragma unroll 1
+                for (; q < qn; q += G) {
+                    const int mj = mn;
+                    const MolRec pj = rn;
+                    if (q + G < qn) {
+                        jn = queue[q + G];
+                        mn = S.pmeta[jn];
+                        rn = load_rec(S.prec, jn);
+                    }
+                    mol_pair(mj, pj);
+                }
+                // the queue is rewritten by the next segment
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
         }
         // surface-site bookkeeping between the two loops (src/forcem.f90:408-413)
         if (i >= S.n_liquid_sites) w[(signbit(zi) ? 0 : 3) + 2] += f[0][2];
@@ -281,11 +346,8 @@ __device__ __forceinline__ void pair_body(const DevSys &S, int bid, int nblk) {
         for (int kw = sub; kw < nw; kw += G) {
             const int wj = wrow[kw] - S.n_move;
             const double4 pw = S.wrec[wj];
-            double dx = __dsub_rn(xi, pw.x); dx = __dsub_rn(dx, __dmul_rn(S.lx, anint_d(__dmul_rn(dx, S.lxi))));
-            double dy = __dsub_rn(yi, pw.y); dy = __dsub_rn(dy, __dmul_rn(S.ly, anint_d(__dmul_rn(dy, S.lyi))));
-            double dz = __dsub_rn(zi, pw.z);
-            if (S.zper_force) dz = __dsub_rn(dz, __dmul_rn(S.lz, anint_d(__dmul_rn(dz, S.lzi))));
-            const double rsq = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
+            double dx, dy, dz;
+            const double rsq = lead_d2(S, xi, yi, zi, pw.x, pw.y, pw.z, dx, dy, dz);
             if (rsq > S.rcutsq) continue;
             const int sgo = signbit(pw.z) ? 0 : 3;
             const int cw = S.wcls[wj];   // 0 / 1: the two crystal types, classes MAX_LEAD_TYPES + cw
@@ -298,7 +360,6 @@ __device__ __forceinline__ void pair_body(const DevSys &S, int bid, int nblk) {
             double wx = 0.0, wy = 0.0, wz = 0.0;
 #pragma unroll
             for (int a = 0; a < 3; a++) {
-                if (a >= nsi_i) break;
                 const double xd = dx + me.rx[a], yd = dy + me.ry[a], zd = dz + me.rz[a];
                 const double r2 = (a == 0) ? rsq : xd * xd + yd * yd + zd * zd;
                 R es, fs;
@@ -360,23 +421,25 @@ __device__ __forceinline__ void pair_body(const DevSys &S, int bid, int nblk) {
     }
 }
 
-template <typename R, int G, bool VIR, bool ACC>
+template <typename R, int G, bool VIR, bool ACC, bool STRIDED>
 __global__ __launch_bounds__(256) void k_pair(DevSys S) {
-    pair_body<R, G, VIR, ACC>(S, blockIdx.x, gridDim.x);
+    pair_body<R, G, VIR, ACC, STRIDED>(S, blockIdx.x, gridDim.x);
 }
 
-#define MDF_PAIR_INST(G)                                            \
-    template __global__ void k_pair<double, G, true, false>(DevSys);  \
-    template __global__ void k_pair<float, G, true, false>(DevSys);   \
-    template __global__ void k_pair<double, G, false, false>(DevSys); \
-    template __global__ void k_pair<float, G, false, false>(DevSys);  \
-    template __global__ void k_pair<double, G, true, true>(DevSys);   \
-    template __global__ void k_pair<float, G, true, true>(DevSys);    \
-    template __global__ void k_pair<double, G, false, true>(DevSys);  \
-    template __global__ void k_pair<float, G, false, true>(DevSys);
+#define MDF_PAIR_INST2(G, ST)                                           \
+    template __global__ void k_pair<double, G, true, false, ST>(DevSys);  \
+    template __global__ void k_pair<float, G, true, false, ST>(DevSys);   \
+    template __global__ void k_pair<double, G, false, false, ST>(DevSys); \
+    template __global__ void k_pair<float, G, false, false, ST>(DevSys);  \
+    template __global__ void k_pair<double, G, true, true, ST>(DevSys);   \
+    template __global__ void k_pair<float, G, true, true, ST>(DevSys);    \
+    template __global__ void k_pair<double, G, false, true, ST>(DevSys);  \
+    template __global__ void k_pair<float, G, false, true, ST>(DevSys);
+#define MDF_PAIR_INST(G) MDF_PAIR_INST2(G, false) MDF_PAIR_INST2(G, true)
 MDF_PAIR_INST(8)
 MDF_PAIR_INST(16)
 MDF_PAIR_INST(32)
+#undef MDF_PAIR_INST2
 #undef MDF_PAIR_INST
 
 // forces of moving sites that are neither leads nor lead-carried are zero before the
@@ -389,23 +452,25 @@ __global__ void k_zero_forces(DevSys S) {
 }  // namespace mdf
 
 namespace mdf {
-// batched form (one lanes-per-lead choice and one erfc form for the whole batch)
-template <typename R, int G, bool VIR, bool ACC>
+// batched form (one lanes-per-lead choice, loop form and erfc form for the whole batch)
+template <typename R, int G, bool VIR, bool ACC, bool STRIDED>
 __global__ __launch_bounds__(256) void k_pair_b(const RepArgs *__restrict__ RA, const int2 *__restrict__ map) {
     const int2 m = map[blockIdx.x];
-    pair_body<R, G, VIR, ACC>(RA[m.x].S, m.y, 0);
+    pair_body<R, G, VIR, ACC, STRIDED>(RA[m.x].S, m.y, 0);
 }
-#define MDF_PAIR_B_INST(G)                                                                        \
-    template __global__ void k_pair_b<double, G, true, false>(const RepArgs *, const int2 *);  \
-    template __global__ void k_pair_b<double, G, false, false>(const RepArgs *, const int2 *); \
-    template __global__ void k_pair_b<float, G, true, false>(const RepArgs *, const int2 *);   \
-    template __global__ void k_pair_b<float, G, false, false>(const RepArgs *, const int2 *);  \
-    template __global__ void k_pair_b<double, G, true, true>(const RepArgs *, const int2 *);   \
-    template __global__ void k_pair_b<double, G, false, true>(const RepArgs *, const int2 *);  \
-    template __global__ void k_pair_b<float, G, true, true>(const RepArgs *, const int2 *);    \
-    template __global__ void k_pair_b<float, G, false, true>(const RepArgs *, const int2 *);
+#define MDF_PAIR_B_INST2(G, ST)                                                                        \
+    template __global__ void k_pair_b<double, G, true, false, ST>(const RepArgs *, const int2 *);  \
+    template __global__ void k_pair_b<double, G, false, false, ST>(const RepArgs *, const int2 *); \
+    template __global__ void k_pair_b<float, G, true, false, ST>(const RepArgs *, const int2 *);   \
+    template __global__ void k_pair_b<float, G, false, false, ST>(const RepArgs *, const int2 *);  \
+    template __global__ void k_pair_b<double, G, true, true, ST>(const RepArgs *, const int2 *);   \
+    template __global__ void k_pair_b<double, G, false, true, ST>(const RepArgs *, const int2 *);  \
+    template __global__ void k_pair_b<float, G, true, true, ST>(const RepArgs *, const int2 *);    \
+    template __global__ void k_pair_b<float, G, false, true, ST>(const RepArgs *, const int2 *);
+#define MDF_PAIR_B_INST(G) MDF_PAIR_B_INST2(G, false) MDF_PAIR_B_INST2(G, true)
 MDF_PAIR_B_INST(8)
 MDF_PAIR_B_INST(16)
 MDF_PAIR_B_INST(32)
+#undef MDF_PAIR_B_INST2
 #undef MDF_PAIR_B_INST
 }  // namespace mdf

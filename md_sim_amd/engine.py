@@ -255,15 +255,15 @@ class ForceEngine:
 
     def autotune(self, reps: int = 10) -> dict:
         """Time candidate launch shapes on the current state and keep the fastest; returns
-        the choice (pair lanes per lead, rolled loop, structure-factor and back-projection
-        blocks per launch)."""
+        the choice (pair lanes per lead, pair loop form - strided: no cut-off compaction -,
+        structure-factor and back-projection blocks per launch)."""
         ch = (C.c_int32 * 4)()
         _check(lib().mdf_autotune(self._h, int(reps), ch))
-        return dict(pair_lanes=ch[0], pair_rolled=bool(ch[1]), sf_blocks=ch[2], bp_blocks=ch[3])
+        return dict(pair_lanes=ch[0], pair_strided=bool(ch[1]), sf_blocks=ch[2], bp_blocks=ch[3])
 
-    def set_launch_shapes(self, pair_lanes: int = 16, pair_rolled: bool = False, sf_blocks: int = 4096,
+    def set_launch_shapes(self, pair_lanes: int = 16, pair_strided: bool = False, sf_blocks: int = 4096,
                           bp_blocks: int = 1024):
-        _check(lib().mdf_set_launch_shapes(self._h, int(pair_lanes), int(bool(pair_rolled)), int(sf_blocks),
+        _check(lib().mdf_set_launch_shapes(self._h, int(pair_lanes), int(bool(pair_strided)), int(sf_blocks),
                                            int(bp_blocks)))
 
     def set_stage_timing(self, on, rebuild_every_step: bool = False):
