@@ -41,7 +41,7 @@ __global__ void k_snapshot(DevSys S);
 __global__ void k_wall_cells(DevSys S, int *wcell_of, int ncz, double z0, double csz);
 __global__ void k_count_cells(const int *item_cell, int n_items, int *cnt);
 __global__ void k_zero_int(int *p, int n);
-template <typename R, int G, bool VIR, bool ACC, bool STRIDED>
+template <typename R, int G, bool VIR, bool ACC, bool STRIDED, int NMC>
 __global__ void k_pair(DevSys S);
 template <typename R, int NS>
 __global__ void k_sf(const double *, const double *, const double *, const double *, int, double, double, double,
@@ -137,7 +137,7 @@ __global__ void k_cell_order_b(const RepArgs *, const int2 *);
 __global__ void k_build_b(const RepArgs *, const int2 *);
 __global__ void k_snapshot_b(const RepArgs *, const int2 *);
 __global__ void k_flags_b(const RepArgs *, int, DevFlags *);
-template <typename R, int G, bool VIR, bool ACC, bool STRIDED>
+template <typename R, int G, bool VIR, bool ACC, bool STRIDED, int NMC>
 __global__ void k_pair_b(const RepArgs *, const int2 *);
 template <typename R>
 __global__ void k_sf_b(const RepArgs *, const int2 *);
@@ -831,12 +831,12 @@ static RecipArgs recip_args(const mdf_system *s) {
         }                                                                                                       \
     } while (0)
 
-template <int G, bool ST>
+template <int G, bool ST, int NMC>
 static void launch_pair_g(mdf_system *s, bool f32, bool vir, bool acc) {
     const int blocks = std::max(1, (s->n_lead + (256 / G) - 1) / (256 / G));
     hipStream_t st = s->stream;
     DevSys &S = s->S;
-#define MDF_PAIR(R, V, A) MDF_TLAUNCH(s, KT_PAIR, (k_pair<R, G, V, A, ST>), blocks, 256, 0, st, S)
+#define MDF_PAIR(R, V, A) MDF_TLAUNCH(s, KT_PAIR, (k_pair<R, G, V, A, ST, NMC>), blocks, 256, 0, st, S)
     if (f32) {
         if (vir) { if (acc) MDF_PAIR(float, true, true); else MDF_PAIR(float, true, false); }
         else { if (acc) MDF_PAIR(float, false, true); else MDF_PAIR(float, false, false); }
@@ -852,10 +852,20 @@ static void launch_pair(mdf_system *s) {
     const bool f32 = s->par.precision == MDF_FP32_ACC64, vir = s->par.virial != 0;
     const bool acc = s->par.erfc_mode == MDF_ERFC_ACCURATE;
     const bool st = s->pair_strided;
+    // one moving lead type: a single energy class of moving partners
+    if (s->slot_type.size() <= 1) {
+        switch (s->pair_g) {
+            case 8: if (st) launch_pair_g<8, true, 1>(s, f32, vir, acc); else launch_pair_g<8, false, 1>(s, f32, vir, acc); break;
+            case 32: if (st) launch_pair_g<32, true, 1>(s, f32, vir, acc); else launch_pair_g<32, false, 1>(s, f32, vir, acc); break;
+            default: if (st) launch_pair_g<16, true, 1>(s, f32, vir, acc); else launch_pair_g<16, false, 1>(s, f32, vir, acc); break;
+        }
+        return;
+    }
+    constexpr int M = MAX_LEAD_TYPES;
     switch (s->pair_g) {
-        case 8: if (st) launch_pair_g<8, true>(s, f32, vir, acc); else launch_pair_g<8, false>(s, f32, vir, acc); break;
-        case 32: if (st) launch_pair_g<32, true>(s, f32, vir, acc); else launch_pair_g<32, false>(s, f32, vir, acc); break;
-        default: if (st) launch_pair_g<16, true>(s, f32, vir, acc); else launch_pair_g<16, false>(s, f32, vir, acc); break;
+        case 8: if (st) launch_pair_g<8, true, M>(s, f32, vir, acc); else launch_pair_g<8, false, M>(s, f32, vir, acc); break;
+        case 32: if (st) launch_pair_g<32, true, M>(s, f32, vir, acc); else launch_pair_g<32, false, M>(s, f32, vir, acc); break;
+        default: if (st) launch_pair_g<16, true, M>(s, f32, vir, acc); else launch_pair_g<16, false, M>(s, f32, vir, acc); break;
     }
 }
 
@@ -2802,6 +2812,7 @@ struct mdf_batch {
     bool pending = false;
     int pair_g = 16;
     bool pair_strided = false;
+    bool one_cls = false;   // every system has one moving lead type
     bool f32 = false, vir = false, acc = false;
 };
 
@@ -2923,15 +2934,20 @@ static void batch_launch(mdf_batch *b) {
     {
         const int2 *mp = b->maps[BK_PAIR].p;
         const int g = grid(BK_PAIR);
-#define MDF_PB2(RT, G, ST)                                                               \
-    do {                                                                                 \
-        if (b->acc) {                                                                    \
-            if (b->vir) k_pair_b<RT, G, true, true, ST><<<g, 256, 0, st>>>(R, mp);       \
-            else k_pair_b<RT, G, false, true, ST><<<g, 256, 0, st>>>(R, mp);             \
-        } else {                                                                         \
-            if (b->vir) k_pair_b<RT, G, true, false, ST><<<g, 256, 0, st>>>(R, mp);      \
-            else k_pair_b<RT, G, false, false, ST><<<g, 256, 0, st>>>(R, mp);            \
-        }                                                                                \
+#define MDF_PB3(RT, G, ST, NM)                                                               \
+    do {                                                                                     \
+        if (b->acc) {                                                                        \
+            if (b->vir) k_pair_b<RT, G, true, true, ST, NM><<<g, 256, 0, st>>>(R, mp);       \
+            else k_pair_b<RT, G, false, true, ST, NM><<<g, 256, 0, st>>>(R, mp);             \
+        } else {                                                                             \
+            if (b->vir) k_pair_b<RT, G, true, false, ST, NM><<<g, 256, 0, st>>>(R, mp);      \
+            else k_pair_b<RT, G, false, false, ST, NM><<<g, 256, 0, st>>>(R, mp);            \
+        }                                                                                    \
+    } while (0)
+#define MDF_PB2(RT, G, ST)                                  \
+    do {                                                    \
+        if (b->one_cls) MDF_PB3(RT, G, ST, 1);              \
+        else MDF_PB3(RT, G, ST, MAX_LEAD_TYPES);            \
     } while (0)
 #define MDF_PB(RT, G)                                  \
     do {                                               \
@@ -2945,6 +2961,7 @@ static void batch_launch(mdf_batch *b) {
         }
 #undef MDF_PB
 #undef MDF_PB2
+#undef MDF_PB3
     }
     if (grid(BK_SF)) {
         if (b->f32) k_sf_b<float><<<grid(BK_SF), SF_BLOCK, 0, st>>>(R, b->maps[BK_SF].p);
@@ -3000,6 +3017,8 @@ int mdf_batch_create(mdf_system *const *systems, int32_t n, mdf_batch **out) {
     b->acc = s0->par.erfc_mode == MDF_ERFC_ACCURATE;
     b->pair_g = s0->pair_g;
     b->pair_strided = s0->pair_strided;
+    b->one_cls = true;
+    for (int i = 0; i < n; i++) b->one_cls = b->one_cls && systems[i]->slot_type.size() <= 1;
     int rc = MDF_OK;
     hipError_t e = hipSetDevice(b->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
