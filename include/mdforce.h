@@ -253,6 +253,18 @@ int mdf_observables_enable(mdf_system *sys, double rdel);
 /* Copy the accumulated observables (either pointer may be NULL); reset non-zero zeroes them. */
 int mdf_get_observables(mdf_system *sys, int64_t *ngc, double *efipos, int32_t *nbins, int32_t reset);
 
+/* ---- ngang, the angle histogram of force_intra (src/forcem.f90:617-625; the array of
+   src/measure.f90:20,96-99, reset at :198, written out at :612-618) ----------------------
+   Accumulated on the device at every step from the call on: per angle term, bin
+   l = int((cos theta + 1) * 100) + 1 (0..201) of the term's group type, with the reference's
+   cosine of a = r_k - r_j, b = r_j - r_i (x, y minimum image; z too with a periodic z mode).
+   ang_type[n_angles] is the group type of each angle term of the topology (the reference's
+   iatype, 1..n_atypes); the counts are int64 [202][n_atypes], the memory of the reference's
+   ngang(n_atypes, 0:201).  ang_type NULL switches it off. */
+int mdf_angle_histogram_enable(mdf_system *sys, const int32_t *ang_type, int32_t n_atypes);
+/* Copy the accumulated counts (ngang may be NULL); reset non-zero zeroes them. */
+int mdf_get_angle_histogram(mdf_system *sys, int64_t *ngang, int32_t reset);
+
 /* ---- Widom test-particle insertions: the reference's chempot (src/forcem.f90:1226-1507,
    called at the end of force_get when lchempot is set, :126; inputs src/main.f90:148-152,
    set-up src/measure.f90:149-166, output src/measure.f90:700-705) --------------------------

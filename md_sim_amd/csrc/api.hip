@@ -118,6 +118,7 @@ static const void *sfm_kernel(int nt) {
 constexpr int SFM_TILES_MAX = 8; // kn tiles of 16 (knmax <= 63)
 __global__ void k_obs_ngc(DevSys S, ObsArgs o);
 __global__ void k_obs_efipos(DevSys S, ObsArgs o);
+__global__ void k_obs_ngang(DevSys S, const int *ang_type, int n_atypes, unsigned long long *hist);
 __global__ void k_chem_scan(DevSys S, ChemArgs c);
 __global__ void k_chem_first(DevSys S, ChemArgs c);
 __global__ void k_chem_rot(DevSys S, ChemArgs c);
@@ -365,6 +366,11 @@ struct mdf_system {
     ObsArgs obs;
     DBuf<unsigned long long> obs_ngc;
     DBuf<double> obs_efi, obs_efipos;
+    // force_intra's angle histogram (off unless mdf_angle_histogram_enable)
+    bool ang_on = false;
+    int n_atypes = 0;
+    DBuf<int> ang_type;
+    DBuf<unsigned long long> ang_hist;
     DBuf<int> sg_msize;
     // Widom insertions (off unless mdf_chempot_enable)
     bool chem_on = false;
@@ -438,6 +444,8 @@ static void free_all(mdf_system *s) {
     mts_free(s);
     s->obs_ngc.release(); s->obs_efi.release(); s->obs_efipos.release();
     s->obs_on = false;
+    s->ang_type.release(); s->ang_hist.release();
+    s->ang_on = false;
     chem_free(s);
     if (s->h_mirror) (void)hipHostFree(s->h_mirror);
     s->h_mirror = nullptr;
@@ -1070,6 +1078,8 @@ static void launch_forces(mdf_system *s) {
         k_obs_ngc<<<std::max(1, (s->n_lead + 15) / 16), 256, 0, st>>>(S, s->obs);
         k_obs_efipos<<<1, 64, 0, st>>>(S, s->obs);
     }
+    if (s->ang_on)
+        k_obs_ngang<<<(S.n_angles + 255) / 256, 256, 0, st>>>(S, s->ang_type.p, s->n_atypes, s->ang_hist.p);
     if (s->ev) (void)hipEventRecord(s->ev[8], st);
     {
         dim3 gc(NCOL, s->n_colblk);
@@ -2431,6 +2441,41 @@ int mdf_get_observables(mdf_system *s, int64_t *ngc, double *efipos, int32_t *nb
     return MDF_OK;
 }
 
+int mdf_angle_histogram_enable(mdf_system *s, const int32_t *ang_type, int32_t n_atypes) {
+    if (!s) return fail(MDF_ERR_ARG, "null handle");
+    HIPCHK(hipSetDevice(s->device));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    drain_all(s);
+    drop_graphs(s);
+    s->ang_type.release(); s->ang_hist.release();
+    s->ang_on = false;
+    s->n_atypes = 0;
+    if (!ang_type) return MDF_OK;
+    const int na = s->top.n_angles;
+    if (na <= 0) return fail(MDF_ERR_ARG, "the system has no angle terms");
+    if (n_atypes < 1 || n_atypes > 4096) return fail(MDF_ERR_ARG, "n_atypes %d", n_atypes);
+    for (int a = 0; a < na; a++)
+        if (ang_type[a] < 1 || ang_type[a] > n_atypes)
+            return fail(MDF_ERR_ARG, "angle %d has type %d outside 1..%d", a, ang_type[a], n_atypes);
+    HIPCHK(s->ang_type.alloc(na));
+    HIPCHK(hipMemcpy(s->ang_type.p, ang_type, sizeof(int) * na, hipMemcpyHostToDevice));
+    HIPCHK(s->ang_hist.alloc((size_t)MDF_NGANG_BINS * n_atypes));
+    HIPCHK(hipMemset(s->ang_hist.p, 0, sizeof(unsigned long long) * s->ang_hist.n));
+    s->n_atypes = n_atypes;
+    s->ang_on = true;
+    return MDF_OK;
+}
+
+int mdf_get_angle_histogram(mdf_system *s, int64_t *ngang, int32_t reset) {
+    if (!s) return fail(MDF_ERR_ARG, "null handle");
+    if (!s->ang_on) return fail(MDF_ERR_ARG, "the angle histogram is off");
+    HIPCHK(hipSetDevice(s->device));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    if (ngang) HIPCHK(hipMemcpy(ngang, s->ang_hist.p, sizeof(int64_t) * s->ang_hist.n, hipMemcpyDeviceToHost));
+    if (reset) HIPCHK(hipMemset(s->ang_hist.p, 0, sizeof(unsigned long long) * s->ang_hist.n));
+    return MDF_OK;
+}
+
 int mdf_chempot_enable(mdf_system *s, const mdf_chempot_params *p) {
     if (!s) return fail(MDF_ERR_ARG, "null handle");
     HIPCHK(hipSetDevice(s->device));
@@ -3007,6 +3052,7 @@ int mdf_batch_create(mdf_system *const *systems, int32_t n, mdf_batch **out) {
     for (int i = 0; i < n; i++) {
         if (systems[i]->mts_on) return fail(MDF_ERR_ARG, "system %d has multiple time steps on (not in batches)", i);
         if (systems[i]->obs_on) return fail(MDF_ERR_ARG, "system %d accumulates observables (not in batches)", i);
+        if (systems[i]->ang_on) return fail(MDF_ERR_ARG, "system %d accumulates the angle histogram (not in batches)", i);
         if (systems[i]->chem_on) return fail(MDF_ERR_ARG, "system %d makes Widom insertions (not in batches)", i);
     }
     mdf_batch *b = new mdf_batch;

@@ -241,6 +241,33 @@ __global__ __launch_bounds__(256) void k_obs_ngc(DevSys S, ObsArgs o) {
 }
 
 // efipos bins of the step's ion z-forces, in site order (one thread: a fixed summation order)
+// force_intra's angle histogram (src/forcem.f90:570-625): one thread per angle term, the
+// reference's cosine in its operation order without contraction (1/|a| = 1/sqrt(a.a)), bin
+// l = int((cos + 1) 100) + 1 of the term's group type; integer counts, so the order of the
+// additions does not matter
+__global__ void k_obs_ngang(DevSys S, const int *__restrict__ ang_type, int n_atypes,
+                            unsigned long long *__restrict__ hist) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= S.n_angles) return;
+    const int i = S.ang_ijk[3 * a], j = S.ang_ijk[3 * a + 1], k = S.ang_ijk[3 * a + 2];
+    auto img = [](double d, double L, double Li) { return __dsub_rn(d, __dmul_rn(L, anint_d(__dmul_rn(d, Li)))); };
+    const double ax = img(__dsub_rn(S.x[k], S.x[j]), S.lx, S.lxi), ay = img(__dsub_rn(S.y[k], S.y[j]), S.ly, S.lyi);
+    const double bx = img(__dsub_rn(S.x[j], S.x[i]), S.lx, S.lxi), by = img(__dsub_rn(S.y[j], S.y[i]), S.ly, S.lyi);
+    double az = __dsub_rn(S.z[k], S.z[j]), bz = __dsub_rn(S.z[j], S.z[i]);
+    if (S.zper_force) { az = img(az, S.lz, S.lzi); bz = img(bz, S.lz, S.lzi); }
+    auto dot = [](double u0, double u1, double u2, double v0, double v1, double v2) {
+        return __dadd_rn(__dadd_rn(__dmul_rn(u0, v0), __dmul_rn(u1, v1)), __dmul_rn(u2, v2));
+    };
+    const double ai = __ddiv_rn(1.0, __dsqrt_rn(dot(ax, ay, az, ax, ay, az)));
+    const double bi = __ddiv_rn(1.0, __dsqrt_rn(dot(bx, by, bz, bx, by, bz)));
+    const double cang = -__dmul_rn(__dmul_rn(dot(ax, ay, az, bx, by, bz), ai), bi);
+    const double u = __dmul_rn(__dadd_rn(cang, 1.0), 100.0);
+    const int l = (u > -1.0 && u < (double)MDF_NGANG_BINS) ? (int)u + 1 : -1;   // the reference's int(): toward zero
+    const int it = ang_type[a];
+    if (l >= 0 && l < MDF_NGANG_BINS && it >= 1 && it <= n_atypes)
+        atomicAdd(&hist[(size_t)l * n_atypes + (it - 1)], 1ull);
+}
+
 __global__ void k_obs_efipos(DevSys S, ObsArgs o) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     if (S.flags->overflow_mm || S.flags->overflow_mw) return;

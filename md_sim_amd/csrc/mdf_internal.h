@@ -252,6 +252,9 @@ struct MtsArgs {
 };
 
 // ---- observables of the reference's hot loops (measurement only) -----------------------
+// bins of force_intra's angle histogram: ngang(:, 0:langmax), langmax = 201 (src/measure.f90:96-99)
+constexpr int MDF_NGANG_BINS = 202;
+
 struct ObsArgs {
     unsigned long long *ngc;   // [nbins][nijt] site-pair distance histogram of the pair loops
     double *efi;               // [n_move] this step's z-force of each mobile ion of types 3, 4

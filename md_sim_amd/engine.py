@@ -57,7 +57,8 @@ def lib() -> C.CDLL:
                    "mdf_batch_read_outputs", "mdf_batch_autotune", "mdf_mts_enable", "mdf_mts_get_stats",
                    "mdf_mts_get_weights", "mdf_mts_set_weights", "mdf_mts_features", "mdf_mts_eval",
                    "mdf_observables_enable", "mdf_get_observables", "mdf_chempot_enable", "mdf_get_chempot",
-                   "mdf_reciprocal_kernels", "mdf_mts_eval_device"):
+                   "mdf_reciprocal_kernels", "mdf_mts_eval_device", "mdf_angle_histogram_enable",
+                   "mdf_get_angle_histogram"):
             getattr(L, fn).restype = C.c_int
         L.mdf_destroy.restype = None
         L.mdf_batch_destroy.restype = None
@@ -176,6 +177,24 @@ class ForceEngine:
         _check(lib().mdf_get_observables(self._h, ngc.ctypes.data_as(C.POINTER(C.c_int64)), _dp(efp), C.byref(nb),
                                          int(bool(reset))))
         return ngc, efp
+
+    def angle_histogram_enable(self, ang_type, n_atypes: int):
+        """Accumulate force_intra's ngang from the next step on: ang_type[n_angles] is each
+        angle term's 1-based group type; None switches it off."""
+        if ang_type is None:
+            _check(lib().mdf_angle_histogram_enable(self._h, None, 0))
+            self._n_atypes = 0
+            return
+        self._ang_type = np.ascontiguousarray(ang_type, np.int32)
+        _check(lib().mdf_angle_histogram_enable(self._h, self._ang_type.ctypes.data_as(C.POINTER(C.c_int32)),
+                                                int(n_atypes)))
+        self._n_atypes = int(n_atypes)
+
+    def angle_histogram(self, reset: bool = False):
+        """ngang [202, n_atypes] int64 accumulated so far."""
+        out = np.zeros((202, self._n_atypes), np.int64)
+        _check(lib().mdf_get_angle_histogram(self._h, out.ctypes.data_as(C.POINTER(C.c_int64)), int(bool(reset))))
+        return out
 
     # -- Widom test-particle insertions (the reference's chempot) ------------------------
     def chempot_enable(self, n_insert: int, wall_min: float, wall_max: float, n_bins: int, bin_mid: int,

@@ -43,7 +43,8 @@ module force
                          MDF_ERFC_AS7126, MDF_ALPHA_REFERENCE, MDF_Z_OPEN, MDF_Z_PERIODIC_OPEN_LIST, &
                          mdf_mts_params, mdf_mts_stats, mdf_mts_enable, mdf_mts_get_stats, MDF_MTS_HELD, &
                          MDF_MTS_NETWORK, mdf_observables_enable, mdf_get_observables, mdf_chempot_params, &
-                         mdf_chempot_enable, mdf_get_chempot
+                         mdf_chempot_enable, mdf_get_chempot, mdf_angle_histogram_enable, &
+                         mdf_get_angle_histogram
     use, intrinsic :: iso_c_binding
     implicit none
 
@@ -64,12 +65,12 @@ module force
     type(mdf_params), private :: par
     type(mdf_outputs), private :: out
     integer(c_int), allocatable, target, private :: c_itype(:), c_nsi(:), c_ijtype(:), c_vkind(:)
-    integer(c_int), allocatable, target, private :: c_cona(:), c_conb(:), c_angi(:), c_angj(:), c_angk(:)
+    integer(c_int), allocatable, target, private :: c_cona(:), c_conb(:), c_angi(:), c_angj(:), c_angk(:), c_angt(:)
     real(c_double), allocatable, target, private :: c_q(:), c_eps4(:), c_sig6(:), c_a(:), c_rhoi(:), c_c(:)
     real(c_double), allocatable, target, private :: c_conl(:), c_angkt(:), c_angth(:)
     real(c_double), allocatable, target, private :: o_fsite(:), o_fcom(:), o_tq(:), o_evdw(:), o_ecoul(:)
     ! one step's observables and insertions, fetched from the device
-    integer(c_int64_t), allocatable, target, private :: o_ngc(:), o_inave(:)
+    integer(c_int64_t), allocatable, target, private :: o_ngc(:), o_inave(:), o_ngang(:)
     real(c_double), allocatable, target, private :: o_efipos(:), o_pave(:)
     integer(c_int32_t), private :: o_nbins = 0
 
@@ -161,7 +162,7 @@ module force
             n = n + nangt(iatype(ix))
         end do
         allocate(c_angi(max(n, 1_8)), c_angj(max(n, 1_8)), c_angk(max(n, 1_8)))
-        allocate(c_angkt(max(n, 1_8)), c_angth(max(n, 1_8)))
+        allocate(c_angkt(max(n, 1_8)), c_angth(max(n, 1_8)), c_angt(max(n, 1_8)))
         k = 0
         do ix = 1, nang
             it = iatype(ix)
@@ -172,6 +173,7 @@ module force
                 c_angk(k) = int(ang3(is, ix) - 1, c_int)
                 c_angkt(k) = kt(it, is)
                 c_angth(k) = theta(it, is)
+                c_angt(k) = int(it, c_int)
             end do
         end do
         top%n_angles = int(n, c_int)
@@ -262,6 +264,16 @@ module force
         end if
         rc = mdf_get_observables(sys, c_null_ptr, c_null_ptr, o_nbins, 0_c_int32_t)
         allocate(o_ngc(max(1, o_nbins*int(nijt))), o_efipos(max(1, 2*o_nbins)))
+        ! force_intra's angle histogram, sized as measure_init allocates ngang(nit-2, 0:201)
+        ! (src/measure.f90:96-99)
+        if (top%n_angles > 0) then
+            rc = mdf_angle_histogram_enable(sys, c_loc(c_angt), int(nit - 2, c_int32_t))
+            if (rc /= MDF_OK) then
+                print *, 'force_init(): ', trim(mdf_error_message())
+                stop 'force_init(): the angle histogram could not be set up'
+            end if
+            allocate(o_ngang(202*max(1, int(nit - 2))))
+        end if
         if (.not. lchempot) return
         ew = 1
         v = env_value('MDF_CHEMPOT_EW3DC')
@@ -300,6 +312,15 @@ module force
         ngc(:, 0:nb-1) = ngc(:, 0:nb-1) + reshape(o_ngc(1:nijt*nb), (/ int(nijt), int(nb) /))
         nb = min(int(o_nbins, 8), int(size(efipos, 2), 8))
         efipos(:, 0:nb-1) = efipos(:, 0:nb-1) + reshape(o_efipos(1:2*nb), (/ 2, int(nb) /))
+        ! the step's angle counts into ngang (src/forcem.f90:617-625)
+        if (allocated(o_ngang)) then
+            rc = mdf_get_angle_histogram(sys, c_loc(o_ngang), 1_c_int32_t)
+            if (rc /= MDF_OK) then
+                print *, 'force_get(): ', trim(mdf_error_message())
+                stop 'force_get(): the angle histogram could not be read'
+            end if
+            if (allocated(ngang)) ngang(:, 0:201) = ngang(:, 0:201) + reshape(o_ngang, (/ int(nit - 2), 202 /))
+        end if
         if (.not. lchempot) return
         rc = mdf_get_chempot(sys, c_loc(o_pave), c_loc(o_inave), sd, 1_c_int32_t)
         if (rc /= MDF_OK) then

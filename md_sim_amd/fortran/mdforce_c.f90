@@ -120,6 +120,19 @@ module mdforce_c
             integer(c_int32_t), value :: reset
         end function mdf_get_observables
 
+        integer(c_int) function mdf_angle_histogram_enable(sys, ang_type, n_atypes) &
+                bind(C, name='mdf_angle_histogram_enable')
+            import :: c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: sys, ang_type
+            integer(c_int32_t), value :: n_atypes
+        end function mdf_angle_histogram_enable
+
+        integer(c_int) function mdf_get_angle_histogram(sys, ngang, reset) bind(C, name='mdf_get_angle_histogram')
+            import :: c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: sys, ngang
+            integer(c_int32_t), value :: reset
+        end function mdf_get_angle_histogram
+
         integer(c_int) function mdf_chempot_enable(sys, p) bind(C, name='mdf_chempot_enable')
             import :: c_int, c_ptr, mdf_chempot_params
             type(c_ptr), value :: sys

@@ -314,7 +314,7 @@ def parse_top(text: str, mols):
                 cons.append([(tuple(r.ints(2)), m["lengths"][k]) for k in range(nsx)])
             nsx = int(r.tokens(2)[1])
             for _ in range(m["count"] if nsx > 0 else 0):
-                angs.append([(tuple(r.ints(3)), m["angles"][k]) for k in range(nsx)])
+                angs.append([(tuple(r.ints(3)), m["angles"][k], it) for k in range(nsx)])
     return cons, angs
 
 
@@ -475,20 +475,22 @@ def load_reference_inputs(fld: str, top: str, cfg: str, inp: str, name: str = "r
         flat = [c for mol in cons for c in mol]
         constraints = (np.array([c[0][0] - 1 for c in flat], np.int32), np.array([c[0][1] - 1 for c in flat], np.int32),
                        np.array([c[1] for c in flat], np.float64))
-    angles = None
+    angles = angle_type = None
     if angs:
         flat = [a for mol in angs for a in mol]
         pi_w = _f32w(3.1415926535898)
         angles = (np.array([a[0][0] - 1 for a in flat], np.int32), np.array([a[0][1] - 1 for a in flat], np.int32),
                   np.array([a[0][2] - 1 for a in flat], np.int32), np.array([a[1][0] for a in flat], np.float64),
                   np.array([a[1][1] * pi_w / 180.0 for a in flat], np.float64))
+        angle_type = np.array([a[2] for a in flat], np.int32)
     return TopologyData(name=name, box=cf["box"], rcut=run["rcut"], skin=run["skin"], kmax=run["kmax"],
                         site_type=np.array(itype, np.int32), charge=np.array(q, np.float64), nsi=nsi,
                         ijtype=ijtype, vdw_kind=kind, vdw_eps4=eps4, vdw_sig6=sig6, vdw_a=A, vdw_rhoi=rhoi,
                         vdw_c=C, n_move=nmove, n_rigid_sites=nriga, n_liquid_sites=nliqa, n_rig=nrig,
                         n_ltype=nltype, wall_lw=cry["lw"], wall_cell=cry["cell"],
                         wall_nss=(cry["sites"][0][1], cry["sites"][1][1]), pos=pos, com=com,
-                        constraints=constraints, angles=angles, dt=run["dt"], meta=dict(run=run))
+                        constraints=constraints, angles=angles, angle_type=angle_type, dt=run["dt"],
+                        meta=dict(run=run))
 
 
 def load_reference_files(directory: str, base: str = None, inp: str = "tot.inp"):

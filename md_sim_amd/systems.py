@@ -356,13 +356,13 @@ def _buckingham_table(rng, n_ion_types: int):
 
 def water_on_slab(n_water: int, ncell: int, nlayers: int, seed: int, rcut: float,
                   kmax: int, a: float = 5.64, lw: float = 40.0, T: float = 300.0,
-                  n_mobile_ions: int = 0, name: str = "slab") -> System:
+                  n_mobile_ions: int = 0, name: str = "slab", gap: float = 2.8) -> System:
     """SPC/E water film above a frozen rock-salt (100) slab, slab-corrected 3-D Ewald.
 
     Lateral box = ncell * a; Lz = 3 Lx so that the reference's k-index rule gives
     kmax x kmax x 3 kmax (src/forcem.f90:673-678).  The crystal's top layer sits on the
     lower wall plane z = -lw/2 used by the reference's wall tail correction
-    (src/forcem.f90:509-543); the water starts 2.8 A above it.
+    (src/forcem.f90:509-543); the water starts ``gap`` (2.8) A above it.
     Optional mobile ions (one cation and one anion type) are mixed into the film."""
     rng = np.random.default_rng(seed)
     lx = ncell * a
@@ -370,7 +370,7 @@ def water_on_slab(n_water: int, ncell: int, nlayers: int, seed: int, rcut: float
     zwall = -0.5 * lw
     wall_pos, wall_kind = rocksalt_slab(ncell, ncell, nlayers, a, z_top=zwall)
     n_items = n_water + n_mobile_ions
-    com_all = water_film(rng, n_items, lx, lx, zwall + 2.8)
+    com_all = water_film(rng, n_items, lx, lx, zwall + gap)
     pick = rng.permutation(n_items)
     wsel = np.sort(pick[:n_water])
     isel = np.sort(pick[n_water:])

@@ -47,6 +47,7 @@ class TopologyData:
     com: np.ndarray                # [nrig, 3]
     constraints: Optional[Tuple[np.ndarray, np.ndarray, np.ndarray]] = None   # 0-based a, b, length
     angles: Optional[Tuple[np.ndarray, np.ndarray, np.ndarray, np.ndarray, np.ndarray]] = None
+    angle_type: Optional[np.ndarray] = None   # 1-based group type of each angle term (iatype)
     z_periodic: bool = False
     dt: float = 1.0
     meta: Dict[str, object] = field(default_factory=dict)
@@ -113,6 +114,7 @@ class TopologyData:
             a = a.reshape(5, na)
             ang = (a[0].astype(np.int32) - 1, a[1].astype(np.int32) - 1, a[2].astype(np.int32) - 1, a[3].copy(),
                    a[4].copy())
+        atype = np.asarray(d["angle_type"]).astype(np.int32) if "angle_type" in d else None
         pos = np.stack([d["x0"], d["y0"], d["z0"]], 1).astype(np.float64)
         com = np.stack([d["cx0"], d["cy0"], d["cz0"]], 1).astype(np.float64) if nrig else np.zeros((0, 3))
         return cls(name=name, box=(float(geo[0]), float(geo[1]), float(geo[2])), rcut=float(geo[9]),
@@ -125,4 +127,5 @@ class TopologyData:
                    vdw_rhoi=np.asarray(d["rhoi"], np.float64), vdw_c=np.asarray(d["cc"], np.float64),
                    n_move=nmove, n_rigid_sites=nriga, n_liquid_sites=nliqa, n_rig=nrig, n_ltype=nltype,
                    wall_lw=float(geo[3]), wall_cell=(float(geo[4]), float(geo[5]), float(geo[6])),
-                   wall_nss=(int(geo[7]), int(geo[8])), pos=pos, com=com, constraints=cons, angles=ang)
+                   wall_nss=(int(geo[7]), int(geo[8])), pos=pos, com=com, constraints=cons, angles=ang,
+                   angle_type=atype)

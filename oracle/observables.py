@@ -76,3 +76,40 @@ def ngc(x, y, z, itype, nsi, ijtype, box, rcut, rdel, nbins, nijt, list1, list2,
             count(dx + (x[i + a] - x[i]), dy + (y[i + a] - y[i]), dz + (z[i + a] - z[i]),
                   itype[i + a], itype[j])
     return out
+
+
+NGANG_BINS = 202   # the reference's ngang(:, 0:langmax), langmax = 201 (src/measure.f90:96-99)
+
+
+def ngang(x, y, z, box, ang_i, ang_j, ang_k, ang_type, n_atypes, z_periodic=False):
+    """force_intra's angle histogram of one call (src/forcem.f90:570-625); int64
+    [NGANG_BINS, n_atypes], the memory of ngang(n_atypes, 0:201).
+
+    Per angle term (0-based sites i, j, k; ang_type = its group's 1-based type iatype):
+    a = r_k - r_j, b = r_j - r_i with the x, y minimum image (z too in the 3-D form),
+    cos = -(a.b) / |a| / |b| with 1/|a| = 1/sqrt(a.a), bin l = int((cos + 1) 100) + 1.
+    """
+    lx, ly, lz = box
+    lxi, lyi, lzi = 1.0 / lx, 1.0 / ly, 1.0 / lz
+    out = np.zeros((NGANG_BINS, n_atypes), np.int64)
+    for i, j, k, it in zip(ang_i, ang_j, ang_k, ang_type):
+        ax = x[k] - x[j]
+        ax = ax - lx * _anint(ax * lxi)
+        ay = y[k] - y[j]
+        ay = ay - ly * _anint(ay * lyi)
+        az = z[k] - z[j]
+        bx = x[j] - x[i]
+        bx = bx - lx * _anint(bx * lxi)
+        by = y[j] - y[i]
+        by = by - ly * _anint(by * lyi)
+        bz = z[j] - z[i]
+        if z_periodic:
+            az = az - lz * _anint(az * lzi)
+            bz = bz - lz * _anint(bz * lzi)
+        ai = 1.0 / np.sqrt(ax * ax + ay * ay + az * az)
+        bi = 1.0 / np.sqrt(bx * bx + by * by + bz * bz)
+        ab = ax * bx + ay * by + az * bz
+        cang = -(ab * ai * bi)
+        l = int((cang + 1.0) * 100.0) + 1
+        out[l, it - 1] += 1
+    return out

@@ -63,6 +63,22 @@ int mdf_get_observables(mdf_system *sys, int64_t *ngc, double *efipos, int32_t *
     return MDF_OK;
 }
 
+/* the angle histogram is not restated here: counts stay zero */
+static int32_t shim_n_atypes = 0;
+int mdf_angle_histogram_enable(mdf_system *sys, const int32_t *ang_type, int32_t n_atypes) {
+    (void)sys;
+    shim_n_atypes = ang_type ? n_atypes : 0;
+    return MDF_OK;
+}
+
+int mdf_get_angle_histogram(mdf_system *sys, int64_t *ngang, int32_t reset) {
+    (void)sys;
+    (void)reset;
+    if (ngang)
+        for (int32_t k = 0; k < 202 * shim_n_atypes; k++) ngang[k] = 0;
+    return MDF_OK;
+}
+
 int mdf_chempot_enable(mdf_system *sys, const mdf_chempot_params *p) {
     (void)sys;
     return p ? MDF_ERR_ARG : MDF_OK;

@@ -126,6 +126,8 @@ program ref_harness
     call force_real
     f1x = fx ; f1y = fy ; f1z = fz
     call force_fouri
+    ! the angle histogram as measure_init leaves it (src/measure.f90:96-99,198)
+    if (allocated(ngang)) ngang = 0
     call force_intra
     f2x = fx ; f2y = fy ; f2z = fz
 
@@ -312,7 +314,22 @@ program ref_harness
         end do
     end do
     call put_r(u, 'angles', cbuf, 5*n)
+    ! each angle term's group type (iatype), the first index of ngang
+    kcnt = 0
+    do i = 1, nang
+        do ix = 1, nangt(iatype(i))
+            kcnt = kcnt + 1
+            cbuf(kcnt) = dble(iatype(i))
+        end do
+    end do
+    call put_r(u, 'angle_type', cbuf, n)
     deallocate(cbuf)
+    ! force_intra's angle histogram ngang(nit-2, 0:201), column-major (src/forcem.f90:617-625)
+    if (allocated(ngang)) then
+        ibuf(1:2) = (/ int(size(ngang, 1), 8), int(size(ngang, 2), 8) /)
+        call put_i(u, 'ngang_shape', ibuf, 2_8)
+        call put_i(u, 'ngang', ngang, int(size(ngang), 8))
+    end if
     allocate(cbuf(12))
     cbuf(1:12) = (/ lx, ly, lz, lw, cx, cy, cz, dble(nss(ntype, 1)), dble(nss(ntype, 2)), rcut, skin, dble(kmax) /)
     call put_r(u, 'geometry', cbuf, 12_8)

@@ -84,6 +84,11 @@ def sample_case():
     out["list_digest"] = np.array(pair_digest(dump["list1"], dump["list2"]))
     out["wlist_digest"] = np.array(pair_digest(dump["listw1"], dump["listw2"]))
     out["n_list"] = np.array([len(dump["list1"]), len(dump["listw1"])])
+    # force_intra's angle histogram ngang(nit-2, 0:201) as [0:201][nit-2], and the group type
+    # (iatype) of each angle term in the order of "angles" (src/forcem.f90:617-625)
+    shp = dump["ngang_shape"]
+    out["ngang"] = dump["ngang"].reshape(int(shp[1]), int(shp[0]))
+    out["angle_type"] = dump["angle_type"].astype(np.int32)
     for f in SAMPLE_FILES:
         out["file_" + f.replace(".", "_")] = np.frombuffer(open(os.path.join(SAMPLE_DIR, f), "rb").read(), np.uint8)
     shutil.rmtree(d, ignore_errors=True)
@@ -130,6 +135,9 @@ def chempot_cases():
 def main():
     if sys.argv[1:] == ["chempot"]:
         chempot_cases()
+        return
+    if sys.argv[1:] == ["sample"]:
+        sample_case()
         return
     if os.path.isdir(SAMPLE_DIR):
         sample_case()

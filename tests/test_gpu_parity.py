@@ -101,7 +101,15 @@ def test_engine_matches_reference_sample():
     txt = lambda k: bytes(g["file_" + k]).decode()
     td = refio.load_reference_inputs(txt("y_fld"), txt("y_top"), txt("y_cfg"), txt("y_inp"), name="sample_rutile")
     e = ForceEngine(td)
+    # force_intra's angle histogram ngang (src/forcem.f90:617-625), from the group types
+    # refio read from the topology file
+    assert np.array_equal(td.angle_type, g["angle_type"])
+    e.angle_histogram_enable(td.angle_type, g["ngang"].shape[1])
     r = e.force_get(*td.positions())
+    assert np.array_equal(e.angle_histogram(), g["ngang"])
+    e.force_get(*td.positions())
+    assert np.array_equal(e.angle_histogram(reset=True), 2 * g["ngang"])
+    assert e.angle_histogram().sum() == 0
     for a, k in enumerate("xyz"):
         assert rel_err(r["f_site"][a], g["f_site_" + k]) < 1e-10
     assert rel_err(r["torque"][1], g["ty"]) < 1e-10 and rel_err(r["f_com"][0], g["fcx"]) < 1e-10

@@ -41,3 +41,35 @@ def test_ngc_bin_width_matters():
     got, ref = _ngc("bulk216", rdel_scale=1.05)
     assert not np.array_equal(got, ref)
     assert got.sum() <= ref.sum()
+
+
+def _sample_angles():
+    g = golden("sample_rutile")
+    a = np.asarray(g["angles"], np.float64)
+    n = len(a) // 5
+    a = a.reshape(5, n)
+    geo = g["geometry"]
+    ijk = [a[r].astype(np.int64) - 1 for r in range(3)]
+    return g, ijk, (float(geo[0]), float(geo[1]), float(geo[2]))
+
+
+def test_ngang_matches_reference_dump():
+    # force_intra's angle histogram on the reference's own sample (54 Ti-O-H angles of the
+    # surface OH groups): the restatement equals the array the reference's loop wrote
+    g, (ai, aj, ak), box = _sample_angles()
+    ref = g["ngang"]
+    got = observables.ngang(g["x0"], g["y0"], g["z0"], box, ai, aj, ak, g["angle_type"], ref.shape[1])
+    assert ref.sum() == len(ai) == 54
+    assert np.array_equal(got, ref)
+
+
+def test_ngang_vertex_and_sign_matter():
+    # the vertex at the end site, or the cosine without its sign, moves the counts
+    g, (ai, aj, ak), box = _sample_angles()
+    ref = g["ngang"]
+    got = observables.ngang(g["x0"], g["y0"], g["z0"], box, aj, ai, ak, g["angle_type"], ref.shape[1])
+    assert not np.array_equal(got, ref)
+    flip = observables.ngang(-g["x0"], -g["y0"], -g["z0"], box, ai, aj, ak, g["angle_type"], ref.shape[1])
+    assert np.array_equal(flip, ref)   # inversion keeps every angle
+    mir = observables.ngang(g["x0"], g["y0"], g["z0"], box, ai, ak, aj, g["angle_type"], ref.shape[1])
+    assert not np.array_equal(mir, ref)
