@@ -143,6 +143,16 @@ __global__ void k_pair_b(const RepArgs *, const int2 *);
 template <typename R>
 __global__ void k_sf_b(const RepArgs *, const int2 *);
 __global__ void k_sf_reduce_b(const RepArgs *, const int2 *);
+template <int NT>
+__global__ void k_sf_mma_b(const RepArgs *, const int2 *);
+template <int NT>
+__global__ void k_bp_mma_b(const RepArgs *, const int2 *);
+__global__ void k_sf_reduce_rows_b(const RepArgs *, const int2 *);
+template <int NT>
+__global__ void k_sf64_b(const RepArgs *, const int2 *);
+__global__ void k_sf64_reduce_b(const RepArgs *, const int2 *);
+template <int NKS, int NG>
+__global__ void k_bp64_b(const RepArgs *, const int2 *);
 template <typename R>
 __global__ void k_bp_b(const RepArgs *, const int2 *);
 __global__ void k_epilogue_b(const RepArgs *, const int2 *);
@@ -2837,6 +2847,7 @@ int mdf_mts_eval_device(const float *weights, int32_t n, const float *feat, cons
 // Many independent systems stepped together: every stage is one launch over all of them
 // (block maps give each block its system), so small systems fill the GPU together and a
 // step of the whole batch costs one graph launch.
+enum { BR_VALU, BR_MMA32, BR_MMA64 };
 enum { BK_GATHER, BK_SKIN, BK_COUNT, BK_FILL, BK_ORDER, BK_BUILD, BK_SNAP, BK_PAIR, BK_SF, BK_SFR, BK_BP, BK_EPI,
        BK_COLSUM, BK_N };
 
@@ -2848,6 +2859,14 @@ struct mdf_batch {
     DBuf<RepArgs> d_rep;
     DBuf<int2> maps[BK_N];
     int nmap[BK_N] = {};
+    // reciprocal sum: the vector-ALU kernels, or the matrix-core kernels of the FP32 or FP64
+    // form when every system runs those alone (then one launch per kernel template: systems
+    // grouped by kn-tile count, or by K-steps and site groups of the FP64 back-projection)
+    int rmode = 0;   // BR_*
+    static constexpr int NGRP = 40;
+    struct Group { int key = 0, n = 0, threads = 256; size_t lds = 0; DBuf<int2> map; };
+    Group gsf[NGRP], gbp[NGRP];
+    int n_gsf = 0, n_gbp = 0;
     std::vector<const double *> h_in;
     DBuf<const double *> d_in;
     DBuf<DevFlags> d_flags;
@@ -2917,6 +2936,30 @@ static void fill_rep(mdf_system *s, RepArgs &r) {
     r.scal = s->d_scal.p;
     r.A = final_args(s);
     r.A.n_kblk = nk > 0 ? s->n_kblk : 0;   // the batch reduces k partials in blocks of SFR_K k-vectors
+    // the matrix-core forms (used when the batch runs them: the k partials are per row then)
+    r.knmax = s->knmax; r.n_nt = s->n_nt; r.sfm_sites = s->sfm_sites;
+    r.sfm_nsb = s->n_sb_mma; r.sfm_nrg = s->n_rg_mma;
+    r.rowkk = s->d_rowkk.p; r.tile_ks = s->d_tile_ks.p; r.tpack = s->d_tpack.p; r.tscale = s->d_tscale.p;
+    r.klmax = s->klmax; r.kmmax = s->kmmax;
+    r.s64_nsb = s->s64_nsb; r.s64_pcols = s->s64_pcols; r.s64_sites = s->s64_sites;
+    r.b64_nsb = std::max(1, (nm + s->b64_sites - 1) / s->b64_sites);
+    r.s64_rowkk = s->d_s64_rowkk.p; r.s64_groups = s->d_s64_groups.p; r.b64_tile = s->d_b64_tile.p;
+    r.b64_rg = s->d_b64_rg.p; r.s64_tilent = s->d_s64_tilent.p; r.s64_pcoff = s->d_s64_pcoff.p;
+    r.s64_sorted_of = s->d_s64_sorted_of.p; r.s64_part = s->d_s64_part.p; r.b64_rinfo = s->d_b64_rinfo.p;
+    r.b64_tpk = s->d_b64_tpk.p;
+}
+
+// the batch's k-sum on the matrix cores: every system's partials, T and k partials in the
+// layout its own kernels use
+static void rep_mode(const mdf_batch *b, const mdf_system *s, RepArgs &r) {
+    const int nk = (int)s->ak.size(), nm = s->top.n_move;
+    if (b->rmode == BR_MMA32) {
+        r.n_groups = (nk > 0 && nm > 0) ? 1 : 0;
+        r.A.n_kblk = nk > 0 ? (int)s->rows.size() : 0;
+    } else if (b->rmode == BR_MMA64) {
+        r.n_groups = (nk > 0 && nm > 0) ? s->b64_nrg : 0;
+        r.A.n_kblk = nk > 0 ? (int)s->rows.size() : 0;
+    }
 }
 
 // block counts of every batched launch for one system
@@ -2933,18 +2976,69 @@ static void rep_blocks(const mdf_batch *b, const mdf_system *s, const RepArgs &r
     cnt[BK_SF] = (nk > 0 && nm > 0) ? r.n_sb * r.n_cg : 0;
     cnt[BK_SFR] = nk > 0 ? r.n_kblk : 0;
     cnt[BK_BP] = (nk > 0 && nm > 0) ? r.n_bpsb * r.n_rg : 0;
+    if (b->rmode != BR_VALU) {   // the matrix-core kernels go by template groups (batch_groups)
+        cnt[BK_SF] = 0;
+        cnt[BK_BP] = 0;
+        cnt[BK_SFR] = nk > 0 ? r.nrows : 0;
+    }
     cnt[BK_EPI] = r.n_epblk;
     cnt[BK_COLSUM] = NCOL * r.n_colblk;
 }
 
 // per-system arguments and block maps to the device; the captured graph reads both at run
 // time, so it stays valid when only their contents change
+// block maps of the matrix-core k-sum, one per kernel template: structure factor by kn tiles
+// (FP32 n_nt, FP64 s64_nt), back-projection by kn tiles (FP32) or by K-steps and site groups
+// (FP64); each with the largest dynamic LDS and the waves per block of its systems
+static int batch_groups(mdf_batch *b) {
+    for (int g = 0; g < mdf_batch::NGRP; g++) { b->gsf[g].map.release(); b->gbp[g].map.release(); b->gsf[g].n = b->gbp[g].n = 0; }
+    b->n_gsf = b->n_gbp = 0;
+    if (b->rmode == BR_VALU) return MDF_OK;
+    std::vector<int2> msf[mdf_batch::NGRP], mbp[mdf_batch::NGRP];
+    auto slot = [](mdf_batch::Group *gs, int &ng, int key) {
+        for (int g = 0; g < ng; g++) if (gs[g].key == key) return g;
+        gs[ng].key = key; gs[ng].lds = 0; gs[ng].threads = 256;
+        return ng++;
+    };
+    for (int i = 0; i < (int)b->sys.size(); i++) {
+        const mdf_system *s = b->sys[i];
+        const int nk = (int)s->ak.size(), nm = s->top.n_move;
+        if (nk == 0 || nm == 0) continue;
+        const RepArgs &r = b->h_rep[i];
+        if (b->rmode == BR_MMA32) {
+            const int g = slot(b->gsf, b->n_gsf, s->n_nt);
+            b->gsf[g].lds = std::max(b->gsf[g].lds, sfm_lds_bytes(s->sfm_sites, s->n_nt));
+            for (int l = 0; l < r.sfm_nsb * r.sfm_nrg; l++) msf[g].push_back(make_int2(i, l));
+            const int h = slot(b->gbp, b->n_gbp, s->n_nt);
+            b->gbp[h].threads = 64 * s->bpm_waves;   // the same for every system of one n_nt
+            const int ns = 16 * s->bpm_waves;
+            for (int l = 0; l < (nm + ns - 1) / ns; l++) mbp[h].push_back(make_int2(i, l));
+        } else {
+            const int g = slot(b->gsf, b->n_gsf, s->s64_nt);
+            b->gsf[g].lds = std::max(b->gsf[g].lds, sf64_lds_bytes(s->klmax, s->kmmax, s->s64_nt));
+            for (int l = 0; l < r.s64_nsb * s->s64_ngrp; l++) msf[g].push_back(make_int2(i, l));
+            const int h = slot(b->gbp, b->n_gbp, s->b64_nks * 4 + s->b64_ng);
+            for (int l = 0; l < r.b64_nsb * s->b64_nrg; l++) mbp[h].push_back(make_int2(i, l));
+        }
+    }
+    for (int g = 0; g < b->n_gsf; g++) {
+        b->gsf[g].n = (int)msf[g].size();
+        HIPCHK(upload(b->gsf[g].map, msf[g].data(), msf[g].size()));
+    }
+    for (int g = 0; g < b->n_gbp; g++) {
+        b->gbp[g].n = (int)mbp[g].size();
+        HIPCHK(upload(b->gbp[g].map, mbp[g].data(), mbp[g].size()));
+    }
+    return MDF_OK;
+}
+
 static int batch_upload(mdf_batch *b, bool maps) {
     const int n = (int)b->sys.size();
     b->h_rep.resize(n);
     std::vector<int2> mp[BK_N];
     for (int i = 0; i < n; i++) {
         fill_rep(b->sys[i], b->h_rep[i]);
+        rep_mode(b, b->sys[i], b->h_rep[i]);
         int cnt[BK_N];
         rep_blocks(b, b->sys[i], b->h_rep[i], cnt);
         for (int k = 0; k < BK_N; k++)
@@ -2957,6 +3051,8 @@ static int batch_upload(mdf_batch *b, bool maps) {
             b->nmap[k] = (int)mp[k].size();
             HIPCHK(upload(b->maps[k], mp[k].data(), mp[k].size()));
         }
+        int rc = batch_groups(b);
+        if (rc != MDF_OK) return rc;
     }
     HIPCHK(hipStreamSynchronize(b->stream));
     return MDF_OK;
@@ -3008,14 +3104,54 @@ static void batch_launch(mdf_batch *b) {
 #undef MDF_PB2
 #undef MDF_PB3
     }
-    if (grid(BK_SF)) {
-        if (b->f32) k_sf_b<float><<<grid(BK_SF), SF_BLOCK, 0, st>>>(R, b->maps[BK_SF].p);
-        else k_sf_b<double><<<grid(BK_SF), SF_BLOCK, 0, st>>>(R, b->maps[BK_SF].p);
-    }
-    if (grid(BK_SFR)) k_sf_reduce_b<<<grid(BK_SFR), SFR_THREADS, 0, st>>>(R, b->maps[BK_SFR].p);
-    if (grid(BK_BP)) {
-        if (b->f32) k_bp_b<float><<<grid(BK_BP), 256, 0, st>>>(R, b->maps[BK_BP].p);
-        else k_bp_b<double><<<grid(BK_BP), 256, 0, st>>>(R, b->maps[BK_BP].p);
+    if (b->rmode == BR_MMA32) {
+        for (int g = 0; g < b->n_gsf; g++) {
+            const mdf_batch::Group &G = b->gsf[g];
+            switch (G.key) {
+#define SFMB(NT) case NT: k_sf_mma_b<NT><<<G.n, 256, G.lds, st>>>(R, G.map.p); break;
+                SFMB(1) SFMB(2) SFMB(3) SFMB(4) SFMB(5) SFMB(6) SFMB(7) SFMB(8)
+#undef SFMB
+            }
+        }
+        if (grid(BK_SFR)) k_sf_reduce_rows_b<<<grid(BK_SFR), SFR_THREADS, 0, st>>>(R, b->maps[BK_SFR].p);
+        for (int g = 0; g < b->n_gbp; g++) {
+            const mdf_batch::Group &G = b->gbp[g];
+            switch (G.key) {
+#define BPMB(NT) case NT: k_bp_mma_b<NT><<<G.n, G.threads, 0, st>>>(R, G.map.p); break;
+                BPMB(1) BPMB(2) BPMB(3) BPMB(4) BPMB(5) BPMB(6) BPMB(7) BPMB(8)
+#undef BPMB
+            }
+        }
+    } else if (b->rmode == BR_MMA64) {
+        for (int g = 0; g < b->n_gsf; g++) {
+            const mdf_batch::Group &G = b->gsf[g];
+            switch (G.key) {
+#define SF64B(NT) case NT: k_sf64_b<NT><<<G.n, 256, G.lds, st>>>(R, G.map.p); break;
+                SF64B(1) SF64B(2) SF64B(3) SF64B(4) SF64B(5) SF64B(6) SF64B(7) SF64B(8)
+#undef SF64B
+            }
+        }
+        if (grid(BK_SFR)) k_sf64_reduce_b<<<grid(BK_SFR), SFR_THREADS, 0, st>>>(R, b->maps[BK_SFR].p);
+        for (int g = 0; g < b->n_gbp; g++) {
+            const mdf_batch::Group &G = b->gbp[g];
+            switch (G.key) {
+#define BP64B(NKS, NG) case NKS * 4 + NG: k_bp64_b<NKS, NG><<<G.n, 256, 0, st>>>(R, G.map.p); break;
+                BP64B(4, 1) BP64B(8, 1) BP64B(12, 1) BP64B(16, 1) BP64B(20, 1) BP64B(24, 1) BP64B(28, 1)
+                BP64B(32, 1) BP64B(4, 2) BP64B(8, 2) BP64B(12, 2) BP64B(16, 2) BP64B(20, 2) BP64B(24, 2)
+                BP64B(28, 2) BP64B(32, 2)
+#undef BP64B
+            }
+        }
+    } else {
+        if (grid(BK_SF)) {
+            if (b->f32) k_sf_b<float><<<grid(BK_SF), SF_BLOCK, 0, st>>>(R, b->maps[BK_SF].p);
+            else k_sf_b<double><<<grid(BK_SF), SF_BLOCK, 0, st>>>(R, b->maps[BK_SF].p);
+        }
+        if (grid(BK_SFR)) k_sf_reduce_b<<<grid(BK_SFR), SFR_THREADS, 0, st>>>(R, b->maps[BK_SFR].p);
+        if (grid(BK_BP)) {
+            if (b->f32) k_bp_b<float><<<grid(BK_BP), 256, 0, st>>>(R, b->maps[BK_BP].p);
+            else k_bp_b<double><<<grid(BK_BP), 256, 0, st>>>(R, b->maps[BK_BP].p);
+        }
     }
     k_epilogue_b<<<grid(BK_EPI), EPI_BLOCK, 0, st>>>(R, b->maps[BK_EPI].p);
     k_colsum_b<<<grid(BK_COLSUM), 256, 0, st>>>(R, b->maps[BK_COLSUM].p);
@@ -3065,6 +3201,17 @@ int mdf_batch_create(mdf_system *const *systems, int32_t n, mdf_batch **out) {
     b->pair_strided = s0->pair_strided;
     b->one_cls = true;
     for (int i = 0; i < n; i++) b->one_cls = b->one_cls && systems[i]->slot_type.size() <= 1;
+    {   // the matrix-core k-sum when every system with a k-sum runs it alone
+        bool m32 = true, m64 = true;
+        for (int i = 0; i < n; i++) {
+            const mdf_system *s = systems[i];
+            if (s->ak.empty() || s->top.n_move == 0) continue;
+            m32 = m32 && s->sf_mma && s->bp_mma;
+            m64 = m64 && s->mma64;
+        }
+        b->rmode = b->f32 ? (m32 ? BR_MMA32 : BR_VALU) : (m64 ? BR_MMA64 : BR_VALU);
+        if (const char *e = getenv("MDF_BATCH_VALU")) if (e[0] == '1') b->rmode = BR_VALU;
+    }
     int rc = MDF_OK;
     hipError_t e = hipSetDevice(b->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);

@@ -328,6 +328,18 @@ struct RepArgs {
     double *fpart;
     double cux, cuy, cuz, pipix, pipiy, pipiz, lamb;
     // epilogue, column sums, final sums
+    // the reciprocal sum on the matrix cores: FP32 form (k_sf_mma, k_sf_reduce_rows, k_bp_mma;
+    // sf_partial holds FP32 partial rows) and FP64 form (k_sf64, k_sf64_reduce, k_bp64)
+    int knmax, n_nt, sfm_sites, sfm_nsb, sfm_nrg;
+    const int2 *rowkk, *tile_ks;
+    unsigned *tpack;
+    int *tscale;
+    int klmax, kmmax, s64_nsb, s64_pcols, s64_sites, b64_nsb;
+    const int2 *s64_rowkk, *s64_groups, *b64_tile, *b64_rg;
+    const int *s64_tilent, *s64_pcoff, *s64_sorted_of;
+    double2 *s64_part;
+    const int4 *b64_rinfo;
+    double *b64_tpk;
     const WallLR *wl;
     int ew3dc, n_groups, n_epblk, colper, n_colblk, pad2;
     double crf0;

@@ -265,12 +265,14 @@ __device__ __forceinline__ int wave_max(int v) {
     return v;
 }
 
+// block (bx, by) of a (site blocks) x (nby row groups) grid
 template <int NT>
-__global__ __launch_bounds__(256) void k_sf_mma(const double *__restrict__ x, const double *__restrict__ y,
-                                                const double *__restrict__ z, const double *__restrict__ q, int n,
-                                                double cux, double cuy, double cuz, const KRow *__restrict__ rows,
-                                                int n_rows, int knmax, int n_nt, float2 *__restrict__ partial,
-                                                int nk, int sb_sites, const MtsState *__restrict__ mts) {
+__device__ __forceinline__ void sfm_body(const double *__restrict__ x, const double *__restrict__ y,
+                                         const double *__restrict__ z, const double *__restrict__ q, int n,
+                                         double cux, double cuy, double cuz, const KRow *__restrict__ rows,
+                                         int n_rows, int knmax, int n_nt, float2 *__restrict__ partial,
+                                         int nk, int sb_sites, const MtsState *__restrict__ mts, int bx, int by,
+                                         int nby) {
     if (mts && !mts->exact) return;   // a surrogate step: no k-sum
     // dynamic LDS, laid out as sfm_lds_bytes(sb_sites, n_nt) counts it
     extern __shared__ __align__(16) unsigned char sfm_smem[];
@@ -284,12 +286,12 @@ __global__ __launch_bounds__(256) void k_sf_mma(const double *__restrict__ x, co
     unsigned *sPh = sEl + SFM_SUB * kp;
     unsigned *sPl = sPh + SFM_SUB * SFM_RP;
     __shared__ int sKl[SFM_ROWS], sKm[SFM_ROWS], sFresh[SFM_ROWS];
-    const int sb = blockIdx.x;
+    const int sb = bx;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     // the group's row tiles [t0, t1) of n_mt, at most SFM_ROWS / 16 of them
     const int n_mt = (n_rows + 15) >> 4;
-    const int t0 = (int)(((long long)blockIdx.y * n_mt) / gridDim.y);
-    const int t1 = (int)(((long long)(blockIdx.y + 1) * n_mt) / gridDim.y);
+    const int t0 = (int)(((long long)by * n_mt) / nby);
+    const int t1 = (int)(((long long)(by + 1) * n_mt) / nby);
     const int row0 = t0 * 16;
     const int nr_g = min(16 * (t1 - t0), n_rows - row0);   // rows of the group
     const int wrow0 = row0 + wave * 16;
@@ -436,10 +438,30 @@ __global__ __launch_bounds__(256) void k_sf_mma(const double *__restrict__ x, co
     }
 }
 
+template <int NT>
+__global__ __launch_bounds__(256) void k_sf_mma(const double *__restrict__ x, const double *__restrict__ y,
+                                                const double *__restrict__ z, const double *__restrict__ q, int n,
+                                                double cux, double cuy, double cuz, const KRow *__restrict__ rows,
+                                                int n_rows, int knmax, int n_nt, float2 *__restrict__ partial,
+                                                int nk, int sb_sites, const MtsState *__restrict__ mts) {
+    sfm_body<NT>(x, y, z, q, n, cux, cuy, cuz, rows, n_rows, knmax, n_nt, partial, nk, sb_sites, mts, blockIdx.x,
+                 blockIdx.y, gridDim.y);
+}
+// batched: block map entry (system, local block), local block = row group * site blocks + site block
+template <int NT>
+__global__ __launch_bounds__(256) void k_sf_mma_b(const RepArgs *__restrict__ RA, const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    const RepArgs &r = RA[m.x];
+    sfm_body<NT>(r.S.x, r.S.y, r.S.z, r.S.q, r.S.n_move, r.cux, r.cuy, r.cuz, r.rows, r.nrows, r.knmax, r.n_nt,
+                 reinterpret_cast<float2 *>(r.sf_partial), r.nk, r.sfm_sites, nullptr, m.y % r.sfm_nsb,
+                 m.y / r.sfm_nsb, r.sfm_nrg);
+}
+
 #define SFM_INST(NT)                                                                                            \
     template __global__ void k_sf_mma<NT>(const double *, const double *, const double *, const double *, int,   \
                                           double, double, double, const KRow *, int, int, int, float2 *, int, int, \
-                                          const MtsState *);
+                                          const MtsState *); \
+    template __global__ void k_sf_mma_b<NT>(const RepArgs *, const int2 *);
 SFM_INST(1) SFM_INST(2) SFM_INST(3) SFM_INST(4) SFM_INST(5) SFM_INST(6) SFM_INST(7) SFM_INST(8)
 #undef SFM_INST
 
@@ -459,23 +481,22 @@ SFM_INST(1) SFM_INST(2) SFM_INST(3) SFM_INST(4) SFM_INST(5) SFM_INST(6) SFM_INST
 constexpr int BPM_WAVES = 8;                 // most waves per block (blockDim.x / 64 may be fewer)
 constexpr int BPM_ROWS_MAX = 1024;           // rows whose (kl, km) the block keeps in LDS
 
+// block bx of the site blocks (16 sites per wave)
 template <int NT>
-__global__ __launch_bounds__(64 * BPM_WAVES) __attribute__((amdgpu_waves_per_eu(NT <= 7 ? 4 : 1))) void k_bp_mma(const double *__restrict__ x, const double *__restrict__ y,
-                                                           const double *__restrict__ z, const double *__restrict__ q,
-                                                           int n, double cux, double cuy, double cuz,
-                                                           const int2 *__restrict__ rowkk,
-                                                           const int2 *__restrict__ tile_ks, int n_rows, int knmax,
-                                                           int n_nt, const unsigned *__restrict__ tpack,
-                                                           const int *__restrict__ tscale, double pipix,
-                                                           double pipiy, double pipiz, double *__restrict__ fpart,
-                                                           const MtsState *__restrict__ mts) {
+__device__ __forceinline__ void bpm_body(const double *__restrict__ x, const double *__restrict__ y,
+                                         const double *__restrict__ z, const double *__restrict__ q, int n,
+                                         double cux, double cuy, double cuz, const int2 *__restrict__ rowkk,
+                                         const int2 *__restrict__ tile_ks, int n_rows, int knmax, int n_nt,
+                                         const unsigned *__restrict__ tpack, const int *__restrict__ tscale,
+                                         double pipix, double pipiy, double pipiz, double *__restrict__ fpart,
+                                         const MtsState *__restrict__ mts, int bx) {
     if (mts && !mts->exact) return;
     __shared__ unsigned sT[2][4][16][16 * NT + 4];   // row stride = 4 mod 8 words
     // per row: kl, km and the exponents of its two planes' scales, one signed byte each
     __shared__ int sInfo[BPM_ROWS_MAX];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, col = lane & 15;
-    const int site = (blockIdx.x * (int)(blockDim.x >> 6) + wave) * 16 + col;
+    const int site = (bx * (int)(blockDim.x >> 6) + wave) * 16 + col;
     const bool ok = site < n;
     const int kn_pad = 16 * NT, q4 = kn_pad >> 2;   // uint4 per (row, plane); n_nt == NT
     const int n_mt = (n_rows + 15) >> 4;
@@ -616,11 +637,33 @@ __global__ __launch_bounds__(64 * BPM_WAVES) __attribute__((amdgpu_waves_per_eu(
     }
 }
 
+template <int NT>
+__global__ __launch_bounds__(64 * BPM_WAVES) __attribute__((amdgpu_waves_per_eu(NT <= 7 ? 4 : 1))) void k_bp_mma(
+    const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
+    const double *__restrict__ q, int n, double cux, double cuy, double cuz, const int2 *__restrict__ rowkk,
+    const int2 *__restrict__ tile_ks, int n_rows, int knmax, int n_nt, const unsigned *__restrict__ tpack,
+    const int *__restrict__ tscale, double pipix, double pipiy, double pipiz, double *__restrict__ fpart,
+    const MtsState *__restrict__ mts) {
+    bpm_body<NT>(x, y, z, q, n, cux, cuy, cuz, rowkk, tile_ks, n_rows, knmax, n_nt, tpack, tscale, pipix, pipiy,
+                 pipiz, fpart, mts, blockIdx.x);
+}
+// batched: block map entry (system, site block); every system of the launch has NT kn tiles
+// and the launch's waves per block
+template <int NT>
+__global__ __launch_bounds__(64 * BPM_WAVES) __attribute__((amdgpu_waves_per_eu(NT <= 7 ? 4 : 1))) void k_bp_mma_b(
+    const RepArgs *__restrict__ RA, const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    const RepArgs &r = RA[m.x];
+    bpm_body<NT>(r.S.x, r.S.y, r.S.z, r.S.q, r.S.n_move, r.cux, r.cuy, r.cuz, r.rowkk, r.tile_ks, r.nrows, r.knmax,
+                 r.n_nt, r.tpack, r.tscale, r.pipix, r.pipiy, r.pipiz, r.fpart, nullptr, m.y);
+}
+
 #define BPM_INST(NT)                                                                                     \
     template __global__ void k_bp_mma<NT>(const double *, const double *, const double *, const double *, int, \
                                           double, double, double, const int2 *, const int2 *, int, int, int,    \
                                           const unsigned *, const int *, double, double, double, double *,      \
-                                          const MtsState *);
+                                          const MtsState *); \
+    template __global__ void k_bp_mma_b<NT>(const RepArgs *, const int2 *);
 BPM_INST(1) BPM_INST(2) BPM_INST(3) BPM_INST(4) BPM_INST(5) BPM_INST(6) BPM_INST(7) BPM_INST(8)
 #undef BPM_INST
 
@@ -758,17 +801,18 @@ __device__ __forceinline__ int pow2_scale(float m) {
     return min(100, max(-100, 15 - ex));          // m 2^(15 - ex) in [2^14, 2^15)
 }
 
-__global__ __launch_bounds__(SFR_THREADS) void k_sf_reduce_rows(
+// row rb of the k-set
+__device__ __forceinline__ void sfr_rows_body(
     const double2 *__restrict__ partial, const float2 *__restrict__ partial32, int n_sb, int nk,
     const KRow *__restrict__ rows, const double2 *__restrict__ s_wall, const double *__restrict__ ak,
     double pipix, double pipiy, double pipiz, double lamb, double2 *__restrict__ s_out, double4 *__restrict__ t_out,
     double *__restrict__ epart, const MtsState *__restrict__ mts, unsigned *__restrict__ tpack,
-    int *__restrict__ tscale, int kn_pad, int knmax) {
+    int *__restrict__ tscale, int kn_pad, int knmax, int rb) {
     if (mts && !mts->exact) return;
     constexpr int SL = SFR_THREADS / SFR_K;
     __shared__ double2 slice[SL][SFR_K];
     const int kk = threadIdx.x % SFR_K, sl = threadIdx.x / SFR_K;
-    const KRow row = rows[blockIdx.x];
+    const KRow row = rows[rb];
     const int len = row.kn1 - row.kn0 + 1;
     double acc[RE_N];
 #pragma unroll
@@ -846,7 +890,7 @@ __global__ __launch_bounds__(SFR_THREADS) void k_sf_reduce_rows(
         const int j = ch * SFR_K + kk;
         if (j < len) {
             const int kn = row.kn0 + j;
-            const size_t base = (size_t)blockIdx.x * 4 * kn_pad + (size_t)(kn + knmax);
+            const size_t base = (size_t)rb * 4 * kn_pad + (size_t)(kn + knmax);
             unsigned h, l;
             split_c((float)ldexp(tr[ch], st), (float)ldexp(ti[ch], st), h, l);
             tpack[base] = h;
@@ -856,13 +900,32 @@ __global__ __launch_bounds__(SFR_THREADS) void k_sf_reduce_rows(
             tpack[base + 3 * kn_pad] = l;
         }
     }
-    if (kk == 0) tscale[blockIdx.x] = (st & 0xff) | ((sz & 0xff) << 8);
+    if (kk == 0) tscale[rb] = (st & 0xff) | ((sz & 0xff) << 8);
 #pragma unroll
     for (int c = 0; c < RE_N; c++)
         for (int o = 32; o > 0; o >>= 1) acc[c] += __shfl_xor(acc[c], o, 64);
     if (kk == 0)
 #pragma unroll
-        for (int c = 0; c < RE_N; c++) epart[blockIdx.x * RE_N + c] = acc[c];
+        for (int c = 0; c < RE_N; c++) epart[rb * RE_N + c] = acc[c];
+}
+
+__global__ __launch_bounds__(SFR_THREADS) void k_sf_reduce_rows(
+    const double2 *__restrict__ partial, const float2 *__restrict__ partial32, int n_sb, int nk,
+    const KRow *__restrict__ rows, const double2 *__restrict__ s_wall, const double *__restrict__ ak,
+    double pipix, double pipiy, double pipiz, double lamb, double2 *__restrict__ s_out, double4 *__restrict__ t_out,
+    double *__restrict__ epart, const MtsState *__restrict__ mts, unsigned *__restrict__ tpack,
+    int *__restrict__ tscale, int kn_pad, int knmax) {
+    sfr_rows_body(partial, partial32, n_sb, nk, rows, s_wall, ak, pipix, pipiy, pipiz, lamb, s_out, t_out, epart, mts,
+                  tpack, tscale, kn_pad, knmax, blockIdx.x);
+}
+// batched: block map entry (system, row)
+__global__ __launch_bounds__(SFR_THREADS) void k_sf_reduce_rows_b(const RepArgs *__restrict__ RA,
+                                                                  const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    const RepArgs &r = RA[m.x];
+    sfr_rows_body(r.sf_partial, r.S.n_move > 0 ? reinterpret_cast<const float2 *>(r.sf_partial) : nullptr,
+                  r.S.n_move > 0 ? r.sfm_nsb : 0, r.nk, r.rows, r.s_wall, r.ak, r.pipix, r.pipiy, r.pipiz, r.lamb,
+                  r.s_out, r.t_out, r.kpart, nullptr, r.tpack, r.tscale, 16 * r.n_nt, r.knmax, m.y);
 }
 
 // T4[k] = (A conj S, kn A conj S): per k-vector the force pieces are Im(T e) and kn Im(T e),

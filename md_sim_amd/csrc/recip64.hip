@@ -64,6 +64,7 @@ struct Sf64Args {
     int pcols;
     double2 *partial;
     int sb_sites;
+    int bx, by;   // the block: site block, row group
 };
 
 // the stage loop of one wave whose tile reaches NTC column tiles (a compile-time count, so
@@ -81,12 +82,12 @@ __device__ __forceinline__ void sf64_run(const Sf64Args &A, int tile, bool activ
     double *sE = reinterpret_cast<double *>(sY + SF64_SUB * kmp);   // [SUB][ep]
     const int lane = threadIdx.x & 63;
     const int g = lane >> 4, c = lane & 15;
-    const int2 grp = A.groups[blockIdx.y];
+    const int2 grp = A.groups[A.by];
     const int ncol_grp = 16 * A.tile_nt[grp.x];   // tiles are sorted widest first
     const int2 kk = A.rowkk[16 * (active ? tile : 0) + c];   // padded past n_rows with (0, 0)
     const int my_kl = kk.x, my_km = abs(kk.y);
     const double ysg = kk.y < 0 ? -1.0 : 1.0;   // e^{-i pi km v} = conj e^{i pi km v}
-    const int s_begin = blockIdx.x * A.sb_sites, s_end = min(A.n, s_begin + A.sb_sites);
+    const int s_begin = A.bx * A.sb_sites, s_end = min(A.n, s_begin + A.sb_sites);
     constexpr int NA = NTC > 0 ? NTC : 1;
     d4v accR[NA], accI[NA];
 #pragma unroll
@@ -177,7 +178,7 @@ __device__ __forceinline__ void sf64_run(const Sf64Args &A, int tile, bool activ
             const int i = 16 * tile + 4 * e + g;
             if (i >= A.n_rows) continue;
             const int off = A.pcoff[i], ncol = A.pcoff[i + 1] - off;
-            double2 *dst = A.partial + (size_t)blockIdx.x * A.pcols + off;
+            double2 *dst = A.partial + (size_t)A.bx * A.pcols + off;
 #pragma unroll
             for (int t = 0; t < NTC; t++) {
                 const int col = 16 * t + c;
@@ -188,18 +189,10 @@ __device__ __forceinline__ void sf64_run(const Sf64Args &A, int tile, bool activ
 }
 
 template <int NT>
-__global__ __launch_bounds__(256) void k_sf64(const double *__restrict__ x, const double *__restrict__ y,
-                                              const double *__restrict__ z, const double *__restrict__ q, int n,
-                                              double cux, double cuy, double cuz, const int2 *__restrict__ rowkk,
-                                              const int *__restrict__ tile_nt, const int2 *__restrict__ groups,
-                                              int n_rows, int klmax, int kmmax, const int *__restrict__ pcoff,
-                                              int pcols, double2 *__restrict__ partial, int sb_sites,
-                                              const MtsState *__restrict__ mts) {
-    if (mts && !mts->exact) return;   // a surrogate step: no k-sum
-    const Sf64Args A{x, y, z, q, n, cux, cuy, cuz, rowkk, tile_nt, groups, n_rows, klmax, kmmax, pcoff, pcols,
-                     partial, sb_sites};
+__device__ __forceinline__ void sf64_body(const Sf64Args &A) {
     const int wave = threadIdx.x >> 6;
-    const int2 grp = groups[blockIdx.y];
+    const int2 grp = A.groups[A.by];
+    const int *tile_nt = A.tile_nt;
     const int tile = grp.x + wave;
     const bool active = tile < grp.y;   // uniform over the wave
     const int nt = active ? tile_nt[tile] : 0;
@@ -217,10 +210,35 @@ __global__ __launch_bounds__(256) void k_sf64(const double *__restrict__ x, cons
     }
 }
 
+template <int NT>
+__global__ __launch_bounds__(256) void k_sf64(const double *__restrict__ x, const double *__restrict__ y,
+                                              const double *__restrict__ z, const double *__restrict__ q, int n,
+                                              double cux, double cuy, double cuz, const int2 *__restrict__ rowkk,
+                                              const int *__restrict__ tile_nt, const int2 *__restrict__ groups,
+                                              int n_rows, int klmax, int kmmax, const int *__restrict__ pcoff,
+                                              int pcols, double2 *__restrict__ partial, int sb_sites,
+                                              const MtsState *__restrict__ mts) {
+    if (mts && !mts->exact) return;   // a surrogate step: no k-sum
+    const Sf64Args A{x, y, z, q, n, cux, cuy, cuz, rowkk, tile_nt, groups, n_rows, klmax, kmmax, pcoff, pcols,
+                     partial, sb_sites, (int)blockIdx.x, (int)blockIdx.y};
+    sf64_body<NT>(A);
+}
+// batched: block map entry (system, local block), local block = row group * site blocks + site block
+template <int NT>
+__global__ __launch_bounds__(256) void k_sf64_b(const RepArgs *__restrict__ RA, const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    const RepArgs &r = RA[m.x];
+    const Sf64Args A{r.S.x, r.S.y, r.S.z, r.S.q, r.S.n_move, r.cux, r.cuy, r.cuz, r.s64_rowkk, r.s64_tilent,
+                     r.s64_groups, r.nrows, r.klmax, r.kmmax, r.s64_pcoff, r.s64_pcols, r.s64_part, r.s64_sites,
+                     m.y % r.s64_nsb, m.y / r.s64_nsb};
+    sf64_body<NT>(A);
+}
+
 #define SF64_INST(NT)                                                                                            \
     template __global__ void k_sf64<NT>(const double *, const double *, const double *, const double *, int,     \
                                         double, double, double, const int2 *, const int *, const int2 *, int, int, \
-                                        int, const int *, int, double2 *, int, const MtsState *);
+                                        int, const int *, int, double2 *, int, const MtsState *);                \
+    template __global__ void k_sf64_b<NT>(const RepArgs *, const int2 *);
 SF64_INST(1) SF64_INST(2) SF64_INST(3) SF64_INST(4) SF64_INST(5) SF64_INST(6) SF64_INST(7) SF64_INST(8)
 #undef SF64_INST
 
@@ -233,17 +251,16 @@ SF64_INST(1) SF64_INST(2) SF64_INST(3) SF64_INST(4) SF64_INST(5) SF64_INST(6) SF
 constexpr int SF64R_COLS = 128;   // columns of a row: 2 E + 1 <= 127
 constexpr int SF64R_SLICES = SFR_THREADS / SF64R_COLS;
 
-__global__ __launch_bounds__(SFR_THREADS) void k_sf64_reduce(
+// row r of the k-set (reference order)
+__device__ __forceinline__ void sf64r_body(
     const double2 *__restrict__ partial, int n_sb, int pcols, const int *__restrict__ pcoff,
     const int *__restrict__ sorted_of, const KRow *__restrict__ rows, const double2 *__restrict__ s_wall,
     const double *__restrict__ ak, double pipix, double pipiy, double pipiz, double lamb,
     double2 *__restrict__ s_out, double4 *__restrict__ t_out, double *__restrict__ epart,
-    const int2 *__restrict__ bptile, double *__restrict__ tpk, const MtsState *__restrict__ mts) {
-    if (mts && !mts->exact) return;
+    const int2 *__restrict__ bptile, double *__restrict__ tpk, int r) {
     __shared__ double2 red[SF64R_SLICES][SF64R_COLS];
     __shared__ double2 sSum[SF64R_COLS];
     __shared__ double2 sTp[64], sTm[64];   // T(row, +n), T(row, -n) (zero outside the row)
-    const int r = blockIdx.x;
     const KRow row = rows[r];
     const int i = sorted_of[r];
     const int off = pcoff[i], ncol = pcoff[i + 1] - off, E = (ncol - 1) / 2;
@@ -336,6 +353,26 @@ __global__ __launch_bounds__(SFR_THREADS) void k_sf64_reduce(
     }
 }
 
+__global__ __launch_bounds__(SFR_THREADS) void k_sf64_reduce(
+    const double2 *__restrict__ partial, int n_sb, int pcols, const int *__restrict__ pcoff,
+    const int *__restrict__ sorted_of, const KRow *__restrict__ rows, const double2 *__restrict__ s_wall,
+    const double *__restrict__ ak, double pipix, double pipiy, double pipiz, double lamb,
+    double2 *__restrict__ s_out, double4 *__restrict__ t_out, double *__restrict__ epart,
+    const int2 *__restrict__ bptile, double *__restrict__ tpk, const MtsState *__restrict__ mts) {
+    if (mts && !mts->exact) return;
+    sf64r_body(partial, n_sb, pcols, pcoff, sorted_of, rows, s_wall, ak, pipix, pipiy, pipiz, lamb, s_out, t_out,
+               epart, bptile, tpk, blockIdx.x);
+}
+// batched: block map entry (system, row)
+__global__ __launch_bounds__(SFR_THREADS) void k_sf64_reduce_b(const RepArgs *__restrict__ RA,
+                                                               const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    const RepArgs &r = RA[m.x];
+    sf64r_body(r.s64_part, r.S.n_move > 0 ? r.s64_nsb : 0, r.s64_pcols, r.s64_pcoff, r.s64_sorted_of, r.rows,
+               r.s_wall, r.ak, r.pipix, r.pipiy, r.pipiz, r.lamb, r.s_out, r.t_out, r.kpart, r.b64_tile, r.b64_tpk,
+               m.y);
+}
+
 // ---- back-projection ------------------------------------------------------------------------
 // Grid (site blocks, row groups); a wave owns NG groups of 16 sites and keeps their cos/sin
 // operand (column 4 ks + g for K-step ks, lane group g) in registers.  The block stages each
@@ -357,21 +394,20 @@ __device__ __forceinline__ void bp64_products(const double *sa, int lane, const 
     }
 }
 
+// block (bx, by) of a (site blocks) x (row groups) grid
 template <int NKS, int NG>
-__global__ __launch_bounds__(64 * BP64_WAVES) void k_bp64(const double *__restrict__ x, const double *__restrict__ y,
-                                                         const double *__restrict__ z, const double *__restrict__ q,
-                                                         int n, double cux, double cuy, double cuz,
-                                                         const int4 *__restrict__ rinfo, int n_rows,
-                                                         const int2 *__restrict__ bptile, const int2 *__restrict__ rgroups,
-                                                         const double *__restrict__ tpk, double pipix, double pipiy,
-                                                         double pipiz, double *__restrict__ fpart,
-                                                         const MtsState *__restrict__ mts) {
-    if (mts && !mts->exact) return;
+__device__ __forceinline__ void bp64_body(const double *__restrict__ x, const double *__restrict__ y,
+                                          const double *__restrict__ z, const double *__restrict__ q, int n,
+                                          double cux, double cuy, double cuz, const int4 *__restrict__ rinfo,
+                                          int n_rows, const int2 *__restrict__ bptile,
+                                          const int2 *__restrict__ rgroups, const double *__restrict__ tpk,
+                                          double pipix, double pipiy, double pipiz, double *__restrict__ fpart,
+                                          int bx, int by) {
     __shared__ double sA[2][NKS][64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, c = lane & 15;
-    const int2 rg = rgroups[blockIdx.y];   // M-tiles [rg.x, rg.y)
-    const int site0 = (blockIdx.x * BP64_WAVES + wave) * 16 * NG;
+    const int2 rg = rgroups[by];   // M-tiles [rg.x, rg.y)
+    const int site0 = (bx * BP64_WAVES + wave) * 16 * NG;
     // an M-tile's operand goes through registers: its global loads are issued before the
     // previous tile's products and written to LDS after them, so their latency is hidden
     constexpr int PF = (NKS * 64 + 64 * BP64_WAVES - 1) / (64 * BP64_WAVES);   // doubles per thread
@@ -477,7 +513,7 @@ __global__ __launch_bounds__(64 * BP64_WAVES) void k_bp64(const double *__restri
         az += __shfl_xor(az, 16, 64); az += __shfl_xor(az, 32, 64);
         const int i = site0 + 16 * ng + c;
         if (g == 0 && i < n) {
-            double *fp = fpart + (size_t)blockIdx.y * 3 * n;
+            double *fp = fpart + (size_t)by * 3 * n;
             fp[i] = qq[ng] * pipix * ax;
             fp[(size_t)n + i] = qq[ng] * pipiy * ay;
             fp[2 * (size_t)n + i] = qq[ng] * pipiz * az;
@@ -485,10 +521,33 @@ __global__ __launch_bounds__(64 * BP64_WAVES) void k_bp64(const double *__restri
     }
 }
 
+template <int NKS, int NG>
+__global__ __launch_bounds__(64 * BP64_WAVES) void k_bp64(const double *__restrict__ x, const double *__restrict__ y,
+                                                         const double *__restrict__ z, const double *__restrict__ q,
+                                                         int n, double cux, double cuy, double cuz,
+                                                         const int4 *__restrict__ rinfo, int n_rows,
+                                                         const int2 *__restrict__ bptile, const int2 *__restrict__ rgroups,
+                                                         const double *__restrict__ tpk, double pipix, double pipiy,
+                                                         double pipiz, double *__restrict__ fpart,
+                                                         const MtsState *__restrict__ mts) {
+    if (mts && !mts->exact) return;
+    bp64_body<NKS, NG>(x, y, z, q, n, cux, cuy, cuz, rinfo, n_rows, bptile, rgroups, tpk, pipix, pipiy, pipiz, fpart,
+                       blockIdx.x, blockIdx.y);
+}
+// batched: block map entry (system, local block), local block = row group * site blocks + site block
+template <int NKS, int NG>
+__global__ __launch_bounds__(64 * BP64_WAVES) void k_bp64_b(const RepArgs *__restrict__ RA, const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    const RepArgs &r = RA[m.x];
+    bp64_body<NKS, NG>(r.S.x, r.S.y, r.S.z, r.S.q, r.S.n_move, r.cux, r.cuy, r.cuz, r.b64_rinfo, r.nrows, r.b64_tile,
+                       r.b64_rg, r.b64_tpk, r.pipix, r.pipiy, r.pipiz, r.fpart, m.y % r.b64_nsb, m.y / r.b64_nsb);
+}
+
 #define BP64_INST(NKS, NG)                                                                                       \
     template __global__ void k_bp64<NKS, NG>(const double *, const double *, const double *, const double *, int,  \
                                              double, double, double, const int4 *, int, const int2 *, const int2 *, \
-                                             const double *, double, double, double, double *, const MtsState *);
+                                             const double *, double, double, double, double *, const MtsState *); \
+    template __global__ void k_bp64_b<NKS, NG>(const RepArgs *, const int2 *);
 BP64_INST(4, 1) BP64_INST(8, 1) BP64_INST(12, 1) BP64_INST(16, 1) BP64_INST(20, 1) BP64_INST(24, 1) BP64_INST(28, 1)
 BP64_INST(32, 1) BP64_INST(4, 2) BP64_INST(8, 2) BP64_INST(12, 2) BP64_INST(16, 2) BP64_INST(20, 2) BP64_INST(24, 2)
 BP64_INST(28, 2) BP64_INST(32, 2)

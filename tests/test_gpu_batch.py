@@ -25,53 +25,65 @@ def _ptrs(ts, s):
     return tuple(t.data_ptr() for t in ts[:3]) + (tuple(t.data_ptr() for t in ts[3:]) if s.nrig else (0, 0, 0))
 
 
-@pytest.mark.parametrize("precision", [abi.FP64, abi.FP32_ACC64])
-def test_batch_equals_single_handles(precision, monkeypatch):
-    """A mixed batch (slab, bulk 3-D form, water-free crystal, replicas of uneven size) gives
-    bit for bit what each handle gives alone with the same kernels.  The batched reciprocal
-    kernels are the vector-ALU ones; a single handle by default runs its reciprocal sum on
-    the matrix cores, which agrees with the batch to FP64 rounding (FP64 form) or at the FP32
-    form's budget."""
-    from md_sim_amd.engine import ForceBatch, ForceEngine
-    _say("case batch vs single, precision", precision)
+def _mixed():
     syss = [systems.config("pore_small"), systems.config("bulk216"), systems.rocksalt_bulk(2, rcut=5.5, kmax=5)]
     syss += systems.replica_set(3, seed=4100)
     kws = [{}, {}, dict(z_mode=abi.Z_PERIODIC)] + [{}] * 3
-    engines, singles, mma, dev = [], [], [], []
+    return syss, kws
+
+
+@pytest.mark.parametrize("precision", [abi.FP64, abi.FP32_ACC64])
+@pytest.mark.parametrize("kernels", ["matrix", "valu"])
+def test_batch_equals_single_handles(precision, kernels, monkeypatch):
+    """A mixed batch (slab, bulk 3-D form, water-free crystal, replicas of uneven size) gives
+    bit for bit what each handle gives alone.  By default the handles run their k-sum on the
+    matrix cores (FP64 products in the FP64 form, FP16 hi/lo products in the FP32 form) and
+    so does the batch, one launch per kernel template over the systems that share it; with
+    the vector-ALU k-sum everywhere (MDF_SF_MMA=0 MDF_BP_MMA=0 MDF_MMA64=0) the batch runs
+    that one."""
+    from md_sim_amd.engine import ForceBatch, ForceEngine
+    _say("case batch vs single, precision", precision, kernels)
+    syss, kws = _mixed()
+    if kernels == "valu":
+        monkeypatch.setenv("MDF_SF_MMA", "0")
+        monkeypatch.setenv("MDF_BP_MMA", "0")
+        monkeypatch.setenv("MDF_MMA64", "0")
+    engines, singles, dev = [], [], []
     for s, kw in zip(syss, kws):
         pos = abi.positions(s)
         e = ForceEngine(s, precision=precision, **kw)
-        mma.append(e.force_get(*pos))
+        assert e.reciprocal_on_matrix_cores() == (kernels == "matrix")
+        singles.append(e.force_get(*pos))
         engines.append(e)
         dev.append(_dev(s, pos))
-    with monkeypatch.context() as m:
-        m.setenv("MDF_SF_MMA", "0")
-        m.setenv("MDF_BP_MMA", "0")
-        m.setenv("MDF_MMA64", "0")
-        for s, kw in zip(syss, kws):
-            ev = ForceEngine(s, precision=precision, **kw)
-            singles.append(ev.force_get(*abi.positions(s)))
-            ev.close()
-    # the batch's force scale: the crystal's forces vanish by symmetry, so its FP32 noise is
-    # measured against the forces the other systems have
-    fscale = max(np.abs(r["f_site"]).max() for r in singles)
     b = ForceBatch(engines)
     for rep in range(3):
         b.compute_device([_ptrs(d, s) for d, s in zip(dev, syss)])
         outs = b.read_outputs()
-        for i, (o, r, rm) in enumerate(zip(outs, singles, mma)):
+        for i, (o, r) in enumerate(zip(outs, singles)):
             assert np.array_equal(o["f_site"], r["f_site"]), (rep, i)
             assert np.array_equal(o["torque"], r["torque"]), (rep, i)
             assert o["e_conf"] == r["e_conf"] and o["e_recip"] == r["e_recip"], (rep, i)
             assert np.array_equal(o["e_vdw"], r["e_vdw"]) and np.array_equal(o["e_coul_real"], r["e_coul_real"])
             assert o["n_pairs_listed"] == r["n_pairs_listed"], (rep, i)
-            if precision == abi.FP64:
-                d = np.abs(rm["f_site"] - o["f_site"]).max()
-                assert d <= 1e-12 * fscale, (rep, i, d, fscale)
-            else:
-                d = np.abs(rm["f_site"] - o["f_site"]).max()
-                assert d <= 1e-5 * fscale, (rep, i, d, fscale)
     b.close()
+    if kernels == "matrix":
+        # the same handles batched with the vector-ALU k-sum forced: forces no longer equal
+        # the handles' bit for bit (so the batch above did run the matrix-core kernels), and
+        # agree to FP64 rounding (FP64 form) or at the FP32 form's budget
+        fscale = max(np.abs(r["f_site"]).max() for r in singles)
+        monkeypatch.setenv("MDF_BATCH_VALU", "1")
+        bv = ForceBatch(engines)
+        monkeypatch.delenv("MDF_BATCH_VALU")
+        bv.compute_device([_ptrs(d, s) for d, s in zip(dev, syss)])
+        outs = bv.read_outputs()
+        assert any(not np.array_equal(o["f_site"], r["f_site"]) for o, r in zip(outs, singles))
+        for i, (o, r) in enumerate(zip(outs, singles)):
+            d = np.abs(r["f_site"] - o["f_site"]).max()
+            assert d <= (1e-12 if precision == abi.FP64 else 1e-5) * fscale, (i, d, fscale)
+        bv.close()
+    for e in engines:
+        e.close()
 
 
 def test_batch_trajectory_matches_oracle(oracle_lib):

@@ -116,16 +116,16 @@ def _batch_step(syss, prec):
 
 
 def test_replica_batch_64_fp64_against_oracle_and_handles(oracle_lib, replicas):
-    """The benchmark's 64-replica batch in FP64: every replica agrees with its own handle to
-    FP64 rounding (the handle runs its reciprocal sum on the FP64 matrix cores, the batch on
-    the vector ALU), and a sample of replicas (smallest, largest, one between) matches the
-    oracle."""
+    """The benchmark's 64-replica batch in FP64: every replica equals its own handle bit for
+    bit (both run the k-sum on the FP64 matrix cores), and a sample of replicas (smallest,
+    largest, one between) matches the oracle."""
     _say("case 64-replica batch fp64")
     engines, out = _batch_step(replicas, abi.FP64)
     for i, (s, e) in enumerate(zip(replicas, engines)):
+        assert e.reciprocal_on_matrix_cores(), i
         r = e.force_get(*abi.positions(s))
-        assert rel_err(out[i]["f_site"], r["f_site"]) < 1e-12, i
-        assert abs(out[i]["e_conf"] - r["e_conf"]) <= 1e-12 * abs(r["e_conf"]), i
+        assert np.array_equal(out[i]["f_site"], r["f_site"]), i
+        assert out[i]["e_conf"] == r["e_conf"], i
     sizes = [s.nrig for s in replicas]
     sample = sorted({int(np.argmin(sizes)), int(np.argmax(sizes)), 17})
     for i in sample:
@@ -140,11 +140,15 @@ def test_replica_batch_64_fp64_against_oracle_and_handles(oracle_lib, replicas):
 
 
 def test_replica_batch_64_fp32_against_fp64(replicas):
-    """The same batch in the FP32 form against FP64 single handles, every replica."""
+    """The same batch in the FP32 form against FP64 single handles, every replica, and bit
+    for bit against its own FP32 handles (the FP16 hi/lo matrix-core k-sum in both)."""
     _say("case 64-replica batch fp32")
     engines, out = _batch_step(replicas, abi.FP32_ACC64)
     worst = 0.0
     for i, s in enumerate(replicas):
+        r32 = engines[i].force_get(*abi.positions(s))
+        assert engines[i].reciprocal_on_matrix_cores(), i
+        assert np.array_equal(out[i]["f_site"], r32["f_site"]) and out[i]["e_conf"] == r32["e_conf"], i
         r64 = _engine(s, precision=abi.FP64).force_get(*abi.positions(s))
         err = rms_rel(out[i]["f_site"], r64["f_site"])
         worst = max(worst, err)
