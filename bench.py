@@ -82,6 +82,8 @@ def parse():
     ap.add_argument("--mts-train", type=int, default=16, help="full-sum training steps before the scheme starts")
     ap.add_argument("--mts-tol", type=float, default=1e-3,
                     help="the scheme falls back to the full sum when its judged force RMS error exceeds this")
+    ap.add_argument("--mts-offline-frames", type=int, default=400,
+                    help="frames per training trajectory of the off-line trained network (0: skip it)")
     return ap.parse_args()
 
 
@@ -433,7 +435,40 @@ def replica_bench(args, prec):
     return out
 
 
-def mts_bench(args, workload):
+def surrogate_offline(args):
+    """Network weights trained off line on exact Ewald k-space forces of the interface and
+    replica workloads (16k waters on 6144 ions, and four replicas of the 64-replica set),
+    with the engine's own training kernels: each source system runs a librating trajectory
+    whose every step is a full sum that trains the network (Adam, two iterations per frame
+    on the change of the k-space force from the frame before), the weights carried from one
+    source to the next.  Returns the weights and a description of the frames."""
+    from md_sim_amd import abi, systems
+    from md_sim_amd.engine import ForceEngine
+    n = args.mts_offline_frames
+    srcs = [(systems.config("slab16k"), 11), (systems.config("slab16k"), 12)]
+    srcs += [(s, 20 + k) for k, s in enumerate(systems.replica_set(64, seed=3000)[:64:16])]
+    w, frames_total, sites_total = None, 0, 0
+    for s, seed in srcs:
+        frames = systems.libration_frames(s, n, seed=seed)
+        e = ForceEngine(s, precision=abi.FP32_ACC64, virial=False)
+        e.force_get(*frames[0])
+        e.mts_enable(interval=4, model="network", train_frames=n + 8, train_iters=2, lr=1e-3, tol=1.0)
+        if w is not None:
+            e.set_mts_weights(w)
+        for f in frames:
+            e.force_get(*f)
+        st = e.mts_stats()
+        assert st["n_surrogate"] == 0, st   # every step a full sum
+        w = e.mts_weights()
+        e.close()
+        frames_total += len(frames)
+        sites_total += len(frames) * s.nmove
+        del frames
+    return w, "%d frames (%.1f M site samples): 2 librating trajectories of slab16k and 1 of each of 4 " \
+              "replicas of the 64-replica set, %d frames each" % (frames_total, sites_total / 1e6, n)
+
+
+def mts_bench(args, workload, offline=None):
     """Surrogate k-space force with multiple time steps against the plain FP32 form, on a
     librating water trajectory (every molecule librates about its COM at H speeds of water
     at 300 K with 2 fs steps while the COM oscillates).  Each run: `mts_train` full-sum
@@ -475,24 +510,36 @@ def mts_bench(args, workload):
     ref = [e.force_get(*f) for f in frames[n_warm:]]
     e.close()
     runs = []
-    for interval, model in ((4, "network"), (2, "network"), (4, "held")):
-        cfg = dict(interval=interval, model=model, train_frames=args.mts_train, train_iters=2, lr=1e-3,
-                   tol=args.mts_tol)
+    variants = [(4, "network", None), (2, "network", None), (4, "held", None)]
+    if offline is not None:
+        variants += [(4, "network", offline[0]), (2, "network", offline[0])]
+    for interval, model, w0 in variants:
+        # with off-line weights: no training on this trajectory, the first full sums only
+        # start the correction history
+        cfg = dict(interval=interval, model=model, train_frames=args.mts_train, train_iters=2 if w0 is None else 0,
+                   lr=1e-3, tol=args.mts_tol)
         e = ForceEngine(s, precision=abi.FP32_ACC64, virial=False)
         e.force_get(*frames[0])
         e.set_launch_shapes(**shapes)
         e.mts_enable(**cfg)
+        if w0 is not None:
+            e.set_mts_weights(w0)
         sps = timed(e)
         st = e.mts_stats()
         e.close()
         e = ForceEngine(s, precision=abi.FP32_ACC64, virial=False)
         e.set_launch_shapes(**shapes)
         e.mts_enable(**cfg)
+        if w0 is not None:
+            e.set_mts_weights(w0)
         outs = [e.force_get(*f) for f in frames]
         e.close()
         errs = [float(np.sqrt(((o["f_site"] - r["f_site"]) ** 2).mean() / (r["f_site"] ** 2).mean()))
                 for o, r in zip(outs[n_warm:], ref)]
-        runs.append({"interval": interval, "model": model, "tol": args.mts_tol, "steps_per_s": sps,
+        runs.append({"interval": interval, "model": model, "tol": args.mts_tol,
+                     "training": "on line, this trajectory" if w0 is None and model == "network" else
+                                 ("off line: " + offline[1] if w0 is not None else "none"),
+                     "steps_per_s": sps,
                      "speedup_vs_plain_fp32": sps / plain_sps,
                      "force_rms_rel_err_mean": float(np.mean(errs)), "force_rms_rel_err_max": float(np.max(errs)),
                      "judged_err_mean": st["err_mean"], "judged_err_max": st["err_max"],
@@ -600,7 +647,8 @@ def main():
         out["replica_batch"] = replica_bench(args, abi.FP32_ACC64)
     mts = [w for w in args.mts_workloads.split(",") if w]
     if mts:
-        out["surrogate_mts"] = [mts_bench(args, w) for w in mts]
+        off = surrogate_offline(args) if args.mts_offline_frames > 0 else None
+        out["surrogate_mts"] = [mts_bench(args, w, off) for w in mts]
     print(json.dumps(out))
 
 
