@@ -2867,11 +2867,19 @@ struct mdf_batch {
     struct Group { int key = 0, n = 0, threads = 256; size_t lds = 0; DBuf<int2> map; };
     Group gsf[NGRP], gbp[NGRP];
     int n_gsf = 0, n_gbp = 0;
-    std::vector<const double *> h_in;
-    DBuf<const double *> d_in;
+    std::vector<const double *> h_in;   // the last step's input pointers
+    // a ring of input-pointer tables (pinned host staging, device copy) and of the step graphs
+    // that read them: a step reuses a slot only after the step NB_RING back has finished, so
+    // the host never waits for the step before it
+    static constexpr int NB_RING = 4;
+    const double **h_in_ring[NB_RING] = {};
+    DBuf<const double *> d_in_ring[NB_RING];
+    hipGraphExec_t graph_ring[NB_RING] = {};
+    hipEvent_t done_ev[NB_RING] = {};
+    bool slot_used[NB_RING] = {};
+    int slot = 0, last_slot = 0;
     DBuf<DevFlags> d_flags;
     std::vector<DevFlags> h_flags;
-    hipGraphExec_t graph = nullptr;
     hipEvent_t in_event = nullptr;
     bool pending = false;
     int pair_g = 16;
@@ -3058,12 +3066,17 @@ static int batch_upload(mdf_batch *b, bool maps) {
     return MDF_OK;
 }
 
-static void batch_launch(mdf_batch *b) {
+static void batch_drop_graphs(mdf_batch *b) {
+    for (int k = 0; k < mdf_batch::NB_RING; k++)
+        if (b->graph_ring[k]) { (void)hipGraphExecDestroy(b->graph_ring[k]); b->graph_ring[k] = nullptr; }
+}
+
+static void batch_launch(mdf_batch *b, const double *const *d_in) {
     const int n = (int)b->sys.size();
     hipStream_t st = b->stream;
     const RepArgs *R = b->d_rep.p;
     auto grid = [&](int k) { return b->nmap[k]; };
-    if (grid(BK_GATHER)) k_gather_b<<<grid(BK_GATHER), 256, 0, st>>>(R, b->maps[BK_GATHER].p, b->d_in.p);
+    if (grid(BK_GATHER)) k_gather_b<<<grid(BK_GATHER), 256, 0, st>>>(R, b->maps[BK_GATHER].p, d_in);
     k_skin_b<<<grid(BK_SKIN), 256, 0, st>>>(R, b->maps[BK_SKIN].p);
     k_decide_b<<<n, 256, 0, st>>>(R);
     k_cell_count_b<<<grid(BK_COUNT), 256, 0, st>>>(R, b->maps[BK_COUNT].p);
@@ -3160,10 +3173,16 @@ static void batch_launch(mdf_batch *b) {
 }
 
 static void batch_free(mdf_batch *b) {
-    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    batch_drop_graphs(b);
     b->d_rep.release();
     for (int k = 0; k < BK_N; k++) b->maps[k].release();
-    b->d_in.release();
+    for (int k = 0; k < mdf_batch::NB_RING; k++) {
+        b->d_in_ring[k].release();
+        if (b->h_in_ring[k]) (void)hipHostFree(b->h_in_ring[k]);
+        b->h_in_ring[k] = nullptr;
+        if (b->done_ev[k]) (void)hipEventDestroy(b->done_ev[k]);
+        b->done_ev[k] = nullptr;
+    }
     b->d_flags.release();
     if (b->in_event) (void)hipEventDestroy(b->in_event);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -3218,7 +3237,11 @@ int mdf_batch_create(mdf_system *const *systems, int32_t n, mdf_batch **out) {
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->in_event, hipEventDisableTiming);
     for (int i = 0; i < n && e == hipSuccess; i++) e = hipStreamSynchronize(systems[i]->stream);
     if (e == hipSuccess) e = b->d_rep.alloc(n);
-    if (e == hipSuccess) e = b->d_in.alloc(6 * (size_t)n);
+    for (int k = 0; k < mdf_batch::NB_RING && e == hipSuccess; k++) {
+        e = b->d_in_ring[k].alloc(6 * (size_t)n);
+        if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_in_ring[k], sizeof(double *) * 6 * (size_t)n, 0);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&b->done_ev[k], hipEventDisableTiming);
+    }
     if (e == hipSuccess) e = b->d_flags.alloc(n);
     if (e != hipSuccess) rc = fail(MDF_ERR_HIP, "batch set-up: %s", hipGetErrorString(e));
     if (rc == MDF_OK) rc = batch_upload(b, true);
@@ -3239,7 +3262,9 @@ int mdf_batch_compute_device(mdf_batch *b, const double *const *x, const double 
     if (!b || !x || !y || !z) return fail(MDF_ERR_ARG, "null argument");
     HIPCHK(hipSetDevice(b->device));
     const int n = (int)b->sys.size();
-    if (b->pending) HIPCHK(hipStreamSynchronize(b->stream));   // the pointer table is about to change
+    // this step's slot: free once the step that used it NB_RING steps back has finished
+    const int sl = b->slot;
+    if (b->slot_used[sl]) HIPCHK(hipEventSynchronize(b->done_ev[sl]));
     for (int i = 0; i < n; i++) {
         if (!x[i] || !y[i] || !z[i]) return fail(MDF_ERR_ARG, "null positions for system %d", i);
         if (b->sys[i]->top.n_rig > 0 && (!cx || !cy || !cz || !cx[i] || !cy[i] || !cz[i]))
@@ -3251,21 +3276,27 @@ int mdf_batch_compute_device(mdf_batch *b, const double *const *x, const double 
         const double *p[6] = {x[i], y[i], z[i], cx ? cx[i] : nullptr, cy ? cy[i] : nullptr, cz ? cz[i] : nullptr};
         for (int c = 0; c < 6; c++) b->h_in[6 * i + c] = p[c];
     }
-    HIPCHK(hipMemcpyAsync(b->d_in.p, b->h_in.data(), sizeof(double *) * 6 * n, hipMemcpyHostToDevice, b->stream));
+    memcpy(b->h_in_ring[sl], b->h_in.data(), sizeof(double *) * 6 * n);
+    HIPCHK(hipMemcpyAsync(b->d_in_ring[sl].p, b->h_in_ring[sl], sizeof(double *) * 6 * n, hipMemcpyHostToDevice,
+                          b->stream));
     HIPCHK(hipEventRecord(b->in_event, (hipStream_t)stream));
     HIPCHK(hipStreamWaitEvent(b->stream, b->in_event, 0));
-    if (!b->graph) {
+    if (!b->graph_ring[sl]) {
         hipGraph_t g = nullptr;
         HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
-        batch_launch(b);
+        batch_launch(b, b->d_in_ring[sl].p);
         hipError_t ec = hipStreamEndCapture(b->stream, &g);
         if (ec != hipSuccess) return fail(MDF_ERR_HIP, "batch graph capture: %s", hipGetErrorString(ec));
-        ec = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+        ec = hipGraphInstantiate(&b->graph_ring[sl], g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         if (ec != hipSuccess) return fail(MDF_ERR_HIP, "batch graph instantiate: %s", hipGetErrorString(ec));
     }
-    HIPCHK(hipGraphLaunch(b->graph, b->stream));
+    HIPCHK(hipGraphLaunch(b->graph_ring[sl], b->stream));
+    HIPCHK(hipEventRecord(b->done_ev[sl], b->stream));
     HIPCHK(hipGetLastError());
+    b->slot_used[sl] = true;
+    b->last_slot = sl;
+    b->slot = (sl + 1) % mdf_batch::NB_RING;
     b->pending = true;
     return MDF_OK;
 }
@@ -3343,7 +3374,7 @@ int mdf_batch_autotune(mdf_batch *b, int32_t reps, int32_t *chosen) {
     wt = std::max(wt, 1.0);
     // new block maps and per-system tables; the captured step holds the old maps
     auto reshape = [&](int sf_total, int bp_total) -> int {
-        if (b->graph) { (void)hipGraphExecDestroy(b->graph); b->graph = nullptr; }
+        batch_drop_graphs(b);
         for (int i = 0; i < n; i++) {
             mdf_system *s = b->sys[i];
             s->sf_target_blocks = std::max(1, (int)llround(sf_total * w[i] / wt));
@@ -3359,9 +3390,9 @@ int mdf_batch_autotune(mdf_batch *b, int32_t reps, int32_t *chosen) {
     int rc = MDF_OK;
     // mean time of one whole batch step (direct launches), after one untimed step
     auto timed = [&](float *ms) -> int {
-        batch_launch(b);
+        batch_launch(b, b->d_in_ring[b->last_slot].p);
         HIPCHK(hipEventRecord(e0, b->stream));
-        for (int r = 0; r < reps; r++) batch_launch(b);
+        for (int r = 0; r < reps; r++) batch_launch(b, b->d_in_ring[b->last_slot].p);
         HIPCHK(hipEventRecord(e1, b->stream));
         HIPCHK(hipEventSynchronize(e1));
         HIPCHK(hipGetLastError());
