@@ -338,6 +338,7 @@ struct mdf_system {
     bool use_graphs = true;
     hipEvent_t in_event = nullptr;   // orders a device-input step after the inputs' producer
     bool pair_strided = false;  // pair loop form: row-strided (true) or cut-off compacted (false)
+    int pair_cell_order = 1;    // pair kernel groups take leads in the list build's cell order
     int pair_g = 16;            // lanes per lead in the pair kernel (8, 16 or 32)
     // launch-shape targets of the reciprocal kernels (blocks per launch; the row groups follow)
     int sf_target_blocks = 4096, bp_target_blocks = 1024;
@@ -1468,6 +1469,8 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
         if (g && g[0] == '0') s->use_graphs = false;
         const char *pr = getenv("MDF_PAIR_STRIDED");
         if (pr && pr[0] == '1') s->pair_strided = true;
+        const char *pco = getenv("MDF_PAIR_CELL_ORDER");
+        if (pco && pco[0] == '0') s->pair_cell_order = 0;
         const char *sm = getenv("MDF_SF_MMA");
         if (sm && sm[0] == '0') s->sf_mma_allowed = false;
         const char *bm = getenv("MDF_BP_MMA");
@@ -1650,6 +1653,7 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
         S.itype = s->itype.p; S.q = s->q.p; S.nsi = s->nsi.p; S.ijtype = s->ijtype.p; S.vkind = s->vkind.p;
         S.eps4 = s->eps4.p; S.sig6 = s->sig6.p; S.aa = s->aa.p; S.rhoi = s->rhoi.p; S.cc = s->cc.p; S.cls = s->cls.p;
         S.pmeta = s->pmeta.p; S.wcls = s->wcls.p; S.ptab = s->ptab.p; S.prec = s->prec.p;
+        S.pair_cell_order = s->pair_cell_order;
         S.lead = s->lead.p; S.con_ptr = s->con_ptr.p; S.con_idx = s->con_idx.p;
         // positions (all sites once; moving part per call)
         HTRY(upload(s->x, x, t.n_total)); HTRY(upload(s->y, y, t.n_total)); HTRY(upload(s->z, z, t.n_total));

@@ -149,6 +149,7 @@ struct DevSys {
     // cells
     Grid g;
     int *cell_cnt, *cell_start, *cell_items;   // moving leads
+    int pair_cell_order;                       // the pair kernel takes leads in cell_items order
     double4 *cell_xyz;                         // the cell-ordered leads' x, y, z and site index
     int *cell_cursor;
     int n_cells_cap;

@@ -281,10 +281,13 @@ __device__ __forceinline__ void pair_body(const DevSys &S, int bid, int nblk) {
     __syncthreads();
     const int tid = threadIdx.x;
     const int grp = tid / G;
-    const int gid = bid * (blockDim.x / G) + grp;
+    const int gslot = bid * (blockDim.x / G) + grp;
     const int sub = tid % G;
     const int gbase = (tid & 63) - sub;   // the group's first lane in the wave
-    const bool active = gid < S.n_lead;
+    const bool active = gslot < S.n_lead;
+    // the lead of this group: in lead order, or in the cell order of the last list build
+    // (neighbouring groups then share most partners in the caches)
+    const int gid = (active && S.pair_cell_order) ? S.cell_items[gslot] : gslot;
     int *queue = s_q[grp];
     constexpr bool F32 = sizeof(R) == 4;
 
