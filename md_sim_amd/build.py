@@ -44,11 +44,15 @@ def build(verbose: bool = False, jobs: int = 5) -> str:
             print(out)
     if failed:
         raise RuntimeError("\n".join("%s:\n%s" % f for f in failed))
-    # Link against a symbol-less stub whose soname is the generic "libamdhip64.so": the
-    # library then needs the HIP runtime under that name, found through the rpath in
-    # /opt/rocm/lib when the process has none, and shared with the runtime a host process
-    # (PyTorch-ROCm bundles one under the same name) has already loaded - one HIP runtime
-    # per process instead of two.
+    return link(objs, LIB, objdir)
+
+
+def link(objs, out: str, objdir: str) -> str:
+    """Link HIP objects into a shared library against a symbol-less stub whose soname is the
+    generic "libamdhip64.so": the library then needs the HIP runtime under that name, found
+    through the rpath in /opt/rocm/lib when the process has none, and shared with the runtime
+    a host process (PyTorch-ROCm bundles one under the same name) has already loaded - one
+    HIP runtime per process instead of two."""
     stubdir = os.path.join(objdir, "stub")
     os.makedirs(stubdir, exist_ok=True)
     stub_src = os.path.join(stubdir, "stub.c")
@@ -57,11 +61,11 @@ def build(verbose: bool = False, jobs: int = 5) -> str:
     subprocess.check_call(["gcc", "-shared", "-fPIC", "-Wl,-soname,libamdhip64.so", "-o",
                            os.path.join(stubdir, "libamdhip64.so"), stub_src])
     rocm_lib = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")
-    tmp = LIB + ".tmp"
+    tmp = out + ".tmp"
     subprocess.check_call(["g++", "-shared", "-fPIC", "-o", tmp, *objs, "-L" + stubdir, "-Wl,--no-as-needed", "-lamdhip64", "-Wl,--as-needed",
                            "-Wl,--disable-new-dtags", "-Wl,-rpath," + rocm_lib])
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

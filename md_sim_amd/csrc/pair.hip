@@ -14,71 +14,9 @@
 // FP64 accumulation); the lead-pair minimum image and the cut-off test are FP64 without
 // contraction in both, so the set of interacting pairs is the reference's.
 #include "mdf_internal.h"
+#include "mdf_math.h"
 
 namespace mdf {
-
-// 1/sqrt(x) and 1/x from the hardware estimates (v_rsq_f64, v_rcp_f64) refined by
-// Newton-Raphson iterations (each doubles the correct bits: ~2^-22 -> 2^-44 -> 2^-88), which
-// is within an ulp of the IEEE result at a fraction of the cost of the IEEE sequences.
-__device__ __forceinline__ double rsqrt_nr(double x) {
-    double y = __builtin_amdgcn_rsq(x);
-    double h = 0.5 * x;
-#pragma unroll
-    for (int it = 0; it < 2; it++) {
-        double r = __builtin_fma(-h * y, y, 0.5);   // 0.5 - x y^2 / 2
-        y = __builtin_fma(y, r, y);                 // y (1.5 - x y^2 / 2)
-    }
-    return y;
-}
-__device__ __forceinline__ double rcp_nr(double x) {
-    double y = __builtin_amdgcn_rcp(x);
-#pragma unroll
-    for (int it = 0; it < 2; it++) {
-        double e = __builtin_fma(-x, y, 1.0);
-        y = __builtin_fma(y, e, y);
-    }
-    return y;
-}
-__device__ __forceinline__ float rsqrt_nr(float x) {
-    float y = __builtin_amdgcn_rsqf(x);
-    float r = __builtin_fmaf(-0.5f * x * y, y, 0.5f);
-    return __builtin_fmaf(y, r, y);
-}
-__device__ __forceinline__ float rcp_nr(float x) {
-    float y = __builtin_amdgcn_rcpf(x);
-    float e = __builtin_fmaf(-x, y, 1.0f);
-    return __builtin_fmaf(y, e, y);
-}
-
-// e^x for x <= 0 (the erfc and Buckingham arguments here), FP64, by Cody and Waite's
-// argument reduction (Software Manual for the Elementary Functions, 1980): x = n ln2 + r with
-// n = rint(x / ln2), |r| <= ln2 / 2, ln2 split into a 21-bit head (n ln2_hi exact) and a tail;
-// e^r by its Taylor series to degree 13, whose truncation error at |r| = ln2 / 2 is 4.2e-18
-// relative; e^x = 2^n e^r by exponent scaling (underflowing to 0 below x = -745).  About 19
-// operations and within 2 ulp, without the range checks a general exp needs.
-__device__ __forceinline__ double exp_neg(double x) {
-    x = x < -800.0 ? -800.0 : x;   // e^-800 is 0 in FP64; NaN passes through
-    const double n = rint(x * 1.4426950408889634);
-    double r = __builtin_fma(-n, 0x1.62e42p-1, x);
-    r = __builtin_fma(-n, 0x1.fdf473de6af28p-22, r);
-    double p = 1.0 / 6227020800.0;            // 1/13!
-    p = __builtin_fma(p, r, 1.0 / 479001600.0);
-    p = __builtin_fma(p, r, 1.0 / 39916800.0);
-    p = __builtin_fma(p, r, 1.0 / 3628800.0);
-    p = __builtin_fma(p, r, 1.0 / 362880.0);
-    p = __builtin_fma(p, r, 1.0 / 40320.0);
-    p = __builtin_fma(p, r, 1.0 / 5040.0);
-    p = __builtin_fma(p, r, 1.0 / 720.0);
-    p = __builtin_fma(p, r, 1.0 / 120.0);
-    p = __builtin_fma(p, r, 1.0 / 24.0);
-    p = __builtin_fma(p, r, 1.0 / 6.0);
-    p = __builtin_fma(p, r, 0.5);
-    p = __builtin_fma(p, r, 1.0);
-    p = __builtin_fma(p, r, 1.0);
-    return ldexp(p, (int)n);
-}
-__device__ __forceinline__ double exp_r(double x) { return exp_neg(x); }
-__device__ __forceinline__ float exp_r(float x) { return expf(x); }
 
 // FP32 form: the hardware estimates v_rsq_f32, v_rcp_f32 and v_exp_f32 (2^x), each within
 // about an ulp, with no refinement: the FP32 form's error is set by its FP32 arithmetic
@@ -90,6 +28,7 @@ __device__ __forceinline__ float exp_hw(float x) { return __builtin_amdgcn_exp2f
 template <typename R, bool ACC>
 struct Coul {
     R alfa, spii, p, a1, a2, a3, a4, a5;
+    const double *e2t;   // 2^(j/64) in LDS (FP64 e^x)
     // energy (erfc(ar) q/r) and force/r  for a site pair at squared distance r2,
     // with qq = facq q_i q_j
     __device__ __forceinline__ void eval(R r2, R qq, R &e, R &f) const {
@@ -100,7 +39,7 @@ struct Coul {
         const R aux = qq * ri;
         const R ar = alfa * r;
         R ex;
-        if constexpr (sizeof(R) == 8) ex = exp_neg(-ar * ar); else ex = exp_hw(-ar * ar);
+        if constexpr (sizeof(R) == 8) ex = exp_neg(-ar * ar, e2t); else ex = exp_hw(-ar * ar);
         R erfx;
         if constexpr (!ACC) {
             R t;
@@ -115,7 +54,8 @@ struct Coul {
 };
 
 template <typename R>
-__device__ __forceinline__ void vdw_eval(int kind, R rsq, R eps4, R sig6, R A, R rhoi, R Cc, R &e, R &f) {
+__device__ __forceinline__ void vdw_eval(int kind, R rsq, R eps4, R sig6, R A, R rhoi, R Cc, const double *e2t, R &e,
+                                         R &f) {
     R rsqi;
     if constexpr (sizeof(R) == 8) rsqi = rcp_nr(rsq); else rsqi = rcp_hw(rsq);
     if (kind == 1) {  // Lennard-Jones, eps4 = 4 eps, sig6 = sigma^6 (src/forcem.f90:315-319)
@@ -128,7 +68,7 @@ __device__ __forceinline__ void vdw_eval(int kind, R rsq, R eps4, R sig6, R A, R
         R rij = rsq * rinv;
         R r6 = Cc * (rsqi * rsqi * rsqi);
         R aux;
-        if constexpr (sizeof(R) == 8) aux = A * exp_neg(-rij * rhoi); else aux = A * exp_hw(-rij * rhoi);
+        if constexpr (sizeof(R) == 8) aux = A * exp_neg(-rij * rhoi, e2t); else aux = A * exp_hw(-rij * rhoi);
         e = aux - r6;
         f = aux * rhoi * rinv - R(6) * rsqi * r6;
     } else {
@@ -192,19 +132,21 @@ __device__ __forceinline__ void wave_sync() {
 // the tests) and append those inside the cut-off to the group's queue in LDS (ballot prefix
 // count, so the queue keeps the row's order); then the lanes take the queue G entries at a
 // time, the next entry's partner record in flight while one is evaluated (LOAD(j) -> record,
-// EVAL(record)).  STRIDED: lane sub takes row entries sub, sub + G, ... directly and EVAL
-// itself skips a partner outside the cut-off.
+// EVAL(record); two records alternate, LOAD(-1) gives a record EVAL skips).  STRIDED: lane
+// sub takes row entries sub, sub + G, ... directly and EVAL itself skips a partner outside
+// the cut-off.
 template <int G, bool STRIDED, typename POS, typename LOAD, typename EVAL>
 __device__ __forceinline__ void row_pass(const DevSys &S, const int *row, int n, int *queue, int sub, int gbase,
                                          double xi, double yi, double zi, POS pos, LOAD load, EVAL eval) {
     if constexpr (STRIDED) {
         int k = sub;
-        auto rn = load(k < n ? row[k] : -1);
+        auto ra = load(k < n ? row[k] : -1);
 #pragma unroll 1
-        for (; k < n; k += G) {
-            const auto rc = rn;
-            if (k + G < n) rn = load(row[k + G]);
-            eval(rc);
+        for (; k < n; k += 2 * G) {
+            const auto rb = load(k + G < n ? row[k + G] : -1);
+            eval(ra);
+            ra = load(k + 2 * G < n ? row[k + 2 * G] : -1);
+            eval(rb);
         }
     } else {
 #pragma unroll 1
@@ -232,12 +174,13 @@ __device__ __forceinline__ void row_pass(const DevSys &S, const int *row, int n,
             }
             wave_sync();
             int q = sub;
-            auto rn = load(q < qn ? queue[q] : -1);
+            auto ra = load(q < qn ? queue[q] : -1);
 #pragma unroll 1
-            for (; q < qn; q += G) {
-                const auto rc = rn;
-                if (q + G < qn) rn = load(queue[q + G]);
-                eval(rc);
+            for (; q < qn; q += 2 * G) {
+                const auto rb = load(q + G < qn ? queue[q + G] : -1);
+                eval(ra);
+                ra = load(q + 2 * G < qn ? queue[q + 2 * G] : -1);
+                eval(rb);
             }
             wave_sync();   // the queue is rewritten by the next segment
         }
@@ -277,7 +220,9 @@ template <typename R, int G, bool VIR, bool ACC, bool STRIDED, int NMC>
 __device__ __forceinline__ void pair_body(const DevSys &S, int bid, int nblk) {
     __shared__ double s_tab[MAX_LEAD_TYPES * NCLS * PTAB_W];
     __shared__ int s_q[256 / G][PAIR_SEG];
+    __shared__ double s_e2[64];
     for (int k = threadIdx.x; k < MAX_LEAD_TYPES * NCLS * PTAB_W; k += blockDim.x) s_tab[k] = S.ptab[k];
+    if constexpr (sizeof(R) == 8) exp2_table_load(s_e2);
     __syncthreads();
     const int tid = threadIdx.x;
     const int grp = tid / G;
@@ -306,6 +251,7 @@ __device__ __forceinline__ void pair_body(const DevSys &S, int bid, int nblk) {
     Coul<R, ACC> cl;
     cl.alfa = (R)S.alfa; cl.spii = (R)S.spii; cl.p = (R)S.p;
     cl.a1 = (R)S.a1; cl.a2 = (R)S.a2; cl.a3 = (R)S.a3; cl.a4 = (R)S.a4; cl.a5 = (R)S.a5;
+    cl.e2t = s_e2;
 
     int i = 0, nsi_i = 0;
     bool wall_any = false;   // frozen partners or surface-site bookkeeping for this lead
@@ -346,7 +292,7 @@ __device__ __forceinline__ void pair_body(const DevSys &S, int bid, int nblk) {
             if (STRIDED && rsq > S.rcutsq) return;
             const double *pt = tab_i + pj.cls * PTAB_W;
             R evd, fvd;
-            vdw_eval<R>((int)pt[0], (R)rsq, (R)pt[1], (R)pt[2], (R)pt[3], (R)pt[4], (R)pt[5], evd, fvd);
+            vdw_eval<R>((int)pt[0], (R)rsq, (R)pt[1], (R)pt[2], (R)pt[3], (R)pt[4], (R)pt[5], s_e2, evd, fvd);
             const R dxr = (R)dx, dyr = (R)dy, dzr = (R)dz;
             R e_c = R(0);
             R fa[3][3];      // this molecule pair's force on the lead's sites
@@ -401,7 +347,9 @@ __device__ __forceinline__ void pair_body(const DevSys &S, int bid, int nblk) {
                              xi, yi, zi, mv_pos, mv_load, mv_eval);
 
         // surface-site bookkeeping between the two loops (src/forcem.f90:408-413)
-        if (i >= S.n_liquid_sites) w[(signbit(zi) ? 0 : 3) + 2] += f[0][2];
+        if (i >= S.n_liquid_sites) {
+            if (signbit(zi)) w[2] += f[0][2]; else w[5] += f[0][2];
+        }
         const int nw = S.n_wall > 0 ? min(S.cnt_mw[gid], S.cap_mw) : 0;
         wall_any = nw > 0 || i >= S.n_liquid_sites;
 
@@ -429,7 +377,7 @@ __device__ __forceinline__ void pair_body(const DevSys &S, int bid, int nblk) {
             const int cw = pr.cw;   // classes MAX_LEAD_TYPES + cw
             const double *pt = tab_i + (MAX_LEAD_TYPES + cw) * PTAB_W;
             R evd, fvd;
-            vdw_eval<R>((int)pt[0], (R)rsq, (R)pt[1], (R)pt[2], (R)pt[3], (R)pt[4], (R)pt[5], evd, fvd);
+            vdw_eval<R>((int)pt[0], (R)rsq, (R)pt[1], (R)pt[2], (R)pt[3], (R)pt[4], (R)pt[5], s_e2, evd, fvd);
             const R qw = (R)pw.w;
             const R dxr = (R)dx, dyr = (R)dy, dzr = (R)dz;
             R e_c = R(0);
@@ -460,9 +408,10 @@ __device__ __forceinline__ void pair_body(const DevSys &S, int bid, int nblk) {
 #pragma unroll
                 for (int c = 0; c < 6; c++) vir[c] += (double)vr[c];
             }
-            if (wall_bk) {
-                if (sgo == 0) { w[0] += (double)wx; w[1] += (double)wy; w[2] += (double)wz; }
-                else { w[3] += (double)wx; w[4] += (double)wy; w[5] += (double)wz; }
+            if (wall_bk) {   // selects, not an index by side: w stays in registers
+                const bool lo = sgo == 0;
+                w[0] += lo ? (double)wx : 0.0; w[1] += lo ? (double)wy : 0.0; w[2] += lo ? (double)wz : 0.0;
+                w[3] += lo ? 0.0 : (double)wx; w[4] += lo ? 0.0 : (double)wy; w[5] += lo ? 0.0 : (double)wz;
             }
             acc_at<2>(evw, cw, (double)evd);
             acc_at<2>(ecw, cw, (double)e_c);

@@ -16,6 +16,8 @@ that every dispatch of the list-build kernels is a full build.
 
 ``--chempot "N WMIN WMAX BINS MID"`` adds the Widom insertions after every step (tot.inp's
 ichmax, lwmin, lwmax, lchmax, ilmid).
+
+``--pair-lanes G [--pair-strided]`` fixes the pair kernel's launch shape (else the default).
 """
 from __future__ import annotations
 
@@ -35,6 +37,8 @@ def main(argv=None):
     ap.add_argument("--mts-train", type=int, default=8)
     ap.add_argument("--rebuild-every-step", action="store_true")
     ap.add_argument("--chempot", default="", help="Widom insertions: 'ichmax lwmin lwmax lchmax ilmid'")
+    ap.add_argument("--pair-lanes", type=int, default=0, help="lanes per lead of the pair kernel (8, 16, 32)")
+    ap.add_argument("--pair-strided", action="store_true", help="strided row pass instead of the compacted one")
     a = ap.parse_args(argv)
     os.environ["MDF_GRAPHS"] = "0"
     import torch
@@ -46,6 +50,8 @@ def main(argv=None):
     prec = abi.FP32_ACC64 if a.precision == "fp32" else abi.FP64
     pos = abi.positions(s)
     eng = ForceEngine(s, precision=prec, virial=False)
+    if a.pair_lanes:
+        eng.set_launch_shapes(pair_lanes=a.pair_lanes, pair_strided=a.pair_strided, sf_blocks=8192, bp_blocks=2048)
     if a.rebuild_every_step:
         eng.set_stage_timing(True, rebuild_every_step=True)
     if a.mts > 1:
