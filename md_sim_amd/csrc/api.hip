@@ -82,7 +82,7 @@ __global__ void k_sf_reduce_rows(const double2 *, const float2 *, int, int, cons
                                  const MtsState *, unsigned *, int *, int, int);
 template <int NT>
 __global__ void k_sf64(const double *, const double *, const double *, const double *, int, double, double, double,
-                       const int2 *, const int *, const int2 *, int, int, int, const int *, int, double2 *, int,
+                       const int2 *, const int4 *, const int *, int, int, int, const int *, int, double2 *, int,
                        const MtsState *);
 __global__ void k_sf64_reduce(const double2 *, int, int, const int *, const int *, const KRow *, const double2 *,
                               const double *, double, double, double, double, double2 *, double4 *, double *,
@@ -309,10 +309,11 @@ struct mdf_system {
     // the FP64 form's reciprocal sum on the matrix cores (k_sf64, k_sf64_reduce, k_bp64)
     bool mma64 = false, mma64_allowed = true;
     int klmax = 0, kmmax = 0;
-    int s64_nt = 1, s64_ngrp = 1, s64_nsb = 1, s64_sites = SF64_SUB, s64_pcols = 1, s64_sites_env = 0;
+    int s64_nt = 1, s64_npass = 1, s64_nsb = 1, s64_sites = SF64_SUB, s64_pcols = 1, s64_sites_env = 0;
     int b64_nks = 4, b64_ng = 2, b64_nrg = 1, b64_nmt = 1, b64_sites = 128, b64_rg_env = 0, b64_ng_env = 0;
-    DBuf<int2> d_s64_rowkk, d_s64_groups, d_b64_tile, d_b64_rg;
-    DBuf<int> d_s64_tilent, d_s64_pcoff, d_s64_sorted_of;
+    DBuf<int2> d_s64_rowkk, d_b64_tile, d_b64_rg;
+    DBuf<int4> d_s64_waves;
+    DBuf<int> d_s64_pncol, d_s64_pcoff, d_s64_sorted_of;
     DBuf<int4> d_b64_rinfo;
     DBuf<double2> d_s64_part;
     DBuf<double> d_b64_tpk;
@@ -446,8 +447,8 @@ static void free_all(mdf_system *s) {
                           &s->colpart, &s->fcom, &s->torque, &s->e_vdw, &s->e_coul, &s->ang_kt, &s->ang_th};
     for (auto *b : db) b->release();
     s->d_rowkk.release(); s->d_tile_ks.release(); s->d_krow.release(); s->d_tpack.release(); s->d_tscale.release();
-    s->d_s64_rowkk.release(); s->d_s64_groups.release(); s->d_b64_tile.release(); s->d_b64_rg.release();
-    s->d_s64_tilent.release(); s->d_s64_pcoff.release(); s->d_s64_sorted_of.release(); s->d_b64_rinfo.release();
+    s->d_s64_rowkk.release(); s->d_s64_waves.release(); s->d_b64_tile.release(); s->d_b64_rg.release();
+    s->d_s64_pncol.release(); s->d_s64_pcoff.release(); s->d_s64_sorted_of.release(); s->d_b64_rinfo.release();
     s->d_s64_part.release(); s->d_b64_tpk.release();
     s->pmeta.release(); s->wcls.release(); s->prec.release(); s->ptab.release(); s->wrec.release();
     s->flags.release(); s->d_bp_row_start.release(); s->d_chunks.release(); s->d_rows.release(); s->d_swall.release(); s->d_s.release();
@@ -656,19 +657,9 @@ static void drop_graphs(mdf_system *s);
 
 // Tables and launch shapes of the FP64 matrix-core reciprocal kernels.
 // Structure factor: rows sorted by kn extent E (widest first, stable), 16-row tiles with their
-// column tiles ceil((2 E + 1) / 16), groups of up to 4 consecutive tiles per block, partial
-// columns per row 2 E + 1 in sorted order.  Back-projection: M-tiles of 4 rows in the
+// column tiles ceil((2 E + 1) / 16), slots of (row tile, column tile) dealt to the waves of
+// each pass, partial columns per row 2 E + 1 in sorted order.  Back-projection: M-tiles of 4 rows in the
 // reference order, K-steps ceil((2 E_max + 1) / 4) each, row groups of about equal K-steps.
-template <int NT>
-static const void *sf64_kernel_nt() { return reinterpret_cast<const void *>(&k_sf64<NT>); }
-static const void *sf64_kernel(int nt) {
-    switch (nt) {
-        case 1: return sf64_kernel_nt<1>(); case 2: return sf64_kernel_nt<2>(); case 3: return sf64_kernel_nt<3>();
-        case 4: return sf64_kernel_nt<4>(); case 5: return sf64_kernel_nt<5>(); case 6: return sf64_kernel_nt<6>();
-        case 7: return sf64_kernel_nt<7>(); default: return sf64_kernel_nt<8>();
-    }
-}
-
 static int setup_mma64(mdf_system *s) {
     const int nr = (int)s->rows.size(), nm = s->top.n_move;
     std::vector<int> E(nr);
@@ -687,26 +678,43 @@ static int setup_mma64(mdf_system *s) {
         pcoff[i + 1] = pcoff[i] + 2 * E[sorted[i]] + 1;
     }
     for (int t = 0; t < n_rt; t++) tile_nt[t] = (2 * E[sorted[16 * t]] + 1 + 15) / 16;
-    std::vector<int2> groups;
-    for (int t = 0; t < n_rt; t += 4) groups.push_back(make_int2(t, std::min(n_rt, t + 4)));
-    s->s64_nt = tile_nt[0];
-    s->s64_ngrp = (int)groups.size();
-    s->s64_pcols = pcoff[nr];
-    // sites per block: about one resident round of blocks over the compute units
-    {
-        int occ = 0;
-        const size_t lds = sf64_lds_bytes(s->klmax, s->kmmax, s->s64_nt);
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, sf64_kernel(s->s64_nt), 256, lds) != hipSuccess ||
-            occ < 1) {
-            (void)hipGetLastError();
-            occ = 1;
+    // work slots (row tile, column tile) in row-tile order, dealt to passes of SF64_WAVES waves
+    // of up to SF64_SLOTS consecutive slots; within a pass wave w gets q or q + 1 slots, the
+    // larger counts first, so that the SIMDs (wave w on SIMD w % 4) get about equal numbers
+    std::vector<int2> slots;
+    for (int t = 0; t < n_rt; t++)
+        for (int c = 0; c < tile_nt[t]; c++) slots.push_back(make_int2(t, c));
+    const int n_sl = (int)slots.size(), per_pass = SF64_WAVES * SF64_SLOTS;
+    const int npass = std::max(1, (n_sl + per_pass - 1) / per_pass);
+    std::vector<int4> waves((size_t)npass * SF64_WAVES, make_int4(0, 0, 0, 0));
+    std::vector<int> pass_ncol(npass, 16);
+    for (int p = 0; p < npass; p++) {
+        const int b0 = (int)((long)p * n_sl / npass), b1 = (int)((long)(p + 1) * n_sl / npass);
+        const int q = (b1 - b0) / SF64_WAVES, rem = (b1 - b0) % SF64_WAVES;
+        int k = b0;
+        for (int w = 0; w < SF64_WAVES; w++) {
+            const int nw = q + (w < rem ? 1 : 0);   // at most SF64_SLOTS
+            int v[SF64_SLOTS] = {0, 0, 0};
+            int code = nw;
+            for (int u = 0; u < nw; u++, k++) {
+                v[u] = (slots[k].x << 8) | slots[k].y;
+                if (u > 0 && slots[k].x == slots[k - 1].x) code |= u == 1 ? 16 : 32;
+                pass_ncol[p] = std::max(pass_ncol[p], 16 * (slots[k].y + 1));
+            }
+            waves[(size_t)p * SF64_WAVES + w] = make_int4(v[0], v[1], v[2], code);
         }
+    }
+    s->s64_nt = tile_nt[0];
+    s->s64_npass = npass;
+    s->s64_pcols = pcoff[nr];
+    // sites per block: one block per compute unit (a block fills a unit's 16 wave slots at 128
+    // registers), at least four stages a block so that partial rows stay few
+    {
         int sites = s->s64_sites_env;
         if (sites <= 0) {
-            const long want = (long)s->n_cu * occ;
-            const long nsb = std::max(1L, (want + s->s64_ngrp - 1) / s->s64_ngrp);
+            const long nsb = std::max(1L, ((long)s->n_cu + npass - 1) / npass);
             sites = (int)((nm + nsb - 1) / nsb);
-            sites = std::max(SF64_SUB, SF64_SUB * ((sites + SF64_SUB - 1) / SF64_SUB));
+            sites = std::max(4 * SF64_SUB, SF64_SUB * ((sites + SF64_SUB - 1) / SF64_SUB));
         }
         s->s64_sites = sites;
         s->s64_nsb = std::max(1, (nm + sites - 1) / sites);
@@ -750,8 +758,8 @@ static int setup_mma64(mdf_system *s) {
     }
     s->b64_nrg = (int)rg.size();
     HIPCHK(upload(s->d_s64_rowkk, rowkk.data(), rowkk.size()));
-    HIPCHK(upload(s->d_s64_tilent, tile_nt.data(), tile_nt.size()));
-    HIPCHK(upload(s->d_s64_groups, groups.data(), groups.size()));
+    HIPCHK(upload(s->d_s64_waves, waves.data(), waves.size()));
+    HIPCHK(upload(s->d_s64_pncol, pass_ncol.data(), pass_ncol.size()));
     HIPCHK(upload(s->d_s64_pcoff, pcoff.data(), pcoff.size()));
     HIPCHK(upload(s->d_s64_sorted_of, sorted_of.data(), sorted_of.size()));
     HIPCHK(upload(s->d_b64_tile, bptile.data(), bptile.size()));
@@ -924,12 +932,12 @@ static void launch_sf(mdf_system *s) {
     const RecipArgs a = recip_args(s);
     DevSys &S = s->S;
     if (s->mma64) {
-        const dim3 grid(s->s64_nsb, s->s64_ngrp);
+        const dim3 grid(s->s64_nsb, s->s64_npass);
         const size_t lds = sf64_lds_bytes(s->klmax, s->kmmax, s->s64_nt);
 #define SF64_LAUNCH(NT)                                                                                         \
     case NT:                                                                                                    \
-        MDF_TLAUNCH(s, KT_SF, k_sf64<NT>, grid, 256, lds, s->stream, S.x, S.y, S.z, S.q, nm, a.cux, a.cuy, a.cuz, \
-                    s->d_s64_rowkk.p, s->d_s64_tilent.p, s->d_s64_groups.p, nrows, s->klmax, s->kmmax,          \
+        MDF_TLAUNCH(s, KT_SF, k_sf64<NT>, grid, 64 * SF64_WAVES, lds, s->stream, S.x, S.y, S.z, S.q, nm, a.cux,  \
+                    a.cuy, a.cuz, s->d_s64_rowkk.p, s->d_s64_waves.p, s->d_s64_pncol.p, nrows, s->klmax, s->kmmax, \
                     s->d_s64_pcoff.p, s->s64_pcols, s->d_s64_part.p, s->s64_sites, mts_state(s));                \
         break;
         switch (s->s64_nt) {
@@ -2955,8 +2963,8 @@ static void fill_rep(mdf_system *s, RepArgs &r) {
     r.klmax = s->klmax; r.kmmax = s->kmmax;
     r.s64_nsb = s->s64_nsb; r.s64_pcols = s->s64_pcols; r.s64_sites = s->s64_sites;
     r.b64_nsb = std::max(1, (nm + s->b64_sites - 1) / s->b64_sites);
-    r.s64_rowkk = s->d_s64_rowkk.p; r.s64_groups = s->d_s64_groups.p; r.b64_tile = s->d_b64_tile.p;
-    r.b64_rg = s->d_b64_rg.p; r.s64_tilent = s->d_s64_tilent.p; r.s64_pcoff = s->d_s64_pcoff.p;
+    r.s64_rowkk = s->d_s64_rowkk.p; r.s64_waves = s->d_s64_waves.p; r.b64_tile = s->d_b64_tile.p;
+    r.b64_rg = s->d_b64_rg.p; r.s64_pncol = s->d_s64_pncol.p; r.s64_pcoff = s->d_s64_pcoff.p;
     r.s64_sorted_of = s->d_s64_sorted_of.p; r.s64_part = s->d_s64_part.p; r.b64_rinfo = s->d_b64_rinfo.p;
     r.b64_tpk = s->d_b64_tpk.p;
 }
@@ -3028,7 +3036,7 @@ static int batch_groups(mdf_batch *b) {
         } else {
             const int g = slot(b->gsf, b->n_gsf, s->s64_nt);
             b->gsf[g].lds = std::max(b->gsf[g].lds, sf64_lds_bytes(s->klmax, s->kmmax, s->s64_nt));
-            for (int l = 0; l < r.s64_nsb * s->s64_ngrp; l++) msf[g].push_back(make_int2(i, l));
+            for (int l = 0; l < r.s64_nsb * s->s64_npass; l++) msf[g].push_back(make_int2(i, l));
             const int h = slot(b->gbp, b->n_gbp, s->b64_nks * 4 + s->b64_ng);
             for (int l = 0; l < r.b64_nsb * s->b64_nrg; l++) mbp[h].push_back(make_int2(i, l));
         }
@@ -3143,7 +3151,7 @@ static void batch_launch(mdf_batch *b, const double *const *d_in) {
         for (int g = 0; g < b->n_gsf; g++) {
             const mdf_batch::Group &G = b->gsf[g];
             switch (G.key) {
-#define SF64B(NT) case NT: k_sf64_b<NT><<<G.n, 256, G.lds, st>>>(R, G.map.p); break;
+#define SF64B(NT) case NT: k_sf64_b<NT><<<G.n, 64 * SF64_WAVES, G.lds, st>>>(R, G.map.p); break;
                 SF64B(1) SF64B(2) SF64B(3) SF64B(4) SF64B(5) SF64B(6) SF64B(7) SF64B(8)
 #undef SF64B
             }

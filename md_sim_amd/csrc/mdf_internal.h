@@ -35,11 +35,14 @@ __host__ __device__ inline size_t sfm_lds_bytes(int sites, int n_nt) {
     return (size_t)sites * 44 + (size_t)2 * 32 * (16 * n_nt + 4) * 4 + (size_t)2 * 32 * 68 * 4;
 }
 constexpr int BP_GROUP_K = 1024;     // k-vectors per back-projection row group (its LDS stage)
-// FP64 reciprocal sum on the matrix cores: sites per LDS stage of the structure factor, and
-// the powers e, e^2, e^4, e^8 of a site's base phases kept for the tables (a table entry
-// n = g + 8 i starts from e^g, g < 8, and steps by e^8)
+// FP64 reciprocal sum on the matrix cores: sites per LDS stage of the structure factor, waves
+// of its block, column tiles per wave, and the powers e, e^2, ..., e^16 of a site's base
+// phases kept for the tables (a table entry n = h + 32 i starts from e^h, h < 32, and steps
+// by e^32)
 constexpr int SF64_SUB = 32;
-constexpr int SF64_POWS = 4;
+constexpr int SF64_WAVES = 16;
+constexpr int SF64_SLOTS = 3;
+constexpr int SF64_POWS = 5;
 // dynamic LDS of k_sf64: base-phase powers, q e^{i pi kl u}, e^{i pi km v} for km >= 0
 // (complex) and the cos/sin columns (16 n_nt + 2 per site) of one stage
 __host__ __device__ inline size_t sf64_lds_bytes(int klmax, int kmmax, int n_nt) {
@@ -336,8 +339,9 @@ struct RepArgs {
     unsigned *tpack;
     int *tscale;
     int klmax, kmmax, s64_nsb, s64_pcols, s64_sites, b64_nsb;
-    const int2 *s64_rowkk, *s64_groups, *b64_tile, *b64_rg;
-    const int *s64_tilent, *s64_pcoff, *s64_sorted_of;
+    const int2 *s64_rowkk, *b64_tile, *b64_rg;
+    const int4 *s64_waves;
+    const int *s64_pncol, *s64_pcoff, *s64_sorted_of;
     double2 *s64_part;
     const int4 *b64_rinfo;
     double *b64_tpk;

@@ -33,7 +33,7 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// e^m from e, e^2, e^4, e^8 (m in 0..15)
+// e^m from e, e^2, e^4, ... (m below 2^SF64_POWS)
 __device__ __forceinline__ double2 cpow_bits(const double2 *pw, int m) {
     double2 r = make_double2(1.0, 0.0);
 #pragma unroll
@@ -43,37 +43,38 @@ __device__ __forceinline__ double2 cpow_bits(const double2 *pw, int m) {
 }
 
 // ---- structure factor -------------------------------------------------------------------
-// Grid (site blocks, row groups); a block of 4 waves takes up to four 16-row tiles of the
-// k-set rows sorted by kn extent (one per wave) and sb_sites sites in stages of SF64_SUB.  Per
-// stage all threads build, in LDS, the base phases e^{i pi u}, e^{i pi v}, e^{i pi w} of the
-// stage's sites and their powers 2, 4, 8; then q e^{i pi kl u} (kl = 0..klmax),
-// e^{i pi km v} (km = 0..kmmax; negative km by conjugation) and the cos/sin columns up to
-// the group's widest row.
-// Each active wave forms its A operand P = (q X)(Y) per lane and runs its tile over the
-// column tiles its rows reach.  Partial sums per site block go to
-// partial[sb][pcoff[row] + col] (columns of a row: 2 E + 1), summed by k_sf64_reduce.
+// Grid (site blocks, passes); a block of SF64_WAVES waves takes a run of sb_sites sites in
+// stages of SF64_SUB and, in one pass, a set of work slots: a slot is one 16-column tile of
+// one 16-row tile of the k-set rows sorted by kn extent, and a wave holds up to SF64_SLOTS
+// slots (consecutive in row-tile order, so at most three row tiles), dealt so that the four
+// SIMDs of a compute unit get about equal numbers.  Per stage all threads build, in LDS, the
+// base phases e^{i pi u}, e^{i pi v}, e^{i pi w} of the stage's sites and their powers
+// 2 ... 16; then q e^{i pi kl u} (kl = 0..klmax), e^{i pi km v} (km = 0..kmmax; negative km by
+// conjugation) and the cos/sin columns the pass reaches: once per site and pass (at slab16k
+// one pass holds every row).  Each wave forms the A operand P = (q X)(Y) of its slots' rows
+// per lane and runs two products per slot and K-step of 4 sites.  Partial sums per site block
+// go to partial[sb][pcoff[row] + col] (columns of a row: 2 E + 1), summed by k_sf64_reduce.
 struct Sf64Args {
     const double *x, *y, *z, *q;
     int n;
     double cux, cuy, cuz;
     const int2 *rowkk;
-    const int *tile_nt;
-    const int2 *groups;
+    const int4 *waves;      // [pass][SF64_WAVES] slots (rt << 8 | ct) and count | same1 << 4 | same2 << 5
+    const int *pass_ncol;   // cos/sin columns a pass reaches
     int n_rows, klmax, kmmax;
     const int *pcoff;
     int pcols;
     double2 *partial;
     int sb_sites;
-    int bx, by;   // the block: site block, row group
+    int bx, by;   // the block: site block, pass
 };
 
-// the stage loop of one wave whose tile reaches NTC column tiles (a compile-time count, so
-// the products run without branches and the accumulators stay in place); NT sizes the LDS
-// column rows of the launch.  Every wave of the block builds tables and meets the barriers.
-// The waves of a block run different instantiations, so all LDS they share is the kernel's
-// (a static __shared__ array here would be one per instantiation): sQ is passed in.
-template <int NTC, int NT>
-__device__ __forceinline__ void sf64_run(const Sf64Args &A, int tile, bool active, double *sQ) {
+// the stage loop of one wave with N slots; S1 (S2): slot 1 (2) is in the row tile of slot 0
+// (1), so its A operand is the same; NT sizes the LDS column rows of the launch.  Every wave
+// of the block builds tables and meets the barriers.  The waves of a block run different
+// instantiations, so all LDS they share is the kernel's: sQ is passed in.
+template <int N, bool S1, bool S2, int NT>
+__device__ __forceinline__ void sf64_run(const Sf64Args &A, const int4 wk, double *sQ) {
     extern __shared__ __align__(16) double sf64_smem[];
     const int klp = A.klmax + 1, kmp = A.kmmax + 1, ep = 16 * NT + 2;
     double2 *sPow = reinterpret_cast<double2 *>(sf64_smem);    // [SUB][3][SF64_POWS]
@@ -82,18 +83,26 @@ __device__ __forceinline__ void sf64_run(const Sf64Args &A, int tile, bool activ
     double *sE = reinterpret_cast<double *>(sY + SF64_SUB * kmp);   // [SUB][ep]
     const int lane = threadIdx.x & 63;
     const int g = lane >> 4, c = lane & 15;
-    const int2 grp = A.groups[A.by];
-    const int ncol_grp = 16 * A.tile_nt[grp.x];   // tiles are sorted widest first
-    const int2 kk = A.rowkk[16 * (active ? tile : 0) + c];   // padded past n_rows with (0, 0)
-    const int my_kl = kk.x, my_km = abs(kk.y);
-    const double ysg = kk.y < 0 ? -1.0 : 1.0;   // e^{-i pi km v} = conj e^{i pi km v}
+    const int ncol = A.pass_ncol[A.by];
+    constexpr int NA = N > 0 ? N : 1;
+    const int sv[3] = {wk.x, wk.y, wk.z};
+    int rt[NA], ct[NA], my_kl[NA], my_km[NA];
+    double ysg[NA];
+#pragma unroll
+    for (int s = 0; s < NA; s++) {
+        rt[s] = N > 0 ? sv[s] >> 8 : 0;
+        ct[s] = N > 0 ? sv[s] & 255 : 0;
+        const int2 kk = A.rowkk[16 * rt[s] + c];   // padded past n_rows with (0, 0)
+        my_kl[s] = kk.x;
+        my_km[s] = abs(kk.y);
+        ysg[s] = kk.y < 0 ? -1.0 : 1.0;   // e^{-i pi km v} = conj e^{i pi km v}
+    }
     const int s_begin = A.bx * A.sb_sites, s_end = min(A.n, s_begin + A.sb_sites);
-    constexpr int NA = NTC > 0 ? NTC : 1;
     d4v accR[NA], accI[NA];
 #pragma unroll
-    for (int t = 0; t < NA; t++) {
-        accR[t] = d4v{0.0, 0.0, 0.0, 0.0};
-        accI[t] = d4v{0.0, 0.0, 0.0, 0.0};
+    for (int s = 0; s < NA; s++) {
+        accR[s] = d4v{0.0, 0.0, 0.0, 0.0};
+        accI[s] = d4v{0.0, 0.0, 0.0, 0.0};
     }
     // thread (site j, axis ax) of a stage's base phases holds the next stage's coordinate and
     // charge in registers: loaded during the products of the stage before
@@ -106,9 +115,8 @@ __device__ __forceinline__ void sf64_run(const Sf64Args &A, int tile, bool activ
         pcoord = ok ? (pax == 0 ? A.x[i] * A.cux : pax == 1 ? A.y[i] * A.cuy : A.z[i] * A.cuz) : 0.0;
         pq = ok ? A.q[i] : 0.0;
     };
-    fetch(s_begin);
-    for (int s0 = s_begin; s0 < s_end; s0 += SF64_SUB) {
-        // base phases and their powers: one (site, axis) per thread
+    // base phases and their powers of the fetched stage: one (site, axis) per thread
+    auto phases = [&]() {
         if (pthr) {
             double sn, cs;
             sincospi(pcoord, &sn, &cs);
@@ -121,68 +129,99 @@ __device__ __forceinline__ void sf64_run(const Sf64Args &A, int tile, bool activ
             }
             if (pax == 0) sQ[pj] = pq;
         }
+    };
+    fetch(s_begin);
+    phases();
+    if (s_begin + SF64_SUB < s_end) fetch(s_begin + SF64_SUB);
+    // table entries: thread (site j, phase h) takes entries h, h + 32, ...
+    constexpr int TPS = SF64_WAVES * 64 / SF64_SUB;   // threads per site
+    static_assert(TPS == 1 << SF64_POWS, "entry phases are the powers' bit patterns");
+    const int tj = threadIdx.x / TPS, th = threadIdx.x % TPS;
+    for (int s0 = s_begin; s0 < s_end; s0 += SF64_SUB) {
+        // the tables are rewritten only after every wave has passed the products of the
+        // stage before (the barrier), and the phases of this stage are in place
         __syncthreads();
-        {   // tables: thread (site j, phase g8) takes entries g8, g8 + 8, ...
-            const int j = threadIdx.x >> 3, g8 = threadIdx.x & 7;
-            const double2 *pu = sPow + (j * 3 + 0) * SF64_POWS;
-            const double2 *pv = sPow + (j * 3 + 1) * SF64_POWS;
-            const double2 *pz = sPow + (j * 3 + 2) * SF64_POWS;
-            const double qj = sQ[j];
-            double2 e = cpow_bits(pu, g8);
-            for (int kl = g8; kl < klp; kl += 8) {
-                sQX[j * klp + kl] = make_double2(qj * e.x, qj * e.y);
-                e = cmul(e, pu[3]);
-            }
-            e = cpow_bits(pv, g8);
-            for (int m = g8; m < kmp; m += 8) {
-                sY[j * kmp + m] = e;
-                e = cmul(e, pv[3]);
-            }
-            e = cpow_bits(pz, g8);
-            for (int nn = g8; 2 * nn - 1 < ncol_grp; nn += 8) {   // c_n at column 2n - 1, s_n at 2n
-                if (nn == 0) sE[j * ep] = 1.0;
-                else {
-                    sE[j * ep + 2 * nn - 1] = e.x;
-                    if (2 * nn < ncol_grp) sE[j * ep + 2 * nn] = e.y;
+        {
+            const double2 *pu = sPow + (tj * 3 + 0) * SF64_POWS;
+            const double2 *pv = sPow + (tj * 3 + 1) * SF64_POWS;
+            const double2 *pz = sPow + (tj * 3 + 2) * SF64_POWS;
+            const double qj = sQ[tj];
+            if (th < klp) {
+                double2 e = cpow_bits(pu, th);
+                const double2 st = cmul(pu[SF64_POWS - 1], pu[SF64_POWS - 1]);   // e^TPS
+                for (int kl = th; kl < klp; kl += TPS) {
+                    sQX[tj * klp + kl] = make_double2(qj * e.x, qj * e.y);
+                    e = cmul(e, st);
                 }
-                e = cmul(e, pz[3]);
+            }
+            if (th < kmp) {
+                double2 e = cpow_bits(pv, th);
+                const double2 st = cmul(pv[SF64_POWS - 1], pv[SF64_POWS - 1]);
+                for (int m = th; m < kmp; m += TPS) {
+                    sY[tj * kmp + m] = e;
+                    e = cmul(e, st);
+                }
+            }
+            {
+                double2 e = cpow_bits(pz, th);
+                const double2 st = cmul(pz[SF64_POWS - 1], pz[SF64_POWS - 1]);
+                for (int nn = th; 2 * nn - 1 < ncol; nn += TPS) {   // c_n at column 2n - 1, s_n at 2n
+                    if (nn == 0) sE[tj * ep] = 1.0;
+                    else {
+                        sE[tj * ep + 2 * nn - 1] = e.x;
+                        if (2 * nn < ncol) sE[tj * ep + 2 * nn] = e.y;
+                    }
+                    e = cmul(e, st);
+                }
             }
         }
         __syncthreads();
-        if (s0 + SF64_SUB < s_end) fetch(s0 + SF64_SUB);
-        if constexpr (NTC > 0) {
-            // the matrix products at raised wave priority while other blocks build tables
-            __builtin_amdgcn_s_setprio(1);
-#pragma unroll
+        // the next stage's phases (their readers, the table build above, are done): the
+        // threads that make them join the products afterwards, the other waves start at once
+        if (s0 + SF64_SUB < s_end) {
+            phases();
+            if (s0 + 2 * SF64_SUB < s_end) fetch(s0 + 2 * SF64_SUB);
+        }
+        if constexpr (N > 0) {
+            // one K-step at a time: the registers hold three slots' accumulators, not the
+            // operands of several K-steps; the other waves of the SIMD cover the LDS latency
+#pragma unroll 1
             for (int ks = 0; ks < SF64_SUB / 4; ks++) {
                 const int js = 4 * ks + g;
-                const double2 X = sQX[js * klp + my_kl], Y0 = sY[js * kmp + my_km];
-                const double yr = Y0.x, yi = ysg * Y0.y;
-                const double pr = X.x * yr - X.y * yi, pi_ = X.x * yi + X.y * yr;
-                const double *er = sE + js * ep + c;
+                double pr[NA], pi_[NA];
 #pragma unroll
-                for (int t = 0; t < NTC; t++) {
-                    const double b = er[16 * t];
-                    accR[t] = mfma64(pr, b, accR[t]);
-                    accI[t] = mfma64(pi_, b, accI[t]);
+                for (int s = 0; s < NA; s++) {
+                    const bool same = (s == 1 && S1) || (s == 2 && S2);
+                    if (same) {
+                        pr[s] = pr[s - 1];
+                        pi_[s] = pi_[s - 1];
+                    } else {
+                        const double2 X = sQX[js * klp + my_kl[s]], Y0 = sY[js * kmp + my_km[s]];
+                        const double yr = Y0.x, yi = ysg[s] * Y0.y;
+                        pr[s] = X.x * yr - X.y * yi;
+                        pi_[s] = X.x * yi + X.y * yr;
+                    }
+                }
+#pragma unroll
+                for (int s = 0; s < NA; s++) {
+                    const double b = sE[js * ep + 16 * ct[s] + c];
+                    accR[s] = mfma64(pr[s], b, accR[s]);
+                    accI[s] = mfma64(pi_[s], b, accI[s]);
                 }
             }
-            __builtin_amdgcn_s_setprio(0);
         }
-        __syncthreads();
     }
-    if constexpr (NTC > 0) {
-        // lane holds rows 4 e + g of the tile at column 16 t + c
+    if constexpr (N > 0) {
+        // lane holds rows 4 e + g of the slot's row tile at column 16 ct + c
 #pragma unroll
-        for (int e = 0; e < 4; e++) {
-            const int i = 16 * tile + 4 * e + g;
-            if (i >= A.n_rows) continue;
-            const int off = A.pcoff[i], ncol = A.pcoff[i + 1] - off;
-            double2 *dst = A.partial + (size_t)A.bx * A.pcols + off;
+        for (int s = 0; s < NA; s++) {
 #pragma unroll
-            for (int t = 0; t < NTC; t++) {
-                const int col = 16 * t + c;
-                if (col < ncol) dst[col] = make_double2(accR[t][e], accI[t][e]);
+            for (int e = 0; e < 4; e++) {
+                const int i = 16 * rt[s] + 4 * e + g;
+                if (i >= A.n_rows) continue;
+                const int off = A.pcoff[i], nc = A.pcoff[i + 1] - off;
+                const int col = 16 * ct[s] + c;
+                if (col < nc) A.partial[(size_t)A.bx * A.pcols + off + col] = make_double2(accR[s][e], accI[s][e]);
             }
         }
     }
@@ -190,53 +229,48 @@ __device__ __forceinline__ void sf64_run(const Sf64Args &A, int tile, bool activ
 
 template <int NT>
 __device__ __forceinline__ void sf64_body(const Sf64Args &A) {
-    const int wave = threadIdx.x >> 6;
-    const int2 grp = A.groups[A.by];
-    const int *tile_nt = A.tile_nt;
-    const int tile = grp.x + wave;
-    const bool active = tile < grp.y;   // uniform over the wave
-    const int nt = active ? tile_nt[tile] : 0;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // uniform: slots in scalar registers
+    const int4 wk = A.waves[A.by * SF64_WAVES + wave];
     __shared__ double sQ[SF64_SUB];
-    switch (nt) {   // wave-uniform: one specialised stage loop per column-tile count
-        case 0: sf64_run<0, NT>(A, tile, false, sQ); break;
-        case 1: if constexpr (NT >= 1) sf64_run<1, NT>(A, tile, true, sQ); break;
-        case 2: if constexpr (NT >= 2) sf64_run<2, NT>(A, tile, true, sQ); break;
-        case 3: if constexpr (NT >= 3) sf64_run<3, NT>(A, tile, true, sQ); break;
-        case 4: if constexpr (NT >= 4) sf64_run<4, NT>(A, tile, true, sQ); break;
-        case 5: if constexpr (NT >= 5) sf64_run<5, NT>(A, tile, true, sQ); break;
-        case 6: if constexpr (NT >= 6) sf64_run<6, NT>(A, tile, true, sQ); break;
-        case 7: if constexpr (NT >= 7) sf64_run<7, NT>(A, tile, true, sQ); break;
-        default: if constexpr (NT >= 8) sf64_run<8, NT>(A, tile, true, sQ); break;
+    switch (wk.w) {   // wave-uniform: one specialised stage loop per slot pattern
+        case 1: sf64_run<1, false, false, NT>(A, wk, sQ); break;
+        case 2: sf64_run<2, false, false, NT>(A, wk, sQ); break;
+        case 2 | 16: sf64_run<2, true, false, NT>(A, wk, sQ); break;
+        case 3: sf64_run<3, false, false, NT>(A, wk, sQ); break;
+        case 3 | 16: sf64_run<3, true, false, NT>(A, wk, sQ); break;
+        case 3 | 32: sf64_run<3, false, true, NT>(A, wk, sQ); break;
+        case 3 | 48: sf64_run<3, true, true, NT>(A, wk, sQ); break;
+        default: sf64_run<0, false, false, NT>(A, wk, sQ); break;
     }
 }
 
 template <int NT>
-__global__ __launch_bounds__(256) void k_sf64(const double *__restrict__ x, const double *__restrict__ y,
-                                              const double *__restrict__ z, const double *__restrict__ q, int n,
-                                              double cux, double cuy, double cuz, const int2 *__restrict__ rowkk,
-                                              const int *__restrict__ tile_nt, const int2 *__restrict__ groups,
-                                              int n_rows, int klmax, int kmmax, const int *__restrict__ pcoff,
-                                              int pcols, double2 *__restrict__ partial, int sb_sites,
-                                              const MtsState *__restrict__ mts) {
+__global__ __launch_bounds__(64 * SF64_WAVES) void k_sf64(
+    const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
+    const double *__restrict__ q, int n, double cux, double cuy, double cuz, const int2 *__restrict__ rowkk,
+    const int4 *__restrict__ waves, const int *__restrict__ pass_ncol, int n_rows, int klmax, int kmmax,
+    const int *__restrict__ pcoff, int pcols, double2 *__restrict__ partial, int sb_sites,
+    const MtsState *__restrict__ mts) {
     if (mts && !mts->exact) return;   // a surrogate step: no k-sum
-    const Sf64Args A{x, y, z, q, n, cux, cuy, cuz, rowkk, tile_nt, groups, n_rows, klmax, kmmax, pcoff, pcols,
+    const Sf64Args A{x, y, z, q, n, cux, cuy, cuz, rowkk, waves, pass_ncol, n_rows, klmax, kmmax, pcoff, pcols,
                      partial, sb_sites, (int)blockIdx.x, (int)blockIdx.y};
     sf64_body<NT>(A);
 }
-// batched: block map entry (system, local block), local block = row group * site blocks + site block
+// batched: block map entry (system, local block), local block = pass * site blocks + site block
 template <int NT>
-__global__ __launch_bounds__(256) void k_sf64_b(const RepArgs *__restrict__ RA, const int2 *__restrict__ map) {
+__global__ __launch_bounds__(64 * SF64_WAVES) void k_sf64_b(const RepArgs *__restrict__ RA,
+                                                            const int2 *__restrict__ map) {
     const int2 m = map[blockIdx.x];
     const RepArgs &r = RA[m.x];
-    const Sf64Args A{r.S.x, r.S.y, r.S.z, r.S.q, r.S.n_move, r.cux, r.cuy, r.cuz, r.s64_rowkk, r.s64_tilent,
-                     r.s64_groups, r.nrows, r.klmax, r.kmmax, r.s64_pcoff, r.s64_pcols, r.s64_part, r.s64_sites,
+    const Sf64Args A{r.S.x, r.S.y, r.S.z, r.S.q, r.S.n_move, r.cux, r.cuy, r.cuz, r.s64_rowkk, r.s64_waves,
+                     r.s64_pncol, r.nrows, r.klmax, r.kmmax, r.s64_pcoff, r.s64_pcols, r.s64_part, r.s64_sites,
                      m.y % r.s64_nsb, m.y / r.s64_nsb};
     sf64_body<NT>(A);
 }
 
 #define SF64_INST(NT)                                                                                            \
     template __global__ void k_sf64<NT>(const double *, const double *, const double *, const double *, int,     \
-                                        double, double, double, const int2 *, const int *, const int2 *, int, int, \
+                                        double, double, double, const int2 *, const int4 *, const int *, int, int, \
                                         int, const int *, int, double2 *, int, const MtsState *);                \
     template __global__ void k_sf64_b<NT>(const RepArgs *, const int2 *);
 SF64_INST(1) SF64_INST(2) SF64_INST(3) SF64_INST(4) SF64_INST(5) SF64_INST(6) SF64_INST(7) SF64_INST(8)
