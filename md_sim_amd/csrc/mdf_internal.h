@@ -36,19 +36,23 @@ __host__ __device__ inline size_t sfm_lds_bytes(int sites, int n_nt) {
 }
 constexpr int BP_GROUP_K = 1024;     // k-vectors per back-projection row group (its LDS stage)
 // FP64 reciprocal sum on the matrix cores: sites per LDS stage of the structure factor, waves
-// of its block, column tiles per wave, and the powers e, e^2, ..., e^16 of a site's base
-// phases kept for the tables (a table entry n = h + 32 i starts from e^h, h < 32, and steps
-// by e^32)
+// of its block, column tiles per wave, and the powers e, e^2, e^4, e^8 of a site's base
+// phases kept for the tables (a table entry n = h + 8 i starts from e^h, h < 8, and steps by
+// e^8; q e^{i pi kl u} and e^{i pi km v} by e^4)
 constexpr int SF64_SUB = 32;
 constexpr int SF64_WAVES = 16;
 constexpr int SF64_SLOTS = 3;
-constexpr int SF64_POWS = 5;
+constexpr int SF64_POWS = 4;
 // dynamic LDS of k_sf64: base-phase powers, q e^{i pi kl u}, e^{i pi km v} for km >= 0
 // (complex) and the cos/sin columns (16 n_nt + 2 per site) of one stage
+// cos/sin columns per site in LDS: at least the 16 n_nt columns the products read, and an odd
+// multiple of 16 doubles, so that the 16-column windows of two consecutive sites of a K-step
+// fall on disjoint halves of the 64 four-byte banks
+__host__ __device__ constexpr int sf64_ep(int n_nt) { return 16 * n_nt + ((n_nt & 1) ? 0 : 16); }
 __host__ __device__ inline size_t sf64_lds_bytes(int klmax, int kmmax, int n_nt) {
-    return (size_t)SF64_SUB * (16 * (3 * SF64_POWS + (klmax + 1) + (kmmax + 1)) + 8 * (16 * n_nt + 2));
+    return (size_t)SF64_SUB * (16 * (3 * SF64_POWS + (klmax + 1) + (kmmax + 1)) + 8 * sf64_ep(n_nt));
 }
-constexpr size_t SF64_LDS_MAX = 64 * 1024 - 512;   // dynamic LDS of a k_sf64 block, at most
+constexpr size_t SF64_LDS_MAX = 159 * 1024;   // dynamic LDS of a k_sf64 block, at most (gfx950: 160 KiB per unit)
 
 // k-set rows (contiguous kn) and structure-factor chunks (<= KC consecutive kn of a row)
 struct KChunk { int kl, km, kn0, len, koff; };

@@ -47,9 +47,9 @@ __device__ __forceinline__ double2 cpow_bits(const double2 *pw, int m) {
 // stages of SF64_SUB and, in one pass, a set of work slots: a slot is one 16-column tile of
 // one 16-row tile of the k-set rows sorted by kn extent, and a wave holds up to SF64_SLOTS
 // slots (consecutive in row-tile order, so at most three row tiles), dealt so that the four
-// SIMDs of a compute unit get about equal numbers.  Per stage all threads build, in LDS, the
-// base phases e^{i pi u}, e^{i pi v}, e^{i pi w} of the stage's sites and their powers
-// 2 ... 16; then q e^{i pi kl u} (kl = 0..klmax), e^{i pi km v} (km = 0..kmmax; negative km by
+// SIMDs of a compute unit get about equal numbers.  Per stage the block builds, in LDS, the
+// base phases e^{i pi u}, e^{i pi v}, e^{i pi w} of the stage's sites and their powers 2, 4,
+// 8; then q e^{i pi kl u} (kl = 0..klmax), e^{i pi km v} (km = 0..kmmax; negative km by
 // conjugation) and the cos/sin columns the pass reaches: once per site and pass (at slab16k
 // one pass holds every row).  Each wave forms the A operand P = (q X)(Y) of its slots' rows
 // per lane and runs two products per slot and K-step of 4 sites.  Partial sums per site block
@@ -76,7 +76,8 @@ struct Sf64Args {
 template <int N, bool S1, bool S2, int NT>
 __device__ __forceinline__ void sf64_run(const Sf64Args &A, const int4 wk, double *sQ) {
     extern __shared__ __align__(16) double sf64_smem[];
-    const int klp = A.klmax + 1, kmp = A.kmmax + 1, ep = 16 * NT + 2;
+    const int klp = A.klmax + 1, kmp = A.kmmax + 1;
+    constexpr int ep = sf64_ep(NT);
     double2 *sPow = reinterpret_cast<double2 *>(sf64_smem);    // [SUB][3][SF64_POWS]
     double2 *sQX = sPow + SF64_SUB * 3 * SF64_POWS;             // [SUB][klp]
     double2 *sY = sQX + SF64_SUB * klp;                         // [SUB][kmp]
@@ -133,39 +134,39 @@ __device__ __forceinline__ void sf64_run(const Sf64Args &A, const int4 wk, doubl
     fetch(s_begin);
     phases();
     if (s_begin + SF64_SUB < s_end) fetch(s_begin + SF64_SUB);
-    // table entries: thread (site j, phase h) takes entries h, h + 32, ...
-    constexpr int TPS = SF64_WAVES * 64 / SF64_SUB;   // threads per site
-    static_assert(TPS == 1 << SF64_POWS, "entry phases are the powers' bit patterns");
-    const int tj = threadIdx.x / TPS, th = threadIdx.x % TPS;
+    // table entries, by the first 512 threads of the block, each wave on one table: threads
+    // 0-127 build q e^{i pi kl u} for kl = h, h + 4, ... (site tid / 4, h = tid % 4), threads
+    // 128-255 e^{i pi km v} likewise, threads 256-511 the cos/sin columns n = h, h + 8, ...
+    // (site (tid - 256) / 8, h < 8): a short product recursion from e^h (at most three
+    // products) in steps of e^4 or e^8
+    static_assert(SF64_SUB * 16 <= SF64_WAVES * 64, "16 table threads per site");
+    const int ax = threadIdx.x < 4 * SF64_SUB ? 0 : threadIdx.x < 8 * SF64_SUB ? 1 : 2;
+    const int tj = ax < 2 ? (threadIdx.x & (4 * SF64_SUB - 1)) >> 2 : (threadIdx.x - 8 * SF64_SUB) >> 3;
+    const int ph = ax < 2 ? threadIdx.x & 3 : threadIdx.x & 7;
+    const bool tthr = threadIdx.x < 16 * SF64_SUB;
     for (int s0 = s_begin; s0 < s_end; s0 += SF64_SUB) {
         // the tables are rewritten only after every wave has passed the products of the
         // stage before (the barrier), and the phases of this stage are in place
         __syncthreads();
-        {
-            const double2 *pu = sPow + (tj * 3 + 0) * SF64_POWS;
-            const double2 *pv = sPow + (tj * 3 + 1) * SF64_POWS;
-            const double2 *pz = sPow + (tj * 3 + 2) * SF64_POWS;
-            const double qj = sQ[tj];
-            if (th < klp) {
-                double2 e = cpow_bits(pu, th);
-                const double2 st = cmul(pu[SF64_POWS - 1], pu[SF64_POWS - 1]);   // e^TPS
-                for (int kl = th; kl < klp; kl += TPS) {
+        if (tthr) {
+            const double2 *pw = sPow + (tj * 3 + ax) * SF64_POWS;
+            double2 e = cpow_bits(pw, ph);
+            if (ax == 0) {
+                const double2 st = pw[2];   // e^4
+                const double qj = sQ[tj];
+                for (int kl = ph; kl < klp; kl += 4) {
                     sQX[tj * klp + kl] = make_double2(qj * e.x, qj * e.y);
                     e = cmul(e, st);
                 }
-            }
-            if (th < kmp) {
-                double2 e = cpow_bits(pv, th);
-                const double2 st = cmul(pv[SF64_POWS - 1], pv[SF64_POWS - 1]);
-                for (int m = th; m < kmp; m += TPS) {
+            } else if (ax == 1) {
+                const double2 st = pw[2];
+                for (int m = ph; m < kmp; m += 4) {
                     sY[tj * kmp + m] = e;
                     e = cmul(e, st);
                 }
-            }
-            {
-                double2 e = cpow_bits(pz, th);
-                const double2 st = cmul(pz[SF64_POWS - 1], pz[SF64_POWS - 1]);
-                for (int nn = th; 2 * nn - 1 < ncol; nn += TPS) {   // c_n at column 2n - 1, s_n at 2n
+            } else {
+                const double2 st = pw[3];   // e^8
+                for (int nn = ph; 2 * nn - 1 < ncol; nn += 8) {   // c_n at column 2n - 1, s_n at 2n
                     if (nn == 0) sE[tj * ep] = 1.0;
                     else {
                         sE[tj * ep + 2 * nn - 1] = e.x;
