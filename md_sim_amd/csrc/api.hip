@@ -737,8 +737,14 @@ static int setup_mma64(mdf_system *s) {
         const bool cont = r >= 4 && s->rows[r - 4].kl == w.kl && s->rows[r - 4].km + 4 == w.km;
         rinfo[r] = make_int4(w.kl, w.km, cont ? 0 : 1, 0);
     }
-    s->b64_nks = 4 * ((nks_max + 3) / 4);
-    s->b64_ng = s->b64_ng_env > 0 ? s->b64_ng_env : (s->b64_nks <= 20 ? 2 : 1);
+    // site groups per wave: three (their operands at the exact K-step count, 9..20, fill the
+    // registers: two waves per SIMD, and 2 n_cu blocks of 192 sites run in one round at
+    // slab16k), else two up to 20 K-steps, else one
+    const int ng_def = nks_max <= 20 ? (nks_max >= 9 ? 3 : 2) : 1;
+    s->b64_ng = s->b64_ng_env > 0 ? s->b64_ng_env : ng_def;
+    if (s->b64_ng == 3 && (nks_max < 9 || nks_max > 20)) s->b64_ng = ng_def;   // not instantiated
+    if (s->b64_ng == 2 && nks_max > 32) s->b64_ng = 1;
+    s->b64_nks = s->b64_ng == 3 ? nks_max : 4 * ((nks_max + 3) / 4);
     s->b64_sites = 4 * 16 * s->b64_ng;   // BP64_WAVES waves of 16 NG sites
     s->b64_nmt = n_mt;
     const int nsb_bp = std::max(1, (nm + s->b64_sites - 1) / s->b64_sites);
@@ -989,7 +995,9 @@ static void launch_bp(mdf_system *s) {
             BP64_LAUNCH(4, 1) BP64_LAUNCH(8, 1) BP64_LAUNCH(12, 1) BP64_LAUNCH(16, 1) BP64_LAUNCH(20, 1)
             BP64_LAUNCH(24, 1) BP64_LAUNCH(28, 1) BP64_LAUNCH(32, 1) BP64_LAUNCH(4, 2) BP64_LAUNCH(8, 2)
             BP64_LAUNCH(12, 2) BP64_LAUNCH(16, 2) BP64_LAUNCH(20, 2) BP64_LAUNCH(24, 2) BP64_LAUNCH(28, 2)
-            BP64_LAUNCH(32, 2)
+            BP64_LAUNCH(32, 2) BP64_LAUNCH(9, 3) BP64_LAUNCH(10, 3) BP64_LAUNCH(11, 3) BP64_LAUNCH(12, 3)
+            BP64_LAUNCH(13, 3) BP64_LAUNCH(14, 3) BP64_LAUNCH(15, 3) BP64_LAUNCH(16, 3) BP64_LAUNCH(17, 3)
+            BP64_LAUNCH(18, 3) BP64_LAUNCH(19, 3) BP64_LAUNCH(20, 3)
         }
 #undef BP64_LAUNCH
         return;
@@ -1488,7 +1496,7 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
         const char *s64 = getenv("MDF_SF64_SITES");
         if (s64 && atoi(s64) > 0) s->s64_sites_env = SF64_SUB * ((atoi(s64) + SF64_SUB - 1) / SF64_SUB);
         const char *g64 = getenv("MDF_BP64_NG");
-        if (g64 && (atoi(g64) == 1 || atoi(g64) == 2)) s->b64_ng_env = atoi(g64);
+        if (g64 && atoi(g64) >= 1 && atoi(g64) <= 3) s->b64_ng_env = atoi(g64);
         const char *r64 = getenv("MDF_BP64_GROUPS");
         if (r64 && atoi(r64) > 0) s->b64_rg_env = std::min(64, atoi(r64));
         // k_sf_mma splits q exp(i phi) of every moving site into FP16 hi/lo parts, which hold
@@ -3163,7 +3171,8 @@ static void batch_launch(mdf_batch *b, const double *const *d_in) {
 #define BP64B(NKS, NG) case NKS * 4 + NG: k_bp64_b<NKS, NG><<<G.n, 256, 0, st>>>(R, G.map.p); break;
                 BP64B(4, 1) BP64B(8, 1) BP64B(12, 1) BP64B(16, 1) BP64B(20, 1) BP64B(24, 1) BP64B(28, 1)
                 BP64B(32, 1) BP64B(4, 2) BP64B(8, 2) BP64B(12, 2) BP64B(16, 2) BP64B(20, 2) BP64B(24, 2)
-                BP64B(28, 2) BP64B(32, 2)
+                BP64B(28, 2) BP64B(32, 2) BP64B(9, 3) BP64B(10, 3) BP64B(11, 3) BP64B(12, 3) BP64B(13, 3)
+                BP64B(14, 3) BP64B(15, 3) BP64B(16, 3) BP64B(17, 3) BP64B(18, 3) BP64B(19, 3) BP64B(20, 3)
 #undef BP64B
             }
         }

@@ -586,6 +586,9 @@ __global__ __launch_bounds__(64 * BP64_WAVES) void k_bp64_b(const RepArgs *__res
 BP64_INST(4, 1) BP64_INST(8, 1) BP64_INST(12, 1) BP64_INST(16, 1) BP64_INST(20, 1) BP64_INST(24, 1) BP64_INST(28, 1)
 BP64_INST(32, 1) BP64_INST(4, 2) BP64_INST(8, 2) BP64_INST(12, 2) BP64_INST(16, 2) BP64_INST(20, 2) BP64_INST(24, 2)
 BP64_INST(28, 2) BP64_INST(32, 2)
+// three site groups per wave at exact K-step counts 9..20 (their operands fill the registers)
+BP64_INST(9, 3) BP64_INST(10, 3) BP64_INST(11, 3) BP64_INST(12, 3) BP64_INST(13, 3) BP64_INST(14, 3) BP64_INST(15, 3)
+BP64_INST(16, 3) BP64_INST(17, 3) BP64_INST(18, 3) BP64_INST(19, 3) BP64_INST(20, 3)
 #undef BP64_INST
 
 }  // namespace mdf
