@@ -278,13 +278,14 @@ SF64_INST(1) SF64_INST(2) SF64_INST(3) SF64_INST(4) SF64_INST(5) SF64_INST(6) SF
 #undef SF64_INST
 
 // ---- reduction, one block per k-set row (reference order) ---------------------------------
-// Column sums over the site blocks (fixed order: SF64R_SLICES slices, then the slices in order),
+// Column sums over the site blocks (fixed order for the row's width: the block's threads are
+// nsl slices of cw >= ncol columns, cw a multiple of 16; slice sl adds site blocks sl, sl + nsl,
+// ..., then the slices are added in order),
 // S(row, +-n) + S_wall, T = A conj S, the energy and virial partials of the row, and the
 // back-projection's coefficient operand for the row: in M-tile m = row / 4 (4 rows x 4 outputs
 // Re G, Im G, Re Gz, Im Gz; output e of row r is matrix row 4 e + r % 4), K-step ks of 4
 // columns, lane (col % 4) * 16 + (4 e + r % 4): tpk[(bpoff[m] + ks) * 64 + lane].
 constexpr int SF64R_COLS = 128;   // columns of a row: 2 E + 1 <= 127
-constexpr int SF64R_SLICES = SFR_THREADS / SF64R_COLS;
 
 // row r of the k-set (reference order)
 __device__ __forceinline__ void sf64r_body(
@@ -293,27 +294,28 @@ __device__ __forceinline__ void sf64r_body(
     const double *__restrict__ ak, double pipix, double pipiy, double pipiz, double lamb,
     double2 *__restrict__ s_out, double4 *__restrict__ t_out, double *__restrict__ epart,
     const int2 *__restrict__ bptile, double *__restrict__ tpk, int r) {
-    __shared__ double2 red[SF64R_SLICES][SF64R_COLS];
+    __shared__ double2 red[SFR_THREADS];   // [slice][cw]
     __shared__ double2 sSum[SF64R_COLS];
     __shared__ double2 sTp[64], sTm[64];   // T(row, +n), T(row, -n) (zero outside the row)
     const KRow row = rows[r];
     const int i = sorted_of[r];
     const int off = pcoff[i], ncol = pcoff[i + 1] - off, E = (ncol - 1) / 2;
-    const int col = threadIdx.x % SF64R_COLS, sl = threadIdx.x / SF64R_COLS;
+    const int cw = 16 * ((ncol + 15) / 16), nsl = SFR_THREADS / cw;
+    const int col = threadIdx.x % cw, sl = threadIdx.x / cw;
     double sr = 0.0, si = 0.0;
-    if (col < ncol) {
+    if (col < ncol && sl < nsl) {
 #pragma unroll 4
-        for (int b = sl; b < n_sb; b += SF64R_SLICES) {
+        for (int b = sl; b < n_sb; b += nsl) {
             const double2 v = partial[(size_t)b * pcols + off + col];
             sr += v.x; si += v.y;
         }
     }
-    red[sl][col] = make_double2(sr, si);
+    if (sl < nsl) red[threadIdx.x] = make_double2(sr, si);
     __syncthreads();
     if (threadIdx.x < SF64R_COLS) {
         double ar = 0.0, ai = 0.0;
-#pragma unroll
-        for (int k = 0; k < SF64R_SLICES; k++) { ar += red[k][threadIdx.x].x; ai += red[k][threadIdx.x].y; }
+        if (threadIdx.x < cw)
+            for (int k = 0; k < nsl; k++) { ar += red[k * cw + threadIdx.x].x; ai += red[k * cw + threadIdx.x].y; }
         sSum[threadIdx.x] = make_double2(ar, ai);   // (sum Pr X, sum Pi X), X = column col
     }
     __syncthreads();
