@@ -263,6 +263,13 @@ struct mdf_system {
     int device = 0;
     hipStream_t stream = nullptr;
     bool own_stream = true;
+    // the captured step graph forks the k-space chain (structure factor, reduction,
+    // back-projection) onto a second stream beside the list and pair kernels, and joins it
+    // before the epilogue; direct launches keep one stream
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    bool kspace_forked = false;   // set while a capture holds the forked chain
+    int fork_env = 1;
     int nijt = 0, n_lead = 0, n_wall = 0;
     DevSys S;
     // host copies
@@ -793,6 +800,8 @@ static int apply_geometry(mdf_system *s) {
 }
 
 static void destroy_events(mdf_system *s) {
+    if (s->ev_fork) { (void)hipEventDestroy(s->ev_fork); s->ev_fork = nullptr; }
+    if (s->ev_join) { (void)hipEventDestroy(s->ev_join); s->ev_join = nullptr; }
     for (int r = 0; r < mdf_system::NRING; r++) {
         for (int k = 0; k <= N_STAGES; k++)
             if (s->ring[r][k]) (void)hipEventDestroy(s->ring[r][k]);
@@ -1050,29 +1059,14 @@ static FinalArgs final_args(const mdf_system *s) {
     return A;
 }
 
-static void launch_forces(mdf_system *s) {
+// the k-space chain on the handle's current stream: structure factor, its reduction (S, T,
+// energies, the back-projection's operand) and the back-projection
+static void launch_kspace(mdf_system *s) {
     DevSys &S = s->S;
     hipStream_t st = s->stream;
     const int nm = s->top.n_move;
     const RecipArgs ra = recip_args(s);
-    const bool mts = s->mts_on;
-    const MtsArgs ma = mts ? mts_args(s) : MtsArgs{};
-    if (s->ev) (void)hipEventRecord(s->ev[2], st);
-    if (mts) k_sg_control<<<1, 64, 0, st>>>(ma, s->mts_again);
-    launch_pair(s);
-    if (s->ev) (void)hipEventRecord(s->ev[3], st);
-    if (mts) {
-        if (s->mts_model) {
-            k_sg_pack<<<std::max(1, (s->top.n_total + 255) / 256), 256, 0, st>>>(S, ma);
-            k_sg_features<SG_FEAT_G><<<std::max(1, (s->n_lead + 256 / SG_FEAT_G - 1) / (256 / SG_FEAT_G)), 256, 0,
-                                       st>>>(S, ma);
-            k_sg_norm<<<1, 1024, 0, st>>>(ma, nm);
-            k_sg_infer<<<s->sg_n_iblk, 256, 0, st>>>(ma, nm, 0, s->flags.p);
-        }
-        k_sg_combine<<<s->sg_n_sblk, 256, 0, st>>>(S, ma);
-    }
-    if (s->ev) (void)hipEventRecord(s->ev[4], st);
-    int nk = (int)s->ak.size();
+    const int nk = (int)s->ak.size();
     launch_sf(s);
     if (s->ev) (void)hipEventRecord(s->ev[5], st);
     if (nk > 0) {
@@ -1096,6 +1090,33 @@ static void launch_forces(mdf_system *s) {
     }
     if (s->ev) (void)hipEventRecord(s->ev[6], st);
     launch_bp(s);
+}
+
+static void launch_forces(mdf_system *s) {
+    DevSys &S = s->S;
+    hipStream_t st = s->stream;
+    const int nm = s->top.n_move;
+    const RecipArgs ra = recip_args(s);
+    const bool mts = s->mts_on;
+    const MtsArgs ma = mts ? mts_args(s) : MtsArgs{};
+    if (s->ev) (void)hipEventRecord(s->ev[2], st);
+    if (mts) k_sg_control<<<1, 64, 0, st>>>(ma, s->mts_again);
+    launch_pair(s);
+    if (s->ev) (void)hipEventRecord(s->ev[3], st);
+    if (mts) {
+        if (s->mts_model) {
+            k_sg_pack<<<std::max(1, (s->top.n_total + 255) / 256), 256, 0, st>>>(S, ma);
+            k_sg_features<SG_FEAT_G><<<std::max(1, (s->n_lead + 256 / SG_FEAT_G - 1) / (256 / SG_FEAT_G)), 256, 0,
+                                       st>>>(S, ma);
+            k_sg_norm<<<1, 1024, 0, st>>>(ma, nm);
+            k_sg_infer<<<s->sg_n_iblk, 256, 0, st>>>(ma, nm, 0, s->flags.p);
+        }
+        k_sg_combine<<<s->sg_n_sblk, 256, 0, st>>>(S, ma);
+    }
+    if (s->ev) (void)hipEventRecord(s->ev[4], st);
+    const int nk = (int)s->ak.size();
+    if (s->kspace_forked) (void)hipStreamWaitEvent(st, s->ev_join, 0);   // the chain ran beside
+    else launch_kspace(s);
     if (s->ev) (void)hipEventRecord(s->ev[7], st);
     const int n_groups = (nk > 0 && nm > 0) ? (s->mma64 ? s->b64_nrg : s->bp_mma ? 1 : s->n_rg) : 0;
     k_epilogue<<<s->n_epblk, EPI_BLOCK, 0, st>>>(S, s->fpart.p, n_groups, s->d_wl.p, s->par.ew3dc, s->crf0,
@@ -1325,8 +1346,22 @@ static int run_step(mdf_system *s, int force_rebuild) {
             hipGraph_t g = nullptr;
             s->ev = nullptr;
             HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeRelaxed));
+            // the k-space chain needs only the gathered positions: forked onto the second
+            // stream first, it runs beside the skin test, list build and pair kernel
+            // (surrogate steps keep one stream: their kernels interleave with the k-sum)
+            if (s->fork_env && s->stream2 && !s->mts_on && !s->ak.empty() && s->top.n_move > 0) {
+                hipStream_t main = s->stream;
+                (void)hipEventRecord(s->ev_fork, main);
+                (void)hipStreamWaitEvent(s->stream2, s->ev_fork, 0);
+                s->stream = s->stream2;
+                launch_kspace(s);
+                s->stream = main;
+                (void)hipEventRecord(s->ev_join, s->stream2);
+                s->kspace_forked = true;
+            }
             launch_lists(s, 0);
             launch_forces(s);
+            s->kspace_forked = false;
             hipError_t ec = hipStreamEndCapture(s->stream, &g);
             s->ev = ev;
             if (ec != hipSuccess) return fail(MDF_ERR_HIP, "graph capture: %s", hipGetErrorString(ec));
@@ -1487,6 +1522,8 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
         if (pr && pr[0] == '1') s->pair_strided = true;
         const char *pco = getenv("MDF_PAIR_CELL_ORDER");
         if (pco && pco[0] == '0') s->pair_cell_order = 0;
+        const char *fk = getenv("MDF_FORK_KSPACE");
+        if (fk && fk[0] == '0') s->fork_env = 0;
         const char *sm = getenv("MDF_SF_MMA");
         if (sm && sm[0] == '0') s->sf_mma_allowed = false;
         const char *bm = getenv("MDF_BP_MMA");
@@ -1534,6 +1571,9 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
     } while (0)
     {
         HTRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+        HTRY(hipStreamCreateWithFlags(&s->stream2, hipStreamNonBlocking));
+        HTRY(hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming));
+        HTRY(hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming));
         HTRY(hipEventCreateWithFlags(&s->in_event, hipEventDisableTiming));
         for (int r = 0; r < mdf_system::NRING; r++)
             for (int k = 0; k <= N_STAGES; k++) HTRY(hipEventCreate(&s->ring[r][k]));
@@ -1961,6 +2001,7 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
 fail_out:
     free_all(s);
     if (s->stream) (void)hipStreamDestroy(s->stream);
+    if (s->stream2) (void)hipStreamDestroy(s->stream2);
     destroy_events(s);
     delete s;
     return rc;
@@ -2237,6 +2278,7 @@ void mdf_destroy(mdf_system *s) {
     destroy_events(s);
     if (s->in_event) (void)hipEventDestroy(s->in_event);
     if (s->stream) (void)hipStreamDestroy(s->stream);
+    if (s->stream2) (void)hipStreamDestroy(s->stream2);
     delete s;
 }
 

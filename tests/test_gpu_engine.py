@@ -150,6 +150,51 @@ def test_device_inputs_and_graph_replays_match_host_path():
     assert np.array_equal(r2["f_site"], ref["f_site"])
 
 
+@pytest.mark.parametrize("prec", [abi.FP64, abi.FP32_ACC64], ids=["fp64", "fp32"])
+def test_forked_kspace_graph_equals_one_stream(prec, monkeypatch):
+    """The captured step graph runs the k-space chain (structure factor, reduction,
+    back-projection) on a second stream beside the skin test, list build and pair kernel,
+    joined before the epilogue.  Over a trajectory of rigid translations with list rebuilds,
+    device-resident and replayed from graphs, every step equals the one-stream graph
+    (MDF_FORK_KSPACE=0) and the direct launches bit for bit."""
+    import torch
+    _say("case forked k-space slab2048", prec)
+    s = systems.config("slab2048")
+    x, y, z, cx, cy, cz = abi.positions(s)
+    rng = np.random.default_rng(11)
+    v = rng.normal(0, 0.05, size=(s.nrig, 3))
+    frames = []
+    for t in range(10):
+        xs, ys, zs = x.copy(), y.copy(), z.copy()
+        d = np.repeat(v * t, 3, axis=0)
+        xs[:s.nriga] += d[:, 0]; ys[:s.nriga] += d[:, 1]; zs[:s.nriga] += d[:, 2]
+        frames.append([torch.tensor(a[:s.nmove] if k < 3 else a, dtype=torch.float64, device="cuda:0")
+                       for k, a in enumerate((xs, ys, zs, cx + v[:, 0] * t, cy + v[:, 1] * t, cz + v[:, 2] * t))])
+
+    def run(env):
+        for k, val in env.items():
+            monkeypatch.setenv(k, val)
+        e = _engine(s, precision=prec)
+        out = []
+        for f in frames:
+            e.compute_device(*[a.data_ptr() for a in f])
+            out.append(e.read_outputs())
+        e.close()
+        for k in env:
+            monkeypatch.delenv(k)
+        return out
+
+    forked = run({})
+    one = run({"MDF_FORK_KSPACE": "0"})
+    direct = run({"MDF_GRAPHS": "0"})
+    assert any(r["list_rebuilt"] for r in forked[1:]), "the trajectory never triggered a rebuild"
+    for t, (a, b, c) in enumerate(zip(forked, one, direct)):
+        for key in ("f_site", "torque", "f_com"):
+            assert np.array_equal(a[key], b[key]) and np.array_equal(a[key], c[key]), (t, key)
+        assert a["e_conf"] == b["e_conf"] == c["e_conf"], t
+        assert a["e_recip"] == b["e_recip"] == c["e_recip"], t
+
+
 def test_list_capacity_overflow_reports_error():
     """A fixed, too small list capacity stops like the reference ('nnlist too big')."""
     from md_sim_amd.engine import MDForceError
