@@ -269,7 +269,7 @@ struct mdf_system {
     hipStream_t stream2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     bool kspace_forked = false;   // set while a capture holds the forked chain
-    int fork_env = 1;
+    int fork_env = -1;            // MDF_FORK_KSPACE: 1 always, 0 never, unset: large systems only
     int nijt = 0, n_lead = 0, n_wall = 0;
     DevSys S;
     // host copies
@@ -1327,6 +1327,8 @@ static int chem_enqueue(mdf_system *s) {
     return MDF_OK;
 }
 
+constexpr int FORK_MIN_SITES = 16384;
+
 // one step: the fixed launch sequence, replayed from a captured graph when possible (a
 // forced rebuild goes through direct launches)
 static int run_step(mdf_system *s, int force_rebuild) {
@@ -1345,11 +1347,27 @@ static int run_step(mdf_system *s, int force_rebuild) {
         if (!s->graph[0]) {
             hipGraph_t g = nullptr;
             s->ev = nullptr;
-            HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeRelaxed));
             // the k-space chain needs only the gathered positions: forked onto the second
             // stream first, it runs beside the skin test, list build and pair kernel
-            // (surrogate steps keep one stream: their kernels interleave with the k-sum)
-            if (s->fork_env && s->stream2 && !s->mts_on && !s->ak.empty() && s->top.n_move > 0) {
+            // (surrogate steps keep one stream: their kernels interleave with the k-sum).
+            // Forked by default only from FORK_MIN_SITES moving sites: below, the chains are
+            // short and many handles stepping side by side (each a stream pair on the few
+            // hardware queues) measured slower forked.
+            bool fork = (s->fork_env > 0 || (s->fork_env < 0 && s->top.n_move >= FORK_MIN_SITES)) && !s->mts_on &&
+                        !s->ak.empty() && s->top.n_move > 0;
+            // the second stream is made on first use, before the capture (handles that never
+            // fork keep one stream: streams are spread over the hardware queues in the order
+            // they are made)
+            if (fork && !s->stream2) {
+                if (hipStreamCreateWithFlags(&s->stream2, hipStreamNonBlocking) != hipSuccess ||
+                    hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming) != hipSuccess ||
+                    hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming) != hipSuccess) {
+                    (void)hipGetLastError();
+                    fork = false;
+                }
+            }
+            HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeRelaxed));
+            if (fork && s->stream2 && s->ev_fork && s->ev_join) {
                 hipStream_t main = s->stream;
                 (void)hipEventRecord(s->ev_fork, main);
                 (void)hipStreamWaitEvent(s->stream2, s->ev_fork, 0);
@@ -1523,7 +1541,7 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
         const char *pco = getenv("MDF_PAIR_CELL_ORDER");
         if (pco && pco[0] == '0') s->pair_cell_order = 0;
         const char *fk = getenv("MDF_FORK_KSPACE");
-        if (fk && fk[0] == '0') s->fork_env = 0;
+        if (fk) s->fork_env = fk[0] == '0' ? 0 : 1;
         const char *sm = getenv("MDF_SF_MMA");
         if (sm && sm[0] == '0') s->sf_mma_allowed = false;
         const char *bm = getenv("MDF_BP_MMA");
@@ -1571,9 +1589,6 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
     } while (0)
     {
         HTRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
-        HTRY(hipStreamCreateWithFlags(&s->stream2, hipStreamNonBlocking));
-        HTRY(hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming));
-        HTRY(hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming));
         HTRY(hipEventCreateWithFlags(&s->in_event, hipEventDisableTiming));
         for (int r = 0; r < mdf_system::NRING; r++)
             for (int k = 0; k <= N_STAGES; k++) HTRY(hipEventCreate(&s->ring[r][k]));
@@ -2917,6 +2932,12 @@ struct mdf_batch {
     std::vector<mdf_system *> sys;
     int device = 0;
     hipStream_t stream = nullptr;
+    // with MDF_FORK_KSPACE_BATCH=1 the k-space kernels run on a second stream beside the skin
+    // test, list build and pair kernel (forked after the gather, joined before the
+    // epilogue); off by default: the 64-replica batch measured 4% slower forked
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    bool fork = false;
     std::vector<RepArgs> h_rep;
     DBuf<RepArgs> d_rep;
     DBuf<int2> maps[BK_N];
@@ -3133,52 +3154,10 @@ static void batch_drop_graphs(mdf_batch *b) {
         if (b->graph_ring[k]) { (void)hipGraphExecDestroy(b->graph_ring[k]); b->graph_ring[k] = nullptr; }
 }
 
-static void batch_launch(mdf_batch *b, const double *const *d_in) {
-    const int n = (int)b->sys.size();
-    hipStream_t st = b->stream;
+// the batch's k-space kernels on stream st: structure factor, reduction, back-projection
+static void batch_kspace(mdf_batch *b, hipStream_t st) {
     const RepArgs *R = b->d_rep.p;
     auto grid = [&](int k) { return b->nmap[k]; };
-    if (grid(BK_GATHER)) k_gather_b<<<grid(BK_GATHER), 256, 0, st>>>(R, b->maps[BK_GATHER].p, d_in);
-    k_skin_b<<<grid(BK_SKIN), 256, 0, st>>>(R, b->maps[BK_SKIN].p);
-    k_decide_b<<<n, 256, 0, st>>>(R);
-    k_cell_count_b<<<grid(BK_COUNT), 256, 0, st>>>(R, b->maps[BK_COUNT].p);
-    k_cell_scan_build_b<<<n, 1024, 0, st>>>(R);
-    k_cell_fill_b<<<grid(BK_FILL), 256, 0, st>>>(R, b->maps[BK_FILL].p);
-    k_cell_order_b<<<grid(BK_ORDER), 256, 0, st>>>(R, b->maps[BK_ORDER].p);
-    k_build_b<<<grid(BK_BUILD), BUILD_THREADS, 0, st>>>(R, b->maps[BK_BUILD].p);
-    k_snapshot_b<<<grid(BK_SNAP), 256, 0, st>>>(R, b->maps[BK_SNAP].p);
-    {
-        const int2 *mp = b->maps[BK_PAIR].p;
-        const int g = grid(BK_PAIR);
-#define MDF_PB3(RT, G, ST, NM)                                                               \
-    do {                                                                                     \
-        if (b->acc) {                                                                        \
-            if (b->vir) k_pair_b<RT, G, true, true, ST, NM><<<g, 256, 0, st>>>(R, mp);       \
-            else k_pair_b<RT, G, false, true, ST, NM><<<g, 256, 0, st>>>(R, mp);             \
-        } else {                                                                             \
-            if (b->vir) k_pair_b<RT, G, true, false, ST, NM><<<g, 256, 0, st>>>(R, mp);      \
-            else k_pair_b<RT, G, false, false, ST, NM><<<g, 256, 0, st>>>(R, mp);            \
-        }                                                                                    \
-    } while (0)
-#define MDF_PB2(RT, G, ST)                                  \
-    do {                                                    \
-        if (b->one_cls) MDF_PB3(RT, G, ST, 1);              \
-        else MDF_PB3(RT, G, ST, MAX_LEAD_TYPES);            \
-    } while (0)
-#define MDF_PB(RT, G)                                  \
-    do {                                               \
-        if (b->pair_strided) MDF_PB2(RT, G, true);     \
-        else MDF_PB2(RT, G, false);                    \
-    } while (0)
-        if (b->f32) {
-            if (b->pair_g == 8) MDF_PB(float, 8); else if (b->pair_g == 32) MDF_PB(float, 32); else MDF_PB(float, 16);
-        } else {
-            if (b->pair_g == 8) MDF_PB(double, 8); else if (b->pair_g == 32) MDF_PB(double, 32); else MDF_PB(double, 16);
-        }
-#undef MDF_PB
-#undef MDF_PB2
-#undef MDF_PB3
-    }
     if (b->rmode == BR_MMA32) {
         for (int g = 0; g < b->n_gsf; g++) {
             const mdf_batch::Group &G = b->gsf[g];
@@ -3229,6 +3208,64 @@ static void batch_launch(mdf_batch *b, const double *const *d_in) {
             else k_bp_b<double><<<grid(BK_BP), 256, 0, st>>>(R, b->maps[BK_BP].p);
         }
     }
+}
+
+static void batch_launch(mdf_batch *b, const double *const *d_in) {
+    const int n = (int)b->sys.size();
+    hipStream_t st = b->stream;
+    const RepArgs *R = b->d_rep.p;
+    auto grid = [&](int k) { return b->nmap[k]; };
+    if (grid(BK_GATHER)) k_gather_b<<<grid(BK_GATHER), 256, 0, st>>>(R, b->maps[BK_GATHER].p, d_in);
+    // the k-space kernels need only the gathered positions: forked here onto the second
+    // stream, joined before the epilogue
+    if (b->fork) {
+        (void)hipEventRecord(b->ev_fork, st);
+        (void)hipStreamWaitEvent(b->stream2, b->ev_fork, 0);
+        batch_kspace(b, b->stream2);
+        (void)hipEventRecord(b->ev_join, b->stream2);
+    }
+    k_skin_b<<<grid(BK_SKIN), 256, 0, st>>>(R, b->maps[BK_SKIN].p);
+    k_decide_b<<<n, 256, 0, st>>>(R);
+    k_cell_count_b<<<grid(BK_COUNT), 256, 0, st>>>(R, b->maps[BK_COUNT].p);
+    k_cell_scan_build_b<<<n, 1024, 0, st>>>(R);
+    k_cell_fill_b<<<grid(BK_FILL), 256, 0, st>>>(R, b->maps[BK_FILL].p);
+    k_cell_order_b<<<grid(BK_ORDER), 256, 0, st>>>(R, b->maps[BK_ORDER].p);
+    k_build_b<<<grid(BK_BUILD), BUILD_THREADS, 0, st>>>(R, b->maps[BK_BUILD].p);
+    k_snapshot_b<<<grid(BK_SNAP), 256, 0, st>>>(R, b->maps[BK_SNAP].p);
+    {
+        const int2 *mp = b->maps[BK_PAIR].p;
+        const int g = grid(BK_PAIR);
+#define MDF_PB3(RT, G, ST, NM)                                                               \
+    do {                                                                                     \
+        if (b->acc) {                                                                        \
+            if (b->vir) k_pair_b<RT, G, true, true, ST, NM><<<g, 256, 0, st>>>(R, mp);       \
+            else k_pair_b<RT, G, false, true, ST, NM><<<g, 256, 0, st>>>(R, mp);             \
+        } else {                                                                             \
+            if (b->vir) k_pair_b<RT, G, true, false, ST, NM><<<g, 256, 0, st>>>(R, mp);      \
+            else k_pair_b<RT, G, false, false, ST, NM><<<g, 256, 0, st>>>(R, mp);            \
+        }                                                                                    \
+    } while (0)
+#define MDF_PB2(RT, G, ST)                                  \
+    do {                                                    \
+        if (b->one_cls) MDF_PB3(RT, G, ST, 1);              \
+        else MDF_PB3(RT, G, ST, MAX_LEAD_TYPES);            \
+    } while (0)
+#define MDF_PB(RT, G)                                  \
+    do {                                               \
+        if (b->pair_strided) MDF_PB2(RT, G, true);     \
+        else MDF_PB2(RT, G, false);                    \
+    } while (0)
+        if (b->f32) {
+            if (b->pair_g == 8) MDF_PB(float, 8); else if (b->pair_g == 32) MDF_PB(float, 32); else MDF_PB(float, 16);
+        } else {
+            if (b->pair_g == 8) MDF_PB(double, 8); else if (b->pair_g == 32) MDF_PB(double, 32); else MDF_PB(double, 16);
+        }
+#undef MDF_PB
+#undef MDF_PB2
+#undef MDF_PB3
+    }
+    if (b->fork) (void)hipStreamWaitEvent(st, b->ev_join, 0);
+    else batch_kspace(b, st);
     k_epilogue_b<<<grid(BK_EPI), EPI_BLOCK, 0, st>>>(R, b->maps[BK_EPI].p);
     k_colsum_b<<<grid(BK_COLSUM), 256, 0, st>>>(R, b->maps[BK_COLSUM].p);
     k_final_b<<<n, 256, 0, st>>>(R);
@@ -3248,7 +3285,10 @@ static void batch_free(mdf_batch *b) {
     }
     b->d_flags.release();
     if (b->in_event) (void)hipEventDestroy(b->in_event);
+    if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+    if (b->ev_join) (void)hipEventDestroy(b->ev_join);
     if (b->stream) (void)hipStreamDestroy(b->stream);
+    if (b->stream2) (void)hipStreamDestroy(b->stream2);
 }
 
 extern "C" {
@@ -3296,7 +3336,13 @@ int mdf_batch_create(mdf_system *const *systems, int32_t n, mdf_batch **out) {
     }
     int rc = MDF_OK;
     hipError_t e = hipSetDevice(b->device);
+    if (const char *fk = getenv("MDF_FORK_KSPACE_BATCH")) b->fork = fk[0] == '1';
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (b->fork) {
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream2, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming);
+    }
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->in_event, hipEventDisableTiming);
     for (int i = 0; i < n && e == hipSuccess; i++) e = hipStreamSynchronize(systems[i]->stream);
     if (e == hipSuccess) e = b->d_rep.alloc(n);

@@ -86,6 +86,35 @@ def test_batch_equals_single_handles(precision, kernels, monkeypatch):
         e.close()
 
 
+@pytest.mark.parametrize("precision", [abi.FP64, abi.FP32_ACC64], ids=["fp64", "fp32"])
+def test_batch_forked_kspace_equals_single_handles(precision, monkeypatch):
+    """With MDF_FORK_KSPACE_BATCH=1 the batch runs its k-space kernels on a second stream
+    beside the skin test, list build and pair kernel (joined before the epilogue); the
+    results stay bit for bit those of the handles alone, over graph replays."""
+    from md_sim_amd.engine import ForceBatch, ForceEngine
+    _say("case forked batch, precision", precision)
+    syss, kws = _mixed()
+    engines, singles, dev = [], [], []
+    for s, kw in zip(syss, kws):
+        pos = abi.positions(s)
+        e = ForceEngine(s, precision=precision, **kw)
+        singles.append(e.force_get(*pos))
+        engines.append(e)
+        dev.append(_dev(s, pos))
+    monkeypatch.setenv("MDF_FORK_KSPACE_BATCH", "1")
+    b = ForceBatch(engines)
+    monkeypatch.delenv("MDF_FORK_KSPACE_BATCH")
+    for rep in range(4):
+        b.compute_device([_ptrs(d, s) for d, s in zip(dev, syss)])
+        outs = b.read_outputs()
+        for i, (o, r) in enumerate(zip(outs, singles)):
+            assert np.array_equal(o["f_site"], r["f_site"]), (rep, i)
+            assert o["e_conf"] == r["e_conf"] and o["e_recip"] == r["e_recip"], (rep, i)
+    b.close()
+    for e in engines:
+        e.close()
+
+
 def test_batch_trajectory_matches_oracle(oracle_lib):
     """Replicas moving independently: rebuilds fire per replica as the reference rule says and
     every replica's forces stay at the FP64 bar."""

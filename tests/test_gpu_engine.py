@@ -184,7 +184,7 @@ def test_forked_kspace_graph_equals_one_stream(prec, monkeypatch):
             monkeypatch.delenv(k)
         return out
 
-    forked = run({})
+    forked = run({"MDF_FORK_KSPACE": "1"})   # slab2048 is below the default size for the fork
     one = run({"MDF_FORK_KSPACE": "0"})
     direct = run({"MDF_GRAPHS": "0"})
     assert any(r["list_rebuilt"] for r in forked[1:]), "the trajectory never triggered a rebuild"
