@@ -1115,8 +1115,14 @@ static void launch_forces(mdf_system *s) {
     }
     if (s->ev) (void)hipEventRecord(s->ev[4], st);
     const int nk = (int)s->ak.size();
-    if (s->kspace_forked) (void)hipStreamWaitEvent(st, s->ev_join, 0);   // the chain ran beside
-    else launch_kspace(s);
+    const dim3 gc(NCOL, s->n_colblk);
+    if (s->kspace_forked) {
+        // the pair kernel's per-lead columns are summed while the k-space chain still runs
+        k_colsum<<<gc, 256, 0, st>>>(S, s->lead_slot.p, s->colper, s->colpart.p);
+        (void)hipStreamWaitEvent(st, s->ev_join, 0);
+    } else {
+        launch_kspace(s);
+    }
     if (s->ev) (void)hipEventRecord(s->ev[7], st);
     const int n_groups = (nk > 0 && nm > 0) ? (s->mma64 ? s->b64_nrg : s->bp_mma ? 1 : s->n_rg) : 0;
     k_epilogue<<<s->n_epblk, EPI_BLOCK, 0, st>>>(S, s->fpart.p, n_groups, s->d_wl.p, s->par.ew3dc, s->crf0,
@@ -1130,9 +1136,8 @@ static void launch_forces(mdf_system *s) {
         k_obs_ngang<<<(S.n_angles + 255) / 256, 256, 0, st>>>(S, s->ang_type.p, s->n_atypes, s->ang_hist.p);
     if (s->ev) (void)hipEventRecord(s->ev[8], st);
     {
-        dim3 gc(NCOL, s->n_colblk);
         const FinalArgs A = final_args(s);
-        k_colsum<<<gc, 256, 0, st>>>(S, s->lead_slot.p, s->colper, s->colpart.p);
+        if (!s->kspace_forked) k_colsum<<<gc, 256, 0, st>>>(S, s->lead_slot.p, s->colper, s->colpart.p);
         k_final<<<1, 256, 0, st>>>(S, A, s->colpart.p, s->kpart.p, s->eppart.p, s->dipz.p, s->e_vdw.p, s->e_coul.p,
                                    s->d_scal.p);
     }

@@ -311,28 +311,6 @@ __global__ __launch_bounds__(256) void k_colsum(DevSys S, const int *__restrict_
     colsum_body(S, lead_slot, per_block, colpart, blockIdx.x, blockIdx.y, gridDim.x, gridDim.y);
 }
 
-// listed pair counts (ipack, wpack) after a rebuild: the per-row counts summed in fixed
-// order by one block of 256 threads
-__device__ void pair_totals(const DevSys &S) {
-    __shared__ long long npair[2][256];
-    const int t = threadIdx.x;
-    const bool rb = S.flags->rebuild != 0;
-    if (rb) {
-        long long a = 0, b = 0;
-        for (int l = t; l < S.n_lead; l += 256) { a += S.cnt_lo[l]; b += S.cnt_mw[l]; }
-        npair[0][t] = a; npair[1][t] = b;
-    }
-    __syncthreads();
-    if (rb && t == 0) {
-        long long a = 0, b = 0;
-        for (int k = 0; k < 256; k++) { a += npair[0][k]; b += npair[1][k]; }
-        S.flags->n_pairs_mm = (int)a;
-        S.flags->n_pairs_mw = (int)b;
-    }
-}
-
-// the same after the list build at creation (no force step follows it)
-__global__ __launch_bounds__(256) void k_pair_totals(DevSys S) { pair_totals(S); }
 
 __device__ __forceinline__ void final_body(const DevSys &S, FinalArgs A, const double *__restrict__ colpart,
                                                const double *__restrict__ kpart, const double *__restrict__ eppart,
@@ -341,7 +319,15 @@ __device__ __forceinline__ void final_body(const DevSys &S, FinalArgs A, const d
     __shared__ double colsum[NCOL][MAX_LEAD_TYPES];
     __shared__ double ksum[8];
     __shared__ double epsum[EP_N];
+    constexpr int NIJT1 = (MAX_TYPES + 1) * (MAX_TYPES + 1);
+    __shared__ int sij[NIJT1];
     const int t = threadIdx.x;
+    // every global read of the kernel is issued up front: the type-pair table and the dipole
+    // here, the partial sums below; one memory latency instead of one per phase
+    const int nt1 = S.n_types + 1;
+    for (int k = t; k < nt1 * nt1; k += blockDim.x) sij[k] = S.ijtype[k];
+    const double dz = t == 0 ? *dipz : 0.0;
+    const bool rb = S.flags->rebuild != 0;
     // columns x slots
     for (int idx = t; idx < NCOL * MAX_LEAD_TYPES; idx += blockDim.x) {
         int col = idx / MAX_LEAD_TYPES, s = idx % MAX_LEAD_TYPES;
@@ -374,19 +360,41 @@ __device__ __forceinline__ void final_body(const DevSys &S, FinalArgs A, const d
             else epsum[t - RE_N] = acc;
         }
     }
-    pair_totals(S);
+    // listed pair counts (ipack, wpack) after a rebuild: the per-row counts, four at a time
+    // with many loads in flight (integer sums: the same in any order)
+    __shared__ long long npair[2][256];
+    if (rb) {
+        long long a = 0, b = 0;
+        const int n4 = S.n_lead >> 2;
+        const int4 *lo4 = reinterpret_cast<const int4 *>(S.cnt_lo);
+        const int4 *mw4 = reinterpret_cast<const int4 *>(S.cnt_mw);
+#pragma unroll 8
+        for (int l = t; l < n4; l += blockDim.x) {
+            const int4 u = lo4[l], w = mw4[l];
+            a += (long long)u.x + u.y + u.z + u.w;
+            b += (long long)w.x + w.y + w.z + w.w;
+        }
+        for (int l = 4 * n4 + t; l < S.n_lead; l += blockDim.x) { a += S.cnt_lo[l]; b += S.cnt_mw[l]; }
+        npair[0][t] = a; npair[1][t] = b;
+    }
+    __syncthreads();
+    if (rb && t == 0) {
+        long long a = 0, b = 0;
+        for (int k = 0; k < (int)blockDim.x; k++) { a += npair[0][k]; b += npair[1][k]; }
+        S.flags->n_pairs_mm = (int)a;
+        S.flags->n_pairs_mw = (int)b;
+    }
     // energies per pair type: thread k adds the (slot, class) columns of type k in (slot,
     // class) order, in LDS, then writes them once
     constexpr int NIJM = MAX_TYPES * (MAX_TYPES + 1) / 2 + 1;
     __shared__ double sev[NIJM], sec[NIJM];
-    const int nt1 = S.n_types + 1;
     for (int k = t; k <= S.nijt; k += blockDim.x) {
         double ev = 0.0, ec = 0.0;
         for (int s = 0; s < A.n_slots; s++) {
             const int ti = A.slot_type[s];
             for (int c = 0; c < NCLS; c++) {
                 const int tc = A.cls_type[c];
-                if (tc <= 0 || S.ijtype[ti * nt1 + tc] != k) continue;
+                if (tc <= 0 || sij[ti * nt1 + tc] != k) continue;
                 ev += colsum[COL_EV + c][s];
                 ec += colsum[COL_EC + c][s];
             }
@@ -402,7 +410,6 @@ __device__ __forceinline__ void final_body(const DevSys &S, FinalArgs A, const d
     o.e_recip_ww = A.e_recip_ww;
     o.e_recip_wl = o.e_recip - o.e_recip_ww - o.e_recip_ll;
     o.e_self = A.e_self;
-    double dz = *dipz;
     o.e_slab = A.ew3dc ? -A.crf0 * S.facq * (dz * dz) : 0.0;
     o.e_wall_lr = epsum[0];
     double sum = 0.0;
@@ -438,6 +445,23 @@ __device__ __forceinline__ void final_body(const DevSys &S, FinalArgs A, const d
     o.nonfinite = !(isfinite(o.e_conf) && isfinite(o.press));
     o.pad = 0;
     *out = o;
+}
+
+// the listed pair counts after the list build at creation (no force step follows it): one
+// block of 256 threads, in fixed order
+__global__ __launch_bounds__(256) void k_pair_totals(DevSys S) {
+    __shared__ long long npair[2][256];
+    const int t = threadIdx.x;
+    long long a = 0, b = 0;
+    for (int l = t; l < S.n_lead; l += 256) { a += S.cnt_lo[l]; b += S.cnt_mw[l]; }
+    npair[0][t] = a; npair[1][t] = b;
+    __syncthreads();
+    if (t == 0) {
+        a = 0; b = 0;
+        for (int k = 0; k < 256; k++) { a += npair[0][k]; b += npair[1][k]; }
+        S.flags->n_pairs_mm = (int)a;
+        S.flags->n_pairs_mw = (int)b;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_final(DevSys S, FinalArgs A, const double *__restrict__ colpart,
