@@ -317,6 +317,7 @@ struct mdf_system {
     bool mma64 = false, mma64_allowed = true;
     int klmax = 0, kmmax = 0;
     int s64_nt = 1, s64_npass = 1, s64_nsb = 1, s64_sites = SF64_SUB, s64_pcols = 1, s64_sites_env = 0;
+    int s64_pass_env = 0;   // MDF_SF64_PASSES: at least this many passes (fewer, larger site blocks)
     int b64_nks = 4, b64_ng = 2, b64_nrg = 1, b64_nmt = 1, b64_sites = 128, b64_rg_env = 0, b64_ng_env = 0;
     DBuf<int2> d_s64_rowkk, d_b64_tile, d_b64_rg;
     DBuf<int4> d_s64_waves;
@@ -692,7 +693,7 @@ static int setup_mma64(mdf_system *s) {
     for (int t = 0; t < n_rt; t++)
         for (int c = 0; c < tile_nt[t]; c++) slots.push_back(make_int2(t, c));
     const int n_sl = (int)slots.size(), per_pass = SF64_WAVES * SF64_SLOTS;
-    const int npass = std::max(1, (n_sl + per_pass - 1) / per_pass);
+    const int npass = std::min(n_sl, std::max(std::max(1, s->s64_pass_env), (n_sl + per_pass - 1) / per_pass));
     std::vector<int4> waves((size_t)npass * SF64_WAVES, make_int4(0, 0, 0, 0));
     std::vector<int> pass_ncol(npass, 16);
     for (int p = 0; p < npass; p++) {
@@ -1555,6 +1556,8 @@ int mdf_create(const mdf_topology *top, const mdf_params *par, const double *x, 
         if (m64 && m64[0] == '0') s->mma64_allowed = false;
         const char *s64 = getenv("MDF_SF64_SITES");
         if (s64 && atoi(s64) > 0) s->s64_sites_env = SF64_SUB * ((atoi(s64) + SF64_SUB - 1) / SF64_SUB);
+        const char *p64 = getenv("MDF_SF64_PASSES");
+        if (p64 && atoi(p64) > 0) s->s64_pass_env = std::min(16, atoi(p64));
         const char *g64 = getenv("MDF_BP64_NG");
         if (g64 && atoi(g64) >= 1 && atoi(g64) <= 3) s->b64_ng_env = atoi(g64);
         const char *r64 = getenv("MDF_BP64_GROUPS");
