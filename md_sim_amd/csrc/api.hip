@@ -668,6 +668,13 @@ static void drop_graphs(mdf_system *s);
 // column tiles ceil((2 E + 1) / 16), slots of (row tile, column tile) dealt to the waves of
 // each pass, partial columns per row 2 E + 1 in sorted order.  Back-projection: M-tiles of 4 rows in the
 // reference order, K-steps ceil((2 E_max + 1) / 4) each, row groups of about equal K-steps.
+// the captured step graph forks the k-space chain onto a second stream from this many moving
+// sites (MDF_FORK_KSPACE=1 / 0 force it on / off)
+constexpr int FORK_MIN_SITES = 16384;
+static bool fork_planned(const mdf_system *s) {
+    return s->fork_env > 0 || (s->fork_env < 0 && s->top.n_move >= FORK_MIN_SITES);
+}
+
 static int setup_mma64(mdf_system *s) {
     const int nr = (int)s->rows.size(), nm = s->top.n_move;
     std::vector<int> E(nr);
@@ -715,12 +722,17 @@ static int setup_mma64(mdf_system *s) {
     s->s64_nt = tile_nt[0];
     s->s64_npass = npass;
     s->s64_pcols = pcoff[nr];
-    // sites per block: one block per compute unit (a block fills a unit's 16 wave slots at 128
-    // registers), at least four stages a block so that partial rows stay few
+    // sites per block: at most one block per compute unit (a block fills a unit's 16 wave slots
+    // at 128 registers), at least four stages a block so that partial rows stay few.  When the
+    // step graph forks, the blocks cover about 7/8 of the units: the skin test and the list
+    // kernels of the other stream run on the rest instead of waiting for the whole structure
+    // factor (slab16k: 220 blocks of 224 sites, 3657 -> 3872 steps/s, the kernel itself
+    // 53 -> 60 us)
     {
         int sites = s->s64_sites_env;
         if (sites <= 0) {
-            const long nsb = std::max(1L, ((long)s->n_cu + npass - 1) / npass);
+            const long units = fork_planned(s) ? ((long)s->n_cu * 7) / 8 : (long)s->n_cu;
+            const long nsb = std::max(1L, (units + npass - 1) / npass);
             sites = (int)((nm + nsb - 1) / nsb);
             sites = std::max(4 * SF64_SUB, SF64_SUB * ((sites + SF64_SUB - 1) / SF64_SUB));
         }
@@ -1333,8 +1345,6 @@ static int chem_enqueue(mdf_system *s) {
     return MDF_OK;
 }
 
-constexpr int FORK_MIN_SITES = 16384;
-
 // one step: the fixed launch sequence, replayed from a captured graph when possible (a
 // forced rebuild goes through direct launches)
 static int run_step(mdf_system *s, int force_rebuild) {
@@ -1359,8 +1369,7 @@ static int run_step(mdf_system *s, int force_rebuild) {
             // Forked by default only from FORK_MIN_SITES moving sites: below, the chains are
             // short and many handles stepping side by side (each a stream pair on the few
             // hardware queues) measured slower forked.
-            bool fork = (s->fork_env > 0 || (s->fork_env < 0 && s->top.n_move >= FORK_MIN_SITES)) && !s->mts_on &&
-                        !s->ak.empty() && s->top.n_move > 0;
+            bool fork = fork_planned(s) && !s->mts_on && !s->ak.empty() && s->top.n_move > 0;
             // the second stream is made on first use, before the capture (handles that never
             // fork keep one stream: streams are spread over the hardware queues in the order
             // they are made)
