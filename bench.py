@@ -326,9 +326,9 @@ def form_rooflines(form, run, s, pc, n_k, n_rows, kernels, traffic):
         sum(tr) if all(v is not None for v in tr) else None)
     listed = pc["listed_moving"] + pc["listed_frozen"]
     out["list_build"] = roofline(
-        "k_cell_count + k_cell_scan_build + k_cell_fill + k_cell_sort + k_build + k_snapshot", "hbm", "HBM",
+        "k_cell_count + k_cell_scan_build + k_cell_fill + k_cell_order + k_build", "hbm", "HBM",
         32.0 * s.ntota + 4.0 * listed, bs["build"], PEAK["hbm"], "GB/s",
-        traffic_of(traffic, form, r"k_cell_count|k_cell_scan_build|k_cell_fill|k_cell_sort|k_build\(|k_snapshot"),
+        traffic_of(traffic, form, r"k_cell_count|k_cell_scan_build|k_cell_fill|k_cell_order|k_build\("),
         steps=run["build_steps"], note="timed in a pass that rebuilds the list on every step")
     return out
 

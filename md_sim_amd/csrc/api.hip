@@ -37,7 +37,6 @@ __global__ void k_cell_sort(const int *start, const int *items_in, int *items_ou
 __global__ void k_cell_xyz(const int *items, int n, int offset, const double *x, const double *y, const double *z,
                            double4 *out);
 __global__ void k_build(DevSys S);
-__global__ void k_snapshot(DevSys S);
 __global__ void k_wall_cells(DevSys S, int *wcell_of, int ncz, double z0, double csz);
 __global__ void k_count_cells(const int *item_cell, int n_items, int *cnt);
 __global__ void k_zero_int(int *p, int n);
@@ -136,7 +135,6 @@ __global__ void k_cell_scan_build_b(const RepArgs *);
 __global__ void k_cell_fill_b(const RepArgs *, const int2 *);
 __global__ void k_cell_order_b(const RepArgs *, const int2 *);
 __global__ void k_build_b(const RepArgs *, const int2 *);
-__global__ void k_snapshot_b(const RepArgs *, const int2 *);
 __global__ void k_flags_b(const RepArgs *, int, DevFlags *);
 template <typename R, int G, bool VIR, bool ACC, bool STRIDED, int NMC>
 __global__ void k_pair_b(const RepArgs *, const int2 *);
@@ -853,7 +851,6 @@ static void launch_lists(mdf_system *s, int force_rebuild) {
     k_cell_fill<<<lb, 256, 0, st>>>(S, s->lead_cell.p, s->cell_items_tmp.p, s->cell_xyz_tmp.p);
     k_cell_order<<<s->order_blocks, 256, 0, st>>>(S, s->cell_items_tmp.p, s->cell_xyz_tmp.p);
     k_build<<<s->build_blocks, BUILD_THREADS, 0, st>>>(S);   // one block per cell
-    k_snapshot<<<std::max(1, (std::max(s->top.n_rig, s->top.n_move) + 255) / 256), 256, 0, st>>>(S);
 }
 
 // phase factors of the reciprocal kernels: 2 pi / L (widened pi) and the same over pi
@@ -2942,7 +2939,7 @@ int mdf_mts_eval_device(const float *weights, int32_t n, const float *feat, cons
 // (block maps give each block its system), so small systems fill the GPU together and a
 // step of the whole batch costs one graph launch.
 enum { BR_VALU, BR_MMA32, BR_MMA64 };
-enum { BK_GATHER, BK_SKIN, BK_COUNT, BK_FILL, BK_ORDER, BK_BUILD, BK_SNAP, BK_PAIR, BK_SF, BK_SFR, BK_BP, BK_EPI,
+enum { BK_GATHER, BK_SKIN, BK_COUNT, BK_FILL, BK_ORDER, BK_BUILD, BK_PAIR, BK_SF, BK_SFR, BK_BP, BK_EPI,
        BK_COLSUM, BK_N };
 
 struct mdf_batch {
@@ -3002,7 +2999,6 @@ static void fill_rep(mdf_system *s, RepArgs &r) {
     r.order_blocks = s->order_blocks;
     r.items_tmp = s->cell_items_tmp.p;
     r.xyz_tmp = s->cell_xyz_tmp.p;
-    r.snap_blocks = std::max(1, (std::max(nr, nm) + 255) / 256);
     r.ncells_xy = s->ncells_xy;
     r.gather_blocks = std::max(1, (std::max(nm, nr) + 255) / 256);
     const RecipArgs a = recip_args(s);
@@ -3079,7 +3075,6 @@ static void rep_blocks(const mdf_batch *b, const mdf_system *s, const RepArgs &r
     cnt[BK_FILL] = r.lead_blocks;
     cnt[BK_ORDER] = r.order_blocks;
     cnt[BK_BUILD] = r.build_blocks;
-    cnt[BK_SNAP] = r.snap_blocks;
     cnt[BK_PAIR] = std::max(1, (s->n_lead + (256 / b->pair_g) - 1) / (256 / b->pair_g));
     cnt[BK_SF] = (nk > 0 && nm > 0) ? r.n_sb * r.n_cg : 0;
     cnt[BK_SFR] = nk > 0 ? r.n_kblk : 0;
@@ -3248,7 +3243,6 @@ static void batch_launch(mdf_batch *b, const double *const *d_in) {
     k_cell_fill_b<<<grid(BK_FILL), 256, 0, st>>>(R, b->maps[BK_FILL].p);
     k_cell_order_b<<<grid(BK_ORDER), 256, 0, st>>>(R, b->maps[BK_ORDER].p);
     k_build_b<<<grid(BK_BUILD), BUILD_THREADS, 0, st>>>(R, b->maps[BK_BUILD].p);
-    k_snapshot_b<<<grid(BK_SNAP), 256, 0, st>>>(R, b->maps[BK_SNAP].p);
     {
         const int2 *mp = b->maps[BK_PAIR].p;
         const int g = grid(BK_PAIR);

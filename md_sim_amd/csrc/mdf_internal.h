@@ -321,7 +321,7 @@ struct RepArgs {
     int skin_blocks, force_rebuild;
     int *lead_cell, *items_tmp;
     double4 *xyz_tmp;
-    int lead_blocks, order_blocks, build_blocks, snap_blocks, ncells_xy, gather_blocks;
+    int lead_blocks, order_blocks, build_blocks, ncells_xy, gather_blocks;
     // reciprocal sum
     const KRow *rows;
     int nrows, cpb, n_sb, n_cg, nk, n_kblk, n_rg, n_bpsb;

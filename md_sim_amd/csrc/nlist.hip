@@ -229,6 +229,15 @@ __device__ __forceinline__ void cell_count_body(const DevSys &S, int *lead_cell_
     const int c = on ? lead_cell(S, S.lead[l], f->ncz, f->z0, f->csz) : 0;
     if (on) lead_cell_out[l] = c;
     (void)wave_agg_add(S.cell_cnt, c, on);
+    // the skin test's reference positions for the lists built now (src/nnlist.f90:99-119):
+    // molecule centres and the moving sites outside rigid molecules (this step's skin test has
+    // read the old ones)
+    const int n_free = S.n_move - S.n_rigid_sites;
+    for (int t = l; t < S.n_rig || t < n_free; t += nblk * blockDim.x) {
+        if (t < S.n_rig) { S.ocx[t] = S.cx[t]; S.ocy[t] = S.cy[t]; S.ocz[t] = S.cz[t]; }
+        const int i = S.n_rigid_sites + t;
+        if (t < n_free) { S.ox[i] = S.x[i]; S.oy[i] = S.y[i]; S.oz[i] = S.z[i]; }
+    }
 }
 
 __global__ __launch_bounds__(256) void k_cell_count(DevSys S, int *lead_cell_out) {
@@ -740,20 +749,6 @@ __global__ __launch_bounds__(BUILD_THREADS) void k_build(DevSys S) {
     build_body(S, blockIdx.x, gridDim.x);
 }
 
-// snapshot for the next skin test (src/nnlist.f90:99-119)
-// reference positions for the next skin tests (src/nnlist.f90:99-119)
-__device__ __forceinline__ void snapshot_body(const DevSys &S, int bid, int nblk) {
-    if (!S.flags->rebuild) return;
-    const int t = bid * blockDim.x + threadIdx.x;
-    if (t < S.n_rig) { S.ocx[t] = S.cx[t]; S.ocy[t] = S.cy[t]; S.ocz[t] = S.cz[t]; }
-    const int i = S.n_rigid_sites + t;
-    if (i < S.n_move) { S.ox[i] = S.x[i]; S.oy[i] = S.y[i]; S.oz[i] = S.z[i]; }
-}
-
-__global__ void k_snapshot(DevSys S) {
-    snapshot_body(S, blockIdx.x, gridDim.x);
-}
-
 // frozen-site cells, built once at creation
 __global__ void k_wall_cells(DevSys S, int *wcell_of, int ncz, double z0, double csz) {
     int w = blockIdx.x * blockDim.x + threadIdx.x;
@@ -808,10 +803,6 @@ __global__ __launch_bounds__(256) void k_cell_order_b(const RepArgs *__restrict_
 __global__ __launch_bounds__(BUILD_THREADS) void k_build_b(const RepArgs *__restrict__ RA, const int2 *__restrict__ map) {
     const int2 m = map[blockIdx.x];
     build_body(RA[m.x].S, m.y, RA[m.x].build_blocks);
-}
-__global__ __launch_bounds__(256) void k_snapshot_b(const RepArgs *__restrict__ RA, const int2 *__restrict__ map) {
-    const int2 m = map[blockIdx.x];
-    snapshot_body(RA[m.x].S, m.y, RA[m.x].snap_blocks);
 }
 // every system's flags side by side, for one copy to the host
 __global__ void k_flags_b(const RepArgs *__restrict__ RA, int n, DevFlags *__restrict__ out) {
