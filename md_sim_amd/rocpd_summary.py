@@ -11,6 +11,14 @@
     ``md_sim_amd.profile_step`` runs last moves a known number of bytes, so each counter is
     scaled by (true bytes) / (what the counter reads for that copy).  --json writes the
     figures to OUT, under KEY when given (other keys of an existing OUT are kept).
+
+``python -m md_sim_amd.rocpd_summary counters DB [--match REGEX]``
+    per kernel, the mean per dispatch of every counter of a ``--pmc`` run (SQ counters summed
+    over their hardware instances, GRBM counters the largest instance: each counts the whole
+    GPU's cycles), and from them: waves, VALU, FP64 matrix and LDS instructions per wave, and
+    the matrix pipe's busy cycles per matrix instruction.  GRBM_GUI_ACTIVE spans the whole
+    dispatch window under the profiler, well beyond the kernel's own run, so busy fractions
+    are left to the kernel's measured duration.
 """
 from __future__ import annotations
 
@@ -49,6 +57,26 @@ def counter_per_dispatch(db, counter):
     return out
 
 
+def counters_per_dispatch(db):
+    """name -> {counter -> list of per-dispatch values}: SQ and other block counters summed
+    over their hardware instances, GRBM counters the largest instance."""
+    per = collections.defaultdict(lambda: collections.defaultdict(float))
+    names = {}
+    with sqlite3.connect(db) as c:
+        for name, disp, cn, cv in c.execute(
+                "select name, dispatch_id, counter_name, counter_value from pmc_events"):
+            names[disp] = name
+            if cn.startswith("GRBM_"):
+                per[disp][cn] = max(per[disp][cn], cv)
+            else:
+                per[disp][cn] += cv
+    out = collections.defaultdict(lambda: collections.defaultdict(list))
+    for disp, cs in per.items():
+        for cn, v in cs.items():
+            out[names[disp]][cn].append(v)
+    return out
+
+
 def short(name, n=90):
     return name if len(name) <= n else name[:n - 3] + "..."
 
@@ -65,6 +93,32 @@ def cmd_kernels(a):
         print("| `%s` | %d | %.1f | %.1f | %.1f | %.2f | %.1f%% |" % (
             short(name), len(v), sum(v) / len(v) / 1e3, min(v) / 1e3, max(v) / 1e3, sum(v) / 1e6,
             100.0 * sum(v) / total))
+
+
+def cmd_counters(a):
+    d = counters_per_dispatch(a.db)
+    names = sorted(d, key=lambda n: -sum(d[n].get("GRBM_GUI_ACTIVE", [0.0])))
+    if a.match:
+        names = [n for n in names if re.search(a.match, n)]
+    cols = sorted({cn for n in names for cn in d[n]})
+    mean = lambda v: sum(v) / len(v) if v else 0.0
+    print("| kernel | dispatches | " + " | ".join(cols) + " |")
+    print("|---|---:|" + "---:|" * len(cols))
+    for n in names:
+        print("| `%s` | %d | " % (short(n, 70), len(next(iter(d[n].values())))) +
+              " | ".join("%.4g" % mean(d[n].get(cn, [])) for cn in cols) + " |")
+    print()
+    print("| kernel | waves | VALU instructions per wave | FP64 matrix instructions per wave | "
+          "LDS instructions per wave | matrix busy cycles per matrix instruction |")
+    print("|---|---:|---:|---:|---:|---:|")
+    for n in names:
+        c = {cn: mean(v) for cn, v in d[n].items()}
+        waves = c.get("SQ_WAVES", 0.0)
+        f = lambda num, den: ("%.1f" % (num / den)) if den > 0 else "-"
+        print("| `%s` | %.0f | %s | %s | %s | %s |" % (
+            short(n, 70), waves, f(c.get("SQ_INSTS_VALU", 0.0), waves), f(c.get("SQ_INSTS_VALU_MFMA_F64", 0.0), waves),
+            f(c.get("SQ_INSTS_LDS", 0.0), waves),
+            f(c.get("SQ_VALU_MFMA_BUSY_CYCLES", 0.0), c.get("SQ_INSTS_VALU_MFMA_F64", 0.0))))
 
 
 def cmd_traffic(a):
@@ -119,8 +173,11 @@ def main(argv=None):
     t.add_argument("--match", default="")
     t.add_argument("--json", default="")
     t.add_argument("--key", default="")
+    cc = sub.add_parser("counters")
+    cc.add_argument("db")
+    cc.add_argument("--match", default="")
     a = ap.parse_args(argv)
-    {"kernels": cmd_kernels, "traffic": cmd_traffic}[a.cmd](a)
+    {"kernels": cmd_kernels, "traffic": cmd_traffic, "counters": cmd_counters}[a.cmd](a)
     return 0
 
 
