@@ -12,6 +12,13 @@
     scaled by (true bytes) / (what the counter reads for that copy).  --json writes the
     figures to OUT, under KEY when given (other keys of an existing OUT are kept).
 
+``python -m md_sim_amd.rocpd_summary timeline DB [--last KERNEL] [--first KERNEL] [--forked]``
+    the steps of a kernel trace of graph replays, each from the dispatch after one KERNEL
+    (default ``k_final``) to the next: step spans, split into steps with and without a list
+    build (``k_build`` longer than 20 us), and the kernel-by-kernel timeline (queue, start,
+    end, duration in us from the step's first dispatch) of the median step of each kind;
+    --forked keeps the steps whose kernels ran on more than one queue (forked graph replays);
+
 ``python -m md_sim_amd.rocpd_summary counters DB [--match REGEX]``
     per kernel, the mean per dispatch of every counter of a ``--pmc`` run (SQ counters summed
     over their hardware instances, GRBM counters the largest instance: each counts the whole
@@ -95,6 +102,41 @@ def cmd_kernels(a):
             100.0 * sum(v) / total))
 
 
+def _kshort(name):
+    m = re.search(r"mdf::(k_\w+)", name)
+    return m.group(1) if m else name[:30]
+
+
+def cmd_timeline(a):
+    with sqlite3.connect(a.db) as c:
+        rows = c.execute("select name, start, end, queue_id from kernels order by start").fetchall()
+    ends = [i for i, r in enumerate(rows) if _kshort(r[0]) == a.last]
+    steps = [rows[i + 1:j + 1] for i, j in zip(ends, ends[1:])]
+    steps = [st for st in steps if any(_kshort(r[0]) == a.first for r in st)]
+    if a.forked:
+        steps = [st for st in steps if len({r[3] for r in st}) > 1]
+    if not steps:
+        sys.exit("no steps between %s dispatches hold %s" % (a.last, a.first))
+    span = lambda st: (max(r[2] for r in st) - min(r[1] for r in st)) / 1e3
+    build = lambda st: any(_kshort(r[0]) == "k_build" and r[2] - r[1] > 20000 for r in st)
+    for title, group in (("steps without a list build", [st for st in steps if not build(st)]),
+                         ("steps with a list build", [st for st in steps if build(st)])):
+        if not group:
+            continue
+        group = sorted(group, key=span)
+        sp = [span(st) for st in group]
+        print("%s: %d, span median %.1f us, min %.1f, max %.1f; the median step:\n" % (
+            title, len(group), sp[len(sp) // 2], sp[0], sp[-1]))
+        st = group[len(group) // 2]
+        t0 = min(r[1] for r in st)
+        print("| kernel | queue | start (us) | end (us) | duration (us) |")
+        print("|---|---:|---:|---:|---:|")
+        for r in st:
+            print("| `%s` | %s | %.1f | %.1f | %.1f |" % (_kshort(r[0]), r[3], (r[1] - t0) / 1e3, (r[2] - t0) / 1e3,
+                                                      (r[2] - r[1]) / 1e3))
+        print()
+
+
 def cmd_counters(a):
     d = counters_per_dispatch(a.db)
     names = sorted(d, key=lambda n: -sum(d[n].get("GRBM_GUI_ACTIVE", [0.0])))
@@ -173,11 +215,16 @@ def main(argv=None):
     t.add_argument("--match", default="")
     t.add_argument("--json", default="")
     t.add_argument("--key", default="")
+    tl = sub.add_parser("timeline")
+    tl.add_argument("db")
+    tl.add_argument("--last", default="k_final")
+    tl.add_argument("--first", default="k_pair")
+    tl.add_argument("--forked", action="store_true")
     cc = sub.add_parser("counters")
     cc.add_argument("db")
     cc.add_argument("--match", default="")
     a = ap.parse_args(argv)
-    {"kernels": cmd_kernels, "traffic": cmd_traffic, "counters": cmd_counters}[a.cmd](a)
+    {"kernels": cmd_kernels, "traffic": cmd_traffic, "counters": cmd_counters, "timeline": cmd_timeline}[a.cmd](a)
     return 0
 
 

@@ -49,3 +49,23 @@ def test_traffic_scales_by_the_calibration_copy(tmp_path):
     k = json.load(open(out))["fp64"]["kernels"]["k_a"]
     assert k["dispatches"] == 2
     assert k["fetch_bytes"] == 2 * 200.0 * 1024 and k["write_bytes"] == 20.0 * 1024
+
+
+def test_timeline_splits_steps_at_the_last_kernel(tmp_path, capsys):
+    db = str(tmp_path / "k.db")
+    with sqlite3.connect(db) as c:
+        c.execute("create table kernels (name text, start int, end int, queue_id int)")
+        rows = []
+        t = 0
+        for step, build_ns in enumerate((1000, 30000, 1000)):
+            for name, dur, q in (("mdf::k_gather(x)", 4000, 2), ("mdf::k_build(x)", build_ns, 4),
+                                 ("mdf::k_pair(x)", 70000, 4), ("mdf::k_final(x)", 8000, 2)):
+                rows.append((name, t, t + dur, q))
+                t += dur
+        c.executemany("insert into kernels values (?, ?, ?, ?)", rows)
+    rocpd_summary.main(["timeline", db])
+    out = capsys.readouterr().out
+    # three steps; the first ends with its k_final and only the next two start after one
+    assert "steps without a list build: 1, span median 83.0 us" in out
+    assert "steps with a list build: 1, span median 112.0 us" in out
+    assert "| `k_build` | 4 | 4.0 | 34.0 | 30.0 |" in out
